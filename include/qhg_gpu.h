@@ -1,0 +1,231 @@
+/*
+ * qhg_gpu.h -- C ABI of the MI355X population step engine.
+ *
+ * One handle = one population on one GPU (one HIP stream).  The entry
+ * points below replace the per-step population interface of QHG4:
+ *
+ *   PopBase::preLoop              QHG4/core/PopBase.h:16-121, SPopulation.cpp:273
+ *   SPopulation::initializeStep   QHG4/core/SPopulation.cpp:394-417
+ *   SPopulation::doActions        QHG4/core/SPopulation.cpp:554-577
+ *   SPopulation::finalizeStep     QHG4/core/SPopulation.cpp:439-477
+ *   PopLooper::doStep             QHG4/core/PopLooper.cpp:166-202
+ *   SPopulation::getNumAgentsArray / getNumAgentsEffective
+ *                                 QHG4/core/SPopulation.h:143-148
+ *   SPopulation::addAgent / writeAgentDataQDFSafe (agent upload/download)
+ *                                 QHG4/core/SPopulation.cpp:1465-1567
+ *
+ * Conventions (follow the reference's error convention,
+ * e.g. QHG4/actions/Verhulst.cpp:80-91): every function returns 0 on
+ * success and a negative code on error; qhg_gpu_last_error() returns a
+ * thread-local message for the last failing call.  Host arrays are borrowed
+ * only for the duration of a call.  Calls on one handle are not re-entrant;
+ * different handles may be used concurrently.
+ *
+ * Agent time fields (BirthTime, Age, LastBirth) are FP64.
+ */
+#ifndef QHG_GPU_H
+#define QHG_GPU_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QHG_ABI_VERSION 1
+
+/* error codes */
+#define QHG_OK              0
+#define QHG_ERR_ARG        -1
+#define QHG_ERR_HIP        -2
+#define QHG_ERR_CAPACITY   -3
+#define QHG_ERR_STATE      -4
+#define QHG_ERR_NOMEM      -5
+
+/* life states, QHG4/core/SPopulation.h:70-74 */
+#define QHG_LIFE_DEAD     0
+#define QHG_LIFE_ALIVE    1
+#define QHG_LIFE_FERTILE  5
+
+/* action kinds (one per supported Action<T> subclass) */
+#define QHG_ACT_GETOLD        1   /* QHG4/actions/GetOld.cpp:37-48            */
+#define QHG_ACT_FERTILITY     2   /* QHG4/actions/Fertility.cpp:49-74         */
+#define QHG_ACT_RANDOMPAIR    3   /* QHG4/actions/RandomPair.cpp:107-279      */
+#define QHG_ACT_VERHULST      4   /* QHG4/actions/Verhulst.cpp:70-115         */
+#define QHG_ACT_VERHULSTVARK  5   /* QHG4/actions/VerhulstVarK.cpp:69-115     */
+#define QHG_ACT_RANDOMMOVE    6   /* QHG4/actions/RandomMove.cpp:65-100       */
+#define QHG_ACT_WEIGHTEDMOVE  7   /* QHG4/actions/WeightedMove.cpp:45-106     */
+#define QHG_ACT_MULTIEVAL     8   /* QHG4/actions/MultiEvaluator.cpp:142-478  */
+#define QHG_ACT_NPPCAP        9   /* QHG4/actions/NPPCapacity.cpp:138-217     */
+#define QHG_ACT_GENETICS     10   /* QHG4/actions/Genetics.cpp:285-337        */
+
+/* MultiEvaluator combination modes, QHG4/actions/MultiEvaluator.h:20-25 */
+#define QHG_ME_ADD_SIMPLE 0
+#define QHG_ME_MUL_SIMPLE 2
+#define QHG_ME_MAX_SIMPLE 3
+#define QHG_ME_MIN_SIMPLE 5
+
+/* inputs a SingleEvaluator can read */
+#define QHG_EV_ALTITUDE   0
+#define QHG_EV_CAPACITY   1   /* NPPCapacity output (K per cell) */
+#define QHG_EV_BASE_NPP   2
+
+#define QHG_MAX_ACTIONS   16
+#define QHG_MAX_EVALS     4
+#define QHG_MAX_PL_POINTS 64
+#define QHG_MAX_BINO      64
+#define QHG_NAME_LEN      48
+
+/* Philox stream tags: the 16-bit action field of the counter */
+#define QHG_STREAM_PAIR     1
+#define QHG_STREAM_LBIRTH   2
+#define QHG_STREAM_LDEATH   3
+#define QHG_STREAM_RMOVE    4
+#define QHG_STREAM_WMOVE    5
+#define QHG_STREAM_NEWBORN  6
+#define QHG_STREAM_GENETICS 7
+
+typedef struct qhg_grid_desc {
+    int32_t        num_cells;
+    int32_t        connectivity;   /* max neighbours per cell: 4 lattice, 6 ico */
+    const int32_t *neighbors;      /* num_cells*connectivity, -1 padded        */
+    const uint8_t *num_neighbors;  /* num_cells                                */
+    const int64_t *cell_ids;       /* num_cells global ids, NULL => index      */
+} qhg_grid_desc;
+
+/* Geography / climate / vegetation arrays (any may be NULL). */
+typedef struct qhg_env_desc {
+    const double  *altitude;
+    const double  *longitude;
+    const double  *latitude;
+    const double  *water;
+    const uint8_t *ice;
+    const uint8_t *coastal;
+    const double  *base_npp;
+    const double  *temperature;   /* annual mean temp (Miami NPP) */
+    const double  *rainfall;      /* annual rainfall  (Miami NPP) */
+} qhg_env_desc;
+
+/* Parameter block.  Field names follow the reference's attribute strings
+ * (e.g. Verhulst_b0 -> verh_b0); the host-side action classes fill it. */
+typedef struct qhg_params {
+    uint64_t seed;
+    int32_t  num_actions;
+    int32_t  action_kind[QHG_MAX_ACTIONS];
+    int32_t  action_prio[QHG_MAX_ACTIONS];
+    char     action_name[QHG_MAX_ACTIONS][QHG_NAME_LEN];
+
+    /* Fertility_min_age, Fertility_max_age, Fertility_interbirth */
+    double fert_min_age, fert_max_age, fert_interbirth;
+    /* Verhulst_b0, Verhulst_d0, Verhulst_theta, Verhulst_K */
+    double verh_b0, verh_d0, verh_theta, verh_K;
+    /* RandomMove_prob, WeightedMove_prob */
+    double rmove_prob, wmove_prob;
+
+    /* MultiEvaluator: Multi_weight_* and the SingleEvaluator PolyLines */
+    int32_t me_mode;
+    int32_t me_num;
+    int32_t me_input[QHG_MAX_EVALS];
+    int32_t me_cumulate[QHG_MAX_EVALS];
+    double  me_weight[QHG_MAX_EVALS];
+    int32_t me_pl_n[QHG_MAX_EVALS];          /* points; 0 = identity       */
+    double  me_pl_x[QHG_MAX_EVALS][QHG_MAX_PL_POINTS];
+    double  me_pl_v[QHG_MAX_EVALS][QHG_MAX_PL_POINTS];
+
+    /* NPPCap_water_factor, NPPCap_coastal_factor, NPPCap_coastal_min_latitude,
+       NPPCap_coastal_max_latitude, NPPCap_NPP_min, NPPCap_NPP_max,
+       NPPCap_K_min, NPPCap_K_max, NPPCap_efficiency */
+    double npp_water_factor, npp_coastal_factor, npp_coastal_min_lat, npp_coastal_max_lat;
+    double npp_min, npp_max, npp_k_min, npp_k_max, npp_efficiency;
+
+    /* Genetics_genome_size, Genetics_num_crossover, Genetics_mutation_rate */
+    int32_t gen_genome_size;        /* loci per strand                       */
+    int32_t gen_num_crossover;      /* >0 fixed count, 0 none, -1 free reco   */
+    double  gen_mutation_rate;
+} qhg_params;
+
+/* Agent records as structure-of-arrays.  Arrays the caller does not need may
+ * be NULL on download.  mate_id is the partner's agent id, -3 when unpaired
+ * this step, -1 when the partner has since died. */
+typedef struct qhg_agents {
+    int64_t   n;
+    int64_t  *id;
+    int32_t  *cell;
+    uint8_t  *life;
+    uint8_t  *gender;
+    double   *birth_time;
+    double   *age;
+    double   *last_birth;
+    int64_t  *mate_id;
+    int32_t  *num_babies;
+    uint64_t *genome;      /* n * 2 * genome_words(gen_genome_size) */
+} qhg_agents;
+
+typedef struct qhg_stats {
+    int64_t  num_alive;
+    int64_t  high_water;     /* slots in use, holes included */
+    int64_t  capacity;
+    int64_t  max_id;
+    int64_t  step;
+    int64_t  births_last, deaths_last, moves_last;
+    int64_t  births_total, deaths_total, moves_total;
+    int64_t  compactions;
+    int32_t  device_error;
+} qhg_stats;
+
+typedef struct qhg_pop qhg_pop;
+
+/* version / diagnostics */
+int         qhg_gpu_abi_version(void);
+const char *qhg_gpu_last_error(void);
+int         qhg_gpu_genome_words(int32_t genome_size);   /* u64 per strand */
+
+/* Binomial mutation-count table (QHG4/utils/BinomialDist.cpp:60-104):
+ * fills cdf[0..*n) and returns 0.  Shared by the device path. */
+int qhg_gpu_binomial_table(double p, int32_t n, double eps, double *cdf, int32_t *count);
+
+/* lifecycle: SPopulation ctor/dtor (QHG4/core/SPopulation.cpp:46-140) */
+int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env,
+                   const qhg_params *params, int64_t capacity, qhg_pop **out);
+int qhg_gpu_destroy(qhg_pop *pop);
+
+/* agents: SPopulation::addAgent / reading agents (QHG4/core/SPopulation.cpp
+ * :1190-1290) and writeAgentDataQDFSafe (:1465-1567).  Upload replaces the
+ * population; agents must have distinct ids. Download returns live agents
+ * in id order; qhg_gpu_num_alive() gives the size to allocate. */
+int qhg_gpu_upload_agents(qhg_pop *pop, const qhg_agents *agents);
+int qhg_gpu_num_alive(qhg_pop *pop, int64_t *n);
+int qhg_gpu_download_agents(qhg_pop *pop, qhg_agents *agents);
+
+/* step phases, PopLooper::doStep order (QHG4/core/PopLooper.cpp:166-202) */
+int qhg_gpu_pre_loop(qhg_pop *pop, double t);
+int qhg_gpu_initialize_step(qhg_pop *pop, double t);
+int qhg_gpu_do_actions(qhg_pop *pop, uint32_t prio, double t);
+int qhg_gpu_finalize_step(qhg_pop *pop);
+/* nsteps whole steps at t = step, step+1, ... (no host sync per step) */
+int qhg_gpu_run(qhg_pop *pop, int32_t nsteps);
+
+/* environment change (EVENT_ID_GEO/VEG/CLIMATE + FLUSH): replace env arrays
+ * and recompute capacities and evaluator weights (QHG4/populations/OoAPop.cpp
+ * :164-209, NPPCapacity.cpp:121-131) */
+int qhg_gpu_set_env(qhg_pop *pop, const qhg_env_desc *env);
+
+/* observables */
+int qhg_gpu_sync(qhg_pop *pop);
+int qhg_gpu_stats(qhg_pop *pop, qhg_stats *out);
+int qhg_gpu_cell_counts(qhg_pop *pop, uint32_t *out);        /* num_cells */
+int qhg_gpu_capacities(qhg_pop *pop, double *out);           /* num_cells */
+int qhg_gpu_env_weights(qhg_pop *pop, double *out);          /* num_cells*(conn+1) */
+int qhg_gpu_compact(qhg_pop *pop);                           /* LBController::compactData */
+
+/* profiling: average device time (ms) per launch of each kernel since the
+ * last reset, measured with HIP events on the handle's stream. */
+int qhg_gpu_set_timing(qhg_pop *pop, int32_t enable);
+int qhg_gpu_kernel_times(qhg_pop *pop, double *avg_ms, int64_t *launches, int32_t max, int32_t *count);
+const char *qhg_gpu_kernel_name(int32_t index);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

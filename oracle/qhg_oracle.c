@@ -1,0 +1,947 @@
+/*
+ * qhg_oracle.c -- CPU oracle of the QHG4 per-step population loop.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker and the CPU baseline; never part of
+ * the product).  A restatement of the reference step loop, action by action,
+ * with Philox4x32-10 draws instead of per-thread WELL512 streams.
+ *
+ * Reference functions restated here (paths relative to the reference root):
+ *   step driver        QHG4/core/PopLooper.cpp:166-202
+ *   initializeStep     QHG4/core/SPopulation.cpp:394-417
+ *   doActions          QHG4/core/SPopulation.cpp:554-577
+ *   finalizeStep       QHG4/core/SPopulation.cpp:439-477
+ *   births/deaths      QHG4/core/SPopulation.cpp:596-724, 838-938
+ *   moves              QHG4/core/SPopulation.cpp:1014-1092
+ *   counts             QHG4/core/SPopulation.cpp:509-536
+ *   actions            QHG4/actions/{GetOld,Fertility,RandomPair,Verhulst,
+ *                      VerhulstVarK,LinearBirth,LinearDeath,RandomMove,
+ *                      WeightedMove,SingleEvaluator,MultiEvaluator,
+ *                      NPPCapacity,Genetics}.cpp
+ *   genome ops         QHG4/genes/GeneUtils.cpp:112-360
+ *   binomial table     QHG4/utils/BinomialDist.cpp:60-104, bino_tools.cpp
+ *   PolyLine           QHG4/utils/PolyLine.cpp:60-89
+ *   WELL conversions   QHG4/utils/WELL512.h:33-39
+ *
+ * Schedule-independent replacements (documented deviations):
+ *   - draws: Philox4x32-10 keyed (seed, step, stream, agent id, draw#);
+ *   - per-cell pairing orders candidates by agent id (reference: slot index);
+ *   - new ids: max_id + 1 + rank of the mother's id among this step's mothers
+ *     (reference: thread-strided IDGen);
+ *   - time fields are FP64 (reference: float).
+ */
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <stdio.h>
+#include <omp.h>
+
+#include "qhg_oracle.h"
+
+/* ------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11; Random123)        */
+/* ------------------------------------------------------------------ */
+#define PH_M0 0xD2511F53u
+#define PH_M1 0xCD9E8D57u
+#define PH_W0 0x9E3779B9u
+#define PH_W1 0xBB67AE85u
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        if (r > 0) { k0 += PH_W0; k1 += PH_W1; }
+        uint64_t p0 = (uint64_t)PH_M0 * c0;
+        uint64_t p1 = (uint64_t)PH_M1 * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+uint32_t orc_draw(uint64_t seed, uint32_t step, uint32_t stream, uint64_t id, uint32_t n) {
+    uint32_t ctr[4] = { step, (stream << 16) | (n >> 2), (uint32_t)id, (uint32_t)(id >> 32) };
+    uint32_t key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) };
+    uint32_t out[4];
+    orc_philox4x32_10(ctr, key, out);
+    return out[n & 3];
+}
+
+/* WELL512 conversion formulas, QHG4/utils/WELL512.h:33-39 (MAX_WRAND = 2^32-1) */
+double orc_wrandd(uint32_t u) {
+    return (1.0 * u) / (1 + 4294967295.0);
+}
+double orc_wrandr(double a, double b, uint32_t u) {
+    return a + ((b - a) * u) / (1 + 4294967295.0);
+}
+uint32_t orc_wrandi(uint32_t a, uint32_t b, uint32_t s, uint32_t u) {
+    uint32_t r = b - a;
+    r += (r % s);
+    return a + s * (uint32_t)((1.0 * r * u) / (s * (1.0 + 4294967295.0)));
+}
+
+/* ------------------------------------------------------------------ */
+/* Binomial lookup table: QHG4/utils/BinomialDist.cpp:60-104 with the  */
+/* Numerical-Recipes incomplete beta of QHG4/utils/bino_tools.cpp      */
+/* ------------------------------------------------------------------ */
+static double orc_gammaln(double xx) {
+    static const double cof[6] = { 76.18009172947146, -86.50532032941677,
+                                   24.01409824083091, -1.231739572450155,
+                                   0.1208650973866179e-2, -0.5395239384953e-5 };
+    double ser = 1.000000000190015;
+    double x = xx, y = xx + 1, tmp = x + 5.5;
+    tmp -= (x + 0.5) * log(tmp);
+    for (int k = 0; k <= 5; k++) { ser += cof[k] / y; y++; }
+    return -tmp + log(2.5066282746310005 * ser / x);
+}
+
+static double orc_betacf(double a, double b, double x) {
+    const int MAXIT = 100; const double EPS = 3.0e-7; const float FPMIN = 1.0e-30f;
+    double qab = a + b, qap = a + 1.0, qam = a - 1.0, c = 1.0;
+    double d = 1.0 - qab * x / qap;
+    if (fabs(d) < FPMIN) d = FPMIN;
+    d = 1.0 / d;
+    double h = d;
+    int m;
+    for (m = 1; m <= MAXIT; m++) {
+        int m2 = 2 * m;
+        double aa = m * (b - m) * x / ((qam + m2) * (a + m2));
+        d = 1.0 + aa * d; if (fabs(d) < FPMIN) d = FPMIN;
+        c = 1.0 + aa / c; if (fabs(c) < FPMIN) c = FPMIN;
+        d = 1.0 / d; h *= d * c;
+        aa = -(a + m) * (qab + m) * x / ((a + m2) * (qap + m2));
+        d = 1.0 + aa * d; if (fabs(d) < FPMIN) d = FPMIN;
+        c = 1.0 + aa / c; if (fabs(c) < FPMIN) c = FPMIN;
+        d = 1.0 / d;
+        double del = d * c;
+        h *= del;
+        if (fabs(del - 1.0) < EPS) break;
+    }
+    if (m > MAXIT) h = -1;
+    return h;
+}
+
+double orc_betai(double a, double b, double x) {
+    if (x < 0.0 || x > 1.0) return -1;
+    double bt = (x == 0.0 || x == 1.0) ? 0.0
+        : exp(orc_gammaln(a + b) - orc_gammaln(a) - orc_gammaln(b) + a * log(x) + b * log(1.0 - x));
+    if (x < (a + 1.0) / (a + b + 2.0)) return bt * orc_betacf(a, b, x) / a;
+    return 1.0 - bt * orc_betacf(b, a, 1.0 - x) / b;
+}
+
+int32_t orc_bino_table(double p, int32_t n, double eps, double *cdf, int32_t max) {
+    if (!(p >= 0 && p < 1 && n > 0)) return -1;
+    int32_t cnt = 0;
+    int k = 1;
+    double d2 = orc_betai(k, n - k + 1, p);
+    while (d2 > eps && k < n) {
+        if (cnt >= max) return -1;
+        cdf[cnt++] = 1 - d2;
+        k++;
+        d2 = orc_betai(k, n - k + 1, p);
+    }
+    if (cnt >= max) return -1;
+    cdf[cnt++] = 1 - d2;
+    return cnt;
+}
+
+int32_t orc_bino_getn(const double *cdf, int32_t nbins, double u) {
+    if (!(u >= 0)) return -1;
+    int32_t i = 0;
+    while (i < nbins && u > cdf[i]) i++;
+    return i;
+}
+
+/* PolyLine::getVal, QHG4/utils/PolyLine.cpp:60-89 (npts = segments + 1) */
+double orc_polyline(int32_t npts, const double *x, const double *v, double X) {
+    if (npts < 2) return X;
+    int32_t ns = npts - 1;
+    if (X >= x[ns]) return v[ns];
+    int32_t i = 0;
+    while (i <= ns && X > x[i]) i++;
+    if (i == 0) return v[0];
+    if (i <= ns) {
+        double a = (v[i] - v[i - 1]) / (x[i] - x[i - 1]);
+        return v[i - 1] + a * (X - x[i - 1]);
+    }
+    return v[ns];
+}
+
+/* ------------------------------------------------------------------ */
+/* 2-bit genome operations, QHG4/genes/GeneUtils.cpp                   */
+/* ------------------------------------------------------------------ */
+int32_t orc_genome_words(int32_t genome_size) { return (genome_size + 31) / 32; }
+
+/* makeMultiMask, GeneUtils.cpp:156-175: walks bits from the MSB, consuming
+ * breaks from the back of the vector (in draw order, not sorted). */
+uint64_t orc_multimask(const uint32_t *breaks, int32_t n) {
+    uint32_t k = 1 - (uint32_t)(n % 2);
+    uint64_t out = 0;
+    int32_t i = n - 1;
+    for (uint32_t j = 0; j < 64; j++) {
+        if (i >= 0 && j == 64 - breaks[i]) { i--; k = 1 - k; }
+        out = (out << 1) + k;
+    }
+    return out;
+}
+
+/* crossOver, GeneUtils.cpp:184-246, with the break positions given
+ * (in draw order).  out and parent hold two strands of nb words each. */
+void orc_crossover(uint64_t *out, const uint64_t *parent, int32_t genome_size,
+                   const uint32_t *break_pos, int32_t nbreaks) {
+    int32_t nb = orc_genome_words(genome_size);
+    /* group breaks by block in draw order (std::map<block, vector>) */
+    uint32_t bb[64][64];
+    int32_t bn[64];
+    memset(bn, 0, sizeof(bn));
+    for (int32_t i = 0; i < nbreaks; i++) {
+        uint32_t blk = break_pos[i] / 64, bit = break_pos[i] % 64;
+        bb[blk][bn[blk]++] = bit;
+    }
+    int32_t last = 0;
+    uint32_t s = 0;
+    for (int32_t blk = 0; blk < nb; blk++) {
+        if (bn[blk] == 0) continue;
+        int32_t cur = (int32_t)(s % 2);
+        for (int32_t q = last; q < blk; q++) {
+            out[q] = parent[cur * nb + q];
+            out[nb + q] = parent[(1 - cur) * nb + q];
+        }
+        uint64_t L = orc_multimask(bb[blk], bn[blk]);
+        uint64_t R = ~L;
+        out[blk] = (L & parent[cur * nb + blk]) | (R & parent[(1 - cur) * nb + blk]);
+        out[blk + nb] = (L & parent[(1 - cur) * nb + blk]) | (R & parent[cur * nb + blk]);
+        s += (uint32_t)bn[blk];
+        last = blk + 1;
+    }
+    if (nb > last) {
+        int32_t cur = (int32_t)(s % 2);
+        for (int32_t q = last; q < nb; q++) {
+            out[q] = parent[cur * nb + q];
+            out[nb + q] = parent[(1 - cur) * nb + q];
+        }
+    }
+}
+
+/* freeReco + makeFreeMask, GeneUtils.cpp:322-360 (mask = hi:lo draws) */
+static void orc_free_reco(uint64_t *out, const uint64_t *parent, int32_t nb,
+                          const uint32_t *draws) {
+    for (int32_t b = 0; b < nb; b++) {
+        uint64_t v = ((uint64_t)draws[2 * b] << 32) + draws[2 * b + 1];
+        v &= 0x5555555555555555ull;
+        v += v << 1;
+        uint64_t L = v, R = ~v;
+        out[b] = (L & parent[b]) | (R & parent[nb + b]);
+        out[nb + b] = (L & parent[nb + b]) | (R & parent[b]);
+    }
+}
+
+/* mutateNucs, GeneUtils.cpp:112-141: genome_size = loci over both strands */
+void orc_mutate_nucs(uint64_t *genome, int32_t genome_size, int32_t nmut,
+                     const uint32_t *pos_draws, const uint32_t *mask_draws) {
+    uint32_t nbits = 2 * (uint32_t)genome_size;
+    for (int32_t i = 0; i < nmut; i++) {
+        uint32_t pos = orc_wrandi(0, nbits, 2, pos_draws[i]);
+        uint64_t mask = (uint64_t)orc_wrandi(1, 4, 1, mask_draws[i]);
+        genome[pos / 64] ^= mask << (pos % 64);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* population state                                                    */
+/* ------------------------------------------------------------------ */
+struct orc {
+    qhg_params P;
+    int32_t  ncells, conn;
+    int32_t *nbr;
+    uint8_t *nn;
+    int64_t *cell_ids;
+    /* environment */
+    double  *alt, *lon, *lat, *water, *npp, *temp, *rain;
+    uint8_t *ice, *coastal;
+    /* per-cell */
+    double  *K;         /* capacities (NPPCapacity output)   */
+    double  *W;         /* env weights ncells*(conn+1)       */
+    double  *single;    /* SingleEvaluator scratch           */
+    double  *bc, *dc;   /* LinearBirth / LinearDeath per cell */
+    uint32_t *count;
+    int      me_first;
+    /* action order (prio, then name) */
+    int32_t  nact;
+    int32_t  act[QHG_MAX_ACTIONS];
+    /* agents, id order */
+    int64_t  n, cap;
+    int64_t *id;
+    int32_t *cell, *mate, *nbabies, *dest;
+    uint8_t *life, *gender, *bflag;
+    double  *birth, *age, *lastb;
+    uint64_t *genome;
+    int32_t  gw;         /* u64 words per genome (both strands) */
+    /* genetics */
+    double   bino[QHG_MAX_BINO];
+    int32_t  nbino;
+    /* counters */
+    int64_t  max_id;
+    uint32_t step;
+    int64_t  births_last, deaths_last, moves_last;
+    int64_t  births_total, deaths_total, moves_total;
+    /* pairing scratch */
+    int32_t *pc_off, *pc_buf;
+};
+
+static int g_threads = 0;
+void orc_set_threads(int32_t n) { g_threads = n; if (n > 0) omp_set_num_threads(n); }
+int32_t orc_get_threads(void) { return omp_get_max_threads(); }
+
+static void *xdup(const void *src, size_t bytes) {
+    if (!src) return NULL;
+    void *p = malloc(bytes);
+    if (p) memcpy(p, src, bytes);
+    return p;
+}
+
+static int has_action(const orc *o, int kind) {
+    for (int i = 0; i < o->nact; i++) if (o->act[i] == kind) return 1;
+    return 0;
+}
+
+static void sort_actions(orc *o) {
+    /* Prioritizer order: prio ascending, then action name (std::map order),
+     * QHG4/core/Prioritizer.cpp:19-30, QHG4/core/SPopulation.cpp:249-257 */
+    int32_t idx[QHG_MAX_ACTIONS];
+    int32_t n = o->P.num_actions;
+    for (int i = 0; i < n; i++) idx[i] = i;
+    for (int i = 1; i < n; i++) {
+        int32_t v = idx[i]; int j = i - 1;
+        while (j >= 0) {
+            int32_t a = idx[j];
+            int less = (o->P.action_prio[v] < o->P.action_prio[a]) ||
+                       (o->P.action_prio[v] == o->P.action_prio[a] &&
+                        strcmp(o->P.action_name[v], o->P.action_name[a]) < 0);
+            if (!less) break;
+            idx[j + 1] = idx[j]; j--;
+        }
+        idx[j + 1] = v;
+    }
+    o->nact = n;
+    for (int i = 0; i < n; i++) o->act[i] = o->P.action_kind[idx[i]];
+}
+
+int orc_create(const qhg_grid_desc *g, const qhg_env_desc *env, const qhg_params *p, orc **out) {
+    if (!g || !p || !out || g->num_cells <= 0 || g->connectivity <= 0) return QHG_ERR_ARG;
+    orc *o = (orc *)calloc(1, sizeof(orc));
+    o->P = *p;
+    o->ncells = g->num_cells;
+    o->conn = g->connectivity;
+    size_t nc = (size_t)o->ncells;
+    o->nbr = (int32_t *)xdup(g->neighbors, nc * o->conn * sizeof(int32_t));
+    o->nn = (uint8_t *)xdup(g->num_neighbors, nc);
+    o->cell_ids = (int64_t *)malloc(nc * sizeof(int64_t));
+    for (size_t c = 0; c < nc; c++) o->cell_ids[c] = g->cell_ids ? g->cell_ids[c] : (int64_t)c;
+    if (env) {
+        o->alt = (double *)xdup(env->altitude, nc * 8);
+        o->lon = (double *)xdup(env->longitude, nc * 8);
+        o->lat = (double *)xdup(env->latitude, nc * 8);
+        o->water = (double *)xdup(env->water, nc * 8);
+        o->npp = (double *)xdup(env->base_npp, nc * 8);
+        o->temp = (double *)xdup(env->temperature, nc * 8);
+        o->rain = (double *)xdup(env->rainfall, nc * 8);
+        o->ice = (uint8_t *)xdup(env->ice, nc);
+        o->coastal = (uint8_t *)xdup(env->coastal, nc);
+    }
+    o->K = (double *)calloc(nc, 8);
+    o->W = (double *)calloc(nc * (o->conn + 1), 8);
+    o->single = (double *)calloc(nc * (o->conn + 1), 8);
+    o->bc = (double *)calloc(nc, 8);
+    o->dc = (double *)calloc(nc, 8);
+    o->count = (uint32_t *)calloc(nc, 4);
+    o->pc_off = (int32_t *)calloc(nc + 1, 4);
+    o->me_first = 1;
+    sort_actions(o);
+    o->gw = 0;
+    if (has_action(o, QHG_ACT_GENETICS)) {
+        o->gw = 2 * orc_genome_words(p->gen_genome_size);
+        if (p->gen_mutation_rate > 0) {
+            o->nbino = orc_bino_table(p->gen_mutation_rate, 2 * p->gen_genome_size, 1e-6,
+                                      o->bino, QHG_MAX_BINO);
+            if (o->nbino < 0) { orc_destroy(o); return QHG_ERR_ARG; }
+        }
+    }
+    *out = o;
+    return 0;
+}
+
+void orc_destroy(orc *o) {
+    if (!o) return;
+    void *ptrs[] = { o->nbr, o->nn, o->cell_ids, o->alt, o->lon, o->lat, o->water, o->npp,
+                     o->temp, o->rain, o->ice, o->coastal, o->K, o->W, o->single, o->bc,
+                     o->dc, o->count, o->pc_off, o->pc_buf, o->id, o->cell, o->mate,
+                     o->nbabies, o->dest, o->life, o->gender, o->bflag, o->birth, o->age,
+                     o->lastb, o->genome };
+    for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
+    free(o);
+}
+
+static void reserve(orc *o, int64_t need) {
+    if (need <= o->cap) return;
+    int64_t nc = o->cap ? o->cap : 1024;
+    while (nc < need) nc *= 2;
+#define GROW(f, T) o->f = (T *)realloc(o->f, (size_t)nc * sizeof(T))
+    GROW(id, int64_t); GROW(cell, int32_t); GROW(mate, int32_t); GROW(nbabies, int32_t);
+    GROW(dest, int32_t); GROW(life, uint8_t); GROW(gender, uint8_t); GROW(bflag, uint8_t);
+    GROW(birth, double); GROW(age, double); GROW(lastb, double); GROW(pc_buf, int32_t);
+#undef GROW
+    if (o->gw) o->genome = (uint64_t *)realloc(o->genome, (size_t)nc * o->gw * 8);
+    o->cap = nc;
+}
+
+static void update_counts(orc *o) {
+    memset(o->count, 0, (size_t)o->ncells * 4);
+    for (int64_t i = 0; i < o->n; i++)
+        if (o->life[i] > 0) o->count[o->cell[i]]++;
+}
+
+int orc_set_agents(orc *o, const qhg_agents *a) {
+    if (!a || a->n < 0 || !a->id || !a->cell || !a->life || !a->gender) return QHG_ERR_ARG;
+    reserve(o, a->n + 1);
+    /* id order */
+    int64_t *perm = (int64_t *)malloc((size_t)(a->n + 1) * sizeof(int64_t));
+    for (int64_t i = 0; i < a->n; i++) perm[i] = i;
+    /* simple merge sort by id */
+    int64_t *tmp = (int64_t *)malloc((size_t)(a->n + 1) * sizeof(int64_t));
+    for (int64_t w = 1; w < a->n; w *= 2) {
+        for (int64_t lo = 0; lo < a->n; lo += 2 * w) {
+            int64_t mid = lo + w < a->n ? lo + w : a->n, hi = lo + 2 * w < a->n ? lo + 2 * w : a->n;
+            int64_t i = lo, j = mid, k = lo;
+            while (i < mid && j < hi) tmp[k++] = (a->id[perm[i]] <= a->id[perm[j]]) ? perm[i++] : perm[j++];
+            while (i < mid) tmp[k++] = perm[i++];
+            while (j < hi) tmp[k++] = perm[j++];
+        }
+        memcpy(perm, tmp, (size_t)a->n * sizeof(int64_t));
+    }
+    free(tmp);
+    o->n = 0;
+    o->max_id = 0;
+    for (int64_t q = 0; q < a->n; q++) {
+        int64_t i = perm[q];
+        if (a->life[i] == 0) continue;
+        int64_t k = o->n++;
+        o->id[k] = a->id[i];
+        o->cell[k] = a->cell[i];
+        o->life[k] = a->life[i];
+        o->gender[k] = a->gender[i];
+        o->birth[k] = a->birth_time ? a->birth_time[i] : 0.0;
+        o->age[k] = a->age ? a->age[i] : 0.0;
+        o->lastb[k] = a->last_birth ? a->last_birth[i] : 0.0;
+        o->mate[k] = -3;
+        o->nbabies[k] = a->num_babies ? a->num_babies[i] : 0;
+        if (o->gw) {
+            if (a->genome) memcpy(o->genome + k * o->gw, a->genome + i * o->gw, (size_t)o->gw * 8);
+            else memset(o->genome + k * o->gw, 0, (size_t)o->gw * 8);
+        }
+        if (a->id[i] > o->max_id) o->max_id = a->id[i];
+    }
+    free(perm);
+    update_counts(o);
+    return 0;
+}
+
+int64_t orc_num_alive(orc *o) {
+    int64_t c = 0;
+    for (int64_t i = 0; i < o->n; i++) c += o->life[i] > 0;
+    return c;
+}
+
+int orc_get_agents(orc *o, qhg_agents *a) {
+    int64_t k = 0;
+    for (int64_t i = 0; i < o->n; i++) {
+        if (o->life[i] == 0) continue;
+        if (a->id) a->id[k] = o->id[i];
+        if (a->cell) a->cell[k] = o->cell[i];
+        if (a->life) a->life[k] = o->life[i];
+        if (a->gender) a->gender[k] = o->gender[i];
+        if (a->birth_time) a->birth_time[k] = o->birth[i];
+        if (a->age) a->age[k] = o->age[i];
+        if (a->last_birth) a->last_birth[k] = o->lastb[i];
+        if (a->mate_id) {
+            int32_t m = o->mate[i];
+            a->mate_id[k] = (m >= 0) ? (o->life[m] > 0 ? o->id[m] : -1) : m;
+        }
+        if (a->num_babies) a->num_babies[k] = o->nbabies[i];
+        if (a->genome && o->gw) memcpy(a->genome + k * o->gw, o->genome + i * o->gw, (size_t)o->gw * 8);
+        k++;
+    }
+    a->n = k;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* NPPCapacity::recalculate, QHG4/actions/NPPCapacity.cpp:138-217;     */
+/* Miami NPP, QHG4/core/NPPCalcMiami.cpp                               */
+/* ------------------------------------------------------------------ */
+static void npp_capacity(orc *o) {
+    const qhg_params *P = &o->P;
+    for (int32_t i = 0; i < o->ncells; i++) {
+        double miami = 0;
+        if (o->temp && o->rain) {
+            double nT = 3000.0 / (1 + exp(1.315 - 0.119 * o->temp[i]));
+            double nP = 3000.0 * (1 - exp(-0.000664 * o->rain[i]));
+            miami = 0.000475 * ((nT < nP) ? nT : nP);
+        }
+        double cap;
+        double a = o->alt ? o->alt[i] : 0.0;
+        if (a > 0) {
+            float A0 = 1500, A1 = 2500;
+            double af = (a < A0) ? 1 : ((A1 - a) / (A1 - A0));
+            if (af < 0) af = 0;
+            double tn = o->npp ? o->npp[i] : 0.0;
+            if (o->lon && o->lat && o->lon[i] > 115.0 && o->lat[i] > -12.0 &&
+                o->lon[i] < 150.0 && o->lat[i] < 1.0) {
+                if ((o->npp ? o->npp[i] : 0.0) < P->npp_min) tn = miami;
+            }
+            tn *= af;
+            if (tn < P->npp_min) cap = P->npp_k_min;
+            else if (tn > P->npp_max) cap = P->npp_k_max;
+            else cap = P->npp_k_min + tn * P->npp_k_max / (P->npp_max - P->npp_min);
+            if (o->coastal && o->lat && o->coastal[i] && o->lat[i] > P->npp_coastal_min_lat &&
+                o->lat[i] < P->npp_coastal_max_lat)
+                cap += P->npp_coastal_factor * P->npp_k_max;
+            if (o->water) cap += o->water[i] * P->npp_water_factor * P->npp_k_max;
+            if (cap > P->npp_k_max) cap = P->npp_k_max;
+        } else {
+            cap = 0;
+        }
+        o->K[i] = cap * P->npp_efficiency;
+    }
+}
+
+/* SingleEvaluator::calcValues + exchangeAndCumulate,
+ * QHG4/actions/SingleEvaluator.cpp:174-243 */
+static void single_eval(orc *o, int e, double *out) {
+    const qhg_params *P = &o->P;
+    int32_t R = o->conn + 1;
+    const double *in = NULL;
+    switch (P->me_input[e]) {
+    case QHG_EV_ALTITUDE: in = o->alt; break;
+    case QHG_EV_CAPACITY: in = o->K; break;
+    case QHG_EV_BASE_NPP: in = o->npp; break;
+    }
+    memset(out, 0, (size_t)o->ncells * R * 8);
+    for (int32_t c = 0; c < o->ncells; c++) {
+        if (!o->ice || !o->ice[c]) {
+            double v;
+            if (P->me_pl_n[e] >= 2)
+                v = orc_polyline(P->me_pl_n[e], P->me_pl_x[e], P->me_pl_v[e], (double)(float)in[c]);
+            else
+                v = in[c];
+            out[(size_t)c * R] = (v > 0) ? v : 0;
+        } else {
+            out[(size_t)c * R] = 0;
+        }
+    }
+    for (int32_t c = 0; c < o->ncells; c++) {
+        double w = out[(size_t)c * R];
+        for (int32_t i = 0; i < o->conn; i++) {
+            double cw = 0;
+            int32_t nb = o->nbr[(size_t)c * o->conn + i];
+            if (nb >= 0) cw = out[(size_t)nb * R];
+            cw = (cw > 0) ? cw : 0;
+            w = P->me_cumulate[e] ? w + cw : cw;
+            out[(size_t)c * R + i + 1] = w;
+        }
+    }
+}
+
+/* MultiEvaluator::initialize, QHG4/actions/MultiEvaluator.cpp:142-478 */
+static void multi_eval(orc *o) {
+    const qhg_params *P = &o->P;
+    int32_t R = o->conn + 1;
+    size_t N = (size_t)o->ncells * R;
+    double init = 0;
+    if (P->me_mode == QHG_ME_MUL_SIMPLE) init = 1.0;
+    else if (P->me_mode == QHG_ME_MAX_SIMPLE) init = -INFINITY;
+    else if (P->me_mode == QHG_ME_MIN_SIMPLE) init = INFINITY;
+    for (size_t i = 0; i < N; i++) o->W[i] = init;
+    for (int e = 0; e < P->me_num; e++) {
+        single_eval(o, e, o->single);
+        double w = P->me_weight[e];
+        for (size_t i = 0; i < N; i++) {
+            double s = o->single[i];
+            switch (P->me_mode) {
+            case QHG_ME_ADD_SIMPLE: o->W[i] += s * w; break;
+            case QHG_ME_MUL_SIMPLE: o->W[i] *= s * w; break;
+            case QHG_ME_MAX_SIMPLE: { double nv = s * w; if (o->W[i] < nv) o->W[i] = nv; } break;
+            case QHG_ME_MIN_SIMPLE: { double nv = s * w; if (o->W[i] > nv) o->W[i] = nv; } break;
+            }
+        }
+    }
+    if (P->me_mode != QHG_ME_MAX_SIMPLE) {
+        for (size_t r = 0; r < N; r += R)
+            for (size_t c = r + 1; c < r + R; c++) o->W[c] += o->W[c - 1];
+    }
+}
+
+int orc_pre_loop(orc *o) {
+    if (has_action(o, QHG_ACT_NPPCAP)) npp_capacity(o);
+    update_counts(o);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* initialize phase                                                    */
+/* ------------------------------------------------------------------ */
+
+/* RandomPair::initialize + findMates, QHG4/actions/RandomPair.cpp:107-279 */
+static void random_pair(orc *o) {
+    int64_t n = o->n;
+    int32_t nc = o->ncells;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) o->mate[i] = -3;
+    /* bucket FERTILE agents by cell in id order: counting sort */
+    int32_t *off = o->pc_off;
+    memset(off, 0, (size_t)(nc + 1) * 4);
+    for (int64_t i = 0; i < n; i++)
+        if (o->life[i] == QHG_LIFE_FERTILE && o->gender[i] <= 1) off[o->cell[i] + 1]++;
+    for (int32_t c = 0; c < nc; c++) off[c + 1] += off[c];
+    int32_t *fill = (int32_t *)malloc((size_t)nc * 4);
+    memcpy(fill, off, (size_t)nc * 4);
+    for (int64_t i = 0; i < n; i++)
+        if (o->life[i] == QHG_LIFE_FERTILE && o->gender[i] <= 1) o->pc_buf[fill[o->cell[i]]++] = (int32_t)i;
+    free(fill);
+    uint32_t step = o->step;
+    uint64_t seed = o->P.seed;
+#pragma omp parallel
+    {
+        int32_t tcap = 0;
+        int32_t *fem = NULL, *mal = NULL;
+        uint8_t *taken = NULL;
+#pragma omp for schedule(dynamic, 64)
+        for (int32_t c = 0; c < nc; c++) {
+            int32_t s = off[c], e = off[c + 1];
+            if (e - s < 2) continue;
+            if (e - s > tcap) {
+                tcap = e - s;
+                fem = (int32_t *)realloc(fem, (size_t)tcap * 4);
+                mal = (int32_t *)realloc(mal, (size_t)tcap * 4);
+                taken = (uint8_t *)realloc(taken, (size_t)tcap);
+            }
+            int32_t F = 0, M = 0;
+            for (int32_t q = s; q < e; q++) {
+                int32_t a = o->pc_buf[q];
+                if (o->gender[a] == 0) fem[F++] = a; else mal[M++] = a;
+            }
+            if (F == 0 || M == 0) continue;
+            if (F <= M) {
+                memset(taken, 0, (size_t)M);
+                int32_t left = M;
+                for (int32_t f = 0; f < F && left > 0; f++) {
+                    uint32_t u = orc_draw(seed, step, QHG_STREAM_PAIR, (uint64_t)o->id[fem[f]], 0);
+                    int32_t choose = (int32_t)orc_wrandr(0, left, u);
+                    int32_t found = -1, m = -1;
+                    while (found < choose) { m++; if (!taken[m]) found++; }
+                    taken[m] = 1;
+                    o->mate[fem[f]] = mal[m];
+                    o->mate[mal[m]] = fem[f];
+                    left--;
+                }
+            } else {
+                memset(taken, 0, (size_t)F);
+                int32_t left = F;
+                for (int32_t m = 0; m < M && left > 0; m++) {
+                    uint32_t u = orc_draw(seed, step, QHG_STREAM_PAIR, (uint64_t)o->id[mal[m]], 0);
+                    int32_t choose = (int32_t)orc_wrandr(0, left, u);
+                    int32_t found = -1, f = -1;
+                    while (found < choose) { f++; if (!taken[f]) found++; }
+                    taken[f] = 1;
+                    o->mate[fem[f]] = mal[m];
+                    o->mate[mal[m]] = fem[f];
+                    left--;
+                }
+            }
+        }
+        free(fem); free(mal); free(taken);
+    }
+}
+
+/* LinearBirth::initialize / LinearDeath::initialize,
+ * QHG4/actions/LinearBirth.cpp:90-115, LinearDeath.cpp:92-123 */
+static void verhulst_init(orc *o, int varK) {
+    const qhg_params *P = &o->P;
+#pragma omp parallel for schedule(static)
+    for (int32_t c = 0; c < o->ncells; c++) {
+        double N = (double)o->count[c];
+        if (!varK) {
+            o->bc[c] = P->verh_b0 + (P->verh_theta - P->verh_b0) * (N / P->verh_K);
+            o->dc[c] = P->verh_d0 + (P->verh_theta - P->verh_d0) * (N / P->verh_K);
+        } else {
+            double K = o->K[c];
+            o->bc[c] = (K <= 0) ? 0 : P->verh_b0 + (P->verh_theta - P->verh_b0) * (N / K);
+            o->dc[c] = (K <= 0) ? 1 : P->verh_d0 + (P->verh_theta - P->verh_d0) * (N / K);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* execute phase: one loop over all agents per action (doActions)      */
+/* ------------------------------------------------------------------ */
+static void act_getold(orc *o, double t) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < o->n; i++)
+        if (o->life[i] > 0) o->age[i] = t - o->birth[i];
+}
+
+static void act_fertility(orc *o, double t) {
+    const qhg_params *P = &o->P;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < o->n; i++) {
+        if (o->life[i] == 0) continue;
+        if (o->gender[i] == 0) {
+            o->life[i] = (o->age[i] > P->fert_min_age && o->age[i] < P->fert_max_age &&
+                          (t - o->lastb[i]) > P->fert_interbirth) ? QHG_LIFE_FERTILE : QHG_LIFE_ALIVE;
+        } else {
+            o->life[i] = (o->age[i] > P->fert_min_age) ? QHG_LIFE_FERTILE : QHG_LIFE_ALIVE;
+        }
+    }
+}
+
+/* Verhulst::execute = LinearBirth::execute then LinearDeath::execute */
+static void act_verhulst(orc *o) {
+    uint32_t step = o->step;
+    uint64_t seed = o->P.seed;
+    int64_t deaths = 0;
+#pragma omp parallel for schedule(static) reduction(+:deaths)
+    for (int64_t i = 0; i < o->n; i++) {
+        if (o->life[i] == 0) continue;
+        int32_t c = o->cell[i];
+        double b = o->bc[c];
+        if (b > 0) {
+            if (o->gender[i] == 0 && o->mate[i] >= 0) {
+                double r = orc_wrandd(orc_draw(seed, step, QHG_STREAM_LBIRTH, (uint64_t)o->id[i], 0));
+                if (r < b) o->bflag[i] = 1;
+            }
+        } else if (b < 0) {
+            double r = orc_wrandd(orc_draw(seed, step, QHG_STREAM_LBIRTH, (uint64_t)o->id[i], 0));
+            if (r < -b) { o->life[i] = 0; deaths++; }
+        }
+        if (o->life[i] > 0) {
+            double r = orc_wrandd(orc_draw(seed, step, QHG_STREAM_LDEATH, (uint64_t)o->id[i], 0));
+            if (r < o->dc[c]) { o->life[i] = 0; deaths++; }
+        }
+    }
+    o->deaths_last += deaths;
+}
+
+static void act_randommove(orc *o) {
+    uint32_t step = o->step;
+    uint64_t seed = o->P.seed;
+    double p = o->P.rmove_prob;
+    int64_t moves = 0;
+#pragma omp parallel for schedule(static) reduction(+:moves)
+    for (int64_t i = 0; i < o->n; i++) {
+        if (o->life[i] == 0) continue;
+        int32_t c = o->cell[i];
+        double r = orc_wrandd(orc_draw(seed, step, QHG_STREAM_RMOVE, (uint64_t)o->id[i], 0));
+        if (r < p) {
+            double r2 = orc_wrandd(orc_draw(seed, step, QHG_STREAM_RMOVE, (uint64_t)o->id[i], 1));
+            int32_t slot = (int32_t)(r2 * (o->nn[c] + 1));
+            if (slot > 0) {
+                int32_t nb = o->nbr[(size_t)c * o->conn + slot - 1];
+                if (nb >= 0) { o->dest[i] = nb; moves++; }
+            }
+        }
+    }
+    o->moves_last += moves;
+}
+
+static void act_weightedmove(orc *o) {
+    uint32_t step = o->step;
+    uint64_t seed = o->P.seed;
+    double p = o->P.wmove_prob;
+    int32_t R = o->conn + 1;
+    int64_t moves = 0;
+#pragma omp parallel for schedule(static) reduction(+:moves)
+    for (int64_t i = 0; i < o->n; i++) {
+        if (o->life[i] == 0) continue;
+        int32_t c = o->cell[i];
+        double r = orc_wrandd(orc_draw(seed, step, QHG_STREAM_WMOVE, (uint64_t)o->id[i], 0));
+        if (r < p) {
+            int32_t nn = o->nn[c];
+            const double *w = o->W + (size_t)c * R;
+            int32_t idx = -1;
+            uint32_t u2 = orc_draw(seed, step, QHG_STREAM_WMOVE, (uint64_t)o->id[i], 1);
+            if (w[0] == w[nn]) {
+                idx = (int32_t)orc_wrandi(0, (uint32_t)nn + 1, 1, u2);
+            } else {
+                double r2 = orc_wrandd(u2) * w[nn];
+                for (int32_t q = 0; q < nn + 1; q++) if (r2 < w[q]) { idx = q; break; }
+            }
+            if (idx > 0) {
+                int32_t nb = o->nbr[(size_t)c * o->conn + idx - 1];
+                if (nb >= 0 && (!o->ice || !o->ice[nb])) { o->dest[i] = nb; moves++; }
+            }
+        }
+    }
+    o->moves_last += moves;
+}
+
+/* ------------------------------------------------------------------ */
+/* finalize: births (recycleDeadSpaceNew + createAgentAtIndex +        */
+/* makePopSpecificOffspring + Genetics::makeOffspring), moves, counts  */
+/* ------------------------------------------------------------------ */
+static void make_genome(orc *o, int64_t baby, int64_t mother, int64_t father, uint64_t nid) {
+    const qhg_params *P = &o->P;
+    int32_t nb = o->gw / 2;
+    uint64_t *out = o->genome + baby * o->gw;
+    const uint64_t *g1 = o->genome + mother * o->gw;
+    const uint64_t *g2 = o->genome + father * o->gw;
+    uint64_t seed = P->seed;
+    uint32_t step = o->step;
+    uint32_t n = 0;
+#define GD() orc_draw(seed, step, QHG_STREAM_GENETICS, nid, n++)
+    int i1 = (int)(2 * 1.0 * orc_wrandd(GD()));
+    int i2 = (int)(2 * 1.0 * orc_wrandd(GD()));
+    uint64_t t1[128], t2[128];
+    int32_t C = P->gen_num_crossover;
+    if (C > 0) {
+        uint32_t br[256];
+        uint32_t nbits = (uint32_t)nb * 64;
+        for (int32_t k = 0; k < C; k++) br[k] = orc_wrandi(0, nbits, 2, GD());
+        orc_crossover(t1, g1, P->gen_genome_size, br, C);
+        for (int32_t k = 0; k < C; k++) br[k] = orc_wrandi(0, nbits, 2, GD());
+        orc_crossover(t2, g2, P->gen_genome_size, br, C);
+        memcpy(out, t1 + i1 * nb, (size_t)nb * 8);
+        memcpy(out + nb, t2 + i2 * nb, (size_t)nb * 8);
+    } else if (C == -1) {
+        uint32_t d[256];
+        for (int32_t k = 0; k < 2 * nb; k++) d[k] = GD();
+        orc_free_reco(t1, g1, nb, d);
+        for (int32_t k = 0; k < 2 * nb; k++) d[k] = GD();
+        orc_free_reco(t2, g2, nb, d);
+        memcpy(out, t1 + i1 * nb, (size_t)nb * 8);
+        memcpy(out + nb, t2 + i2 * nb, (size_t)nb * 8);
+    } else {
+        memcpy(out, g1 + i1 * nb, (size_t)nb * 8);
+        memcpy(out + nb, g2 + i2 * nb, (size_t)nb * 8);
+    }
+    if (P->gen_mutation_rate > 0) {
+        int32_t k = orc_bino_getn(o->bino, o->nbino, orc_wrandd(GD()));
+        for (int32_t j = 0; j < k; j++) {
+            uint32_t pd = GD();
+            uint32_t md = GD();
+            orc_mutate_nucs(out, 2 * P->gen_genome_size, 1, &pd, &md);
+        }
+    }
+#undef GD
+}
+
+static void finalize(orc *o, double t) {
+    /* births in mother-id order */
+    int64_t nb = 0;
+    for (int64_t i = 0; i < o->n; i++) nb += o->bflag[i];
+    reserve(o, o->n + nb + 1);
+    int64_t base = o->n;
+    int64_t k = 0;
+    for (int64_t i = 0; i < base; i++) {
+        if (!o->bflag[i]) continue;
+        int64_t s = base + k;
+        int64_t nid = o->max_id + 1 + k;
+        o->id[s] = nid;
+        o->cell[s] = o->cell[i];                 /* registered cell (pre-move) */
+        o->birth[s] = t;
+        uint32_t u = orc_draw(o->P.seed, o->step, QHG_STREAM_NEWBORN, (uint64_t)nid, 0);
+        o->gender[s] = (uint8_t)(2 * orc_wrandd(u));
+        o->life[s] = (o->gender[s] == 0) ? QHG_LIFE_FERTILE : QHG_LIFE_ALIVE;
+        o->age[s] = 0.0;
+        o->lastb[s] = 0.0;
+        o->mate[s] = -3;
+        o->nbabies[s] = 0;
+        o->dest[s] = -1;
+        o->bflag[s] = 0;
+        o->nbabies[i]++;
+        if (o->gw) make_genome(o, s, i, o->mate[i], (uint64_t)nid);
+        k++;
+    }
+    o->n = base + nb;
+    o->max_id += nb;
+    o->births_last = nb;
+    /* performMoves */
+    for (int64_t i = 0; i < base; i++) {
+        if (o->dest[i] >= 0) { o->cell[i] = o->dest[i]; o->dest[i] = -1; }
+        o->bflag[i] = 0;
+    }
+    /* drop the dead (slot recycling is invisible to parity) and remap mates */
+    int32_t *newidx = (int32_t *)malloc((size_t)o->n * 4);
+    int64_t w = 0;
+    for (int64_t i = 0; i < o->n; i++) newidx[i] = (o->life[i] > 0) ? (int32_t)w++ : -1;
+    w = 0;
+    for (int64_t i = 0; i < o->n; i++) {
+        if (o->life[i] == 0) continue;
+        int32_t m = o->mate[i];
+        o->id[w] = o->id[i]; o->cell[w] = o->cell[i]; o->life[w] = o->life[i];
+        o->gender[w] = o->gender[i]; o->birth[w] = o->birth[i]; o->age[w] = o->age[i];
+        o->lastb[w] = o->lastb[i]; o->nbabies[w] = o->nbabies[i];
+        o->mate[w] = (m >= 0) ? newidx[m] : m;
+        o->dest[w] = -1; o->bflag[w] = 0;
+        if (o->gw && w != i) memcpy(o->genome + w * o->gw, o->genome + i * o->gw, (size_t)o->gw * 8);
+        w++;
+    }
+    free(newidx);
+    o->n = w;
+    update_counts(o);
+}
+
+int orc_step(orc *o) {
+    double t = (double)o->step;
+    o->births_last = o->deaths_last = o->moves_last = 0;
+    for (int64_t i = 0; i < o->n; i++) { o->dest[i] = -1; o->bflag[i] = 0; }
+    /* initializeStep: every action's initialize, in prio order */
+    for (int a = 0; a < o->nact; a++) {
+        switch (o->act[a]) {
+        case QHG_ACT_RANDOMPAIR: random_pair(o); break;
+        case QHG_ACT_VERHULST: verhulst_init(o, 0); break;
+        case QHG_ACT_VERHULSTVARK: verhulst_init(o, 1); break;
+        case QHG_ACT_MULTIEVAL: if (o->me_first) { multi_eval(o); o->me_first = 0; } break;
+        default: break;
+        }
+    }
+    /* doActions for every prio level */
+    for (int a = 0; a < o->nact; a++) {
+        switch (o->act[a]) {
+        case QHG_ACT_GETOLD: act_getold(o, t); break;
+        case QHG_ACT_FERTILITY: act_fertility(o, t); break;
+        case QHG_ACT_VERHULST:
+        case QHG_ACT_VERHULSTVARK: act_verhulst(o); break;
+        case QHG_ACT_RANDOMMOVE: act_randommove(o); break;
+        case QHG_ACT_WEIGHTEDMOVE: act_weightedmove(o); break;
+        default: break;
+        }
+    }
+    finalize(o, t);
+    o->births_total += o->births_last;
+    o->deaths_total += o->deaths_last;
+    o->moves_total += o->moves_last;
+    o->step++;
+    return 0;
+}
+
+int orc_run(orc *o, int32_t nsteps) {
+    for (int32_t s = 0; s < nsteps; s++) orc_step(o);
+    return 0;
+}
+
+void orc_stats(orc *o, qhg_stats *s) {
+    memset(s, 0, sizeof(*s));
+    s->num_alive = orc_num_alive(o);
+    s->high_water = o->n;
+    s->capacity = o->cap;
+    s->max_id = o->max_id;
+    s->step = o->step;
+    s->births_last = o->births_last; s->deaths_last = o->deaths_last; s->moves_last = o->moves_last;
+    s->births_total = o->births_total; s->deaths_total = o->deaths_total; s->moves_total = o->moves_total;
+}
+
+void orc_cell_counts(orc *o, uint32_t *out) { memcpy(out, o->count, (size_t)o->ncells * 4); }
+void orc_capacities(orc *o, double *out) { memcpy(out, o->K, (size_t)o->ncells * 8); }
+void orc_env_weights(orc *o, double *out) { memcpy(out, o->W, (size_t)o->ncells * (o->conn + 1) * 8); }

@@ -1,0 +1,61 @@
+/*
+ * qhg_oracle.h -- CPU oracle of the QHG4 per-step population loop.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the
+ * MI355X path and the CPU baseline that bench.py times beside it.  Nothing
+ * in the product (qhg4_amd/) links, loads or calls it.
+ *
+ * It restates the reference algorithm (cited per function in the .c file)
+ * with every WELL512 draw replaced by a counter-based Philox4x32-10 draw
+ * keyed by (seed, step, stream, agent id, draw#).  Agents are kept in id
+ * order; slot recycling of the reference allocator is not modelled (slot
+ * indices are not part of parity).
+ */
+#ifndef QHG_ORACLE_H
+#define QHG_ORACLE_H
+
+#include "../include/qhg_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc orc;
+
+int      orc_create(const qhg_grid_desc *grid, const qhg_env_desc *env,
+                    const qhg_params *params, orc **out);
+void     orc_destroy(orc *o);
+int      orc_set_agents(orc *o, const qhg_agents *a);
+int64_t  orc_num_alive(orc *o);
+int      orc_get_agents(orc *o, qhg_agents *a);
+int      orc_pre_loop(orc *o);
+int      orc_step(orc *o);
+int      orc_run(orc *o, int32_t nsteps);
+void     orc_stats(orc *o, qhg_stats *s);
+void     orc_cell_counts(orc *o, uint32_t *out);
+void     orc_capacities(orc *o, double *out);
+void     orc_env_weights(orc *o, double *out);
+void     orc_set_threads(int32_t n);
+int32_t  orc_get_threads(void);
+
+/* building blocks, exported so tests can pin them one by one */
+void     orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+uint32_t orc_draw(uint64_t seed, uint32_t step, uint32_t stream, uint64_t id, uint32_t n);
+double   orc_wrandd(uint32_t u);
+double   orc_wrandr(double a, double b, uint32_t u);
+uint32_t orc_wrandi(uint32_t a, uint32_t b, uint32_t s, uint32_t u);
+uint64_t orc_multimask(const uint32_t *breaks, int32_t n);
+void     orc_crossover(uint64_t *out, const uint64_t *parent, int32_t genome_size,
+                       const uint32_t *break_pos, int32_t nbreaks);
+void     orc_mutate_nucs(uint64_t *genome, int32_t genome_size, int32_t nmut,
+                         const uint32_t *pos_draws, const uint32_t *mask_draws);
+int32_t  orc_bino_table(double p, int32_t n, double eps, double *cdf, int32_t max);
+int32_t  orc_bino_getn(const double *cdf, int32_t nbins, double u);
+double   orc_betai(double a, double b, double x);
+double   orc_polyline(int32_t npts, const double *x, const double *v, double X);
+int32_t  orc_genome_words(int32_t genome_size);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,75 @@
+"""Loader for the in-tree HIP library (qhg4_amd/libqhg4_amd.so).
+
+There is no CPU fallback: if the library is missing or no GPU is present the
+calls raise.
+"""
+import ctypes as C
+import os
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqhg4_amd.so")
+_lib = None
+
+# every symbol declared in include/qhg_gpu.h
+EXPORTS = [
+    "qhg_gpu_abi_version", "qhg_gpu_last_error", "qhg_gpu_genome_words", "qhg_gpu_binomial_table",
+    "qhg_gpu_create", "qhg_gpu_destroy", "qhg_gpu_upload_agents", "qhg_gpu_num_alive",
+    "qhg_gpu_download_agents", "qhg_gpu_pre_loop", "qhg_gpu_initialize_step", "qhg_gpu_do_actions",
+    "qhg_gpu_finalize_step", "qhg_gpu_run", "qhg_gpu_set_env", "qhg_gpu_sync", "qhg_gpu_stats",
+    "qhg_gpu_cell_counts", "qhg_gpu_capacities", "qhg_gpu_env_weights", "qhg_gpu_compact",
+    "qhg_gpu_set_timing", "qhg_gpu_kernel_times", "qhg_gpu_kernel_name",
+]
+
+
+class QhgError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise QhgError("%s not built: run `python -m qhg4_amd.build` (or __graft_entry__.build())" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    h = C.c_void_p
+    sig = {
+        "qhg_gpu_abi_version": ([], C.c_int),
+        "qhg_gpu_last_error": ([], C.c_char_p),
+        "qhg_gpu_genome_words": ([C.c_int32], C.c_int),
+        "qhg_gpu_binomial_table": ([C.c_double, C.c_int32, C.c_double, P(C.c_double), P(C.c_int32)], C.c_int),
+        "qhg_gpu_create": ([P(abi.GridDesc), P(abi.EnvDesc), P(abi.Params), C.c_int64, P(h)], C.c_int),
+        "qhg_gpu_destroy": ([h], C.c_int),
+        "qhg_gpu_upload_agents": ([h, P(abi.Agents)], C.c_int),
+        "qhg_gpu_num_alive": ([h, P(C.c_int64)], C.c_int),
+        "qhg_gpu_download_agents": ([h, P(abi.Agents)], C.c_int),
+        "qhg_gpu_pre_loop": ([h, C.c_double], C.c_int),
+        "qhg_gpu_initialize_step": ([h, C.c_double], C.c_int),
+        "qhg_gpu_do_actions": ([h, C.c_uint32, C.c_double], C.c_int),
+        "qhg_gpu_finalize_step": ([h], C.c_int),
+        "qhg_gpu_run": ([h, C.c_int32], C.c_int),
+        "qhg_gpu_set_env": ([h, P(abi.EnvDesc)], C.c_int),
+        "qhg_gpu_sync": ([h], C.c_int),
+        "qhg_gpu_stats": ([h, P(abi.Stats)], C.c_int),
+        "qhg_gpu_cell_counts": ([h, P(C.c_uint32)], C.c_int),
+        "qhg_gpu_capacities": ([h, P(C.c_double)], C.c_int),
+        "qhg_gpu_env_weights": ([h, P(C.c_double)], C.c_int),
+        "qhg_gpu_compact": ([h], C.c_int),
+        "qhg_gpu_set_timing": ([h, C.c_int32], C.c_int),
+        "qhg_gpu_kernel_times": ([h, P(C.c_double), P(C.c_int64), C.c_int32, P(C.c_int32)], C.c_int),
+        "qhg_gpu_kernel_name": ([C.c_int32], C.c_char_p),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def check(rc, what):
+    if rc != 0:
+        raise QhgError("%s failed (%d): %s" % (what, rc, lib().qhg_gpu_last_error().decode()))

@@ -1,0 +1,204 @@
+// Per-cell environment kernels (run at pre-loop and on environment events)
+// and the stable compaction of the agent slots.
+#pragma once
+#include "step_kernels.hpp"
+
+namespace qhg {
+
+struct EnvPtrs {
+    const double *alt, *lon, *lat, *water, *npp, *temp, *rain;
+    const uint8_t *ice, *coastal;
+};
+
+struct CapParams {
+    double water_factor, coastal_factor, coastal_min_lat, coastal_max_lat;
+    double npp_min, npp_max, k_min, k_max, efficiency;
+};
+
+// NPPCapacity::recalculate, QHG4/actions/NPPCapacity.cpp:138-217, with the
+// Miami NPP of QHG4/core/NPPCalcMiami.cpp for the Oceania substitution.
+__global__ void __launch_bounds__(kBlock) k_capacity(Ctx x, EnvPtrs e, CapParams P) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < x.ncells;
+         i += (int64_t)gridDim.x * kBlock) {
+        double miami = 0;
+        if (e.temp && e.rain) {
+            const double nT = 3000.0 / (1 + exp(1.315 - 0.119 * e.temp[i]));
+            const double nP = 3000.0 * (1 - exp(-0.000664 * e.rain[i]));
+            miami = 0.000475 * ((nT < nP) ? nT : nP);
+        }
+        double cap;
+        const double a = e.alt ? e.alt[i] : 0.0;
+        if (a > 0) {
+            const float A0 = 1500, A1 = 2500;
+            double af = (a < A0) ? 1 : ((A1 - a) / (A1 - A0));
+            if (af < 0) af = 0;
+            const double npp = e.npp ? e.npp[i] : 0.0;
+            double tn = npp;
+            if (e.lon && e.lat && e.lon[i] > 115.0 && e.lat[i] > -12.0 && e.lon[i] < 150.0 && e.lat[i] < 1.0) {
+                if (npp < P.npp_min) tn = miami;
+            }
+            tn *= af;
+            if (tn < P.npp_min) cap = P.k_min;
+            else if (tn > P.npp_max) cap = P.k_max;
+            else cap = P.k_min + tn * P.k_max / (P.npp_max - P.npp_min);
+            if (e.coastal && e.lat && e.coastal[i] && e.lat[i] > P.coastal_min_lat && e.lat[i] < P.coastal_max_lat)
+                cap += P.coastal_factor * P.k_max;
+            if (e.water) cap += e.water[i] * P.water_factor * P.k_max;
+            if (cap > P.k_max) cap = P.k_max;
+        } else {
+            cap = 0;
+        }
+        x.K[i] = cap * P.efficiency;
+    }
+}
+
+struct EvalParams {
+    int32_t mode, cumulate, npts, first, last, pad;
+    double weight;
+    double px[QHG_MAX_PL_POINTS], pv[QHG_MAX_PL_POINTS];
+};
+
+// PolyLine::getVal, QHG4/utils/PolyLine.cpp:60-89
+__device__ double polyline(const EvalParams &p, double X) {
+    const int ns = p.npts - 1;
+    if (X >= p.px[ns]) return p.pv[ns];
+    int i = 0;
+    while (i <= ns && X > p.px[i]) i++;
+    if (i == 0) return p.pv[0];
+    if (i <= ns) {
+        const double a = (p.pv[i] - p.pv[i - 1]) / (p.px[i] - p.px[i - 1]);
+        return p.pv[i - 1] + a * (X - p.px[i - 1]);
+    }
+    return p.pv[ns];
+}
+
+// SingleEvaluator::calcValues, QHG4/actions/SingleEvaluator.cpp:174-207
+__global__ void __launch_bounds__(kBlock) k_eval_values(Ctx x, const double *in, double *tmp, EvalParams p) {
+    for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < x.ncells;
+         c += (int64_t)gridDim.x * kBlock) {
+        double v = 0;
+        if (x.ice == nullptr || !x.ice[c]) {
+            v = (p.npts >= 2) ? polyline(p, (double)(float)in[c]) : in[c];
+            v = (v > 0) ? v : 0;
+        }
+        tmp[c] = v;
+    }
+}
+
+// SingleEvaluator::exchangeAndCumulate (SingleEvaluator.cpp:216-243) folded
+// into MultiEvaluator's combination (MultiEvaluator.cpp:221-478); the last
+// evaluator also takes the in-row prefix sum.
+__global__ void __launch_bounds__(kBlock) k_eval_combine(Ctx x, const double *tmp, EvalParams p) {
+    const int R = x.conn + 1;
+    for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < x.ncells;
+         c += (int64_t)gridDim.x * kBlock) {
+        double *w = x.W + c * R;
+        double s = tmp[c];
+        double acc = s;
+        for (int j = 0; j < R; j++) {
+            double sj;
+            if (j == 0) {
+                sj = s;
+            } else {
+                const int32_t nb = x.nbr[c * x.conn + j - 1];
+                double cw = (nb >= 0) ? tmp[nb] : 0.0;
+                cw = (cw > 0) ? cw : 0;
+                acc = p.cumulate ? acc + cw : cw;
+                sj = acc;
+            }
+            double o;
+            if (p.first) {
+                o = (p.mode == QHG_ME_MUL_SIMPLE) ? 1.0
+                  : (p.mode == QHG_ME_MAX_SIMPLE) ? -INFINITY
+                  : (p.mode == QHG_ME_MIN_SIMPLE) ? INFINITY : 0.0;
+            } else {
+                o = w[j];
+            }
+            const double nv = sj * p.weight;
+            switch (p.mode) {
+            case QHG_ME_ADD_SIMPLE: o += nv; break;
+            case QHG_ME_MUL_SIMPLE: o *= nv; break;
+            case QHG_ME_MAX_SIMPLE: if (o < nv) o = nv; break;
+            case QHG_ME_MIN_SIMPLE: if (o > nv) o = nv; break;
+            }
+            w[j] = o;
+        }
+        if (p.last && p.mode != QHG_ME_MAX_SIMPLE) {
+            for (int j = 1; j < R; j++) w[j] += w[j - 1];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// stable compaction of live slots into the other buffer set
+// (LBController::compactData, QHG4/utils/LBController.cpp:351-445, done as a
+// scan).  Pass 1 scatters every field but mate and records old->new slot;
+// pass 2 remaps mates (a dead partner becomes -1); k_commit_compact sets hw.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_compact_scan(Ctx x, AgentPtrs dst, int32_t *newidx,
+                                                         uint64_t *tiles, int64_t *hw_next) {
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_wave[kBlock / 64];
+    __shared__ uint64_t s_excl;
+    const int64_t hw = x.st->hw;
+    while (true) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(&x.st->tile_ctr_cmp, 1u);
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        const int64_t base = (int64_t)tile * kActTile;
+        if (base >= hw) break;
+        const int64_t i = base + threadIdx.x;
+        const uint32_t alive = (i < hw) ? (x.a.life[i] != 0) : 0u;
+        uint32_t tot;
+        const uint32_t rank = block_excl_scan(alive, s_wave, tot);
+        if (threadIdx.x == 0) {
+            const uint64_t ex = lookback(tiles, tile, tot);
+            s_excl = ex;
+            if (base + kActTile >= hw) *hw_next = (int64_t)(ex + tot);
+        }
+        __syncthreads();
+        if (i < hw) {
+            if (alive) {
+                const int64_t j = (int64_t)s_excl + rank;
+                newidx[i] = (int32_t)j;
+                dst.id[j] = x.a.id[i];
+                dst.cell[j] = x.a.cell[i];
+                dst.life[j] = x.a.life[i];
+                dst.gender[j] = x.a.gender[i];
+                dst.birth[j] = x.a.birth[i];
+                dst.age[j] = x.a.age[i];
+                dst.lastb[j] = x.a.lastb[i];
+                dst.nbabies[j] = x.a.nbabies[i];
+                if (x.gwords) {
+                    const uint64_t *src = x.a.genome + i * x.gwords;
+                    uint64_t *d = dst.genome + j * x.gwords;
+                    for (int q = 0; q < x.gwords; q++) d[q] = src[q];
+                }
+            } else {
+                newidx[i] = -1;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_compact_mate(Ctx x, AgentPtrs dst, const int32_t *newidx) {
+    const int64_t hw = x.st->hw;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < hw;
+         i += (int64_t)gridDim.x * kBlock) {
+        const int32_t j = newidx[i];
+        if (j < 0) continue;
+        const int32_t m = x.a.mate[i];
+        dst.mate[j] = (m >= 0) ? newidx[m] : m;
+    }
+}
+
+__global__ void k_commit_compact(DevStatus *st, const int64_t *hw_next) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st->hw = *hw_next;
+        st->n_alive = *hw_next;
+        st->tile_ctr_cmp = 0;
+    }
+}
+
+}  // namespace qhg

@@ -1,0 +1,694 @@
+// Device side of the population step engine (gfx950, wave64).
+//
+// One step = five launches on the handle's stream, all with fixed grids so a
+// step can be captured once into a hipGraph:
+//   k_bucket   per agent slot   RandomPair::initialize (mate := -3) and the
+//                               per-cell bucketing of FERTILE agents
+//                               (QHG4/actions/RandomPair.cpp:107-181)
+//   k_pair     per cell         sort candidates by id, pick the k-th
+//                               unpaired partner (RandomPair.cpp:183-276)
+//   k_act      per agent slot   every Action<T>::execute of the step fused in
+//                               priority order (QHG4/core/SPopulation.cpp:554-577)
+//                               plus a decoupled look-back scan that writes
+//                               the birth records in mother-id order
+//   k_newborn  per birth        createAgentAtIndex + makePopSpecificOffspring
+//                               + Genetics::makeOffspring
+//                               (SPopulation.cpp:838-918, Genetics.cpp:285-337)
+//   k_cellscan per cell         updateNumAgentsPerCell result -> per-cell
+//                               offsets for the next pairing; step bookkeeping
+//                               (SPopulation.cpp:509-544)
+//
+// Agents live in structure-of-arrays slots [0, hw) in id order: births are
+// appended at hw with increasing ids and compaction is stable, so slot order
+// is id order.  Dead agents stay as holes (life == 0) until compaction.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "philox.hpp"
+#include "../../include/qhg_gpu.h"
+
+namespace qhg {
+
+constexpr int kBlock = 256;          // threads per block (4 waves)
+constexpr int kActTile = 256;        // agent slots per look-back tile
+constexpr int kCellTile = 1024;      // cells per look-back tile (4 per thread)
+constexpr int kMaxOps = QHG_MAX_ACTIONS;
+constexpr int kMaxCross = 32;        // device limit on Genetics_num_crossover
+
+struct DevStatus {
+    int64_t hw, n_alive, max_id, step;
+    double t;
+    int64_t births_cur, deaths_cur, moves_cur;
+    int64_t births_last, deaths_last, moves_last;
+    int64_t births_total, deaths_total, moves_total;
+    uint32_t tile_ctr_act, tile_ctr_cell, tile_ctr_cmp, pad0;
+    int32_t error, pad1;
+};
+
+// device error bits
+constexpr int32_t kErrCapacity = 1;   // births did not fit
+constexpr int32_t kErrBucket = 2;     // fertile agents exceed the cell's live count
+
+struct AgentPtrs {
+    int64_t *id;
+    int32_t *cell;
+    uint8_t *life;
+    uint8_t *gender;
+    double *birth;
+    double *age;
+    double *lastb;
+    int32_t *mate;
+    int32_t *nbabies;
+    uint64_t *genome;
+};
+
+struct Ctx {
+    AgentPtrs a;
+    int64_t cap;
+    int32_t gwords;        // u64 per genome (both strands), 0 = no genetics
+    int32_t ncells, conn;
+    const int32_t *nbr;
+    const uint8_t *nn;
+    double *K;
+    double *W;
+    const uint8_t *ice;
+    uint32_t *cnt[2];
+    uint32_t *off;         // ncells + 1
+    uint32_t *fillF, *fillM;
+    uint32_t *bucket;      // cap
+    int32_t *brec;         // 3 * cap: mother slot, father slot, cell
+    uint64_t *tiles_act, *tiles_cell;
+    int64_t ntiles_act_max;
+    DevStatus *st;
+    uint64_t seed;
+};
+
+struct Prog {
+    int32_t nops;
+    int32_t ops[kMaxOps];
+    int32_t load_age;        // Fertility runs before GetOld (reads stored age)
+    int32_t has_pair;
+    double fert_min, fert_max, fert_inter;
+    double b0, d0, theta, Kconst;
+    double rmove_p, wmove_p;
+    int32_t gen_size, gen_cross, nbino, pad;
+    double bino[QHG_MAX_BINO];
+};
+
+// ---------------------------------------------------------------------------
+// decoupled look-back (single pass scan), one word per tile:
+// [63:62] flag (1 aggregate, 2 inclusive), [61:0] value.  Tile words are
+// zeroed by the kernel launched before the scanning kernel.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kFlagAgg = 1ull << 62;
+constexpr uint64_t kFlagInc = 2ull << 62;
+constexpr uint64_t kValMask = (1ull << 62) - 1;
+
+__device__ __forceinline__ void tile_publish(uint64_t *w, uint64_t v) {
+    __hip_atomic_store(w, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t tile_read(uint64_t *w) {
+    return __hip_atomic_load(w, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Called by one thread. Returns the exclusive prefix of `tile`.
+__device__ uint64_t lookback(uint64_t *tiles, uint32_t tile, uint64_t agg) {
+    if (tile == 0) {
+        tile_publish(&tiles[0], kFlagInc | agg);
+        return 0;
+    }
+    tile_publish(&tiles[tile], kFlagAgg | agg);
+    uint64_t excl = 0;
+    int64_t j = (int64_t)tile - 1;
+    while (true) {
+        uint64_t w = tile_read(&tiles[j]);
+        uint64_t flag = w & ~kValMask;
+        if (flag == 0) { __builtin_amdgcn_s_sleep(1); continue; }
+        excl += w & kValMask;
+        if (flag == kFlagInc) break;
+        j--;
+    }
+    tile_publish(&tiles[tile], kFlagInc | (excl + agg));
+    return excl;
+}
+
+// exclusive block scan of one u32 per thread (256 threads, wave64)
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_wave, uint32_t &total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) s_wave[wave] = x;
+    __syncthreads();
+    uint32_t wpre = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; w++) {
+        uint32_t s = s_wave[w];
+        if (w < wave) wpre += s;
+        tot += s;
+    }
+    total = tot;
+    return wpre + x - v;
+}
+
+// ---------------------------------------------------------------------------
+// k_bucket: RandomPair::initialize -- mate := -3 for all, FERTILE agents
+// go into their cell's region [off[c], off[c+1]): females from the front,
+// males from the back (region size = live count of the cell >= F + M).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_bucket(Ctx x) {
+    const int64_t hw = x.st->hw;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < hw;
+         i += (int64_t)gridDim.x * kBlock) {
+        const uint8_t l = x.a.life[i];
+        if (l == 0) continue;
+        x.a.mate[i] = -3;
+        if (l != QHG_LIFE_FERTILE) continue;
+        const int32_t c = x.a.cell[i];
+        const uint8_t g = x.a.gender[i];
+        const uint32_t lo = x.off[c], hi = x.off[c + 1];
+        if (g == 0) {
+            const uint32_t p = atomicAdd(&x.fillF[c], 1u);
+            if (lo + p < hi) x.bucket[lo + p] = (uint32_t)i;
+            else atomicOr(&x.st->error, kErrBucket);
+        } else if (g == 1) {
+            const uint32_t p = atomicAdd(&x.fillM[c], 1u);
+            if (p < hi - lo) x.bucket[hi - 1 - p] = (uint32_t)i;
+            else atomicOr(&x.st->error, kErrBucket);
+        }
+    }
+}
+
+__device__ void sort_u32(uint32_t *v, uint32_t n) {
+    if (n < 2) return;
+    if (n <= 48) {
+        for (uint32_t i = 1; i < n; i++) {
+            uint32_t k = v[i];
+            int32_t j = (int32_t)i - 1;
+            while (j >= 0 && v[j] > k) { v[j + 1] = v[j]; j--; }
+            v[j + 1] = k;
+        }
+        return;
+    }
+    // shell sort (Ciura gaps extended by x2.25)
+    const uint32_t gaps[] = {1750, 701, 301, 132, 57, 23, 10, 4, 1};
+    uint32_t g0 = 1750;
+    while (g0 * 9 / 4 < n) g0 = g0 * 9 / 4;
+    for (uint32_t gap = g0; gap > 1750; gap = gap * 4 / 9) {
+        for (uint32_t i = gap; i < n; i++) {
+            uint32_t k = v[i];
+            uint32_t j = i;
+            while (j >= gap && v[j - gap] > k) { v[j] = v[j - gap]; j -= gap; }
+            v[j] = k;
+        }
+    }
+    for (int gi = 0; gi < 9; gi++) {
+        uint32_t gap = gaps[gi];
+        if (gap >= n) continue;
+        for (uint32_t i = gap; i < n; i++) {
+            uint32_t k = v[i];
+            uint32_t j = i;
+            while (j >= gap && v[j - gap] > k) { v[j] = v[j - gap]; j -= gap; }
+            v[j] = k;
+        }
+    }
+}
+
+// position of the k-th (0-based) set bit of v (k < popcount(v))
+__device__ __forceinline__ int select_bit(uint64_t v, int k) {
+    int pos = 0;
+    int c = __popc((uint32_t)v);
+    if (k >= c) { k -= c; v >>= 32; pos += 32; }
+    c = __popc((uint32_t)v & 0xFFFFu);
+    if (k >= c) { k -= c; v >>= 16; pos += 16; }
+    c = __popc((uint32_t)v & 0xFFu);
+    if (k >= c) { k -= c; v >>= 8; pos += 8; }
+    c = __popc((uint32_t)v & 0xFu);
+    if (k >= c) { k -= c; v >>= 4; pos += 4; }
+    c = __popc((uint32_t)v & 0x3u);
+    if (k >= c) { k -= c; v >>= 2; pos += 2; }
+    c = (int)(v & 1u);
+    if (k >= c) { pos += 1; }
+    return pos;
+}
+
+// ---------------------------------------------------------------------------
+// k_pair: RandomPair::findMates per cell (RandomPair.cpp:183-276).  The
+// smaller sex chooses in id order; each choice takes the k-th still
+// unpaired member of the other sex, k = (int)wrandr(0, left).
+// Also clears the look-back tiles and tile counter of k_act.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_pair(Ctx x, int32_t do_pair) {
+    const int64_t gtid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t gsize = (int64_t)gridDim.x * kBlock;
+    {
+        const int64_t hw = x.st->hw;
+        const int64_t nt = (hw + kActTile - 1) / kActTile + 1;
+        for (int64_t j = gtid; j < nt && j < x.ntiles_act_max; j += gsize) x.tiles_act[j] = 0;
+        if (gtid == 0) x.st->tile_ctr_act = 0;
+    }
+    if (!do_pair) return;
+    const uint32_t step = (uint32_t)x.st->step;
+    for (int64_t c = gtid; c < x.ncells; c += gsize) {
+        const uint32_t F = x.fillF[c], M = x.fillM[c];
+        if (F == 0 || M == 0) continue;
+        if (F + M > x.off[c + 1] - x.off[c]) { atomicOr(&x.st->error, kErrBucket); continue; }
+        uint32_t *fb = x.bucket + x.off[c];
+        uint32_t *mb = x.bucket + x.off[c + 1] - M;
+        sort_u32(fb, F);
+        sort_u32(mb, M);
+        const bool fem_choose = F <= M;
+        uint32_t *chooser = fem_choose ? fb : mb;
+        uint32_t *pool = fem_choose ? mb : fb;
+        const uint32_t nch = fem_choose ? F : M;
+        const uint32_t np = fem_choose ? M : F;
+        uint32_t left = np;
+        if (np <= 64) {
+            uint64_t avail = (np == 64) ? ~0ull : ((1ull << np) - 1ull);
+            for (uint32_t j = 0; j < nch && left > 0; j++) {
+                const uint32_t a = chooser[j];
+                const uint32_t u = draw(x.seed, step, QHG_STREAM_PAIR, (uint64_t)x.a.id[a], 0);
+                const int k = (int)wrandr(0, (double)(int32_t)left, u);
+                const int pos = select_bit(avail, k);
+                avail &= ~(1ull << pos);
+                const uint32_t b = pool[pos];
+                x.a.mate[a] = (int32_t)b;
+                x.a.mate[b] = (int32_t)a;
+                left--;
+            }
+        } else {
+            // large cell: mark taken partners with the top bit (slots < 2^31)
+            for (uint32_t j = 0; j < nch && left > 0; j++) {
+                const uint32_t a = chooser[j];
+                const uint32_t u = draw(x.seed, step, QHG_STREAM_PAIR, (uint64_t)x.a.id[a], 0);
+                const int k = (int)wrandr(0, (double)(int32_t)left, u);
+                int found = -1;
+                uint32_t m = 0;
+                for (m = 0; m < np; m++) {
+                    if (!(pool[m] & 0x80000000u)) { found++; if (found == k) break; }
+                }
+                const uint32_t b = pool[m];
+                pool[m] = b | 0x80000000u;
+                x.a.mate[a] = (int32_t)b;
+                x.a.mate[b] = (int32_t)a;
+                left--;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_act: fused doActions.  Each agent runs the step's actions in priority
+// order; a dead agent stops (SPopulation.cpp:566-571).  Moves are applied at
+// the end of the agent's program (performMoves, SPopulation.cpp:1058-1092):
+// actions of this step see the pre-move cell, births are registered in it.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_act(Ctx x, Prog p) {
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_wave[kBlock / 64];
+    __shared__ uint64_t s_excl;
+    const int64_t hw = x.st->hw;
+    const uint32_t step = (uint32_t)x.st->step;
+    const double t = x.st->t;
+    const uint32_t *cnt_prev = x.cnt[step & 1];
+    uint32_t *cnt_next = x.cnt[(step + 1) & 1];
+    const int32_t R = x.conn + 1;
+    uint32_t my_deaths = 0, my_moves = 0;
+
+    while (true) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(&x.st->tile_ctr_act, 1u);
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        const int64_t base = (int64_t)tile * kActTile;
+        if (base >= hw) break;
+        const int64_t i = base + threadIdx.x;
+        uint32_t bflag = 0;
+        int32_t bcell = 0, bfather = -1;
+        if (i < hw) {
+            uint8_t life = x.a.life[i];
+            if (life != 0) {
+                const uint8_t life0 = life;
+                const int32_t c = x.a.cell[i];
+                const uint8_t g = x.a.gender[i];
+                const int64_t id = x.a.id[i];
+                double age = p.load_age ? x.a.age[i] : 0.0;
+                bool age_dirty = false;
+                int32_t dest = -1;
+                for (int q = 0; q < p.nops; q++) {
+                    const int op = p.ops[q];
+                    if (op == QHG_ACT_GETOLD) {
+                        // GetOld::execute, QHG4/actions/GetOld.cpp:37-48
+                        age = t - x.a.birth[i];
+                        age_dirty = true;
+                    } else if (op == QHG_ACT_FERTILITY) {
+                        // Fertility::execute, QHG4/actions/Fertility.cpp:49-74
+                        if (g == 0) {
+                            const double lb = x.a.lastb[i];
+                            life = (age > p.fert_min && age < p.fert_max && (t - lb) > p.fert_inter)
+                                       ? QHG_LIFE_FERTILE : QHG_LIFE_ALIVE;
+                        } else {
+                            life = (age > p.fert_min) ? QHG_LIFE_FERTILE : QHG_LIFE_ALIVE;
+                        }
+                    } else if (op == QHG_ACT_VERHULST || op == QHG_ACT_VERHULSTVARK) {
+                        // LinearBirth::initialize/execute + LinearDeath (Verhulst.cpp:101-115)
+                        const double N = (double)cnt_prev[c];
+                        double b, d;
+                        if (op == QHG_ACT_VERHULST) {
+                            b = p.b0 + (p.theta - p.b0) * (N / p.Kconst);
+                            d = p.d0 + (p.theta - p.d0) * (N / p.Kconst);
+                        } else {
+                            const double K = x.K[c];
+                            b = (K <= 0) ? 0.0 : p.b0 + (p.theta - p.b0) * (N / K);
+                            d = (K <= 0) ? 1.0 : p.d0 + (p.theta - p.d0) * (N / K);
+                        }
+                        if (b > 0) {
+                            if (g == 0) {
+                                const int32_t m = x.a.mate[i];
+                                if (m >= 0) {
+                                    const double r = wrandd(draw(x.seed, step, QHG_STREAM_LBIRTH, (uint64_t)id, 0));
+                                    if (r < b) { bflag = 1; bcell = c; bfather = m; }
+                                }
+                            }
+                        } else if (b < 0) {
+                            const double r = wrandd(draw(x.seed, step, QHG_STREAM_LBIRTH, (uint64_t)id, 0));
+                            if (r < -b) life = 0;
+                        }
+                        if (life != 0) {
+                            const double r = wrandd(draw(x.seed, step, QHG_STREAM_LDEATH, (uint64_t)id, 0));
+                            if (r < d) life = 0;
+                        }
+                        if (life == 0) { my_deaths++; break; }
+                    } else if (op == QHG_ACT_RANDOMMOVE) {
+                        // RandomMove::execute, QHG4/actions/RandomMove.cpp:65-100
+                        uint32_t u[4];
+                        draw4(x.seed, step, QHG_STREAM_RMOVE, (uint64_t)id, 0, u);
+                        if (wrandd(u[0]) < p.rmove_p) {
+                            const int nn = x.nn[c];
+                            const int slot = (int)(wrandd(u[1]) * (nn + 1));
+                            if (slot > 0) {
+                                const int32_t nb = x.nbr[(int64_t)c * x.conn + slot - 1];
+                                if (nb >= 0) { if (dest < 0) my_moves++; dest = nb; }
+                            }
+                        }
+                    } else if (op == QHG_ACT_WEIGHTEDMOVE) {
+                        // WeightedMove::execute, QHG4/actions/WeightedMove.cpp:45-106
+                        uint32_t u[4];
+                        draw4(x.seed, step, QHG_STREAM_WMOVE, (uint64_t)id, 0, u);
+                        if (wrandd(u[0]) < p.wmove_p) {
+                            const int nn = x.nn[c];
+                            const double *w = x.W + (int64_t)c * R;
+                            const double wl = w[nn];
+                            int idx = -1;
+                            if (w[0] == wl) {
+                                idx = (int)wrandi(0, (uint32_t)nn + 1, 1, u[1]);
+                            } else {
+                                const double r2 = wrandd(u[1]) * wl;
+                                for (int q2 = 0; q2 < nn + 1; q2++) {
+                                    if (r2 < w[q2]) { idx = q2; break; }
+                                }
+                            }
+                            if (idx > 0) {
+                                const int32_t nb = x.nbr[(int64_t)c * x.conn + idx - 1];
+                                if (nb >= 0 && (x.ice == nullptr || !x.ice[nb])) {
+                                    if (dest < 0) my_moves++;
+                                    dest = nb;
+                                }
+                            }
+                        }
+                    }
+                    // RandomPair, MultiEvaluator, NPPCapacity, Genetics: no execute
+                }
+                if (life != life0) x.a.life[i] = life;
+                if (age_dirty) x.a.age[i] = age;
+                if (bflag) x.a.nbabies[i] += 1;
+                int32_t cfin = c;
+                if (dest >= 0) { x.a.cell[i] = dest; cfin = dest; }
+                if (life != 0) atomicAdd(&cnt_next[cfin], 1u);
+            }
+        }
+        // births of this tile, in slot (= mother id) order
+        uint32_t tile_total;
+        const uint32_t rank = block_excl_scan(bflag, s_wave, tile_total);
+        if (threadIdx.x == 0) {
+            uint64_t ex = lookback(x.tiles_act, tile, tile_total);
+            s_excl = ex;
+            if (base + kActTile >= hw) x.st->births_cur = (int64_t)(ex + tile_total);
+        }
+        __syncthreads();
+        if (bflag) {
+            const int64_t r = (int64_t)s_excl + rank;
+            x.brec[3 * r + 0] = (int32_t)i;
+            x.brec[3 * r + 1] = bfather;
+            x.brec[3 * r + 2] = bcell;
+        }
+        __syncthreads();
+    }
+    // one atomic per block for the step counters
+    __shared__ uint32_t s_red[2];
+    if (threadIdx.x == 0) { s_red[0] = 0; s_red[1] = 0; }
+    __syncthreads();
+    if (my_deaths) atomicAdd(&s_red[0], my_deaths);
+    if (my_moves) atomicAdd(&s_red[1], my_moves);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_red[0]) atomicAdd((unsigned long long *)&x.st->deaths_cur, (unsigned long long)s_red[0]);
+        if (s_red[1]) atomicAdd((unsigned long long *)&x.st->moves_cur, (unsigned long long)s_red[1]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// genetics helpers: GeneUtils::makeMultiMask / crossOver (GeneUtils.cpp:156-246)
+// ---------------------------------------------------------------------------
+__device__ uint64_t multimask_block(const uint32_t *pos, int n, uint32_t blk) {
+    // breaks of this block, in draw order, as makeMultiMask sees them
+    int cnt = 0;
+    for (int q = 0; q < n; q++) cnt += (pos[q] >> 6) == blk;
+    uint32_t k = 1 - (uint32_t)(cnt % 2);
+    uint64_t out = 0;
+    // walk the block's breaks from the back of the vector
+    int i = n - 1;
+    while (i >= 0 && (pos[i] >> 6) != blk) i--;
+    for (uint32_t j = 0; j < 64; j++) {
+        if (i >= 0 && j == 64 - (pos[i] & 63u)) {
+            i--;
+            while (i >= 0 && (pos[i] >> 6) != blk) i--;
+            k = 1 - k;
+        }
+        out = (out << 1) + k;
+    }
+    return out;
+}
+
+// one strand (sel) of crossOver(parent) -> out[0..nb)
+__device__ void crossover_strand(uint64_t *out, const uint64_t *par, int nb, const uint32_t *pos,
+                                 int n, int sel) {
+    for (int b = 0; b < nb; b++) {
+        int before = 0, in = 0;
+        for (int q = 0; q < n; q++) {
+            const uint32_t blk = pos[q] >> 6;
+            before += (int)blk < b;
+            in += (int)blk == b;
+        }
+        const int cur = before & 1;
+        const uint64_t gc = par[cur * nb + b], go = par[(1 - cur) * nb + b];
+        const uint64_t L = in ? multimask_block(pos, n, (uint32_t)b) : ~0ull;
+        out[b] = (sel == 0) ? ((L & gc) | (~L & go)) : ((L & go) | (~L & gc));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_newborn: one thread per birth record (rank r): slot hw + r, id
+// max_id + 1 + r.  Also clears the look-back tiles / counter of k_cellscan.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_newborn(Ctx x, Prog p) {
+    const int64_t gtid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t gsize = (int64_t)gridDim.x * kBlock;
+    {
+        const int64_t nt = (x.ncells + kCellTile - 1) / kCellTile + 1;
+        for (int64_t j = gtid; j < nt; j += gsize) x.tiles_cell[j] = 0;
+        if (gtid == 0) x.st->tile_ctr_cell = 0;
+    }
+    const int64_t nb = x.st->births_cur;
+    const int64_t hw = x.st->hw;
+    const int64_t max_id = x.st->max_id;
+    const uint32_t step = (uint32_t)x.st->step;
+    const double t = x.st->t;
+    uint32_t *cnt_next = x.cnt[(step + 1) & 1];
+    if (hw + nb > x.cap) {
+        // no slot for the births: drop them and flag the step as failed
+        if (gtid == 0) {
+            atomicOr(&x.st->error, kErrCapacity);
+            x.st->births_cur = 0;
+        }
+        return;
+    }
+    for (int64_t r = gtid; r < nb; r += gsize) {
+        const int32_t mother = x.brec[3 * r + 0];
+        const int32_t father = x.brec[3 * r + 1];
+        const int32_t c = x.brec[3 * r + 2];
+        const int64_t s = hw + r;
+        const int64_t nid = max_id + 1 + r;
+        x.a.id[s] = nid;
+        x.a.cell[s] = c;
+        x.a.birth[s] = t;
+        x.a.age[s] = 0.0;
+        x.a.lastb[s] = 0.0;
+        x.a.mate[s] = -3;
+        x.a.nbabies[s] = 0;
+        const uint32_t u = draw(x.seed, step, QHG_STREAM_NEWBORN, (uint64_t)nid, 0);
+        const uint8_t g = (uint8_t)(2 * wrandd(u));
+        x.a.gender[s] = g;
+        x.a.life[s] = (g == 0) ? QHG_LIFE_FERTILE : QHG_LIFE_ALIVE;
+        atomicAdd(&cnt_next[c], 1u);
+        if (x.gwords) {
+            // Genetics::makeOffspring, QHG4/actions/Genetics.cpp:285-337
+            const int nbk = x.gwords / 2;
+            const uint64_t *g1 = x.a.genome + (int64_t)mother * x.gwords;
+            const uint64_t *g2 = x.a.genome + (int64_t)father * x.gwords;
+            uint64_t *out = x.a.genome + s * x.gwords;
+            uint32_t n = 0;
+#define GD() draw(x.seed, step, QHG_STREAM_GENETICS, (uint64_t)nid, n++)
+            const int i1 = (int)(2 * 1.0 * wrandd(GD()));
+            const int i2 = (int)(2 * 1.0 * wrandd(GD()));
+            const int C = p.gen_cross;
+            if (C > 0) {
+                uint32_t pos[kMaxCross];
+                const uint32_t nbits = (uint32_t)nbk * 64u;
+                for (int q = 0; q < C; q++) pos[q] = wrandi(0, nbits, 2, GD());
+                crossover_strand(out, g1, nbk, pos, C, i1);
+                for (int q = 0; q < C; q++) pos[q] = wrandi(0, nbits, 2, GD());
+                crossover_strand(out + nbk, g2, nbk, pos, C, i2);
+            } else if (C == -1) {
+                // freeReco + makeFreeMask, GeneUtils.cpp:322-360
+                for (int par = 0; par < 2; par++) {
+                    const uint64_t *gp = par ? g2 : g1;
+                    const int sel = par ? i2 : i1;
+                    for (int b = 0; b < nbk; b++) {
+                        const uint32_t hi = GD();
+                        const uint32_t lo = GD();
+                        uint64_t v = ((uint64_t)hi << 32) + lo;
+                        v &= 0x5555555555555555ull;
+                        v += v << 1;
+                        const uint64_t a0 = gp[b], a1 = gp[nbk + b];
+                        out[par * nbk + b] = (sel == 0) ? ((v & a0) | (~v & a1)) : ((v & a1) | (~v & a0));
+                    }
+                }
+            } else {
+                for (int b = 0; b < nbk; b++) out[b] = g1[i1 * nbk + b];
+                for (int b = 0; b < nbk; b++) out[nbk + b] = g2[i2 * nbk + b];
+            }
+            if (p.nbino > 0) {
+                // BinomialDist::getN then GeneUtils::mutateNucs (GeneUtils.cpp:112-141)
+                const double um = wrandd(GD());
+                int k = 0;
+                while (k < p.nbino && um > p.bino[k]) k++;
+                const uint32_t nbits_all = 2u * 2u * (uint32_t)p.gen_size;
+                for (int j = 0; j < k; j++) {
+                    const uint32_t bp = wrandi(0, nbits_all, 2, GD());
+                    const uint64_t mk = (uint64_t)wrandi(1, 4, 1, GD());
+                    out[bp >> 6] ^= mk << (bp & 63u);
+                }
+            }
+#undef GD
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_cellscan: exclusive scan of the new per-cell counts -> off[] for the
+// next pairing; clears the fill counters and the count buffer that the next
+// step accumulates into.  The last tile closes the step's bookkeeping.
+// mode 0: end of step (src = cnt[(step+1)&1]); mode 1: pre-loop (src =
+// cnt[step&1], no bookkeeping).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_cellscan(Ctx x, int32_t mode) {
+    __shared__ uint32_t s_tile, s_step;
+    __shared__ uint32_t s_wave[kBlock / 64];
+    __shared__ uint64_t s_excl;
+    const int64_t nc = x.ncells;
+    constexpr int per = kCellTile / kBlock;
+    bool first = true;
+    while (true) {
+        // The block holding the last tile advances st->step.  Thread 0 reads
+        // the step before its acq_rel tile grab, so every block that holds a
+        // tile saw the step of this launch.
+        if (threadIdx.x == 0) {
+            if (first) s_step = (uint32_t)__hip_atomic_load(&x.st->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_tile = __hip_atomic_fetch_add(&x.st->tile_ctr_cell, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        first = false;
+        const uint32_t step = s_step;
+        const uint32_t *src = mode ? x.cnt[step & 1] : x.cnt[(step + 1) & 1];
+        uint32_t *clr = mode ? x.cnt[(step + 1) & 1] : x.cnt[step & 1];
+        const uint32_t tile = s_tile;
+        const int64_t base = (int64_t)tile * kCellTile;
+        if (base >= nc) break;
+        uint32_t v[per];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int q = 0; q < per; q++) {
+            const int64_t c = base + (int64_t)threadIdx.x * per + q;
+            v[q] = (c < nc) ? src[c] : 0u;
+            sum += v[q];
+        }
+        uint32_t tot;
+        const uint32_t pre = block_excl_scan(sum, s_wave, tot);
+        if (threadIdx.x == 0) s_excl = lookback(x.tiles_cell, tile, tot);
+        __syncthreads();
+        uint64_t run = s_excl + pre;
+#pragma unroll
+        for (int q = 0; q < per; q++) {
+            const int64_t c = base + (int64_t)threadIdx.x * per + q;
+            if (c < nc) {
+                x.off[c] = (uint32_t)run;
+                x.fillF[c] = 0;
+                x.fillM[c] = 0;
+                clr[c] = 0;
+            }
+            run += v[q];
+        }
+        if (base + kCellTile >= nc && threadIdx.x == kBlock - 1) {
+            x.off[nc] = (uint32_t)run;
+            DevStatus *st = x.st;
+            st->n_alive = (int64_t)run;
+            if (mode == 0) {
+                const int64_t b = st->births_cur;
+                st->hw += b;
+                st->max_id += b;
+                st->births_last = b;
+                st->deaths_last = st->deaths_cur;
+                st->moves_last = st->moves_cur;
+                st->births_total += b;
+                st->deaths_total += st->deaths_cur;
+                st->moves_total += st->moves_cur;
+                st->births_cur = 0;
+                st->deaths_cur = 0;
+                st->moves_cur = 0;
+                st->step += 1;
+                st->t += 1.0;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_set_time(DevStatus *st, double t) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) st->t = t;
+}
+
+// live-agent histogram (pre-loop / after upload or compaction)
+__global__ void __launch_bounds__(kBlock) k_hist(Ctx x) {
+    const int64_t hw = x.st->hw;
+    uint32_t *cnt = x.cnt[x.st->step & 1];
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < hw;
+         i += (int64_t)gridDim.x * kBlock) {
+        if (x.a.life[i] != 0) atomicAdd(&cnt[x.a.cell[i]], 1u);
+    }
+}
+
+}  // namespace qhg

@@ -1,0 +1,125 @@
+"""GpuPopulation: Python handle over the C ABI, with the method names of the
+reference's population interface (PopBase / SPopulation,
+QHG4/core/PopBase.h:16-121, QHG4/core/SPopulation.h:86-150).
+
+Every step runs on the GPU through libqhg4_amd.so; there is no host fallback.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from ._lib import lib, check
+from .scenarios import grid_desc, env_desc
+
+
+class GpuPopulation:
+    def __init__(self, grid, env, params, agents=None, capacity=0):
+        self.grid = grid
+        self.env = dict(env or {})
+        self.params = params
+        self._gdesc = grid_desc(grid)
+        self._edesc = env_desc(self.env)
+        h = C.c_void_p()
+        n0 = 0 if agents is None else len(agents["id"])
+        cap = int(capacity) if capacity else max(4 * n0, 1 << 20)
+        check(lib().qhg_gpu_create(C.byref(self._gdesc), C.byref(self._edesc), C.byref(params), cap,
+                                   C.byref(h)), "qhg_gpu_create")
+        self.h = h
+        self.gwords = 0
+        for i in range(params.num_actions):
+            if params.action_kind[i] == abi.ACT_GENETICS:
+                self.gwords = 2 * abi.genome_words(params.gen_genome_size)
+        if agents is not None:
+            self.add_agents(agents)
+
+    @classmethod
+    def from_scenario(cls, sc, capacity=0):
+        return cls(sc["grid"], sc["env"], sc["params"], sc["agents"], capacity)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().qhg_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # -- agents ----------------------------------------------------------
+    def add_agents(self, agents):
+        n = len(agents["id"])
+        s = abi.agents_struct(agents, n)
+        check(lib().qhg_gpu_upload_agents(self.h, C.byref(s)), "qhg_gpu_upload_agents")
+
+    def getNumAgentsEffective(self):
+        n = C.c_int64()
+        check(lib().qhg_gpu_num_alive(self.h, C.byref(n)), "qhg_gpu_num_alive")
+        return n.value
+
+    def agents(self):
+        n = self.getNumAgentsEffective()
+        out = abi.alloc_agents(n, self.gwords)
+        s = abi.agents_struct(out, n)
+        check(lib().qhg_gpu_download_agents(self.h, C.byref(s)), "qhg_gpu_download_agents")
+        if self.gwords:
+            out["genome"] = out["genome"].reshape(n, self.gwords)
+        return out
+
+    # -- step phases (PopLooper::doStep order) -----------------------------
+    def preLoop(self, t=0.0):
+        check(lib().qhg_gpu_pre_loop(self.h, float(t)), "qhg_gpu_pre_loop")
+
+    def initializeStep(self, t):
+        check(lib().qhg_gpu_initialize_step(self.h, float(t)), "qhg_gpu_initialize_step")
+
+    def doActions(self, prio, t):
+        check(lib().qhg_gpu_do_actions(self.h, int(prio), float(t)), "qhg_gpu_do_actions")
+
+    def finalizeStep(self):
+        check(lib().qhg_gpu_finalize_step(self.h), "qhg_gpu_finalize_step")
+
+    def run(self, nsteps):
+        check(lib().qhg_gpu_run(self.h, int(nsteps)), "qhg_gpu_run")
+
+    def sync(self):
+        check(lib().qhg_gpu_sync(self.h), "qhg_gpu_sync")
+
+    def compactData(self):
+        check(lib().qhg_gpu_compact(self.h), "qhg_gpu_compact")
+
+    def set_env(self, env):
+        self.env.update(env)
+        self._edesc = env_desc(self.env)
+        check(lib().qhg_gpu_set_env(self.h, C.byref(self._edesc)), "qhg_gpu_set_env")
+
+    # -- observables -------------------------------------------------------
+    def stats(self):
+        s = abi.Stats()
+        check(lib().qhg_gpu_stats(self.h, C.byref(s)), "qhg_gpu_stats")
+        return s.as_dict()
+
+    def getNumAgentsArray(self):
+        out = np.zeros(self.grid["num_cells"], dtype=np.uint32)
+        check(lib().qhg_gpu_cell_counts(self.h, abi.ptr(out, C.c_uint32)), "qhg_gpu_cell_counts")
+        return out
+
+    def capacities(self):
+        out = np.zeros(self.grid["num_cells"], dtype=np.float64)
+        check(lib().qhg_gpu_capacities(self.h, abi.ptr(out, C.c_double)), "qhg_gpu_capacities")
+        return out
+
+    def env_weights(self):
+        out = np.zeros(self.grid["num_cells"] * (self.grid["connectivity"] + 1), dtype=np.float64)
+        check(lib().qhg_gpu_env_weights(self.h, abi.ptr(out, C.c_double)), "qhg_gpu_env_weights")
+        return out
+
+    def set_timing(self, on):
+        check(lib().qhg_gpu_set_timing(self.h, 1 if on else 0), "qhg_gpu_set_timing")
+
+    def kernel_times(self):
+        n = 16
+        avg = (C.c_double * n)()
+        cnt = (C.c_int64 * n)()
+        got = C.c_int32()
+        check(lib().qhg_gpu_kernel_times(self.h, avg, cnt, n, C.byref(got)), "qhg_gpu_kernel_times")
+        return {lib().qhg_gpu_kernel_name(i).decode(): (avg[i], cnt[i]) for i in range(got.value)}

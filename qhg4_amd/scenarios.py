@@ -1,0 +1,198 @@
+"""Scenario builders: grid + environment + parameters + initial agents.
+
+C1  plain model on a 128x128 non-periodic lattice (tut_SexualPop action set,
+    QHG4/populations/tut_SexualPop.cpp:24-43, priorities of
+    QHG4/tutorial_data/xmldat/tut_Sexual.xml without ATanDeath).
+C2  OoA-style model on a 40,962-cell icosahedral grid (OoAPop action set,
+    QHG4/populations/OoAPop.cpp:32-88, priorities of
+    QHG4/tutorial_data/xmldat/tut_EnvironCapAlt.xml, MultiEvaluator in
+    multiplicative mode, no OldAgeDeath).
+C3  C2 plus 2-bit diploid genetics (OoANavGen2bitPop,
+    QHG4/populations/OoANavGen2bitPop.cpp:75-166, 226-241).
+
+All inputs are synthetic and seeded; the same dict feeds the MI355X path and
+the oracle.
+"""
+import numpy as np
+
+from . import abi
+from .grids import rect_grid, ico_grid, value_noise, bfs_patch
+
+ALT_SIGMOID_THRESHOLD = 2500.0
+ALT_SIGMOID_WIDTH = 400.0
+
+
+def make_params(seed, actions, **kw):
+    """actions: list of (kind, prio[, name]); kw: qhg_params fields."""
+    p = abi.Params()
+    p.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    p.num_actions = len(actions)
+    for i, a in enumerate(actions):
+        kind, prio = a[0], a[1]
+        name = a[2] if len(a) > 2 else abi.ACTION_NAMES[kind]
+        p.action_kind[i] = kind
+        p.action_prio[i] = prio
+        p.action_name[i].value = name.encode()
+    p.npp_efficiency = 1.0
+    for k, v in kw.items():
+        if k in ("me_input", "me_cumulate", "me_weight", "me_pl_n"):
+            arr = getattr(p, k)
+            for i, x in enumerate(v):
+                arr[i] = x
+        elif k == "me_pl":
+            for e, pts in enumerate(v):
+                p.me_pl_n[e] = len(pts)
+                for j, (x, y) in enumerate(pts):
+                    p.me_pl_x[e][j] = x
+                    p.me_pl_v[e][j] = y
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def sigmoid_polyline(lo=-4000.0, hi=6000.0, npts=41):
+    """Altitude preference 1/(1+exp((a-2500)/400)) sampled as PolyLine points."""
+    xs = np.linspace(lo, hi, npts)
+    return [(float(x), float(1.0 / (1.0 + np.exp((x - ALT_SIGMOID_THRESHOLD) / ALT_SIGMOID_WIDTH))))
+            for x in xs]
+
+
+def _agents(rng, cells, n, life, last_birth, first_id=1):
+    age = rng.uniform(0.0, 40.0, n)
+    return {
+        "id": np.arange(first_id, first_id + n, dtype=np.int64),
+        "cell": cells.astype(np.int32),
+        "life": np.full(n, life, dtype=np.uint8),
+        "gender": (rng.random(n) < 0.5).astype(np.uint8),
+        "birth_time": -age,
+        "age": age,
+        "last_birth": np.full(n, last_birth, dtype=np.float64),
+        "mate_id": np.full(n, -3, dtype=np.int64),
+        "num_babies": np.zeros(n, dtype=np.int32),
+    }
+
+
+def c1(seed=42, width=128, height=128, n_agents=10000):
+    """Plain model, small lattice.  No Fertility limits are named for C1, so
+    Fertility runs with min age 0 and no upper bound: every agent older than
+    0 can pair, which is the 'uniform random female-male pairing'."""
+    grid = rect_grid(width, height, periodic=False)
+    rng = np.random.default_rng(seed)
+    cells = rng.integers(0, grid["num_cells"], n_agents)
+    agents = _agents(rng, cells, n_agents, abi.QHG_LIFE_FERTILE, 0.0)
+    params = make_params(
+        seed,
+        [(abi.ACT_FERTILITY, 2), (abi.ACT_RANDOMPAIR, 3), (abi.ACT_VERHULST, 6),
+         (abi.ACT_RANDOMMOVE, 7), (abi.ACT_GETOLD, 8)],
+        fert_min_age=0.0, fert_max_age=1e30, fert_interbirth=-1.0,
+        verh_b0=0.1, verh_d0=0.05, verh_theta=0.01, verh_K=30.0,
+        rmove_prob=0.1)
+    return {"name": "C1", "grid": grid, "env": {}, "params": params, "agents": agents,
+            "steps": 1000}
+
+
+def ico_env(grid, seed):
+    xyz = grid["xyz"]
+    alt = -4000.0 + 10000.0 * value_noise(xyz, seed * 2 + 1)
+    npp = np.clip(value_noise(xyz, seed * 2 + 2) * 1.2, 0.0, 1.5)
+    return {"altitude": alt, "base_npp": npp, "longitude": grid["lon"], "latitude": grid["lat"]}
+
+
+def ooa_params(seed, genetics=False, wmove_prob=0.2, b0=0.8, d0=0.001, theta=0.1, k_scale=50.0):
+    """VerhulstVarK values are those of QHG4/tutorial_data/xmldat/tut_EnvironCapAlt.xml
+    (C2 names none; with C1's b0=0.1, d0=0.05 the paired-female birth rate
+    cannot outpace deaths and every run dies out)."""
+    actions = [(abi.ACT_NPPCAP, 1), (abi.ACT_GETOLD, 2), (abi.ACT_WEIGHTEDMOVE, 4),
+               (abi.ACT_MULTIEVAL, 5, "MultiEvaluator[Alt+NPP]"), (abi.ACT_FERTILITY, 6),
+               (abi.ACT_RANDOMPAIR, 7), (abi.ACT_VERHULSTVARK, 8)]
+    kw = {}
+    if genetics:
+        actions.append((abi.ACT_GENETICS, 9))
+        kw = dict(gen_genome_size=256, gen_num_crossover=1, gen_mutation_rate=1e-3)
+    return make_params(
+        seed, actions,
+        fert_min_age=15.0, fert_max_age=45.0, fert_interbirth=3.0,
+        verh_b0=b0, verh_d0=d0, verh_theta=theta,
+        wmove_prob=wmove_prob,
+        me_mode=abi.ME_MUL_SIMPLE, me_num=2,
+        me_input=[abi.EV_ALTITUDE, abi.EV_CAPACITY], me_cumulate=[0, 0], me_weight=[1.0, 1.0],
+        me_pl=[sigmoid_polyline(), []],
+        npp_min=0.0, npp_max=1.5, npp_k_min=0.0, npp_k_max=1.5 * k_scale, npp_efficiency=1.0,
+        **kw)
+
+
+def c2(seed=42, level=6, n_agents=100000, patch_cells=4000, genetics=False):
+    """100k agents in a BFS patch of land cells.  The patch is 4,000 cells
+    (~25 agents per cell, about half of K): a 200-cell patch puts 500 agents
+    in cells with K <= 75, where b = b0 + (theta-b0)*N/K < -1 and the
+    Verhulst rule kills every agent in step 0."""
+    grid = ico_grid(level)
+    env = ico_env(grid, seed)
+    alt, npp = env["altitude"], env["base_npp"]
+    land = alt > 0
+    good = land & (alt < 1500)
+    start = int(np.argmax(np.where(good, npp, -1.0)))
+    patch = bfs_patch(grid, start, patch_cells, land)
+    rng = np.random.default_rng(seed + 7)
+    cells = patch[rng.integers(0, len(patch), n_agents)]
+    agents = _agents(rng, cells, n_agents, abi.QHG_LIFE_ALIVE, -1.0)
+    params = ooa_params(seed, genetics=genetics)
+    out = {"name": "C2", "grid": grid, "env": env, "params": params, "agents": agents,
+           "steps": 2000}
+    if genetics:
+        out["agents"]["genome"] = initial_genomes(rng, n_agents, 256)
+    return out
+
+
+def initial_genomes(rng, n, genome_size, muts=8):
+    """A shared random base genome with ``muts`` random nucleotide changes
+    per agent (stand-in for GenomeCreator; input data only)."""
+    w = abi.genome_words(genome_size)
+    base = rng.integers(0, 2 ** 63, size=2 * w, dtype=np.int64).astype(np.uint64)
+    g = np.tile(base, (n, 1))
+    pos = rng.integers(0, 2 * genome_size, size=(n, muts))
+    val = rng.integers(1, 4, size=(n, muts)).astype(np.uint64)
+    rows = np.repeat(np.arange(n), muts)
+    word = (pos // 32).ravel()
+    shift = (2 * (pos % 32)).ravel().astype(np.uint64)
+    np.bitwise_xor.at(g, (rows, word), val.ravel() << shift)
+    return np.ascontiguousarray(g.ravel())
+
+
+def c3(seed=42, level=8, n_agents=2_000_000):
+    """Scale run with genetics: agents spread uniformly over land cells below
+    1500 m."""
+    grid = ico_grid(level)
+    env = ico_env(grid, seed)
+    alt = env["altitude"]
+    hab = np.flatnonzero((alt > 0) & (alt < 1500) & (env["base_npp"] > 0.05)).astype(np.int32)
+    rng = np.random.default_rng(seed + 11)
+    cells = hab[rng.integers(0, len(hab), n_agents)]
+    agents = _agents(rng, cells, n_agents, abi.QHG_LIFE_ALIVE, -1.0)
+    agents["genome"] = initial_genomes(rng, n_agents, 256)
+    params = ooa_params(seed, genetics=True)
+    return {"name": "C3", "grid": grid, "env": env, "params": params, "agents": agents,
+            "steps": 5000}
+
+
+def grid_desc(grid):
+    """GridDesc struct pointing into the grid dict's arrays (keep the dict alive)."""
+    g = abi.GridDesc()
+    g.num_cells = int(grid["num_cells"])
+    g.connectivity = int(grid["connectivity"])
+    g.neighbors = abi.ptr(grid["neighbors"], abi.C.c_int32)
+    g.num_neighbors = abi.ptr(grid["num_neighbors"], abi.C.c_uint8)
+    g.cell_ids = abi.ptr(grid.get("cell_ids"), abi.C.c_int64)
+    return g
+
+
+def env_desc(env):
+    e = abi.EnvDesc()
+    for k in ("altitude", "longitude", "latitude", "water", "base_npp", "temperature", "rainfall"):
+        arr = env.get(k)
+        if arr is not None:
+            arr = env[k] = np.ascontiguousarray(arr, dtype=np.float64)
+        setattr(e, k, abi.ptr(arr, abi.C.c_double))
+    for k in ("ice", "coastal"):
+        setattr(e, k, abi.ptr(env.get(k), abi.C.c_uint8))
+    return e

@@ -1,0 +1,91 @@
+"""MI355X path vs the oracle, bit-exact, on the same seeded inputs."""
+import numpy as np
+import pytest
+
+from qhg4_amd import abi, scenarios
+from qhg4_amd.population import GpuPopulation
+from oracle.oracle import Oracle
+from tests.helpers import assert_same_state, tut_verhulst
+
+pytestmark = pytest.mark.gpu
+
+
+def _lockstep(sc, steps, check_every, use_run=False, capacity=0, compact_at=()):
+    gpu = GpuPopulation.from_scenario(sc, capacity=capacity)
+    ora = Oracle(sc)
+    gpu.preLoop()
+    ora.pre_loop()
+    assert_same_state(gpu.agents(), ora.agents(), "after preLoop")
+    done = 0
+    while done < steps:
+        n = min(check_every, steps - done)
+        if use_run:
+            gpu.run(n)
+        else:
+            for s in range(n):
+                t = float(done + s)
+                gpu.initializeStep(t)
+                gpu.doActions(0, t)
+                gpu.finalizeStep()
+        ora.run(n)
+        done += n
+        if done in compact_at:
+            gpu.compactData()
+        where = "%s step %d" % (sc["name"], done)
+        assert_same_state(gpu.agents(), ora.agents(), where)
+        assert np.array_equal(gpu.getNumAgentsArray(), ora.cell_counts()), where + ": counts"
+        gs, os_ = gpu.stats(), ora.stats()
+        for k in ("num_alive", "max_id", "births_last", "deaths_last", "moves_last",
+                  "births_total", "deaths_total", "moves_total"):
+            assert gs[k] == os_[k], "%s: stat %s %d vs %d" % (where, k, gs[k], os_[k])
+        assert gs["device_error"] == 0
+    return gpu, ora
+
+
+def test_c1_as_specified():
+    print("case: C1 128x128, 10k agents, 150 steps", flush=True)
+    _lockstep(scenarios.c1(), 150, 25)
+
+
+def test_c1_tutorial_rates():
+    print("case: C1 lattice with tut_Sexual Verhulst values, 200 steps", flush=True)
+    sc = tut_verhulst(scenarios.c1(seed=3))
+    gpu, ora = _lockstep(sc, 200, 40, use_run=True)
+    assert ora.stats()["num_alive"] > 10000
+
+
+def test_c2_small():
+    print("case: C2 model, ico level 5, 20k agents, 120 steps", flush=True)
+    sc = scenarios.c2(seed=5, level=5, n_agents=20000, patch_cells=800)
+    gpu = GpuPopulation.from_scenario(sc)
+    ora = Oracle(sc)
+    gpu.preLoop()
+    ora.pre_loop()
+    assert np.array_equal(gpu.capacities().view(np.uint64), ora.capacities().view(np.uint64))
+    gpu.run(1)
+    ora.run(1)
+    w_g, w_o = gpu.env_weights(), ora.env_weights()
+    assert np.array_equal(w_g.view(np.uint64), w_o.view(np.uint64)), "env weights differ"
+    for k in range(6):
+        gpu.run(20)
+        ora.run(20)
+        assert_same_state(gpu.agents(), ora.agents(), "C2 step %d" % (1 + 20 * (k + 1)))
+
+
+def test_c3_small_genetics():
+    print("case: C3 model (genetics), ico level 4, 4k agents, 80 steps", flush=True)
+    sc = scenarios.c2(seed=9, level=4, n_agents=4000, patch_cells=200, genetics=True)
+    sc["name"] = "C3-small"
+    _lockstep(sc, 80, 20, use_run=True)
+
+
+def test_compaction_keeps_state():
+    print("case: forced compaction, C1 tutorial rates", flush=True)
+    sc = tut_verhulst(scenarios.c1(seed=11, width=64, height=64, n_agents=3000))
+    _lockstep(sc, 90, 15, compact_at=(15, 45, 60))
+
+
+def test_growth_beyond_capacity():
+    print("case: capacity growth from a tiny initial capacity", flush=True)
+    sc = tut_verhulst(scenarios.c1(seed=13, width=64, height=64, n_agents=3000))
+    _lockstep(sc, 120, 30, use_run=True, capacity=1024)

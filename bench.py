@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X population step engine on the C2 workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (default): the OoA-style model on a 40,962-cell icosahedral grid,
+100,000 initial agents, seed 42; W untimed steps, then K timed steps
+(W + K = 2,000 = the full C2 run by default).  A step is one full population
+step (initialize, fused actions, births, per-cell recount) on the device.
+
+value = sum over the timed steps of the live count at step start / wall time
+of the timed steps (inputs resident in HBM, no host sync inside the region).
+
+Per-kernel device time comes from HIP events on the handle's stream in a
+second, bit-identical run of the same steps (the simulation is deterministic);
+roofline.achieved = algorithmic bytes per launch (qhg4_amd/csrc/bytes_model.h)
+/ average launch duration of the dominant kernel.
+
+cpu_baseline: the CPU oracle (the reference algorithm restated in C with the
+same Philox draws, OpenMP) on the box's host cores, timed on the same steps,
+bounded to about 20 s.
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "agent-steps/sec at 1 MI355X + %HBM peak; ensemble vs plain speedup, p50/p99"
+
+
+def bytes_model():
+    path = os.path.join(ROOT, "qhg4_amd", "csrc", "bytes_model.h")
+    out = {}
+    for ln in open(path):
+        m = re.match(r"#define\s+QHG_B_(\w+)\s+(\d+)", ln)
+        if m:
+            out[m.group(1)] = int(m.group(2))
+    return out
+
+
+def kernel_bytes(bm, d, ncells, conn, ops):
+    """Algorithmic bytes of each step kernel summed over the measured steps.
+    d: deltas of live agent-steps, slot-steps, births, moves, fertile."""
+    from qhg4_amd import abi
+    live, slots, births, moves, fert, steps = (d["live"], d["slots"], d["births"], d["moves"],
+                                              d["fertile"], d["steps"])
+    per_live = bm["ACT_LIVE_BASE"]
+    per_cell = bm["ACT_CELL_COUNT"]
+    if abi.ACT_GETOLD in ops:
+        per_live += bm["ACT_LIVE_GETOLD"]
+    if abi.ACT_FERTILITY in ops:
+        per_live += bm["ACT_LIVE_FERT"]
+    if abi.ACT_VERHULST in ops or abi.ACT_VERHULSTVARK in ops:
+        per_live += bm["ACT_LIVE_VERH"]
+        per_cell += bm["ACT_CELL_VERH"]
+    if abi.ACT_WEIGHTEDMOVE in ops:
+        per_cell += bm["ACT_CELL_WMOVE"]
+    if abi.ACT_RANDOMMOVE in ops:
+        per_cell += bm["ACT_CELL_RMOVE"]
+    act = (slots * bm["ACT_SLOT"] + live * per_live + steps * ncells * per_cell
+           + moves * bm["ACT_MOVER"] + births * bm["ACT_BIRTH"])
+    nb = births * bm["NEWBORN_BIRTH"]
+    if abi.ACT_GENETICS in ops:
+        nb += births * bm["NEWBORN_GENOME"]
+    out = {"k_act": act, "k_newborn": nb,
+           "k_cellscan": steps * ncells * bm["SCAN_CELL"]}
+    if abi.ACT_RANDOMPAIR in ops:
+        out["k_bucket"] = slots * bm["BUCKET_SLOT"] + live * bm["BUCKET_LIVE"] + fert * bm["BUCKET_FERTILE"]
+        out["k_pair"] = steps * ncells * bm["PAIR_CELL"] + fert * bm["PAIR_FERTILE"]
+    return out
+
+
+def make_workload(name, seed):
+    from qhg4_amd import scenarios
+    if name == "c2":
+        sc = scenarios.c2(seed=seed)
+        desc = "C2: OoA model (NPPCapacity, GetOld, WeightedMove, MultiEvaluator, Fertility, " \
+               "RandomPair, VerhulstVarK) on a 40,962-cell icosahedral grid, 100,000 initial agents"
+    elif name == "c1":
+        sc = scenarios.c1(seed=seed)
+        desc = "C1: plain model on a 128x128 lattice, 10,000 initial agents"
+    elif name == "c3":
+        sc = scenarios.c3(seed=seed)
+        desc = "C3: C2 model + 2x256-locus 2-bit genetics on a 655,362-cell icosahedral grid, 2M agents"
+    else:
+        raise SystemExit("unknown workload %s" % name)
+    return sc, desc
+
+
+def delta(s1, s0, steps):
+    return {"live": s1["agent_steps"] - s0["agent_steps"], "slots": s1["slot_steps"] - s0["slot_steps"],
+            "births": s1["births_total"] - s0["births_total"], "moves": s1["moves_total"] - s0["moves_total"],
+            "fertile": s1["fertile_total"] - s0["fertile_total"], "steps": steps}
+
+
+def gpu_run(sc, warmup, steps, timing):
+    from qhg4_amd.population import GpuPopulation
+    pop = GpuPopulation.from_scenario(sc, capacity=1 << 23)
+    pop.preLoop(0.0)
+    if warmup:
+        pop.run(warmup)
+    pop.sync()
+    if timing:
+        pop.set_timing(True)
+    s0 = pop.stats()
+    t0 = time.perf_counter()
+    pop.run(steps)
+    pop.sync()
+    t1 = time.perf_counter()
+    s1 = pop.stats()
+    times = pop.kernel_times() if timing else None
+    pop.close()
+    return t1 - t0, s0, s1, times
+
+
+def cpu_baseline(sc, warmup, steps, budget_s):
+    from oracle.oracle import Oracle, lib
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    lib().orc_set_threads(threads)
+    o = Oracle(sc)
+    o.pre_loop()
+    if warmup:
+        o.run(warmup)
+    s0 = o.stats()
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps and time.perf_counter() - t0 < budget_s:
+        o.step()
+        done += 1
+    t1 = time.perf_counter()
+    s1 = o.stats()
+    agent_steps = s1["agent_steps"] - s0["agent_steps"]
+    return {"value": agent_steps / (t1 - t0), "unit": "agent-steps/s", "cores": threads, "kind": "port",
+            "sample": "oracle (reference step loop restated in C, OpenMP, same Philox draws) on steps "
+                      "%d..%d of the same workload (%d of the %d timed steps, %.1f s, %d agent-steps)"
+                      % (warmup, warmup + done, done, steps, t1 - t0, agent_steps)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1900)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-kernel-timing", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the population step has no multi-GPU path); "
+              "use --gpus 1", file=sys.stderr)
+        return 2
+
+    from qhg4_amd import abi
+    sc, desc = make_workload(args.workload, args.seed)
+    print("bench: workload %s, warmup %d, steps %d" % (args.workload, args.warmup, args.steps),
+          file=sys.stderr, flush=True)
+    wall, s0, s1, _ = gpu_run(sc, args.warmup, args.steps, timing=False)
+    d = delta(s1, s0, args.steps)
+    value = d["live"] / wall
+    print("bench: timed region %.3f s, %d agent-steps" % (wall, d["live"]), file=sys.stderr, flush=True)
+
+    roof = None
+    kernels = {}
+    if not args.no_kernel_timing:
+        _, t0s, t1s, times = gpu_run(sc, args.warmup, args.steps, timing=True)
+        if t1s["agent_steps"] != s1["agent_steps"] or t1s["max_id"] != s1["max_id"]:
+            raise SystemExit("bench: the timing run diverged from the timed run")
+        p = sc["params"]
+        ops = {p.action_kind[i] for i in range(p.num_actions)}
+        kb = kernel_bytes(bytes_model(), d, sc["grid"]["num_cells"], sc["grid"]["connectivity"], ops)
+        for name, bts in kb.items():
+            avg_ms, n = times.get(name, (0.0, 0))
+            if not n:
+                continue
+            per_launch = bts / n
+            gbps = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+            kernels[name] = {"avg_us": round(avg_ms * 1e3, 3), "launches": n,
+                             "bytes_per_launch": int(per_launch), "GBps": round(gbps, 1)}
+        dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
+        kd = kernels[dom]
+        roof = {"bound": "hbm", "achieved": kd["GBps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": round(kd["GBps"] / HBM_PEAK_GBPS, 5), "traffic": None, "kernel": dom,
+                "bytes_per_launch": kd["bytes_per_launch"], "avg_launch_us": kd["avg_us"]}
+
+    cpu = None if args.no_cpu else cpu_baseline(sc, args.warmup, args.steps, args.cpu_seconds)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "agent-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": desc, "cells": int(sc["grid"]["num_cells"]),
+                   "initial_agents": int(len(sc["agents"]["id"])), "seed": args.seed,
+                   "steps_simulated": args.warmup + args.steps,
+                   "mean_live_agents": round(d["live"] / args.steps, 1),
+                   "final_live_agents": int(s1["num_alive"])},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

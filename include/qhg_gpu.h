@@ -172,6 +172,10 @@ typedef struct qhg_stats {
     int64_t  births_total, deaths_total, moves_total;
     int64_t  compactions;
     int32_t  device_error;
+    int32_t  pad;
+    int64_t  agent_steps;    /* sum over steps of the live count at step start */
+    int64_t  slot_steps;     /* sum over steps of the high-water mark at step start */
+    int64_t  fertile_total;  /* agents bucketed for pairing, all steps */
 } qhg_stats;
 
 typedef struct qhg_pop qhg_pop;

@@ -287,6 +287,7 @@ struct orc {
     uint32_t step;
     int64_t  births_last, deaths_last, moves_last;
     int64_t  births_total, deaths_total, moves_total;
+    int64_t  agent_steps;
     /* pairing scratch */
     int32_t *pc_off, *pc_buf;
 };
@@ -894,6 +895,7 @@ static void finalize(orc *o, double t) {
 
 int orc_step(orc *o) {
     double t = (double)o->step;
+    o->agent_steps += orc_num_alive(o);
     o->births_last = o->deaths_last = o->moves_last = 0;
     for (int64_t i = 0; i < o->n; i++) { o->dest[i] = -1; o->bflag[i] = 0; }
     /* initializeStep: every action's initialize, in prio order */
@@ -940,6 +942,7 @@ void orc_stats(orc *o, qhg_stats *s) {
     s->step = o->step;
     s->births_last = o->births_last; s->deaths_last = o->deaths_last; s->moves_last = o->moves_last;
     s->births_total = o->births_total; s->deaths_total = o->deaths_total; s->moves_total = o->moves_total;
+    s->agent_steps = o->agent_steps;
 }
 
 void orc_cell_counts(orc *o, uint32_t *out) { memcpy(out, o->count, (size_t)o->ncells * 4); }

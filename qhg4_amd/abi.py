@@ -142,6 +142,10 @@ class Stats(C.Structure):
         ("moves_total", C.c_int64),
         ("compactions", C.c_int64),
         ("device_error", C.c_int32),
+        ("pad", C.c_int32),
+        ("agent_steps", C.c_int64),
+        ("slot_steps", C.c_int64),
+        ("fertile_total", C.c_int64),
     ]
 
     def as_dict(self):

@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(kBlock) k_eval_combine(Ctx x, const double *tm
 __global__ void __launch_bounds__(kBlock) k_compact_scan(Ctx x, AgentPtrs dst, int32_t *newidx,
                                                          uint64_t *tiles, int64_t *hw_next) {
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_wave[kBlock / 64];
+    __shared__ uint32_t s_cnt[kActItems * (kBlock / 64)];
     __shared__ uint64_t s_excl;
     const int64_t hw = x.st->hw;
     while (true) {
@@ -147,19 +147,28 @@ __global__ void __launch_bounds__(kBlock) k_compact_scan(Ctx x, AgentPtrs dst, i
         const uint32_t tile = s_tile;
         const int64_t base = (int64_t)tile * kActTile;
         if (base >= hw) break;
-        const int64_t i = base + threadIdx.x;
-        const uint32_t alive = (i < hw) ? (x.a.life[i] != 0) : 0u;
-        uint32_t tot;
-        const uint32_t rank = block_excl_scan(alive, s_wave, tot);
-        if (threadIdx.x == 0) {
+        uint32_t alive[kActItems];
+#pragma unroll
+        for (int k = 0; k < kActItems; k++) {
+            const int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
+            alive[k] = (i < hw) ? (x.a.life[i] != 0) : 0u;
+        }
+        uint32_t rank[kActItems], tot;
+        tile_ranks(alive, rank, s_cnt, tot);
+        if (threadIdx.x < 64) {
             const uint64_t ex = lookback(tiles, tile, tot);
-            s_excl = ex;
-            if (base + kActTile >= hw) *hw_next = (int64_t)(ex + tot);
+            if (threadIdx.x == 0) {
+                s_excl = ex;
+                if (base + kActTile >= hw) *hw_next = (int64_t)(ex + tot);
+            }
         }
         __syncthreads();
-        if (i < hw) {
-            if (alive) {
-                const int64_t j = (int64_t)s_excl + rank;
+#pragma unroll
+        for (int k = 0; k < kActItems; k++) {
+            const int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
+            if (i >= hw) continue;
+            if (alive[k]) {
+                const int64_t j = (int64_t)s_excl + rank[k];
                 newidx[i] = (int32_t)j;
                 dst.id[j] = x.a.id[i];
                 dst.cell[j] = x.a.cell[i];

@@ -44,7 +44,7 @@ enum KernelId { KB_BUCKET, KB_PAIR, KB_ACT, KB_NEWBORN, KB_CELLSCAN, KB_HIST, KB
 const char *kKernelNames[KB_COUNT] = {"k_bucket", "k_pair", "k_act", "k_newborn", "k_cellscan",
                                       "k_hist", "k_capacity", "k_eval", "k_compact"};
 
-constexpr int kRing = 8;
+constexpr int kRing = kStatusRing;
 
 struct AgentBufs {
     AgentPtrs p{};
@@ -88,7 +88,8 @@ struct qhg_pop {
     bool initialized = false; // initialize phase launched for the current step
 
     // asynchronous status readback ring
-    DevStatus *h_ring = nullptr;     // pinned, kRing entries
+    DevStatus *h_ring = nullptr;     // pinned, kRing entries, written by k_cellscan
+    DevStatus *d_ring = nullptr;     // device view of h_ring
     hipEvent_t ring_ev[kRing] = {};
     int64_t ring_step[kRing];
     int64_t steps_enqueued = 0;      // step index of the next step to enqueue
@@ -127,6 +128,7 @@ struct qhg_pop {
         x.tiles_cell = d_tiles_cell;
         x.ntiles_act_max = bufs[cur].cap / kActTile + 2;
         x.st = d_st;
+        x.hring = d_ring;
         x.seed = P.seed;
         return x;
     }
@@ -383,13 +385,16 @@ int compact_into(qhg_pop *p, int64_t new_cap) {
 }
 
 void poll_ring(qhg_pop *p) {
-    // newest completed readback wins
+    // newest completed snapshot wins; its own step field says which step it
+    // belongs to (a slot may already hold a later step's snapshot)
     for (int k = 0; k < kRing; k++) {
         if (p->ring_step[k] < 0) continue;
         if (hipEventQuery(p->ring_ev[k]) != hipSuccess) continue;
-        if (p->ring_step[k] > p->known_step) {
-            p->known = p->h_ring[k];
-            p->known_step = p->ring_step[k];
+        DevStatus snap;
+        std::memcpy(&snap, (const void *)&p->h_ring[k], sizeof(snap));
+        if (snap.step > p->known_step) {
+            p->known = snap;
+            p->known_step = snap.step;
         }
         p->ring_step[k] = -1;
     }
@@ -401,45 +406,70 @@ int wait_ring_all(qhg_pop *p) {
     return 0;
 }
 
-// Keep hw + worst-case births of every step not yet observed below capacity,
-// compact when holes dominate.
+// oldest step whose snapshot the host has not consumed (or -1)
+int oldest_slot(const qhg_pop *p) {
+    int best = -1;
+    for (int k = 0; k < kRing; k++)
+        if (p->ring_step[k] >= 0 && (best < 0 || p->ring_step[k] < p->ring_step[best])) best = k;
+    return best;
+}
+
+double hw_bound(const DevStatus &k, int64_t lag) {
+    // births per step <= live/2 (one per mated female), live grows <= x1.5
+    double hw = (double)k.hw, alive = (double)k.n_alive;
+    for (int64_t j = 0; j <= lag; j++) {
+        hw += std::ceil(alive / 2) + 1;
+        alive = alive * 1.5 + 1;
+    }
+    return hw;
+}
+
+// Keep hw + worst-case births of every step not yet observed below capacity
+// (waiting for the oldest in-flight snapshot when the bound is too loose),
+// grow the buffers when even the exact status needs it, and compact in
+// stream order when holes dominate.
 int ensure_capacity(qhg_pop *p) {
     poll_ring(p);
-    auto bound = [&](const DevStatus &k, int64_t lag) {
-        double hw = (double)k.hw, alive = (double)k.n_alive;
-        for (int64_t j = 0; j <= lag; j++) { hw += std::ceil(alive / 2) + 1; alive *= 1.5; alive += 1; }
-        return hw;
-    };
     const int64_t cap = p->bufs[p->cur].cap;
-    int64_t lag = p->steps_enqueued - p->known_step;
-    if (lag < 0) lag = 0;
-    if (bound(p->known, lag) > (double)cap || lag > kRing - 2) {
-        int rc = wait_ring_all(p);
-        if (rc) return rc;
-        DevStatus s;
-        rc = read_status_sync(p, &s);
-        if (rc) return rc;
-        p->known = s;
-        p->known_step = p->steps_enqueued;
-        lag = 0;
-        if (s.error) return fail(QHG_ERR_CAPACITY, "device reported capacity overflow");
-        if (bound(s, 0) > (double)cap) {
-            int64_t need = (int64_t)bound(s, 4);
-            int64_t nc = cap;
-            if (s.hw - s.n_alive < (s.n_alive / 2) || need > cap) {
-                while ((double)nc < 1.25 * (double)need) nc *= 2;
-            }
-            rc = compact_into(p, nc);
-            if (rc) return rc;
-            p->last_compact_step = p->steps_enqueued;
-            rc = read_status_sync(p, &s);
+    int64_t lag = std::max<int64_t>(0, p->steps_enqueued - p->known_step);
+    while ((hw_bound(p->known, lag) > (double)cap || lag > kRing - 2) && lag > 0) {
+        const int k = oldest_slot(p);
+        if (k < 0) {
+            // nothing in flight: read the status directly
+            DevStatus s;
+            int rc = read_status_sync(p, &s);
             if (rc) return rc;
             p->known = s;
+            p->known_step = s.step;
+        } else {
+            HIPCHK(hipEventSynchronize(p->ring_ev[k]));
+            poll_ring(p);
         }
+        lag = std::max<int64_t>(0, p->steps_enqueued - p->known_step);
+    }
+    if (p->known.error) return fail(QHG_ERR_CAPACITY, "device reported an error (flags 0x" +
+                                    std::to_string(p->known.error) + ")");
+    if (hw_bound(p->known, 0) > (double)cap) {
+        // even the exact status does not leave room for one step
+        const DevStatus &s = p->known;
+        int64_t need = (int64_t)hw_bound(s, 4);
+        int64_t nc = cap;
+        if (2 * s.n_alive > cap || need > 2 * cap) {
+            while ((double)nc < 1.25 * (double)need) nc *= 2;
+        }
+        int rc = compact_into(p, nc);
+        if (rc) return rc;
+        p->last_compact_step = p->steps_enqueued;
+        DevStatus s2;
+        rc = read_status_sync(p, &s2);
+        if (rc) return rc;
+        p->known = s2;
+        p->known_step = s2.step;
+        if (hw_bound(s2, 0) > (double)p->bufs[p->cur].cap) return fail(QHG_ERR_CAPACITY, "capacity growth failed");
         return 0;
     }
     // holes dominate: compact in stream order (no host sync); the known
-    // high-water mark stays a valid upper bound until the next readback
+    // high-water mark stays a valid upper bound until the next snapshot
     const DevStatus &k = p->known;
     if (k.hw - k.n_alive > std::max<int64_t>(k.n_alive, 1 << 16) && p->known_step > p->last_compact_step) {
         int rc = compact_into(p, cap);
@@ -455,7 +485,6 @@ int enqueue_readback(qhg_pop *p) {
         HIPCHK(hipEventSynchronize(p->ring_ev[slot]));
         poll_ring(p);
     }
-    HIPCHK(hipMemcpyAsync(&p->h_ring[slot], p->d_st, sizeof(DevStatus), hipMemcpyDeviceToHost, p->stream));
     HIPCHK(hipEventRecord(p->ring_ev[slot], p->stream));
     p->ring_step[slot] = p->steps_enqueued;
     return 0;
@@ -585,8 +614,11 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
         CK(upload_array(&p->d_ice, env->ice, nc, p->stream));
         CK(upload_array(&p->d_coastal, env->coastal, nc, p->stream));
     }
-    if (hipHostMalloc((void **)&p->h_ring, sizeof(DevStatus) * kRing, hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void **)&p->h_ring, sizeof(DevStatus) * kRing, hipHostMallocMapped) != hipSuccess)
         return cleanup_fail(fail(QHG_ERR_NOMEM, "pinned status ring"));
+    std::memset(p->h_ring, 0, sizeof(DevStatus) * kRing);
+    if (hipHostGetDevicePointer((void **)&p->d_ring, p->h_ring, 0) != hipSuccess)
+        return cleanup_fail(fail(QHG_ERR_HIP, "status ring device pointer"));
     for (int k = 0; k < kRing; k++) {
         if (hipEventCreateWithFlags(&p->ring_ev[k], hipEventDisableTiming) != hipSuccess)
             return cleanup_fail(fail(QHG_ERR_HIP, "event"));
@@ -879,6 +911,9 @@ int qhg_gpu_stats(qhg_pop *p, qhg_stats *o) {
     o->moves_total = s.moves_total;
     o->compactions = p->compactions;
     o->device_error = s.error;
+    o->agent_steps = s.agent_steps;
+    o->slot_steps = s.slot_steps;
+    o->fertile_total = s.fertile_total;
     return 0;
 }
 

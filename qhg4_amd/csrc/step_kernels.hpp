@@ -31,10 +31,12 @@
 namespace qhg {
 
 constexpr int kBlock = 256;          // threads per block (4 waves)
-constexpr int kActTile = 256;        // agent slots per look-back tile
+constexpr int kActItems = 4;         // agent slots per thread per tile
+constexpr int kActTile = kBlock * kActItems;   // agent slots per look-back tile
 constexpr int kCellTile = 1024;      // cells per look-back tile (4 per thread)
 constexpr int kMaxOps = QHG_MAX_ACTIONS;
 constexpr int kMaxCross = 32;        // device limit on Genetics_num_crossover
+constexpr int kStatusRing = 8;       // host-mapped status snapshots, one per step
 
 struct DevStatus {
     int64_t hw, n_alive, max_id, step;
@@ -42,6 +44,9 @@ struct DevStatus {
     int64_t births_cur, deaths_cur, moves_cur;
     int64_t births_last, deaths_last, moves_last;
     int64_t births_total, deaths_total, moves_total;
+    int64_t agent_steps;     // sum over steps of the live count at step start
+    int64_t slot_steps;      // sum over steps of the high-water mark at step start
+    int64_t fertile_total;   // agents bucketed for pairing, all steps
     uint32_t tile_ctr_act, tile_ctr_cell, tile_ctr_cmp, pad0;
     int32_t error, pad1;
 };
@@ -81,6 +86,7 @@ struct Ctx {
     uint64_t *tiles_act, *tiles_cell;
     int64_t ntiles_act_max;
     DevStatus *st;
+    DevStatus *hring;      // host-mapped ring of kStatusRing snapshots (may be null)
     uint64_t seed;
 };
 
@@ -112,24 +118,46 @@ __device__ __forceinline__ uint64_t tile_read(uint64_t *w) {
     return __hip_atomic_load(w, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Called by one thread. Returns the exclusive prefix of `tile`.
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)v, d, 64);
+        const uint32_t hi = __shfl_xor((uint32_t)(v >> 32), d, 64);
+        v += ((uint64_t)hi << 32) | lo;
+    }
+    return v;
+}
+
+// Called by all 64 lanes of one wave.  Publishes the tile's aggregate, then
+// reads 64 predecessor tiles per probe: the nearest inclusive prefix ends the
+// walk; aggregates on the way are summed.  Returns the exclusive prefix of
+// `tile` (same value in every lane) and publishes the tile's inclusive value.
 __device__ uint64_t lookback(uint64_t *tiles, uint32_t tile, uint64_t agg) {
+    const int lane = threadIdx.x & 63;
     if (tile == 0) {
-        tile_publish(&tiles[0], kFlagInc | agg);
+        if (lane == 0) tile_publish(&tiles[0], kFlagInc | agg);
         return 0;
     }
-    tile_publish(&tiles[tile], kFlagAgg | agg);
+    if (lane == 0) tile_publish(&tiles[tile], kFlagAgg | agg);
     uint64_t excl = 0;
-    int64_t j = (int64_t)tile - 1;
+    int64_t top = (int64_t)tile - 1;
     while (true) {
-        uint64_t w = tile_read(&tiles[j]);
-        uint64_t flag = w & ~kValMask;
-        if (flag == 0) { __builtin_amdgcn_s_sleep(1); continue; }
-        excl += w & kValMask;
-        if (flag == kFlagInc) break;
-        j--;
+        const int64_t j = top - lane;
+        const uint64_t w = (j >= 0) ? tile_read(&tiles[j]) : kFlagInc;
+        const uint64_t flag = w & ~kValMask;
+        const uint64_t inc = __ballot(flag == kFlagInc);
+        const uint64_t nready = __ballot(flag == 0);
+        const int first = inc ? (__ffsll((unsigned long long)inc) - 1) : 64;
+        const uint64_t need = (first >= 63) ? ~0ull : ((2ull << first) - 1ull);
+        if (nready & need) {
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        excl += wave_sum_u64(lane <= first ? (w & kValMask) : 0ull);
+        if (first < 64) break;
+        top -= 64;
     }
-    tile_publish(&tiles[tile], kFlagInc | (excl + agg));
+    if (lane == 0) tile_publish(&tiles[tile], kFlagInc | (excl + agg));
     return excl;
 }
 
@@ -161,6 +189,10 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_wave
 // males from the back (region size = live count of the cell >= F + M).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_bucket(Ctx x) {
+    __shared__ uint32_t s_fert;
+    if (threadIdx.x == 0) s_fert = 0;
+    __syncthreads();
+    uint32_t my_fert = 0;
     const int64_t hw = x.st->hw;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < hw;
          i += (int64_t)gridDim.x * kBlock) {
@@ -168,6 +200,7 @@ __global__ void __launch_bounds__(kBlock) k_bucket(Ctx x) {
         if (l == 0) continue;
         x.a.mate[i] = -3;
         if (l != QHG_LIFE_FERTILE) continue;
+        my_fert++;
         const int32_t c = x.a.cell[i];
         const uint8_t g = x.a.gender[i];
         const uint32_t lo = x.off[c], hi = x.off[c + 1];
@@ -181,6 +214,10 @@ __global__ void __launch_bounds__(kBlock) k_bucket(Ctx x) {
             else atomicOr(&x.st->error, kErrBucket);
         }
     }
+    if (my_fert) atomicAdd(&s_fert, my_fert);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_fert)
+        atomicAdd((unsigned long long *)&x.st->fertile_total, (unsigned long long)s_fert);
 }
 
 __device__ void sort_u32(uint32_t *v, uint32_t n) {
@@ -307,16 +344,177 @@ __global__ void __launch_bounds__(kBlock) k_pair(Ctx x, int32_t do_pair) {
 // the end of the agent's program (performMoves, SPopulation.cpp:1058-1092):
 // actions of this step see the pre-move cell, births are registered in it.
 // ---------------------------------------------------------------------------
+struct StepView {
+    uint32_t step;
+    double t;
+    const uint32_t *cnt_prev;
+    uint32_t *cnt_next;
+};
+
+// One agent's program for this step.  Returns 1 and the birth record fields
+// when the agent registers a birth.
+__device__ __forceinline__ uint32_t act_one(const Ctx &x, const Prog &p, const StepView &v, int64_t i,
+                                            int32_t &bcell, int32_t &bfather, uint32_t &deaths,
+                                            uint32_t &moves) {
+    uint8_t life = x.a.life[i];
+    if (life == 0) return 0;
+    const uint8_t life0 = life;
+    const int32_t c = x.a.cell[i];
+    const int64_t id = x.a.id[i];
+    const uint32_t step = v.step;
+    const double t = v.t;
+    const int32_t R = x.conn + 1;
+    uint8_t g = 0;
+    bool have_g = false;
+    double age = p.load_age ? x.a.age[i] : 0.0;
+    bool age_dirty = false;
+    int32_t dest = -1;
+    uint32_t bflag = 0;
+    for (int q = 0; q < p.nops; q++) {
+        const int op = p.ops[q];
+        if (op == QHG_ACT_GETOLD) {
+            // GetOld::execute, QHG4/actions/GetOld.cpp:37-48
+            age = t - x.a.birth[i];
+            age_dirty = true;
+        } else if (op == QHG_ACT_FERTILITY) {
+            // Fertility::execute, QHG4/actions/Fertility.cpp:49-74
+            if (!have_g) { g = x.a.gender[i]; have_g = true; }
+            if (g == 0) {
+                const double lb = x.a.lastb[i];
+                life = (age > p.fert_min && age < p.fert_max && (t - lb) > p.fert_inter)
+                           ? QHG_LIFE_FERTILE : QHG_LIFE_ALIVE;
+            } else {
+                life = (age > p.fert_min) ? QHG_LIFE_FERTILE : QHG_LIFE_ALIVE;
+            }
+        } else if (op == QHG_ACT_VERHULST || op == QHG_ACT_VERHULSTVARK) {
+            // LinearBirth::initialize/execute then LinearDeath (Verhulst.cpp:101-115;
+            // LinearBirth.cpp:90-168; LinearDeath.cpp:92-153)
+            const double N = (double)v.cnt_prev[c];
+            double b, d;
+            if (op == QHG_ACT_VERHULST) {
+                b = p.b0 + (p.theta - p.b0) * (N / p.Kconst);
+                d = p.d0 + (p.theta - p.d0) * (N / p.Kconst);
+            } else {
+                const double K = x.K[c];
+                b = (K <= 0) ? 0.0 : p.b0 + (p.theta - p.b0) * (N / K);
+                d = (K <= 0) ? 1.0 : p.d0 + (p.theta - p.d0) * (N / K);
+            }
+            if (b > 0) {
+                if (!have_g) { g = x.a.gender[i]; have_g = true; }
+                if (g == 0) {
+                    const int32_t m = x.a.mate[i];
+                    if (m >= 0) {
+                        const double r = wrandd(draw(x.seed, step, QHG_STREAM_LBIRTH, (uint64_t)id, 0));
+                        if (r < b) { bflag = 1; bcell = c; bfather = m; }
+                    }
+                }
+            } else if (b < 0) {
+                const double r = wrandd(draw(x.seed, step, QHG_STREAM_LBIRTH, (uint64_t)id, 0));
+                if (r < -b) life = 0;
+            }
+            if (life != 0) {
+                const double r = wrandd(draw(x.seed, step, QHG_STREAM_LDEATH, (uint64_t)id, 0));
+                if (r < d) life = 0;
+            }
+            if (life == 0) { deaths++; break; }
+        } else if (op == QHG_ACT_RANDOMMOVE) {
+            // RandomMove::execute, QHG4/actions/RandomMove.cpp:65-100
+            uint32_t u[4];
+            draw4(x.seed, step, QHG_STREAM_RMOVE, (uint64_t)id, 0, u);
+            if (wrandd(u[0]) < p.rmove_p) {
+                const int nn = x.nn[c];
+                const int slot = (int)(wrandd(u[1]) * (nn + 1));
+                if (slot > 0) {
+                    const int32_t nb = x.nbr[(int64_t)c * x.conn + slot - 1];
+                    if (nb >= 0) { if (dest < 0) moves++; dest = nb; }
+                }
+            }
+        } else if (op == QHG_ACT_WEIGHTEDMOVE) {
+            // WeightedMove::execute, QHG4/actions/WeightedMove.cpp:45-106
+            uint32_t u[4];
+            draw4(x.seed, step, QHG_STREAM_WMOVE, (uint64_t)id, 0, u);
+            if (wrandd(u[0]) < p.wmove_p) {
+                const int nn = x.nn[c];
+                const double *w = x.W + (int64_t)c * R;
+                const double wl = w[nn];
+                int idx = -1;
+                if (w[0] == wl) {
+                    idx = (int)wrandi(0, (uint32_t)nn + 1, 1, u[1]);
+                } else {
+                    const double r2 = wrandd(u[1]) * wl;
+                    for (int q2 = 0; q2 < nn + 1; q2++) {
+                        if (r2 < w[q2]) { idx = q2; break; }
+                    }
+                }
+                if (idx > 0) {
+                    const int32_t nb = x.nbr[(int64_t)c * x.conn + idx - 1];
+                    if (nb >= 0 && (x.ice == nullptr || !x.ice[nb])) {
+                        if (dest < 0) moves++;
+                        dest = nb;
+                    }
+                }
+            }
+        }
+        // RandomPair, MultiEvaluator, NPPCapacity, Genetics: no execute
+    }
+    if (life != life0) x.a.life[i] = life;
+    if (age_dirty) x.a.age[i] = age;
+    if (bflag) x.a.nbabies[i] += 1;
+    int32_t cfin = c;
+    if (dest >= 0) { x.a.cell[i] = dest; cfin = dest; }
+    if (life != 0) atomicAdd(&v.cnt_next[cfin], 1u);
+    return bflag;
+}
+
+// exclusive ranks of flags over a tile laid out as kActItems sub-tiles of
+// kBlock slots (slot = base + k*kBlock + tid): rank[k] for this thread's
+// item k, tile total.  s_cnt has kActItems * waves entries.
+__device__ __forceinline__ void tile_ranks(const uint32_t (&flag)[kActItems], uint32_t (&rank)[kActItems],
+                                           uint32_t *s_cnt, uint32_t &total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int W = kBlock / 64;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t inwave[kActItems];
+#pragma unroll
+    for (int k = 0; k < kActItems; k++) {
+        const uint64_t m = __ballot(flag[k] != 0);
+        inwave[k] = (uint32_t)__popcll(m & lt);
+        if (lane == 0) s_cnt[k * W + wave] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    uint32_t run = 0, tot = 0;
+#pragma unroll
+    for (int j = 0; j < kActItems * W; j++) {
+        const uint32_t c = s_cnt[j];
+        tot += c;
+    }
+#pragma unroll
+    for (int k = 0; k < kActItems; k++) {
+        run = 0;
+        for (int j = 0; j < k * W + wave; j++) run += s_cnt[j];
+        rank[k] = run + inwave[k];
+    }
+    total = tot;
+}
+
+// ---------------------------------------------------------------------------
+// k_act: fused doActions.  Each agent runs the step's actions in priority
+// order; a dead agent stops (SPopulation.cpp:566-571).  Moves are applied at
+// the end of the agent's program (performMoves, SPopulation.cpp:1058-1092):
+// actions of this step see the pre-move cell, births are registered in it.
+// Birth records are written in slot (= mother id) order through a
+// decoupled look-back over 1024-slot tiles.
+// ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_act(Ctx x, Prog p) {
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_wave[kBlock / 64];
+    __shared__ uint32_t s_cnt[kActItems * (kBlock / 64)];
     __shared__ uint64_t s_excl;
     const int64_t hw = x.st->hw;
-    const uint32_t step = (uint32_t)x.st->step;
-    const double t = x.st->t;
-    const uint32_t *cnt_prev = x.cnt[step & 1];
-    uint32_t *cnt_next = x.cnt[(step + 1) & 1];
-    const int32_t R = x.conn + 1;
+    StepView v;
+    v.step = (uint32_t)x.st->step;
+    v.t = x.st->t;
+    v.cnt_prev = x.cnt[v.step & 1];
+    v.cnt_next = x.cnt[(v.step + 1) & 1];
     uint32_t my_deaths = 0, my_moves = 0;
 
     while (true) {
@@ -325,125 +523,35 @@ __global__ void __launch_bounds__(kBlock) k_act(Ctx x, Prog p) {
         const uint32_t tile = s_tile;
         const int64_t base = (int64_t)tile * kActTile;
         if (base >= hw) break;
-        const int64_t i = base + threadIdx.x;
-        uint32_t bflag = 0;
-        int32_t bcell = 0, bfather = -1;
-        if (i < hw) {
-            uint8_t life = x.a.life[i];
-            if (life != 0) {
-                const uint8_t life0 = life;
-                const int32_t c = x.a.cell[i];
-                const uint8_t g = x.a.gender[i];
-                const int64_t id = x.a.id[i];
-                double age = p.load_age ? x.a.age[i] : 0.0;
-                bool age_dirty = false;
-                int32_t dest = -1;
-                for (int q = 0; q < p.nops; q++) {
-                    const int op = p.ops[q];
-                    if (op == QHG_ACT_GETOLD) {
-                        // GetOld::execute, QHG4/actions/GetOld.cpp:37-48
-                        age = t - x.a.birth[i];
-                        age_dirty = true;
-                    } else if (op == QHG_ACT_FERTILITY) {
-                        // Fertility::execute, QHG4/actions/Fertility.cpp:49-74
-                        if (g == 0) {
-                            const double lb = x.a.lastb[i];
-                            life = (age > p.fert_min && age < p.fert_max && (t - lb) > p.fert_inter)
-                                       ? QHG_LIFE_FERTILE : QHG_LIFE_ALIVE;
-                        } else {
-                            life = (age > p.fert_min) ? QHG_LIFE_FERTILE : QHG_LIFE_ALIVE;
-                        }
-                    } else if (op == QHG_ACT_VERHULST || op == QHG_ACT_VERHULSTVARK) {
-                        // LinearBirth::initialize/execute + LinearDeath (Verhulst.cpp:101-115)
-                        const double N = (double)cnt_prev[c];
-                        double b, d;
-                        if (op == QHG_ACT_VERHULST) {
-                            b = p.b0 + (p.theta - p.b0) * (N / p.Kconst);
-                            d = p.d0 + (p.theta - p.d0) * (N / p.Kconst);
-                        } else {
-                            const double K = x.K[c];
-                            b = (K <= 0) ? 0.0 : p.b0 + (p.theta - p.b0) * (N / K);
-                            d = (K <= 0) ? 1.0 : p.d0 + (p.theta - p.d0) * (N / K);
-                        }
-                        if (b > 0) {
-                            if (g == 0) {
-                                const int32_t m = x.a.mate[i];
-                                if (m >= 0) {
-                                    const double r = wrandd(draw(x.seed, step, QHG_STREAM_LBIRTH, (uint64_t)id, 0));
-                                    if (r < b) { bflag = 1; bcell = c; bfather = m; }
-                                }
-                            }
-                        } else if (b < 0) {
-                            const double r = wrandd(draw(x.seed, step, QHG_STREAM_LBIRTH, (uint64_t)id, 0));
-                            if (r < -b) life = 0;
-                        }
-                        if (life != 0) {
-                            const double r = wrandd(draw(x.seed, step, QHG_STREAM_LDEATH, (uint64_t)id, 0));
-                            if (r < d) life = 0;
-                        }
-                        if (life == 0) { my_deaths++; break; }
-                    } else if (op == QHG_ACT_RANDOMMOVE) {
-                        // RandomMove::execute, QHG4/actions/RandomMove.cpp:65-100
-                        uint32_t u[4];
-                        draw4(x.seed, step, QHG_STREAM_RMOVE, (uint64_t)id, 0, u);
-                        if (wrandd(u[0]) < p.rmove_p) {
-                            const int nn = x.nn[c];
-                            const int slot = (int)(wrandd(u[1]) * (nn + 1));
-                            if (slot > 0) {
-                                const int32_t nb = x.nbr[(int64_t)c * x.conn + slot - 1];
-                                if (nb >= 0) { if (dest < 0) my_moves++; dest = nb; }
-                            }
-                        }
-                    } else if (op == QHG_ACT_WEIGHTEDMOVE) {
-                        // WeightedMove::execute, QHG4/actions/WeightedMove.cpp:45-106
-                        uint32_t u[4];
-                        draw4(x.seed, step, QHG_STREAM_WMOVE, (uint64_t)id, 0, u);
-                        if (wrandd(u[0]) < p.wmove_p) {
-                            const int nn = x.nn[c];
-                            const double *w = x.W + (int64_t)c * R;
-                            const double wl = w[nn];
-                            int idx = -1;
-                            if (w[0] == wl) {
-                                idx = (int)wrandi(0, (uint32_t)nn + 1, 1, u[1]);
-                            } else {
-                                const double r2 = wrandd(u[1]) * wl;
-                                for (int q2 = 0; q2 < nn + 1; q2++) {
-                                    if (r2 < w[q2]) { idx = q2; break; }
-                                }
-                            }
-                            if (idx > 0) {
-                                const int32_t nb = x.nbr[(int64_t)c * x.conn + idx - 1];
-                                if (nb >= 0 && (x.ice == nullptr || !x.ice[nb])) {
-                                    if (dest < 0) my_moves++;
-                                    dest = nb;
-                                }
-                            }
-                        }
-                    }
-                    // RandomPair, MultiEvaluator, NPPCapacity, Genetics: no execute
-                }
-                if (life != life0) x.a.life[i] = life;
-                if (age_dirty) x.a.age[i] = age;
-                if (bflag) x.a.nbabies[i] += 1;
-                int32_t cfin = c;
-                if (dest >= 0) { x.a.cell[i] = dest; cfin = dest; }
-                if (life != 0) atomicAdd(&cnt_next[cfin], 1u);
+        uint32_t bflag[kActItems];
+        int32_t bcell[kActItems], bfather[kActItems];
+#pragma unroll
+        for (int k = 0; k < kActItems; k++) {
+            const int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
+            bflag[k] = 0;
+            bcell[k] = 0;
+            bfather[k] = -1;
+            if (i < hw) bflag[k] = act_one(x, p, v, i, bcell[k], bfather[k], my_deaths, my_moves);
+        }
+        uint32_t rank[kActItems], tile_total;
+        tile_ranks(bflag, rank, s_cnt, tile_total);
+        if (threadIdx.x < 64) {
+            const uint64_t ex = lookback(x.tiles_act, tile, tile_total);
+            if (threadIdx.x == 0) {
+                s_excl = ex;
+                if (base + kActTile >= hw) x.st->births_cur = (int64_t)(ex + tile_total);
             }
         }
-        // births of this tile, in slot (= mother id) order
-        uint32_t tile_total;
-        const uint32_t rank = block_excl_scan(bflag, s_wave, tile_total);
-        if (threadIdx.x == 0) {
-            uint64_t ex = lookback(x.tiles_act, tile, tile_total);
-            s_excl = ex;
-            if (base + kActTile >= hw) x.st->births_cur = (int64_t)(ex + tile_total);
-        }
         __syncthreads();
-        if (bflag) {
-            const int64_t r = (int64_t)s_excl + rank;
-            x.brec[3 * r + 0] = (int32_t)i;
-            x.brec[3 * r + 1] = bfather;
-            x.brec[3 * r + 2] = bcell;
+#pragma unroll
+        for (int k = 0; k < kActItems; k++) {
+            if (bflag[k]) {
+                const int64_t r = (int64_t)s_excl + rank[k];
+                const int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
+                x.brec[3 * r + 0] = (int32_t)i;
+                x.brec[3 * r + 1] = bfather[k];
+                x.brec[3 * r + 2] = bcell[k];
+            }
         }
         __syncthreads();
     }
@@ -638,7 +746,10 @@ __global__ void __launch_bounds__(kBlock) k_cellscan(Ctx x, int32_t mode) {
         }
         uint32_t tot;
         const uint32_t pre = block_excl_scan(sum, s_wave, tot);
-        if (threadIdx.x == 0) s_excl = lookback(x.tiles_cell, tile, tot);
+        if (threadIdx.x < 64) {
+            const uint64_t ex = lookback(x.tiles_cell, tile, tot);
+            if (threadIdx.x == 0) s_excl = ex;
+        }
         __syncthreads();
         uint64_t run = s_excl + pre;
 #pragma unroll
@@ -655,6 +766,10 @@ __global__ void __launch_bounds__(kBlock) k_cellscan(Ctx x, int32_t mode) {
         if (base + kCellTile >= nc && threadIdx.x == kBlock - 1) {
             x.off[nc] = (uint32_t)run;
             DevStatus *st = x.st;
+            if (mode == 0) {
+                st->agent_steps += st->n_alive;
+                st->slot_steps += st->hw;
+            }
             st->n_alive = (int64_t)run;
             if (mode == 0) {
                 const int64_t b = st->births_cur;
@@ -671,6 +786,14 @@ __global__ void __launch_bounds__(kBlock) k_cellscan(Ctx x, int32_t mode) {
                 st->moves_cur = 0;
                 st->step += 1;
                 st->t += 1.0;
+                if (x.hring) {
+                    // snapshot for the host, slot = new step mod ring size
+                    DevStatus *dst = x.hring + (st->step % kStatusRing);
+                    const uint64_t *srcw = reinterpret_cast<const uint64_t *>(st);
+                    uint64_t *dstw = reinterpret_cast<uint64_t *>(dst);
+                    for (int q = 0; q < (int)(sizeof(DevStatus) / 8); q++) dstw[q] = srcw[q];
+                    __threadfence_system();
+                }
             }
         }
         __syncthreads();

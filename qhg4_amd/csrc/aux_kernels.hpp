@@ -132,43 +132,81 @@ __global__ void __launch_bounds__(kBlock) k_eval_combine(Ctx x, const double *tm
 // ---------------------------------------------------------------------------
 // stable compaction of live slots into the other buffer set
 // (LBController::compactData, QHG4/utils/LBController.cpp:351-445, done as a
-// scan).  Pass 1 scatters every field but mate and records old->new slot;
-// pass 2 remaps mates (a dead partner becomes -1); k_commit_compact sets hw.
+// scan): k_compact_count (live slots per tile) -> k_scan_tiles (one block)
+// -> k_compact_scatter (every field but mate, old->new slot map) ->
+// k_compact_mate (mates remapped, a dead partner becomes -1) ->
+// k_commit_compact (hw := live count).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_compact_scan(Ctx x, AgentPtrs dst, int32_t *newidx,
-                                                         uint64_t *tiles, int64_t *hw_next) {
-    __shared__ uint32_t s_tile;
+__global__ void __launch_bounds__(kBlock) k_compact_count(Ctx x, uint32_t *cnt) {
     __shared__ uint32_t s_cnt[kActItems * (kBlock / 64)];
-    __shared__ uint64_t s_excl;
     const int64_t hw = x.st->hw;
-    while (true) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(&x.st->tile_ctr_cmp, 1u);
-        __syncthreads();
-        const uint32_t tile = s_tile;
-        const int64_t base = (int64_t)tile * kActTile;
-        if (base >= hw) break;
-        uint32_t alive[kActItems];
+    const int64_t ntiles = (hw + kActTile - 1) / kActTile;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int64_t base = tile * kActTile;
+        uint32_t alive[kActItems], rank[kActItems], tot;
 #pragma unroll
         for (int k = 0; k < kActItems; k++) {
             const int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
             alive[k] = (i < hw) ? (x.a.life[i] != 0) : 0u;
         }
-        uint32_t rank[kActItems], tot;
         tile_ranks(alive, rank, s_cnt, tot);
-        if (threadIdx.x < 64) {
-            const uint64_t ex = lookback(tiles, tile, tot);
-            if (threadIdx.x == 0) {
-                s_excl = ex;
-                if (base + kActTile >= hw) *hw_next = (int64_t)(ex + tot);
-            }
-        }
+        if (threadIdx.x == 0) cnt[tile] = tot;
         __syncthreads();
+    }
+}
+
+// one block: off[t] = sum of cnt[0..t), *total = sum of all
+__global__ void __launch_bounds__(1024) k_scan_tiles(const uint32_t *cnt, uint32_t *off, const DevStatus *st,
+                                                      int64_t *total) {
+    __shared__ uint32_t s_w[1024 / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t ntiles = (st->hw + kActTile - 1) / kActTile;
+    uint64_t run = 0;
+    for (int64_t b0 = 0; b0 < ntiles; b0 += 1024) {
+        const int64_t t = b0 + threadIdx.x;
+        const uint32_t c = (t < ntiles) ? cnt[t] : 0u;
+        uint32_t y = c;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(y, d, 64);
+            if (lane >= d) y += o;
+        }
+        if (lane == 63) s_w[wave] = y;
+        __syncthreads();
+        uint32_t wpre = 0, tot = 0;
+        for (int w = 0; w < 1024 / 64; w++) {
+            const uint32_t sv = s_w[w];
+            if (w < wave) wpre += sv;
+            tot += sv;
+        }
+        if (t < ntiles) off[t] = (uint32_t)(run + wpre + y - c);
+        run += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = (int64_t)run;
+}
+
+__global__ void __launch_bounds__(kBlock) k_compact_scatter(Ctx x, AgentPtrs dst, int32_t *newidx,
+                                                            const uint32_t *off) {
+    __shared__ uint32_t s_cnt[kActItems * (kBlock / 64)];
+    const int64_t hw = x.st->hw;
+    const int64_t ntiles = (hw + kActTile - 1) / kActTile;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int64_t base = tile * kActTile;
+        uint32_t alive[kActItems], rank[kActItems], tot;
+#pragma unroll
+        for (int k = 0; k < kActItems; k++) {
+            const int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
+            alive[k] = (i < hw) ? (x.a.life[i] != 0) : 0u;
+        }
+        tile_ranks(alive, rank, s_cnt, tot);
+        const int64_t o = off[tile];
 #pragma unroll
         for (int k = 0; k < kActItems; k++) {
             const int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
             if (i >= hw) continue;
             if (alive[k]) {
-                const int64_t j = (int64_t)s_excl + rank[k];
+                const int64_t j = o + rank[k];
                 newidx[i] = (int32_t)j;
                 dst.id[j] = x.a.id[i];
                 dst.cell[j] = x.a.cell[i];
@@ -206,7 +244,6 @@ __global__ void k_commit_compact(DevStatus *st, const int64_t *hw_next) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         st->hw = *hw_next;
         st->n_alive = *hw_next;
-        st->tile_ctr_cmp = 0;
     }
 }
 

@@ -40,9 +40,9 @@ int fail(int code, const std::string &msg) {
     } while (0)
 
 enum KernelId { KB_BUCKET, KB_PAIR, KB_ACT, KB_NEWBORN, KB_CELLSCAN, KB_HIST, KB_CAPACITY,
-                KB_EVAL, KB_COMPACT, KB_COUNT };
+                KB_EVAL, KB_COMPACT, KB_TILESCAN, KB_COUNT };
 const char *kKernelNames[KB_COUNT] = {"k_bucket", "k_pair", "k_act", "k_newborn", "k_cellscan",
-                                      "k_hist", "k_capacity", "k_eval", "k_compact"};
+                                      "k_hist", "k_capacity", "k_eval", "k_compact", "k_tilescan"};
 
 constexpr int kRing = kStatusRing;
 
@@ -73,7 +73,8 @@ struct qhg_pop {
     uint32_t *d_bucket = nullptr;
     int32_t *d_brec = nullptr;
     int32_t *d_newidx = nullptr;
-    uint64_t *d_tiles_act = nullptr, *d_tiles_cell = nullptr, *d_tiles_cmp = nullptr;
+    uint32_t *d_tile_cnt = nullptr, *d_tile_off = nullptr, *d_blk_fert = nullptr;
+    uint64_t *d_tiles_cell = nullptr;
     int64_t *d_hw_next = nullptr;
     DevStatus *d_st = nullptr;
     // env on device
@@ -124,7 +125,9 @@ struct qhg_pop {
         x.fillM = d_fillM;
         x.bucket = d_bucket;
         x.brec = d_brec;
-        x.tiles_act = d_tiles_act;
+        x.tile_cnt = d_tile_cnt;
+        x.tile_off = d_tile_off;
+        x.blk_fert = d_blk_fert;
         x.tiles_cell = d_tiles_cell;
         x.ntiles_act_max = bufs[cur].cap / kActTile + 2;
         x.st = d_st;
@@ -144,7 +147,15 @@ int grid_for(int64_t n, int max_blocks) {
 }
 
 template <typename... KArgs, typename... Args>
+int launch_bs(qhg_pop *p, int kid, void (*kern)(KArgs...), int grid, int block, Args... args);
+
+template <typename... KArgs, typename... Args>
 int launch(qhg_pop *p, int kid, void (*kern)(KArgs...), int grid, Args... args) {
+    return launch_bs(p, kid, kern, grid, kBlock, args...);
+}
+
+template <typename... KArgs, typename... Args>
+int launch_bs(qhg_pop *p, int kid, void (*kern)(KArgs...), int grid, int block, Args... args) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (p->timing) {
         size_t slot = p->ev_used.size();
@@ -159,7 +170,7 @@ int launch(qhg_pop *p, int kid, void (*kern)(KArgs...), int grid, Args... args) 
         p->ev_used.push_back({kid, slot});
         HIPCHK(hipEventRecord(e0, p->stream));
     }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, p->stream, args...);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, p->stream, args...);
     HIPCHK(hipGetLastError());
     if (p->timing) HIPCHK(hipEventRecord(e1, p->stream));
     return 0;
@@ -209,15 +220,14 @@ int alloc_slot_scratch(qhg_pop *p, int64_t cap) {
     if (p->d_bucket) (void)hipFree(p->d_bucket);
     if (p->d_brec) (void)hipFree(p->d_brec);
     if (p->d_newidx) (void)hipFree(p->d_newidx);
-    if (p->d_tiles_act) (void)hipFree(p->d_tiles_act);
-    if (p->d_tiles_cmp) (void)hipFree(p->d_tiles_cmp);
-    HIPCHK(hipMalloc(&p->d_bucket, (size_t)cap * 4));
-    HIPCHK(hipMalloc(&p->d_brec, (size_t)cap * 12));
-    HIPCHK(hipMalloc(&p->d_newidx, (size_t)cap * 4));
+    if (p->d_tile_cnt) (void)hipFree(p->d_tile_cnt);
+    if (p->d_tile_off) (void)hipFree(p->d_tile_off);
     const size_t nt = (size_t)(cap / kActTile + 2);
-    HIPCHK(hipMalloc(&p->d_tiles_act, nt * 8));
-    HIPCHK(hipMalloc(&p->d_tiles_cmp, nt * 8));
-    HIPCHK(hipMemsetAsync(p->d_tiles_act, 0, nt * 8, p->stream));
+    HIPCHK(hipMalloc(&p->d_bucket, (size_t)cap * 4));
+    HIPCHK(hipMalloc(&p->d_brec, ((size_t)cap + kActTile) * 12));
+    HIPCHK(hipMalloc(&p->d_newidx, (size_t)cap * 4));
+    HIPCHK(hipMalloc(&p->d_tile_cnt, nt * 12));
+    HIPCHK(hipMalloc(&p->d_tile_off, nt * 4));
     return 0;
 }
 
@@ -368,10 +378,14 @@ int compact_into(qhg_pop *p, int64_t new_cap) {
         if (old_newidx) (void)hipFree(old_newidx);
     }
     Ctx x = p->ctx();
-    const size_t nt = (size_t)(p->bufs[p->cur].cap / kActTile + 2);
-    HIPCHK(hipMemsetAsync(p->d_tiles_cmp, 0, nt * 8, p->stream));
-    int rc = launch(p, KB_COMPACT, k_compact_scan, p->num_cu * 4, x, p->bufs[other].p, p->d_newidx,
-                    p->d_tiles_cmp, p->d_hw_next);
+    const int g = p->num_cu * 4;
+    int rc = launch(p, KB_COMPACT, k_compact_count, g, x, p->d_tile_cnt);
+    if (rc) return rc;
+    rc = launch_bs(p, KB_COMPACT, k_scan_tiles, 1, 1024, (const uint32_t *)p->d_tile_cnt, p->d_tile_off,
+                   (const DevStatus *)p->d_st, p->d_hw_next);
+    if (rc) return rc;
+    rc = launch(p, KB_COMPACT, k_compact_scatter, g, x, p->bufs[other].p, p->d_newidx,
+                (const uint32_t *)p->d_tile_off);
     if (rc) return rc;
     rc = launch(p, KB_COMPACT, k_compact_mate, 2048, x, p->bufs[other].p, (const int32_t *)p->d_newidx);
     if (rc) return rc;
@@ -400,11 +414,6 @@ void poll_ring(qhg_pop *p) {
     }
 }
 
-int wait_ring_all(qhg_pop *p) {
-    HIPCHK(hipStreamSynchronize(p->stream));
-    poll_ring(p);
-    return 0;
-}
 
 // oldest step whose snapshot the host has not consumed (or -1)
 int oldest_slot(const qhg_pop *p) {
@@ -498,7 +507,7 @@ int phase_initialize(qhg_pop *p) {
     }
     Ctx x = p->ctx();
     if (p->has_pair) {
-        int rc = launch(p, KB_BUCKET, k_bucket, p->num_cu * 8, x);
+        int rc = launch(p, KB_BUCKET, k_bucket, kBucketBlocks, x);
         if (rc) return rc;
     }
     int rc = launch(p, KB_PAIR, k_pair, grid_for(p->ncells, p->num_cu * 8), x, (int32_t)p->has_pair);
@@ -512,7 +521,10 @@ int phase_act(qhg_pop *p) {
         int rc = phase_initialize(p);
         if (rc) return rc;
     }
-    int rc = launch(p, KB_ACT, k_act, p->num_cu * 8, p->ctx(), p->prog);
+    Ctx x = p->ctx();
+    int rc = launch(p, KB_ACT, k_act, p->num_cu * 4, x, p->prog);
+    if (rc) return rc;
+    rc = launch_bs(p, KB_TILESCAN, k_tilescan, 1, 1024, x);
     if (rc) return rc;
     p->acted = true;
     return 0;
@@ -597,7 +609,8 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
         hipMalloc(&p->d_cnt[1], nc * 4) != hipSuccess || hipMalloc(&p->d_off, (nc + 1) * 4) != hipSuccess ||
         hipMalloc(&p->d_fillF, nc * 4) != hipSuccess || hipMalloc(&p->d_fillM, nc * 4) != hipSuccess ||
         hipMalloc(&p->d_tiles_cell, (nc / kCellTile + 2) * 8) != hipSuccess ||
-        hipMalloc(&p->d_hw_next, 8) != hipSuccess || hipMalloc(&p->d_st, sizeof(DevStatus)) != hipSuccess)
+        hipMalloc(&p->d_hw_next, 8) != hipSuccess || hipMalloc(&p->d_st, sizeof(DevStatus)) != hipSuccess ||
+        hipMalloc(&p->d_blk_fert, kBucketBlocks * 4) != hipSuccess)
         return cleanup_fail(fail(QHG_ERR_NOMEM, "device allocation failed"));
     (void)hipMemsetAsync(p->d_K, 0, nc * 8, p->stream);
     (void)hipMemsetAsync(p->d_W, 0, nc * (p->conn + 1) * 8, p->stream);
@@ -607,6 +620,7 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
     (void)hipMemsetAsync(p->d_fillF, 0, nc * 4, p->stream);
     (void)hipMemsetAsync(p->d_fillM, 0, nc * 4, p->stream);
     (void)hipMemsetAsync(p->d_st, 0, sizeof(DevStatus), p->stream);
+    (void)hipMemsetAsync(p->d_blk_fert, 0, kBucketBlocks * 4, p->stream);
     if (env) {
         const double *src[7] = {env->altitude, env->longitude, env->latitude, env->water,
                                 env->base_npp, env->temperature, env->rainfall};
@@ -639,7 +653,7 @@ int qhg_gpu_destroy(qhg_pop *p) {
     free_bufs(p->bufs[1]);
     void *ptrs[] = {p->d_nbr, p->d_nn, p->d_K, p->d_W, p->d_tmp, p->d_ice, p->d_cnt[0], p->d_cnt[1],
                     p->d_off, p->d_fillF, p->d_fillM, p->d_bucket, p->d_brec, p->d_newidx,
-                    p->d_tiles_act, p->d_tiles_cell, p->d_tiles_cmp, p->d_hw_next, p->d_st,
+                    p->d_tile_cnt, p->d_tile_off, p->d_blk_fert, p->d_tiles_cell, p->d_hw_next, p->d_st,
                     p->d_coastal};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
@@ -726,6 +740,7 @@ int qhg_gpu_upload_agents(qhg_pop *p, const qhg_agents *a) {
     st.max_id = max_id;
     st.births_cur = st.deaths_cur = st.moves_cur = 0;
     st.tile_ctr_act = st.tile_ctr_cell = st.tile_ctr_cmp = 0;
+    st.error = 0;
     HIPCHK(hipMemcpyAsync(p->d_st, &st, sizeof(st), hipMemcpyHostToDevice, s));
     int rc = recount(p);
     if (rc) return rc;

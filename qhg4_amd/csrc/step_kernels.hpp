@@ -37,6 +37,7 @@ constexpr int kCellTile = 1024;      // cells per look-back tile (4 per thread)
 constexpr int kMaxOps = QHG_MAX_ACTIONS;
 constexpr int kMaxCross = 32;        // device limit on Genetics_num_crossover
 constexpr int kStatusRing = 8;       // host-mapped status snapshots, one per step
+constexpr int kBucketBlocks = 512;   // fixed grid of k_bucket (per-block fertile counts)
 
 struct DevStatus {
     int64_t hw, n_alive, max_id, step;
@@ -82,8 +83,11 @@ struct Ctx {
     uint32_t *off;         // ncells + 1
     uint32_t *fillF, *fillM;
     uint32_t *bucket;      // cap
-    int32_t *brec;         // 3 * cap: mother slot, father slot, cell
-    uint64_t *tiles_act, *tiles_cell;
+    int32_t *brec;         // 3 * (cap + kActTile): birth records staged per act tile
+    uint32_t *tile_cnt;    // 3 per act tile: births, deaths, moves
+    uint32_t *tile_off;    // births before each act tile (exclusive prefix)
+    uint32_t *blk_fert;    // fertile agents bucketed per k_bucket block
+    uint64_t *tiles_cell;
     int64_t ntiles_act_max;
     DevStatus *st;
     DevStatus *hring;      // host-mapped ring of kStatusRing snapshots (may be null)
@@ -132,7 +136,10 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 // reads 64 predecessor tiles per probe: the nearest inclusive prefix ends the
 // walk; aggregates on the way are summed.  Returns the exclusive prefix of
 // `tile` (same value in every lane) and publishes the tile's inclusive value.
-__device__ uint64_t lookback(uint64_t *tiles, uint32_t tile, uint64_t agg) {
+constexpr int32_t kErrLookback = 4;   // a look-back wait exceeded its spin limit
+constexpr uint32_t kLookbackSpins = 1u << 22;
+
+__device__ uint64_t lookback(uint64_t *tiles, uint32_t tile, uint64_t agg, int32_t *err) {
     const int lane = threadIdx.x & 63;
     if (tile == 0) {
         if (lane == 0) tile_publish(&tiles[0], kFlagInc | agg);
@@ -141,6 +148,7 @@ __device__ uint64_t lookback(uint64_t *tiles, uint32_t tile, uint64_t agg) {
     if (lane == 0) tile_publish(&tiles[tile], kFlagAgg | agg);
     uint64_t excl = 0;
     int64_t top = (int64_t)tile - 1;
+    uint32_t spins = 0;
     while (true) {
         const int64_t j = top - lane;
         const uint64_t w = (j >= 0) ? tile_read(&tiles[j]) : kFlagInc;
@@ -150,8 +158,14 @@ __device__ uint64_t lookback(uint64_t *tiles, uint32_t tile, uint64_t agg) {
         const int first = inc ? (__ffsll((unsigned long long)inc) - 1) : 64;
         const uint64_t need = (first >= 63) ? ~0ull : ((2ull << first) - 1ull);
         if (nready & need) {
-            __builtin_amdgcn_s_sleep(1);
-            continue;
+            // every wave exits: after the spin limit the step is flagged as
+            // failed instead of waiting on
+            if (++spins < kLookbackSpins) {
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            if (lane == 0) atomicOr(err, kErrLookback);
+            break;
         }
         excl += wave_sum_u64(lane <= first ? (w & kValMask) : 0ull);
         if (first < 64) break;
@@ -216,8 +230,7 @@ __global__ void __launch_bounds__(kBlock) k_bucket(Ctx x) {
     }
     if (my_fert) atomicAdd(&s_fert, my_fert);
     __syncthreads();
-    if (threadIdx.x == 0 && s_fert)
-        atomicAdd((unsigned long long *)&x.st->fertile_total, (unsigned long long)s_fert);
+    if (threadIdx.x == 0) x.blk_fert[blockIdx.x] = s_fert;
 }
 
 __device__ void sort_u32(uint32_t *v, uint32_t n) {
@@ -282,13 +295,18 @@ __device__ __forceinline__ int select_bit(uint64_t v, int k) {
 __global__ void __launch_bounds__(kBlock) k_pair(Ctx x, int32_t do_pair) {
     const int64_t gtid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t gsize = (int64_t)gridDim.x * kBlock;
-    {
-        const int64_t hw = x.st->hw;
-        const int64_t nt = (hw + kActTile - 1) / kActTile + 1;
-        for (int64_t j = gtid; j < nt && j < x.ntiles_act_max; j += gsize) x.tiles_act[j] = 0;
-        if (gtid == 0) x.st->tile_ctr_act = 0;
-    }
     if (!do_pair) return;
+    if (blockIdx.x == 0) {
+        // fold the per-block fertile counts of k_bucket (diagnostics)
+        __shared__ uint32_t s_sum;
+        if (threadIdx.x == 0) s_sum = 0;
+        __syncthreads();
+        uint32_t v = 0;
+        for (int j = threadIdx.x; j < kBucketBlocks; j += kBlock) v += x.blk_fert[j];
+        if (v) atomicAdd(&s_sum, v);
+        __syncthreads();
+        if (threadIdx.x == 0) x.st->fertile_total += s_sum;
+    }
     const uint32_t step = (uint32_t)x.st->step;
     for (int64_t c = gtid; c < x.ncells; c += gsize) {
         const uint32_t F = x.fillF[c], M = x.fillM[c];
@@ -338,12 +356,6 @@ __global__ void __launch_bounds__(kBlock) k_pair(Ctx x, int32_t do_pair) {
     }
 }
 
-// ---------------------------------------------------------------------------
-// k_act: fused doActions.  Each agent runs the step's actions in priority
-// order; a dead agent stops (SPopulation.cpp:566-571).  Moves are applied at
-// the end of the agent's program (performMoves, SPopulation.cpp:1058-1092):
-// actions of this step see the pre-move cell, births are registered in it.
-// ---------------------------------------------------------------------------
 struct StepView {
     uint32_t step;
     double t;
@@ -502,29 +514,28 @@ __device__ __forceinline__ void tile_ranks(const uint32_t (&flag)[kActItems], ui
 // order; a dead agent stops (SPopulation.cpp:566-571).  Moves are applied at
 // the end of the agent's program (performMoves, SPopulation.cpp:1058-1092):
 // actions of this step see the pre-move cell, births are registered in it.
-// Birth records are written in slot (= mother id) order through a
-// decoupled look-back over 1024-slot tiles.
+// Tiles of kActTile slots are assigned statically (tile = block, block +
+// grid, ...).  A tile stages its birth records in slot (= mother id) order
+// in its own region of brec and writes its birth/death/move counts;
+// k_tilescan turns the counts into global ranks.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_act(Ctx x, Prog p) {
-    __shared__ uint32_t s_tile;
     __shared__ uint32_t s_cnt[kActItems * (kBlock / 64)];
-    __shared__ uint64_t s_excl;
+    __shared__ uint32_t s_red[2];
     const int64_t hw = x.st->hw;
     StepView v;
     v.step = (uint32_t)x.st->step;
     v.t = x.st->t;
     v.cnt_prev = x.cnt[v.step & 1];
     v.cnt_next = x.cnt[(v.step + 1) & 1];
-    uint32_t my_deaths = 0, my_moves = 0;
+    const int64_t ntiles = (hw + kActTile - 1) / kActTile;
 
-    while (true) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(&x.st->tile_ctr_act, 1u);
-        __syncthreads();
-        const uint32_t tile = s_tile;
-        const int64_t base = (int64_t)tile * kActTile;
-        if (base >= hw) break;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int64_t base = tile * kActTile;
+        uint32_t my_deaths = 0, my_moves = 0;
         uint32_t bflag[kActItems];
         int32_t bcell[kActItems], bfather[kActItems];
+        if (threadIdx.x == 0) { s_red[0] = 0; s_red[1] = 0; }
 #pragma unroll
         for (int k = 0; k < kActItems; k++) {
             const int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
@@ -534,37 +545,79 @@ __global__ void __launch_bounds__(kBlock) k_act(Ctx x, Prog p) {
             if (i < hw) bflag[k] = act_one(x, p, v, i, bcell[k], bfather[k], my_deaths, my_moves);
         }
         uint32_t rank[kActItems], tile_total;
-        tile_ranks(bflag, rank, s_cnt, tile_total);
-        if (threadIdx.x < 64) {
-            const uint64_t ex = lookback(x.tiles_act, tile, tile_total);
-            if (threadIdx.x == 0) {
-                s_excl = ex;
-                if (base + kActTile >= hw) x.st->births_cur = (int64_t)(ex + tile_total);
-            }
-        }
-        __syncthreads();
+        tile_ranks(bflag, rank, s_cnt, tile_total);   // contains a barrier
+        int32_t *rec = x.brec + 3 * base;
 #pragma unroll
         for (int k = 0; k < kActItems; k++) {
             if (bflag[k]) {
-                const int64_t r = (int64_t)s_excl + rank[k];
                 const int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
-                x.brec[3 * r + 0] = (int32_t)i;
-                x.brec[3 * r + 1] = bfather[k];
-                x.brec[3 * r + 2] = bcell[k];
+                rec[3 * rank[k] + 0] = (int32_t)i;
+                rec[3 * rank[k] + 1] = bfather[k];
+                rec[3 * rank[k] + 2] = bcell[k];
             }
+        }
+        if (my_deaths) atomicAdd(&s_red[0], my_deaths);
+        if (my_moves) atomicAdd(&s_red[1], my_moves);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            x.tile_cnt[3 * tile + 0] = tile_total;
+            x.tile_cnt[3 * tile + 1] = s_red[0];
+            x.tile_cnt[3 * tile + 2] = s_red[1];
         }
         __syncthreads();
     }
-    // one atomic per block for the step counters
-    __shared__ uint32_t s_red[2];
-    if (threadIdx.x == 0) { s_red[0] = 0; s_red[1] = 0; }
+}
+
+// ---------------------------------------------------------------------------
+// k_tilescan (one block): exclusive prefix of the per-tile birth counts and
+// the step's birth/death/move totals.  Flags and drops the births when they
+// do not fit the capacity.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_tilescan(Ctx x) {
+    __shared__ uint32_t s_w[1024 / 64];
+    __shared__ uint64_t s_tot[3];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t hw = x.st->hw;
+    const int64_t ntiles = (hw + kActTile - 1) / kActTile;
+    if (threadIdx.x < 3) s_tot[threadIdx.x] = 0;
     __syncthreads();
-    if (my_deaths) atomicAdd(&s_red[0], my_deaths);
-    if (my_moves) atomicAdd(&s_red[1], my_moves);
+    uint64_t run = 0, dsum = 0, msum = 0;
+    for (int64_t b0 = 0; b0 < ntiles; b0 += 1024) {
+        const int64_t t = b0 + threadIdx.x;
+        const uint32_t nb = (t < ntiles) ? x.tile_cnt[3 * t] : 0u;
+        if (t < ntiles) { dsum += x.tile_cnt[3 * t + 1]; msum += x.tile_cnt[3 * t + 2]; }
+        uint32_t y = nb;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(y, d, 64);
+            if (lane >= d) y += o;
+        }
+        if (lane == 63) s_w[wave] = y;
+        __syncthreads();
+        uint32_t wpre = 0, tot = 0;
+        for (int w = 0; w < 1024 / 64; w++) {
+            const uint32_t sv = s_w[w];
+            if (w < wave) wpre += sv;
+            tot += sv;
+        }
+        if (t < ntiles) x.tile_off[t] = (uint32_t)(run + wpre + y - nb);
+        run += tot;
+        __syncthreads();
+    }
+    if (dsum) atomicAdd((unsigned long long *)&s_tot[1], (unsigned long long)dsum);
+    if (msum) atomicAdd((unsigned long long *)&s_tot[2], (unsigned long long)msum);
     __syncthreads();
     if (threadIdx.x == 0) {
-        if (s_red[0]) atomicAdd((unsigned long long *)&x.st->deaths_cur, (unsigned long long)s_red[0]);
-        if (s_red[1]) atomicAdd((unsigned long long *)&x.st->moves_cur, (unsigned long long)s_red[1]);
+        DevStatus *st = x.st;
+        st->deaths_cur = (int64_t)s_tot[1];
+        st->moves_cur = (int64_t)s_tot[2];
+        if (hw + (int64_t)run > x.cap) {
+            // no slot for the births: drop them and flag the step as failed
+            st->error |= kErrCapacity;
+            st->births_cur = 0;
+        } else {
+            st->births_cur = (int64_t)run;
+        }
     }
 }
 
@@ -620,24 +673,23 @@ __global__ void __launch_bounds__(kBlock) k_newborn(Ctx x, Prog p) {
         for (int64_t j = gtid; j < nt; j += gsize) x.tiles_cell[j] = 0;
         if (gtid == 0) x.st->tile_ctr_cell = 0;
     }
-    const int64_t nb = x.st->births_cur;
+    const int64_t nbt = x.st->births_cur;
     const int64_t hw = x.st->hw;
     const int64_t max_id = x.st->max_id;
     const uint32_t step = (uint32_t)x.st->step;
     const double t = x.st->t;
     uint32_t *cnt_next = x.cnt[(step + 1) & 1];
-    if (hw + nb > x.cap) {
-        // no slot for the births: drop them and flag the step as failed
-        if (gtid == 0) {
-            atomicOr(&x.st->error, kErrCapacity);
-            x.st->births_cur = 0;
-        }
-        return;
-    }
-    for (int64_t r = gtid; r < nb; r += gsize) {
-        const int32_t mother = x.brec[3 * r + 0];
-        const int32_t father = x.brec[3 * r + 1];
-        const int32_t c = x.brec[3 * r + 2];
+    if (nbt == 0) return;
+    const int64_t ntiles = (hw + kActTile - 1) / kActTile;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+      const uint32_t nbirth = x.tile_cnt[3 * tile];
+      const int64_t off = x.tile_off[tile];
+      const int32_t *rec = x.brec + 3 * tile * kActTile;
+      for (uint32_t j = threadIdx.x; j < nbirth; j += kBlock) {
+        const int64_t r = off + j;
+        const int32_t mother = rec[3 * j + 0];
+        const int32_t father = rec[3 * j + 1];
+        const int32_t c = rec[3 * j + 2];
         const int64_t s = hw + r;
         const int64_t nid = max_id + 1 + r;
         x.a.id[s] = nid;
@@ -703,6 +755,7 @@ __global__ void __launch_bounds__(kBlock) k_newborn(Ctx x, Prog p) {
             }
 #undef GD
         }
+      }
     }
 }
 
@@ -747,7 +800,7 @@ __global__ void __launch_bounds__(kBlock) k_cellscan(Ctx x, int32_t mode) {
         uint32_t tot;
         const uint32_t pre = block_excl_scan(sum, s_wave, tot);
         if (threadIdx.x < 64) {
-            const uint64_t ex = lookback(x.tiles_cell, tile, tot);
+            const uint64_t ex = lookback(x.tiles_cell, tile, tot, &x.st->error);
             if (threadIdx.x == 0) s_excl = ex;
         }
         __syncthreads();

@@ -139,6 +139,9 @@ struct qhg_pop {
 
 namespace {
 
+// k_cellscan runs one block per tile of kCellTile cells
+int cell_tiles(const qhg_pop *p) { return (int)((p->ncells + kCellTile - 1) / kCellTile); }
+
 int grid_for(int64_t n, int max_blocks) {
     int64_t b = (n + kBlock - 1) / kBlock;
     if (b < 1) b = 1;
@@ -264,6 +267,7 @@ int build_prog(qhg_pop *p) {
         const int k = P.action_kind[idx[q]];
         if (k < QHG_ACT_GETOLD || k > QHG_ACT_GENETICS) return fail(QHG_ERR_ARG, "unknown action kind");
         g.ops[q] = k;
+        g.opword |= (uint64_t)k << (4 * q);
         if (k == QHG_ACT_GETOLD) seen_getold = true;
         if (k == QHG_ACT_FERTILITY && !seen_getold) g.load_age = 1;
     }
@@ -280,6 +284,16 @@ int build_prog(qhg_pop *p) {
     p->has_me = has_kind(P, QHG_ACT_MULTIEVAL);
     p->has_cap = has_kind(P, QHG_ACT_NPPCAP);
     g.has_pair = p->has_pair;
+    for (int q = 0; q < g.nops; q++) {
+        switch (g.ops[q]) {
+        case QHG_ACT_GETOLD: g.need_birth = 1; break;
+        case QHG_ACT_FERTILITY: g.need_gender = 1; g.need_lastb = 1; break;
+        case QHG_ACT_VERHULST: g.need_gender = 1; g.need_mate = 1; g.need_count = 1; break;
+        case QHG_ACT_VERHULSTVARK: g.need_gender = 1; g.need_mate = 1; g.need_count = 1; g.need_K = 1; break;
+        case QHG_ACT_RANDOMMOVE: case QHG_ACT_WEIGHTEDMOVE: g.need_move = 1; break;
+        default: break;
+        }
+    }
     if (has_kind(P, QHG_ACT_GENETICS)) {
         if (P.gen_genome_size <= 0 || P.gen_genome_size > 64 * 32) return fail(QHG_ERR_ARG, "Genetics_genome_size out of range");
         if (P.gen_num_crossover < -1 || P.gen_num_crossover > kMaxCross) return fail(QHG_ERR_ARG, "Genetics_num_crossover out of range");
@@ -348,7 +362,7 @@ int recount(qhg_pop *p) {
     HIPCHK(hipMemsetAsync(&p->d_st->tile_ctr_cell, 0, 4, p->stream));
     int rc = launch(p, KB_HIST, k_hist, 2048, x);
     if (rc) return rc;
-    return launch(p, KB_CELLSCAN, k_cellscan, grid_for((p->ncells + 3) / 4, 1024), x, (int32_t)1);
+    return launch(p, KB_CELLSCAN, k_cellscan, cell_tiles(p), x, (int32_t)1);
 }
 
 int read_status_sync(qhg_pop *p, DevStatus *out) {
@@ -510,7 +524,8 @@ int phase_initialize(qhg_pop *p) {
         int rc = launch(p, KB_BUCKET, k_bucket, kBucketBlocks, x);
         if (rc) return rc;
     }
-    int rc = launch(p, KB_PAIR, k_pair, grid_for(p->ncells, p->num_cu * 8), x, (int32_t)p->has_pair);
+    const int pgrid = (int)std::min<int64_t>((p->ncells + kPairBlock - 1) / kPairBlock, (int64_t)p->num_cu * 16);
+    int rc = launch_bs(p, KB_PAIR, k_pair, std::max(pgrid, 1), kPairBlock, x, (int32_t)p->has_pair);
     if (rc) return rc;
     p->initialized = true;
     return 0;
@@ -538,7 +553,7 @@ int phase_finalize(qhg_pop *p) {
     Ctx x = p->ctx();
     int rc = launch(p, KB_NEWBORN, k_newborn, p->num_cu * 4, x, p->prog);
     if (rc) return rc;
-    rc = launch(p, KB_CELLSCAN, k_cellscan, grid_for((p->ncells + 3) / 4, 1024), x, (int32_t)0);
+    rc = launch(p, KB_CELLSCAN, k_cellscan, cell_tiles(p), x, (int32_t)0);
     if (rc) return rc;
     p->acted = false;
     p->initialized = false;
@@ -591,6 +606,14 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, p->device) == hipSuccess && prop.multiProcessorCount > 0)
         p->num_cu = prop.multiProcessorCount;
+    {
+        // the per-cell look-back scan needs all of its blocks resident at once
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cellscan, kBlock, 0) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        if ((int64_t)cell_tiles(p) > (int64_t)per_cu * p->num_cu)
+            return cleanup_fail(fail(QHG_ERR_ARG, "grid too large: more cell tiles than resident blocks"));
+    }
     if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup_fail(fail(QHG_ERR_HIP, "stream"));
     const size_t nc = (size_t)p->ncells;

@@ -97,8 +97,11 @@ struct Ctx {
 struct Prog {
     int32_t nops;
     int32_t ops[kMaxOps];
+    uint64_t opword;         // ops packed 4 bits each (op q in bits 4q..4q+3)
     int32_t load_age;        // Fertility runs before GetOld (reads stored age)
     int32_t has_pair;
+    // fields the program reads (set by the host from the action list)
+    int32_t need_gender, need_birth, need_lastb, need_mate, need_count, need_K, need_move;
     double fert_min, fert_max, fert_inter;
     double b0, d0, theta, Kconst;
     double rmove_p, wmove_p;
@@ -208,23 +211,37 @@ __global__ void __launch_bounds__(kBlock) k_bucket(Ctx x) {
     __syncthreads();
     uint32_t my_fert = 0;
     const int64_t hw = x.st->hw;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < hw;
-         i += (int64_t)gridDim.x * kBlock) {
-        const uint8_t l = x.a.life[i];
-        if (l == 0) continue;
-        x.a.mate[i] = -3;
-        if (l != QHG_LIFE_FERTILE) continue;
-        my_fert++;
-        const int32_t c = x.a.cell[i];
-        const uint8_t g = x.a.gender[i];
-        const uint32_t lo = x.off[c], hi = x.off[c + 1];
-        if (g == 0) {
-            const uint32_t p = atomicAdd(&x.fillF[c], 1u);
-            if (lo + p < hi) x.bucket[lo + p] = (uint32_t)i;
-            else atomicOr(&x.st->error, kErrBucket);
-        } else if (g == 1) {
-            const uint32_t p = atomicAdd(&x.fillM[c], 1u);
-            if (p < hi - lo) x.bucket[hi - 1 - p] = (uint32_t)i;
+    constexpr int I = 4;
+    for (int64_t b0 = (int64_t)blockIdx.x * kBlock * I; b0 < hw; b0 += (int64_t)gridDim.x * kBlock * I) {
+        uint8_t l[I];
+        int32_t c[I];
+        uint8_t g[I];
+#pragma unroll
+        for (int k = 0; k < I; k++) {
+            const int64_t i = b0 + (int64_t)k * kBlock + threadIdx.x;
+            l[k] = (i < hw) ? x.a.life[i] : (uint8_t)0;
+        }
+#pragma unroll
+        for (int k = 0; k < I; k++) {
+            const int64_t i = b0 + (int64_t)k * kBlock + threadIdx.x;
+            if (l[k] == 0) continue;
+            x.a.mate[i] = -3;
+            if (l[k] == QHG_LIFE_FERTILE) { c[k] = x.a.cell[i]; g[k] = x.a.gender[i]; }
+        }
+        uint32_t pos[I], lo[I], hi[I];
+#pragma unroll
+        for (int k = 0; k < I; k++) {
+            if (l[k] != QHG_LIFE_FERTILE || g[k] > 1) continue;
+            lo[k] = x.off[c[k]];
+            hi[k] = x.off[c[k] + 1];
+            pos[k] = atomicAdd(g[k] == 0 ? &x.fillF[c[k]] : &x.fillM[c[k]], 1u);
+        }
+#pragma unroll
+        for (int k = 0; k < I; k++) {
+            if (l[k] != QHG_LIFE_FERTILE || g[k] > 1) continue;
+            const int64_t i = b0 + (int64_t)k * kBlock + threadIdx.x;
+            my_fert++;
+            if (pos[k] < hi[k] - lo[k]) x.bucket[g[k] == 0 ? lo[k] + pos[k] : hi[k] - 1 - pos[k]] = (uint32_t)i;
             else atomicOr(&x.st->error, kErrBucket);
         }
     }
@@ -289,12 +306,60 @@ __device__ __forceinline__ int select_bit(uint64_t v, int k) {
 // ---------------------------------------------------------------------------
 // k_pair: RandomPair::findMates per cell (RandomPair.cpp:183-276).  The
 // smaller sex chooses in id order; each choice takes the k-th still
-// unpaired member of the other sex, k = (int)wrandr(0, left).
-// Also clears the look-back tiles and tile counter of k_act.
+// unpaired member of the other sex, k = (int)wrandr(0, left).  One thread
+// per cell; candidates (slots, i.e. id order) are sorted in LDS when a sex
+// has at most kPairLds of them, in the bucket array otherwise.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_pair(Ctx x, int32_t do_pair) {
-    const int64_t gtid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int64_t gsize = (int64_t)gridDim.x * kBlock;
+constexpr int kPairBlock = 64;
+constexpr int kPairLds = 32;
+
+__device__ __forceinline__ void pair_select(const Ctx &x, uint32_t step, const uint32_t *chooser, uint32_t nch,
+                                            uint32_t *pool, uint32_t np) {
+    uint32_t left = np;
+    if (np <= 64) {
+        uint64_t avail = (np == 64) ? ~0ull : ((1ull << np) - 1ull);
+        for (uint32_t j = 0; j < nch && left > 0; j++) {
+            const uint32_t a = chooser[j];
+            const uint32_t u = draw(x.seed, step, QHG_STREAM_PAIR, (uint64_t)x.a.id[a], 0);
+            const int k = (int)wrandr(0, (double)(int32_t)left, u);
+            const int pos = select_bit(avail, k);
+            avail &= ~(1ull << pos);
+            const uint32_t b = pool[pos];
+            x.a.mate[a] = (int32_t)b;
+            x.a.mate[b] = (int32_t)a;
+            left--;
+        }
+    } else {
+        // mark taken partners with the top bit (slots < 2^31)
+        for (uint32_t j = 0; j < nch && left > 0; j++) {
+            const uint32_t a = chooser[j];
+            const uint32_t u = draw(x.seed, step, QHG_STREAM_PAIR, (uint64_t)x.a.id[a], 0);
+            const int k = (int)wrandr(0, (double)(int32_t)left, u);
+            int found = -1;
+            uint32_t m = 0;
+            for (m = 0; m < np; m++) {
+                if (!(pool[m] & 0x80000000u)) { found++; if (found == k) break; }
+            }
+            const uint32_t b = pool[m];
+            pool[m] = b | 0x80000000u;
+            x.a.mate[a] = (int32_t)b;
+            x.a.mate[b] = (int32_t)a;
+            left--;
+        }
+    }
+}
+
+__device__ __forceinline__ void lds_insertion_sort(uint32_t *v, uint32_t n) {
+    for (uint32_t i = 1; i < n; i++) {
+        const uint32_t k = v[i];
+        int32_t j = (int32_t)i - 1;
+        while (j >= 0 && v[j] > k) { v[j + 1] = v[j]; j--; }
+        v[j + 1] = k;
+    }
+}
+
+__global__ void __launch_bounds__(kPairBlock) k_pair(Ctx x, int32_t do_pair) {
+    __shared__ uint32_t s_buf[kPairBlock * 2 * kPairLds];
     if (!do_pair) return;
     if (blockIdx.x == 0) {
         // fold the per-block fertile counts of k_bucket (diagnostics)
@@ -302,56 +367,34 @@ __global__ void __launch_bounds__(kBlock) k_pair(Ctx x, int32_t do_pair) {
         if (threadIdx.x == 0) s_sum = 0;
         __syncthreads();
         uint32_t v = 0;
-        for (int j = threadIdx.x; j < kBucketBlocks; j += kBlock) v += x.blk_fert[j];
+        for (int j = threadIdx.x; j < kBucketBlocks; j += kPairBlock) v += x.blk_fert[j];
         if (v) atomicAdd(&s_sum, v);
         __syncthreads();
         if (threadIdx.x == 0) x.st->fertile_total += s_sum;
     }
     const uint32_t step = (uint32_t)x.st->step;
-    for (int64_t c = gtid; c < x.ncells; c += gsize) {
+    uint32_t *lf = s_buf + threadIdx.x * 2 * kPairLds;
+    uint32_t *lm = lf + kPairLds;
+    for (int64_t c = (int64_t)blockIdx.x * kPairBlock + threadIdx.x; c < x.ncells;
+         c += (int64_t)gridDim.x * kPairBlock) {
         const uint32_t F = x.fillF[c], M = x.fillM[c];
         if (F == 0 || M == 0) continue;
-        if (F + M > x.off[c + 1] - x.off[c]) { atomicOr(&x.st->error, kErrBucket); continue; }
-        uint32_t *fb = x.bucket + x.off[c];
-        uint32_t *mb = x.bucket + x.off[c + 1] - M;
-        sort_u32(fb, F);
-        sort_u32(mb, M);
-        const bool fem_choose = F <= M;
-        uint32_t *chooser = fem_choose ? fb : mb;
-        uint32_t *pool = fem_choose ? mb : fb;
-        const uint32_t nch = fem_choose ? F : M;
-        const uint32_t np = fem_choose ? M : F;
-        uint32_t left = np;
-        if (np <= 64) {
-            uint64_t avail = (np == 64) ? ~0ull : ((1ull << np) - 1ull);
-            for (uint32_t j = 0; j < nch && left > 0; j++) {
-                const uint32_t a = chooser[j];
-                const uint32_t u = draw(x.seed, step, QHG_STREAM_PAIR, (uint64_t)x.a.id[a], 0);
-                const int k = (int)wrandr(0, (double)(int32_t)left, u);
-                const int pos = select_bit(avail, k);
-                avail &= ~(1ull << pos);
-                const uint32_t b = pool[pos];
-                x.a.mate[a] = (int32_t)b;
-                x.a.mate[b] = (int32_t)a;
-                left--;
-            }
+        const uint32_t lo = x.off[c], hi = x.off[c + 1];
+        if (F + M > hi - lo) { atomicOr(&x.st->error, kErrBucket); continue; }
+        uint32_t *fb = x.bucket + lo;
+        uint32_t *mb = x.bucket + hi - M;
+        if (F <= kPairLds && M <= kPairLds) {
+            for (uint32_t j = 0; j < F; j++) lf[j] = fb[j];
+            for (uint32_t j = 0; j < M; j++) lm[j] = mb[j];
+            lds_insertion_sort(lf, F);
+            lds_insertion_sort(lm, M);
+            if (F <= M) pair_select(x, step, lf, F, lm, M);
+            else pair_select(x, step, lm, M, lf, F);
         } else {
-            // large cell: mark taken partners with the top bit (slots < 2^31)
-            for (uint32_t j = 0; j < nch && left > 0; j++) {
-                const uint32_t a = chooser[j];
-                const uint32_t u = draw(x.seed, step, QHG_STREAM_PAIR, (uint64_t)x.a.id[a], 0);
-                const int k = (int)wrandr(0, (double)(int32_t)left, u);
-                int found = -1;
-                uint32_t m = 0;
-                for (m = 0; m < np; m++) {
-                    if (!(pool[m] & 0x80000000u)) { found++; if (found == k) break; }
-                }
-                const uint32_t b = pool[m];
-                pool[m] = b | 0x80000000u;
-                x.a.mate[a] = (int32_t)b;
-                x.a.mate[b] = (int32_t)a;
-                left--;
-            }
+            sort_u32(fb, F);
+            sort_u32(mb, M);
+            if (F <= M) pair_select(x, step, fb, F, mb, M);
+            else pair_select(x, step, mb, M, fb, F);
         }
     }
 }
@@ -363,119 +406,113 @@ struct StepView {
     uint32_t *cnt_next;
 };
 
-// One agent's program for this step.  Returns 1 and the birth record fields
-// when the agent registers a birth.
-__device__ __forceinline__ uint32_t act_one(const Ctx &x, const Prog &p, const StepView &v, int64_t i,
-                                            int32_t &bcell, int32_t &bfather, uint32_t &deaths,
-                                            uint32_t &moves) {
-    uint8_t life = x.a.life[i];
-    if (life == 0) return 0;
-    const uint8_t life0 = life;
-    const int32_t c = x.a.cell[i];
-    const int64_t id = x.a.id[i];
+// Per-item state of one tile, kept in registers (loops over k unroll).
+struct ActItem {
+    uint8_t life, life0, g;
+    int32_t c, mate, nn;
+    int64_t id;
+    double birth, lastb, age, K;
+    uint32_t N;
+    bool age_dirty;
+    uint32_t bflag;
+    int32_t npend;            // registered moves (<= 2), resolved after the program
+    int32_t pk0, pk1;         // their action kinds
+    uint32_t pu0, pu1;        // their target draws
+};
+
+// The action program of one agent (compute only; every field it reads was
+// loaded by the caller).  A dead agent stops.  Moves are only registered
+// here (RandomMove / WeightedMove draw their decision now) and resolved
+// afterwards, as performMoves applies them after all actions.
+__device__ __forceinline__ void act_program(const Ctx &x, const Prog &p, const StepView &v, ActItem &a,
+                                            uint32_t &deaths) {
     const uint32_t step = v.step;
     const double t = v.t;
-    const int32_t R = x.conn + 1;
-    uint8_t g = 0;
-    bool have_g = false;
-    double age = p.load_age ? x.a.age[i] : 0.0;
-    bool age_dirty = false;
-    int32_t dest = -1;
-    uint32_t bflag = 0;
-    for (int q = 0; q < p.nops; q++) {
-        const int op = p.ops[q];
+    uint64_t word = p.opword;
+    for (int q = 0; q < p.nops; q++, word >>= 4) {
+        const int op = (int)(word & 15u);
         if (op == QHG_ACT_GETOLD) {
             // GetOld::execute, QHG4/actions/GetOld.cpp:37-48
-            age = t - x.a.birth[i];
-            age_dirty = true;
+            a.age = t - a.birth;
+            a.age_dirty = true;
         } else if (op == QHG_ACT_FERTILITY) {
             // Fertility::execute, QHG4/actions/Fertility.cpp:49-74
-            if (!have_g) { g = x.a.gender[i]; have_g = true; }
-            if (g == 0) {
-                const double lb = x.a.lastb[i];
-                life = (age > p.fert_min && age < p.fert_max && (t - lb) > p.fert_inter)
-                           ? QHG_LIFE_FERTILE : QHG_LIFE_ALIVE;
+            if (a.g == 0) {
+                a.life = (a.age > p.fert_min && a.age < p.fert_max && (t - a.lastb) > p.fert_inter)
+                             ? QHG_LIFE_FERTILE : QHG_LIFE_ALIVE;
             } else {
-                life = (age > p.fert_min) ? QHG_LIFE_FERTILE : QHG_LIFE_ALIVE;
+                a.life = (a.age > p.fert_min) ? QHG_LIFE_FERTILE : QHG_LIFE_ALIVE;
             }
         } else if (op == QHG_ACT_VERHULST || op == QHG_ACT_VERHULSTVARK) {
             // LinearBirth::initialize/execute then LinearDeath (Verhulst.cpp:101-115;
             // LinearBirth.cpp:90-168; LinearDeath.cpp:92-153)
-            const double N = (double)v.cnt_prev[c];
+            const double N = (double)a.N;
             double b, d;
             if (op == QHG_ACT_VERHULST) {
                 b = p.b0 + (p.theta - p.b0) * (N / p.Kconst);
                 d = p.d0 + (p.theta - p.d0) * (N / p.Kconst);
             } else {
-                const double K = x.K[c];
-                b = (K <= 0) ? 0.0 : p.b0 + (p.theta - p.b0) * (N / K);
-                d = (K <= 0) ? 1.0 : p.d0 + (p.theta - p.d0) * (N / K);
+                b = (a.K <= 0) ? 0.0 : p.b0 + (p.theta - p.b0) * (N / a.K);
+                d = (a.K <= 0) ? 1.0 : p.d0 + (p.theta - p.d0) * (N / a.K);
             }
             if (b > 0) {
-                if (!have_g) { g = x.a.gender[i]; have_g = true; }
-                if (g == 0) {
-                    const int32_t m = x.a.mate[i];
-                    if (m >= 0) {
-                        const double r = wrandd(draw(x.seed, step, QHG_STREAM_LBIRTH, (uint64_t)id, 0));
-                        if (r < b) { bflag = 1; bcell = c; bfather = m; }
-                    }
+                if (a.g == 0 && a.mate >= 0) {
+                    const double r = wrandd(draw(x.seed, step, QHG_STREAM_LBIRTH, (uint64_t)a.id, 0));
+                    if (r < b) a.bflag = 1;
                 }
             } else if (b < 0) {
-                const double r = wrandd(draw(x.seed, step, QHG_STREAM_LBIRTH, (uint64_t)id, 0));
-                if (r < -b) life = 0;
+                const double r = wrandd(draw(x.seed, step, QHG_STREAM_LBIRTH, (uint64_t)a.id, 0));
+                if (r < -b) a.life = 0;
             }
-            if (life != 0) {
-                const double r = wrandd(draw(x.seed, step, QHG_STREAM_LDEATH, (uint64_t)id, 0));
-                if (r < d) life = 0;
+            if (a.life != 0) {
+                const double r = wrandd(draw(x.seed, step, QHG_STREAM_LDEATH, (uint64_t)a.id, 0));
+                if (r < d) a.life = 0;
             }
-            if (life == 0) { deaths++; break; }
-        } else if (op == QHG_ACT_RANDOMMOVE) {
-            // RandomMove::execute, QHG4/actions/RandomMove.cpp:65-100
+            if (a.life == 0) { deaths++; break; }
+        } else if (op == QHG_ACT_RANDOMMOVE || op == QHG_ACT_WEIGHTEDMOVE) {
+            // RandomMove::execute (RandomMove.cpp:65-100) / WeightedMove::execute
+            // (WeightedMove.cpp:45-106): decision draw now, target below
+            const bool rm = op == QHG_ACT_RANDOMMOVE;
             uint32_t u[4];
-            draw4(x.seed, step, QHG_STREAM_RMOVE, (uint64_t)id, 0, u);
-            if (wrandd(u[0]) < p.rmove_p) {
-                const int nn = x.nn[c];
-                const int slot = (int)(wrandd(u[1]) * (nn + 1));
-                if (slot > 0) {
-                    const int32_t nb = x.nbr[(int64_t)c * x.conn + slot - 1];
-                    if (nb >= 0) { if (dest < 0) moves++; dest = nb; }
-                }
-            }
-        } else if (op == QHG_ACT_WEIGHTEDMOVE) {
-            // WeightedMove::execute, QHG4/actions/WeightedMove.cpp:45-106
-            uint32_t u[4];
-            draw4(x.seed, step, QHG_STREAM_WMOVE, (uint64_t)id, 0, u);
-            if (wrandd(u[0]) < p.wmove_p) {
-                const int nn = x.nn[c];
-                const double *w = x.W + (int64_t)c * R;
-                const double wl = w[nn];
-                int idx = -1;
-                if (w[0] == wl) {
-                    idx = (int)wrandi(0, (uint32_t)nn + 1, 1, u[1]);
-                } else {
-                    const double r2 = wrandd(u[1]) * wl;
-                    for (int q2 = 0; q2 < nn + 1; q2++) {
-                        if (r2 < w[q2]) { idx = q2; break; }
-                    }
-                }
-                if (idx > 0) {
-                    const int32_t nb = x.nbr[(int64_t)c * x.conn + idx - 1];
-                    if (nb >= 0 && (x.ice == nullptr || !x.ice[nb])) {
-                        if (dest < 0) moves++;
-                        dest = nb;
-                    }
-                }
+            draw4(x.seed, step, rm ? QHG_STREAM_RMOVE : QHG_STREAM_WMOVE, (uint64_t)a.id, 0, u);
+            if (wrandd(u[0]) < (rm ? p.rmove_p : p.wmove_p)) {
+                if (a.npend == 0) { a.pk0 = op; a.pu0 = u[1]; a.npend = 1; }
+                else if (a.npend == 1) { a.pk1 = op; a.pu1 = u[1]; a.npend = 2; }
             }
         }
         // RandomPair, MultiEvaluator, NPPCapacity, Genetics: no execute
     }
-    if (life != life0) x.a.life[i] = life;
-    if (age_dirty) x.a.age[i] = age;
-    if (bflag) x.a.nbabies[i] += 1;
-    int32_t cfin = c;
-    if (dest >= 0) { x.a.cell[i] = dest; cfin = dest; }
-    if (life != 0) atomicAdd(&v.cnt_next[cfin], 1u);
-    return bflag;
+}
+
+// Target cell of a registered move, -1 when it stays (slot 0 or a -1 hole).
+// w (conn+1 cumulative weights) and nbr (conn neighbours) are register rows,
+// indexed with compile-time indices only.
+__device__ __forceinline__ int32_t move_target(const Ctx &x, int kind, uint32_t u, int nn, const double (&w)[7],
+                                               const int32_t (&nbr)[6]) {
+    int idx = -1;
+    if (kind == QHG_ACT_RANDOMMOVE) {
+        idx = (int)(wrandd(u) * (nn + 1));
+    } else {
+        double wl = w[0];
+#pragma unroll
+        for (int q = 1; q < 7; q++) if (q == nn) wl = w[q];
+        if (w[0] == wl) {
+            idx = (int)wrandi(0, (uint32_t)nn + 1, 1, u);
+        } else {
+            const double r2 = wrandd(u) * wl;
+#pragma unroll
+            for (int q = 6; q >= 0; q--) {
+                if (q <= nn && r2 < w[q]) idx = q;   // first q with r2 < w[q]
+            }
+        }
+    }
+    if (idx <= 0) return -1;
+    int32_t nb = -1;
+#pragma unroll
+    for (int q = 0; q < 6; q++) if (q == idx - 1) nb = nbr[q];
+    if (nb < 0) return -1;
+    if (kind == QHG_ACT_WEIGHTEDMOVE && x.ice != nullptr && x.ice[nb]) return -1;
+    return nb;
 }
 
 // exclusive ranks of flags over a tile laid out as kActItems sub-tiles of
@@ -529,20 +566,92 @@ __global__ void __launch_bounds__(kBlock) k_act(Ctx x, Prog p) {
     v.cnt_prev = x.cnt[v.step & 1];
     v.cnt_next = x.cnt[(v.step + 1) & 1];
     const int64_t ntiles = (hw + kActTile - 1) / kActTile;
+    const int R = x.conn + 1;
 
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int64_t base = tile * kActTile;
         uint32_t my_deaths = 0, my_moves = 0;
-        uint32_t bflag[kActItems];
-        int32_t bcell[kActItems], bfather[kActItems];
         if (threadIdx.x == 0) { s_red[0] = 0; s_red[1] = 0; }
+        ActItem a[kActItems];
+        // stage A: life of every slot of the tile
 #pragma unroll
         for (int k = 0; k < kActItems; k++) {
             const int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
-            bflag[k] = 0;
-            bcell[k] = 0;
-            bfather[k] = -1;
-            if (i < hw) bflag[k] = act_one(x, p, v, i, bcell[k], bfather[k], my_deaths, my_moves);
+            a[k].life = (i < hw) ? x.a.life[i] : (uint8_t)0;
+            a[k].life0 = a[k].life;
+            a[k].age_dirty = false;
+            a[k].bflag = 0;
+            a[k].npend = 0;
+            a[k].g = 0;
+            a[k].mate = -1;
+            a[k].nn = 0;
+            a[k].N = 0;
+            a[k].K = 0;
+            a[k].birth = a[k].lastb = a[k].age = 0;
+        }
+        // stage B: per-agent fields of the live slots (independent loads)
+#pragma unroll
+        for (int k = 0; k < kActItems; k++) {
+            if (!a[k].life) continue;
+            const int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
+            a[k].c = x.a.cell[i];
+            a[k].id = x.a.id[i];
+            if (p.need_gender) a[k].g = x.a.gender[i];
+            if (p.need_birth) a[k].birth = x.a.birth[i];
+            if (p.need_lastb) a[k].lastb = x.a.lastb[i];
+            if (p.need_mate) a[k].mate = x.a.mate[i];
+            if (p.load_age) a[k].age = x.a.age[i];
+        }
+        // stage C: per-cell fields
+#pragma unroll
+        for (int k = 0; k < kActItems; k++) {
+            if (!a[k].life) continue;
+            if (p.need_count) a[k].N = v.cnt_prev[a[k].c];
+            if (p.need_K) a[k].K = x.K[a[k].c];
+            if (p.need_move) a[k].nn = x.nn[a[k].c];
+        }
+        // stage D: the action program (compute only)
+#pragma unroll
+        for (int k = 0; k < kActItems; k++) {
+            if (a[k].life) act_program(x, p, v, a[k], my_deaths);
+        }
+        // stage E: resolve registered moves (weight and neighbour rows);
+        // the last registered move that lands wins
+        int32_t dest[kActItems];
+#pragma unroll
+        for (int k = 0; k < kActItems; k++) {
+            dest[k] = -1;
+            if (a[k].npend == 0) continue;
+            const int32_t c = a[k].c;
+            int32_t nbr[6];
+            double w[7];
+#pragma unroll
+            for (int q = 0; q < 6; q++) nbr[q] = (q < x.conn) ? x.nbr[(int64_t)c * x.conn + q] : -1;
+            const bool need_w = a[k].pk0 == QHG_ACT_WEIGHTEDMOVE ||
+                                (a[k].npend > 1 && a[k].pk1 == QHG_ACT_WEIGHTEDMOVE);
+#pragma unroll
+            for (int q = 0; q < 7; q++) w[q] = (need_w && q < R) ? x.W[(int64_t)c * R + q] : 0.0;
+            int32_t nb = move_target(x, a[k].pk0, a[k].pu0, a[k].nn, w, nbr);
+            if (nb >= 0) dest[k] = nb;
+            if (a[k].npend > 1) {
+                nb = move_target(x, a[k].pk1, a[k].pu1, a[k].nn, w, nbr);
+                if (nb >= 0) dest[k] = nb;
+            }
+            if (dest[k] >= 0) my_moves++;
+        }
+        // stage F: write back
+        uint32_t bflag[kActItems];
+#pragma unroll
+        for (int k = 0; k < kActItems; k++) {
+            bflag[k] = a[k].bflag;
+            if (!a[k].life0) continue;
+            const int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
+            if (a[k].life != a[k].life0) x.a.life[i] = a[k].life;
+            if (a[k].age_dirty) x.a.age[i] = a[k].age;
+            if (a[k].bflag) x.a.nbabies[i] += 1;
+            int32_t cfin = a[k].c;
+            if (dest[k] >= 0) { x.a.cell[i] = dest[k]; cfin = dest[k]; }
+            if (a[k].life != 0) atomicAdd(&v.cnt_next[cfin], 1u);
         }
         uint32_t rank[kActItems], tile_total;
         tile_ranks(bflag, rank, s_cnt, tile_total);   // contains a barrier
@@ -552,8 +661,8 @@ __global__ void __launch_bounds__(kBlock) k_act(Ctx x, Prog p) {
             if (bflag[k]) {
                 const int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
                 rec[3 * rank[k] + 0] = (int32_t)i;
-                rec[3 * rank[k] + 1] = bfather[k];
-                rec[3 * rank[k] + 2] = bcell[k];
+                rec[3 * rank[k] + 1] = a[k].mate;
+                rec[3 * rank[k] + 2] = a[k].c;
             }
         }
         if (my_deaths) atomicAdd(&s_red[0], my_deaths);
@@ -772,23 +881,19 @@ __global__ void __launch_bounds__(kBlock) k_cellscan(Ctx x, int32_t mode) {
     __shared__ uint64_t s_excl;
     const int64_t nc = x.ncells;
     constexpr int per = kCellTile / kBlock;
-    bool first = true;
-    while (true) {
-        // The block holding the last tile advances st->step.  Thread 0 reads
-        // the step before its acq_rel tile grab, so every block that holds a
-        // tile saw the step of this launch.
-        if (threadIdx.x == 0) {
-            if (first) s_step = (uint32_t)__hip_atomic_load(&x.st->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_tile = __hip_atomic_fetch_add(&x.st->tile_ctr_cell, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        }
+    // One tile per block; the grid is exactly the number of tiles and small
+    // enough that every block is resident, so the look-back always drains.
+    // The block of the last tile advances st->step only after every other
+    // block has published its aggregate, i.e. after every block read the step.
+    {
+        if (threadIdx.x == 0) s_step = (uint32_t)x.st->step;
         __syncthreads();
-        first = false;
         const uint32_t step = s_step;
         const uint32_t *src = mode ? x.cnt[step & 1] : x.cnt[(step + 1) & 1];
         uint32_t *clr = mode ? x.cnt[(step + 1) & 1] : x.cnt[step & 1];
-        const uint32_t tile = s_tile;
+        const uint32_t tile = blockIdx.x;
         const int64_t base = (int64_t)tile * kCellTile;
-        if (base >= nc) break;
+        if (base >= nc) return;
         uint32_t v[per];
         uint32_t sum = 0;
 #pragma unroll

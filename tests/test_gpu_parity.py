@@ -89,3 +89,26 @@ def test_growth_beyond_capacity():
     print("case: capacity growth from a tiny initial capacity", flush=True)
     sc = tut_verhulst(scenarios.c1(seed=13, width=64, height=64, n_agents=3000))
     _lockstep(sc, 120, 30, use_run=True, capacity=1024)
+
+
+def test_crowded_cells_large_pairing_paths():
+    """A few cells with hundreds of fertile agents of each sex: exercises the
+    in-bucket sort (> 32 per sex) and the linear k-th-unpaired selection
+    (> 64 candidates) of the pairing kernel."""
+    print("case: crowded cells, 6x6 lattice, 3000 agents in 6 cells", flush=True)
+    sc = tut_verhulst(scenarios.c1(seed=17, width=6, height=6, n_agents=3000))
+    rng = np.random.default_rng(17)
+    sc["agents"]["cell"] = rng.choice([0, 7, 14, 21, 28, 35], 3000).astype(np.int32)
+    sc["params"].verh_K = 2000.0
+    _lockstep(sc, 30, 5)
+
+
+def test_empty_and_single_agent():
+    print("case: empty population and a single agent", flush=True)
+    sc = scenarios.c1(seed=1, width=8, height=8, n_agents=1)
+    _lockstep(sc, 10, 5)
+    sc = scenarios.c1(seed=1, width=8, height=8, n_agents=0)
+    gpu = GpuPopulation.from_scenario(sc)
+    gpu.preLoop()
+    gpu.run(5)
+    assert gpu.stats()["num_alive"] == 0 and gpu.agents()["id"].size == 0

@@ -745,6 +745,7 @@ int qhg_gpu_upload_agents(qhg_pop *p, const qhg_agents *a) {
     }
     const AgentPtrs &d = p->bufs[p->cur].p;
     hipStream_t s = p->stream;
+    if (n > 0) {
     HIPCHK(hipMemcpyAsync(d.id, id.data(), n * 8, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(d.cell, cell.data(), n * 4, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(d.life, life.data(), n, hipMemcpyHostToDevice, s));
@@ -755,6 +756,7 @@ int qhg_gpu_upload_agents(qhg_pop *p, const qhg_agents *a) {
     HIPCHK(hipMemcpyAsync(d.mate, mate.data(), n * 4, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(d.nbabies, nbab.data(), n * 4, hipMemcpyHostToDevice, s));
     if (p->gwords) HIPCHK(hipMemcpyAsync(d.genome, genome.data(), (size_t)n * p->gwords * 8, hipMemcpyHostToDevice, s));
+    }
     DevStatus st;
     HIPCHK(hipMemcpyAsync(&st, p->d_st, sizeof(st), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));

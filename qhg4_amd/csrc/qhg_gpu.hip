@@ -40,9 +40,9 @@ int fail(int code, const std::string &msg) {
     } while (0)
 
 enum KernelId { KB_BUCKET, KB_PAIR, KB_ACT, KB_NEWBORN, KB_CELLSCAN, KB_HIST, KB_CAPACITY,
-                KB_EVAL, KB_COMPACT, KB_TILESCAN, KB_COUNT };
+                KB_EVAL, KB_COMPACT, KB_COUNT };
 const char *kKernelNames[KB_COUNT] = {"k_bucket", "k_pair", "k_act", "k_newborn", "k_cellscan",
-                                      "k_hist", "k_capacity", "k_eval", "k_compact", "k_tilescan"};
+                                      "k_hist", "k_capacity", "k_eval", "k_compact"};
 
 constexpr int kRing = kStatusRing;
 
@@ -73,7 +73,7 @@ struct qhg_pop {
     uint32_t *d_bucket = nullptr;
     int32_t *d_brec = nullptr;
     int32_t *d_newidx = nullptr;
-    uint32_t *d_tile_cnt = nullptr, *d_tile_off = nullptr, *d_blk_fert = nullptr;
+    uint32_t *d_tile_cnt = nullptr, *d_tile_off = nullptr, *d_super = nullptr, *d_blk_fert = nullptr;
     uint64_t *d_tiles_cell = nullptr;
     int64_t *d_hw_next = nullptr;
     DevStatus *d_st = nullptr;
@@ -126,7 +126,7 @@ struct qhg_pop {
         x.bucket = d_bucket;
         x.brec = d_brec;
         x.tile_cnt = d_tile_cnt;
-        x.tile_off = d_tile_off;
+        x.super_cnt = d_super;
         x.blk_fert = d_blk_fert;
         x.tiles_cell = d_tiles_cell;
         x.ntiles_act_max = bufs[cur].cap / kActTile + 2;
@@ -225,12 +225,15 @@ int alloc_slot_scratch(qhg_pop *p, int64_t cap) {
     if (p->d_newidx) (void)hipFree(p->d_newidx);
     if (p->d_tile_cnt) (void)hipFree(p->d_tile_cnt);
     if (p->d_tile_off) (void)hipFree(p->d_tile_off);
+    if (p->d_super) (void)hipFree(p->d_super);
     const size_t nt = (size_t)(cap / kActTile + 2);
     HIPCHK(hipMalloc(&p->d_bucket, (size_t)cap * 4));
     HIPCHK(hipMalloc(&p->d_brec, ((size_t)cap + kActTile) * 12));
     HIPCHK(hipMalloc(&p->d_newidx, (size_t)cap * 4));
     HIPCHK(hipMalloc(&p->d_tile_cnt, nt * 12));
     HIPCHK(hipMalloc(&p->d_tile_off, nt * 4));
+    HIPCHK(hipMalloc(&p->d_super, (nt / kSuperTiles + 2) * 12));
+    HIPCHK(hipMemsetAsync(p->d_super, 0, (nt / kSuperTiles + 2) * 12, p->stream));
     return 0;
 }
 
@@ -359,7 +362,6 @@ int recount(qhg_pop *p) {
     HIPCHK(hipMemsetAsync(p->d_cnt[1], 0, (size_t)p->ncells * 4, p->stream));
     const size_t ntc = (size_t)(p->ncells / kCellTile + 2);
     HIPCHK(hipMemsetAsync(p->d_tiles_cell, 0, ntc * 8, p->stream));
-    HIPCHK(hipMemsetAsync(&p->d_st->tile_ctr_cell, 0, 4, p->stream));
     int rc = launch(p, KB_HIST, k_hist, 2048, x);
     if (rc) return rc;
     return launch(p, KB_CELLSCAN, k_cellscan, cell_tiles(p), x, (int32_t)1);
@@ -539,8 +541,6 @@ int phase_act(qhg_pop *p) {
     Ctx x = p->ctx();
     int rc = launch(p, KB_ACT, k_act, p->num_cu * 4, x, p->prog);
     if (rc) return rc;
-    rc = launch_bs(p, KB_TILESCAN, k_tilescan, 1, 1024, x);
-    if (rc) return rc;
     p->acted = true;
     return 0;
 }
@@ -676,7 +676,7 @@ int qhg_gpu_destroy(qhg_pop *p) {
     free_bufs(p->bufs[1]);
     void *ptrs[] = {p->d_nbr, p->d_nn, p->d_K, p->d_W, p->d_tmp, p->d_ice, p->d_cnt[0], p->d_cnt[1],
                     p->d_off, p->d_fillF, p->d_fillM, p->d_bucket, p->d_brec, p->d_newidx,
-                    p->d_tile_cnt, p->d_tile_off, p->d_blk_fert, p->d_tiles_cell, p->d_hw_next, p->d_st,
+                    p->d_tile_cnt, p->d_tile_off, p->d_super, p->d_blk_fert, p->d_tiles_cell, p->d_hw_next, p->d_st,
                     p->d_coastal};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);

@@ -38,6 +38,7 @@ constexpr int kMaxOps = QHG_MAX_ACTIONS;
 constexpr int kMaxCross = 32;        // device limit on Genetics_num_crossover
 constexpr int kStatusRing = 8;       // host-mapped status snapshots, one per step
 constexpr int kBucketBlocks = 512;   // fixed grid of k_bucket (per-block fertile counts)
+constexpr int kSuperTiles = 64;      // act tiles per supertile (birth offsets without a scan kernel)
 
 struct DevStatus {
     int64_t hw, n_alive, max_id, step;
@@ -85,7 +86,7 @@ struct Ctx {
     uint32_t *bucket;      // cap
     int32_t *brec;         // 3 * (cap + kActTile): birth records staged per act tile
     uint32_t *tile_cnt;    // 3 per act tile: births, deaths, moves
-    uint32_t *tile_off;    // births before each act tile (exclusive prefix)
+    uint32_t *super_cnt;   // 3 per group of kSuperTiles act tiles (sums of tile_cnt)
     uint32_t *blk_fert;    // fertile agents bucketed per k_bucket block
     uint64_t *tiles_cell;
     int64_t ntiles_act_max;
@@ -349,6 +350,25 @@ __device__ __forceinline__ void pair_select(const Ctx &x, uint32_t step, const u
     }
 }
 
+// pair_select for a cell staged in LDS: chooser ids preloaded, pool <= 32
+__device__ __forceinline__ void pair_select_lds(const Ctx &x, uint32_t step, const uint32_t *chooser,
+                                                const uint64_t *ids, uint32_t nch, const uint32_t *pool,
+                                                uint32_t np) {
+    uint32_t left = np;
+    uint64_t avail = (np == 64) ? ~0ull : ((1ull << np) - 1ull);
+    for (uint32_t j = 0; j < nch && left > 0; j++) {
+        const uint32_t a = chooser[j];
+        const uint32_t u = draw(x.seed, step, QHG_STREAM_PAIR, ids[j], 0);
+        const int k = (int)wrandr(0, (double)(int32_t)left, u);
+        const int pos = select_bit(avail, k);
+        avail &= ~(1ull << pos);
+        const uint32_t b = pool[pos];
+        x.a.mate[a] = (int32_t)b;
+        x.a.mate[b] = (int32_t)a;
+        left--;
+    }
+}
+
 __device__ __forceinline__ void lds_insertion_sort(uint32_t *v, uint32_t n) {
     for (uint32_t i = 1; i < n; i++) {
         const uint32_t k = v[i];
@@ -360,6 +380,13 @@ __device__ __forceinline__ void lds_insertion_sort(uint32_t *v, uint32_t n) {
 
 __global__ void __launch_bounds__(kPairBlock) k_pair(Ctx x, int32_t do_pair) {
     __shared__ uint32_t s_buf[kPairBlock * 2 * kPairLds];
+    __shared__ uint64_t s_ids[kPairBlock * kPairLds];
+    {
+        const int64_t ntiles = (x.st->hw + kActTile - 1) / kActTile;
+        const int64_t ns = 3 * ((ntiles + kSuperTiles - 1) / kSuperTiles);
+        for (int64_t j = (int64_t)blockIdx.x * kPairBlock + threadIdx.x; j < ns; j += (int64_t)gridDim.x * kPairBlock)
+            x.super_cnt[j] = 0;
+    }
     if (!do_pair) return;
     if (blockIdx.x == 0) {
         // fold the per-block fertile counts of k_bucket (diagnostics)
@@ -388,8 +415,14 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(Ctx x, int32_t do_pair) {
             for (uint32_t j = 0; j < M; j++) lm[j] = mb[j];
             lds_insertion_sort(lf, F);
             lds_insertion_sort(lm, M);
-            if (F <= M) pair_select(x, step, lf, F, lm, M);
-            else pair_select(x, step, lm, M, lf, F);
+            const bool fem = F <= M;
+            const uint32_t *ch = fem ? lf : lm;
+            const uint32_t nch = fem ? F : M;
+            uint64_t *ids = s_ids + threadIdx.x * kPairLds;
+#pragma unroll
+            for (int j = 0; j < kPairLds; j++)
+                if ((uint32_t)j < nch) ids[j] = (uint64_t)x.a.id[ch[j]];
+            pair_select_lds(x, step, ch, ids, nch, fem ? lm : lf, fem ? M : F);
         } else {
             sort_u32(fb, F);
             sort_u32(mb, M);
@@ -553,8 +586,8 @@ __device__ __forceinline__ void tile_ranks(const uint32_t (&flag)[kActItems], ui
 // actions of this step see the pre-move cell, births are registered in it.
 // Tiles of kActTile slots are assigned statically (tile = block, block +
 // grid, ...).  A tile stages its birth records in slot (= mother id) order
-// in its own region of brec and writes its birth/death/move counts;
-// k_tilescan turns the counts into global ranks.
+// in its own region of brec and writes its birth/death/move counts, also
+// summed per supertile; k_newborn derives the global birth ranks from them.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_act(Ctx x, Prog p) {
     __shared__ uint32_t s_cnt[kActItems * (kBlock / 64)];
@@ -649,9 +682,15 @@ __global__ void __launch_bounds__(kBlock) k_act(Ctx x, Prog p) {
             if (a[k].life != a[k].life0) x.a.life[i] = a[k].life;
             if (a[k].age_dirty) x.a.age[i] = a[k].age;
             if (a[k].bflag) x.a.nbabies[i] += 1;
-            int32_t cfin = a[k].c;
-            if (dest[k] >= 0) { x.a.cell[i] = dest[k]; cfin = dest[k]; }
-            if (a[k].life != 0) atomicAdd(&v.cnt_next[cfin], 1u);
+            // the next step's counts start as a copy of this step's: only
+            // deaths and moves of survivors change them
+            if (dest[k] >= 0) x.a.cell[i] = dest[k];
+            if (a[k].life == 0) {
+                atomicSub(&v.cnt_next[a[k].c], 1u);
+            } else if (dest[k] >= 0) {
+                atomicSub(&v.cnt_next[a[k].c], 1u);
+                atomicAdd(&v.cnt_next[dest[k]], 1u);
+            }
         }
         uint32_t rank[kActItems], tile_total;
         tile_ranks(bflag, rank, s_cnt, tile_total);   // contains a barrier
@@ -672,61 +711,12 @@ __global__ void __launch_bounds__(kBlock) k_act(Ctx x, Prog p) {
             x.tile_cnt[3 * tile + 0] = tile_total;
             x.tile_cnt[3 * tile + 1] = s_red[0];
             x.tile_cnt[3 * tile + 2] = s_red[1];
+            uint32_t *sc = x.super_cnt + 3 * (tile / kSuperTiles);
+            if (tile_total) atomicAdd(&sc[0], tile_total);
+            if (s_red[0]) atomicAdd(&sc[1], s_red[0]);
+            if (s_red[1]) atomicAdd(&sc[2], s_red[1]);
         }
         __syncthreads();
-    }
-}
-
-// ---------------------------------------------------------------------------
-// k_tilescan (one block): exclusive prefix of the per-tile birth counts and
-// the step's birth/death/move totals.  Flags and drops the births when they
-// do not fit the capacity.
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) k_tilescan(Ctx x) {
-    __shared__ uint32_t s_w[1024 / 64];
-    __shared__ uint64_t s_tot[3];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t hw = x.st->hw;
-    const int64_t ntiles = (hw + kActTile - 1) / kActTile;
-    if (threadIdx.x < 3) s_tot[threadIdx.x] = 0;
-    __syncthreads();
-    uint64_t run = 0, dsum = 0, msum = 0;
-    for (int64_t b0 = 0; b0 < ntiles; b0 += 1024) {
-        const int64_t t = b0 + threadIdx.x;
-        const uint32_t nb = (t < ntiles) ? x.tile_cnt[3 * t] : 0u;
-        if (t < ntiles) { dsum += x.tile_cnt[3 * t + 1]; msum += x.tile_cnt[3 * t + 2]; }
-        uint32_t y = nb;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t o = __shfl_up(y, d, 64);
-            if (lane >= d) y += o;
-        }
-        if (lane == 63) s_w[wave] = y;
-        __syncthreads();
-        uint32_t wpre = 0, tot = 0;
-        for (int w = 0; w < 1024 / 64; w++) {
-            const uint32_t sv = s_w[w];
-            if (w < wave) wpre += sv;
-            tot += sv;
-        }
-        if (t < ntiles) x.tile_off[t] = (uint32_t)(run + wpre + y - nb);
-        run += tot;
-        __syncthreads();
-    }
-    if (dsum) atomicAdd((unsigned long long *)&s_tot[1], (unsigned long long)dsum);
-    if (msum) atomicAdd((unsigned long long *)&s_tot[2], (unsigned long long)msum);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        DevStatus *st = x.st;
-        st->deaths_cur = (int64_t)s_tot[1];
-        st->moves_cur = (int64_t)s_tot[2];
-        if (hw + (int64_t)run > x.cap) {
-            // no slot for the births: drop them and flag the step as failed
-            st->error |= kErrCapacity;
-            st->births_cur = 0;
-        } else {
-            st->births_cur = (int64_t)run;
-        }
     }
 }
 
@@ -774,25 +764,62 @@ __device__ void crossover_strand(uint64_t *out, const uint64_t *par, int nb, con
 // k_newborn: one thread per birth record (rank r): slot hw + r, id
 // max_id + 1 + r.  Also clears the look-back tiles / counter of k_cellscan.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t block_sum_u64(uint64_t v, uint64_t *s_red) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    v = wave_sum_u64(v);
+    __syncthreads();
+    if (lane == 0) s_red[wave] = v;
+    __syncthreads();
+    uint64_t t = 0;
+    for (int w = 0; w < kBlock / 64; w++) t += s_red[w];
+    return t;
+}
+
 __global__ void __launch_bounds__(kBlock) k_newborn(Ctx x, Prog p) {
+    __shared__ uint64_t s_red[kBlock / 64];
     const int64_t gtid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t gsize = (int64_t)gridDim.x * kBlock;
     {
         const int64_t nt = (x.ncells + kCellTile - 1) / kCellTile + 1;
         for (int64_t j = gtid; j < nt; j += gsize) x.tiles_cell[j] = 0;
-        if (gtid == 0) x.st->tile_ctr_cell = 0;
     }
-    const int64_t nbt = x.st->births_cur;
     const int64_t hw = x.st->hw;
     const int64_t max_id = x.st->max_id;
     const uint32_t step = (uint32_t)x.st->step;
     const double t = x.st->t;
     uint32_t *cnt_next = x.cnt[(step + 1) & 1];
-    if (nbt == 0) return;
     const int64_t ntiles = (hw + kActTile - 1) / kActTile;
+    const int64_t nsuper = (ntiles + kSuperTiles - 1) / kSuperTiles;
+    // step totals from the supertile sums (every block needs the birth total)
+    uint64_t b = 0, d = 0, m = 0;
+    for (int64_t sidx = threadIdx.x; sidx < nsuper; sidx += kBlock) {
+        b += x.super_cnt[3 * sidx];
+        d += x.super_cnt[3 * sidx + 1];
+        m += x.super_cnt[3 * sidx + 2];
+    }
+    const int64_t nbt = (int64_t)block_sum_u64(b, s_red);
+    const bool fits = hw + nbt <= x.cap;
+    if (blockIdx.x == 0) {
+        const int64_t dt = (int64_t)block_sum_u64(d, s_red);
+        const int64_t mt = (int64_t)block_sum_u64(m, s_red);
+        if (threadIdx.x == 0) {
+            x.st->deaths_cur = dt;
+            x.st->moves_cur = mt;
+            // no slot for the births: drop them and flag the step as failed
+            x.st->births_cur = fits ? nbt : 0;
+            if (!fits) x.st->error |= kErrCapacity;
+        }
+    }
+    if (nbt == 0 || !fits) return;
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
       const uint32_t nbirth = x.tile_cnt[3 * tile];
-      const int64_t off = x.tile_off[tile];
+      // births before this tile: earlier supertiles + earlier tiles of its supertile
+      const int64_t s0 = tile / kSuperTiles;
+      uint64_t pre = 0;
+      for (int64_t q = threadIdx.x; q < s0; q += kBlock) pre += x.super_cnt[3 * q];
+      for (int64_t q = s0 * kSuperTiles + threadIdx.x; q < tile; q += kBlock) pre += x.tile_cnt[3 * q];
+      const int64_t off = (int64_t)block_sum_u64(pre, s_red);
+      if (nbirth == 0) continue;
       const int32_t *rec = x.brec + 3 * tile * kActTile;
       for (uint32_t j = threadIdx.x; j < nbirth; j += kBlock) {
         const int64_t r = off + j;
@@ -870,8 +897,8 @@ __global__ void __launch_bounds__(kBlock) k_newborn(Ctx x, Prog p) {
 
 // ---------------------------------------------------------------------------
 // k_cellscan: exclusive scan of the new per-cell counts -> off[] for the
-// next pairing; clears the fill counters and the count buffer that the next
-// step accumulates into.  The last tile closes the step's bookkeeping.
+// next pairing; clears the fill counters and copies the counts into the
+// buffer that the next step updates incrementally.  The last tile closes the step's bookkeeping.
 // mode 0: end of step (src = cnt[(step+1)&1]); mode 1: pre-loop (src =
 // cnt[step&1], no bookkeeping).
 // ---------------------------------------------------------------------------
@@ -917,7 +944,7 @@ __global__ void __launch_bounds__(kBlock) k_cellscan(Ctx x, int32_t mode) {
                 x.off[c] = (uint32_t)run;
                 x.fillF[c] = 0;
                 x.fillM[c] = 0;
-                clr[c] = 0;
+                clr[c] = v[q];
             }
             run += v[q];
         }

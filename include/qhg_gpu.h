@@ -77,14 +77,21 @@ extern "C" {
 #define QHG_MAX_BINO      64
 #define QHG_NAME_LEN      48
 
-/* Philox stream tags: the 16-bit action field of the counter */
-#define QHG_STREAM_PAIR     1
-#define QHG_STREAM_LBIRTH   2
-#define QHG_STREAM_LDEATH   3
-#define QHG_STREAM_RMOVE    4
-#define QHG_STREAM_WMOVE    5
-#define QHG_STREAM_NEWBORN  6
-#define QHG_STREAM_GENETICS 7
+/* Philox stream tags: the 16-bit stream field of the counter.  Each agent
+ * draws from its own (step, stream, id) counters: one pairing word, one
+ * block of action words, and for a newborn its gender and genetics words. */
+#define QHG_STREAM_PAIR     1   /* word 0: the chooser's pick in RandomPair     */
+#define QHG_STREAM_ACT      2   /* per-agent action words, QHG_ACTW_*           */
+#define QHG_STREAM_NEWBORN  6   /* word 0: gender of the newborn                */
+#define QHG_STREAM_GENETICS 7   /* Genetics::makeOffspring draws, in order      */
+
+/* words of QHG_STREAM_ACT (one Philox block covers words 0-3) */
+#define QHG_ACTW_WMOVE_GO   0   /* WeightedMove: move?                          */
+#define QHG_ACTW_WMOVE_DIR  1   /* WeightedMove: target slot                    */
+#define QHG_ACTW_LBIRTH     2   /* LinearBirth                                  */
+#define QHG_ACTW_LDEATH     3   /* LinearDeath                                  */
+#define QHG_ACTW_RMOVE_GO   4   /* RandomMove: move?                            */
+#define QHG_ACTW_RMOVE_DIR  5   /* RandomMove: target slot                      */
 
 typedef struct qhg_grid_desc {
     int32_t        num_cells;

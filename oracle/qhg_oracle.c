@@ -719,15 +719,15 @@ static void act_verhulst(orc *o) {
         double b = o->bc[c];
         if (b > 0) {
             if (o->gender[i] == 0 && o->mate[i] >= 0) {
-                double r = orc_wrandd(orc_draw(seed, step, QHG_STREAM_LBIRTH, (uint64_t)o->id[i], 0));
+                double r = orc_wrandd(orc_draw(seed, step, QHG_STREAM_ACT, (uint64_t)o->id[i], QHG_ACTW_LBIRTH));
                 if (r < b) o->bflag[i] = 1;
             }
         } else if (b < 0) {
-            double r = orc_wrandd(orc_draw(seed, step, QHG_STREAM_LBIRTH, (uint64_t)o->id[i], 0));
+            double r = orc_wrandd(orc_draw(seed, step, QHG_STREAM_ACT, (uint64_t)o->id[i], QHG_ACTW_LBIRTH));
             if (r < -b) { o->life[i] = 0; deaths++; }
         }
         if (o->life[i] > 0) {
-            double r = orc_wrandd(orc_draw(seed, step, QHG_STREAM_LDEATH, (uint64_t)o->id[i], 0));
+            double r = orc_wrandd(orc_draw(seed, step, QHG_STREAM_ACT, (uint64_t)o->id[i], QHG_ACTW_LDEATH));
             if (r < o->dc[c]) { o->life[i] = 0; deaths++; }
         }
     }
@@ -743,9 +743,9 @@ static void act_randommove(orc *o) {
     for (int64_t i = 0; i < o->n; i++) {
         if (o->life[i] == 0) continue;
         int32_t c = o->cell[i];
-        double r = orc_wrandd(orc_draw(seed, step, QHG_STREAM_RMOVE, (uint64_t)o->id[i], 0));
+        double r = orc_wrandd(orc_draw(seed, step, QHG_STREAM_ACT, (uint64_t)o->id[i], QHG_ACTW_RMOVE_GO));
         if (r < p) {
-            double r2 = orc_wrandd(orc_draw(seed, step, QHG_STREAM_RMOVE, (uint64_t)o->id[i], 1));
+            double r2 = orc_wrandd(orc_draw(seed, step, QHG_STREAM_ACT, (uint64_t)o->id[i], QHG_ACTW_RMOVE_DIR));
             int32_t slot = (int32_t)(r2 * (o->nn[c] + 1));
             if (slot > 0) {
                 int32_t nb = o->nbr[(size_t)c * o->conn + slot - 1];
@@ -766,12 +766,12 @@ static void act_weightedmove(orc *o) {
     for (int64_t i = 0; i < o->n; i++) {
         if (o->life[i] == 0) continue;
         int32_t c = o->cell[i];
-        double r = orc_wrandd(orc_draw(seed, step, QHG_STREAM_WMOVE, (uint64_t)o->id[i], 0));
+        double r = orc_wrandd(orc_draw(seed, step, QHG_STREAM_ACT, (uint64_t)o->id[i], QHG_ACTW_WMOVE_GO));
         if (r < p) {
             int32_t nn = o->nn[c];
             const double *w = o->W + (size_t)c * R;
             int32_t idx = -1;
-            uint32_t u2 = orc_draw(seed, step, QHG_STREAM_WMOVE, (uint64_t)o->id[i], 1);
+            uint32_t u2 = orc_draw(seed, step, QHG_STREAM_ACT, (uint64_t)o->id[i], QHG_ACTW_WMOVE_DIR);
             if (w[0] == w[nn]) {
                 idx = (int32_t)orc_wrandi(0, (uint32_t)nn + 1, 1, u2);
             } else {

@@ -44,8 +44,8 @@ MAX_EVALS = 4
 MAX_PL_POINTS = 64
 NAME_LEN = 48
 
-STREAM_PAIR, STREAM_LBIRTH, STREAM_LDEATH = 1, 2, 3
-STREAM_RMOVE, STREAM_WMOVE, STREAM_NEWBORN, STREAM_GENETICS = 4, 5, 6, 7
+STREAM_PAIR, STREAM_ACT, STREAM_NEWBORN, STREAM_GENETICS = 1, 2, 6, 7
+ACTW_WMOVE_GO, ACTW_WMOVE_DIR, ACTW_LBIRTH, ACTW_LDEATH, ACTW_RMOVE_GO, ACTW_RMOVE_DIR = 0, 1, 2, 3, 4, 5
 
 
 class GridDesc(C.Structure):

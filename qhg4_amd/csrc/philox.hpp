@@ -21,7 +21,7 @@
 namespace qhg {
 
 QHG_HD void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
-                          uint32_t k0, uint32_t k1, uint32_t out[4]) {
+                          uint32_t k0, uint32_t k1, uint32_t *out) {
 #pragma unroll
     for (int r = 0; r < 10; r++) {
         if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
@@ -39,7 +39,7 @@ QHG_HD void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
 
 // four consecutive draws n = 4*block .. 4*block+3 of one (stream, id)
 QHG_HD void draw4(uint64_t seed, uint32_t step, uint32_t stream, uint64_t id,
-                  uint32_t block, uint32_t out[4]) {
+                  uint32_t block, uint32_t *out) {
     philox4x32_10(step, (stream << 16) | block, (uint32_t)id, (uint32_t)(id >> 32),
                   (uint32_t)seed, (uint32_t)(seed >> 32), out);
 }

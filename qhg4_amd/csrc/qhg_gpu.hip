@@ -70,7 +70,7 @@ struct qhg_pop {
     uint8_t *d_ice = nullptr;
     uint32_t *d_cnt[2] = {nullptr, nullptr};
     uint32_t *d_off = nullptr, *d_fillF = nullptr, *d_fillM = nullptr;
-    uint32_t *d_bucket = nullptr;
+    uint64_t *d_bucket = nullptr;
     int32_t *d_brec = nullptr;
     int32_t *d_newidx = nullptr;
     uint32_t *d_tile_cnt = nullptr, *d_tile_off = nullptr, *d_super = nullptr, *d_blk_fert = nullptr;
@@ -227,7 +227,7 @@ int alloc_slot_scratch(qhg_pop *p, int64_t cap) {
     if (p->d_tile_off) (void)hipFree(p->d_tile_off);
     if (p->d_super) (void)hipFree(p->d_super);
     const size_t nt = (size_t)(cap / kActTile + 2);
-    HIPCHK(hipMalloc(&p->d_bucket, (size_t)cap * 4));
+    HIPCHK(hipMalloc(&p->d_bucket, (size_t)cap * 8));
     HIPCHK(hipMalloc(&p->d_brec, ((size_t)cap + kActTile) * 12));
     HIPCHK(hipMalloc(&p->d_newidx, (size_t)cap * 4));
     HIPCHK(hipMalloc(&p->d_tile_cnt, nt * 12));
@@ -291,9 +291,12 @@ int build_prog(qhg_pop *p) {
         switch (g.ops[q]) {
         case QHG_ACT_GETOLD: g.need_birth = 1; break;
         case QHG_ACT_FERTILITY: g.need_gender = 1; g.need_lastb = 1; break;
-        case QHG_ACT_VERHULST: g.need_gender = 1; g.need_mate = 1; g.need_count = 1; break;
-        case QHG_ACT_VERHULSTVARK: g.need_gender = 1; g.need_mate = 1; g.need_count = 1; g.need_K = 1; break;
-        case QHG_ACT_RANDOMMOVE: case QHG_ACT_WEIGHTEDMOVE: g.need_move = 1; break;
+        case QHG_ACT_VERHULST: g.need_gender = 1; g.need_mate = 1; g.need_count = 1; g.need_act0 = 1; break;
+        case QHG_ACT_VERHULSTVARK:
+            g.need_gender = 1; g.need_mate = 1; g.need_count = 1; g.need_K = 1; g.need_act0 = 1;
+            break;
+        case QHG_ACT_RANDOMMOVE: g.need_move = 1; g.need_act1 = 1; break;
+        case QHG_ACT_WEIGHTEDMOVE: g.need_move = 1; g.need_act0 = 1; break;
         default: break;
         }
     }
@@ -539,7 +542,7 @@ int phase_act(qhg_pop *p) {
         if (rc) return rc;
     }
     Ctx x = p->ctx();
-    int rc = launch(p, KB_ACT, k_act, p->num_cu * 4, x, p->prog);
+    int rc = launch(p, KB_ACT, k_act, p->num_cu * 8, x, p->prog);
     if (rc) return rc;
     p->acted = true;
     return 0;

@@ -31,7 +31,7 @@
 namespace qhg {
 
 constexpr int kBlock = 256;          // threads per block (4 waves)
-constexpr int kActItems = 4;         // agent slots per thread per tile
+constexpr int kActItems = 1;         // agent slots per thread per tile
 constexpr int kActTile = kBlock * kActItems;   // agent slots per look-back tile
 constexpr int kCellTile = 1024;      // cells per look-back tile (4 per thread)
 constexpr int kMaxOps = QHG_MAX_ACTIONS;
@@ -83,7 +83,7 @@ struct Ctx {
     uint32_t *cnt[2];
     uint32_t *off;         // ncells + 1
     uint32_t *fillF, *fillM;
-    uint32_t *bucket;      // cap
+    uint64_t *bucket;      // cap entries: slot << 32 | the agent's pairing draw
     int32_t *brec;         // 3 * (cap + kActTile): birth records staged per act tile
     uint32_t *tile_cnt;    // 3 per act tile: births, deaths, moves
     uint32_t *super_cnt;   // 3 per group of kSuperTiles act tiles (sums of tile_cnt)
@@ -103,6 +103,7 @@ struct Prog {
     int32_t has_pair;
     // fields the program reads (set by the host from the action list)
     int32_t need_gender, need_birth, need_lastb, need_mate, need_count, need_K, need_move;
+    int32_t need_act0, need_act1;   // Philox blocks 0 / 1 of the ACT stream
     double fert_min, fert_max, fert_inter;
     double b0, d0, theta, Kconst;
     double rmove_p, wmove_p;
@@ -119,11 +120,14 @@ constexpr uint64_t kFlagAgg = 1ull << 62;
 constexpr uint64_t kFlagInc = 2ull << 62;
 constexpr uint64_t kValMask = (1ull << 62) - 1;
 
+// The look-back only exchanges the tile words themselves (no other data
+// written in the same launch is read by another block), so relaxed
+// agent-scope atomics suffice and no cache write-back is needed.
 __device__ __forceinline__ void tile_publish(uint64_t *w, uint64_t v) {
-    __hip_atomic_store(w, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint64_t tile_read(uint64_t *w) {
-    return __hip_atomic_load(w, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
@@ -212,11 +216,13 @@ __global__ void __launch_bounds__(kBlock) k_bucket(Ctx x) {
     __syncthreads();
     uint32_t my_fert = 0;
     const int64_t hw = x.st->hw;
+    const uint32_t step = (uint32_t)x.st->step;
     constexpr int I = 4;
     for (int64_t b0 = (int64_t)blockIdx.x * kBlock * I; b0 < hw; b0 += (int64_t)gridDim.x * kBlock * I) {
         uint8_t l[I];
         int32_t c[I];
         uint8_t g[I];
+        int64_t id[I];
 #pragma unroll
         for (int k = 0; k < I; k++) {
             const int64_t i = b0 + (int64_t)k * kBlock + threadIdx.x;
@@ -227,7 +233,7 @@ __global__ void __launch_bounds__(kBlock) k_bucket(Ctx x) {
             const int64_t i = b0 + (int64_t)k * kBlock + threadIdx.x;
             if (l[k] == 0) continue;
             x.a.mate[i] = -3;
-            if (l[k] == QHG_LIFE_FERTILE) { c[k] = x.a.cell[i]; g[k] = x.a.gender[i]; }
+            if (l[k] == QHG_LIFE_FERTILE) { c[k] = x.a.cell[i]; g[k] = x.a.gender[i]; id[k] = x.a.id[i]; }
         }
         uint32_t pos[I], lo[I], hi[I];
 #pragma unroll
@@ -242,7 +248,10 @@ __global__ void __launch_bounds__(kBlock) k_bucket(Ctx x) {
             if (l[k] != QHG_LIFE_FERTILE || g[k] > 1) continue;
             const int64_t i = b0 + (int64_t)k * kBlock + threadIdx.x;
             my_fert++;
-            if (pos[k] < hi[k] - lo[k]) x.bucket[g[k] == 0 ? lo[k] + pos[k] : hi[k] - 1 - pos[k]] = (uint32_t)i;
+            // the draw this agent uses if it chooses a partner (RandomPair.cpp:212, :249)
+            const uint32_t u = draw(x.seed, step, QHG_STREAM_PAIR, (uint64_t)id[k], 0);
+            const uint64_t e = ((uint64_t)i << 32) | u;
+            if (pos[k] < hi[k] - lo[k]) x.bucket[g[k] == 0 ? lo[k] + pos[k] : hi[k] - 1 - pos[k]] = e;
             else atomicOr(&x.st->error, kErrBucket);
         }
     }
@@ -251,34 +260,25 @@ __global__ void __launch_bounds__(kBlock) k_bucket(Ctx x) {
     if (threadIdx.x == 0) x.blk_fert[blockIdx.x] = s_fert;
 }
 
-__device__ void sort_u32(uint32_t *v, uint32_t n) {
+__device__ void sort_u64(uint64_t *v, uint32_t n) {
     if (n < 2) return;
-    if (n <= 48) {
-        for (uint32_t i = 1; i < n; i++) {
-            uint32_t k = v[i];
-            int32_t j = (int32_t)i - 1;
-            while (j >= 0 && v[j] > k) { v[j + 1] = v[j]; j--; }
-            v[j + 1] = k;
-        }
-        return;
-    }
-    // shell sort (Ciura gaps extended by x2.25)
+    // shell sort (Ciura gaps, extended by x2.25 above 1750)
     const uint32_t gaps[] = {1750, 701, 301, 132, 57, 23, 10, 4, 1};
     uint32_t g0 = 1750;
     while (g0 * 9 / 4 < n) g0 = g0 * 9 / 4;
     for (uint32_t gap = g0; gap > 1750; gap = gap * 4 / 9) {
         for (uint32_t i = gap; i < n; i++) {
-            uint32_t k = v[i];
+            const uint64_t k = v[i];
             uint32_t j = i;
             while (j >= gap && v[j - gap] > k) { v[j] = v[j - gap]; j -= gap; }
             v[j] = k;
         }
     }
     for (int gi = 0; gi < 9; gi++) {
-        uint32_t gap = gaps[gi];
+        const uint32_t gap = gaps[gi];
         if (gap >= n) continue;
         for (uint32_t i = gap; i < n; i++) {
-            uint32_t k = v[i];
+            const uint64_t k = v[i];
             uint32_t j = i;
             while (j >= gap && v[j - gap] > k) { v[j] = v[j - gap]; j -= gap; }
             v[j] = k;
@@ -314,35 +314,36 @@ __device__ __forceinline__ int select_bit(uint64_t v, int k) {
 constexpr int kPairBlock = 64;
 constexpr int kPairLds = 32;
 
-__device__ __forceinline__ void pair_select(const Ctx &x, uint32_t step, const uint32_t *chooser, uint32_t nch,
-                                            uint32_t *pool, uint32_t np) {
+// One cell's selection loop.  chooser/pool entries are slot << 32 | draw
+// (both sorted by slot = id); the chooser's draw picks k = (int)wrandr(0,
+// left), the k-th still unpaired pool member.
+__device__ __forceinline__ void pair_select(const Ctx &x, const uint64_t *chooser, uint32_t nch,
+                                            uint64_t *pool, uint32_t np) {
     uint32_t left = np;
     if (np <= 64) {
         uint64_t avail = (np == 64) ? ~0ull : ((1ull << np) - 1ull);
         for (uint32_t j = 0; j < nch && left > 0; j++) {
-            const uint32_t a = chooser[j];
-            const uint32_t u = draw(x.seed, step, QHG_STREAM_PAIR, (uint64_t)x.a.id[a], 0);
-            const int k = (int)wrandr(0, (double)(int32_t)left, u);
+            const uint64_t ea = chooser[j];
+            const int k = (int)wrandr(0, (double)(int32_t)left, (uint32_t)ea);
             const int pos = select_bit(avail, k);
             avail &= ~(1ull << pos);
-            const uint32_t b = pool[pos];
+            const uint32_t a = (uint32_t)(ea >> 32), b = (uint32_t)(pool[pos] >> 32);
             x.a.mate[a] = (int32_t)b;
             x.a.mate[b] = (int32_t)a;
             left--;
         }
     } else {
-        // mark taken partners with the top bit (slots < 2^31)
+        // more than 64 candidates: mark taken partners with bit 63 (slots < 2^31)
         for (uint32_t j = 0; j < nch && left > 0; j++) {
-            const uint32_t a = chooser[j];
-            const uint32_t u = draw(x.seed, step, QHG_STREAM_PAIR, (uint64_t)x.a.id[a], 0);
-            const int k = (int)wrandr(0, (double)(int32_t)left, u);
+            const uint64_t ea = chooser[j];
+            const int k = (int)wrandr(0, (double)(int32_t)left, (uint32_t)ea);
             int found = -1;
             uint32_t m = 0;
             for (m = 0; m < np; m++) {
-                if (!(pool[m] & 0x80000000u)) { found++; if (found == k) break; }
+                if (!(pool[m] >> 63)) { found++; if (found == k) break; }
             }
-            const uint32_t b = pool[m];
-            pool[m] = b | 0x80000000u;
+            const uint32_t a = (uint32_t)(ea >> 32), b = (uint32_t)(pool[m] >> 32);
+            pool[m] |= 1ull << 63;
             x.a.mate[a] = (int32_t)b;
             x.a.mate[b] = (int32_t)a;
             left--;
@@ -350,28 +351,9 @@ __device__ __forceinline__ void pair_select(const Ctx &x, uint32_t step, const u
     }
 }
 
-// pair_select for a cell staged in LDS: chooser ids preloaded, pool <= 32
-__device__ __forceinline__ void pair_select_lds(const Ctx &x, uint32_t step, const uint32_t *chooser,
-                                                const uint64_t *ids, uint32_t nch, const uint32_t *pool,
-                                                uint32_t np) {
-    uint32_t left = np;
-    uint64_t avail = (np == 64) ? ~0ull : ((1ull << np) - 1ull);
-    for (uint32_t j = 0; j < nch && left > 0; j++) {
-        const uint32_t a = chooser[j];
-        const uint32_t u = draw(x.seed, step, QHG_STREAM_PAIR, ids[j], 0);
-        const int k = (int)wrandr(0, (double)(int32_t)left, u);
-        const int pos = select_bit(avail, k);
-        avail &= ~(1ull << pos);
-        const uint32_t b = pool[pos];
-        x.a.mate[a] = (int32_t)b;
-        x.a.mate[b] = (int32_t)a;
-        left--;
-    }
-}
-
-__device__ __forceinline__ void lds_insertion_sort(uint32_t *v, uint32_t n) {
+__device__ __forceinline__ void lds_insertion_sort(uint64_t *v, uint32_t n) {
     for (uint32_t i = 1; i < n; i++) {
-        const uint32_t k = v[i];
+        const uint64_t k = v[i];
         int32_t j = (int32_t)i - 1;
         while (j >= 0 && v[j] > k) { v[j + 1] = v[j]; j--; }
         v[j + 1] = k;
@@ -379,8 +361,7 @@ __device__ __forceinline__ void lds_insertion_sort(uint32_t *v, uint32_t n) {
 }
 
 __global__ void __launch_bounds__(kPairBlock) k_pair(Ctx x, int32_t do_pair) {
-    __shared__ uint32_t s_buf[kPairBlock * 2 * kPairLds];
-    __shared__ uint64_t s_ids[kPairBlock * kPairLds];
+    __shared__ uint64_t s_buf[kPairBlock * 2 * kPairLds];
     {
         const int64_t ntiles = (x.st->hw + kActTile - 1) / kActTile;
         const int64_t ns = 3 * ((ntiles + kSuperTiles - 1) / kSuperTiles);
@@ -399,35 +380,28 @@ __global__ void __launch_bounds__(kPairBlock) k_pair(Ctx x, int32_t do_pair) {
         __syncthreads();
         if (threadIdx.x == 0) x.st->fertile_total += s_sum;
     }
-    const uint32_t step = (uint32_t)x.st->step;
-    uint32_t *lf = s_buf + threadIdx.x * 2 * kPairLds;
-    uint32_t *lm = lf + kPairLds;
+    uint64_t *lf = s_buf + threadIdx.x * 2 * kPairLds;
+    uint64_t *lm = lf + kPairLds;
     for (int64_t c = (int64_t)blockIdx.x * kPairBlock + threadIdx.x; c < x.ncells;
          c += (int64_t)gridDim.x * kPairBlock) {
         const uint32_t F = x.fillF[c], M = x.fillM[c];
         if (F == 0 || M == 0) continue;
         const uint32_t lo = x.off[c], hi = x.off[c + 1];
         if (F + M > hi - lo) { atomicOr(&x.st->error, kErrBucket); continue; }
-        uint32_t *fb = x.bucket + lo;
-        uint32_t *mb = x.bucket + hi - M;
+        uint64_t *fb = x.bucket + lo;
+        uint64_t *mb = x.bucket + hi - M;
         if (F <= kPairLds && M <= kPairLds) {
             for (uint32_t j = 0; j < F; j++) lf[j] = fb[j];
             for (uint32_t j = 0; j < M; j++) lm[j] = mb[j];
             lds_insertion_sort(lf, F);
             lds_insertion_sort(lm, M);
-            const bool fem = F <= M;
-            const uint32_t *ch = fem ? lf : lm;
-            const uint32_t nch = fem ? F : M;
-            uint64_t *ids = s_ids + threadIdx.x * kPairLds;
-#pragma unroll
-            for (int j = 0; j < kPairLds; j++)
-                if ((uint32_t)j < nch) ids[j] = (uint64_t)x.a.id[ch[j]];
-            pair_select_lds(x, step, ch, ids, nch, fem ? lm : lf, fem ? M : F);
+            if (F <= M) pair_select(x, lf, F, lm, M);
+            else pair_select(x, lm, M, lf, F);
         } else {
-            sort_u32(fb, F);
-            sort_u32(mb, M);
-            if (F <= M) pair_select(x, step, fb, F, mb, M);
-            else pair_select(x, step, mb, M, fb, F);
+            sort_u64(fb, F);
+            sort_u64(mb, M);
+            if (F <= M) pair_select(x, fb, F, mb, M);
+            else pair_select(x, mb, M, fb, F);
         }
     }
 }
@@ -451,6 +425,7 @@ struct ActItem {
     int32_t npend;            // registered moves (<= 2), resolved after the program
     int32_t pk0, pk1;         // their action kinds
     uint32_t pu0, pu1;        // their target draws
+    uint32_t u[8];            // words 0..7 of the agent's QHG_STREAM_ACT counter
 };
 
 // The action program of one agent (compute only; every field it reads was
@@ -461,6 +436,8 @@ __device__ __forceinline__ void act_program(const Ctx &x, const Prog &p, const S
                                             uint32_t &deaths) {
     const uint32_t step = v.step;
     const double t = v.t;
+    if (p.need_act0) draw4(x.seed, step, QHG_STREAM_ACT, (uint64_t)a.id, 0, &a.u[0]);
+    if (p.need_act1) draw4(x.seed, step, QHG_STREAM_ACT, (uint64_t)a.id, 1, &a.u[4]);
     uint64_t word = p.opword;
     for (int q = 0; q < p.nops; q++, word >>= 4) {
         const int op = (int)(word & 15u);
@@ -490,27 +467,24 @@ __device__ __forceinline__ void act_program(const Ctx &x, const Prog &p, const S
             }
             if (b > 0) {
                 if (a.g == 0 && a.mate >= 0) {
-                    const double r = wrandd(draw(x.seed, step, QHG_STREAM_LBIRTH, (uint64_t)a.id, 0));
-                    if (r < b) a.bflag = 1;
+                    if (wrandd(a.u[QHG_ACTW_LBIRTH]) < b) a.bflag = 1;
                 }
             } else if (b < 0) {
-                const double r = wrandd(draw(x.seed, step, QHG_STREAM_LBIRTH, (uint64_t)a.id, 0));
-                if (r < -b) a.life = 0;
+                if (wrandd(a.u[QHG_ACTW_LBIRTH]) < -b) a.life = 0;
             }
             if (a.life != 0) {
-                const double r = wrandd(draw(x.seed, step, QHG_STREAM_LDEATH, (uint64_t)a.id, 0));
-                if (r < d) a.life = 0;
+                if (wrandd(a.u[QHG_ACTW_LDEATH]) < d) a.life = 0;
             }
             if (a.life == 0) { deaths++; break; }
         } else if (op == QHG_ACT_RANDOMMOVE || op == QHG_ACT_WEIGHTEDMOVE) {
             // RandomMove::execute (RandomMove.cpp:65-100) / WeightedMove::execute
             // (WeightedMove.cpp:45-106): decision draw now, target below
             const bool rm = op == QHG_ACT_RANDOMMOVE;
-            uint32_t u[4];
-            draw4(x.seed, step, rm ? QHG_STREAM_RMOVE : QHG_STREAM_WMOVE, (uint64_t)a.id, 0, u);
-            if (wrandd(u[0]) < (rm ? p.rmove_p : p.wmove_p)) {
-                if (a.npend == 0) { a.pk0 = op; a.pu0 = u[1]; a.npend = 1; }
-                else if (a.npend == 1) { a.pk1 = op; a.pu1 = u[1]; a.npend = 2; }
+            const uint32_t go = rm ? a.u[QHG_ACTW_RMOVE_GO] : a.u[QHG_ACTW_WMOVE_GO];
+            const uint32_t dir = rm ? a.u[QHG_ACTW_RMOVE_DIR] : a.u[QHG_ACTW_WMOVE_DIR];
+            if (wrandd(go) < (rm ? p.rmove_p : p.wmove_p)) {
+                if (a.npend == 0) { a.pk0 = op; a.pu0 = dir; a.npend = 1; }
+                else if (a.npend == 1) { a.pk1 = op; a.pu1 = dir; a.npend = 2; }
             }
         }
         // RandomPair, MultiEvaluator, NPPCapacity, Genetics: no execute
@@ -977,7 +951,6 @@ __global__ void __launch_bounds__(kBlock) k_cellscan(Ctx x, int32_t mode) {
                     const uint64_t *srcw = reinterpret_cast<const uint64_t *>(st);
                     uint64_t *dstw = reinterpret_cast<uint64_t *>(dst);
                     for (int q = 0; q < (int)(sizeof(DevStatus) / 8); q++) dstw[q] = srcw[q];
-                    __threadfence_system();
                 }
             }
         }

@@ -152,6 +152,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="also run one plain device copy of BYTES bytes (torch), the reference kernel "
+                         "for scaling rocprof FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the population step has no multi-GPU path); "
@@ -215,6 +218,14 @@ def main():
         "kernels": kernels,
     }
     print(json.dumps(out))
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(args.copy_calibration // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("bench: copy calibration %d bytes done" % args.copy_calibration, file=sys.stderr)
     return 0
 
 

@@ -219,14 +219,26 @@ def main():
     }
     print(json.dumps(out))
     if args.copy_calibration:
-        import torch
-        src = torch.ones(args.copy_calibration // 4, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        print("bench: copy calibration %d bytes done" % args.copy_calibration, file=sys.stderr)
+        copy_calibration(args.copy_calibration)
     return 0
+
+
+def copy_calibration(nbytes):
+    """One fill (writes nbytes) and one device-to-device copy (reads and
+    writes nbytes) through the HIP runtime: kernels of known traffic that
+    scale the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    n = C.c_size_t(nbytes)
+    for name, rc in (("hipMalloc", hip.hipMalloc(C.byref(src), n)), ("hipMalloc", hip.hipMalloc(C.byref(dst), n)),
+                     ("hipMemset", hip.hipMemset(src, 1, n)), ("sync", hip.hipDeviceSynchronize()),
+                     ("hipMemcpy", hip.hipMemcpy(dst, src, n, 3)), ("sync", hip.hipDeviceSynchronize())):
+        if rc != 0:
+            raise SystemExit("copy calibration: %s failed (%d)" % (name, rc))
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print("bench: copy calibration %d bytes done" % nbytes, file=sys.stderr)
 
 
 if __name__ == "__main__":

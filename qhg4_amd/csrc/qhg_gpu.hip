@@ -45,6 +45,9 @@ const char *kKernelNames[KB_COUNT] = {"k_bucket", "k_pair", "k_act", "k_newborn"
                                       "k_hist", "k_capacity", "k_eval", "k_compact"};
 
 constexpr int kRing = kStatusRing;
+#ifndef QHG_ACT_GRID_PER_CU
+#define QHG_ACT_GRID_PER_CU 8   // k_act blocks per compute unit
+#endif
 
 struct AgentBufs {
     AgentPtrs p{};
@@ -542,7 +545,7 @@ int phase_act(qhg_pop *p) {
         if (rc) return rc;
     }
     Ctx x = p->ctx();
-    int rc = launch(p, KB_ACT, k_act, p->num_cu * 8, x, p->prog);
+    int rc = launch(p, KB_ACT, k_act, p->num_cu * QHG_ACT_GRID_PER_CU, x, p->prog);
     if (rc) return rc;
     p->acted = true;
     return 0;

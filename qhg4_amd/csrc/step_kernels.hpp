@@ -31,7 +31,13 @@
 namespace qhg {
 
 constexpr int kBlock = 256;          // threads per block (4 waves)
-constexpr int kActItems = 1;         // agent slots per thread per tile
+#ifndef QHG_ACT_ITEMS
+#define QHG_ACT_ITEMS 1
+#endif
+#ifndef QHG_PAIR_LDS
+#define QHG_PAIR_LDS 32
+#endif
+constexpr int kActItems = QHG_ACT_ITEMS;   // agent slots per thread per tile
 constexpr int kActTile = kBlock * kActItems;   // agent slots per look-back tile
 constexpr int kCellTile = 1024;      // cells per look-back tile (4 per thread)
 constexpr int kMaxOps = QHG_MAX_ACTIONS;
@@ -312,7 +318,7 @@ __device__ __forceinline__ int select_bit(uint64_t v, int k) {
 // has at most kPairLds of them, in the bucket array otherwise.
 // ---------------------------------------------------------------------------
 constexpr int kPairBlock = 64;
-constexpr int kPairLds = 32;
+constexpr int kPairLds = QHG_PAIR_LDS;
 
 // One cell's selection loop.  chooser/pool entries are slot << 32 | draw
 // (both sorted by slot = id); the chooser's draw picks k = (int)wrandr(0,

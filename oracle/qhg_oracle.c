@@ -25,8 +25,8 @@
  * Schedule-independent replacements (documented deviations):
  *   - draws: Philox4x32-10 keyed (seed, step, stream, agent id, draw#);
  *   - per-cell pairing orders candidates by agent id (reference: slot index);
- *   - new ids: max_id + 1 + rank of the mother's id among this step's mothers
- *     (reference: thread-strided IDGen);
+ *   - new ids: max_id + 1 + rank of the birth in (birth cell, mother id)
+ *     order among this step's births (reference: thread-strided IDGen);
  *   - time fields are FP64 (reference: float).
  */
 #include <stdlib.h>
@@ -838,14 +838,22 @@ static void make_genome(orc *o, int64_t baby, int64_t mother, int64_t father, ui
 }
 
 static void finalize(orc *o, double t) {
-    /* births in mother-id order */
+    /* births ranked by (birth cell, mother id): a stable counting sort of the
+     * mothers (id order) by the cell they registered the birth in */
     int64_t nb = 0;
     for (int64_t i = 0; i < o->n; i++) nb += o->bflag[i];
     reserve(o, o->n + nb + 1);
     int64_t base = o->n;
-    int64_t k = 0;
-    for (int64_t i = 0; i < base; i++) {
-        if (!o->bflag[i]) continue;
+    int64_t *morder = (int64_t *)malloc((size_t)(nb + 1) * sizeof(int64_t));
+    {
+        int64_t *cstart = (int64_t *)calloc((size_t)o->ncells + 1, sizeof(int64_t));
+        for (int64_t i = 0; i < base; i++) if (o->bflag[i]) cstart[o->cell[i] + 1]++;
+        for (int32_t c = 0; c < o->ncells; c++) cstart[c + 1] += cstart[c];
+        for (int64_t i = 0; i < base; i++) if (o->bflag[i]) morder[cstart[o->cell[i]]++] = i;
+        free(cstart);
+    }
+    for (int64_t k = 0; k < nb; k++) {
+        const int64_t i = morder[k];
         int64_t s = base + k;
         int64_t nid = o->max_id + 1 + k;
         o->id[s] = nid;
@@ -862,8 +870,8 @@ static void finalize(orc *o, double t) {
         o->bflag[s] = 0;
         o->nbabies[i]++;
         if (o->gw) make_genome(o, s, i, o->mate[i], (uint64_t)nid);
-        k++;
     }
+    free(morder);
     o->n = base + nb;
     o->max_id += nb;
     o->births_last = nb;

@@ -1,9 +1,10 @@
-// Per-cell environment kernels (run at pre-loop and on environment events)
-// and the stable compaction of the agent slots.
+// Per-cell environment kernels, run at pre-loop and on environment events.
 #pragma once
-#include "step_kernels.hpp"
+#include "cell_kernels.hpp"
 
 namespace qhg {
+
+constexpr int kBlock = 256;   // threads per block of the per-cell environment kernels
 
 struct EnvPtrs {
     const double *alt, *lon, *lat, *water, *npp, *temp, *rain;
@@ -126,124 +127,6 @@ __global__ void __launch_bounds__(kBlock) k_eval_combine(Ctx x, const double *tm
         if (p.last && p.mode != QHG_ME_MAX_SIMPLE) {
             for (int j = 1; j < R; j++) w[j] += w[j - 1];
         }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// stable compaction of live slots into the other buffer set
-// (LBController::compactData, QHG4/utils/LBController.cpp:351-445, done as a
-// scan): k_compact_count (live slots per tile) -> k_scan_tiles (one block)
-// -> k_compact_scatter (every field but mate, old->new slot map) ->
-// k_compact_mate (mates remapped, a dead partner becomes -1) ->
-// k_commit_compact (hw := live count).
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_compact_count(Ctx x, uint32_t *cnt) {
-    __shared__ uint32_t s_cnt[kActItems * (kBlock / 64)];
-    const int64_t hw = x.st->hw;
-    const int64_t ntiles = (hw + kActTile - 1) / kActTile;
-    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const int64_t base = tile * kActTile;
-        uint32_t alive[kActItems], rank[kActItems], tot;
-#pragma unroll
-        for (int k = 0; k < kActItems; k++) {
-            const int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
-            alive[k] = (i < hw) ? (x.a.life[i] != 0) : 0u;
-        }
-        tile_ranks(alive, rank, s_cnt, tot);
-        if (threadIdx.x == 0) cnt[tile] = tot;
-        __syncthreads();
-    }
-}
-
-// one block: off[t] = sum of cnt[0..t), *total = sum of all
-__global__ void __launch_bounds__(1024) k_scan_tiles(const uint32_t *cnt, uint32_t *off, const DevStatus *st,
-                                                      int64_t *total) {
-    __shared__ uint32_t s_w[1024 / 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t ntiles = (st->hw + kActTile - 1) / kActTile;
-    uint64_t run = 0;
-    for (int64_t b0 = 0; b0 < ntiles; b0 += 1024) {
-        const int64_t t = b0 + threadIdx.x;
-        const uint32_t c = (t < ntiles) ? cnt[t] : 0u;
-        uint32_t y = c;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t o = __shfl_up(y, d, 64);
-            if (lane >= d) y += o;
-        }
-        if (lane == 63) s_w[wave] = y;
-        __syncthreads();
-        uint32_t wpre = 0, tot = 0;
-        for (int w = 0; w < 1024 / 64; w++) {
-            const uint32_t sv = s_w[w];
-            if (w < wave) wpre += sv;
-            tot += sv;
-        }
-        if (t < ntiles) off[t] = (uint32_t)(run + wpre + y - c);
-        run += tot;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *total = (int64_t)run;
-}
-
-__global__ void __launch_bounds__(kBlock) k_compact_scatter(Ctx x, AgentPtrs dst, int32_t *newidx,
-                                                            const uint32_t *off) {
-    __shared__ uint32_t s_cnt[kActItems * (kBlock / 64)];
-    const int64_t hw = x.st->hw;
-    const int64_t ntiles = (hw + kActTile - 1) / kActTile;
-    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const int64_t base = tile * kActTile;
-        uint32_t alive[kActItems], rank[kActItems], tot;
-#pragma unroll
-        for (int k = 0; k < kActItems; k++) {
-            const int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
-            alive[k] = (i < hw) ? (x.a.life[i] != 0) : 0u;
-        }
-        tile_ranks(alive, rank, s_cnt, tot);
-        const int64_t o = off[tile];
-#pragma unroll
-        for (int k = 0; k < kActItems; k++) {
-            const int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
-            if (i >= hw) continue;
-            if (alive[k]) {
-                const int64_t j = o + rank[k];
-                newidx[i] = (int32_t)j;
-                dst.id[j] = x.a.id[i];
-                dst.cell[j] = x.a.cell[i];
-                dst.life[j] = x.a.life[i];
-                dst.gender[j] = x.a.gender[i];
-                dst.birth[j] = x.a.birth[i];
-                dst.age[j] = x.a.age[i];
-                dst.lastb[j] = x.a.lastb[i];
-                dst.nbabies[j] = x.a.nbabies[i];
-                if (x.gwords) {
-                    const uint64_t *src = x.a.genome + i * x.gwords;
-                    uint64_t *d = dst.genome + j * x.gwords;
-                    for (int q = 0; q < x.gwords; q++) d[q] = src[q];
-                }
-            } else {
-                newidx[i] = -1;
-            }
-        }
-        __syncthreads();
-    }
-}
-
-__global__ void __launch_bounds__(kBlock) k_compact_mate(Ctx x, AgentPtrs dst, const int32_t *newidx) {
-    const int64_t hw = x.st->hw;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < hw;
-         i += (int64_t)gridDim.x * kBlock) {
-        const int32_t j = newidx[i];
-        if (j < 0) continue;
-        const int32_t m = x.a.mate[i];
-        dst.mate[j] = (m >= 0) ? newidx[m] : m;
-    }
-}
-
-__global__ void k_commit_compact(DevStatus *st, const int64_t *hw_next) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        st->hw = *hw_next;
-        st->n_alive = *hw_next;
     }
 }
 

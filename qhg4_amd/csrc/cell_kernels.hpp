@@ -1,0 +1,890 @@
+// Device side of the population step engine (gfx950, wave64).
+//
+// Agents are stored cell-sorted: the live agents of cell c occupy slots
+// [off[c], off[c+1]) in id order, in one of two structure-of-arrays buffers
+// (current / next, swapped every step).  A step is three launches, none of
+// them with global atomics:
+//
+//   k_cell_act      one wave per cell: RandomPair::initialize/findMates on
+//                   the cell's contiguous segment (QHG4/actions/RandomPair.cpp
+//                   :107-279), then every Action<T>::execute of the step fused
+//                   in priority order (QHG4/core/SPopulation.cpp:554-577).
+//                   Writes each agent's outcome (dead / stays / moves to
+//                   neighbour d / gave birth) and the per-cell counts of
+//                   stayers, movers per direction and births.
+//   k_cell_scan     per cell: new count = stayers + movers arriving from the
+//                   neighbours + births; one decoupled look-back scan gives the
+//                   new segment offsets and the birth offsets
+//                   (updateNumAgentsPerCell, SPopulation.cpp:509-536).
+//   k_cell_scatter  one wave per destination cell: stayers and arriving movers
+//                   merged in id order into the next buffer, newborns appended
+//                   (performMoves, recycleDeadSpaceNew, createAgentAtIndex,
+//                   makePopSpecificOffspring, Genetics::makeOffspring;
+//                   SPopulation.cpp:596-1092, Genetics.cpp:285-337).
+//
+// Per-cell tables (count, K, weights, neighbours) are read once per cell.
+// Genomes live in a separate pool addressed by a handle, so they are only
+// touched at births and at occasional pool compactions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "philox.hpp"
+#include "../../include/qhg_gpu.h"
+
+namespace qhg {
+
+constexpr int kWave = 64;
+constexpr int kCellBlock = 64;        // per-cell kernels: one wave per block
+constexpr int kLdsCell = 512;         // cells up to this size keep their lists in LDS,
+                                      // larger ones in segment-indexed global scratch
+constexpr int kMaxRuns = 15;          // id-sorted runs merged per destination cell
+constexpr int kScanBlock = 256;       // k_cell_scan threads per block
+constexpr int kCellTile = 1024;       // cells per look-back tile (4 per thread)
+constexpr int kMaxCross = 32;         // device limit on Genetics_num_crossover
+constexpr int kStatusRing = 8;        // host-mapped status snapshots
+
+constexpr int32_t kErrCapacity = 1;   // the next population does not fit
+constexpr int32_t kErrLookback = 4;   // a look-back wait exceeded its spin limit
+constexpr int32_t kErrGather = 8;     // scatter found a count mismatch
+constexpr uint32_t kLookbackSpins = 1u << 22;
+
+// outcome byte of an agent in a step
+constexpr uint8_t kDecBirth = 0x80;   // registered a birth
+constexpr uint8_t kDecDead = 0x40;    // died
+constexpr uint8_t kDecDirMask = 0x07; // 0 stays, d+1 moves to neighbour d
+
+struct DevStatus {
+    int64_t n_alive[2];      // live agents at the start of a step of this parity
+    int64_t max_id[2];
+    int64_t gpool[2];        // genome pool high-water mark
+    int64_t step;            // steps completed
+    double t;
+    int64_t births_last, deaths_last, moves_last;
+    int64_t births_total, deaths_total, moves_total;
+    int64_t agent_steps, fertile_total;
+    int32_t error, pad;
+};
+
+struct AgentBuf {
+    int64_t *id;
+    int64_t *mate_id;        // partner id of this step's pairing, -3 when unpaired
+    uint8_t *life;
+    uint8_t *gender;
+    double *birth;
+    double *age;
+    double *lastb;
+    int32_t *nbabies;
+    int32_t *ghandle;        // genome pool slot (Genetics only)
+};
+
+struct Ctx {
+    AgentBuf cur, nxt;
+    const uint32_t *off_cur;  // ncells + 1 segment offsets of the current buffer
+    uint32_t *off_nxt;
+    int64_t cap;
+    int32_t ncells, conn;
+    const int32_t *nbr;
+    const uint8_t *nn;
+    double *K;
+    double *W;
+    const uint8_t *ice;
+    const int32_t *in_off;    // incoming neighbour lists (CSR): the cells
+    const int32_t *in_src;    // in_src[k] whose neighbour slot in_dir[k] is
+    const uint8_t *in_dir;    // this cell, k in [in_off[c], in_off[c+1])
+    uint8_t *dec;             // per slot of the current buffer: outcome byte
+    int32_t *bfather;         // per slot: father slot of a registered birth
+    int32_t *gfem, *gmal, *gmate;   // per-slot list scratch of large cells
+    uint8_t *gflag;
+    int32_t *gcand;           // per slot of the next buffer: scatter scratch
+    int64_t *gcid;            //   of large cells
+    uint32_t *cstay, *cbirth, *cmv;   // per cell; cmv is ncells * conn
+    uint32_t *birth_off;      // ncells + 1: births before each cell
+    uint64_t *tiles;          // look-back words of k_cell_scan
+    int32_t ntiles;
+    int32_t nact_blocks;      // grid of k_cell_act
+    uint32_t *wave_part;      // 3 per k_cell_act block: deaths, moves, fertile
+    uint64_t *gpool;          // genome pool, gwords u64 per handle
+    int32_t gwords;
+    int64_t gcap;
+    DevStatus *st;
+    DevStatus *hring;         // host-mapped ring of kStatusRing snapshots
+    uint64_t seed;
+};
+
+struct StepArgs {
+    uint32_t step;            // step index (Philox counter word)
+    uint32_t parity;          // step & 1: which status halves are current
+    double t;                 // time of the step
+};
+
+struct Prog {
+    int32_t nops;
+    int32_t ops[QHG_MAX_ACTIONS];
+    uint64_t opword;          // ops packed 4 bits each (op q in bits 4q..4q+3)
+    int32_t load_age;         // Fertility runs before GetOld (reads the stored age)
+    int32_t has_pair;
+    int32_t need_gender, need_birth, need_lastb, need_act0, need_act1;
+    double fert_min, fert_max, fert_inter;
+    double b0, d0, theta, Kconst;
+    double rmove_p, wmove_p;
+    int32_t gen_size, gen_cross, nbino, pad;
+    double bino[QHG_MAX_BINO];
+};
+
+// ---------------------------------------------------------------------------
+// wave / block helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const int lane = threadIdx.x & 63;
+    return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)v, d, 64);
+        const uint32_t hi = __shfl_xor((uint32_t)(v >> 32), d, 64);
+        v += ((uint64_t)hi << 32) | lo;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t lo = __shfl_up((uint32_t)v, d, 64);
+        const uint32_t hi = __shfl_up((uint32_t)(v >> 32), d, 64);
+        if (lane >= d) v += ((uint64_t)hi << 32) | lo;
+    }
+    return v;
+}
+
+// position of the k-th (0-based) set bit of v (k < popcount(v))
+__device__ __forceinline__ int select_bit(uint64_t v, int k) {
+    int pos = 0;
+    int c = __popc((uint32_t)v);
+    if (k >= c) { k -= c; v >>= 32; pos += 32; }
+    c = __popc((uint32_t)v & 0xFFFFu);
+    if (k >= c) { k -= c; v >>= 16; pos += 16; }
+    c = __popc((uint32_t)v & 0xFFu);
+    if (k >= c) { k -= c; v >>= 8; pos += 8; }
+    c = __popc((uint32_t)v & 0xFu);
+    if (k >= c) { k -= c; v >>= 4; pos += 4; }
+    c = __popc((uint32_t)v & 0x3u);
+    if (k >= c) { k -= c; v >>= 2; pos += 2; }
+    c = (int)(v & 1u);
+    if (k >= c) { pos += 1; }
+    return pos;
+}
+
+// ---------------------------------------------------------------------------
+// decoupled look-back, one 64-bit word per tile: [63:62] flag (1 aggregate,
+// 2 inclusive), [61:0] value.  Only the tile words are exchanged between
+// blocks, so relaxed agent-scope atomics suffice.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kFlagAgg = 1ull << 62;
+constexpr uint64_t kFlagInc = 2ull << 62;
+constexpr uint64_t kValMask = (1ull << 62) - 1;
+
+__device__ __forceinline__ void tile_publish(uint64_t *w, uint64_t v) {
+    __hip_atomic_store(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t tile_read(uint64_t *w) {
+    return __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Called by all 64 lanes of one wave; returns the exclusive prefix of `tile`.
+// Every wait is bounded: past the spin limit the step is flagged as failed.
+__device__ uint64_t lookback(uint64_t *tiles, uint32_t tile, uint64_t agg, int32_t *err) {
+    const int lane = threadIdx.x & 63;
+    if (tile == 0) {
+        if (lane == 0) tile_publish(&tiles[0], kFlagInc | agg);
+        return 0;
+    }
+    if (lane == 0) tile_publish(&tiles[tile], kFlagAgg | agg);
+    uint64_t excl = 0;
+    int64_t top = (int64_t)tile - 1;
+    uint32_t spins = 0;
+    while (true) {
+        const int64_t j = top - lane;
+        const uint64_t w = (j >= 0) ? tile_read(&tiles[j]) : kFlagInc;
+        const uint64_t flag = w & ~kValMask;
+        const uint64_t inc = __ballot(flag == kFlagInc);
+        const uint64_t nready = __ballot(flag == 0);
+        const int first = inc ? (__ffsll((unsigned long long)inc) - 1) : 64;
+        const uint64_t need = (first >= 63) ? ~0ull : ((2ull << first) - 1ull);
+        if (nready & need) {
+            if (++spins < kLookbackSpins) {
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            if (lane == 0) atomicOr(err, kErrLookback);
+            break;
+        }
+        excl += wave_sum_u64(lane <= first ? (w & kValMask) : 0ull);
+        if (first < 64) break;
+        top -= 64;
+    }
+    if (lane == 0) tile_publish(&tiles[tile], kFlagInc | (excl + agg));
+    return excl;
+}
+
+// ---------------------------------------------------------------------------
+// one agent's action program
+// ---------------------------------------------------------------------------
+struct ActItem {
+    uint8_t life, g;
+    int64_t id;
+    int32_t mate;             // partner's local index in the cell, -1 when unpaired
+    double birth, lastb, age;
+    bool age_dirty;
+    uint32_t bflag;
+    int32_t npend;            // registered moves (<= 2), resolved after the program
+    int32_t pk0, pk1;         // their action kinds
+    uint32_t pu0, pu1;        // their target draws
+    uint32_t u[8];            // words 0..7 of the agent's QHG_STREAM_ACT counter
+};
+
+struct CellView {             // the per-cell values every agent of the cell reads
+    uint32_t N;               // live agents at step start (getNumAgents)
+    double K;
+    int nn;
+};
+
+// Every action of the program in priority order (compute only; the caller
+// loaded every field).  A dead agent stops.  Moves are only registered here
+// (their decision draws are taken now) and resolved afterwards, as
+// performMoves applies them after all actions.
+__device__ __forceinline__ void act_program(const Ctx &x, const Prog &p, const StepArgs &sa, const CellView &cv,
+                                            ActItem &a, uint32_t &deaths) {
+    const double t = sa.t;
+    if (p.need_act0) draw4(x.seed, sa.step, QHG_STREAM_ACT, (uint64_t)a.id, 0, &a.u[0]);
+    if (p.need_act1) draw4(x.seed, sa.step, QHG_STREAM_ACT, (uint64_t)a.id, 1, &a.u[4]);
+    uint64_t word = p.opword;
+    for (int q = 0; q < p.nops; q++, word >>= 4) {
+        const int op = (int)(word & 15u);
+        if (op == QHG_ACT_GETOLD) {
+            // GetOld::execute, QHG4/actions/GetOld.cpp:37-48
+            a.age = t - a.birth;
+            a.age_dirty = true;
+        } else if (op == QHG_ACT_FERTILITY) {
+            // Fertility::execute, QHG4/actions/Fertility.cpp:49-74
+            if (a.g == 0) {
+                a.life = (a.age > p.fert_min && a.age < p.fert_max && (t - a.lastb) > p.fert_inter)
+                             ? QHG_LIFE_FERTILE : QHG_LIFE_ALIVE;
+            } else {
+                a.life = (a.age > p.fert_min) ? QHG_LIFE_FERTILE : QHG_LIFE_ALIVE;
+            }
+        } else if (op == QHG_ACT_VERHULST || op == QHG_ACT_VERHULSTVARK) {
+            // LinearBirth::initialize/execute then LinearDeath (Verhulst.cpp:101-115;
+            // LinearBirth.cpp:90-168; LinearDeath.cpp:92-153)
+            const double N = (double)cv.N;
+            double b, d;
+            if (op == QHG_ACT_VERHULST) {
+                b = p.b0 + (p.theta - p.b0) * (N / p.Kconst);
+                d = p.d0 + (p.theta - p.d0) * (N / p.Kconst);
+            } else {
+                b = (cv.K <= 0) ? 0.0 : p.b0 + (p.theta - p.b0) * (N / cv.K);
+                d = (cv.K <= 0) ? 1.0 : p.d0 + (p.theta - p.d0) * (N / cv.K);
+            }
+            if (b > 0) {
+                if (a.g == 0 && a.mate >= 0) {
+                    if (wrandd(a.u[QHG_ACTW_LBIRTH]) < b) a.bflag = 1;
+                }
+            } else if (b < 0) {
+                if (wrandd(a.u[QHG_ACTW_LBIRTH]) < -b) a.life = 0;
+            }
+            if (a.life != 0) {
+                if (wrandd(a.u[QHG_ACTW_LDEATH]) < d) a.life = 0;
+            }
+            if (a.life == 0) { deaths++; break; }
+        } else if (op == QHG_ACT_RANDOMMOVE || op == QHG_ACT_WEIGHTEDMOVE) {
+            // RandomMove::execute (RandomMove.cpp:65-100) / WeightedMove::execute
+            // (WeightedMove.cpp:45-106): decision draw now, target below
+            const bool rm = op == QHG_ACT_RANDOMMOVE;
+            const uint32_t go = rm ? a.u[QHG_ACTW_RMOVE_GO] : a.u[QHG_ACTW_WMOVE_GO];
+            const uint32_t dir = rm ? a.u[QHG_ACTW_RMOVE_DIR] : a.u[QHG_ACTW_WMOVE_DIR];
+            if (wrandd(go) < (rm ? p.rmove_p : p.wmove_p)) {
+                if (a.npend == 0) { a.pk0 = op; a.pu0 = dir; a.npend = 1; }
+                else if (a.npend == 1) { a.pk1 = op; a.pu1 = dir; a.npend = 2; }
+            }
+        }
+        // RandomPair, MultiEvaluator, NPPCapacity, Genetics: no execute
+    }
+}
+
+// Neighbour slot a registered move lands on, or -1 when the agent stays
+// (slot 0, a -1 hole, or ice for WeightedMove).  w: the cell's cumulative
+// weight row, nbr: its neighbour row (registers, compile-time indices only).
+__device__ __forceinline__ int move_dir(const Ctx &x, int kind, uint32_t u, int nn, const double (&w)[7],
+                                        const int32_t (&nbr)[6]) {
+    int idx = -1;
+    if (kind == QHG_ACT_RANDOMMOVE) {
+        idx = (int)(wrandd(u) * (nn + 1));
+    } else {
+        double wl = w[0];
+#pragma unroll
+        for (int q = 1; q < 7; q++) if (q == nn) wl = w[q];
+        if (w[0] == wl) {
+            idx = (int)wrandi(0, (uint32_t)nn + 1, 1, u);
+        } else {
+            const double r2 = wrandd(u) * wl;
+#pragma unroll
+            for (int q = 6; q >= 0; q--) {
+                if (q <= nn && r2 < w[q]) idx = q;   // first q with r2 < w[q]
+            }
+        }
+    }
+    if (idx <= 0) return -1;
+    int32_t nb = -1;
+#pragma unroll
+    for (int q = 0; q < 6; q++) if (q == idx - 1) nb = nbr[q];
+    if (nb < 0) return -1;
+    if (kind == QHG_ACT_WEIGHTEDMOVE && x.ice != nullptr && x.ice[nb]) return -1;
+    return idx - 1;
+}
+
+// ---------------------------------------------------------------------------
+// k_cell_act: pairing + fused actions, one wave per cell
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs sa) {
+    __shared__ int32_t s_fem[kLdsCell], s_mal[kLdsCell], s_mate[kLdsCell];
+    __shared__ uint8_t s_flag[kLdsCell];
+    const int lane = threadIdx.x;
+    const uint64_t lt = lanemask_lt();
+    const int R = x.conn + 1;
+    uint32_t my_deaths = 0, my_moves = 0, my_fert = 0;
+    // a failed step (capacity, look-back, gather) freezes the population:
+    // later steps already enqueued touch nothing
+    if (x.st->error) return;
+
+    // look-back words of this step's k_cell_scan start at zero
+    for (int32_t i = blockIdx.x * kCellBlock + lane; i < x.ntiles; i += gridDim.x * kCellBlock) x.tiles[i] = 0;
+
+    for (int32_t c = blockIdx.x; c < x.ncells; c += gridDim.x) {
+        const uint32_t s = x.off_cur[c];
+        const int32_t n = (int32_t)(x.off_cur[c + 1] - s);
+        if (n == 0) {
+            if (lane == 0) { x.cstay[c] = 0; x.cbirth[c] = 0; }
+            if (lane < x.conn) x.cmv[(int64_t)c * x.conn + lane] = 0;
+            continue;
+        }
+        const int nchunks = (n + kWave - 1) / kWave;
+        // per-cell lists: LDS, or the cell's own segment range of the global
+        // scratch arrays when the cell is large
+        const bool big = n > kLdsCell;
+        int32_t *fem = big ? x.gfem + s : s_fem;
+        int32_t *mal = big ? x.gmal + s : s_mal;
+        int32_t *mate = big ? x.gmate + s : s_mate;
+        uint8_t *flag = big ? x.gflag + s : s_flag;
+
+        // ---- RandomPair: FERTILE females and males of the cell, id order
+        if (p.has_pair) {
+            int32_t F = 0, M = 0;
+            for (int ch = 0; ch < nchunks; ch++) {
+                const int32_t li = ch * kWave + lane;
+                const bool valid = li < n;
+                const uint8_t l = valid ? x.cur.life[s + li] : (uint8_t)0;
+                const uint8_t g = valid ? x.cur.gender[s + li] : (uint8_t)2;
+                const bool isF = valid && l == QHG_LIFE_FERTILE && g == 0;
+                const bool isM = valid && l == QHG_LIFE_FERTILE && g == 1;
+                const uint64_t mF = __ballot(isF), mM = __ballot(isM);
+                if (isF) fem[F + __popcll(mF & lt)] = li;
+                if (isM) mal[M + __popcll(mM & lt)] = li;
+                F += __popcll(mF);
+                M += __popcll(mM);
+                if (valid) mate[li] = -1;
+            }
+            my_fert += (lane == 0) ? (uint32_t)(F + M) : 0u;
+            if (F > 0 && M > 0) {
+                // findMates, RandomPair.cpp:183-276: the smaller sex chooses in
+                // id order, each pick the k-th still unpaired of the other sex
+                const bool fsel = F <= M;
+                const int32_t *chs = fsel ? fem : mal;
+                const int32_t *pool = fsel ? mal : fem;
+                const int32_t nch = fsel ? F : M;
+                const int32_t np = fsel ? M : F;
+                for (int32_t j = lane; j < np; j += kWave) flag[j] = 1;
+                __syncthreads();
+                int32_t left = np;
+                for (int32_t cb = 0; cb < nch; cb += kWave) {
+                    const int32_t j = cb + lane;
+                    uint32_t u = 0;
+                    if (j < nch) u = draw(x.seed, sa.step, QHG_STREAM_PAIR, (uint64_t)x.cur.id[s + chs[j]], 0);
+                    const int32_t cnt = (nch - cb < kWave) ? (nch - cb) : kWave;
+                    for (int32_t q = 0; q < cnt && left > 0; q++) {
+                        const uint32_t uq = (uint32_t)__shfl((int)u, q, 64);
+                        int k = (int)wrandr(0, (double)left, uq);
+                        int32_t pos = -1;
+                        for (int32_t base = 0; base < np; base += kWave) {
+                            const bool f = base + lane < np && flag[base + lane];
+                            const uint64_t m = __ballot(f);
+                            const int pc = __popcll(m);
+                            if (k < pc) { pos = base + select_bit(m, k); break; }
+                            k -= pc;
+                        }
+                        if (pos < 0) {   // cannot happen while left counts the set flags
+                            if (lane == 0) atomicOr(&x.st->error, kErrGather);
+                            break;
+                        }
+                        const int32_t a = chs[cb + q], b = pool[pos];
+                        if (lane == 0) {
+                            flag[pos] = 0;
+                            mate[a] = b;
+                            mate[b] = a;
+                        }
+                        __syncthreads();
+                        left--;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+
+        // ---- the cell's tables, read once
+        CellView cv;
+        cv.N = (uint32_t)n;
+        cv.K = x.K[c];
+        cv.nn = x.nn[c];
+        int32_t nbr[6];
+        double w[7];
+#pragma unroll
+        for (int q = 0; q < 6; q++) nbr[q] = (q < x.conn) ? x.nbr[(int64_t)c * x.conn + q] : -1;
+#pragma unroll
+        for (int q = 0; q < 7; q++) w[q] = (q < R) ? x.W[(int64_t)c * R + q] : 0.0;
+
+        // ---- actions, 64 agents at a time
+        uint32_t stay = 0, births = 0;
+        uint32_t mv[6] = {0, 0, 0, 0, 0, 0};
+        for (int ch = 0; ch < nchunks; ch++) {
+            const int32_t li = ch * kWave + lane;
+            const bool valid = li < n;
+            const uint32_t slot = s + (uint32_t)li;
+            ActItem a;
+            a.life = 0;
+            a.g = 0;
+            a.id = 0;
+            a.mate = -1;
+            a.birth = a.lastb = a.age = 0;
+            a.age_dirty = false;
+            a.bflag = 0;
+            a.npend = 0;
+            a.pk0 = a.pk1 = 0;
+            a.pu0 = a.pu1 = 0;
+            if (valid) {
+                a.life = x.cur.life[slot];
+                a.id = x.cur.id[slot];
+                if (p.need_gender) a.g = x.cur.gender[slot];
+                if (p.need_birth) a.birth = x.cur.birth[slot];
+                if (p.need_lastb) a.lastb = x.cur.lastb[slot];
+                if (p.load_age) a.age = x.cur.age[slot];
+                if (p.has_pair) a.mate = mate[li];
+            }
+            const uint8_t life0 = a.life;
+            int dir = -1;
+            if (valid && a.life != 0) {
+                act_program(x, p, sa, cv, a, my_deaths);
+                if (a.npend > 0) {
+                    // every registered move counts; the last one wins
+                    int d = move_dir(x, a.pk0, a.pu0, cv.nn, w, nbr);
+                    if (d >= 0) { dir = d; my_moves++; }
+                    if (a.npend > 1) {
+                        d = move_dir(x, a.pk1, a.pu1, cv.nn, w, nbr);
+                        if (d >= 0) { dir = d; my_moves++; }
+                    }
+                }
+            }
+            const bool alive = valid && a.life != 0;
+            if (valid) {
+                uint8_t code = (uint8_t)(dir >= 0 ? dir + 1 : 0);
+                if (!alive) code |= kDecDead;
+                if (a.bflag) code |= kDecBirth;
+                x.dec[slot] = code;
+                if (a.life != life0) x.cur.life[slot] = a.life;
+                if (a.age_dirty) x.cur.age[slot] = a.age;
+                if (p.has_pair) x.cur.mate_id[slot] = (a.mate >= 0) ? x.cur.id[s + a.mate] : -3;
+                if (a.bflag) {
+                    x.cur.nbabies[slot] += 1;
+                    x.bfather[slot] = (int32_t)(s + a.mate);
+                }
+            }
+            stay += __popcll(__ballot(alive && dir < 0));
+            births += __popcll(__ballot(valid && a.bflag));
+#pragma unroll
+            for (int d = 0; d < 6; d++) mv[d] += __popcll(__ballot(alive && dir == d));
+        }
+        if (lane == 0) { x.cstay[c] = stay; x.cbirth[c] = births; }
+#pragma unroll
+        for (int d = 0; d < 6; d++)
+            if (lane == d && d < x.conn) x.cmv[(int64_t)c * x.conn + d] = mv[d];
+        __syncthreads();
+    }
+    const uint64_t dm = wave_sum_u64(((uint64_t)my_moves << 32) | my_deaths);
+    const uint32_t fert = (uint32_t)wave_sum_u64(my_fert);
+    if (lane == 0) {
+        x.wave_part[3 * blockIdx.x + 0] = (uint32_t)dm;
+        x.wave_part[3 * blockIdx.x + 1] = (uint32_t)(dm >> 32);
+        x.wave_part[3 * blockIdx.x + 2] = fert;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_cell_scan: new per-cell counts and birth counts, scanned together
+// (low 32 bits agents, high bits births) with a decoupled look-back over
+// tiles of kCellTile cells.  One tile per block; the grid is exactly the
+// number of tiles and small enough to be resident at once.  The block of
+// the last tile closes the step's bookkeeping in the other status half.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kScanBlock) k_cell_scan(Ctx x, StepArgs sa) {
+    __shared__ uint64_t s_w[kScanBlock / 64];
+    __shared__ uint64_t s_excl;
+    __shared__ uint64_t s_red[3][kScanBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int per = kCellTile / kScanBlock;
+    const int64_t nc = x.ncells;
+    const uint32_t tile = blockIdx.x;
+    if (x.st->error) return;
+    const int64_t base = (int64_t)tile * kCellTile;
+    uint64_t v[per];
+    uint64_t sum = 0;
+#pragma unroll
+    for (int q = 0; q < per; q++) {
+        const int64_t c = base + (int64_t)threadIdx.x * per + q;
+        uint64_t val = 0;
+        if (c < nc) {
+            uint32_t cnt = x.cstay[c];
+            const uint32_t b = x.cbirth[c];
+            for (int32_t k = x.in_off[c]; k < x.in_off[c + 1]; k++)
+                cnt += x.cmv[(int64_t)x.in_src[k] * x.conn + x.in_dir[k]];
+            val = (uint64_t)(cnt + b) | ((uint64_t)b << 32);
+        }
+        v[q] = val;
+        sum += val;
+    }
+    // block exclusive scan of the per-thread sums
+    const uint64_t incl = wave_incl_scan_u64(sum);
+    if (lane == 63) s_w[wave] = incl;
+    __syncthreads();
+    uint64_t wpre = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kScanBlock / 64; w++) {
+        if (w < wave) wpre += s_w[w];
+        tot += s_w[w];
+    }
+    if (threadIdx.x < 64) {
+        const uint64_t ex = lookback(x.tiles, tile, tot, &x.st->error);
+        if (threadIdx.x == 0) s_excl = ex;
+    }
+    __syncthreads();
+    uint64_t run = s_excl + wpre + incl - sum;
+#pragma unroll
+    for (int q = 0; q < per; q++) {
+        const int64_t c = base + (int64_t)threadIdx.x * per + q;
+        if (c < nc) {
+            x.off_nxt[c] = (uint32_t)run;
+            x.birth_off[c] = (uint32_t)(run >> 32);
+        }
+        run += v[q];
+    }
+    if (base + kCellTile >= nc) {
+        // last tile: step totals (the k_cell_act partials) and bookkeeping
+        uint64_t dsum = 0, msum = 0, fsum = 0;
+        for (int j = threadIdx.x; j < x.nact_blocks; j += kScanBlock) {
+            dsum += x.wave_part[3 * j + 0];
+            msum += x.wave_part[3 * j + 1];
+            fsum += x.wave_part[3 * j + 2];
+        }
+        dsum = wave_sum_u64(dsum);
+        msum = wave_sum_u64(msum);
+        fsum = wave_sum_u64(fsum);
+        if (lane == 0) { s_red[0][wave] = dsum; s_red[1][wave] = msum; s_red[2][wave] = fsum; }
+        __syncthreads();
+        if (threadIdx.x == kScanBlock - 1) {
+            uint64_t D = 0, Mv = 0, Fe = 0;
+            for (int w = 0; w < kScanBlock / 64; w++) { D += s_red[0][w]; Mv += s_red[1][w]; Fe += s_red[2][w]; }
+            const int64_t total = (int64_t)(uint32_t)run;
+            const int64_t B = (int64_t)(run >> 32);
+            x.off_nxt[nc] = (uint32_t)total;
+            x.birth_off[nc] = (uint32_t)B;
+            DevStatus *st = x.st;
+            const int p = (int)sa.parity, q = 1 - p;
+            if (total > x.cap || (x.gwords && st->gpool[p] + B > x.gcap)) st->error |= kErrCapacity;
+            st->agent_steps += st->n_alive[p];
+            st->n_alive[q] = total;
+            st->max_id[q] = st->max_id[p] + B;
+            st->gpool[q] = st->gpool[p] + (x.gwords ? B : 0);
+            st->births_last = B;
+            st->deaths_last = (int64_t)D;
+            st->moves_last = (int64_t)Mv;
+            st->births_total += B;
+            st->deaths_total += (int64_t)D;
+            st->moves_total += (int64_t)Mv;
+            st->fertile_total += (int64_t)Fe;
+            st->step = (int64_t)sa.step + 1;
+            st->t = sa.t + 1.0;
+            if (x.hring) {
+                DevStatus *dst = x.hring + (st->step % kStatusRing);
+                const uint64_t *srcw = reinterpret_cast<const uint64_t *>(st);
+                uint64_t *dstw = reinterpret_cast<uint64_t *>(dst);
+                for (int k = 0; k < (int)(sizeof(DevStatus) / 8); k++) dstw[k] = srcw[k];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// genetics helpers: GeneUtils::makeMultiMask / crossOver (GeneUtils.cpp:156-246)
+// ---------------------------------------------------------------------------
+__device__ uint64_t multimask_block(const uint32_t *pos, int n, uint32_t blk) {
+    // breaks of this block, in draw order, as makeMultiMask sees them
+    int cnt = 0;
+    for (int q = 0; q < n; q++) cnt += (pos[q] >> 6) == blk;
+    uint32_t k = 1 - (uint32_t)(cnt % 2);
+    uint64_t out = 0;
+    int i = n - 1;
+    while (i >= 0 && (pos[i] >> 6) != blk) i--;
+    for (uint32_t j = 0; j < 64; j++) {
+        if (i >= 0 && j == 64 - (pos[i] & 63u)) {
+            i--;
+            while (i >= 0 && (pos[i] >> 6) != blk) i--;
+            k = 1 - k;
+        }
+        out = (out << 1) + k;
+    }
+    return out;
+}
+
+// strand `sel` of crossOver(parent) -> out[0..nb)
+__device__ void crossover_strand(uint64_t *out, const uint64_t *par, int nb, const uint32_t *pos, int n, int sel) {
+    for (int b = 0; b < nb; b++) {
+        int before = 0, in = 0;
+        for (int q = 0; q < n; q++) {
+            const uint32_t blk = pos[q] >> 6;
+            before += (int)blk < b;
+            in += (int)blk == b;
+        }
+        const int cur = before & 1;
+        const uint64_t gc = par[cur * nb + b], go = par[(1 - cur) * nb + b];
+        const uint64_t L = in ? multimask_block(pos, n, (uint32_t)b) : ~0ull;
+        out[b] = (sel == 0) ? ((L & gc) | (~L & go)) : ((L & go) | (~L & gc));
+    }
+}
+
+// Genetics::makeOffspring (Genetics.cpp:285-337) into pool handle gh
+__device__ void make_genome(const Ctx &x, const Prog &p, uint32_t step, uint64_t nid, int64_t gh, int64_t gm,
+                            int64_t gf) {
+    const int nbk = x.gwords / 2;
+    const uint64_t *g1 = x.gpool + gm * x.gwords;
+    const uint64_t *g2 = x.gpool + gf * x.gwords;
+    uint64_t *out = x.gpool + gh * x.gwords;
+    uint32_t n = 0;
+#define GD() draw(x.seed, step, QHG_STREAM_GENETICS, nid, n++)
+    const int i1 = (int)(2 * 1.0 * wrandd(GD()));
+    const int i2 = (int)(2 * 1.0 * wrandd(GD()));
+    const int C = p.gen_cross;
+    if (C > 0) {
+        uint32_t pos[kMaxCross];
+        const uint32_t nbits = (uint32_t)nbk * 64u;
+        for (int q = 0; q < C; q++) pos[q] = wrandi(0, nbits, 2, GD());
+        crossover_strand(out, g1, nbk, pos, C, i1);
+        for (int q = 0; q < C; q++) pos[q] = wrandi(0, nbits, 2, GD());
+        crossover_strand(out + nbk, g2, nbk, pos, C, i2);
+    } else if (C == -1) {
+        // freeReco + makeFreeMask, GeneUtils.cpp:322-360
+        for (int par = 0; par < 2; par++) {
+            const uint64_t *gp = par ? g2 : g1;
+            const int sel = par ? i2 : i1;
+            for (int b = 0; b < nbk; b++) {
+                const uint32_t hi = GD();
+                const uint32_t lo = GD();
+                uint64_t v = ((uint64_t)hi << 32) + lo;
+                v &= 0x5555555555555555ull;
+                v += v << 1;
+                const uint64_t a0 = gp[b], a1 = gp[nbk + b];
+                out[par * nbk + b] = (sel == 0) ? ((v & a0) | (~v & a1)) : ((v & a1) | (~v & a0));
+            }
+        }
+    } else {
+        for (int b = 0; b < nbk; b++) out[b] = g1[i1 * nbk + b];
+        for (int b = 0; b < nbk; b++) out[nbk + b] = g2[i2 * nbk + b];
+    }
+    if (p.nbino > 0) {
+        // BinomialDist::getN then GeneUtils::mutateNucs (GeneUtils.cpp:112-141)
+        const double um = wrandd(GD());
+        int k = 0;
+        while (k < p.nbino && um > p.bino[k]) k++;
+        const uint32_t nbits_all = 2u * 2u * (uint32_t)p.gen_size;
+        for (int j = 0; j < k; j++) {
+            const uint32_t bp = wrandi(0, nbits_all, 2, GD());
+            const uint64_t mk = (uint64_t)wrandi(1, 4, 1, GD());
+            out[bp >> 6] ^= mk << (bp & 63u);
+        }
+    }
+#undef GD
+}
+
+// ---------------------------------------------------------------------------
+// k_cell_scatter: next buffer segment of one destination cell
+// ---------------------------------------------------------------------------
+// Append the slots of [s, s+n) whose outcome byte satisfies want(code) to
+// list[cnt..] (at most lim entries in total); returns the new count.
+template <typename Want>
+__device__ __forceinline__ int32_t gather_slots(const Ctx &x, uint32_t s, int32_t n, int32_t *list, int32_t cnt,
+                                                int32_t lim, Want want) {
+    const int lane = threadIdx.x;
+    const uint64_t lt = lanemask_lt();
+    for (int32_t b = 0; b < n; b += kWave) {
+        const int32_t li = b + lane;
+        const bool take = li < n && want(x.dec[s + li]);
+        const uint64_t m = __ballot(take);
+        if (take) {
+            const int32_t at = cnt + __popcll(m & lt);
+            if (at < lim) list[at] = (int32_t)(s + li);
+        }
+        cnt += __popcll(m);
+    }
+    return cnt;
+}
+
+// number of entries of the id-sorted run v[lo, hi) below key
+__device__ __forceinline__ int32_t count_below(const int64_t *v, int32_t lo, int32_t hi, int64_t key) {
+    const int32_t base = lo;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (v[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo - base;
+}
+
+__global__ void __launch_bounds__(kCellBlock) k_cell_scatter(Ctx x, Prog p, StepArgs sa) {
+    __shared__ int32_t s_cand[kLdsCell];
+    __shared__ int64_t s_cid[kLdsCell];
+    __shared__ int32_t s_run[kMaxRuns + 1];
+    const int lane = threadIdx.x;
+    const uint64_t lt = lanemask_lt();
+    const int pr = (int)sa.parity;
+    if (x.st->error) return;
+    const int64_t max_id = x.st->max_id[pr];
+    const int64_t gbase = x.st->gpool[pr];
+
+    for (int32_t c = blockIdx.x; c < x.ncells; c += gridDim.x) {
+        const uint32_t ns = x.off_nxt[c];
+        const int32_t nnew = (int32_t)(x.off_nxt[c + 1] - ns);
+        if (nnew == 0) continue;
+        const int32_t nb = (int32_t)x.cbirth[c];
+        const int32_t m = nnew - nb;     // stayers + arriving movers
+        const uint32_t s = x.off_cur[c];
+        const int32_t n = (int32_t)(x.off_cur[c + 1] - s);
+        const bool big = m > kLdsCell;
+        int32_t *cand = big ? x.gcand + ns : s_cand;
+        int64_t *cid = big ? x.gcid + ns : s_cid;
+        // runs: the stayers, then the movers from each incoming neighbour
+        // slot; every run is a subsequence of an id-sorted segment
+        int32_t cnt = gather_slots(x, s, n, cand, 0, m, [](uint8_t d) {
+            return !(d & kDecDead) && (d & kDecDirMask) == 0;
+        });
+        int32_t nrun = 1;
+        if (lane == 0) { s_run[0] = 0; s_run[1] = cnt; }
+        for (int32_t k = x.in_off[c]; k < x.in_off[c + 1]; k++) {
+            const int32_t src = x.in_src[k];
+            const uint8_t want_dir = (uint8_t)(x.in_dir[k] + 1);
+            const uint32_t ss = x.off_cur[src];
+            const int32_t sn = (int32_t)(x.off_cur[src + 1] - ss);
+            const int32_t before = cnt;
+            cnt = gather_slots(x, ss, sn, cand, cnt, m, [want_dir](uint8_t d) {
+                return !(d & kDecDead) && (d & kDecDirMask) == want_dir;
+            });
+            if (cnt > before) {
+                nrun++;
+                if (lane == 0 && nrun <= kMaxRuns) s_run[nrun] = cnt;
+            }
+        }
+        if (cnt != m || nrun > kMaxRuns) {
+            if (lane == 0) atomicOr(&x.st->error, kErrGather);
+            continue;
+        }
+        __syncthreads();
+        for (int32_t i = lane; i < cnt; i += kWave) cid[i] = x.cur.id[cand[i]];
+        __syncthreads();
+        // rank by id (unique): own position in its run + entries below it in
+        // every other run; then copy the record
+        for (int32_t i = lane; i < cnt; i += kWave) {
+            const int64_t my = cid[i];
+            int32_t r = 0;
+            for (int32_t q = 0; q < nrun; q++) {
+                const int32_t lo = s_run[q], hi = s_run[q + 1];
+                r += (i >= lo && i < hi) ? (i - lo) : count_below(cid, lo, hi, my);
+            }
+            const int32_t from = cand[i];
+            const int64_t to = (int64_t)ns + r;
+            x.nxt.id[to] = my;
+            x.nxt.mate_id[to] = x.cur.mate_id[from];
+            x.nxt.life[to] = x.cur.life[from];
+            x.nxt.gender[to] = x.cur.gender[from];
+            x.nxt.birth[to] = x.cur.birth[from];
+            x.nxt.age[to] = x.cur.age[from];
+            x.nxt.lastb[to] = x.cur.lastb[from];
+            x.nxt.nbabies[to] = x.cur.nbabies[from];
+            if (x.gwords) x.nxt.ghandle[to] = x.cur.ghandle[from];
+        }
+        // newborns of the cell's mothers, in mother (= id) order, after the
+        // existing agents (their ids are larger than every existing id)
+        int32_t rb = 0;
+        const int64_t boff = x.birth_off[c];
+        for (int32_t b = 0; b < n; b += kWave) {
+            const int32_t li = b + lane;
+            const bool isb = li < n && (x.dec[s + li] & kDecBirth);
+            const uint64_t mb = __ballot(isb);
+            if (isb) {
+                const int32_t r = rb + __popcll(mb & lt);
+                const int64_t to = (int64_t)ns + m + r;
+                const int64_t rg = boff + r;
+                const int64_t nid = max_id + 1 + rg;
+                // createAgentAtIndex + makePopSpecificOffspring (SPopulation.cpp:880-918)
+                x.nxt.id[to] = nid;
+                x.nxt.mate_id[to] = -3;
+                x.nxt.birth[to] = sa.t;
+                x.nxt.age[to] = 0.0;
+                x.nxt.lastb[to] = 0.0;
+                x.nxt.nbabies[to] = 0;
+                const uint32_t u = draw(x.seed, sa.step, QHG_STREAM_NEWBORN, (uint64_t)nid, 0);
+                const uint8_t g = (uint8_t)(2 * wrandd(u));
+                x.nxt.gender[to] = g;
+                x.nxt.life[to] = (g == 0) ? QHG_LIFE_FERTILE : QHG_LIFE_ALIVE;
+                if (x.gwords) {
+                    const int64_t gh = gbase + rg;
+                    x.nxt.ghandle[to] = (int32_t)gh;
+                    const int32_t mslot = (int32_t)(s + li);
+                    make_genome(x, p, sa.step, (uint64_t)nid, gh, x.cur.ghandle[mslot],
+                                x.cur.ghandle[x.bfather[mslot]]);
+                }
+            }
+            rb += __popcll(mb);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// genome pool compaction: the live agents' genomes to handles 0..n-1 in slot
+// order (into a second pool); the handles are rewritten
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_gpool_compact(Ctx x, uint64_t *dst, int32_t parity) {
+    const int64_t n = x.st->n_alive[parity];
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int64_t h = x.cur.ghandle[i];
+        const uint64_t *src = x.gpool + h * x.gwords;
+        uint64_t *d = dst + i * x.gwords;
+        for (int q = 0; q < x.gwords; q++) d[q] = src[q];
+        x.cur.ghandle[i] = (int32_t)i;
+    }
+}
+
+__global__ void k_gpool_commit(DevStatus *st, int32_t parity) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) st->gpool[parity] = st->n_alive[parity];
+}
+
+}  // namespace qhg

@@ -1,10 +1,12 @@
 // Host side of the C ABI declared in include/qhg_gpu.h.
 //
 // A qhg_pop owns every device buffer of one population and one HIP stream
-// (default priority).  The per-step entry points only enqueue kernels; the
-// host learns the population size from a status block copied back
-// asynchronously, and keeps the capacity ahead of the worst-case growth
-// (births <= live/2 per step) without a per-step synchronisation.
+// (default priority).  Agents live cell-sorted in one of two buffers that
+// swap every step (the current buffer of step s is buf[s & 1]); the per-step
+// entry points only enqueue kernels.  The host learns the population size
+// from status snapshots written into pinned memory by the device, and keeps
+// the capacity ahead of the worst-case growth (births <= live/2 per step)
+// without a per-step synchronisation.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,7 +21,7 @@
 #include "../../include/qhg_gpu.h"
 #include "aux_kernels.hpp"
 #include "bino.hpp"
-#include "step_kernels.hpp"
+#include "cell_kernels.hpp"
 
 using namespace qhg;
 
@@ -39,20 +41,36 @@ int fail(int code, const std::string &msg) {
             return fail(QHG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-enum KernelId { KB_BUCKET, KB_PAIR, KB_ACT, KB_NEWBORN, KB_CELLSCAN, KB_HIST, KB_CAPACITY,
-                KB_EVAL, KB_COMPACT, KB_COUNT };
-const char *kKernelNames[KB_COUNT] = {"k_bucket", "k_pair", "k_act", "k_newborn", "k_cellscan",
-                                      "k_hist", "k_capacity", "k_eval", "k_compact"};
+enum KernelId { KB_ACT, KB_SCAN, KB_SCATTER, KB_CAPACITY, KB_EVAL, KB_GPOOL, KB_COUNT };
+const char *kKernelNames[KB_COUNT] = {"k_cell_act", "k_cell_scan", "k_cell_scatter",
+                                      "k_capacity", "k_eval", "k_gpool_compact"};
 
 constexpr int kRing = kStatusRing;
-#ifndef QHG_ACT_GRID_PER_CU
-#define QHG_ACT_GRID_PER_CU 8   // k_act blocks per compute unit
+#ifndef QHG_CELL_GRID_PER_CU
+#define QHG_CELL_GRID_PER_CU 16   // k_cell_act / k_cell_scatter blocks per compute unit
 #endif
 
-struct AgentBufs {
-    AgentPtrs p{};
-    int64_t cap = 0;
-};
+void free_agentbuf(AgentBuf &a) {
+    void *ptrs[] = {a.id, a.mate_id, a.life, a.gender, a.birth, a.age, a.lastb, a.nbabies, a.ghandle};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    a = AgentBuf{};
+}
+
+int alloc_agentbuf(AgentBuf &a, int64_t cap, bool genomes) {
+    a = AgentBuf{};
+    const size_t n = (size_t)cap;
+    HIPCHK(hipMalloc(&a.id, n * 8));
+    HIPCHK(hipMalloc(&a.mate_id, n * 8));
+    HIPCHK(hipMalloc(&a.life, n));
+    HIPCHK(hipMalloc(&a.gender, n));
+    HIPCHK(hipMalloc(&a.birth, n * 8));
+    HIPCHK(hipMalloc(&a.age, n * 8));
+    HIPCHK(hipMalloc(&a.lastb, n * 8));
+    HIPCHK(hipMalloc(&a.nbabies, n * 4));
+    if (genomes) HIPCHK(hipMalloc(&a.ghandle, n * 4));
+    return 0;
+}
 
 }  // namespace
 
@@ -63,44 +81,50 @@ struct qhg_pop {
     hipStream_t stream = nullptr;
     int device = 0;
     int num_cu = 256;
+    int act_grid = 1, scatter_grid = 1, ntiles = 1;
 
-    // device buffers
-    AgentBufs bufs[2];
-    int cur = 0;
+    // agents: two cell-sorted buffers of cap slots, segment offsets per buffer
+    AgentBuf buf[2];
+    int64_t cap = 0;
+    uint32_t *d_off[2] = {nullptr, nullptr};
+    // per-slot step scratch (cap entries)
+    uint8_t *d_dec = nullptr, *d_gflag = nullptr;
+    int32_t *d_bfather = nullptr, *d_gfem = nullptr, *d_gmal = nullptr, *d_gmate = nullptr, *d_gcand = nullptr;
+    int64_t *d_gcid = nullptr;
+    // genome pool (gcap handles), a second pool for compaction
+    uint64_t *d_gpool[2] = {nullptr, nullptr};
+    int gp_cur = 0;
+    int64_t gcap = 0;
+    // grid and per-cell tables
     int32_t *d_nbr = nullptr;
     uint8_t *d_nn = nullptr;
+    int32_t *d_in_off = nullptr, *d_in_src = nullptr;
+    uint8_t *d_in_dir = nullptr;
     double *d_K = nullptr, *d_W = nullptr, *d_tmp = nullptr;
-    uint8_t *d_ice = nullptr;
-    uint32_t *d_cnt[2] = {nullptr, nullptr};
-    uint32_t *d_off = nullptr, *d_fillF = nullptr, *d_fillM = nullptr;
-    uint64_t *d_bucket = nullptr;
-    int32_t *d_brec = nullptr;
-    int32_t *d_newidx = nullptr;
-    uint32_t *d_tile_cnt = nullptr, *d_tile_off = nullptr, *d_super = nullptr, *d_blk_fert = nullptr;
-    uint64_t *d_tiles_cell = nullptr;
-    int64_t *d_hw_next = nullptr;
-    DevStatus *d_st = nullptr;
-    // env on device
+    uint8_t *d_ice = nullptr, *d_coastal = nullptr;
     double *d_env[7] = {};   // alt lon lat water npp temp rain
-    uint8_t *d_coastal = nullptr;
-    std::vector<int64_t> cell_ids;
+    uint32_t *d_cstay = nullptr, *d_cbirth = nullptr, *d_cmv = nullptr, *d_birth_off = nullptr;
+    uint64_t *d_tiles = nullptr;
+    uint32_t *d_wave_part = nullptr;
+    DevStatus *d_st = nullptr;
 
     Prog prog{};
     bool has_pair = false, has_me = false, has_cap = false;
     bool me_done = false;
-    bool acted = false;       // k_act launched for the current step
-    bool initialized = false; // initialize phase launched for the current step
+    bool acted = false;       // k_cell_act launched for the current step
+    bool initialized = false; // initialize phase done for the current step
+    double t_next = 0.0;      // time of the next step
 
     // asynchronous status readback ring
-    DevStatus *h_ring = nullptr;     // pinned, kRing entries, written by k_cellscan
+    DevStatus *h_ring = nullptr;     // pinned, kRing entries, written by k_cell_scan
     DevStatus *d_ring = nullptr;     // device view of h_ring
     hipEvent_t ring_ev[kRing] = {};
     int64_t ring_step[kRing];
-    int64_t steps_enqueued = 0;      // step index of the next step to enqueue
+    int64_t steps_enqueued = 0;      // index of the next step to enqueue
     DevStatus known{};               // newest status known on the host
-    int64_t known_step = -1;         // status after step known_step-1 ... (= value of st.step)
+    int64_t known_step = 0;          // its step field: the step it precedes
+    int64_t pool_reset_step = -1;    // a pool compaction was enqueued before this step
     int64_t compactions = 0;
-    int64_t last_compact_step = -1;
 
     // timing
     bool timing = false;
@@ -109,11 +133,16 @@ struct qhg_pop {
     double t_sum[KB_COUNT] = {};
     int64_t t_cnt[KB_COUNT] = {};
 
+    int cur() const { return (int)(steps_enqueued & 1); }
+
     Ctx ctx() const {
+        const int b = cur();
         Ctx x{};
-        x.a = bufs[cur].p;
-        x.cap = bufs[cur].cap;
-        x.gwords = gwords;
+        x.cur = buf[b];
+        x.nxt = buf[1 - b];
+        x.off_cur = d_off[b];
+        x.off_nxt = d_off[1 - b];
+        x.cap = cap;
         x.ncells = ncells;
         x.conn = conn;
         x.nbr = d_nbr;
@@ -121,43 +150,50 @@ struct qhg_pop {
         x.K = d_K;
         x.W = d_W;
         x.ice = d_ice;
-        x.cnt[0] = d_cnt[0];
-        x.cnt[1] = d_cnt[1];
-        x.off = d_off;
-        x.fillF = d_fillF;
-        x.fillM = d_fillM;
-        x.bucket = d_bucket;
-        x.brec = d_brec;
-        x.tile_cnt = d_tile_cnt;
-        x.super_cnt = d_super;
-        x.blk_fert = d_blk_fert;
-        x.tiles_cell = d_tiles_cell;
-        x.ntiles_act_max = bufs[cur].cap / kActTile + 2;
+        x.in_off = d_in_off;
+        x.in_src = d_in_src;
+        x.in_dir = d_in_dir;
+        x.dec = d_dec;
+        x.bfather = d_bfather;
+        x.gfem = d_gfem;
+        x.gmal = d_gmal;
+        x.gmate = d_gmate;
+        x.gflag = d_gflag;
+        x.gcand = d_gcand;
+        x.gcid = d_gcid;
+        x.cstay = d_cstay;
+        x.cbirth = d_cbirth;
+        x.cmv = d_cmv;
+        x.birth_off = d_birth_off;
+        x.tiles = d_tiles;
+        x.ntiles = ntiles;
+        x.nact_blocks = act_grid;
+        x.wave_part = d_wave_part;
+        x.gpool = d_gpool[gp_cur];
+        x.gwords = gwords;
+        x.gcap = gcap;
         x.st = d_st;
         x.hring = d_ring;
         x.seed = P.seed;
         return x;
     }
+
+    StepArgs step_args() const {
+        StepArgs a{};
+        a.step = (uint32_t)steps_enqueued;
+        a.parity = (uint32_t)cur();
+        a.t = t_next;
+        return a;
+    }
 };
 
 namespace {
-
-// k_cellscan runs one block per tile of kCellTile cells
-int cell_tiles(const qhg_pop *p) { return (int)((p->ncells + kCellTile - 1) / kCellTile); }
 
 int grid_for(int64_t n, int max_blocks) {
     int64_t b = (n + kBlock - 1) / kBlock;
     if (b < 1) b = 1;
     if (b > max_blocks) b = max_blocks;
     return (int)b;
-}
-
-template <typename... KArgs, typename... Args>
-int launch_bs(qhg_pop *p, int kid, void (*kern)(KArgs...), int grid, int block, Args... args);
-
-template <typename... KArgs, typename... Args>
-int launch(qhg_pop *p, int kid, void (*kern)(KArgs...), int grid, Args... args) {
-    return launch_bs(p, kid, kern, grid, kBlock, args...);
 }
 
 template <typename... KArgs, typename... Args>
@@ -182,6 +218,11 @@ int launch_bs(qhg_pop *p, int kid, void (*kern)(KArgs...), int grid, int block, 
     return 0;
 }
 
+template <typename... KArgs, typename... Args>
+int launch(qhg_pop *p, int kid, void (*kern)(KArgs...), int grid, Args... args) {
+    return launch_bs(p, kid, kern, grid, kBlock, args...);
+}
+
 int collect_timing(qhg_pop *p) {
     if (p->ev_used.empty()) return 0;
     HIPCHK(hipStreamSynchronize(p->stream));
@@ -195,48 +236,26 @@ int collect_timing(qhg_pop *p) {
     return 0;
 }
 
-void free_bufs(AgentBufs &b) {
-    AgentPtrs &a = b.p;
-    void *ptrs[] = {a.id, a.cell, a.life, a.gender, a.birth, a.age, a.lastb, a.mate, a.nbabies, a.genome};
+void free_slot_scratch(qhg_pop *p) {
+    void *ptrs[] = {p->d_dec, p->d_gflag, p->d_bfather, p->d_gfem, p->d_gmal, p->d_gmate, p->d_gcand, p->d_gcid};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
-    b = AgentBufs{};
+    p->d_dec = p->d_gflag = nullptr;
+    p->d_bfather = p->d_gfem = p->d_gmal = p->d_gmate = p->d_gcand = nullptr;
+    p->d_gcid = nullptr;
 }
 
-int alloc_bufs(AgentBufs &b, int64_t cap, int32_t gwords) {
-    AgentPtrs &a = b.p;
-    a = AgentPtrs{};
-    const size_t n = (size_t)cap;
-    HIPCHK(hipMalloc(&a.id, n * 8));
-    HIPCHK(hipMalloc(&a.cell, n * 4));
-    HIPCHK(hipMalloc(&a.life, n));
-    HIPCHK(hipMalloc(&a.gender, n));
-    HIPCHK(hipMalloc(&a.birth, n * 8));
-    HIPCHK(hipMalloc(&a.age, n * 8));
-    HIPCHK(hipMalloc(&a.lastb, n * 8));
-    HIPCHK(hipMalloc(&a.mate, n * 4));
-    HIPCHK(hipMalloc(&a.nbabies, n * 4));
-    if (gwords) HIPCHK(hipMalloc(&a.genome, n * (size_t)gwords * 8));
-    b.cap = cap;
-    return 0;
-}
-
-// per-slot scratch that scales with capacity
 int alloc_slot_scratch(qhg_pop *p, int64_t cap) {
-    if (p->d_bucket) (void)hipFree(p->d_bucket);
-    if (p->d_brec) (void)hipFree(p->d_brec);
-    if (p->d_newidx) (void)hipFree(p->d_newidx);
-    if (p->d_tile_cnt) (void)hipFree(p->d_tile_cnt);
-    if (p->d_tile_off) (void)hipFree(p->d_tile_off);
-    if (p->d_super) (void)hipFree(p->d_super);
-    const size_t nt = (size_t)(cap / kActTile + 2);
-    HIPCHK(hipMalloc(&p->d_bucket, (size_t)cap * 8));
-    HIPCHK(hipMalloc(&p->d_brec, ((size_t)cap + kActTile) * 12));
-    HIPCHK(hipMalloc(&p->d_newidx, (size_t)cap * 4));
-    HIPCHK(hipMalloc(&p->d_tile_cnt, nt * 12));
-    HIPCHK(hipMalloc(&p->d_tile_off, nt * 4));
-    HIPCHK(hipMalloc(&p->d_super, (nt / kSuperTiles + 2) * 12));
-    HIPCHK(hipMemsetAsync(p->d_super, 0, (nt / kSuperTiles + 2) * 12, p->stream));
+    free_slot_scratch(p);
+    const size_t n = (size_t)cap;
+    HIPCHK(hipMalloc(&p->d_dec, n));
+    HIPCHK(hipMalloc(&p->d_gflag, n));
+    HIPCHK(hipMalloc(&p->d_bfather, n * 4));
+    HIPCHK(hipMalloc(&p->d_gfem, n * 4));
+    HIPCHK(hipMalloc(&p->d_gmal, n * 4));
+    HIPCHK(hipMalloc(&p->d_gmate, n * 4));
+    HIPCHK(hipMalloc(&p->d_gcand, n * 4));
+    HIPCHK(hipMalloc(&p->d_gcid, n * 8));
     return 0;
 }
 
@@ -294,12 +313,10 @@ int build_prog(qhg_pop *p) {
         switch (g.ops[q]) {
         case QHG_ACT_GETOLD: g.need_birth = 1; break;
         case QHG_ACT_FERTILITY: g.need_gender = 1; g.need_lastb = 1; break;
-        case QHG_ACT_VERHULST: g.need_gender = 1; g.need_mate = 1; g.need_count = 1; g.need_act0 = 1; break;
-        case QHG_ACT_VERHULSTVARK:
-            g.need_gender = 1; g.need_mate = 1; g.need_count = 1; g.need_K = 1; g.need_act0 = 1;
-            break;
-        case QHG_ACT_RANDOMMOVE: g.need_move = 1; g.need_act1 = 1; break;
-        case QHG_ACT_WEIGHTEDMOVE: g.need_move = 1; g.need_act0 = 1; break;
+        case QHG_ACT_VERHULST:
+        case QHG_ACT_VERHULSTVARK: g.need_gender = 1; g.need_act0 = 1; break;
+        case QHG_ACT_RANDOMMOVE: g.need_act1 = 1; break;
+        case QHG_ACT_WEIGHTEDMOVE: g.need_act0 = 1; break;
         default: break;
         }
     }
@@ -361,68 +378,87 @@ int run_evaluators(qhg_pop *p) {
     return 0;
 }
 
-// counts of live agents per cell + offsets (pre-loop, upload, compaction)
-int recount(qhg_pop *p) {
-    Ctx x = p->ctx();
-    HIPCHK(hipMemsetAsync(p->d_cnt[0], 0, (size_t)p->ncells * 4, p->stream));
-    HIPCHK(hipMemsetAsync(p->d_cnt[1], 0, (size_t)p->ncells * 4, p->stream));
-    const size_t ntc = (size_t)(p->ncells / kCellTile + 2);
-    HIPCHK(hipMemsetAsync(p->d_tiles_cell, 0, ntc * 8, p->stream));
-    int rc = launch(p, KB_HIST, k_hist, 2048, x);
-    if (rc) return rc;
-    return launch(p, KB_CELLSCAN, k_cellscan, cell_tiles(p), x, (int32_t)1);
-}
-
 int read_status_sync(qhg_pop *p, DevStatus *out) {
     HIPCHK(hipMemcpyAsync(out, p->d_st, sizeof(DevStatus), hipMemcpyDeviceToHost, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
     return 0;
 }
 
-int compact_into(qhg_pop *p, int64_t new_cap) {
-    // stable compaction into the other buffer set (reallocated when the
-    // capacity changes)
-    const int other = 1 - p->cur;
-    if (p->bufs[other].cap != new_cap) {
-        HIPCHK(hipStreamSynchronize(p->stream));
-        free_bufs(p->bufs[other]);
-        int rc = alloc_bufs(p->bufs[other], new_cap, p->gwords);
-        if (rc) return rc;
-    }
-    if (new_cap > p->bufs[p->cur].cap) {
-        HIPCHK(hipStreamSynchronize(p->stream));
-        int32_t *old_newidx = p->d_newidx;
-        p->d_newidx = nullptr;
-        // scratch for the bigger capacity; newidx needs the old size
-        int rc = alloc_slot_scratch(p, new_cap);
-        if (rc) return rc;
-        (void)old_newidx;
-        if (old_newidx) (void)hipFree(old_newidx);
-    }
-    Ctx x = p->ctx();
-    const int g = p->num_cu * 4;
-    int rc = launch(p, KB_COMPACT, k_compact_count, g, x, p->d_tile_cnt);
+int refresh_known(qhg_pop *p) {
+    DevStatus s;
+    int rc = read_status_sync(p, &s);
     if (rc) return rc;
-    rc = launch_bs(p, KB_COMPACT, k_scan_tiles, 1, 1024, (const uint32_t *)p->d_tile_cnt, p->d_tile_off,
-                   (const DevStatus *)p->d_st, p->d_hw_next);
-    if (rc) return rc;
-    rc = launch(p, KB_COMPACT, k_compact_scatter, g, x, p->bufs[other].p, p->d_newidx,
-                (const uint32_t *)p->d_tile_off);
-    if (rc) return rc;
-    rc = launch(p, KB_COMPACT, k_compact_mate, 2048, x, p->bufs[other].p, (const int32_t *)p->d_newidx);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_commit_compact, dim3(1), dim3(64), 0, p->stream, p->d_st, (const int64_t *)p->d_hw_next);
-    HIPCHK(hipGetLastError());
-    p->cur = other;
-    p->compactions++;
-    // the status now has hw == n_alive; the host-side knowledge stays an
-    // upper bound until the next readback
+    p->known = s;
+    p->known_step = p->steps_enqueued;
     return 0;
+}
+
+// Genome pool compaction in stream order: the genomes of the current
+// buffer's agents move to handles 0..n-1 of the other pool (reallocated to
+// new_gcap first when that differs).
+int compact_pool(qhg_pop *p, int64_t new_gcap) {
+    if (!p->gwords) return 0;
+    const int other = 1 - p->gp_cur;
+    if (new_gcap != p->gcap || !p->d_gpool[other]) {
+        HIPCHK(hipStreamSynchronize(p->stream));
+        if (p->d_gpool[other]) (void)hipFree(p->d_gpool[other]);
+        p->d_gpool[other] = nullptr;
+        HIPCHK(hipMalloc(&p->d_gpool[other], (size_t)new_gcap * p->gwords * 8));
+    }
+    const int32_t par = p->cur();
+    int rc = launch(p, KB_GPOOL, k_gpool_compact, p->num_cu * 4, p->ctx(), p->d_gpool[other], par);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_gpool_commit, dim3(1), dim3(64), 0, p->stream, p->d_st, par);
+    HIPCHK(hipGetLastError());
+    if (new_gcap != p->gcap) {
+        // the old pool is no longer referenced once the copy has run
+        HIPCHK(hipStreamSynchronize(p->stream));
+        (void)hipFree(p->d_gpool[p->gp_cur]);
+        p->d_gpool[p->gp_cur] = nullptr;
+        p->gcap = new_gcap;
+    }
+    p->gp_cur = other;
+    p->pool_reset_step = p->steps_enqueued;
+    p->compactions++;
+    return 0;
+}
+
+// Grow both agent buffers (and the per-slot scratch) to new_cap: the
+// current buffer's n live agents are copied over.  Needs an exact status.
+int grow_agents(qhg_pop *p, int64_t new_cap, int64_t n) {
+    HIPCHK(hipStreamSynchronize(p->stream));
+    const int b = p->cur();
+    AgentBuf nb[2];
+    for (int k = 0; k < 2; k++) {
+        int rc = alloc_agentbuf(nb[k], new_cap, p->gwords != 0);
+        if (rc) { free_agentbuf(nb[0]); free_agentbuf(nb[1]); return rc; }
+    }
+    const AgentBuf &o = p->buf[b];
+    hipStream_t s = p->stream;
+    const size_t m = (size_t)n;
+    if (m > 0) {
+        HIPCHK(hipMemcpyAsync(nb[b].id, o.id, m * 8, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(nb[b].mate_id, o.mate_id, m * 8, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(nb[b].life, o.life, m, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(nb[b].gender, o.gender, m, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(nb[b].birth, o.birth, m * 8, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(nb[b].age, o.age, m * 8, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(nb[b].lastb, o.lastb, m * 8, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(nb[b].nbabies, o.nbabies, m * 4, hipMemcpyDeviceToDevice, s));
+        if (p->gwords) HIPCHK(hipMemcpyAsync(nb[b].ghandle, o.ghandle, m * 4, hipMemcpyDeviceToDevice, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    free_agentbuf(p->buf[0]);
+    free_agentbuf(p->buf[1]);
+    p->buf[0] = nb[0];
+    p->buf[1] = nb[1];
+    p->cap = new_cap;
+    return alloc_slot_scratch(p, new_cap);
 }
 
 void poll_ring(qhg_pop *p) {
     // newest completed snapshot wins; its own step field says which step it
-    // belongs to (a slot may already hold a later step's snapshot)
+    // precedes (a slot may already hold a later step's snapshot)
     for (int k = 0; k < kRing; k++) {
         if (p->ring_step[k] < 0) continue;
         if (hipEventQuery(p->ring_ev[k]) != hipSuccess) continue;
@@ -436,7 +472,6 @@ void poll_ring(qhg_pop *p) {
     }
 }
 
-
 // oldest step whose snapshot the host has not consumed (or -1)
 int oldest_slot(const qhg_pop *p) {
     int best = -1;
@@ -445,73 +480,80 @@ int oldest_slot(const qhg_pop *p) {
     return best;
 }
 
-double hw_bound(const DevStatus &k, int64_t lag) {
-    // births per step <= live/2 (one per mated female), live grows <= x1.5
-    double hw = (double)k.hw, alive = (double)k.n_alive;
-    for (int64_t j = 0; j <= lag; j++) {
-        hw += std::ceil(alive / 2) + 1;
-        alive = alive * 1.5 + 1;
+// Upper bounds on the live count and the genome pool high-water mark at the
+// start of step steps_enqueued + extra, from the newest known status: every
+// step at most adds ceil(live/2) births (one per paired female).
+struct Bounds { double live, pool; };
+Bounds bounds(const qhg_pop *p, int extra) {
+    const int kb = (int)(p->known_step & 1);
+    double live = (double)p->known.n_alive[kb], pool = (double)p->known.gpool[kb];
+    for (int64_t s = p->known_step; s < p->steps_enqueued + extra; s++) {
+        if (s == p->pool_reset_step) pool = live;
+        const double b = std::ceil(live / 2);
+        live += b;
+        pool += b;
     }
-    return hw;
+    if (p->steps_enqueued + extra == p->pool_reset_step) pool = live;
+    return {live, pool};
 }
 
-// Keep hw + worst-case births of every step not yet observed below capacity
-// (waiting for the oldest in-flight snapshot when the bound is too loose),
-// grow the buffers when even the exact status needs it, and compact in
-// stream order when holes dominate.
+// Before enqueuing a step: make room for its worst case.  A genome pool
+// that may overflow is compacted in stream order first (afterwards it holds
+// exactly the live agents); while the bounds are too loose the host waits
+// for the oldest in-flight snapshot; with the exact status the buffers grow
+// when even that leaves no room for one step.
 int ensure_capacity(qhg_pop *p) {
     poll_ring(p);
-    const int64_t cap = p->bufs[p->cur].cap;
-    int64_t lag = std::max<int64_t>(0, p->steps_enqueued - p->known_step);
-    while ((hw_bound(p->known, lag) > (double)cap || lag > kRing - 2) && lag > 0) {
+    while (true) {
+        const int64_t lag = p->steps_enqueued - p->known_step;
+        const Bounds nx = bounds(p, 1);
+        const bool live_fits = nx.live <= (double)p->cap;
+        bool pool_fits = !p->gwords || nx.pool <= (double)p->gcap;
+        if (!pool_fits && nx.live <= (double)p->gcap && p->pool_reset_step != p->steps_enqueued) {
+            int rc = compact_pool(p, p->gcap);
+            if (rc) return rc;
+            continue;
+        }
+        if ((live_fits && pool_fits && lag <= kRing - 2) || lag <= 0) break;
         const int k = oldest_slot(p);
         if (k < 0) {
-            // nothing in flight: read the status directly
-            DevStatus s;
-            int rc = read_status_sync(p, &s);
+            int rc = refresh_known(p);
             if (rc) return rc;
-            p->known = s;
-            p->known_step = s.step;
         } else {
             HIPCHK(hipEventSynchronize(p->ring_ev[k]));
             poll_ring(p);
         }
-        lag = std::max<int64_t>(0, p->steps_enqueued - p->known_step);
     }
-    if (p->known.error) return fail(QHG_ERR_CAPACITY, "device reported an error (flags 0x" +
-                                    std::to_string(p->known.error) + ")");
-    if (hw_bound(p->known, 0) > (double)cap) {
-        // even the exact status does not leave room for one step
-        const DevStatus &s = p->known;
-        int64_t need = (int64_t)hw_bound(s, 4);
-        int64_t nc = cap;
-        if (2 * s.n_alive > cap || need > 2 * cap) {
-            while ((double)nc < 1.25 * (double)need) nc *= 2;
+    if (p->known.error)
+        return fail(QHG_ERR_STATE, "device reported an error (flags 0x" + std::to_string(p->known.error) + ")");
+    const int kb = (int)(p->known_step & 1);
+    if (bounds(p, 1).live > (double)p->cap) {
+        // exact status (lag 0) and still no room for one step: grow
+        const double need = bounds(p, 4).live;
+        int64_t nc = p->cap;
+        while ((double)nc < 1.25 * need) nc *= 2;
+        int rc = grow_agents(p, nc, p->known.n_alive[kb]);
+        if (rc) return rc;
+    }
+    if (p->gwords) {
+        const Bounds now = bounds(p, 0);
+        if (bounds(p, 1).pool > (double)p->gcap) {
+            // even a compacted pool is too small: compact into a larger one
+            const double need = bounds(p, 4).live;
+            int64_t ng = p->gcap;
+            while ((double)ng < 1.25 * need) ng *= 2;
+            int rc = compact_pool(p, ng);
+            if (rc) return rc;
+        } else if (now.pool > 2.0 * now.live + 65536.0) {
+            int rc = compact_pool(p, p->gcap);
+            if (rc) return rc;
         }
-        int rc = compact_into(p, nc);
-        if (rc) return rc;
-        p->last_compact_step = p->steps_enqueued;
-        DevStatus s2;
-        rc = read_status_sync(p, &s2);
-        if (rc) return rc;
-        p->known = s2;
-        p->known_step = s2.step;
-        if (hw_bound(s2, 0) > (double)p->bufs[p->cur].cap) return fail(QHG_ERR_CAPACITY, "capacity growth failed");
-        return 0;
-    }
-    // holes dominate: compact in stream order (no host sync); the known
-    // high-water mark stays a valid upper bound until the next snapshot
-    const DevStatus &k = p->known;
-    if (k.hw - k.n_alive > std::max<int64_t>(k.n_alive, 1 << 16) && p->known_step > p->last_compact_step) {
-        int rc = compact_into(p, cap);
-        if (rc) return rc;
-        p->last_compact_step = p->steps_enqueued;
     }
     return 0;
 }
 
 int enqueue_readback(qhg_pop *p) {
-    int slot = (int)(p->steps_enqueued % kRing);
+    const int slot = (int)(p->steps_enqueued % kRing);
     if (p->ring_step[slot] >= 0) {
         HIPCHK(hipEventSynchronize(p->ring_ev[slot]));
         poll_ring(p);
@@ -522,19 +564,13 @@ int enqueue_readback(qhg_pop *p) {
 }
 
 int phase_initialize(qhg_pop *p) {
+    int rc = ensure_capacity(p);
+    if (rc) return rc;
     if (p->has_me && !p->me_done) {
-        int rc = run_evaluators(p);
+        rc = run_evaluators(p);
         if (rc) return rc;
         p->me_done = true;
     }
-    Ctx x = p->ctx();
-    if (p->has_pair) {
-        int rc = launch(p, KB_BUCKET, k_bucket, kBucketBlocks, x);
-        if (rc) return rc;
-    }
-    const int pgrid = (int)std::min<int64_t>((p->ncells + kPairBlock - 1) / kPairBlock, (int64_t)p->num_cu * 16);
-    int rc = launch_bs(p, KB_PAIR, k_pair, std::max(pgrid, 1), kPairBlock, x, (int32_t)p->has_pair);
-    if (rc) return rc;
     p->initialized = true;
     return 0;
 }
@@ -544,8 +580,7 @@ int phase_act(qhg_pop *p) {
         int rc = phase_initialize(p);
         if (rc) return rc;
     }
-    Ctx x = p->ctx();
-    int rc = launch(p, KB_ACT, k_act, p->num_cu * QHG_ACT_GRID_PER_CU, x, p->prog);
+    int rc = launch_bs(p, KB_ACT, k_cell_act, p->act_grid, kCellBlock, p->ctx(), p->prog, p->step_args());
     if (rc) return rc;
     p->acted = true;
     return 0;
@@ -556,14 +591,16 @@ int phase_finalize(qhg_pop *p) {
         int rc = phase_act(p);
         if (rc) return rc;
     }
-    Ctx x = p->ctx();
-    int rc = launch(p, KB_NEWBORN, k_newborn, p->num_cu * 4, x, p->prog);
+    const Ctx x = p->ctx();
+    const StepArgs sa = p->step_args();
+    int rc = launch_bs(p, KB_SCAN, k_cell_scan, p->ntiles, kScanBlock, x, sa);
     if (rc) return rc;
-    rc = launch(p, KB_CELLSCAN, k_cellscan, cell_tiles(p), x, (int32_t)0);
+    rc = launch_bs(p, KB_SCATTER, k_cell_scatter, p->scatter_grid, kCellBlock, x, p->prog, sa);
     if (rc) return rc;
     p->acted = false;
     p->initialized = false;
     p->steps_enqueued++;
+    p->t_next += 1.0;
     return enqueue_readback(p);
 }
 
@@ -591,17 +628,43 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
     if (grid->num_cells <= 0 || grid->connectivity <= 0 || grid->connectivity > 6 || !grid->neighbors ||
         !grid->num_neighbors)
         return fail(QHG_ERR_ARG, "bad grid description");
-    for (int64_t c = 0; c < (int64_t)grid->num_cells * grid->connectivity; c++)
-        if (grid->neighbors[c] < -1 || grid->neighbors[c] >= grid->num_cells)
+    const int32_t nc32 = grid->num_cells, conn = grid->connectivity;
+    for (int64_t c = 0; c < (int64_t)nc32 * conn; c++)
+        if (grid->neighbors[c] < -1 || grid->neighbors[c] >= nc32)
             return fail(QHG_ERR_ARG, "neighbour index out of range");
-    for (int32_t c = 0; c < grid->num_cells; c++)
-        if (grid->num_neighbors[c] > grid->connectivity) return fail(QHG_ERR_ARG, "num_neighbors > connectivity");
+    for (int32_t c = 0; c < nc32; c++)
+        if (grid->num_neighbors[c] > conn) return fail(QHG_ERR_ARG, "num_neighbors > connectivity");
+    // incoming neighbour lists: (src, slot) for every slot a move can take
+    std::vector<int32_t> in_off((size_t)nc32 + 1, 0), in_src;
+    std::vector<uint8_t> in_dir;
+    for (int32_t s = 0; s < nc32; s++)
+        for (int d = 0; d < grid->num_neighbors[s]; d++) {
+            const int32_t c = grid->neighbors[(size_t)s * conn + d];
+            if (c >= 0) in_off[(size_t)c + 1]++;
+        }
+    for (int32_t c = 0; c < nc32; c++) {
+        if (in_off[(size_t)c + 1] > kMaxRuns - 1) return fail(QHG_ERR_ARG, "a cell has more than 14 incoming neighbour slots");
+        in_off[(size_t)c + 1] += in_off[c];
+    }
+    in_src.resize(std::max<size_t>(1, (size_t)in_off[nc32]));
+    in_dir.resize(in_src.size());
+    {
+        std::vector<int32_t> fill(in_off.begin(), in_off.end() - 1);
+        for (int32_t s = 0; s < nc32; s++)
+            for (int d = 0; d < grid->num_neighbors[s]; d++) {
+                const int32_t c = grid->neighbors[(size_t)s * conn + d];
+                if (c < 0) continue;
+                in_src[fill[c]] = s;
+                in_dir[fill[c]] = (uint8_t)d;
+                fill[c]++;
+            }
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(QHG_ERR_HIP, "no HIP device");
     qhg_pop *p = new qhg_pop();
     p->P = *params;
-    p->ncells = grid->num_cells;
-    p->conn = grid->connectivity;
+    p->ncells = nc32;
+    p->conn = conn;
     int rc = build_prog(p);
     if (rc) { delete p; return rc; }
     if (p->has_me || p->has_cap) {
@@ -612,44 +675,53 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, p->device) == hipSuccess && prop.multiProcessorCount > 0)
         p->num_cu = prop.multiProcessorCount;
+    p->ntiles = (int)((p->ncells + kCellTile - 1) / kCellTile);
     {
-        // the per-cell look-back scan needs all of its blocks resident at once
+        // the look-back scan needs all of its blocks resident at once
         int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cellscan, kBlock, 0) != hipSuccess || per_cu < 1)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cell_scan, kScanBlock, 0) != hipSuccess ||
+            per_cu < 1)
             per_cu = 1;
-        if ((int64_t)cell_tiles(p) > (int64_t)per_cu * p->num_cu)
+        if ((int64_t)p->ntiles > (int64_t)per_cu * p->num_cu)
             return cleanup_fail(fail(QHG_ERR_ARG, "grid too large: more cell tiles than resident blocks"));
     }
+    p->act_grid = (int)std::min<int64_t>(p->ncells, (int64_t)p->num_cu * QHG_CELL_GRID_PER_CU);
+    p->scatter_grid = p->act_grid;
     if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup_fail(fail(QHG_ERR_HIP, "stream"));
     const size_t nc = (size_t)p->ncells;
     if (capacity < 1024) capacity = 1024;
-    capacity = (capacity + kActTile - 1) / kActTile * kActTile;
+    capacity = (capacity + 255) / 256 * 256;
 #define CK(x) do { int r_ = (x); if (r_) return cleanup_fail(r_); } while (0)
-    CK(alloc_bufs(p->bufs[0], capacity, p->gwords));
-    CK(alloc_bufs(p->bufs[1], capacity, p->gwords));
+    p->cap = capacity;
+    CK(alloc_agentbuf(p->buf[0], capacity, p->gwords != 0));
+    CK(alloc_agentbuf(p->buf[1], capacity, p->gwords != 0));
     CK(alloc_slot_scratch(p, capacity));
+    if (p->gwords) {
+        p->gcap = capacity;
+        if (hipMalloc(&p->d_gpool[0], (size_t)p->gcap * p->gwords * 8) != hipSuccess)
+            return cleanup_fail(fail(QHG_ERR_NOMEM, "genome pool"));
+    }
     CK(upload_array(&p->d_nbr, grid->neighbors, nc * p->conn, p->stream));
     CK(upload_array(&p->d_nn, grid->num_neighbors, nc, p->stream));
-    p->cell_ids.resize(nc);
-    for (size_t c = 0; c < nc; c++) p->cell_ids[c] = grid->cell_ids ? grid->cell_ids[c] : (int64_t)c;
+    CK(upload_array(&p->d_in_off, in_off.data(), in_off.size(), p->stream));
+    CK(upload_array(&p->d_in_src, in_src.data(), in_src.size(), p->stream));
+    CK(upload_array(&p->d_in_dir, in_dir.data(), in_dir.size(), p->stream));
     if (hipMalloc(&p->d_K, nc * 8) != hipSuccess || hipMalloc(&p->d_W, nc * (p->conn + 1) * 8) != hipSuccess ||
-        hipMalloc(&p->d_tmp, nc * 8) != hipSuccess || hipMalloc(&p->d_cnt[0], nc * 4) != hipSuccess ||
-        hipMalloc(&p->d_cnt[1], nc * 4) != hipSuccess || hipMalloc(&p->d_off, (nc + 1) * 4) != hipSuccess ||
-        hipMalloc(&p->d_fillF, nc * 4) != hipSuccess || hipMalloc(&p->d_fillM, nc * 4) != hipSuccess ||
-        hipMalloc(&p->d_tiles_cell, (nc / kCellTile + 2) * 8) != hipSuccess ||
-        hipMalloc(&p->d_hw_next, 8) != hipSuccess || hipMalloc(&p->d_st, sizeof(DevStatus)) != hipSuccess ||
-        hipMalloc(&p->d_blk_fert, kBucketBlocks * 4) != hipSuccess)
+        hipMalloc(&p->d_tmp, nc * 8) != hipSuccess || hipMalloc(&p->d_off[0], (nc + 1) * 4) != hipSuccess ||
+        hipMalloc(&p->d_off[1], (nc + 1) * 4) != hipSuccess || hipMalloc(&p->d_cstay, nc * 4) != hipSuccess ||
+        hipMalloc(&p->d_cbirth, nc * 4) != hipSuccess || hipMalloc(&p->d_cmv, nc * p->conn * 4) != hipSuccess ||
+        hipMalloc(&p->d_birth_off, (nc + 1) * 4) != hipSuccess ||
+        hipMalloc(&p->d_tiles, (size_t)p->ntiles * 8) != hipSuccess ||
+        hipMalloc(&p->d_wave_part, (size_t)p->act_grid * 3 * 4) != hipSuccess ||
+        hipMalloc(&p->d_st, sizeof(DevStatus)) != hipSuccess)
         return cleanup_fail(fail(QHG_ERR_NOMEM, "device allocation failed"));
     (void)hipMemsetAsync(p->d_K, 0, nc * 8, p->stream);
     (void)hipMemsetAsync(p->d_W, 0, nc * (p->conn + 1) * 8, p->stream);
-    (void)hipMemsetAsync(p->d_cnt[0], 0, nc * 4, p->stream);
-    (void)hipMemsetAsync(p->d_cnt[1], 0, nc * 4, p->stream);
-    (void)hipMemsetAsync(p->d_off, 0, (nc + 1) * 4, p->stream);
-    (void)hipMemsetAsync(p->d_fillF, 0, nc * 4, p->stream);
-    (void)hipMemsetAsync(p->d_fillM, 0, nc * 4, p->stream);
+    (void)hipMemsetAsync(p->d_off[0], 0, (nc + 1) * 4, p->stream);
+    (void)hipMemsetAsync(p->d_off[1], 0, (nc + 1) * 4, p->stream);
+    (void)hipMemsetAsync(p->d_tiles, 0, (size_t)p->ntiles * 8, p->stream);
     (void)hipMemsetAsync(p->d_st, 0, sizeof(DevStatus), p->stream);
-    (void)hipMemsetAsync(p->d_blk_fert, 0, kBucketBlocks * 4, p->stream);
     if (env) {
         const double *src[7] = {env->altitude, env->longitude, env->latitude, env->water,
                                 env->base_npp, env->temperature, env->rainfall};
@@ -678,12 +750,12 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
 int qhg_gpu_destroy(qhg_pop *p) {
     if (!p) return 0;
     if (p->stream) (void)hipStreamSynchronize(p->stream);
-    free_bufs(p->bufs[0]);
-    free_bufs(p->bufs[1]);
-    void *ptrs[] = {p->d_nbr, p->d_nn, p->d_K, p->d_W, p->d_tmp, p->d_ice, p->d_cnt[0], p->d_cnt[1],
-                    p->d_off, p->d_fillF, p->d_fillM, p->d_bucket, p->d_brec, p->d_newidx,
-                    p->d_tile_cnt, p->d_tile_off, p->d_super, p->d_blk_fert, p->d_tiles_cell, p->d_hw_next, p->d_st,
-                    p->d_coastal};
+    free_agentbuf(p->buf[0]);
+    free_agentbuf(p->buf[1]);
+    free_slot_scratch(p);
+    void *ptrs[] = {p->d_off[0], p->d_off[1], p->d_gpool[0], p->d_gpool[1], p->d_nbr, p->d_nn, p->d_in_off,
+                    p->d_in_src, p->d_in_dir, p->d_K, p->d_W, p->d_tmp, p->d_ice, p->d_coastal, p->d_cstay,
+                    p->d_cbirth, p->d_cmv, p->d_birth_off, p->d_tiles, p->d_wave_part, p->d_st};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     for (int k = 0; k < 7; k++)
@@ -709,28 +781,49 @@ int qhg_gpu_upload_agents(qhg_pop *p, const qhg_agents *a) {
         if (a->id[i] < 0) return fail(QHG_ERR_ARG, "upload: negative agent id");
         order.push_back(i);
     }
-    std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return a->id[x] < a->id[y]; });
-    for (size_t k = 1; k < order.size(); k++)
-        if (a->id[order[k]] == a->id[order[k - 1]]) return fail(QHG_ERR_ARG, "upload: duplicate agent id");
+    // cell-sorted layout: by cell, then id
+    std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+        if (a->cell[x] != a->cell[y]) return a->cell[x] < a->cell[y];
+        return a->id[x] < a->id[y];
+    });
+    {
+        std::vector<int64_t> ids;
+        ids.reserve(order.size());
+        for (int64_t i : order) ids.push_back(a->id[i]);
+        std::sort(ids.begin(), ids.end());
+        for (size_t k = 1; k < ids.size(); k++)
+            if (ids[k] == ids[k - 1]) return fail(QHG_ERR_ARG, "upload: duplicate agent id");
+    }
     const int64_t n = (int64_t)order.size();
-    int64_t cap = p->bufs[p->cur].cap;
-    if (2 * n + 1024 > cap) {
-        int64_t nc = cap;
+    if (n > (int64_t)0x7fffffff / 2) return fail(QHG_ERR_ARG, "upload: too many agents");
+    HIPCHK(hipStreamSynchronize(p->stream));
+    if (2 * n + 1024 > p->cap) {
+        int64_t nc = p->cap;
         while (nc < 2 * n + 1024) nc *= 2;
-        HIPCHK(hipStreamSynchronize(p->stream));
-        free_bufs(p->bufs[0]);
-        free_bufs(p->bufs[1]);
-        int rc = alloc_bufs(p->bufs[0], nc, p->gwords);
+        free_agentbuf(p->buf[0]);
+        free_agentbuf(p->buf[1]);
+        int rc = alloc_agentbuf(p->buf[0], nc, p->gwords != 0);
         if (rc) return rc;
-        rc = alloc_bufs(p->bufs[1], nc, p->gwords);
+        rc = alloc_agentbuf(p->buf[1], nc, p->gwords != 0);
         if (rc) return rc;
         rc = alloc_slot_scratch(p, nc);
         if (rc) return rc;
-        p->cur = 0;
-        cap = nc;
+        p->cap = nc;
     }
-    std::vector<int64_t> id(n);
-    std::vector<int32_t> cell(n), mate(n, -3), nbab(n, 0);
+    if (p->gwords && 2 * n + 1024 > p->gcap) {
+        int64_t ng = p->gcap;
+        while (ng < 2 * n + 1024) ng *= 2;
+        for (int k = 0; k < 2; k++) {
+            if (p->d_gpool[k]) (void)hipFree(p->d_gpool[k]);
+            p->d_gpool[k] = nullptr;
+        }
+        HIPCHK(hipMalloc(&p->d_gpool[0], (size_t)ng * p->gwords * 8));
+        p->gp_cur = 0;
+        p->gcap = ng;
+    }
+    std::vector<int64_t> id(n), mate(n, -3);
+    std::vector<int32_t> nbab(n, 0), gh(n);
+    std::vector<uint32_t> off((size_t)p->ncells + 1, 0);
     std::vector<uint8_t> life(n), gender(n);
     std::vector<double> birth(n, 0.0), age(n, 0.0), lastb(n, 0.0);
     std::vector<uint64_t> genome((size_t)n * p->gwords);
@@ -738,46 +831,51 @@ int qhg_gpu_upload_agents(qhg_pop *p, const qhg_agents *a) {
     for (int64_t k = 0; k < n; k++) {
         const int64_t i = order[k];
         id[k] = a->id[i];
-        cell[k] = a->cell[i];
+        off[(size_t)a->cell[i] + 1]++;
         life[k] = a->life[i];
         gender[k] = a->gender[i];
         if (a->birth_time) birth[k] = a->birth_time[i];
         if (a->age) age[k] = a->age[i];
         if (a->last_birth) lastb[k] = a->last_birth[i];
         if (a->num_babies) nbab[k] = a->num_babies[i];
+        gh[k] = (int32_t)k;
         if (p->gwords)
             std::memcpy(&genome[(size_t)k * p->gwords], a->genome + (size_t)i * p->gwords, (size_t)p->gwords * 8);
         max_id = std::max(max_id, id[k]);
     }
-    const AgentPtrs &d = p->bufs[p->cur].p;
+    for (int32_t c = 0; c < p->ncells; c++) off[(size_t)c + 1] += off[c];
+    const int b = p->cur();
+    const AgentBuf &d = p->buf[b];
     hipStream_t s = p->stream;
+    HIPCHK(hipMemcpyAsync(p->d_off[b], off.data(), off.size() * 4, hipMemcpyHostToDevice, s));
     if (n > 0) {
-    HIPCHK(hipMemcpyAsync(d.id, id.data(), n * 8, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(d.cell, cell.data(), n * 4, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(d.life, life.data(), n, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(d.gender, gender.data(), n, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(d.birth, birth.data(), n * 8, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(d.age, age.data(), n * 8, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(d.lastb, lastb.data(), n * 8, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(d.mate, mate.data(), n * 4, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(d.nbabies, nbab.data(), n * 4, hipMemcpyHostToDevice, s));
-    if (p->gwords) HIPCHK(hipMemcpyAsync(d.genome, genome.data(), (size_t)n * p->gwords * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.id, id.data(), n * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.mate_id, mate.data(), n * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.life, life.data(), n, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.gender, gender.data(), n, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.birth, birth.data(), n * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.age, age.data(), n * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.lastb, lastb.data(), n * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.nbabies, nbab.data(), n * 4, hipMemcpyHostToDevice, s));
+        if (p->gwords) {
+            HIPCHK(hipMemcpyAsync(d.ghandle, gh.data(), n * 4, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(p->d_gpool[p->gp_cur], genome.data(), (size_t)n * p->gwords * 8,
+                                  hipMemcpyHostToDevice, s));
+        }
     }
     DevStatus st;
     HIPCHK(hipMemcpyAsync(&st, p->d_st, sizeof(st), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    st.hw = n;
-    st.n_alive = n;
-    st.max_id = max_id;
-    st.births_cur = st.deaths_cur = st.moves_cur = 0;
-    st.tile_ctr_act = st.tile_ctr_cell = st.tile_ctr_cmp = 0;
+    st.n_alive[b] = n;
+    st.max_id[b] = max_id;
+    st.gpool[b] = p->gwords ? n : 0;
+    st.step = p->steps_enqueued;
     st.error = 0;
     HIPCHK(hipMemcpyAsync(p->d_st, &st, sizeof(st), hipMemcpyHostToDevice, s));
-    int rc = recount(p);
-    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(s));
     p->known = st;
     p->known_step = p->steps_enqueued;
+    p->pool_reset_step = -1;
     return 0;
 }
 
@@ -786,7 +884,7 @@ int qhg_gpu_num_alive(qhg_pop *p, int64_t *n) {
     DevStatus s;
     int rc = read_status_sync(p, &s);
     if (rc) return rc;
-    *n = s.n_alive;
+    *n = s.n_alive[p->cur()];
     return 0;
 }
 
@@ -795,33 +893,47 @@ int qhg_gpu_download_agents(qhg_pop *p, qhg_agents *a) {
     DevStatus s;
     int rc = read_status_sync(p, &s);
     if (rc) return rc;
-    const int64_t hw = s.hw;
-    const AgentPtrs &d = p->bufs[p->cur].p;
-    std::vector<int64_t> id(hw);
-    std::vector<int32_t> cell(hw), mate(hw), nbab(hw);
-    std::vector<uint8_t> life(hw), gender(hw);
-    std::vector<double> birth(hw), age(hw), lastb(hw);
-    std::vector<uint64_t> genome;
+    const int b = p->cur();
+    const int64_t n = s.n_alive[b];
+    const int64_t np = p->gwords ? s.gpool[b] : 0;
+    const AgentBuf &d = p->buf[b];
+    std::vector<int64_t> id(n), mate(n);
+    std::vector<int32_t> nbab(n), gh(n);
+    std::vector<uint32_t> off((size_t)p->ncells + 1);
+    std::vector<uint8_t> life(n), gender(n);
+    std::vector<double> birth(n), age(n), lastb(n);
+    std::vector<uint64_t> pool;
     hipStream_t st = p->stream;
-    if (hw > 0) {
-        HIPCHK(hipMemcpyAsync(id.data(), d.id, hw * 8, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(cell.data(), d.cell, hw * 4, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(mate.data(), d.mate, hw * 4, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(nbab.data(), d.nbabies, hw * 4, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(life.data(), d.life, hw, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(gender.data(), d.gender, hw, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(birth.data(), d.birth, hw * 8, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(age.data(), d.age, hw * 8, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(lastb.data(), d.lastb, hw * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(off.data(), p->d_off[b], off.size() * 4, hipMemcpyDeviceToHost, st));
+    if (n > 0) {
+        HIPCHK(hipMemcpyAsync(id.data(), d.id, n * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(mate.data(), d.mate_id, n * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(nbab.data(), d.nbabies, n * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(life.data(), d.life, n, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(gender.data(), d.gender, n, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(birth.data(), d.birth, n * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(age.data(), d.age, n * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(lastb.data(), d.lastb, n * 8, hipMemcpyDeviceToHost, st));
         if (p->gwords && a->genome) {
-            genome.resize((size_t)hw * p->gwords);
-            HIPCHK(hipMemcpyAsync(genome.data(), d.genome, (size_t)hw * p->gwords * 8, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(gh.data(), d.ghandle, n * 4, hipMemcpyDeviceToHost, st));
+            pool.resize((size_t)np * p->gwords);
+            if (np > 0)
+                HIPCHK(hipMemcpyAsync(pool.data(), p->d_gpool[p->gp_cur], (size_t)np * p->gwords * 8,
+                                      hipMemcpyDeviceToHost, st));
         }
-        HIPCHK(hipStreamSynchronize(st));
     }
-    int64_t k = 0;
-    for (int64_t i = 0; i < hw; i++) {
-        if (life[i] == 0) continue;
+    HIPCHK(hipStreamSynchronize(st));
+    if ((int64_t)off[p->ncells] != n) return fail(QHG_ERR_STATE, "download: segment offsets disagree with the count");
+    std::vector<int32_t> cell(n);
+    for (int32_t c = 0; c < p->ncells; c++)
+        for (uint32_t i = off[c]; i < off[(size_t)c + 1]; i++) cell[i] = c;
+    std::vector<int64_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return id[x] < id[y]; });
+    std::vector<int64_t> sorted_ids(n);
+    for (int64_t k = 0; k < n; k++) sorted_ids[k] = id[order[k]];
+    for (int64_t k = 0; k < n; k++) {
+        const int64_t i = order[k];
         if (a->id) a->id[k] = id[i];
         if (a->cell) a->cell[k] = cell[i];
         if (a->life) a->life[k] = life[i];
@@ -831,38 +943,37 @@ int qhg_gpu_download_agents(qhg_pop *p, qhg_agents *a) {
         if (a->last_birth) a->last_birth[k] = lastb[i];
         if (a->num_babies) a->num_babies[k] = nbab[i];
         if (a->mate_id) {
-            const int32_t m = mate[i];
-            a->mate_id[k] = (m >= 0) ? ((m < hw && life[m] != 0) ? id[m] : -1) : m;
+            // the partner's id while it lives, -1 once it has died
+            const int64_t m = mate[i];
+            a->mate_id[k] = (m >= 0) ? (std::binary_search(sorted_ids.begin(), sorted_ids.end(), m) ? m : -1) : m;
         }
-        if (a->genome && p->gwords)
-            std::memcpy(a->genome + (size_t)k * p->gwords, &genome[(size_t)i * p->gwords], (size_t)p->gwords * 8);
-        k++;
+        if (a->genome && p->gwords) {
+            const int64_t h = gh[i];
+            if (h < 0 || h >= np) return fail(QHG_ERR_STATE, "download: genome handle out of range");
+            std::memcpy(a->genome + (size_t)k * p->gwords, &pool[(size_t)h * p->gwords], (size_t)p->gwords * 8);
+        }
     }
-    a->n = k;
+    a->n = n;
     return 0;
 }
 
 int qhg_gpu_pre_loop(qhg_pop *p, double t) {
     if (!p) return fail(QHG_ERR_ARG, "null handle");
-    hipLaunchKernelGGL(k_set_time, dim3(1), dim3(64), 0, p->stream, p->d_st, t);
-    HIPCHK(hipGetLastError());
+    p->t_next = t;
     if (p->has_cap) {
         int rc = run_capacity(p);
         if (rc) return rc;
     }
-    int rc = recount(p);
-    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(p->stream));
     return 0;
 }
 
 int qhg_gpu_initialize_step(qhg_pop *p, double t) {
     if (!p) return fail(QHG_ERR_ARG, "null handle");
-    int rc = ensure_capacity(p);
+    int rc = phase_initialize(p);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_set_time, dim3(1), dim3(64), 0, p->stream, p->d_st, t);
-    HIPCHK(hipGetLastError());
-    return phase_initialize(p);
+    p->t_next = t;
+    return 0;
 }
 
 int qhg_gpu_do_actions(qhg_pop *p, uint32_t prio, double t) {
@@ -884,9 +995,7 @@ int qhg_gpu_finalize_step(qhg_pop *p) {
 int qhg_gpu_run(qhg_pop *p, int32_t nsteps) {
     if (!p) return fail(QHG_ERR_ARG, "null handle");
     for (int32_t s = 0; s < nsteps; s++) {
-        int rc = ensure_capacity(p);
-        if (rc) return rc;
-        rc = phase_initialize(p);
+        int rc = phase_initialize(p);
         if (rc) return rc;
         rc = phase_act(p);
         if (rc) return rc;
@@ -943,11 +1052,12 @@ int qhg_gpu_stats(qhg_pop *p, qhg_stats *o) {
     DevStatus s;
     int rc = read_status_sync(p, &s);
     if (rc) return rc;
+    const int b = p->cur();
     std::memset(o, 0, sizeof(*o));
-    o->num_alive = s.n_alive;
-    o->high_water = s.hw;
-    o->capacity = p->bufs[p->cur].cap;
-    o->max_id = s.max_id;
+    o->num_alive = s.n_alive[b];
+    o->high_water = s.n_alive[b];
+    o->capacity = p->cap;
+    o->max_id = s.max_id[b];
     o->step = s.step;
     o->births_last = s.births_last;
     o->deaths_last = s.deaths_last;
@@ -958,19 +1068,17 @@ int qhg_gpu_stats(qhg_pop *p, qhg_stats *o) {
     o->compactions = p->compactions;
     o->device_error = s.error;
     o->agent_steps = s.agent_steps;
-    o->slot_steps = s.slot_steps;
+    o->slot_steps = s.agent_steps;
     o->fertile_total = s.fertile_total;
     return 0;
 }
 
 int qhg_gpu_cell_counts(qhg_pop *p, uint32_t *out) {
     if (!p || !out) return fail(QHG_ERR_ARG, "null argument");
-    DevStatus s;
-    int rc = read_status_sync(p, &s);
-    if (rc) return rc;
-    // counts of the step about to run = cnt[step & 1]
-    HIPCHK(hipMemcpyAsync(out, p->d_cnt[s.step & 1], (size_t)p->ncells * 4, hipMemcpyDeviceToHost, p->stream));
+    std::vector<uint32_t> off((size_t)p->ncells + 1);
+    HIPCHK(hipMemcpyAsync(off.data(), p->d_off[p->cur()], off.size() * 4, hipMemcpyDeviceToHost, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
+    for (int32_t c = 0; c < p->ncells; c++) out[c] = off[(size_t)c + 1] - off[c];
     return 0;
 }
 
@@ -990,8 +1098,11 @@ int qhg_gpu_env_weights(qhg_pop *p, double *out) {
 
 int qhg_gpu_compact(qhg_pop *p) {
     if (!p) return fail(QHG_ERR_ARG, "null handle");
-    int rc = compact_into(p, p->bufs[p->cur].cap);
-    if (rc) return rc;
+    // the agent buffers never hold holes; the genome pool does
+    if (p->gwords) {
+        int rc = compact_pool(p, p->gcap);
+        if (rc) return rc;
+    }
     HIPCHK(hipStreamSynchronize(p->stream));
     return 0;
 }

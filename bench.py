@@ -44,36 +44,37 @@ def bytes_model():
     return out
 
 
-def kernel_bytes(bm, d, ncells, conn, ops):
+def grid_edges(grid):
+    """Incoming (source cell, neighbour slot) pairs a move can take."""
+    import numpy as np
+    nbr = np.asarray(grid["neighbors"]).reshape(grid["num_cells"], grid["connectivity"])
+    nn = np.asarray(grid["num_neighbors"])
+    slot = np.arange(grid["connectivity"])[None, :]
+    return int(((nbr >= 0) & (slot < nn[:, None])).sum())
+
+
+def kernel_bytes(bm, d, grid, ops, gwords):
     """Algorithmic bytes of each step kernel summed over the measured steps.
-    d: deltas of live agent-steps, slot-steps, births, moves, fertile."""
+    d: deltas of live agent-steps, occupied cell-steps, births, deaths."""
     from qhg4_amd import abi
-    live, slots, births, moves, fert, steps = (d["live"], d["slots"], d["births"], d["moves"],
-                                              d["fertile"], d["steps"])
+    ncells, conn = grid["num_cells"], grid["connectivity"]
+    live, occ, births, deaths, steps = d["live"], d["occupied"], d["births"], d["deaths"], d["steps"]
     per_live = bm["ACT_LIVE_BASE"]
-    per_cell = bm["ACT_CELL_COUNT"]
+    if abi.ACT_RANDOMPAIR in ops:
+        per_live += bm["ACT_LIVE_PAIR"]
     if abi.ACT_GETOLD in ops:
         per_live += bm["ACT_LIVE_GETOLD"]
     if abi.ACT_FERTILITY in ops:
         per_live += bm["ACT_LIVE_FERT"]
-    if abi.ACT_VERHULST in ops or abi.ACT_VERHULSTVARK in ops:
-        per_live += bm["ACT_LIVE_VERH"]
-        per_cell += bm["ACT_CELL_VERH"]
-    if abi.ACT_WEIGHTEDMOVE in ops:
-        per_cell += bm["ACT_CELL_WMOVE"]
-    if abi.ACT_RANDOMMOVE in ops:
-        per_cell += bm["ACT_CELL_RMOVE"]
-    act = (slots * bm["ACT_SLOT"] + live * per_live + steps * ncells * per_cell
-           + moves * bm["ACT_MOVER"] + births * bm["ACT_BIRTH"])
-    nb = births * bm["NEWBORN_BIRTH"]
-    if abi.ACT_GENETICS in ops:
-        nb += births * bm["NEWBORN_GENOME"]
-    out = {"k_act": act, "k_newborn": nb,
-           "k_cellscan": steps * ncells * bm["SCAN_CELL"]}
-    if abi.ACT_RANDOMPAIR in ops:
-        out["k_bucket"] = slots * bm["BUCKET_SLOT"] + live * bm["BUCKET_LIVE"] + fert * bm["BUCKET_FERTILE"]
-        out["k_pair"] = steps * ncells * bm["PAIR_CELL"] + fert * bm["PAIR_FERTILE"]
-    return out
+    per_cell = bm["ACT_CELL"] + conn * bm["ACT_CELL_PER_CONN"]
+    per_occ = bm["ACT_OCC"] + conn * bm["ACT_OCC_PER_CONN"] + bm["ACT_OCC_W_EXTRA"]
+    act = steps * ncells * per_cell + occ * per_occ + live * per_live + births * bm["ACT_BIRTH"]
+    scan = steps * (ncells * bm["SCAN_CELL"] + grid_edges(grid) * bm["SCAN_EDGE"])
+    survivors = live - deaths
+    per_surv = bm["SCAT_SURVIVOR"] + (bm["SCAT_SURVIVOR_GEN"] if gwords else 0)
+    per_birth = bm["SCAT_BIRTH"] + ((bm["SCAT_BIRTH_GEN"] + bm["SCAT_GENOME_WORDS"] * gwords * 8) if gwords else 0)
+    scat = steps * ncells * bm["SCAT_CELL"] + live * bm["SCAT_LIVE"] + survivors * per_surv + births * per_birth
+    return {"k_cell_act": act, "k_cell_scan": scan, "k_cell_scatter": scat}
 
 
 def make_workload(name, seed):
@@ -94,9 +95,10 @@ def make_workload(name, seed):
 
 
 def delta(s1, s0, steps):
-    return {"live": s1["agent_steps"] - s0["agent_steps"], "slots": s1["slot_steps"] - s0["slot_steps"],
-            "births": s1["births_total"] - s0["births_total"], "moves": s1["moves_total"] - s0["moves_total"],
-            "fertile": s1["fertile_total"] - s0["fertile_total"], "steps": steps}
+    return {"live": s1["agent_steps"] - s0["agent_steps"],
+            "occupied": s1["occupied_cell_steps"] - s0["occupied_cell_steps"],
+            "births": s1["births_total"] - s0["births_total"], "deaths": s1["deaths_total"] - s0["deaths_total"],
+            "moves": s1["moves_total"] - s0["moves_total"], "steps": steps}
 
 
 def gpu_run(sc, warmup, steps, timing):
@@ -178,7 +180,8 @@ def main():
             raise SystemExit("bench: the timing run diverged from the timed run")
         p = sc["params"]
         ops = {p.action_kind[i] for i in range(p.num_actions)}
-        kb = kernel_bytes(bytes_model(), d, sc["grid"]["num_cells"], sc["grid"]["connectivity"], ops)
+        gw = 2 * abi.genome_words(p.gen_genome_size) if abi.ACT_GENETICS in ops else 0
+        kb = kernel_bytes(bytes_model(), d, sc["grid"], ops, gw)
         for name, bts in kb.items():
             avg_ms, n = times.get(name, (0.0, 0))
             if not n:

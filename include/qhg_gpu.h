@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define QHG_ABI_VERSION 1
+#define QHG_ABI_VERSION 2
 
 /* error codes */
 #define QHG_OK              0
@@ -171,7 +171,7 @@ typedef struct qhg_agents {
 
 typedef struct qhg_stats {
     int64_t  num_alive;
-    int64_t  high_water;     /* slots in use, holes included */
+    int64_t  high_water;     /* slots in use (the buffers hold no holes: = num_alive) */
     int64_t  capacity;
     int64_t  max_id;
     int64_t  step;
@@ -181,8 +181,9 @@ typedef struct qhg_stats {
     int32_t  device_error;
     int32_t  pad;
     int64_t  agent_steps;    /* sum over steps of the live count at step start */
-    int64_t  slot_steps;     /* sum over steps of the high-water mark at step start */
-    int64_t  fertile_total;  /* agents bucketed for pairing, all steps */
+    int64_t  slot_steps;     /* sum over steps of the slots in use at step start */
+    int64_t  fertile_total;  /* FERTILE agents considered for pairing, all steps */
+    int64_t  occupied_cell_steps;  /* sum over steps of the cells holding agents */
 } qhg_stats;
 
 typedef struct qhg_pop qhg_pop;
@@ -228,7 +229,7 @@ int qhg_gpu_stats(qhg_pop *pop, qhg_stats *out);
 int qhg_gpu_cell_counts(qhg_pop *pop, uint32_t *out);        /* num_cells */
 int qhg_gpu_capacities(qhg_pop *pop, double *out);           /* num_cells */
 int qhg_gpu_env_weights(qhg_pop *pop, double *out);          /* num_cells*(conn+1) */
-int qhg_gpu_compact(qhg_pop *pop);                           /* LBController::compactData */
+int qhg_gpu_compact(qhg_pop *pop);   /* LBController::compactData: here the genome pool */
 
 /* profiling: average device time (ms) per launch of each kernel since the
  * last reset, measured with HIP events on the handle's stream. */

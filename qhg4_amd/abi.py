@@ -146,6 +146,7 @@ class Stats(C.Structure):
         ("agent_steps", C.c_int64),
         ("slot_steps", C.c_int64),
         ("fertile_total", C.c_int64),
+        ("occupied_cell_steps", C.c_int64),
     ]
 
     def as_dict(self):

@@ -2,45 +2,45 @@
  * bytes_model.h -- algorithmic HBM bytes of each step kernel.
  *
  * Compulsory traffic: every field a kernel must read is read once, every
- * field it must write is written once; per-cell tables are counted once per
- * cell per launch.  Caches may make a kernel move less than this (hits on
- * per-cell rows) or more (re-reads); rocprof FETCH_SIZE/WRITE_SIZE give the
- * measured traffic.  bench.py reads these constants from this file.
+ * field it must write is written once; per-cell rows are counted once per
+ * cell per launch.  Caches may make a kernel move less than this or more
+ * (re-reads); rocprof FETCH_SIZE/WRITE_SIZE give the measured traffic.
+ * bench.py reads these constants from this file.
  *
- * Units: bytes.  "live" = agents alive at step start, "slot" = every slot in
- * [0, high-water) including holes, "mover" = agents that move, "birth" = new
- * agents, "fertile" = FERTILE agents bucketed for pairing.
+ * Units: bytes.  "cell" = every cell of the grid, "occupied" = cells holding
+ * agents at step start, "live" = agents alive at step start, "survivor" =
+ * live agents that do not die in the step, "birth" = new agents, "edge" = an
+ * incoming (source cell, neighbour slot) pair of the grid, "conn" = the grid
+ * connectivity, "gwords" = 64-bit words of one genome (both strands).
  */
 #ifndef QHG_BYTES_MODEL_H
 #define QHG_BYTES_MODEL_H
 
-/* k_act (fused doActions) */
-#define QHG_B_ACT_SLOT          1   /* life byte of every slot                          */
-#define QHG_B_ACT_LIVE_BASE    13   /* cell 4 + id 8 (Philox key) + life 1              */
-#define QHG_B_ACT_LIVE_GETOLD  16   /* birth time r 8 + age w 8                         */
-#define QHG_B_ACT_LIVE_FERT    10   /* gender 1 + last birth 8 + life w 1               */
-#define QHG_B_ACT_LIVE_VERH     4   /* mate 4                                           */
-#define QHG_B_ACT_CELL_VERH    12   /* count 4 + K 8 (VarK) / count 4 (+8 unused)       */
-#define QHG_B_ACT_CELL_COUNT    4   /* next-step count written                          */
-#define QHG_B_ACT_CELL_WMOVE   81   /* weight row 7*8 + neighbours 6*4 + count byte 1   */
-#define QHG_B_ACT_CELL_RMOVE   25   /* neighbours 6*4 + count byte 1                    */
-#define QHG_B_ACT_MOVER         4   /* new cell written                                 */
-#define QHG_B_ACT_BIRTH        20   /* birth record 12 + mother's NumBabies r/w 8       */
+/* k_cell_act (RandomPair::initialize/findMates + fused doActions) */
+#define QHG_B_ACT_CELL          12  /* offset r 4 + stayer count w 4 + birth count w 4     */
+#define QHG_B_ACT_CELL_PER_CONN  4  /* mover count per neighbour slot w                     */
+#define QHG_B_ACT_OCC            9  /* K r 8 + neighbour count r 1                           */
+#define QHG_B_ACT_OCC_PER_CONN  12  /* neighbour index r 4 + weight r 8 (one more weight)    */
+#define QHG_B_ACT_OCC_W_EXTRA    8  /*   ... the extra weight of the row (conn+1 weights)   */
+#define QHG_B_ACT_LIVE_BASE     10  /* id r 8 (Philox key) + life r 1 + outcome w 1          */
+#define QHG_B_ACT_LIVE_PAIR     10  /* gender r 1 + life r 1 (pairing pass) + partner id w 8 */
+#define QHG_B_ACT_LIVE_GETOLD   16  /* birth time r 8 + age w 8                              */
+#define QHG_B_ACT_LIVE_FERT     10  /* gender r 1 + last birth r 8 + life w 1                */
+#define QHG_B_ACT_BIRTH         12  /* mother's NumBabies r/w 8 + father slot w 4            */
 
-/* k_newborn (createAgentAtIndex + Genetics::makeOffspring) */
-#define QHG_B_NEWBORN_BIRTH    62   /* record r 12 + agent w 46 + count 4               */
-#define QHG_B_NEWBORN_GENOME  384   /* 2 parent genomes r 2*128 + child w 128 (2x256 loci) */
+/* k_cell_scan (next counts -> segment and birth offsets) */
+#define QHG_B_SCAN_CELL         24  /* stayers r 4 + births r 4 + in-list offset r 4 + next
+                                       offset w 4 + birth offset w 4 + cell-tile share 4    */
+#define QHG_B_SCAN_EDGE          9  /* source r 4 + slot r 1 + mover count r 4               */
 
-/* k_bucket (RandomPair::initialize bucketing) */
-#define QHG_B_BUCKET_SLOT       1   /* life byte                                        */
-#define QHG_B_BUCKET_LIVE       4   /* mate := -3                                       */
-#define QHG_B_BUCKET_FERTILE   17   /* cell 4 + gender 1 + bucket w 4 + fill 4 + off 4  */
-
-/* k_pair (per-cell pairing) */
-#define QHG_B_PAIR_CELL        16   /* fill F/M 8 + offsets 8                           */
-#define QHG_B_PAIR_FERTILE     20   /* bucket r+w 8 + id 8 + mate w 4                   */
-
-/* k_cellscan (counts -> offsets) */
-#define QHG_B_SCAN_CELL        20   /* count r 4 + offset w 4 + fills w 8 + clear w 4   */
+/* k_cell_scatter (performMoves + createAgentAtIndex + makeOffspring) */
+#define QHG_B_SCAT_CELL          8  /* next offset r 4 (+ neighbour's) + birth count r 4     */
+#define QHG_B_SCAT_LIVE          1  /* outcome byte r of every agent at step start           */
+#define QHG_B_SCAT_SURVIVOR     93  /* record r 46 + w 46 (id, partner, life, gender, birth,
+                                       age, last birth, NumBabies) + outcome r 1             */
+#define QHG_B_SCAT_SURVIVOR_GEN  8  /* genome handle r 4 + w 4                               */
+#define QHG_B_SCAT_BIRTH        54  /* newborn record w 46 + birth offset r 4 + father r 4   */
+#define QHG_B_SCAT_BIRTH_GEN    12  /* handle w 4 + mother/father handles r 8                */
+#define QHG_B_SCAT_GENOME_WORDS  3  /* x gwords x 8: two parent genomes r + child w          */
 
 #endif

@@ -1,29 +1,36 @@
 // Device side of the population step engine (gfx950, wave64).
 //
-// Agents are stored cell-sorted: the live agents of cell c occupy slots
-// [off[c], off[c+1]) in id order, in one of two structure-of-arrays buffers
-// (current / next, swapped every step).  A step is three launches, none of
-// them with global atomics:
+// Agents are stored by cell: the live agents of cell c occupy the segment
+// [off[c], off[c+1]) of one of two structure-of-arrays buffers (current /
+// next, swapped every step).  Inside a segment the order is the one the
+// previous step wrote (stayers, then arrivals per incoming neighbour slot,
+// then newborns); every order-dependent rule sorts by agent id itself, so
+// results do not depend on it.  A step is three launches (four with
+// Genetics), none with global atomics on agent data:
 //
-//   k_cell_act      one wave per cell: RandomPair::initialize/findMates on
-//                   the cell's contiguous segment (QHG4/actions/RandomPair.cpp
-//                   :107-279), then every Action<T>::execute of the step fused
-//                   in priority order (QHG4/core/SPopulation.cpp:554-577).
-//                   Writes each agent's outcome (dead / stays / moves to
-//                   neighbour d / gave birth) and the per-cell counts of
-//                   stayers, movers per direction and births.
-//   k_cell_scan     per cell: new count = stayers + movers arriving from the
-//                   neighbours + births; one decoupled look-back scan gives the
-//                   new segment offsets and the birth offsets
+//   k_cell_act      one wave per occupied cell: RandomPair::initialize /
+//                   findMates on the cell's segment (QHG4/actions/
+//                   RandomPair.cpp:107-279), then every Action<T>::execute of
+//                   the step fused in priority order (QHG4/core/
+//                   SPopulation.cpp:554-577).  Writes each agent's outcome
+//                   (dead / stays / moves to neighbour slot d / gave birth)
+//                   and per cell the counts of stayers, movers per slot and
+//                   births.
+//   k_cell_scan     per cell: next count = stayers + arrivals + births; one
+//                   decoupled look-back scan gives the next segment offsets,
+//                   the birth offsets (new ids), and the absolute position of
+//                   every incoming (source cell, slot) run
 //                   (updateNumAgentsPerCell, SPopulation.cpp:509-536).
-//   k_cell_scatter  one wave per destination cell: stayers and arriving movers
-//                   merged in id order into the next buffer, newborns appended
-//                   (performMoves, recycleDeadSpaceNew, createAgentAtIndex,
-//                   makePopSpecificOffspring, Genetics::makeOffspring;
-//                   SPopulation.cpp:596-1092, Genetics.cpp:285-337).
+//   k_cell_scatter  one wave per occupied source cell: stayers and movers
+//                   pushed to their next positions, newborns of the cell's
+//                   mothers appended in mother-id order (performMoves,
+//                   recycleDeadSpaceNew, createAgentAtIndex,
+//                   makePopSpecificOffspring; SPopulation.cpp:596-918).
+//   k_genome        one thread per birth: Genetics::makeOffspring
+//                   (Genetics.cpp:285-337) into the genome pool.
 //
 // Per-cell tables (count, K, weights, neighbours) are read once per cell.
-// Genomes live in a separate pool addressed by a handle, so they are only
+// Genomes live in a pool addressed by a per-agent handle, so they are only
 // touched at births and at occasional pool compactions.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -36,34 +43,45 @@ namespace qhg {
 
 constexpr int kWave = 64;
 constexpr int kCellBlock = 64;        // per-cell kernels: one wave per block
-constexpr int kLdsCell = 512;         // cells up to this size keep their lists in LDS,
+constexpr int kLdsCell = 256;         // cells up to this size keep their lists in LDS,
                                       // larger ones in segment-indexed global scratch
-constexpr int kMaxRuns = 15;          // id-sorted runs merged per destination cell
+constexpr int kMaxIn = 8;             // incoming neighbour slots per cell
 constexpr int kScanBlock = 256;       // k_cell_scan threads per block
-constexpr int kCellTile = 1024;       // cells per look-back tile (4 per thread)
+constexpr int kScanPer = 2;           // cells per k_cell_scan thread
+constexpr int kCellTile = kScanBlock * kScanPer;   // cells per look-back tile
 constexpr int kMaxCross = 32;         // device limit on Genetics_num_crossover
 constexpr int kStatusRing = 8;        // host-mapped status snapshots
 
 constexpr int32_t kErrCapacity = 1;   // the next population does not fit
 constexpr int32_t kErrLookback = 4;   // a look-back wait exceeded its spin limit
-constexpr int32_t kErrGather = 8;     // scatter found a count mismatch
+constexpr int32_t kErrGather = 8;     // an internal count disagreed
 constexpr uint32_t kLookbackSpins = 1u << 22;
+constexpr int kParts = 3;             // per-block partial sums of k_cell_act
 
 // outcome byte of an agent in a step
 constexpr uint8_t kDecBirth = 0x80;   // registered a birth
 constexpr uint8_t kDecDead = 0x40;    // died
-constexpr uint8_t kDecDirMask = 0x07; // 0 stays, d+1 moves to neighbour d
+constexpr uint8_t kDecDirMask = 0x07; // 0 stays, d+1 moves to neighbour slot d
 
 struct DevStatus {
     int64_t n_alive[2];      // live agents at the start of a step of this parity
+    int64_t n_occ[2];        // occupied cells at the start of a step of this parity
     int64_t max_id[2];
     int64_t gpool[2];        // genome pool high-water mark
     int64_t step;            // steps completed
     double t;
     int64_t births_last, deaths_last, moves_last;
     int64_t births_total, deaths_total, moves_total;
-    int64_t agent_steps, fertile_total;
+    int64_t agent_steps, fertile_total, occupied_total;
     int32_t error, pad;
+};
+
+// an occupied cell of a step: index, segment start, agent count
+struct OccCell {
+    int32_t c;
+    uint32_t s;
+    uint32_t n;
+    uint32_t pad;
 };
 
 struct AgentBuf {
@@ -82,6 +100,8 @@ struct Ctx {
     AgentBuf cur, nxt;
     const uint32_t *off_cur;  // ncells + 1 segment offsets of the current buffer
     uint32_t *off_nxt;
+    const OccCell *occ_cur;   // occupied cells of this step, cell order
+    OccCell *occ_nxt;         // ... of the next step (written by k_cell_scan)
     int64_t cap;
     int32_t ncells, conn;
     const int32_t *nbr;
@@ -89,21 +109,23 @@ struct Ctx {
     double *K;
     double *W;
     const uint8_t *ice;
-    const int32_t *in_off;    // incoming neighbour lists (CSR): the cells
-    const int32_t *in_src;    // in_src[k] whose neighbour slot in_dir[k] is
-    const uint8_t *in_dir;    // this cell, k in [in_off[c], in_off[c+1])
+    const int32_t *in_e;      // ncells * in_w: incoming runs src*conn+slot, -1 padded
+    int32_t in_w;             // <= kMaxIn
     uint8_t *dec;             // per slot of the current buffer: outcome byte
-    int32_t *bfather;         // per slot: father slot of a registered birth
-    int32_t *gfem, *gmal, *gmate;   // per-slot list scratch of large cells
+    int32_t *bfather;         // per slot: father slot of a registered birth (Genetics)
+    int32_t *brec;            // per birth: mother slot, father slot (Genetics)
+    int32_t *gfu, *gmu, *gfs, *gms, *gmate;   // per-slot list scratch of large cells
+    int64_t *gfid, *gmid;
     uint8_t *gflag;
-    int32_t *gcand;           // per slot of the next buffer: scatter scratch
-    int64_t *gcid;            //   of large cells
     uint32_t *cstay, *cbirth, *cmv;   // per cell; cmv is ncells * conn
     uint32_t *birth_off;      // ncells + 1: births before each cell
-    uint64_t *tiles;          // look-back words of k_cell_scan
+    uint32_t *bpos;           // per cell: next-buffer slot of its first newborn
+    uint32_t *rpos;           // ncells * conn: next-buffer slot of each (cell, slot) run
+    uint64_t *tiles;          // look-back words of k_cell_scan: agents | births,
+    uint64_t *tiles2;         //   and occupied cells
     int32_t ntiles;
     int32_t nact_blocks;      // grid of k_cell_act
-    uint32_t *wave_part;      // 3 per k_cell_act block: deaths, moves, fertile
+    uint32_t *wave_part;      // kParts per k_cell_act block: deaths, moves, fertile
     uint64_t *gpool;          // genome pool, gwords u64 per handle
     int32_t gwords;
     int64_t gcap;
@@ -131,6 +153,19 @@ struct Prog {
     int32_t gen_size, gen_cross, nbino, pad;
     double bino[QHG_MAX_BINO];
 };
+
+// Ordering point between the lanes of a single-wave work-group: LDS (and a
+// wave's own global) accesses execute in program order, so only the
+// compiler must not move memory operations across; unlike __syncthreads it
+// does not wait for loads still in flight.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int k) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, k);
+}
 
 // ---------------------------------------------------------------------------
 // wave / block helpers
@@ -195,24 +230,37 @@ __device__ __forceinline__ uint64_t tile_read(uint64_t *w) {
     return __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Called by all 64 lanes of one wave; returns the exclusive prefix of `tile`.
-// Every wait is bounded: past the spin limit the step is flagged as failed.
-__device__ uint64_t lookback(uint64_t *tiles, uint32_t tile, uint64_t agg, int32_t *err) {
+// Look-back over a pair of words per tile (two independent sums).  Called
+// by all 64 lanes of one wave; returns the exclusive prefixes of `tile`.
+// A predecessor counts only when both of its words carry the same flag
+// (the publisher writes them one after the other).  Every wait is bounded:
+// past the spin limit the step is flagged as failed.
+__device__ void lookback2(uint64_t *t1, uint64_t *t2, uint32_t tile, uint64_t agg1, uint64_t agg2, int32_t *err,
+                          uint64_t &ex1, uint64_t &ex2) {
     const int lane = threadIdx.x & 63;
+    ex1 = 0;
+    ex2 = 0;
     if (tile == 0) {
-        if (lane == 0) tile_publish(&tiles[0], kFlagInc | agg);
-        return 0;
+        if (lane == 0) {
+            tile_publish(&t1[0], kFlagInc | agg1);
+            tile_publish(&t2[0], kFlagInc | agg2);
+        }
+        return;
     }
-    if (lane == 0) tile_publish(&tiles[tile], kFlagAgg | agg);
-    uint64_t excl = 0;
+    if (lane == 0) {
+        tile_publish(&t1[tile], kFlagAgg | agg1);
+        tile_publish(&t2[tile], kFlagAgg | agg2);
+    }
     int64_t top = (int64_t)tile - 1;
     uint32_t spins = 0;
     while (true) {
         const int64_t j = top - lane;
-        const uint64_t w = (j >= 0) ? tile_read(&tiles[j]) : kFlagInc;
-        const uint64_t flag = w & ~kValMask;
-        const uint64_t inc = __ballot(flag == kFlagInc);
-        const uint64_t nready = __ballot(flag == 0);
+        const uint64_t w1 = (j >= 0) ? tile_read(&t1[j]) : kFlagInc;
+        const uint64_t w2 = (j >= 0) ? tile_read(&t2[j]) : kFlagInc;
+        const uint64_t f1 = w1 & ~kValMask, f2 = w2 & ~kValMask;
+        const bool ready = f1 == f2 && f1 != 0;
+        const uint64_t inc = __ballot(ready && f1 == kFlagInc);
+        const uint64_t nready = __ballot(!ready);
         const int first = inc ? (__ffsll((unsigned long long)inc) - 1) : 64;
         const uint64_t need = (first >= 63) ? ~0ull : ((2ull << first) - 1ull);
         if (nready & need) {
@@ -223,12 +271,15 @@ __device__ uint64_t lookback(uint64_t *tiles, uint32_t tile, uint64_t agg, int32
             if (lane == 0) atomicOr(err, kErrLookback);
             break;
         }
-        excl += wave_sum_u64(lane <= first ? (w & kValMask) : 0ull);
+        ex1 += wave_sum_u64(lane <= first ? (w1 & kValMask) : 0ull);
+        ex2 += wave_sum_u64(lane <= first ? (w2 & kValMask) : 0ull);
         if (first < 64) break;
         top -= 64;
     }
-    if (lane == 0) tile_publish(&tiles[tile], kFlagInc | (excl + agg));
-    return excl;
+    if (lane == 0) {
+        tile_publish(&t1[tile], kFlagInc | (ex1 + agg1));
+        tile_publish(&t2[tile], kFlagInc | (ex2 + agg2));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -347,132 +398,156 @@ __device__ __forceinline__ int move_dir(const Ctx &x, int kind, uint32_t u, int 
 }
 
 // ---------------------------------------------------------------------------
-// k_cell_act: pairing + fused actions, one wave per cell
+// k_cell_act: pairing + fused actions, one wave per occupied cell
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs sa) {
-    __shared__ int32_t s_fem[kLdsCell], s_mal[kLdsCell], s_mate[kLdsCell];
-    __shared__ uint8_t s_flag[kLdsCell];
+struct PairLists {            // per-cell lists, in LDS or in global scratch
+    int32_t *fu, *mu;         // local indices of FERTILE females / males, slot order
+    int64_t *fid, *mid;       // their ids
+    int32_t *fs, *ms;         // id order: positions into fu/mu
+    int32_t *mate;            // partner's local index, -1 unpaired
+    uint8_t *flag;            // still unpaired (pool side)
+};
+
+// rank of each entry of ids[0..n) among them (ids are distinct) -> ord[rank]
+__device__ __forceinline__ void id_order(const int64_t *ids, int32_t n, int32_t *ord) {
+    const int lane = threadIdx.x;
+    for (int32_t j = lane; j < n; j += kWave) {
+        const int64_t my = ids[j];
+        int32_t r = 0;
+        for (int32_t k = 0; k < n; k++) r += ids[k] < my;
+        ord[r] = j;
+    }
+}
+
+__device__ __forceinline__ void cell_act(const Ctx &x, const Prog &p, const StepArgs &sa, int32_t c, uint32_t s,
+                                         int32_t n, const PairLists &L, uint32_t &my_deaths, uint32_t &my_moves,
+                                         uint32_t &my_fert) {
     const int lane = threadIdx.x;
     const uint64_t lt = lanemask_lt();
     const int R = x.conn + 1;
-    uint32_t my_deaths = 0, my_moves = 0, my_fert = 0;
-    // a failed step (capacity, look-back, gather) freezes the population:
-    // later steps already enqueued touch nothing
-    if (x.st->error) return;
+    const int nchunks = (n + kWave - 1) / kWave;
 
-    // look-back words of this step's k_cell_scan start at zero
-    for (int32_t i = blockIdx.x * kCellBlock + lane; i < x.ntiles; i += gridDim.x * kCellBlock) x.tiles[i] = 0;
-
-    for (int32_t c = blockIdx.x; c < x.ncells; c += gridDim.x) {
-        const uint32_t s = x.off_cur[c];
-        const int32_t n = (int32_t)(x.off_cur[c + 1] - s);
-        if (n == 0) {
-            if (lane == 0) { x.cstay[c] = 0; x.cbirth[c] = 0; }
-            if (lane < x.conn) x.cmv[(int64_t)c * x.conn + lane] = 0;
-            continue;
-        }
-        const int nchunks = (n + kWave - 1) / kWave;
-        // per-cell lists: LDS, or the cell's own segment range of the global
-        // scratch arrays when the cell is large
-        const bool big = n > kLdsCell;
-        int32_t *fem = big ? x.gfem + s : s_fem;
-        int32_t *mal = big ? x.gmal + s : s_mal;
-        int32_t *mate = big ? x.gmate + s : s_mate;
-        uint8_t *flag = big ? x.gflag + s : s_flag;
-
-        // ---- RandomPair: FERTILE females and males of the cell, id order
-        if (p.has_pair) {
-            int32_t F = 0, M = 0;
-            for (int ch = 0; ch < nchunks; ch++) {
-                const int32_t li = ch * kWave + lane;
-                const bool valid = li < n;
-                const uint8_t l = valid ? x.cur.life[s + li] : (uint8_t)0;
-                const uint8_t g = valid ? x.cur.gender[s + li] : (uint8_t)2;
-                const bool isF = valid && l == QHG_LIFE_FERTILE && g == 0;
-                const bool isM = valid && l == QHG_LIFE_FERTILE && g == 1;
-                const uint64_t mF = __ballot(isF), mM = __ballot(isM);
-                if (isF) fem[F + __popcll(mF & lt)] = li;
-                if (isM) mal[M + __popcll(mM & lt)] = li;
-                F += __popcll(mF);
-                M += __popcll(mM);
-                if (valid) mate[li] = -1;
-            }
-            my_fert += (lane == 0) ? (uint32_t)(F + M) : 0u;
-            if (F > 0 && M > 0) {
-                // findMates, RandomPair.cpp:183-276: the smaller sex chooses in
-                // id order, each pick the k-th still unpaired of the other sex
-                const bool fsel = F <= M;
-                const int32_t *chs = fsel ? fem : mal;
-                const int32_t *pool = fsel ? mal : fem;
-                const int32_t nch = fsel ? F : M;
-                const int32_t np = fsel ? M : F;
-                for (int32_t j = lane; j < np; j += kWave) flag[j] = 1;
-                __syncthreads();
-                int32_t left = np;
-                for (int32_t cb = 0; cb < nch; cb += kWave) {
-                    const int32_t j = cb + lane;
-                    uint32_t u = 0;
-                    if (j < nch) u = draw(x.seed, sa.step, QHG_STREAM_PAIR, (uint64_t)x.cur.id[s + chs[j]], 0);
-                    const int32_t cnt = (nch - cb < kWave) ? (nch - cb) : kWave;
-                    for (int32_t q = 0; q < cnt && left > 0; q++) {
-                        const uint32_t uq = (uint32_t)__shfl((int)u, q, 64);
-                        int k = (int)wrandr(0, (double)left, uq);
-                        int32_t pos = -1;
-                        for (int32_t base = 0; base < np; base += kWave) {
-                            const bool f = base + lane < np && flag[base + lane];
-                            const uint64_t m = __ballot(f);
-                            const int pc = __popcll(m);
-                            if (k < pc) { pos = base + select_bit(m, k); break; }
-                            k -= pc;
-                        }
-                        if (pos < 0) {   // cannot happen while left counts the set flags
-                            if (lane == 0) atomicOr(&x.st->error, kErrGather);
-                            break;
-                        }
-                        const int32_t a = chs[cb + q], b = pool[pos];
-                        if (lane == 0) {
-                            flag[pos] = 0;
-                            mate[a] = b;
-                            mate[b] = a;
-                        }
-                        __syncthreads();
-                        left--;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-
-        // ---- the cell's tables, read once
-        CellView cv;
-        cv.N = (uint32_t)n;
-        cv.K = x.K[c];
-        cv.nn = x.nn[c];
-        int32_t nbr[6];
-        double w[7];
+    // one round trip: the cell's tables and the first chunk's fields
+    CellView cv;
+    cv.N = (uint32_t)n;
+    cv.K = x.K[c];
+    cv.nn = x.nn[c];
+    int32_t nbr[6];
+    double w[7];
 #pragma unroll
-        for (int q = 0; q < 6; q++) nbr[q] = (q < x.conn) ? x.nbr[(int64_t)c * x.conn + q] : -1;
+    for (int q = 0; q < 6; q++) nbr[q] = (q < x.conn) ? x.nbr[(int64_t)c * x.conn + q] : -1;
 #pragma unroll
-        for (int q = 0; q < 7; q++) w[q] = (q < R) ? x.W[(int64_t)c * R + q] : 0.0;
+    for (int q = 0; q < 7; q++) w[q] = (q < R) ? x.W[(int64_t)c * R + q] : 0.0;
+    const bool v0 = lane < n;
+    const uint32_t slot0 = s + (uint32_t)lane;
+    uint8_t life0 = 0, g0 = 2;
+    int64_t id0 = 0;
+    double birth0 = 0, lastb0 = 0, age0 = 0;
+    if (v0) {
+        life0 = x.cur.life[slot0];
+        g0 = x.cur.gender[slot0];
+        id0 = x.cur.id[slot0];
+        if (p.need_birth) birth0 = x.cur.birth[slot0];
+        if (p.need_lastb) lastb0 = x.cur.lastb[slot0];
+        if (p.load_age) age0 = x.cur.age[slot0];
+    }
 
-        // ---- actions, 64 agents at a time
-        uint32_t stay = 0, births = 0;
-        uint32_t mv[6] = {0, 0, 0, 0, 0, 0};
+    // ---- RandomPair: FERTILE females and males of the cell
+    if (p.has_pair) {
+        int32_t F = 0, M = 0;
         for (int ch = 0; ch < nchunks; ch++) {
             const int32_t li = ch * kWave + lane;
             const bool valid = li < n;
-            const uint32_t slot = s + (uint32_t)li;
-            ActItem a;
+            uint8_t l = life0, g = g0;
+            int64_t id = id0;
+            if (ch > 0 && valid) {
+                l = x.cur.life[s + li];
+                g = x.cur.gender[s + li];
+                id = x.cur.id[s + li];
+            }
+            const bool isF = valid && l == QHG_LIFE_FERTILE && g == 0;
+            const bool isM = valid && l == QHG_LIFE_FERTILE && g == 1;
+            const uint64_t mF = __ballot(isF), mM = __ballot(isM);
+            if (isF) { const int32_t at = F + __popcll(mF & lt); L.fu[at] = li; L.fid[at] = id; }
+            if (isM) { const int32_t at = M + __popcll(mM & lt); L.mu[at] = li; L.mid[at] = id; }
+            F += __popcll(mF);
+            M += __popcll(mM);
+            if (valid) L.mate[li] = -1;
+        }
+        my_fert += (lane == 0) ? (uint32_t)(F + M) : 0u;
+        if (F > 0 && M > 0) {
+            // findMates, RandomPair.cpp:183-276: the smaller sex chooses in id
+            // order, each pick the k-th (id order) still unpaired of the other
+            wave_sync();
+            id_order(L.fid, F, L.fs);
+            id_order(L.mid, M, L.ms);
+            const bool fsel = F <= M;
+            const int32_t *chs = fsel ? L.fs : L.ms;
+            const int32_t *chl = fsel ? L.fu : L.mu;
+            const int64_t *chid = fsel ? L.fid : L.mid;
+            const int32_t *pool = fsel ? L.ms : L.fs;
+            const int32_t *pll = fsel ? L.mu : L.fu;
+            const int32_t nch = fsel ? F : M;
+            const int32_t np = fsel ? M : F;
+            for (int32_t j = lane; j < np; j += kWave) L.flag[j] = 1;
+            wave_sync();
+            int32_t left = np;
+            for (int32_t cb = 0; cb < nch; cb += kWave) {
+                const int32_t j = cb + lane;
+                uint32_t u = 0;
+                if (j < nch) u = draw(x.seed, sa.step, QHG_STREAM_PAIR, (uint64_t)chid[chs[j]], 0);
+                const int32_t cnt = (nch - cb < kWave) ? (nch - cb) : kWave;
+                for (int32_t q = 0; q < cnt && left > 0; q++) {
+                    const uint32_t uq = readlane_u32(u, q);
+                    int k = (int)wrandr(0, (double)left, uq);
+                    int32_t pos = -1;
+                    for (int32_t base = 0; base < np; base += kWave) {
+                        const bool f = base + lane < np && L.flag[base + lane];
+                        const uint64_t m = __ballot(f);
+                        const int pc = __popcll(m);
+                        if (k < pc) { pos = base + select_bit(m, k); break; }
+                        k -= pc;
+                    }
+                    if (pos < 0) {   // cannot happen while left counts the set flags
+                        if (lane == 0) atomicOr(&x.st->error, kErrGather);
+                        break;
+                    }
+                    const int32_t a = chl[chs[cb + q]], b = pll[pool[pos]];
+                    if (lane == 0) {
+                        L.flag[pos] = 0;
+                        L.mate[a] = b;
+                        L.mate[b] = a;
+                    }
+                    wave_sync();
+                    left--;
+                }
+            }
+        }
+        wave_sync();
+    }
+
+    // ---- actions, 64 agents at a time
+    uint32_t stay = 0, births = 0;
+    uint32_t mv[6] = {0, 0, 0, 0, 0, 0};
+    for (int ch = 0; ch < nchunks; ch++) {
+        const int32_t li = ch * kWave + lane;
+        const bool valid = li < n;
+        const uint32_t slot = s + (uint32_t)li;
+        ActItem a;
+        a.life = life0;
+        a.g = g0;
+        a.id = id0;
+        a.birth = birth0;
+        a.lastb = lastb0;
+        a.age = age0;
+        a.mate = -1;
+        a.age_dirty = false;
+        a.bflag = 0;
+        a.npend = 0;
+        a.pk0 = a.pk1 = 0;
+        a.pu0 = a.pu1 = 0;
+        if (ch > 0) {
             a.life = 0;
-            a.g = 0;
-            a.id = 0;
-            a.mate = -1;
-            a.birth = a.lastb = a.age = 0;
-            a.age_dirty = false;
-            a.bflag = 0;
-            a.npend = 0;
-            a.pk0 = a.pk1 = 0;
-            a.pu0 = a.pu1 = 0;
             if (valid) {
                 a.life = x.cur.life[slot];
                 a.id = x.cur.id[slot];
@@ -480,157 +555,233 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs
                 if (p.need_birth) a.birth = x.cur.birth[slot];
                 if (p.need_lastb) a.lastb = x.cur.lastb[slot];
                 if (p.load_age) a.age = x.cur.age[slot];
-                if (p.has_pair) a.mate = mate[li];
             }
-            const uint8_t life0 = a.life;
-            int dir = -1;
-            if (valid && a.life != 0) {
-                act_program(x, p, sa, cv, a, my_deaths);
-                if (a.npend > 0) {
-                    // every registered move counts; the last one wins
-                    int d = move_dir(x, a.pk0, a.pu0, cv.nn, w, nbr);
-                    if (d >= 0) { dir = d; my_moves++; }
-                    if (a.npend > 1) {
-                        d = move_dir(x, a.pk1, a.pu1, cv.nn, w, nbr);
-                        if (d >= 0) { dir = d; my_moves++; }
-                    }
-                }
-            }
-            const bool alive = valid && a.life != 0;
-            if (valid) {
-                uint8_t code = (uint8_t)(dir >= 0 ? dir + 1 : 0);
-                if (!alive) code |= kDecDead;
-                if (a.bflag) code |= kDecBirth;
-                x.dec[slot] = code;
-                if (a.life != life0) x.cur.life[slot] = a.life;
-                if (a.age_dirty) x.cur.age[slot] = a.age;
-                if (p.has_pair) x.cur.mate_id[slot] = (a.mate >= 0) ? x.cur.id[s + a.mate] : -3;
-                if (a.bflag) {
-                    x.cur.nbabies[slot] += 1;
-                    x.bfather[slot] = (int32_t)(s + a.mate);
-                }
-            }
-            stay += __popcll(__ballot(alive && dir < 0));
-            births += __popcll(__ballot(valid && a.bflag));
-#pragma unroll
-            for (int d = 0; d < 6; d++) mv[d] += __popcll(__ballot(alive && dir == d));
         }
-        if (lane == 0) { x.cstay[c] = stay; x.cbirth[c] = births; }
+        if (valid && p.has_pair) a.mate = L.mate[li];
+        const uint8_t lifeS = a.life;
+        int dir = -1;
+        if (valid && a.life != 0) {
+            act_program(x, p, sa, cv, a, my_deaths);
+            if (a.npend > 0) {
+                // every registered move counts; the last one wins
+                int d = move_dir(x, a.pk0, a.pu0, cv.nn, w, nbr);
+                if (d >= 0) { dir = d; my_moves++; }
+                if (a.npend > 1) {
+                    d = move_dir(x, a.pk1, a.pu1, cv.nn, w, nbr);
+                    if (d >= 0) { dir = d; my_moves++; }
+                }
+            }
+        }
+        const bool alive = valid && a.life != 0;
+        if (valid) {
+            uint8_t code = (uint8_t)(dir >= 0 ? dir + 1 : 0);
+            if (!alive) code |= kDecDead;
+            if (a.bflag) code |= kDecBirth;
+            x.dec[slot] = code;
+            if (a.life != lifeS) x.cur.life[slot] = a.life;
+            if (a.age_dirty) x.cur.age[slot] = a.age;
+            if (p.has_pair) x.cur.mate_id[slot] = (a.mate >= 0) ? x.cur.id[s + a.mate] : -3;
+            if (a.bflag) {
+                x.cur.nbabies[slot] += 1;
+                if (x.gwords) x.bfather[slot] = (int32_t)(s + a.mate);
+            }
+        }
+        stay += __popcll(__ballot(alive && dir < 0));
+        births += __popcll(__ballot(valid && a.bflag));
 #pragma unroll
-        for (int d = 0; d < 6; d++)
-            if (lane == d && d < x.conn) x.cmv[(int64_t)c * x.conn + d] = mv[d];
-        __syncthreads();
+        for (int d = 0; d < 6; d++) mv[d] += __popcll(__ballot(alive && dir == d));
+    }
+    if (lane == 0) { x.cstay[c] = stay; x.cbirth[c] = births; }
+#pragma unroll
+    for (int d = 0; d < 6; d++)
+        if (lane == d && d < x.conn) x.cmv[(int64_t)c * x.conn + d] = mv[d];
+    wave_sync();
+}
+
+__global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs sa) {
+    __shared__ int32_t s_fu[kLdsCell], s_mu[kLdsCell], s_fs[kLdsCell], s_ms[kLdsCell], s_mate[kLdsCell];
+    __shared__ int64_t s_fid[kLdsCell], s_mid[kLdsCell];
+    __shared__ uint8_t s_flag[kLdsCell];
+    const int lane = threadIdx.x;
+    uint32_t my_deaths = 0, my_moves = 0, my_fert = 0;
+    // a failed step (capacity, look-back, count) freezes the population:
+    // later steps already enqueued touch nothing
+    if (x.st->error) return;
+
+    // look-back words of this step's k_cell_scan start at zero
+    for (int32_t i = blockIdx.x * kCellBlock + lane; i < x.ntiles; i += gridDim.x * kCellBlock) {
+        x.tiles[i] = 0;
+        x.tiles2[i] = 0;
+    }
+
+    // occupied cells round-robin over the waves (entry blockIdx + k*grid);
+    // empty cells keep the zero counts the previous scatter left
+    const int64_t nocc = x.st->n_occ[sa.parity];
+    const int64_t G = gridDim.x;
+    for (int64_t base = blockIdx.x; base < nocc; base += kWave * G) {
+        const int64_t i = base + (int64_t)lane * G;
+        OccCell e{0, 0, 0, 0};
+        if (i < nocc) e = x.occ_cur[i];
+        uint64_t todo = __ballot(i < nocc);
+        while (todo) {
+            const int k = __ffsll((unsigned long long)todo) - 1;
+            todo &= todo - 1;
+            const int32_t c = (int32_t)readlane_u32((uint32_t)e.c, k);
+            const uint32_t s = readlane_u32(e.s, k);
+            const int32_t n = (int32_t)readlane_u32(e.n, k);
+            PairLists L;
+            if (n > kLdsCell) {
+                L = PairLists{x.gfu + s, x.gmu + s, x.gfid + s, x.gmid + s, x.gfs + s, x.gms + s, x.gmate + s,
+                              x.gflag + s};
+            } else {
+                L = PairLists{s_fu, s_mu, s_fid, s_mid, s_fs, s_ms, s_mate, s_flag};
+            }
+            cell_act(x, p, sa, c, s, n, L, my_deaths, my_moves, my_fert);
+        }
     }
     const uint64_t dm = wave_sum_u64(((uint64_t)my_moves << 32) | my_deaths);
-    const uint32_t fert = (uint32_t)wave_sum_u64(my_fert);
+    const uint32_t fe = (uint32_t)wave_sum_u64(my_fert);
     if (lane == 0) {
-        x.wave_part[3 * blockIdx.x + 0] = (uint32_t)dm;
-        x.wave_part[3 * blockIdx.x + 1] = (uint32_t)(dm >> 32);
-        x.wave_part[3 * blockIdx.x + 2] = fert;
+        uint32_t *wp = x.wave_part + kParts * blockIdx.x;
+        wp[0] = (uint32_t)dm;
+        wp[1] = (uint32_t)(dm >> 32);
+        wp[2] = fe;
     }
 }
 
 // ---------------------------------------------------------------------------
-// k_cell_scan: new per-cell counts and birth counts, scanned together
-// (low 32 bits agents, high bits births) with a decoupled look-back over
-// tiles of kCellTile cells.  One tile per block; the grid is exactly the
-// number of tiles and small enough to be resident at once.  The block of
-// the last tile closes the step's bookkeeping in the other status half.
+// k_cell_scan: next per-cell counts and birth counts, scanned together
+// (low 32 bits agents, high bits births), and the occupied cells of the next
+// step, with a decoupled look-back over tiles of kCellTile cells.  Blocks
+// take tiles blockIdx, blockIdx + grid, ...; the grid fits on the device.  Also writes the
+// absolute next-buffer position of every incoming run and of each cell's
+// newborns.  The block of the last tile closes the step's bookkeeping in the
+// other status half.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kScanBlock) k_cell_scan(Ctx x, StepArgs sa) {
-    __shared__ uint64_t s_w[kScanBlock / 64];
-    __shared__ uint64_t s_excl;
-    __shared__ uint64_t s_red[3][kScanBlock / 64];
+    __shared__ uint64_t s_w[kScanBlock / 64], s_w2[kScanBlock / 64];
+    __shared__ uint64_t s_excl, s_excl2;
+    __shared__ uint64_t s_red[kParts][kScanBlock / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    constexpr int per = kCellTile / kScanBlock;
     const int64_t nc = x.ncells;
-    const uint32_t tile = blockIdx.x;
     if (x.st->error) return;
-    const int64_t base = (int64_t)tile * kCellTile;
-    uint64_t v[per];
-    uint64_t sum = 0;
+    // tiles in increasing order per block: a block only ever waits on tiles
+    // that resident blocks hold, whatever the grid size
+    for (uint32_t tile = blockIdx.x; tile < (uint32_t)x.ntiles; tile += gridDim.x) {
+        const int64_t base = (int64_t)tile * kCellTile;
+        uint32_t stay[kScanPer];
+        int32_t e[kScanPer][kMaxIn];
+        uint64_t v[kScanPer];
+        uint64_t sum = 0, osum = 0;
 #pragma unroll
-    for (int q = 0; q < per; q++) {
-        const int64_t c = base + (int64_t)threadIdx.x * per + q;
-        uint64_t val = 0;
-        if (c < nc) {
-            uint32_t cnt = x.cstay[c];
-            const uint32_t b = x.cbirth[c];
-            for (int32_t k = x.in_off[c]; k < x.in_off[c + 1]; k++)
-                cnt += x.cmv[(int64_t)x.in_src[k] * x.conn + x.in_dir[k]];
-            val = (uint64_t)(cnt + b) | ((uint64_t)b << 32);
-        }
-        v[q] = val;
-        sum += val;
-    }
-    // block exclusive scan of the per-thread sums
-    const uint64_t incl = wave_incl_scan_u64(sum);
-    if (lane == 63) s_w[wave] = incl;
-    __syncthreads();
-    uint64_t wpre = 0, tot = 0;
+        for (int q = 0; q < kScanPer; q++) {
+            const int64_t c = base + (int64_t)threadIdx.x * kScanPer + q;
+            const bool ok = c < nc;
+            stay[q] = ok ? x.cstay[c] : 0u;
+            const uint32_t b = ok ? x.cbirth[c] : 0u;
 #pragma unroll
-    for (int w = 0; w < kScanBlock / 64; w++) {
-        if (w < wave) wpre += s_w[w];
-        tot += s_w[w];
-    }
-    if (threadIdx.x < 64) {
-        const uint64_t ex = lookback(x.tiles, tile, tot, &x.st->error);
-        if (threadIdx.x == 0) s_excl = ex;
-    }
-    __syncthreads();
-    uint64_t run = s_excl + wpre + incl - sum;
+            for (int k = 0; k < kMaxIn; k++) e[q][k] = (ok && k < x.in_w) ? x.in_e[c * x.in_w + k] : -1;
+            uint32_t inc = 0;
 #pragma unroll
-    for (int q = 0; q < per; q++) {
-        const int64_t c = base + (int64_t)threadIdx.x * per + q;
-        if (c < nc) {
-            x.off_nxt[c] = (uint32_t)run;
-            x.birth_off[c] = (uint32_t)(run >> 32);
+            for (int k = 0; k < kMaxIn; k++) inc += (e[q][k] >= 0) ? x.cmv[e[q][k]] : 0u;
+            v[q] = ok ? ((uint64_t)(stay[q] + inc + b) | ((uint64_t)b << 32)) : 0ull;
+            sum += v[q];
+            osum += ((uint32_t)v[q] != 0u) ? 1u : 0u;
         }
-        run += v[q];
-    }
-    if (base + kCellTile >= nc) {
-        // last tile: step totals (the k_cell_act partials) and bookkeeping
-        uint64_t dsum = 0, msum = 0, fsum = 0;
-        for (int j = threadIdx.x; j < x.nact_blocks; j += kScanBlock) {
-            dsum += x.wave_part[3 * j + 0];
-            msum += x.wave_part[3 * j + 1];
-            fsum += x.wave_part[3 * j + 2];
-        }
-        dsum = wave_sum_u64(dsum);
-        msum = wave_sum_u64(msum);
-        fsum = wave_sum_u64(fsum);
-        if (lane == 0) { s_red[0][wave] = dsum; s_red[1][wave] = msum; s_red[2][wave] = fsum; }
+        // block exclusive scan of the per-thread sums
+        const uint64_t incl = wave_incl_scan_u64(sum);
+        const uint64_t oincl = wave_incl_scan_u64(osum);
+        if (lane == 63) { s_w[wave] = incl; s_w2[wave] = oincl; }
         __syncthreads();
-        if (threadIdx.x == kScanBlock - 1) {
-            uint64_t D = 0, Mv = 0, Fe = 0;
-            for (int w = 0; w < kScanBlock / 64; w++) { D += s_red[0][w]; Mv += s_red[1][w]; Fe += s_red[2][w]; }
-            const int64_t total = (int64_t)(uint32_t)run;
-            const int64_t B = (int64_t)(run >> 32);
-            x.off_nxt[nc] = (uint32_t)total;
-            x.birth_off[nc] = (uint32_t)B;
-            DevStatus *st = x.st;
-            const int p = (int)sa.parity, q = 1 - p;
-            if (total > x.cap || (x.gwords && st->gpool[p] + B > x.gcap)) st->error |= kErrCapacity;
-            st->agent_steps += st->n_alive[p];
-            st->n_alive[q] = total;
-            st->max_id[q] = st->max_id[p] + B;
-            st->gpool[q] = st->gpool[p] + (x.gwords ? B : 0);
-            st->births_last = B;
-            st->deaths_last = (int64_t)D;
-            st->moves_last = (int64_t)Mv;
-            st->births_total += B;
-            st->deaths_total += (int64_t)D;
-            st->moves_total += (int64_t)Mv;
-            st->fertile_total += (int64_t)Fe;
-            st->step = (int64_t)sa.step + 1;
-            st->t = sa.t + 1.0;
-            if (x.hring) {
-                DevStatus *dst = x.hring + (st->step % kStatusRing);
-                const uint64_t *srcw = reinterpret_cast<const uint64_t *>(st);
-                uint64_t *dstw = reinterpret_cast<uint64_t *>(dst);
-                for (int k = 0; k < (int)(sizeof(DevStatus) / 8); k++) dstw[k] = srcw[k];
+        uint64_t wpre = 0, tot = 0, owpre = 0, otot = 0;
+#pragma unroll
+        for (int w = 0; w < kScanBlock / 64; w++) {
+            if (w < wave) { wpre += s_w[w]; owpre += s_w2[w]; }
+            tot += s_w[w];
+            otot += s_w2[w];
+        }
+        if (threadIdx.x < 64) {
+            uint64_t ex, ex2;
+            lookback2(x.tiles, x.tiles2, tile, tot, otot, &x.st->error, ex, ex2);
+            if (threadIdx.x == 0) { s_excl = ex; s_excl2 = ex2; }
+        }
+        __syncthreads();
+        uint64_t run = s_excl + wpre + incl - sum;
+        uint64_t orun = s_excl2 + owpre + oincl - osum;
+#pragma unroll
+        for (int q = 0; q < kScanPer; q++) {
+            const int64_t c = base + (int64_t)threadIdx.x * kScanPer + q;
+            if (c < nc) {
+                const uint32_t o = (uint32_t)run;
+                x.off_nxt[c] = o;
+                x.birth_off[c] = (uint32_t)(run >> 32);
+                // run positions: the mover counts are read again (cache-resident)
+                // rather than kept in registers through the look-back
+                uint32_t cm[kMaxIn];
+#pragma unroll
+                for (int k = 0; k < kMaxIn; k++) cm[k] = (e[q][k] >= 0) ? x.cmv[e[q][k]] : 0u;
+                uint32_t pos = o + stay[q];
+#pragma unroll
+                for (int k = 0; k < kMaxIn; k++) {
+                    if (e[q][k] >= 0) x.rpos[e[q][k]] = pos;
+                    pos += cm[k];
+                }
+                x.bpos[c] = pos;
+                const uint32_t nn = (uint32_t)v[q];
+                if (nn != 0) {
+                    x.occ_nxt[orun] = OccCell{(int32_t)c, o, nn, 0u};
+                    orun++;
+                }
+            }
+            run += v[q];
+        }
+        if (base + kCellTile >= nc) {
+            // last tile: step totals (the k_cell_act partials) and bookkeeping
+            uint64_t part[kParts] = {0, 0, 0};
+            for (int j = threadIdx.x; j < x.nact_blocks; j += kScanBlock) {
+#pragma unroll
+                for (int q = 0; q < kParts; q++) part[q] += x.wave_part[kParts * j + q];
+            }
+#pragma unroll
+            for (int q = 0; q < kParts; q++) {
+                part[q] = wave_sum_u64(part[q]);
+                if (lane == 0) s_red[q][wave] = part[q];
+            }
+            __syncthreads();
+            if (threadIdx.x == kScanBlock - 1) {
+                uint64_t D = 0, Mv = 0, Fe = 0;
+                for (int w = 0; w < kScanBlock / 64; w++) { D += s_red[0][w]; Mv += s_red[1][w]; Fe += s_red[2][w]; }
+                const int64_t total = (int64_t)(uint32_t)run;
+                const int64_t B = (int64_t)(run >> 32);
+                x.off_nxt[nc] = (uint32_t)total;
+                x.birth_off[nc] = (uint32_t)B;
+                DevStatus *st = x.st;
+                const int p = (int)sa.parity, q = 1 - p;
+                if (total > x.cap || (x.gwords && st->gpool[p] + B > x.gcap)) st->error |= kErrCapacity;
+                st->agent_steps += st->n_alive[p];
+                st->occupied_total += st->n_occ[p];
+                st->n_alive[q] = total;
+                st->n_occ[q] = (int64_t)orun;
+                st->max_id[q] = st->max_id[p] + B;
+                st->gpool[q] = st->gpool[p] + (x.gwords ? B : 0);
+                st->births_last = B;
+                st->deaths_last = (int64_t)D;
+                st->moves_last = (int64_t)Mv;
+                st->births_total += B;
+                st->deaths_total += (int64_t)D;
+                st->moves_total += (int64_t)Mv;
+                st->fertile_total += (int64_t)Fe;
+                st->step = (int64_t)sa.step + 1;
+                st->t = sa.t + 1.0;
+                if (x.hring) {
+                    DevStatus *dst = x.hring + (st->step % kStatusRing);
+                    const uint64_t *srcw = reinterpret_cast<const uint64_t *>(st);
+                    uint64_t *dstw = reinterpret_cast<uint64_t *>(dst);
+                    for (int k = 0; k < (int)(sizeof(DevStatus) / 8); k++) dstw[k] = srcw[k];
+                }
             }
         }
+    __syncthreads();
     }
 }
 
@@ -726,145 +877,148 @@ __device__ void make_genome(const Ctx &x, const Prog &p, uint32_t step, uint64_t
 }
 
 // ---------------------------------------------------------------------------
-// k_cell_scatter: next buffer segment of one destination cell
+// k_cell_scatter: agents of one occupied source cell to the next buffer
 // ---------------------------------------------------------------------------
-// Append the slots of [s, s+n) whose outcome byte satisfies want(code) to
-// list[cnt..] (at most lim entries in total); returns the new count.
-template <typename Want>
-__device__ __forceinline__ int32_t gather_slots(const Ctx &x, uint32_t s, int32_t n, int32_t *list, int32_t cnt,
-                                                int32_t lim, Want want) {
+__device__ __forceinline__ void cell_scatter(const Ctx &x, const StepArgs &sa, int32_t c, uint32_t s, int32_t n,
+                                             int32_t *mli, int64_t *mid, int64_t max_id, int64_t gbase) {
     const int lane = threadIdx.x;
     const uint64_t lt = lanemask_lt();
+    const uint32_t nbase = x.off_nxt[c];
+    const uint32_t bp = x.bpos[c];
+    const uint32_t boff = x.birth_off[c];
+    uint32_t rp[6];
+#pragma unroll
+    for (int d = 0; d < 6; d++) rp[d] = (d < x.conn) ? x.rpos[(int64_t)c * x.conn + d] : 0u;
+    uint32_t nstay = 0, nmoth = 0;
+    uint32_t nrun[6] = {0, 0, 0, 0, 0, 0};
     for (int32_t b = 0; b < n; b += kWave) {
         const int32_t li = b + lane;
-        const bool take = li < n && want(x.dec[s + li]);
-        const uint64_t m = __ballot(take);
-        if (take) {
-            const int32_t at = cnt + __popcll(m & lt);
-            if (at < lim) list[at] = (int32_t)(s + li);
+        const bool valid = li < n;
+        const uint32_t from = s + (uint32_t)li;
+        uint8_t code = kDecDead;
+        int64_t id = 0, mate = 0;
+        uint8_t life = 0, g = 0;
+        double birth = 0, age = 0, lastb = 0;
+        int32_t nbab = 0, gh = 0;
+        if (valid) {
+            code = x.dec[from];
+            id = x.cur.id[from];
+            mate = x.cur.mate_id[from];
+            life = x.cur.life[from];
+            g = x.cur.gender[from];
+            birth = x.cur.birth[from];
+            age = x.cur.age[from];
+            lastb = x.cur.lastb[from];
+            nbab = x.cur.nbabies[from];
+            if (x.gwords) gh = x.cur.ghandle[from];
         }
-        cnt += __popcll(m);
+        const bool alive = valid && !(code & kDecDead);
+        const uint32_t dir = code & kDecDirMask;
+        int64_t to = -1;
+        const uint64_t ms = __ballot(alive && dir == 0);
+        if (alive && dir == 0) to = (int64_t)nbase + nstay + __popcll(ms & lt);
+        nstay += __popcll(ms);
+#pragma unroll
+        for (int d = 0; d < 6; d++) {
+            const bool mine = alive && dir == (uint32_t)(d + 1);
+            const uint64_t mm = __ballot(mine);
+            if (mine) to = (int64_t)rp[d] + nrun[d] + __popcll(mm & lt);
+            nrun[d] += __popcll(mm);
+        }
+        if (to >= 0) {
+            x.nxt.id[to] = id;
+            x.nxt.mate_id[to] = mate;
+            x.nxt.life[to] = life;
+            x.nxt.gender[to] = g;
+            x.nxt.birth[to] = birth;
+            x.nxt.age[to] = age;
+            x.nxt.lastb[to] = lastb;
+            x.nxt.nbabies[to] = nbab;
+            if (x.gwords) x.nxt.ghandle[to] = gh;
+        }
+        const bool isb = valid && (code & kDecBirth);
+        const uint64_t mb = __ballot(isb);
+        if (isb) {
+            const uint32_t at = nmoth + __popcll(mb & lt);
+            mli[at] = li;
+            mid[at] = id;
+        }
+        nmoth += __popcll(mb);
     }
-    return cnt;
+    // the counts of this cell were consumed by k_cell_scan: leave zeros for
+    // the next step (k_cell_act writes occupied cells only)
+    if (lane == 0) { x.cstay[c] = 0; x.cbirth[c] = 0; }
+    if (lane < x.conn) x.cmv[(int64_t)c * x.conn + lane] = 0;
+    if (nmoth == 0) return;
+    wave_sync();
+    // newborns in mother-id order: createAgentAtIndex + makePopSpecificOffspring
+    // (SPopulation.cpp:880-918); new ids rank births by (cell, mother id)
+    for (uint32_t j = lane; j < nmoth; j += kWave) {
+        const int64_t my = mid[j];
+        uint32_t r = 0;
+        for (uint32_t k = 0; k < nmoth; k++) r += mid[k] < my;
+        const int64_t to = (int64_t)bp + r;
+        const int64_t rg = (int64_t)boff + r;
+        const int64_t nid = max_id + 1 + rg;
+        x.nxt.id[to] = nid;
+        x.nxt.mate_id[to] = -3;
+        x.nxt.birth[to] = sa.t;
+        x.nxt.age[to] = 0.0;
+        x.nxt.lastb[to] = 0.0;
+        x.nxt.nbabies[to] = 0;
+        const uint32_t u = draw(x.seed, sa.step, QHG_STREAM_NEWBORN, (uint64_t)nid, 0);
+        const uint8_t gg = (uint8_t)(2 * wrandd(u));
+        x.nxt.gender[to] = gg;
+        x.nxt.life[to] = (gg == 0) ? QHG_LIFE_FERTILE : QHG_LIFE_ALIVE;
+        if (x.gwords) {
+            const uint32_t mslot = s + (uint32_t)mli[j];
+            x.nxt.ghandle[to] = (int32_t)(gbase + rg);
+            x.brec[2 * rg] = (int32_t)mslot;
+            x.brec[2 * rg + 1] = x.bfather[mslot];
+        }
+    }
+    wave_sync();
 }
 
-// number of entries of the id-sorted run v[lo, hi) below key
-__device__ __forceinline__ int32_t count_below(const int64_t *v, int32_t lo, int32_t hi, int64_t key) {
-    const int32_t base = lo;
-    while (lo < hi) {
-        const int32_t mid = (lo + hi) >> 1;
-        if (v[mid] < key) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo - base;
-}
-
-__global__ void __launch_bounds__(kCellBlock) k_cell_scatter(Ctx x, Prog p, StepArgs sa) {
-    __shared__ int32_t s_cand[kLdsCell];
-    __shared__ int64_t s_cid[kLdsCell];
-    __shared__ int32_t s_run[kMaxRuns + 1];
+__global__ void __launch_bounds__(kCellBlock) k_cell_scatter(Ctx x, StepArgs sa) {
+    __shared__ int32_t s_mli[kLdsCell];
+    __shared__ int64_t s_mid[kLdsCell];
     const int lane = threadIdx.x;
-    const uint64_t lt = lanemask_lt();
     const int pr = (int)sa.parity;
     if (x.st->error) return;
     const int64_t max_id = x.st->max_id[pr];
     const int64_t gbase = x.st->gpool[pr];
+    const int64_t nocc = x.st->n_occ[pr];
+    const int64_t G = gridDim.x;
+    for (int64_t base = blockIdx.x; base < nocc; base += kWave * G) {
+        const int64_t i = base + (int64_t)lane * G;
+        OccCell e{0, 0, 0, 0};
+        if (i < nocc) e = x.occ_cur[i];
+        uint64_t todo = __ballot(i < nocc);
+        while (todo) {
+            const int k = __ffsll((unsigned long long)todo) - 1;
+            todo &= todo - 1;
+            const int32_t c = (int32_t)readlane_u32((uint32_t)e.c, k);
+            const uint32_t s = readlane_u32(e.s, k);
+            const int32_t n = (int32_t)readlane_u32(e.n, k);
+            // mothers of a large cell go to the cell's own range of global scratch
+            int32_t *mli = (n > kLdsCell) ? x.gfu + s : s_mli;
+            int64_t *mid = (n > kLdsCell) ? x.gfid + s : s_mid;
+            cell_scatter(x, sa, c, s, n, mli, mid, max_id, gbase);
+        }
+    }
+}
 
-    for (int32_t c = blockIdx.x; c < x.ncells; c += gridDim.x) {
-        const uint32_t ns = x.off_nxt[c];
-        const int32_t nnew = (int32_t)(x.off_nxt[c + 1] - ns);
-        if (nnew == 0) continue;
-        const int32_t nb = (int32_t)x.cbirth[c];
-        const int32_t m = nnew - nb;     // stayers + arriving movers
-        const uint32_t s = x.off_cur[c];
-        const int32_t n = (int32_t)(x.off_cur[c + 1] - s);
-        const bool big = m > kLdsCell;
-        int32_t *cand = big ? x.gcand + ns : s_cand;
-        int64_t *cid = big ? x.gcid + ns : s_cid;
-        // runs: the stayers, then the movers from each incoming neighbour
-        // slot; every run is a subsequence of an id-sorted segment
-        int32_t cnt = gather_slots(x, s, n, cand, 0, m, [](uint8_t d) {
-            return !(d & kDecDead) && (d & kDecDirMask) == 0;
-        });
-        int32_t nrun = 1;
-        if (lane == 0) { s_run[0] = 0; s_run[1] = cnt; }
-        for (int32_t k = x.in_off[c]; k < x.in_off[c + 1]; k++) {
-            const int32_t src = x.in_src[k];
-            const uint8_t want_dir = (uint8_t)(x.in_dir[k] + 1);
-            const uint32_t ss = x.off_cur[src];
-            const int32_t sn = (int32_t)(x.off_cur[src + 1] - ss);
-            const int32_t before = cnt;
-            cnt = gather_slots(x, ss, sn, cand, cnt, m, [want_dir](uint8_t d) {
-                return !(d & kDecDead) && (d & kDecDirMask) == want_dir;
-            });
-            if (cnt > before) {
-                nrun++;
-                if (lane == 0 && nrun <= kMaxRuns) s_run[nrun] = cnt;
-            }
-        }
-        if (cnt != m || nrun > kMaxRuns) {
-            if (lane == 0) atomicOr(&x.st->error, kErrGather);
-            continue;
-        }
-        __syncthreads();
-        for (int32_t i = lane; i < cnt; i += kWave) cid[i] = x.cur.id[cand[i]];
-        __syncthreads();
-        // rank by id (unique): own position in its run + entries below it in
-        // every other run; then copy the record
-        for (int32_t i = lane; i < cnt; i += kWave) {
-            const int64_t my = cid[i];
-            int32_t r = 0;
-            for (int32_t q = 0; q < nrun; q++) {
-                const int32_t lo = s_run[q], hi = s_run[q + 1];
-                r += (i >= lo && i < hi) ? (i - lo) : count_below(cid, lo, hi, my);
-            }
-            const int32_t from = cand[i];
-            const int64_t to = (int64_t)ns + r;
-            x.nxt.id[to] = my;
-            x.nxt.mate_id[to] = x.cur.mate_id[from];
-            x.nxt.life[to] = x.cur.life[from];
-            x.nxt.gender[to] = x.cur.gender[from];
-            x.nxt.birth[to] = x.cur.birth[from];
-            x.nxt.age[to] = x.cur.age[from];
-            x.nxt.lastb[to] = x.cur.lastb[from];
-            x.nxt.nbabies[to] = x.cur.nbabies[from];
-            if (x.gwords) x.nxt.ghandle[to] = x.cur.ghandle[from];
-        }
-        // newborns of the cell's mothers, in mother (= id) order, after the
-        // existing agents (their ids are larger than every existing id)
-        int32_t rb = 0;
-        const int64_t boff = x.birth_off[c];
-        for (int32_t b = 0; b < n; b += kWave) {
-            const int32_t li = b + lane;
-            const bool isb = li < n && (x.dec[s + li] & kDecBirth);
-            const uint64_t mb = __ballot(isb);
-            if (isb) {
-                const int32_t r = rb + __popcll(mb & lt);
-                const int64_t to = (int64_t)ns + m + r;
-                const int64_t rg = boff + r;
-                const int64_t nid = max_id + 1 + rg;
-                // createAgentAtIndex + makePopSpecificOffspring (SPopulation.cpp:880-918)
-                x.nxt.id[to] = nid;
-                x.nxt.mate_id[to] = -3;
-                x.nxt.birth[to] = sa.t;
-                x.nxt.age[to] = 0.0;
-                x.nxt.lastb[to] = 0.0;
-                x.nxt.nbabies[to] = 0;
-                const uint32_t u = draw(x.seed, sa.step, QHG_STREAM_NEWBORN, (uint64_t)nid, 0);
-                const uint8_t g = (uint8_t)(2 * wrandd(u));
-                x.nxt.gender[to] = g;
-                x.nxt.life[to] = (g == 0) ? QHG_LIFE_FERTILE : QHG_LIFE_ALIVE;
-                if (x.gwords) {
-                    const int64_t gh = gbase + rg;
-                    x.nxt.ghandle[to] = (int32_t)gh;
-                    const int32_t mslot = (int32_t)(s + li);
-                    make_genome(x, p, sa.step, (uint64_t)nid, gh, x.cur.ghandle[mslot],
-                                x.cur.ghandle[x.bfather[mslot]]);
-                }
-            }
-            rb += __popcll(mb);
-        }
-        __syncthreads();
+// k_genome: one thread per birth of the step (Genetics::makeOffspring)
+__global__ void __launch_bounds__(256) k_genome(Ctx x, Prog p, StepArgs sa) {
+    if (x.st->error) return;
+    const int pr = (int)sa.parity;
+    const int64_t B = x.st->births_last;
+    const int64_t max_id = x.st->max_id[pr];
+    const int64_t gbase = x.st->gpool[pr];
+    for (int64_t rg = (int64_t)blockIdx.x * 256 + threadIdx.x; rg < B; rg += (int64_t)gridDim.x * 256) {
+        const int32_t ms = x.brec[2 * rg], fs = x.brec[2 * rg + 1];
+        make_genome(x, p, sa.step, (uint64_t)(max_id + 1 + rg), gbase + rg, x.cur.ghandle[ms], x.cur.ghandle[fs]);
     }
 }
 

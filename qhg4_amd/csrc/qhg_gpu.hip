@@ -41,8 +41,8 @@ int fail(int code, const std::string &msg) {
             return fail(QHG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-enum KernelId { KB_ACT, KB_SCAN, KB_SCATTER, KB_CAPACITY, KB_EVAL, KB_GPOOL, KB_COUNT };
-const char *kKernelNames[KB_COUNT] = {"k_cell_act", "k_cell_scan", "k_cell_scatter",
+enum KernelId { KB_ACT, KB_SCAN, KB_SCATTER, KB_GENOME, KB_CAPACITY, KB_EVAL, KB_GPOOL, KB_COUNT };
+const char *kKernelNames[KB_COUNT] = {"k_cell_act", "k_cell_scan", "k_cell_scatter", "k_genome",
                                       "k_capacity", "k_eval", "k_gpool_compact"};
 
 constexpr int kRing = kStatusRing;
@@ -81,16 +81,18 @@ struct qhg_pop {
     hipStream_t stream = nullptr;
     int device = 0;
     int num_cu = 256;
-    int act_grid = 1, scatter_grid = 1, ntiles = 1;
+    int act_grid = 1, scatter_grid = 1, scan_grid = 1, ntiles = 1;
 
     // agents: two cell-sorted buffers of cap slots, segment offsets per buffer
     AgentBuf buf[2];
     int64_t cap = 0;
     uint32_t *d_off[2] = {nullptr, nullptr};
+    OccCell *d_occ[2] = {nullptr, nullptr};   // occupied-cell lists per buffer
     // per-slot step scratch (cap entries)
     uint8_t *d_dec = nullptr, *d_gflag = nullptr;
-    int32_t *d_bfather = nullptr, *d_gfem = nullptr, *d_gmal = nullptr, *d_gmate = nullptr, *d_gcand = nullptr;
-    int64_t *d_gcid = nullptr;
+    int32_t *d_bfather = nullptr, *d_brec = nullptr;
+    int32_t *d_gfu = nullptr, *d_gmu = nullptr, *d_gfs = nullptr, *d_gms = nullptr, *d_gmate = nullptr;
+    int64_t *d_gfid = nullptr, *d_gmid = nullptr;
     // genome pool (gcap handles), a second pool for compaction
     uint64_t *d_gpool[2] = {nullptr, nullptr};
     int gp_cur = 0;
@@ -98,13 +100,14 @@ struct qhg_pop {
     // grid and per-cell tables
     int32_t *d_nbr = nullptr;
     uint8_t *d_nn = nullptr;
-    int32_t *d_in_off = nullptr, *d_in_src = nullptr;
-    uint8_t *d_in_dir = nullptr;
+    int32_t *d_in_e = nullptr;
+    int32_t in_w = 1;
     double *d_K = nullptr, *d_W = nullptr, *d_tmp = nullptr;
     uint8_t *d_ice = nullptr, *d_coastal = nullptr;
     double *d_env[7] = {};   // alt lon lat water npp temp rain
     uint32_t *d_cstay = nullptr, *d_cbirth = nullptr, *d_cmv = nullptr, *d_birth_off = nullptr;
-    uint64_t *d_tiles = nullptr;
+    uint32_t *d_bpos = nullptr, *d_rpos = nullptr;
+    uint64_t *d_tiles = nullptr, *d_tiles2 = nullptr;
     uint32_t *d_wave_part = nullptr;
     DevStatus *d_st = nullptr;
 
@@ -142,6 +145,8 @@ struct qhg_pop {
         x.nxt = buf[1 - b];
         x.off_cur = d_off[b];
         x.off_nxt = d_off[1 - b];
+        x.occ_cur = d_occ[b];
+        x.occ_nxt = d_occ[1 - b];
         x.cap = cap;
         x.ncells = ncells;
         x.conn = conn;
@@ -150,22 +155,27 @@ struct qhg_pop {
         x.K = d_K;
         x.W = d_W;
         x.ice = d_ice;
-        x.in_off = d_in_off;
-        x.in_src = d_in_src;
-        x.in_dir = d_in_dir;
+        x.in_e = d_in_e;
+        x.in_w = in_w;
         x.dec = d_dec;
         x.bfather = d_bfather;
-        x.gfem = d_gfem;
-        x.gmal = d_gmal;
+        x.brec = d_brec;
+        x.gfu = d_gfu;
+        x.gmu = d_gmu;
+        x.gfs = d_gfs;
+        x.gms = d_gms;
         x.gmate = d_gmate;
+        x.gfid = d_gfid;
+        x.gmid = d_gmid;
         x.gflag = d_gflag;
-        x.gcand = d_gcand;
-        x.gcid = d_gcid;
         x.cstay = d_cstay;
         x.cbirth = d_cbirth;
         x.cmv = d_cmv;
         x.birth_off = d_birth_off;
+        x.bpos = d_bpos;
+        x.rpos = d_rpos;
         x.tiles = d_tiles;
+        x.tiles2 = d_tiles2;
         x.ntiles = ntiles;
         x.nact_blocks = act_grid;
         x.wave_part = d_wave_part;
@@ -237,25 +247,33 @@ int collect_timing(qhg_pop *p) {
 }
 
 void free_slot_scratch(qhg_pop *p) {
-    void *ptrs[] = {p->d_dec, p->d_gflag, p->d_bfather, p->d_gfem, p->d_gmal, p->d_gmate, p->d_gcand, p->d_gcid};
+    void *ptrs[] = {p->d_dec, p->d_gflag, p->d_bfather, p->d_brec, p->d_gfu, p->d_gmu, p->d_gfs, p->d_gms,
+                    p->d_gmate, p->d_gfid, p->d_gmid};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     p->d_dec = p->d_gflag = nullptr;
-    p->d_bfather = p->d_gfem = p->d_gmal = p->d_gmate = p->d_gcand = nullptr;
-    p->d_gcid = nullptr;
+    p->d_bfather = p->d_brec = p->d_gfu = p->d_gmu = p->d_gfs = p->d_gms = p->d_gmate = nullptr;
+    p->d_gfid = p->d_gmid = nullptr;
 }
 
+// per-slot scratch: outcome bytes, birth records, and the list scratch of
+// cells too large for LDS (indexed by segment slot, so cells never overlap)
 int alloc_slot_scratch(qhg_pop *p, int64_t cap) {
     free_slot_scratch(p);
     const size_t n = (size_t)cap;
     HIPCHK(hipMalloc(&p->d_dec, n));
     HIPCHK(hipMalloc(&p->d_gflag, n));
-    HIPCHK(hipMalloc(&p->d_bfather, n * 4));
-    HIPCHK(hipMalloc(&p->d_gfem, n * 4));
-    HIPCHK(hipMalloc(&p->d_gmal, n * 4));
+    HIPCHK(hipMalloc(&p->d_gfu, n * 4));
+    HIPCHK(hipMalloc(&p->d_gmu, n * 4));
+    HIPCHK(hipMalloc(&p->d_gfs, n * 4));
+    HIPCHK(hipMalloc(&p->d_gms, n * 4));
     HIPCHK(hipMalloc(&p->d_gmate, n * 4));
-    HIPCHK(hipMalloc(&p->d_gcand, n * 4));
-    HIPCHK(hipMalloc(&p->d_gcid, n * 8));
+    HIPCHK(hipMalloc(&p->d_gfid, n * 8));
+    HIPCHK(hipMalloc(&p->d_gmid, n * 8));
+    if (p->gwords) {
+        HIPCHK(hipMalloc(&p->d_bfather, n * 4));
+        HIPCHK(hipMalloc(&p->d_brec, n * 8));
+    }
     return 0;
 }
 
@@ -593,10 +611,14 @@ int phase_finalize(qhg_pop *p) {
     }
     const Ctx x = p->ctx();
     const StepArgs sa = p->step_args();
-    int rc = launch_bs(p, KB_SCAN, k_cell_scan, p->ntiles, kScanBlock, x, sa);
+    int rc = launch_bs(p, KB_SCAN, k_cell_scan, p->scan_grid, kScanBlock, x, sa);
     if (rc) return rc;
-    rc = launch_bs(p, KB_SCATTER, k_cell_scatter, p->scatter_grid, kCellBlock, x, p->prog, sa);
+    rc = launch_bs(p, KB_SCATTER, k_cell_scatter, p->scatter_grid, kCellBlock, x, sa);
     if (rc) return rc;
+    if (p->gwords) {
+        rc = launch(p, KB_GENOME, k_genome, p->num_cu * 4, x, p->prog, sa);
+        if (rc) return rc;
+    }
     p->acted = false;
     p->initialized = false;
     p->steps_enqueued++;
@@ -634,29 +656,25 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
             return fail(QHG_ERR_ARG, "neighbour index out of range");
     for (int32_t c = 0; c < nc32; c++)
         if (grid->num_neighbors[c] > conn) return fail(QHG_ERR_ARG, "num_neighbors > connectivity");
-    // incoming neighbour lists: (src, slot) for every slot a move can take
-    std::vector<int32_t> in_off((size_t)nc32 + 1, 0), in_src;
-    std::vector<uint8_t> in_dir;
+    // incoming runs per cell, padded: src*conn + slot for every slot a move
+    // can take (slots below num_neighbors), -1 beyond the cell's in-degree
+    std::vector<int32_t> indeg((size_t)nc32, 0);
     for (int32_t s = 0; s < nc32; s++)
         for (int d = 0; d < grid->num_neighbors[s]; d++) {
             const int32_t c = grid->neighbors[(size_t)s * conn + d];
-            if (c >= 0) in_off[(size_t)c + 1]++;
+            if (c >= 0) indeg[c]++;
         }
-    for (int32_t c = 0; c < nc32; c++) {
-        if (in_off[(size_t)c + 1] > kMaxRuns - 1) return fail(QHG_ERR_ARG, "a cell has more than 14 incoming neighbour slots");
-        in_off[(size_t)c + 1] += in_off[c];
-    }
-    in_src.resize(std::max<size_t>(1, (size_t)in_off[nc32]));
-    in_dir.resize(in_src.size());
+    int32_t in_w = 1;
+    for (int32_t c = 0; c < nc32; c++) in_w = std::max(in_w, indeg[c]);
+    if (in_w > kMaxIn) return fail(QHG_ERR_ARG, "a cell has more than 8 incoming neighbour slots");
+    std::vector<int32_t> in_e((size_t)nc32 * in_w, -1);
     {
-        std::vector<int32_t> fill(in_off.begin(), in_off.end() - 1);
+        std::vector<int32_t> fill((size_t)nc32, 0);
         for (int32_t s = 0; s < nc32; s++)
             for (int d = 0; d < grid->num_neighbors[s]; d++) {
                 const int32_t c = grid->neighbors[(size_t)s * conn + d];
                 if (c < 0) continue;
-                in_src[fill[c]] = s;
-                in_dir[fill[c]] = (uint8_t)d;
-                fill[c]++;
+                in_e[(size_t)c * in_w + fill[c]++] = s * conn + d;
             }
     }
     int ndev = 0;
@@ -665,6 +683,7 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
     p->P = *params;
     p->ncells = nc32;
     p->conn = conn;
+    p->in_w = in_w;
     int rc = build_prog(p);
     if (rc) { delete p; return rc; }
     if (p->has_me || p->has_cap) {
@@ -677,13 +696,12 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
         p->num_cu = prop.multiProcessorCount;
     p->ntiles = (int)((p->ncells + kCellTile - 1) / kCellTile);
     {
-        // the look-back scan needs all of its blocks resident at once
+        // the look-back scan runs at most as many blocks as fit on the device
         int per_cu = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cell_scan, kScanBlock, 0) != hipSuccess ||
             per_cu < 1)
             per_cu = 1;
-        if ((int64_t)p->ntiles > (int64_t)per_cu * p->num_cu)
-            return cleanup_fail(fail(QHG_ERR_ARG, "grid too large: more cell tiles than resident blocks"));
+        p->scan_grid = (int)std::min<int64_t>(p->ntiles, (int64_t)per_cu * p->num_cu);
     }
     p->act_grid = (int)std::min<int64_t>(p->ncells, (int64_t)p->num_cu * QHG_CELL_GRID_PER_CU);
     p->scatter_grid = p->act_grid;
@@ -704,16 +722,18 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
     }
     CK(upload_array(&p->d_nbr, grid->neighbors, nc * p->conn, p->stream));
     CK(upload_array(&p->d_nn, grid->num_neighbors, nc, p->stream));
-    CK(upload_array(&p->d_in_off, in_off.data(), in_off.size(), p->stream));
-    CK(upload_array(&p->d_in_src, in_src.data(), in_src.size(), p->stream));
-    CK(upload_array(&p->d_in_dir, in_dir.data(), in_dir.size(), p->stream));
+    CK(upload_array(&p->d_in_e, in_e.data(), in_e.size(), p->stream));
     if (hipMalloc(&p->d_K, nc * 8) != hipSuccess || hipMalloc(&p->d_W, nc * (p->conn + 1) * 8) != hipSuccess ||
         hipMalloc(&p->d_tmp, nc * 8) != hipSuccess || hipMalloc(&p->d_off[0], (nc + 1) * 4) != hipSuccess ||
         hipMalloc(&p->d_off[1], (nc + 1) * 4) != hipSuccess || hipMalloc(&p->d_cstay, nc * 4) != hipSuccess ||
         hipMalloc(&p->d_cbirth, nc * 4) != hipSuccess || hipMalloc(&p->d_cmv, nc * p->conn * 4) != hipSuccess ||
-        hipMalloc(&p->d_birth_off, (nc + 1) * 4) != hipSuccess ||
+        hipMalloc(&p->d_birth_off, (nc + 1) * 4) != hipSuccess || hipMalloc(&p->d_bpos, nc * 4) != hipSuccess ||
+        hipMalloc(&p->d_rpos, nc * p->conn * 4) != hipSuccess ||
         hipMalloc(&p->d_tiles, (size_t)p->ntiles * 8) != hipSuccess ||
-        hipMalloc(&p->d_wave_part, (size_t)p->act_grid * 3 * 4) != hipSuccess ||
+        hipMalloc(&p->d_tiles2, (size_t)p->ntiles * 8) != hipSuccess ||
+        hipMalloc(&p->d_occ[0], nc * sizeof(OccCell)) != hipSuccess ||
+        hipMalloc(&p->d_occ[1], nc * sizeof(OccCell)) != hipSuccess ||
+        hipMalloc(&p->d_wave_part, (size_t)p->act_grid * kParts * 4) != hipSuccess ||
         hipMalloc(&p->d_st, sizeof(DevStatus)) != hipSuccess)
         return cleanup_fail(fail(QHG_ERR_NOMEM, "device allocation failed"));
     (void)hipMemsetAsync(p->d_K, 0, nc * 8, p->stream);
@@ -721,6 +741,11 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
     (void)hipMemsetAsync(p->d_off[0], 0, (nc + 1) * 4, p->stream);
     (void)hipMemsetAsync(p->d_off[1], 0, (nc + 1) * 4, p->stream);
     (void)hipMemsetAsync(p->d_tiles, 0, (size_t)p->ntiles * 8, p->stream);
+    (void)hipMemsetAsync(p->d_tiles2, 0, (size_t)p->ntiles * 8, p->stream);
+    // per-cell counts start at zero; k_cell_scatter restores zeros after use
+    (void)hipMemsetAsync(p->d_cstay, 0, nc * 4, p->stream);
+    (void)hipMemsetAsync(p->d_cbirth, 0, nc * 4, p->stream);
+    (void)hipMemsetAsync(p->d_cmv, 0, nc * p->conn * 4, p->stream);
     (void)hipMemsetAsync(p->d_st, 0, sizeof(DevStatus), p->stream);
     if (env) {
         const double *src[7] = {env->altitude, env->longitude, env->latitude, env->water,
@@ -753,9 +778,10 @@ int qhg_gpu_destroy(qhg_pop *p) {
     free_agentbuf(p->buf[0]);
     free_agentbuf(p->buf[1]);
     free_slot_scratch(p);
-    void *ptrs[] = {p->d_off[0], p->d_off[1], p->d_gpool[0], p->d_gpool[1], p->d_nbr, p->d_nn, p->d_in_off,
-                    p->d_in_src, p->d_in_dir, p->d_K, p->d_W, p->d_tmp, p->d_ice, p->d_coastal, p->d_cstay,
-                    p->d_cbirth, p->d_cmv, p->d_birth_off, p->d_tiles, p->d_wave_part, p->d_st};
+    void *ptrs[] = {p->d_off[0], p->d_off[1], p->d_gpool[0], p->d_gpool[1], p->d_nbr, p->d_nn, p->d_in_e,
+                    p->d_K, p->d_W, p->d_tmp, p->d_ice, p->d_coastal, p->d_cstay, p->d_cbirth, p->d_cmv,
+                    p->d_birth_off, p->d_bpos, p->d_rpos, p->d_tiles, p->d_tiles2, p->d_occ[0], p->d_occ[1],
+                    p->d_wave_part, p->d_st};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     for (int k = 0; k < 7; k++)
@@ -843,8 +869,19 @@ int qhg_gpu_upload_agents(qhg_pop *p, const qhg_agents *a) {
             std::memcpy(&genome[(size_t)k * p->gwords], a->genome + (size_t)i * p->gwords, (size_t)p->gwords * 8);
         max_id = std::max(max_id, id[k]);
     }
-    for (int32_t c = 0; c < p->ncells; c++) off[(size_t)c + 1] += off[c];
+    std::vector<OccCell> occ;
+    for (int32_t c = 0; c < p->ncells; c++) {
+        if (off[(size_t)c + 1] > 0) occ.push_back(OccCell{c, off[c], off[(size_t)c + 1], 0u});
+        off[(size_t)c + 1] += off[c];
+    }
     const int b = p->cur();
+    const size_t ncz = (size_t)p->ncells;
+    HIPCHK(hipMemsetAsync(p->d_cstay, 0, ncz * 4, p->stream));
+    HIPCHK(hipMemsetAsync(p->d_cbirth, 0, ncz * 4, p->stream));
+    HIPCHK(hipMemsetAsync(p->d_cmv, 0, ncz * p->conn * 4, p->stream));
+    if (!occ.empty())
+        HIPCHK(hipMemcpyAsync(p->d_occ[b], occ.data(), occ.size() * sizeof(OccCell), hipMemcpyHostToDevice,
+                              p->stream));
     const AgentBuf &d = p->buf[b];
     hipStream_t s = p->stream;
     HIPCHK(hipMemcpyAsync(p->d_off[b], off.data(), off.size() * 4, hipMemcpyHostToDevice, s));
@@ -867,6 +904,7 @@ int qhg_gpu_upload_agents(qhg_pop *p, const qhg_agents *a) {
     HIPCHK(hipMemcpyAsync(&st, p->d_st, sizeof(st), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     st.n_alive[b] = n;
+    st.n_occ[b] = (int64_t)occ.size();
     st.max_id[b] = max_id;
     st.gpool[b] = p->gwords ? n : 0;
     st.step = p->steps_enqueued;
@@ -1070,6 +1108,7 @@ int qhg_gpu_stats(qhg_pop *p, qhg_stats *o) {
     o->agent_steps = s.agent_steps;
     o->slot_steps = s.agent_steps;
     o->fertile_total = s.fertile_total;
+    o->occupied_cell_steps = s.occupied_total;
     return 0;
 }
 

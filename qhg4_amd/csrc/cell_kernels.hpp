@@ -45,6 +45,9 @@ constexpr int kWave = 64;
 constexpr int kCellBlock = 64;        // per-cell kernels: one wave per block
 constexpr int kLdsCell = 256;         // cells up to this size keep their lists in LDS,
                                       // larger ones in segment-indexed global scratch
+constexpr int kGroupSlots = 256;      // agents of one packed group of cells
+constexpr int kGroupCells = 16;       // cells of one packed group
+constexpr int kGroupCellMax = 64;     // largest cell the packed pairing takes
 constexpr int kMaxIn = 8;             // incoming neighbour slots per cell
 constexpr int kScanBlock = 256;       // k_cell_scan threads per block
 constexpr int kScanPer = 2;           // cells per k_cell_scan thread
@@ -138,6 +141,8 @@ struct StepArgs {
     uint32_t step;            // step index (Philox counter word)
     uint32_t parity;          // step & 1: which status halves are current
     double t;                 // time of the step
+    int32_t gcells;           // occupied cells per wave group (1..kGroupCells)
+    int32_t pad;
 };
 
 struct Prog {
@@ -165,6 +170,24 @@ __device__ __forceinline__ void wave_sync() {
 
 __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int k) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, k);
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t o = (uint32_t)__shfl_xor((int)v, d, 64);
+        v = (o > v) ? o : v;
+    }
+    return v;
+}
+
+// index of the group cell holding group position q; eb holds the cells'
+// first positions in lanes 0..ng-1 (increasing)
+__device__ __forceinline__ int group_cell_of(uint32_t q, uint32_t eb, int ng) {
+    int k = 0;
+    for (int kk = 1; kk < ng; kk++)
+        if (q >= readlane_u32(eb, kk)) k = kk;
+    return k;
 }
 
 // ---------------------------------------------------------------------------
@@ -432,12 +455,6 @@ __device__ __forceinline__ void cell_act(const Ctx &x, const Prog &p, const Step
     cv.N = (uint32_t)n;
     cv.K = x.K[c];
     cv.nn = x.nn[c];
-    int32_t nbr[6];
-    double w[7];
-#pragma unroll
-    for (int q = 0; q < 6; q++) nbr[q] = (q < x.conn) ? x.nbr[(int64_t)c * x.conn + q] : -1;
-#pragma unroll
-    for (int q = 0; q < 7; q++) w[q] = (q < R) ? x.W[(int64_t)c * R + q] : 0.0;
     const bool v0 = lane < n;
     const uint32_t slot0 = s + (uint32_t)lane;
     uint8_t life0 = 0, g0 = 2;
@@ -563,6 +580,13 @@ __device__ __forceinline__ void cell_act(const Ctx &x, const Prog &p, const Step
         if (valid && a.life != 0) {
             act_program(x, p, sa, cv, a, my_deaths);
             if (a.npend > 0) {
+                // the cell's neighbour and weight rows, for movers only
+                int32_t nbr[6];
+                double w[7];
+#pragma unroll
+                for (int qq = 0; qq < 6; qq++) nbr[qq] = (qq < x.conn) ? x.nbr[(int64_t)c * x.conn + qq] : -1;
+#pragma unroll
+                for (int qq = 0; qq < 7; qq++) w[qq] = (qq < R) ? x.W[(int64_t)c * R + qq] : 0.0;
                 // every registered move counts; the last one wins
                 int d = move_dir(x, a.pk0, a.pu0, cv.nn, w, nbr);
                 if (d >= 0) { dir = d; my_moves++; }
@@ -598,34 +622,275 @@ __device__ __forceinline__ void cell_act(const Ctx &x, const Prog &p, const Step
     wave_sync();
 }
 
-__global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs sa) {
-    __shared__ int32_t s_fu[kLdsCell], s_mu[kLdsCell], s_fs[kLdsCell], s_ms[kLdsCell], s_mate[kLdsCell];
-    __shared__ int64_t s_fid[kLdsCell], s_mid[kLdsCell];
-    __shared__ uint8_t s_flag[kLdsCell];
+// ---------------------------------------------------------------------------
+// packed groups: several small cells in one wave, one lane per agent
+// ---------------------------------------------------------------------------
+enum { kCntF = 0, kCntM, kCntStay, kCntBirth, kCntMv, kCntN = kCntMv + 6 };
+
+struct GroupActLds {
+    int64_t fid[kGroupSlots], mid[kGroupSlots];     // ids of the FERTILE lists
+    uint32_t ud[kGroupSlots];                       // choosers' draws, id order
+    uint32_t cnt[kGroupCells][kCntN];               // per-cell counters
+    int16_t flist[kGroupSlots], mlist[kGroupSlots]; // group positions of the FERTILE agents
+    int16_t fs[kGroupSlots], ms[kGroupSlots];       // id order: list positions
+    int16_t mate[kGroupSlots];                      // partner's group position, -1
+};
+
+struct CellActLds {
+    int32_t fu[kLdsCell], mu[kLdsCell], fs[kLdsCell], ms[kLdsCell], mate[kLdsCell];
+    int64_t fid[kLdsCell], mid[kLdsCell];
+    uint8_t flag[kLdsCell];
+};
+
+union ActLds {
+    GroupActLds g;
+    CellActLds c;
+};
+
+// The cells of a group are consecutive occupied cells, so their segments
+// form one slot range [S0, S0+T).  Lane k < ng holds cell k's entry e and
+// eb = e.s - S0.  Each cell's lists live at its own positions [eb, eb+n).
+__device__ __forceinline__ void group_act(const Ctx &x, const Prog &p, const StepArgs &sa, int ng, const OccCell &e,
+                                          uint32_t eb, uint32_t S0, uint32_t T, GroupActLds &G,
+                                          uint32_t &my_deaths, uint32_t &my_moves, uint32_t &my_fert) {
+    const int lane = threadIdx.x;
+    const int R = x.conn + 1;
+    const int nchunks = (int)((T + kWave - 1) / kWave);
+    if (lane < ng) {
+#pragma unroll
+        for (int q = 0; q < kCntN; q++) G.cnt[lane][q] = 0;
+    }
+
+    // one round trip: the first chunk's fields and its cells' tables
+    const uint32_t q0 = (uint32_t)lane;
+    const bool v0 = q0 < T;
+    const int k0 = group_cell_of(q0, eb, ng);
+    const int32_t c0 = __shfl(e.c, k0, 64);
+    const uint32_t slot0 = S0 + q0;
+    uint8_t life0 = 0, g0 = 2;
+    int64_t id0 = 0;
+    double birth0 = 0, lastb0 = 0, age0 = 0;
+    double K0 = 0;
+    int nn0 = 0;
+    if (v0) {
+        life0 = x.cur.life[slot0];
+        g0 = x.cur.gender[slot0];
+        id0 = x.cur.id[slot0];
+        if (p.need_birth) birth0 = x.cur.birth[slot0];
+        if (p.need_lastb) lastb0 = x.cur.lastb[slot0];
+        if (p.load_age) age0 = x.cur.age[slot0];
+        K0 = x.K[c0];
+        nn0 = x.nn[c0];
+    }
+    wave_sync();
+
+    // ---- RandomPair (RandomPair.cpp:107-279), every cell of the group at once
+    if (p.has_pair) {
+        for (int ch = 0; ch < nchunks; ch++) {
+            const uint32_t q = (uint32_t)(ch * kWave + lane);
+            const bool valid = q < T;
+            const int k = (ch == 0) ? k0 : group_cell_of(q, eb, ng);
+            const uint32_t base = (uint32_t)__shfl((int)eb, k, 64);
+            uint8_t l = life0, g = g0;
+            int64_t id = id0;
+            if (ch > 0 && valid) {
+                l = x.cur.life[S0 + q];
+                g = x.cur.gender[S0 + q];
+                id = x.cur.id[S0 + q];
+            }
+            if (valid) {
+                G.mate[q] = -1;
+                if (l == QHG_LIFE_FERTILE && g == 0) {
+                    const uint32_t r = atomicAdd(&G.cnt[k][kCntF], 1u);
+                    G.flist[base + r] = (int16_t)q;
+                    G.fid[base + r] = id;
+                } else if (l == QHG_LIFE_FERTILE && g == 1) {
+                    const uint32_t r = atomicAdd(&G.cnt[k][kCntM], 1u);
+                    G.mlist[base + r] = (int16_t)q;
+                    G.mid[base + r] = id;
+                }
+            }
+        }
+        wave_sync();
+        // id order inside each cell's lists; the choosing sex (the smaller,
+        // females on a tie) takes its pick draw now
+        for (int ch = 0; ch < nchunks; ch++) {
+            const uint32_t q = (uint32_t)(ch * kWave + lane);
+            const bool valid = q < T;
+            const int k = (ch == 0) ? k0 : group_cell_of(q, eb, ng);
+            const uint32_t base = (uint32_t)__shfl((int)eb, k, 64);
+            if (!valid) continue;
+            const uint32_t Fk = G.cnt[k][kCntF], Mk = G.cnt[k][kCntM];
+            const uint32_t t = q - base;
+            if (t < Fk) {
+                const int64_t my = G.fid[q];
+                uint32_t r = 0;
+                for (uint32_t u = 0; u < Fk; u++) r += G.fid[base + u] < my;
+                G.fs[base + r] = (int16_t)q;
+                if (Fk <= Mk) G.ud[base + r] = draw(x.seed, sa.step, QHG_STREAM_PAIR, (uint64_t)my, 0);
+            }
+            if (t < Mk) {
+                const int64_t my = G.mid[q];
+                uint32_t r = 0;
+                for (uint32_t u = 0; u < Mk; u++) r += G.mid[base + u] < my;
+                G.ms[base + r] = (int16_t)q;
+                if (Mk < Fk) G.ud[base + r] = draw(x.seed, sa.step, QHG_STREAM_PAIR, (uint64_t)my, 0);
+            }
+        }
+        wave_sync();
+        // findMates (RandomPair.cpp:183-276): lane k picks for cell k, each
+        // pick the k-th (id order) still unpaired of the other sex
+        if (lane < ng) {
+            const uint32_t Fk = G.cnt[lane][kCntF], Mk = G.cnt[lane][kCntM];
+            my_fert += Fk + Mk;
+            if (Fk > 0 && Mk > 0) {
+                const bool fsel = Fk <= Mk;
+                const uint32_t nchs = fsel ? Fk : Mk, np = fsel ? Mk : Fk;
+                const int16_t *chs = fsel ? G.fs : G.ms;
+                const int16_t *pls = fsel ? G.ms : G.fs;
+                const int16_t *chl = fsel ? G.flist : G.mlist;
+                const int16_t *pll = fsel ? G.mlist : G.flist;
+                uint64_t mask = (np >= 64) ? ~0ull : ((1ull << np) - 1ull);
+                uint32_t left = np;
+                for (uint32_t q = 0; q < nchs; q++) {
+                    const int kp = (int)wrandr(0, (double)left, G.ud[eb + q]);
+                    const int pos = select_bit(mask, kp);
+                    mask &= ~(1ull << pos);
+                    const int16_t ja = chl[chs[eb + q]], jb = pll[pls[eb + pos]];
+                    G.mate[ja] = jb;
+                    G.mate[jb] = ja;
+                    left--;
+                }
+            }
+        }
+        wave_sync();
+    }
+
+    // ---- actions, one lane per agent
+    for (int ch = 0; ch < nchunks; ch++) {
+        const uint32_t q = (uint32_t)(ch * kWave + lane);
+        const bool valid = q < T;
+        const uint32_t slot = S0 + q;
+        const int k = (ch == 0) ? k0 : group_cell_of(q, eb, ng);
+        const int32_t c = __shfl(e.c, k, 64);
+        const uint32_t nk = (uint32_t)__shfl((int)e.n, k, 64);
+        ActItem a;
+        a.life = life0;
+        a.g = g0;
+        a.id = id0;
+        a.birth = birth0;
+        a.lastb = lastb0;
+        a.age = age0;
+        a.mate = -1;
+        a.age_dirty = false;
+        a.bflag = 0;
+        a.npend = 0;
+        a.pk0 = a.pk1 = 0;
+        a.pu0 = a.pu1 = 0;
+        CellView cv;
+        cv.N = nk;
+        cv.K = K0;
+        cv.nn = nn0;
+        if (ch > 0) {
+            a.life = 0;
+            if (valid) {
+                a.life = x.cur.life[slot];
+                a.id = x.cur.id[slot];
+                if (p.need_gender) a.g = x.cur.gender[slot];
+                if (p.need_birth) a.birth = x.cur.birth[slot];
+                if (p.need_lastb) a.lastb = x.cur.lastb[slot];
+                if (p.load_age) a.age = x.cur.age[slot];
+                cv.K = x.K[c];
+                cv.nn = x.nn[c];
+            }
+        }
+        if (valid && p.has_pair) a.mate = G.mate[q];
+        const uint8_t lifeS = a.life;
+        int dir = -1;
+        if (valid && a.life != 0) {
+            act_program(x, p, sa, cv, a, my_deaths);
+            if (a.npend > 0) {
+                // the cell's neighbour and weight rows, for movers only
+                int32_t nbr[6];
+                double w[7];
+#pragma unroll
+                for (int qq = 0; qq < 6; qq++) nbr[qq] = (qq < x.conn) ? x.nbr[(int64_t)c * x.conn + qq] : -1;
+#pragma unroll
+                for (int qq = 0; qq < 7; qq++) w[qq] = (qq < R) ? x.W[(int64_t)c * R + qq] : 0.0;
+                // every registered move counts; the last one wins
+                int d = move_dir(x, a.pk0, a.pu0, cv.nn, w, nbr);
+                if (d >= 0) { dir = d; my_moves++; }
+                if (a.npend > 1) {
+                    d = move_dir(x, a.pk1, a.pu1, cv.nn, w, nbr);
+                    if (d >= 0) { dir = d; my_moves++; }
+                }
+            }
+        }
+        const bool alive = valid && a.life != 0;
+        if (valid) {
+            uint8_t code = (uint8_t)(dir >= 0 ? dir + 1 : 0);
+            if (!alive) code |= kDecDead;
+            if (a.bflag) code |= kDecBirth;
+            x.dec[slot] = code;
+            if (a.life != lifeS) x.cur.life[slot] = a.life;
+            if (a.age_dirty) x.cur.age[slot] = a.age;
+            if (p.has_pair) x.cur.mate_id[slot] = (a.mate >= 0) ? x.cur.id[S0 + a.mate] : -3;
+            if (a.bflag) {
+                x.cur.nbabies[slot] += 1;
+                if (x.gwords) x.bfather[slot] = (int32_t)(S0 + a.mate);
+                atomicAdd(&G.cnt[k][kCntBirth], 1u);
+            }
+            if (alive) atomicAdd(&G.cnt[k][dir < 0 ? kCntStay : kCntMv + dir], 1u);
+        }
+    }
+    wave_sync();
+    if (lane < ng) {
+        x.cstay[e.c] = G.cnt[lane][kCntStay];
+        x.cbirth[e.c] = G.cnt[lane][kCntBirth];
+        for (int d = 0; d < x.conn; d++) x.cmv[(int64_t)e.c * x.conn + d] = G.cnt[lane][kCntMv + d];
+    }
+    wave_sync();
+}
+
+// four waves per SIMD: at most 128 VGPRs
+__global__ void __launch_bounds__(kCellBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_cell_act(Ctx x, Prog p, StepArgs sa) {
+    __shared__ ActLds lds;
     const int lane = threadIdx.x;
     uint32_t my_deaths = 0, my_moves = 0, my_fert = 0;
     // a failed step (capacity, look-back, count) freezes the population:
     // later steps already enqueued touch nothing
     if (x.st->error) return;
 
-    // look-back words of this step's k_cell_scan start at zero
+    // look-back words of this step's k_cell_scan start at zero, and so do
+    // the step's death and move counts (k_cell_scan adds them up)
     for (int32_t i = blockIdx.x * kCellBlock + lane; i < x.ntiles; i += gridDim.x * kCellBlock) {
         x.tiles[i] = 0;
         x.tiles2[i] = 0;
     }
+    if (blockIdx.x == 0 && lane == 0) {
+        x.st->deaths_last = 0;
+        x.st->moves_last = 0;
+    }
 
-    // occupied cells round-robin over the waves (entry blockIdx + k*grid);
-    // empty cells keep the zero counts the previous scatter left
+    // groups of gcells consecutive occupied cells, round-robin over the
+    // waves; empty cells keep the zero counts the previous scatter left
     const int64_t nocc = x.st->n_occ[sa.parity];
-    const int64_t G = gridDim.x;
-    for (int64_t base = blockIdx.x; base < nocc; base += kWave * G) {
-        const int64_t i = base + (int64_t)lane * G;
+    const int64_t gc = sa.gcells;
+    const int64_t ngroups = (nocc + gc - 1) / gc;
+    for (int64_t gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
+        const int64_t i0 = gi * gc;
+        const int ng = (int)((nocc - i0 < gc) ? (nocc - i0) : gc);
         OccCell e{0, 0, 0, 0};
-        if (i < nocc) e = x.occ_cur[i];
-        uint64_t todo = __ballot(i < nocc);
-        while (todo) {
-            const int k = __ffsll((unsigned long long)todo) - 1;
-            todo &= todo - 1;
+        if (lane < ng) e = x.occ_cur[i0 + lane];
+        const uint32_t S0 = readlane_u32(e.s, 0);
+        const uint32_t T = readlane_u32(e.s, ng - 1) + readlane_u32(e.n, ng - 1) - S0;
+        const uint32_t nmax = wave_max_u32(lane < ng ? e.n : 0u);
+        if (T <= (uint32_t)kGroupSlots && nmax <= (uint32_t)kGroupCellMax) {
+            group_act(x, p, sa, ng, e, e.s - S0, S0, T, lds.g, my_deaths, my_moves, my_fert);
+            continue;
+        }
+        for (int k = 0; k < ng; k++) {
             const int32_t c = (int32_t)readlane_u32((uint32_t)e.c, k);
             const uint32_t s = readlane_u32(e.s, k);
             const int32_t n = (int32_t)readlane_u32(e.n, k);
@@ -634,7 +899,8 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs
                 L = PairLists{x.gfu + s, x.gmu + s, x.gfid + s, x.gmid + s, x.gfs + s, x.gms + s, x.gmate + s,
                               x.gflag + s};
             } else {
-                L = PairLists{s_fu, s_mu, s_fid, s_mid, s_fs, s_ms, s_mate, s_flag};
+                L = PairLists{lds.c.fu, lds.c.mu, lds.c.fid, lds.c.mid, lds.c.fs, lds.c.ms, lds.c.mate,
+                              lds.c.flag};
             }
             cell_act(x, p, sa, c, s, n, L, my_deaths, my_moves, my_fert);
         }
@@ -661,10 +927,29 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs
 __global__ void __launch_bounds__(kScanBlock) k_cell_scan(Ctx x, StepArgs sa) {
     __shared__ uint64_t s_w[kScanBlock / 64], s_w2[kScanBlock / 64];
     __shared__ uint64_t s_excl, s_excl2;
-    __shared__ uint64_t s_red[kParts][kScanBlock / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t nc = x.ncells;
     if (x.st->error) return;
+    {
+        // this block's slice of the k_cell_act partials: deaths, moves and
+        // FERTILE counts, added to the status with one atomic each
+        uint64_t D = 0, Mv = 0, Fe = 0;
+        for (int32_t j = blockIdx.x * kScanBlock + threadIdx.x; j < x.nact_blocks; j += gridDim.x * kScanBlock) {
+            D += x.wave_part[kParts * j + 0];
+            Mv += x.wave_part[kParts * j + 1];
+            Fe += x.wave_part[kParts * j + 2];
+        }
+        D = wave_sum_u64(D);
+        Mv = wave_sum_u64(Mv);
+        Fe = wave_sum_u64(Fe);
+        if (lane == 0 && (D | Mv | Fe) != 0) {
+            atomicAdd((unsigned long long *)&x.st->deaths_last, (unsigned long long)D);
+            atomicAdd((unsigned long long *)&x.st->deaths_total, (unsigned long long)D);
+            atomicAdd((unsigned long long *)&x.st->moves_last, (unsigned long long)Mv);
+            atomicAdd((unsigned long long *)&x.st->moves_total, (unsigned long long)Mv);
+            atomicAdd((unsigned long long *)&x.st->fertile_total, (unsigned long long)Fe);
+        }
+    }
     // tiles in increasing order per block: a block only ever waits on tiles
     // that resident blocks hold, whatever the grid size
     for (uint32_t tile = blockIdx.x; tile < (uint32_t)x.ntiles; tile += gridDim.x) {
@@ -735,53 +1020,28 @@ __global__ void __launch_bounds__(kScanBlock) k_cell_scan(Ctx x, StepArgs sa) {
             }
             run += v[q];
         }
-        if (base + kCellTile >= nc) {
-            // last tile: step totals (the k_cell_act partials) and bookkeeping
-            uint64_t part[kParts] = {0, 0, 0};
-            for (int j = threadIdx.x; j < x.nact_blocks; j += kScanBlock) {
-#pragma unroll
-                for (int q = 0; q < kParts; q++) part[q] += x.wave_part[kParts * j + q];
-            }
-#pragma unroll
-            for (int q = 0; q < kParts; q++) {
-                part[q] = wave_sum_u64(part[q]);
-                if (lane == 0) s_red[q][wave] = part[q];
-            }
-            __syncthreads();
-            if (threadIdx.x == kScanBlock - 1) {
-                uint64_t D = 0, Mv = 0, Fe = 0;
-                for (int w = 0; w < kScanBlock / 64; w++) { D += s_red[0][w]; Mv += s_red[1][w]; Fe += s_red[2][w]; }
-                const int64_t total = (int64_t)(uint32_t)run;
-                const int64_t B = (int64_t)(run >> 32);
-                x.off_nxt[nc] = (uint32_t)total;
-                x.birth_off[nc] = (uint32_t)B;
-                DevStatus *st = x.st;
-                const int p = (int)sa.parity, q = 1 - p;
-                if (total > x.cap || (x.gwords && st->gpool[p] + B > x.gcap)) st->error |= kErrCapacity;
-                st->agent_steps += st->n_alive[p];
-                st->occupied_total += st->n_occ[p];
-                st->n_alive[q] = total;
-                st->n_occ[q] = (int64_t)orun;
-                st->max_id[q] = st->max_id[p] + B;
-                st->gpool[q] = st->gpool[p] + (x.gwords ? B : 0);
-                st->births_last = B;
-                st->deaths_last = (int64_t)D;
-                st->moves_last = (int64_t)Mv;
-                st->births_total += B;
-                st->deaths_total += (int64_t)D;
-                st->moves_total += (int64_t)Mv;
-                st->fertile_total += (int64_t)Fe;
-                st->step = (int64_t)sa.step + 1;
-                st->t = sa.t + 1.0;
-                if (x.hring) {
-                    DevStatus *dst = x.hring + (st->step % kStatusRing);
-                    const uint64_t *srcw = reinterpret_cast<const uint64_t *>(st);
-                    uint64_t *dstw = reinterpret_cast<uint64_t *>(dst);
-                    for (int k = 0; k < (int)(sizeof(DevStatus) / 8); k++) dstw[k] = srcw[k];
-                }
-            }
+        if (base + kCellTile >= nc && threadIdx.x == kScanBlock - 1) {
+            // last tile: the next step's sizes (the totals the step kernels
+            // need; deaths, moves and the host snapshot follow in k_cell_scatter)
+            const int64_t total = (int64_t)(uint32_t)run;
+            const int64_t B = (int64_t)(run >> 32);
+            x.off_nxt[nc] = (uint32_t)total;
+            x.birth_off[nc] = (uint32_t)B;
+            DevStatus *st = x.st;
+            const int p = (int)sa.parity, q = 1 - p;
+            if (total > x.cap || (x.gwords && st->gpool[p] + B > x.gcap)) st->error |= kErrCapacity;
+            st->agent_steps += st->n_alive[p];
+            st->occupied_total += st->n_occ[p];
+            st->n_alive[q] = total;
+            st->n_occ[q] = (int64_t)orun;
+            st->max_id[q] = st->max_id[p] + B;
+            st->gpool[q] = st->gpool[p] + (x.gwords ? B : 0);
+            st->births_last = B;
+            st->births_total += B;
+            st->step = (int64_t)sa.step + 1;
+            st->t = sa.t + 1.0;
         }
-    __syncthreads();
+        __syncthreads();
     }
 }
 
@@ -980,30 +1240,203 @@ __device__ __forceinline__ void cell_scatter(const Ctx &x, const StepArgs &sa, i
     wave_sync();
 }
 
+enum { kScStay = 0, kScMv = 1, kScMoth = 7, kScN = 8 };
+
+struct GroupScatLds {
+    int64_t mid[kGroupSlots];                 // ids of the mothers, per cell region
+    uint32_t cnt[kGroupCells][kScN];          // per-cell running counts
+    int16_t mj[kGroupSlots];                  // group positions of the mothers
+};
+
+struct CellScatLds {
+    int32_t mli[kLdsCell];
+    int64_t mid[kLdsCell];
+};
+
+union ScatLds {
+    GroupScatLds g;
+    CellScatLds c;
+};
+
+// all cells of a group (see group_act): stayers and movers ranked per cell
+// and run by masked ballots, running counts across chunks in LDS
+__device__ __forceinline__ void group_scatter(const Ctx &x, const StepArgs &sa, int ng, const OccCell &e,
+                                              uint32_t eb, uint32_t S0, uint32_t T, GroupScatLds &G,
+                                              int64_t max_id, int64_t gbase) {
+    const int lane = threadIdx.x;
+    const uint64_t lt = lanemask_lt();
+    const int nchunks = (int)((T + kWave - 1) / kWave);
+    if (lane < ng) {
+#pragma unroll
+        for (int q = 0; q < kScN; q++) G.cnt[lane][q] = 0;
+    }
+    wave_sync();
+    for (int ch = 0; ch < nchunks; ch++) {
+        const uint32_t q = (uint32_t)(ch * kWave + lane);
+        const bool valid = q < T;
+        const uint32_t from = S0 + q;
+        const int k = group_cell_of(q, eb, ng);
+        const int32_t c = __shfl(e.c, k, 64);
+        const uint32_t base = (uint32_t)__shfl((int)eb, k, 64);
+        const uint32_t nk = (uint32_t)__shfl((int)e.n, k, 64);
+        uint8_t code = kDecDead;
+        int64_t id = 0, mate = 0;
+        uint8_t life = 0, g = 0;
+        double birth = 0, age = 0, lastb = 0;
+        int32_t nbab = 0, gh = 0;
+        uint32_t nbase = 0;
+        uint32_t rp[6] = {0, 0, 0, 0, 0, 0};
+        if (valid) {
+            code = x.dec[from];
+            id = x.cur.id[from];
+            mate = x.cur.mate_id[from];
+            life = x.cur.life[from];
+            g = x.cur.gender[from];
+            birth = x.cur.birth[from];
+            age = x.cur.age[from];
+            lastb = x.cur.lastb[from];
+            nbab = x.cur.nbabies[from];
+            if (x.gwords) gh = x.cur.ghandle[from];
+            nbase = x.off_nxt[c];
+#pragma unroll
+            for (int d = 0; d < 6; d++)
+                if (d < x.conn) rp[d] = x.rpos[(int64_t)c * x.conn + d];
+        }
+        // lanes of this lane's cell inside the chunk
+        const int lo = (int)base - ch * kWave;
+        const int hi = (int)(base + nk) - ch * kWave;
+        const uint64_t cm = (valid ? ((hi >= 64) ? ~0ull : ((1ull << hi) - 1ull)) : 0ull) &
+                            ~((lo <= 0) ? 0ull : ((1ull << lo) - 1ull));
+        const bool last_of_cell = valid && ((int)lane == ((hi >= 64) ? 63 : hi - 1));
+        const bool alive = valid && !(code & kDecDead);
+        const uint32_t dir = code & kDecDirMask;
+        int64_t to = -1;
+        const uint64_t ms = __ballot(alive && dir == 0);
+        const uint32_t c_stay = valid ? G.cnt[k][kScStay] : 0u;
+        if (alive && dir == 0) to = (int64_t)nbase + c_stay + __popcll(ms & lt & cm);
+        uint32_t add[7];
+        add[0] = __popcll(ms & cm);
+#pragma unroll
+        for (int d = 0; d < 6; d++) {
+            const bool mine = alive && dir == (uint32_t)(d + 1);
+            const uint64_t mm = __ballot(mine);
+            if (mine) to = (int64_t)rp[d] + G.cnt[k][kScMv + d] + __popcll(mm & lt & cm);
+            add[1 + d] = __popcll(mm & cm);
+        }
+        if (to >= 0) {
+            x.nxt.id[to] = id;
+            x.nxt.mate_id[to] = mate;
+            x.nxt.life[to] = life;
+            x.nxt.gender[to] = g;
+            x.nxt.birth[to] = birth;
+            x.nxt.age[to] = age;
+            x.nxt.lastb[to] = lastb;
+            x.nxt.nbabies[to] = nbab;
+            if (x.gwords) x.nxt.ghandle[to] = gh;
+        }
+        if (valid && (code & kDecBirth)) {
+            const uint32_t r = atomicAdd(&G.cnt[k][kScMoth], 1u);
+            G.mj[base + r] = (int16_t)q;
+            G.mid[base + r] = id;
+        }
+        wave_sync();
+        if (last_of_cell) {
+#pragma unroll
+            for (int d = 0; d < 7; d++) G.cnt[k][kScStay + d] += add[d];
+        }
+        wave_sync();
+    }
+    // newborns in mother-id order per cell: createAgentAtIndex +
+    // makePopSpecificOffspring (SPopulation.cpp:880-918); new ids rank births
+    // by (cell, mother id)
+    for (int ch = 0; ch < nchunks; ch++) {
+        const uint32_t q = (uint32_t)(ch * kWave + lane);
+        const bool valid = q < T;
+        const int k = group_cell_of(q, eb, ng);
+        const int32_t c = __shfl(e.c, k, 64);
+        const uint32_t base = (uint32_t)__shfl((int)eb, k, 64);
+        if (!valid) continue;
+        const uint32_t nm = G.cnt[k][kScMoth];
+        const uint32_t t = q - base;
+        if (t >= nm) continue;
+        const int64_t my = G.mid[q];
+        uint32_t r = 0;
+        for (uint32_t u = 0; u < nm; u++) r += G.mid[base + u] < my;
+        const int64_t to = (int64_t)x.bpos[c] + r;
+        const int64_t rg = (int64_t)x.birth_off[c] + r;
+        const int64_t nid = max_id + 1 + rg;
+        x.nxt.id[to] = nid;
+        x.nxt.mate_id[to] = -3;
+        x.nxt.birth[to] = sa.t;
+        x.nxt.age[to] = 0.0;
+        x.nxt.lastb[to] = 0.0;
+        x.nxt.nbabies[to] = 0;
+        const uint32_t u = draw(x.seed, sa.step, QHG_STREAM_NEWBORN, (uint64_t)nid, 0);
+        const uint8_t gg = (uint8_t)(2 * wrandd(u));
+        x.nxt.gender[to] = gg;
+        x.nxt.life[to] = (gg == 0) ? QHG_LIFE_FERTILE : QHG_LIFE_ALIVE;
+        if (x.gwords) {
+            const uint32_t mslot = S0 + (uint32_t)G.mj[q];
+            x.nxt.ghandle[to] = (int32_t)(gbase + rg);
+            x.brec[2 * rg] = (int32_t)mslot;
+            x.brec[2 * rg + 1] = x.bfather[mslot];
+        }
+    }
+    // the counts of these cells were consumed by k_cell_scan: leave zeros
+    // for the next step (k_cell_act writes occupied cells only)
+    if (lane < ng) {
+        x.cstay[e.c] = 0;
+        x.cbirth[e.c] = 0;
+        for (int d = 0; d < x.conn; d++) x.cmv[(int64_t)e.c * x.conn + d] = 0;
+    }
+    wave_sync();
+}
+
+// Block 0 of the k_cell_scatter grid (dispatched first, so it overlaps the
+// scatter work) copies the finished step's status into host memory.
+__device__ void status_snapshot(const Ctx &x) {
+    if (threadIdx.x != 0 || !x.hring) return;
+    const DevStatus *st = x.st;
+    DevStatus *dst = x.hring + (st->step % kStatusRing);
+    const uint64_t *srcw = reinterpret_cast<const uint64_t *>(st);
+    uint64_t *dstw = reinterpret_cast<uint64_t *>(dst);
+    for (int k = 0; k < (int)(sizeof(DevStatus) / 8); k++) dstw[k] = srcw[k];
+}
+
 __global__ void __launch_bounds__(kCellBlock) k_cell_scatter(Ctx x, StepArgs sa) {
-    __shared__ int32_t s_mli[kLdsCell];
-    __shared__ int64_t s_mid[kLdsCell];
+    __shared__ ScatLds lds;
     const int lane = threadIdx.x;
     const int pr = (int)sa.parity;
     if (x.st->error) return;
+    if (blockIdx.x == 0) {
+        status_snapshot(x);
+        return;
+    }
+    const int64_t G = gridDim.x - 1;
+    const int64_t b0 = blockIdx.x - 1;
     const int64_t max_id = x.st->max_id[pr];
     const int64_t gbase = x.st->gpool[pr];
     const int64_t nocc = x.st->n_occ[pr];
-    const int64_t G = gridDim.x;
-    for (int64_t base = blockIdx.x; base < nocc; base += kWave * G) {
-        const int64_t i = base + (int64_t)lane * G;
+    const int64_t gc = sa.gcells;
+    const int64_t ngroups = (nocc + gc - 1) / gc;
+    for (int64_t gi = b0; gi < ngroups; gi += G) {
+        const int64_t i0 = gi * gc;
+        const int ng = (int)((nocc - i0 < gc) ? (nocc - i0) : gc);
         OccCell e{0, 0, 0, 0};
-        if (i < nocc) e = x.occ_cur[i];
-        uint64_t todo = __ballot(i < nocc);
-        while (todo) {
-            const int k = __ffsll((unsigned long long)todo) - 1;
-            todo &= todo - 1;
+        if (lane < ng) e = x.occ_cur[i0 + lane];
+        const uint32_t S0 = readlane_u32(e.s, 0);
+        const uint32_t T = readlane_u32(e.s, ng - 1) + readlane_u32(e.n, ng - 1) - S0;
+        if (T <= (uint32_t)kGroupSlots) {
+            group_scatter(x, sa, ng, e, e.s - S0, S0, T, lds.g, max_id, gbase);
+            continue;
+        }
+        for (int k = 0; k < ng; k++) {
             const int32_t c = (int32_t)readlane_u32((uint32_t)e.c, k);
             const uint32_t s = readlane_u32(e.s, k);
             const int32_t n = (int32_t)readlane_u32(e.n, k);
             // mothers of a large cell go to the cell's own range of global scratch
-            int32_t *mli = (n > kLdsCell) ? x.gfu + s : s_mli;
-            int64_t *mid = (n > kLdsCell) ? x.gfid + s : s_mid;
+            int32_t *mli = (n > kLdsCell) ? x.gfu + s : lds.c.mli;
+            int64_t *mid = (n > kLdsCell) ? x.gfid + s : lds.c.mid;
             cell_scatter(x, sa, c, s, n, mli, mid, max_id, gbase);
         }
     }

@@ -193,6 +193,12 @@ struct qhg_pop {
         a.step = (uint32_t)steps_enqueued;
         a.parity = (uint32_t)cur();
         a.t = t_next;
+        // cells per wave group from the newest known mean occupancy: about
+        // one wave of agents per group (results do not depend on it)
+        const int kb = (int)(known_step & 1);
+        const double occ = (double)std::max<int64_t>(1, known.n_occ[kb]);
+        const double mean = std::max(1.0, (double)known.n_alive[kb] / occ);
+        a.gcells = (int32_t)std::max(1.0, std::min((double)kGroupCells, std::floor(64.0 / mean)));
         return a;
     }
 };
@@ -613,7 +619,8 @@ int phase_finalize(qhg_pop *p) {
     const StepArgs sa = p->step_args();
     int rc = launch_bs(p, KB_SCAN, k_cell_scan, p->scan_grid, kScanBlock, x, sa);
     if (rc) return rc;
-    rc = launch_bs(p, KB_SCATTER, k_cell_scatter, p->scatter_grid, kCellBlock, x, sa);
+    // one more block than the groups use: the step's bookkeeping
+    rc = launch_bs(p, KB_SCATTER, k_cell_scatter, p->scatter_grid + 1, kCellBlock, x, sa);
     if (rc) return rc;
     if (p->gwords) {
         rc = launch(p, KB_GENOME, k_genome, p->num_cu * 4, x, p->prog, sa);

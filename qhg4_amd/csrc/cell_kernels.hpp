@@ -120,7 +120,10 @@ struct Ctx {
     int32_t *gfu, *gmu, *gfs, *gms, *gmate;   // per-slot list scratch of large cells
     int64_t *gfid, *gmid;
     uint8_t *gflag;
-    uint32_t *cstay, *cbirth, *cmv;   // per cell; cmv is ncells * conn
+    uint32_t *cstay, *cbirth;  // per cell: stayers, births (zero outside a step)
+    uint32_t *cin;            // ncells * in_w: movers arriving per incoming run
+    const int32_t *outk;      // ncells * conn: cin index of each (cell, slot) run,
+                              // -1 where the slot takes no moves
     uint32_t *birth_off;      // ncells + 1: births before each cell
     uint32_t *bpos;           // per cell: next-buffer slot of its first newborn
     uint32_t *rpos;           // ncells * conn: next-buffer slot of each (cell, slot) run
@@ -450,24 +453,11 @@ __device__ __forceinline__ void cell_act(const Ctx &x, const Prog &p, const Step
     const int R = x.conn + 1;
     const int nchunks = (n + kWave - 1) / kWave;
 
-    // one round trip: the cell's tables and the first chunk's fields
+    // large cells (rare): fields are loaded per chunk in each pass
     CellView cv;
     cv.N = (uint32_t)n;
     cv.K = x.K[c];
     cv.nn = x.nn[c];
-    const bool v0 = lane < n;
-    const uint32_t slot0 = s + (uint32_t)lane;
-    uint8_t life0 = 0, g0 = 2;
-    int64_t id0 = 0;
-    double birth0 = 0, lastb0 = 0, age0 = 0;
-    if (v0) {
-        life0 = x.cur.life[slot0];
-        g0 = x.cur.gender[slot0];
-        id0 = x.cur.id[slot0];
-        if (p.need_birth) birth0 = x.cur.birth[slot0];
-        if (p.need_lastb) lastb0 = x.cur.lastb[slot0];
-        if (p.load_age) age0 = x.cur.age[slot0];
-    }
 
     // ---- RandomPair: FERTILE females and males of the cell
     if (p.has_pair) {
@@ -475,9 +465,9 @@ __device__ __forceinline__ void cell_act(const Ctx &x, const Prog &p, const Step
         for (int ch = 0; ch < nchunks; ch++) {
             const int32_t li = ch * kWave + lane;
             const bool valid = li < n;
-            uint8_t l = life0, g = g0;
-            int64_t id = id0;
-            if (ch > 0 && valid) {
+            uint8_t l = 0, g = 2;
+            int64_t id = 0;
+            if (valid) {
                 l = x.cur.life[s + li];
                 g = x.cur.gender[s + li];
                 id = x.cur.id[s + li];
@@ -551,28 +541,25 @@ __device__ __forceinline__ void cell_act(const Ctx &x, const Prog &p, const Step
         const bool valid = li < n;
         const uint32_t slot = s + (uint32_t)li;
         ActItem a;
-        a.life = life0;
-        a.g = g0;
-        a.id = id0;
-        a.birth = birth0;
-        a.lastb = lastb0;
-        a.age = age0;
+        a.life = 0;
+        a.g = 0;
+        a.id = 0;
+        a.birth = 0;
+        a.lastb = 0;
+        a.age = 0;
         a.mate = -1;
         a.age_dirty = false;
         a.bflag = 0;
         a.npend = 0;
         a.pk0 = a.pk1 = 0;
         a.pu0 = a.pu1 = 0;
-        if (ch > 0) {
-            a.life = 0;
-            if (valid) {
-                a.life = x.cur.life[slot];
-                a.id = x.cur.id[slot];
-                if (p.need_gender) a.g = x.cur.gender[slot];
-                if (p.need_birth) a.birth = x.cur.birth[slot];
-                if (p.need_lastb) a.lastb = x.cur.lastb[slot];
-                if (p.load_age) a.age = x.cur.age[slot];
-            }
+        if (valid) {
+            a.life = x.cur.life[slot];
+            a.id = x.cur.id[slot];
+            if (p.need_gender) a.g = x.cur.gender[slot];
+            if (p.need_birth) a.birth = x.cur.birth[slot];
+            if (p.need_lastb) a.lastb = x.cur.lastb[slot];
+            if (p.load_age) a.age = x.cur.age[slot];
         }
         if (valid && p.has_pair) a.mate = L.mate[li];
         const uint8_t lifeS = a.life;
@@ -615,10 +602,13 @@ __device__ __forceinline__ void cell_act(const Ctx &x, const Prog &p, const Step
 #pragma unroll
         for (int d = 0; d < 6; d++) mv[d] += __popcll(__ballot(alive && dir == d));
     }
-    if (lane == 0) { x.cstay[c] = stay; x.cbirth[c] = births; }
+    if (lane == 0) {
+        x.cstay[c] = stay;
+        if (births) x.cbirth[c] = births;
+    }
 #pragma unroll
     for (int d = 0; d < 6; d++)
-        if (lane == d && d < x.conn) x.cmv[(int64_t)c * x.conn + d] = mv[d];
+        if (lane == d && d < x.conn && mv[d]) x.cin[x.outk[(int64_t)c * x.conn + d]] = mv[d];
     wave_sync();
 }
 
@@ -628,12 +618,20 @@ __device__ __forceinline__ void cell_act(const Ctx &x, const Prog &p, const Step
 enum { kCntF = 0, kCntM, kCntStay, kCntBirth, kCntMv, kCntN = kCntMv + 6 };
 
 struct GroupActLds {
-    int64_t fid[kGroupSlots], mid[kGroupSlots];     // ids of the FERTILE lists
-    uint32_t ud[kGroupSlots];                       // choosers' draws, id order
+    // FERTILE lists of cell k at positions [eb_k, eb_k + n_k): females from
+    // the front (entry r at eb_k + r), males from the back (at eb_k + n_k-1-r)
+    int64_t lid[kGroupSlots];                       // ids of the list entries
+    int64_t pid[kGroupSlots];                       // partner id per agent
+    double K[kGroupCells];                          // per-cell tables
+    double W[kGroupCells][7];
+    int32_t nbr[kGroupCells][6];
+    int32_t outk[kGroupCells][6];
+    uint32_t ud[kGroupSlots];                       // choosers' draws, id rank
     uint32_t cnt[kGroupCells][kCntN];               // per-cell counters
-    int16_t flist[kGroupSlots], mlist[kGroupSlots]; // group positions of the FERTILE agents
-    int16_t fs[kGroupSlots], ms[kGroupSlots];       // id order: list positions
+    int16_t list[kGroupSlots];                      // group positions of the entries
+    int16_t srt[kGroupSlots];                       // id order: list positions
     int16_t mate[kGroupSlots];                      // partner's group position, -1
+    uint8_t nn[kGroupCells];
 };
 
 struct CellActLds {
@@ -656,32 +654,59 @@ __device__ __forceinline__ void group_act(const Ctx &x, const Prog &p, const Ste
     const int lane = threadIdx.x;
     const int R = x.conn + 1;
     const int nchunks = (int)((T + kWave - 1) / kWave);
-    if (lane < ng) {
-#pragma unroll
-        for (int q = 0; q < kCntN; q++) G.cnt[lane][q] = 0;
-    }
 
-    // one round trip: the first chunk's fields and its cells' tables
+    // one round trip: the cells' tables (staged in LDS) and the first
+    // chunk's fields
+    double tK = 0, tW0 = 0, tW1 = 0;
+    int32_t tN0 = -1, tN1 = -1, tO0 = -1, tO1 = -1;
+    uint8_t tnn = 0;
+    const int wk0 = lane / 7, wj0 = lane % 7, wk1 = (lane + 64) / 7, wj1 = (lane + 64) % 7;
+    const int nk0 = lane / 6, nj0 = lane % 6, nk1 = (lane + 64) / 6, nj1 = (lane + 64) % 6;
+    const int32_t cw0 = __shfl(e.c, wk0 < ng ? wk0 : 0, 64), cw1 = __shfl(e.c, wk1 < ng ? wk1 : 0, 64);
+    const int32_t cn0 = __shfl(e.c, nk0 < ng ? nk0 : 0, 64), cn1 = __shfl(e.c, nk1 < ng ? nk1 : 0, 64);
+    if (lane < ng) {
+        tK = x.K[e.c];
+        tnn = x.nn[e.c];
+    }
+    if (wk0 < ng && wj0 < R) tW0 = x.W[(int64_t)cw0 * R + wj0];
+    if (wk1 < ng && wj1 < R) tW1 = x.W[(int64_t)cw1 * R + wj1];
+    if (nk0 < ng && nj0 < x.conn) {
+        tN0 = x.nbr[(int64_t)cn0 * x.conn + nj0];
+        tO0 = x.outk[(int64_t)cn0 * x.conn + nj0];
+    }
+    if (nk1 < ng && nj1 < x.conn) {
+        tN1 = x.nbr[(int64_t)cn1 * x.conn + nj1];
+        tO1 = x.outk[(int64_t)cn1 * x.conn + nj1];
+    }
     const uint32_t q0 = (uint32_t)lane;
     const bool v0 = q0 < T;
     const int k0 = group_cell_of(q0, eb, ng);
-    const int32_t c0 = __shfl(e.c, k0, 64);
     const uint32_t slot0 = S0 + q0;
     uint8_t life0 = 0, g0 = 2;
     int64_t id0 = 0;
     double birth0 = 0, lastb0 = 0, age0 = 0;
-    double K0 = 0;
-    int nn0 = 0;
+    int32_t nbab0 = 0;
     if (v0) {
         life0 = x.cur.life[slot0];
         g0 = x.cur.gender[slot0];
         id0 = x.cur.id[slot0];
+        nbab0 = x.cur.nbabies[slot0];
         if (p.need_birth) birth0 = x.cur.birth[slot0];
         if (p.need_lastb) lastb0 = x.cur.lastb[slot0];
         if (p.load_age) age0 = x.cur.age[slot0];
-        K0 = x.K[c0];
-        nn0 = x.nn[c0];
     }
+    // per-cell tables to LDS: their loads were issued before the fields', so
+    // waiting for them leaves the field loads in flight
+    if (lane < ng) {
+        G.K[lane] = tK;
+        G.nn[lane] = tnn;
+#pragma unroll
+        for (int q = 0; q < kCntN; q++) G.cnt[lane][q] = 0;
+    }
+    if (wk0 < ng && wj0 < 7) G.W[wk0][wj0] = tW0;
+    if (wk1 < ng && wj1 < 7) G.W[wk1][wj1] = tW1;
+    if (nk0 < ng && nj0 < 6) { G.nbr[nk0][nj0] = tN0; G.outk[nk0][nj0] = tO0; }
+    if (nk1 < ng && nj1 < 6) { G.nbr[nk1][nj1] = tN1; G.outk[nk1][nj1] = tO1; }
     wave_sync();
 
     // ---- RandomPair (RandomPair.cpp:107-279), every cell of the group at once
@@ -691,6 +716,7 @@ __device__ __forceinline__ void group_act(const Ctx &x, const Prog &p, const Ste
             const bool valid = q < T;
             const int k = (ch == 0) ? k0 : group_cell_of(q, eb, ng);
             const uint32_t base = (uint32_t)__shfl((int)eb, k, 64);
+            const uint32_t nk = (uint32_t)__shfl((int)e.n, k, 64);
             uint8_t l = life0, g = g0;
             int64_t id = id0;
             if (ch > 0 && valid) {
@@ -702,12 +728,12 @@ __device__ __forceinline__ void group_act(const Ctx &x, const Prog &p, const Ste
                 G.mate[q] = -1;
                 if (l == QHG_LIFE_FERTILE && g == 0) {
                     const uint32_t r = atomicAdd(&G.cnt[k][kCntF], 1u);
-                    G.flist[base + r] = (int16_t)q;
-                    G.fid[base + r] = id;
+                    G.list[base + r] = (int16_t)q;
+                    G.lid[base + r] = id;
                 } else if (l == QHG_LIFE_FERTILE && g == 1) {
                     const uint32_t r = atomicAdd(&G.cnt[k][kCntM], 1u);
-                    G.mlist[base + r] = (int16_t)q;
-                    G.mid[base + r] = id;
+                    G.list[base + nk - 1 - r] = (int16_t)q;
+                    G.lid[base + nk - 1 - r] = id;
                 }
             }
         }
@@ -719,21 +745,22 @@ __device__ __forceinline__ void group_act(const Ctx &x, const Prog &p, const Ste
             const bool valid = q < T;
             const int k = (ch == 0) ? k0 : group_cell_of(q, eb, ng);
             const uint32_t base = (uint32_t)__shfl((int)eb, k, 64);
+            const uint32_t nk = (uint32_t)__shfl((int)e.n, k, 64);
             if (!valid) continue;
             const uint32_t Fk = G.cnt[k][kCntF], Mk = G.cnt[k][kCntM];
             const uint32_t t = q - base;
             if (t < Fk) {
-                const int64_t my = G.fid[q];
+                const int64_t my = G.lid[q];
                 uint32_t r = 0;
-                for (uint32_t u = 0; u < Fk; u++) r += G.fid[base + u] < my;
-                G.fs[base + r] = (int16_t)q;
+                for (uint32_t u = 0; u < Fk; u++) r += G.lid[base + u] < my;
+                G.srt[base + r] = (int16_t)q;
                 if (Fk <= Mk) G.ud[base + r] = draw(x.seed, sa.step, QHG_STREAM_PAIR, (uint64_t)my, 0);
-            }
-            if (t < Mk) {
-                const int64_t my = G.mid[q];
+            } else if (nk - 1 - t < Mk) {
+                const int64_t my = G.lid[q];
+                const uint32_t top = base + nk - 1;
                 uint32_t r = 0;
-                for (uint32_t u = 0; u < Mk; u++) r += G.mid[base + u] < my;
-                G.ms[base + r] = (int16_t)q;
+                for (uint32_t u = 0; u < Mk; u++) r += G.lid[top - u] < my;
+                G.srt[top - r] = (int16_t)q;
                 if (Mk < Fk) G.ud[base + r] = draw(x.seed, sa.step, QHG_STREAM_PAIR, (uint64_t)my, 0);
             }
         }
@@ -746,25 +773,26 @@ __device__ __forceinline__ void group_act(const Ctx &x, const Prog &p, const Ste
             if (Fk > 0 && Mk > 0) {
                 const bool fsel = Fk <= Mk;
                 const uint32_t nchs = fsel ? Fk : Mk, np = fsel ? Mk : Fk;
-                const int16_t *chs = fsel ? G.fs : G.ms;
-                const int16_t *pls = fsel ? G.ms : G.fs;
-                const int16_t *chl = fsel ? G.flist : G.mlist;
-                const int16_t *pll = fsel ? G.mlist : G.flist;
+                const uint32_t top = eb + e.n - 1;
                 uint64_t mask = (np >= 64) ? ~0ull : ((1ull << np) - 1ull);
                 uint32_t left = np;
                 for (uint32_t q = 0; q < nchs; q++) {
                     const int kp = (int)wrandr(0, (double)left, G.ud[eb + q]);
                     const int pos = select_bit(mask, kp);
                     mask &= ~(1ull << pos);
-                    const int16_t ja = chl[chs[eb + q]], jb = pll[pls[eb + pos]];
+                    const int16_t la = G.srt[fsel ? eb + q : top - q];
+                    const int16_t lb = G.srt[fsel ? top - pos : eb + pos];
+                    const int16_t ja = G.list[la], jb = G.list[lb];
                     G.mate[ja] = jb;
                     G.mate[jb] = ja;
+                    G.pid[ja] = G.lid[lb];
+                    G.pid[jb] = G.lid[la];
                     left--;
                 }
             }
         }
-        wave_sync();
     }
+    wave_sync();
 
     // ---- actions, one lane per agent
     for (int ch = 0; ch < nchunks; ch++) {
@@ -772,7 +800,6 @@ __device__ __forceinline__ void group_act(const Ctx &x, const Prog &p, const Ste
         const bool valid = q < T;
         const uint32_t slot = S0 + q;
         const int k = (ch == 0) ? k0 : group_cell_of(q, eb, ng);
-        const int32_t c = __shfl(e.c, k, 64);
         const uint32_t nk = (uint32_t)__shfl((int)e.n, k, 64);
         ActItem a;
         a.life = life0;
@@ -787,36 +814,35 @@ __device__ __forceinline__ void group_act(const Ctx &x, const Prog &p, const Ste
         a.npend = 0;
         a.pk0 = a.pk1 = 0;
         a.pu0 = a.pu1 = 0;
-        CellView cv;
-        cv.N = nk;
-        cv.K = K0;
-        cv.nn = nn0;
+        int32_t nbab = nbab0;
         if (ch > 0) {
             a.life = 0;
             if (valid) {
                 a.life = x.cur.life[slot];
                 a.id = x.cur.id[slot];
+                nbab = x.cur.nbabies[slot];
                 if (p.need_gender) a.g = x.cur.gender[slot];
                 if (p.need_birth) a.birth = x.cur.birth[slot];
                 if (p.need_lastb) a.lastb = x.cur.lastb[slot];
                 if (p.load_age) a.age = x.cur.age[slot];
-                cv.K = x.K[c];
-                cv.nn = x.nn[c];
             }
         }
+        CellView cv;
+        cv.N = nk;
+        cv.K = G.K[k];
+        cv.nn = G.nn[k];
         if (valid && p.has_pair) a.mate = G.mate[q];
         const uint8_t lifeS = a.life;
         int dir = -1;
         if (valid && a.life != 0) {
             act_program(x, p, sa, cv, a, my_deaths);
             if (a.npend > 0) {
-                // the cell's neighbour and weight rows, for movers only
                 int32_t nbr[6];
                 double w[7];
 #pragma unroll
-                for (int qq = 0; qq < 6; qq++) nbr[qq] = (qq < x.conn) ? x.nbr[(int64_t)c * x.conn + qq] : -1;
+                for (int qq = 0; qq < 6; qq++) nbr[qq] = (qq < x.conn) ? G.nbr[k][qq] : -1;
 #pragma unroll
-                for (int qq = 0; qq < 7; qq++) w[qq] = (qq < R) ? x.W[(int64_t)c * R + qq] : 0.0;
+                for (int qq = 0; qq < 7; qq++) w[qq] = (qq < R) ? G.W[k][qq] : 0.0;
                 // every registered move counts; the last one wins
                 int d = move_dir(x, a.pk0, a.pu0, cv.nn, w, nbr);
                 if (d >= 0) { dir = d; my_moves++; }
@@ -834,9 +860,9 @@ __device__ __forceinline__ void group_act(const Ctx &x, const Prog &p, const Ste
             x.dec[slot] = code;
             if (a.life != lifeS) x.cur.life[slot] = a.life;
             if (a.age_dirty) x.cur.age[slot] = a.age;
-            if (p.has_pair) x.cur.mate_id[slot] = (a.mate >= 0) ? x.cur.id[S0 + a.mate] : -3;
+            if (p.has_pair) x.cur.mate_id[slot] = (a.mate >= 0) ? G.pid[q] : -3;
             if (a.bflag) {
-                x.cur.nbabies[slot] += 1;
+                x.cur.nbabies[slot] = nbab + 1;
                 if (x.gwords) x.bfather[slot] = (int32_t)(S0 + a.mate);
                 atomicAdd(&G.cnt[k][kCntBirth], 1u);
             }
@@ -846,21 +872,32 @@ __device__ __forceinline__ void group_act(const Ctx &x, const Prog &p, const Ste
     wave_sync();
     if (lane < ng) {
         x.cstay[e.c] = G.cnt[lane][kCntStay];
-        x.cbirth[e.c] = G.cnt[lane][kCntBirth];
-        for (int d = 0; d < x.conn; d++) x.cmv[(int64_t)e.c * x.conn + d] = G.cnt[lane][kCntMv + d];
+        if (G.cnt[lane][kCntBirth]) x.cbirth[e.c] = G.cnt[lane][kCntBirth];
+        for (int d = 0; d < x.conn; d++) {
+            const uint32_t m = G.cnt[lane][kCntMv + d];
+            if (m) x.cin[G.outk[lane][d]] = m;
+        }
     }
     wave_sync();
 }
 
-// four waves per SIMD: at most 128 VGPRs
-__global__ void __launch_bounds__(kCellBlock) __attribute__((amdgpu_waves_per_eu(4)))
-k_cell_act(Ctx x, Prog p, StepArgs sa) {
+__global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs sa) {
     __shared__ ActLds lds;
     const int lane = threadIdx.x;
     uint32_t my_deaths = 0, my_moves = 0, my_fert = 0;
+    // one round trip: the status and this wave's first group of occupied
+    // cells (read ahead of the count check: every index below ncells exists)
+    const int32_t err = x.st->error;
+    const int64_t nocc = x.st->n_occ[sa.parity];
+    const int64_t gc = sa.gcells;
+    OccCell e_first{0, 0, 0, 0};
+    {
+        const int64_t idx = (int64_t)blockIdx.x * gc + lane;
+        if (lane < gc && idx < x.ncells) e_first = x.occ_cur[idx];
+    }
     // a failed step (capacity, look-back, count) freezes the population:
     // later steps already enqueued touch nothing
-    if (x.st->error) return;
+    if (err) return;
 
     // look-back words of this step's k_cell_scan start at zero, and so do
     // the step's death and move counts (k_cell_scan adds them up)
@@ -874,15 +911,13 @@ k_cell_act(Ctx x, Prog p, StepArgs sa) {
     }
 
     // groups of gcells consecutive occupied cells, round-robin over the
-    // waves; empty cells keep the zero counts the previous scatter left
-    const int64_t nocc = x.st->n_occ[sa.parity];
-    const int64_t gc = sa.gcells;
+    // waves; empty cells keep the zero counts the previous scan left
     const int64_t ngroups = (nocc + gc - 1) / gc;
     for (int64_t gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
         const int64_t i0 = gi * gc;
         const int ng = (int)((nocc - i0 < gc) ? (nocc - i0) : gc);
         OccCell e{0, 0, 0, 0};
-        if (lane < ng) e = x.occ_cur[i0 + lane];
+        if (lane < ng) e = (gi == blockIdx.x) ? e_first : x.occ_cur[i0 + lane];
         const uint32_t S0 = readlane_u32(e.s, 0);
         const uint32_t T = readlane_u32(e.s, ng - 1) + readlane_u32(e.n, ng - 1) - S0;
         const uint32_t nmax = wave_max_u32(lane < ng ? e.n : 0u);
@@ -930,33 +965,16 @@ __global__ void __launch_bounds__(kScanBlock) k_cell_scan(Ctx x, StepArgs sa) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t nc = x.ncells;
     if (x.st->error) return;
-    {
-        // this block's slice of the k_cell_act partials: deaths, moves and
-        // FERTILE counts, added to the status with one atomic each
-        uint64_t D = 0, Mv = 0, Fe = 0;
-        for (int32_t j = blockIdx.x * kScanBlock + threadIdx.x; j < x.nact_blocks; j += gridDim.x * kScanBlock) {
-            D += x.wave_part[kParts * j + 0];
-            Mv += x.wave_part[kParts * j + 1];
-            Fe += x.wave_part[kParts * j + 2];
-        }
-        D = wave_sum_u64(D);
-        Mv = wave_sum_u64(Mv);
-        Fe = wave_sum_u64(Fe);
-        if (lane == 0 && (D | Mv | Fe) != 0) {
-            atomicAdd((unsigned long long *)&x.st->deaths_last, (unsigned long long)D);
-            atomicAdd((unsigned long long *)&x.st->deaths_total, (unsigned long long)D);
-            atomicAdd((unsigned long long *)&x.st->moves_last, (unsigned long long)Mv);
-            atomicAdd((unsigned long long *)&x.st->moves_total, (unsigned long long)Mv);
-            atomicAdd((unsigned long long *)&x.st->fertile_total, (unsigned long long)Fe);
-        }
-    }
     // tiles in increasing order per block: a block only ever waits on tiles
     // that resident blocks hold, whatever the grid size
     for (uint32_t tile = blockIdx.x; tile < (uint32_t)x.ntiles; tile += gridDim.x) {
         const int64_t base = (int64_t)tile * kCellTile;
         uint32_t stay[kScanPer];
         int32_t e[kScanPer][kMaxIn];
+        uint32_t cm[kScanPer][kMaxIn];
         uint64_t v[kScanPer];
+        // one round trip: stayers, births, arrivals per incoming run and the
+        // runs' source slots; the counts go back to zero once read
         uint64_t sum = 0, osum = 0;
 #pragma unroll
         for (int q = 0; q < kScanPer; q++) {
@@ -965,10 +983,19 @@ __global__ void __launch_bounds__(kScanBlock) k_cell_scan(Ctx x, StepArgs sa) {
             stay[q] = ok ? x.cstay[c] : 0u;
             const uint32_t b = ok ? x.cbirth[c] : 0u;
 #pragma unroll
-            for (int k = 0; k < kMaxIn; k++) e[q][k] = (ok && k < x.in_w) ? x.in_e[c * x.in_w + k] : -1;
+            for (int k = 0; k < kMaxIn; k++) {
+                const bool in = ok && k < x.in_w;
+                e[q][k] = in ? x.in_e[c * x.in_w + k] : -1;
+                cm[q][k] = in ? x.cin[c * x.in_w + k] : 0u;
+            }
             uint32_t inc = 0;
 #pragma unroll
-            for (int k = 0; k < kMaxIn; k++) inc += (e[q][k] >= 0) ? x.cmv[e[q][k]] : 0u;
+            for (int k = 0; k < kMaxIn; k++) {
+                inc += cm[q][k];
+                if (cm[q][k]) x.cin[c * x.in_w + k] = 0;
+            }
+            if (stay[q]) x.cstay[c] = 0;
+            if (b) x.cbirth[c] = 0;
             v[q] = ok ? ((uint64_t)(stay[q] + inc + b) | ((uint64_t)b << 32)) : 0ull;
             sum += v[q];
             osum += ((uint32_t)v[q] != 0u) ? 1u : 0u;
@@ -1000,16 +1027,11 @@ __global__ void __launch_bounds__(kScanBlock) k_cell_scan(Ctx x, StepArgs sa) {
                 const uint32_t o = (uint32_t)run;
                 x.off_nxt[c] = o;
                 x.birth_off[c] = (uint32_t)(run >> 32);
-                // run positions: the mover counts are read again (cache-resident)
-                // rather than kept in registers through the look-back
-                uint32_t cm[kMaxIn];
-#pragma unroll
-                for (int k = 0; k < kMaxIn; k++) cm[k] = (e[q][k] >= 0) ? x.cmv[e[q][k]] : 0u;
                 uint32_t pos = o + stay[q];
 #pragma unroll
                 for (int k = 0; k < kMaxIn; k++) {
                     if (e[q][k] >= 0) x.rpos[e[q][k]] = pos;
-                    pos += cm[k];
+                    pos += cm[q][k];
                 }
                 x.bpos[c] = pos;
                 const uint32_t nn = (uint32_t)v[q];
@@ -1042,6 +1064,26 @@ __global__ void __launch_bounds__(kScanBlock) k_cell_scan(Ctx x, StepArgs sa) {
             st->t = sa.t + 1.0;
         }
         __syncthreads();
+    }
+    {
+        // off the critical path: this block's slice of the k_cell_act
+        // partials (deaths, moves, FERTILE counts), one atomic each
+        uint64_t D = 0, Mv = 0, Fe = 0;
+        for (int32_t j = blockIdx.x * kScanBlock + threadIdx.x; j < x.nact_blocks; j += gridDim.x * kScanBlock) {
+            D += x.wave_part[kParts * j + 0];
+            Mv += x.wave_part[kParts * j + 1];
+            Fe += x.wave_part[kParts * j + 2];
+        }
+        D = wave_sum_u64(D);
+        Mv = wave_sum_u64(Mv);
+        Fe = wave_sum_u64(Fe);
+        if (lane == 0 && (D | Mv | Fe) != 0) {
+            atomicAdd((unsigned long long *)&x.st->deaths_last, (unsigned long long)D);
+            atomicAdd((unsigned long long *)&x.st->deaths_total, (unsigned long long)D);
+            atomicAdd((unsigned long long *)&x.st->moves_last, (unsigned long long)Mv);
+            atomicAdd((unsigned long long *)&x.st->moves_total, (unsigned long long)Mv);
+            atomicAdd((unsigned long long *)&x.st->fertile_total, (unsigned long long)Fe);
+        }
     }
 }
 
@@ -1205,10 +1247,6 @@ __device__ __forceinline__ void cell_scatter(const Ctx &x, const StepArgs &sa, i
         }
         nmoth += __popcll(mb);
     }
-    // the counts of this cell were consumed by k_cell_scan: leave zeros for
-    // the next step (k_cell_act writes occupied cells only)
-    if (lane == 0) { x.cstay[c] = 0; x.cbirth[c] = 0; }
-    if (lane < x.conn) x.cmv[(int64_t)c * x.conn + lane] = 0;
     if (nmoth == 0) return;
     wave_sync();
     // newborns in mother-id order: createAgentAtIndex + makePopSpecificOffspring
@@ -1382,13 +1420,6 @@ __device__ __forceinline__ void group_scatter(const Ctx &x, const StepArgs &sa, 
             x.brec[2 * rg + 1] = x.bfather[mslot];
         }
     }
-    // the counts of these cells were consumed by k_cell_scan: leave zeros
-    // for the next step (k_cell_act writes occupied cells only)
-    if (lane < ng) {
-        x.cstay[e.c] = 0;
-        x.cbirth[e.c] = 0;
-        for (int d = 0; d < x.conn; d++) x.cmv[(int64_t)e.c * x.conn + d] = 0;
-    }
     wave_sync();
 }
 
@@ -1407,23 +1438,31 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_scatter(Ctx x, StepArgs sa)
     __shared__ ScatLds lds;
     const int lane = threadIdx.x;
     const int pr = (int)sa.parity;
-    if (x.st->error) return;
-    if (blockIdx.x == 0) {
-        status_snapshot(x);
-        return;
-    }
     const int64_t G = gridDim.x - 1;
-    const int64_t b0 = blockIdx.x - 1;
+    const int64_t b0 = (int64_t)blockIdx.x - 1;
+    // one round trip: the status and this wave's first group (read ahead of
+    // the count check: every index below ncells exists)
+    const int32_t err = x.st->error;
     const int64_t max_id = x.st->max_id[pr];
     const int64_t gbase = x.st->gpool[pr];
     const int64_t nocc = x.st->n_occ[pr];
     const int64_t gc = sa.gcells;
+    OccCell e_first{0, 0, 0, 0};
+    {
+        const int64_t idx = b0 * gc + lane;
+        if (b0 >= 0 && lane < gc && idx < x.ncells) e_first = x.occ_cur[idx];
+    }
+    if (err) return;
+    if (blockIdx.x == 0) {
+        status_snapshot(x);
+        return;
+    }
     const int64_t ngroups = (nocc + gc - 1) / gc;
     for (int64_t gi = b0; gi < ngroups; gi += G) {
         const int64_t i0 = gi * gc;
         const int ng = (int)((nocc - i0 < gc) ? (nocc - i0) : gc);
         OccCell e{0, 0, 0, 0};
-        if (lane < ng) e = x.occ_cur[i0 + lane];
+        if (lane < ng) e = (gi == b0) ? e_first : x.occ_cur[i0 + lane];
         const uint32_t S0 = readlane_u32(e.s, 0);
         const uint32_t T = readlane_u32(e.s, ng - 1) + readlane_u32(e.n, ng - 1) - S0;
         if (T <= (uint32_t)kGroupSlots) {

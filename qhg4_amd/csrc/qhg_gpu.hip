@@ -46,6 +46,9 @@ const char *kKernelNames[KB_COUNT] = {"k_cell_act", "k_cell_scan", "k_cell_scatt
                                       "k_capacity", "k_eval", "k_gpool_compact"};
 
 constexpr int kRing = kStatusRing;
+#ifndef QHG_GROUP_AGENTS
+#define QHG_GROUP_AGENTS 64.0     // target agents per wave group
+#endif
 #ifndef QHG_CELL_GRID_PER_CU
 #define QHG_CELL_GRID_PER_CU 16   // k_cell_act / k_cell_scatter blocks per compute unit
 #endif
@@ -105,7 +108,8 @@ struct qhg_pop {
     double *d_K = nullptr, *d_W = nullptr, *d_tmp = nullptr;
     uint8_t *d_ice = nullptr, *d_coastal = nullptr;
     double *d_env[7] = {};   // alt lon lat water npp temp rain
-    uint32_t *d_cstay = nullptr, *d_cbirth = nullptr, *d_cmv = nullptr, *d_birth_off = nullptr;
+    uint32_t *d_cstay = nullptr, *d_cbirth = nullptr, *d_cin = nullptr, *d_birth_off = nullptr;
+    int32_t *d_outk = nullptr;
     uint32_t *d_bpos = nullptr, *d_rpos = nullptr;
     uint64_t *d_tiles = nullptr, *d_tiles2 = nullptr;
     uint32_t *d_wave_part = nullptr;
@@ -170,7 +174,8 @@ struct qhg_pop {
         x.gflag = d_gflag;
         x.cstay = d_cstay;
         x.cbirth = d_cbirth;
-        x.cmv = d_cmv;
+        x.cin = d_cin;
+        x.outk = d_outk;
         x.birth_off = d_birth_off;
         x.bpos = d_bpos;
         x.rpos = d_rpos;
@@ -198,7 +203,7 @@ struct qhg_pop {
         const int kb = (int)(known_step & 1);
         const double occ = (double)std::max<int64_t>(1, known.n_occ[kb]);
         const double mean = std::max(1.0, (double)known.n_alive[kb] / occ);
-        a.gcells = (int32_t)std::max(1.0, std::min((double)kGroupCells, std::floor(64.0 / mean)));
+        a.gcells = (int32_t)std::max(1.0, std::min((double)kGroupCells, std::floor(QHG_GROUP_AGENTS / mean)));
         return a;
     }
 };
@@ -675,13 +680,17 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
     for (int32_t c = 0; c < nc32; c++) in_w = std::max(in_w, indeg[c]);
     if (in_w > kMaxIn) return fail(QHG_ERR_ARG, "a cell has more than 8 incoming neighbour slots");
     std::vector<int32_t> in_e((size_t)nc32 * in_w, -1);
+    // and the reverse map: the incoming-run index of every (cell, slot)
+    std::vector<int32_t> outk((size_t)nc32 * conn, -1);
     {
         std::vector<int32_t> fill((size_t)nc32, 0);
         for (int32_t s = 0; s < nc32; s++)
             for (int d = 0; d < grid->num_neighbors[s]; d++) {
                 const int32_t c = grid->neighbors[(size_t)s * conn + d];
                 if (c < 0) continue;
-                in_e[(size_t)c * in_w + fill[c]++] = s * conn + d;
+                const int32_t k = c * in_w + fill[c]++;
+                in_e[(size_t)k] = s * conn + d;
+                outk[(size_t)s * conn + d] = k;
             }
     }
     int ndev = 0;
@@ -730,10 +739,11 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
     CK(upload_array(&p->d_nbr, grid->neighbors, nc * p->conn, p->stream));
     CK(upload_array(&p->d_nn, grid->num_neighbors, nc, p->stream));
     CK(upload_array(&p->d_in_e, in_e.data(), in_e.size(), p->stream));
+    CK(upload_array(&p->d_outk, outk.data(), outk.size(), p->stream));
     if (hipMalloc(&p->d_K, nc * 8) != hipSuccess || hipMalloc(&p->d_W, nc * (p->conn + 1) * 8) != hipSuccess ||
         hipMalloc(&p->d_tmp, nc * 8) != hipSuccess || hipMalloc(&p->d_off[0], (nc + 1) * 4) != hipSuccess ||
         hipMalloc(&p->d_off[1], (nc + 1) * 4) != hipSuccess || hipMalloc(&p->d_cstay, nc * 4) != hipSuccess ||
-        hipMalloc(&p->d_cbirth, nc * 4) != hipSuccess || hipMalloc(&p->d_cmv, nc * p->conn * 4) != hipSuccess ||
+        hipMalloc(&p->d_cbirth, nc * 4) != hipSuccess || hipMalloc(&p->d_cin, nc * p->in_w * 4) != hipSuccess ||
         hipMalloc(&p->d_birth_off, (nc + 1) * 4) != hipSuccess || hipMalloc(&p->d_bpos, nc * 4) != hipSuccess ||
         hipMalloc(&p->d_rpos, nc * p->conn * 4) != hipSuccess ||
         hipMalloc(&p->d_tiles, (size_t)p->ntiles * 8) != hipSuccess ||
@@ -749,10 +759,10 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
     (void)hipMemsetAsync(p->d_off[1], 0, (nc + 1) * 4, p->stream);
     (void)hipMemsetAsync(p->d_tiles, 0, (size_t)p->ntiles * 8, p->stream);
     (void)hipMemsetAsync(p->d_tiles2, 0, (size_t)p->ntiles * 8, p->stream);
-    // per-cell counts start at zero; k_cell_scatter restores zeros after use
+    // per-cell counts start at zero; k_cell_scan restores zeros after use
     (void)hipMemsetAsync(p->d_cstay, 0, nc * 4, p->stream);
     (void)hipMemsetAsync(p->d_cbirth, 0, nc * 4, p->stream);
-    (void)hipMemsetAsync(p->d_cmv, 0, nc * p->conn * 4, p->stream);
+    (void)hipMemsetAsync(p->d_cin, 0, nc * p->in_w * 4, p->stream);
     (void)hipMemsetAsync(p->d_st, 0, sizeof(DevStatus), p->stream);
     if (env) {
         const double *src[7] = {env->altitude, env->longitude, env->latitude, env->water,
@@ -786,7 +796,7 @@ int qhg_gpu_destroy(qhg_pop *p) {
     free_agentbuf(p->buf[1]);
     free_slot_scratch(p);
     void *ptrs[] = {p->d_off[0], p->d_off[1], p->d_gpool[0], p->d_gpool[1], p->d_nbr, p->d_nn, p->d_in_e,
-                    p->d_K, p->d_W, p->d_tmp, p->d_ice, p->d_coastal, p->d_cstay, p->d_cbirth, p->d_cmv,
+                    p->d_K, p->d_W, p->d_tmp, p->d_ice, p->d_coastal, p->d_cstay, p->d_cbirth, p->d_cin, p->d_outk,
                     p->d_birth_off, p->d_bpos, p->d_rpos, p->d_tiles, p->d_tiles2, p->d_occ[0], p->d_occ[1],
                     p->d_wave_part, p->d_st};
     for (void *q : ptrs)
@@ -885,7 +895,7 @@ int qhg_gpu_upload_agents(qhg_pop *p, const qhg_agents *a) {
     const size_t ncz = (size_t)p->ncells;
     HIPCHK(hipMemsetAsync(p->d_cstay, 0, ncz * 4, p->stream));
     HIPCHK(hipMemsetAsync(p->d_cbirth, 0, ncz * 4, p->stream));
-    HIPCHK(hipMemsetAsync(p->d_cmv, 0, ncz * p->conn * 4, p->stream));
+    HIPCHK(hipMemsetAsync(p->d_cin, 0, ncz * p->in_w * 4, p->stream));
     if (!occ.empty())
         HIPCHK(hipMemcpyAsync(p->d_occ[b], occ.data(), occ.size() * sizeof(OccCell), hipMemcpyHostToDevice,
                               p->stream));

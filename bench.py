@@ -11,10 +11,11 @@ step (initialize, fused actions, births, per-cell recount) on the device.
 value = sum over the timed steps of the live count at step start / wall time
 of the timed steps (inputs resident in HBM, no host sync inside the region).
 
-Per-kernel device time comes from HIP events on the handle's stream in a
-second, bit-identical run of the same steps (the simulation is deterministic);
-roofline.achieved = algorithmic bytes per launch (qhg4_amd/csrc/bytes_model.h)
-/ average launch duration of the dominant kernel.
+Per-kernel device time comes from HIP events attached to each dispatch on the
+handle's stream in a second, bit-identical run of the same steps (the
+simulation is deterministic); roofline.achieved = algorithmic bytes per launch
+(qhg4_amd/csrc/bytes_model.h) / average launch duration of the dominant kernel;
+roofline.traffic = its measured HBM bytes per launch (rocprofv3 counters).
 
 cpu_baseline: the CPU oracle (the reference algorithm restated in C with the
 same Philox draws, OpenMP) on the box's host cores, timed on the same steps,
@@ -31,6 +32,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on the
+# default workload and steps (100 warm-up + 1,900 timed), each counter scaled by
+# what it reads for a 256 MiB device copy (FETCH x2, WRITE x1); see
+# profiles/step_kernels_c2_counters.md.  Reported as roofline.traffic only for
+# that exact workload.
+MEASURED_TRAFFIC = {("c2", 100, 1900): {"k_cell_act": 22.28e6, "k_cell_scatter": 41.93e6, "k_cell_scan": 7.14e6}}
 METRIC = "agent-steps/sec at 1 MI355X + %HBM peak; ensemble vs plain speedup, p50/p99"
 
 
@@ -192,8 +200,9 @@ def main():
                              "bytes_per_launch": int(per_launch), "GBps": round(gbps, 1)}
         dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
         kd = kernels[dom]
+        traffic = MEASURED_TRAFFIC.get((args.workload, args.warmup, args.steps), {}).get(dom)
         roof = {"bound": "hbm", "achieved": kd["GBps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": round(kd["GBps"] / HBM_PEAK_GBPS, 5), "traffic": None, "kernel": dom,
+                "frac": round(kd["GBps"] / HBM_PEAK_GBPS, 5), "traffic": traffic, "kernel": dom,
                 "bytes_per_launch": kd["bytes_per_launch"], "avg_launch_us": kd["avg_us"]}
 
     cpu = None if args.no_cpu else cpu_baseline(sc, args.warmup, args.steps, args.cpu_seconds)

@@ -7,6 +7,7 @@
 // from status snapshots written into pinned memory by the device, and keeps
 // the capacity ahead of the worst-case growth (births <= live/2 per step)
 // without a per-step synchronisation.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -231,11 +232,14 @@ int launch_bs(qhg_pop *p, int kid, void (*kern)(KArgs...), int grid, int block, 
         e0 = p->ev_pool[slot].first;
         e1 = p->ev_pool[slot].second;
         p->ev_used.push_back({kid, slot});
-        HIPCHK(hipEventRecord(e0, p->stream));
+        // events attached to the dispatch itself: they time the kernel, not
+        // the gap before it
+        hipExtLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, p->stream, e0, e1, 0, args...);
+        HIPCHK(hipGetLastError());
+        return 0;
     }
     hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, p->stream, args...);
     HIPCHK(hipGetLastError());
-    if (p->timing) HIPCHK(hipEventRecord(e1, p->stream));
     return 0;
 }
 

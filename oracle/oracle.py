@@ -17,8 +17,15 @@ LIB_PATH = os.path.join(HERE, "_build", "libqhg_oracle.so")
 _lib = None
 
 
+REFERENCE_TREE = "/root/reference/QHG4"
+
+
 def build():
+    """Compile the oracle; where the reference sources are present, also the
+    reference utility library the pin tests compare against (oracle/_ref)."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
+    if os.path.isdir(REFERENCE_TREE):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
 
 
 def lib():

@@ -226,8 +226,8 @@ void orc_crossover(uint64_t *out, const uint64_t *parent, int32_t genome_size,
 }
 
 /* freeReco + makeFreeMask, GeneUtils.cpp:322-360 (mask = hi:lo draws) */
-static void orc_free_reco(uint64_t *out, const uint64_t *parent, int32_t nb,
-                          const uint32_t *draws) {
+void orc_free_reco(uint64_t *out, const uint64_t *parent, int32_t nb,
+                   const uint32_t *draws) {
     for (int32_t b = 0; b < nb; b++) {
         uint64_t v = ((uint64_t)draws[2 * b] << 32) + draws[2 * b + 1];
         v &= 0x5555555555555555ull;

@@ -47,6 +47,8 @@ uint32_t orc_wrandi(uint32_t a, uint32_t b, uint32_t s, uint32_t u);
 uint64_t orc_multimask(const uint32_t *breaks, int32_t n);
 void     orc_crossover(uint64_t *out, const uint64_t *parent, int32_t genome_size,
                        const uint32_t *break_pos, int32_t nbreaks);
+/* GeneUtils::freeReco (GeneUtils.cpp:322-360): 2 draws per block, hi then lo */
+void     orc_free_reco(uint64_t *out, const uint64_t *parent, int32_t nb, const uint32_t *draws);
 void     orc_mutate_nucs(uint64_t *genome, int32_t genome_size, int32_t nmut,
                          const uint32_t *pos_draws, const uint32_t *mask_draws);
 int32_t  orc_bino_table(double p, int32_t n, double eps, double *cdf, int32_t max);

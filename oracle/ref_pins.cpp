@@ -1,0 +1,107 @@
+// Test infrastructure: C entry points over the reference's own utility code
+// (WELL512 conversions, GeneUtils mutation/recombination, BinomialDist,
+// bino_tools), compiled from the reference sources by `make -C oracle ref`
+// into oracle/_ref/libqhg_ref.so.  The tests feed these and the oracle the
+// same raw 32-bit draws and compare the results bit for bit; nothing outside
+// tests/ loads this library.
+//
+// The reference functions draw from a WELL512 object themselves; every entry
+// point here starts two generators from the same state: one is handed to the
+// reference function, the other reports the raw wrand() words that the first
+// consumed (each conversion takes exactly one word, QHG4/utils/WELL512.h:33-39).
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "BinomialDist.h"
+#include "GeneUtils.h"
+#include "WELL512.h"
+#include "bino_tools.h"
+
+double gammaln(double xx);   // QHG4/utils/bino_tools.cpp:13
+
+namespace {
+
+// makeMultiMask is protected (QHG4/genes/GeneUtils.h:75)
+struct GeneProbe : public GeneUtils {
+    static ulong multimask(const std::vector<uint> &v) { return makeMultiMask(v); }
+};
+
+void raw_words(const uint32_t *state, uint32_t index, int n, uint32_t *out) {
+    uint32_t st[16];
+    std::memcpy(st, state, sizeof(st));
+    WELL512 w(st, index);
+    for (int i = 0; i < n; i++) out[i] = w.wrand();
+}
+
+}  // namespace
+
+extern "C" {
+
+// n draws of each conversion from the same state: raw, wrandd, wrandr(a,b),
+// wrandi(ia, ib, is)   (QHG4/utils/WELL512.h:30-39)
+void ref_well_draws(const uint32_t *state, uint32_t index, int n, uint32_t *raw, double *d, double *r,
+                    uint32_t *iv, double a, double b, uint32_t ia, uint32_t ib, uint32_t is) {
+    raw_words(state, index, n, raw);
+    uint32_t s1[16], s2[16], s3[16];
+    std::memcpy(s1, state, sizeof(s1));
+    std::memcpy(s2, state, sizeof(s2));
+    std::memcpy(s3, state, sizeof(s3));
+    WELL512 w1(s1, index), w2(s2, index), w3(s3, index);
+    for (int i = 0; i < n; i++) {
+        d[i] = w1.wrandd();
+        r[i] = w2.wrandr(a, b);
+        iv[i] = w3.wrandi(ia, ib, is);
+    }
+}
+
+// GeneUtils::makeMultiMask (QHG4/genes/GeneUtils.cpp:156-175)
+uint64_t ref_multimask(const uint32_t *breaks, int n) {
+    std::vector<uint> v(breaks, breaks + n);
+    return GeneProbe::multimask(v);
+}
+
+// GeneUtils::crossOver (QHG4/genes/GeneUtils.cpp:184-246): out gets both
+// strands; raw gets the nbreaks words it drew
+void ref_crossover(uint64_t *out, const uint64_t *parent, int genome_size, int nbreaks, const uint32_t *state,
+                   uint32_t *raw) {
+    raw_words(state, 0, nbreaks, raw);
+    uint32_t st[16];
+    std::memcpy(st, state, sizeof(st));
+    WELL512 w(st, 0);
+    GeneUtils::crossOver((ulong *)out, (const ulong *)parent, genome_size, nbreaks, &w);
+}
+
+// GeneUtils::freeReco (QHG4/genes/GeneUtils.cpp:322-360): 2 words per block
+void ref_free_reco(uint64_t *out, uint64_t *parent, int nblocks, const uint32_t *state, uint32_t *raw) {
+    raw_words(state, 0, 2 * nblocks, raw);
+    uint32_t st[16];
+    std::memcpy(st, state, sizeof(st));
+    WELL512 w(st, 0);
+    GeneUtils::freeReco((ulong *)out, (ulong *)parent, nblocks, &w);
+}
+
+// GeneUtils::mutateNucs (QHG4/genes/GeneUtils.cpp:112-141): 2 words per
+// mutation (position, mask)
+void ref_mutate_nucs(uint64_t *genome, int genome_size, int nmut, const uint32_t *state, uint32_t *raw) {
+    raw_words(state, 0, 2 * nmut, raw);
+    uint32_t st[16];
+    std::memcpy(st, state, sizeof(st));
+    WELL512 w(st, 0);
+    GeneUtils::mutateNucs((ulong *)genome, genome_size, nmut, &w);
+}
+
+// BinomialDist::create + getN (QHG4/utils/BinomialDist.cpp:10-104)
+int ref_bino_getn(double p, int n, double eps, const double *u, int m, int32_t *out) {
+    BinomialDist *bd = BinomialDist::create(p, n, eps);
+    if (!bd) return -1;
+    for (int i = 0; i < m; i++) out[i] = bd->getN(u[i]);
+    delete bd;
+    return 0;
+}
+
+// bino_tools.cpp:13 and :94
+double ref_gammaln(double x) { return gammaln(x); }
+double ref_beta(double a, double b, double x) { return beta(a, b, x); }
+
+}  // extern "C"

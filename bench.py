@@ -80,9 +80,12 @@ def kernel_bytes(bm, d, grid, ops, gwords):
     scan = steps * (ncells * bm["SCAN_CELL"] + grid_edges(grid) * bm["SCAN_EDGE"])
     survivors = live - deaths
     per_surv = bm["SCAT_SURVIVOR"] + (bm["SCAT_SURVIVOR_GEN"] if gwords else 0)
-    per_birth = bm["SCAT_BIRTH"] + ((bm["SCAT_BIRTH_GEN"] + bm["SCAT_GENOME_WORDS"] * gwords * 8) if gwords else 0)
+    per_birth = bm["SCAT_BIRTH"] + (bm["SCAT_BIRTH_GEN"] if gwords else 0)
     scat = steps * ncells * bm["SCAT_CELL"] + live * bm["SCAT_LIVE"] + survivors * per_surv + births * per_birth
-    return {"k_cell_act": act, "k_cell_scan": scan, "k_cell_scatter": scat}
+    out = {"k_cell_act": act, "k_cell_scan": scan, "k_cell_scatter": scat}
+    if gwords:
+        out["k_genome"] = births * bm["SCAT_GENOME_WORDS"] * gwords * 8
+    return out
 
 
 def make_workload(name, seed):

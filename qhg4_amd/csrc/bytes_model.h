@@ -40,7 +40,9 @@
                                        age, last birth, NumBabies) + outcome r 1             */
 #define QHG_B_SCAT_SURVIVOR_GEN  8  /* genome handle r 4 + w 4                               */
 #define QHG_B_SCAT_BIRTH        54  /* newborn record w 46 + birth offset r 4 + father r 4   */
-#define QHG_B_SCAT_BIRTH_GEN    12  /* handle w 4 + mother/father handles r 8                */
+#define QHG_B_SCAT_BIRTH_GEN    12  /* handle w 4 + birth record w 8                         */
+
+/* k_genome (Genetics::makeOffspring), per birth */
 #define QHG_B_SCAT_GENOME_WORDS  3  /* x gwords x 8: two parent genomes r + child w          */
 
 #endif

@@ -1088,95 +1088,11 @@ __global__ void __launch_bounds__(kScanBlock) k_cell_scan(Ctx x, StepArgs sa) {
 }
 
 // ---------------------------------------------------------------------------
-// genetics helpers: GeneUtils::makeMultiMask / crossOver (GeneUtils.cpp:156-246)
+// genetics: GeneUtils::makeMultiMask / crossOver / freeReco / mutateNucs
+// (GeneUtils.cpp:112-246, :322-360), one genome word per lane
 // ---------------------------------------------------------------------------
-__device__ uint64_t multimask_block(const uint32_t *pos, int n, uint32_t blk) {
-    // breaks of this block, in draw order, as makeMultiMask sees them
-    int cnt = 0;
-    for (int q = 0; q < n; q++) cnt += (pos[q] >> 6) == blk;
-    uint32_t k = 1 - (uint32_t)(cnt % 2);
-    uint64_t out = 0;
-    int i = n - 1;
-    while (i >= 0 && (pos[i] >> 6) != blk) i--;
-    for (uint32_t j = 0; j < 64; j++) {
-        if (i >= 0 && j == 64 - (pos[i] & 63u)) {
-            i--;
-            while (i >= 0 && (pos[i] >> 6) != blk) i--;
-            k = 1 - k;
-        }
-        out = (out << 1) + k;
-    }
-    return out;
-}
 
-// strand `sel` of crossOver(parent) -> out[0..nb)
-__device__ void crossover_strand(uint64_t *out, const uint64_t *par, int nb, const uint32_t *pos, int n, int sel) {
-    for (int b = 0; b < nb; b++) {
-        int before = 0, in = 0;
-        for (int q = 0; q < n; q++) {
-            const uint32_t blk = pos[q] >> 6;
-            before += (int)blk < b;
-            in += (int)blk == b;
-        }
-        const int cur = before & 1;
-        const uint64_t gc = par[cur * nb + b], go = par[(1 - cur) * nb + b];
-        const uint64_t L = in ? multimask_block(pos, n, (uint32_t)b) : ~0ull;
-        out[b] = (sel == 0) ? ((L & gc) | (~L & go)) : ((L & go) | (~L & gc));
-    }
-}
 
-// Genetics::makeOffspring (Genetics.cpp:285-337) into pool handle gh
-__device__ void make_genome(const Ctx &x, const Prog &p, uint32_t step, uint64_t nid, int64_t gh, int64_t gm,
-                            int64_t gf) {
-    const int nbk = x.gwords / 2;
-    const uint64_t *g1 = x.gpool + gm * x.gwords;
-    const uint64_t *g2 = x.gpool + gf * x.gwords;
-    uint64_t *out = x.gpool + gh * x.gwords;
-    uint32_t n = 0;
-#define GD() draw(x.seed, step, QHG_STREAM_GENETICS, nid, n++)
-    const int i1 = (int)(2 * 1.0 * wrandd(GD()));
-    const int i2 = (int)(2 * 1.0 * wrandd(GD()));
-    const int C = p.gen_cross;
-    if (C > 0) {
-        uint32_t pos[kMaxCross];
-        const uint32_t nbits = (uint32_t)nbk * 64u;
-        for (int q = 0; q < C; q++) pos[q] = wrandi(0, nbits, 2, GD());
-        crossover_strand(out, g1, nbk, pos, C, i1);
-        for (int q = 0; q < C; q++) pos[q] = wrandi(0, nbits, 2, GD());
-        crossover_strand(out + nbk, g2, nbk, pos, C, i2);
-    } else if (C == -1) {
-        // freeReco + makeFreeMask, GeneUtils.cpp:322-360
-        for (int par = 0; par < 2; par++) {
-            const uint64_t *gp = par ? g2 : g1;
-            const int sel = par ? i2 : i1;
-            for (int b = 0; b < nbk; b++) {
-                const uint32_t hi = GD();
-                const uint32_t lo = GD();
-                uint64_t v = ((uint64_t)hi << 32) + lo;
-                v &= 0x5555555555555555ull;
-                v += v << 1;
-                const uint64_t a0 = gp[b], a1 = gp[nbk + b];
-                out[par * nbk + b] = (sel == 0) ? ((v & a0) | (~v & a1)) : ((v & a1) | (~v & a0));
-            }
-        }
-    } else {
-        for (int b = 0; b < nbk; b++) out[b] = g1[i1 * nbk + b];
-        for (int b = 0; b < nbk; b++) out[nbk + b] = g2[i2 * nbk + b];
-    }
-    if (p.nbino > 0) {
-        // BinomialDist::getN then GeneUtils::mutateNucs (GeneUtils.cpp:112-141)
-        const double um = wrandd(GD());
-        int k = 0;
-        while (k < p.nbino && um > p.bino[k]) k++;
-        const uint32_t nbits_all = 2u * 2u * (uint32_t)p.gen_size;
-        for (int j = 0; j < k; j++) {
-            const uint32_t bp = wrandi(0, nbits_all, 2, GD());
-            const uint64_t mk = (uint64_t)wrandi(1, 4, 1, GD());
-            out[bp >> 6] ^= mk << (bp & 63u);
-        }
-    }
-#undef GD
-}
 
 // ---------------------------------------------------------------------------
 // k_cell_scatter: agents of one occupied source cell to the next buffer
@@ -1481,16 +1397,137 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_scatter(Ctx x, StepArgs sa)
     }
 }
 
-// k_genome: one thread per birth of the step (Genetics::makeOffspring)
-__global__ void __launch_bounds__(256) k_genome(Ctx x, Prog p, StepArgs sa) {
+// bits lo..hi (inclusive, lo <= hi <= 63)
+__device__ __forceinline__ uint64_t bit_range(int lo, int hi) {
+    const uint64_t up = (hi >= 63) ? ~0ull : ((1ull << (hi + 1)) - 1ull);
+    return up & ~((1ull << lo) - 1ull);
+}
+
+// Word w of a child genome (Genetics::makeOffspring, Genetics.cpp:285-337):
+// strand s = w / nbk comes from parent s (0 mother, 1 father), block
+// b = w % nbk.  The draws follow makeOffspring's order: i1, i2, the mother's
+// crossover draws, the father's, the mutation count, then (position, mask)
+// pairs; mutations that hit this word are applied here.  Draws 0-3 and the
+// block holding the mutation count are computed once.
+__device__ uint64_t child_word(const Ctx &x, const Prog &p, uint32_t step, uint64_t nid, const uint64_t *par,
+                               int s, int b, int nbk, int w) {
+    uint32_t d0[4];
+    draw4(x.seed, step, QHG_STREAM_GENETICS, nid, 0, d0);
+    auto D = [&](uint32_t n) -> uint32_t {
+        if (n < 4) {
+            uint32_t v = d0[0];
+#pragma unroll
+            for (int q = 1; q < 4; q++) if ((uint32_t)q == n) v = d0[q];
+            return v;
+        }
+        return draw(x.seed, step, QHG_STREAM_GENETICS, nid, n);
+    };
+    const int sel = (int)(2 * 1.0 * wrandd(s ? d0[1] : d0[0]));   // i1 for the mother, i2 for the father
+    const int C = p.gen_cross;
+    uint64_t word;
+    uint32_t nmut;
+    if (C > 0) {
+        // GeneUtils::crossOver (GeneUtils.cpp:184-246) restricted to block b:
+        // this parent's breaks are draws 2 + s*C + q, q < C
+        const uint32_t nbits = (uint32_t)nbk * 64u;
+        int before = 0, in = 0;
+        for (int q = 0; q < C; q++) {
+            const uint32_t bp = wrandi(0, nbits, 2, D(2 + (uint32_t)(s * C + q)));
+            before += (int)(bp >> 6) < b;
+            in += (int)(bp >> 6) == b;
+        }
+        const int cur = before & 1;
+        const uint64_t gc = par[cur * nbk + b], go = par[(1 - cur) * nbk + b];
+        uint64_t L = ~0ull;
+        if (in) {
+            // makeMultiMask (GeneUtils.cpp:156-175) over the block's breaks in
+            // draw order, walked from the last: bit 63-j takes the current
+            // value; a break at bit v flips it at j = 64-v, but only if that j
+            // is still ahead, otherwise the walk stops there (unsorted breaks,
+            // repeats and breaks at bit 0 included)
+            uint32_t k = 1u - (uint32_t)(in % 2);
+            int jc = 0;
+            uint64_t m = 0;
+            for (int q = C - 1; q >= 0; q--) {
+                const uint32_t bp = wrandi(0, nbits, 2, D(2 + (uint32_t)(s * C + q)));
+                if ((int)(bp >> 6) != b) continue;
+                const int t = 64 - (int)(bp & 63u);
+                if (t > 63 || t <= jc) break;
+                if (k) m |= bit_range(64 - t, 63 - jc);
+                k = 1u - k;
+                jc = t;
+            }
+            if (k) m |= bit_range(0, 63 - jc);
+            L = m;
+        }
+        word = (sel == 0) ? ((L & gc) | (~L & go)) : ((L & go) | (~L & gc));
+        nmut = 2 + 2 * (uint32_t)C;
+    } else if (C == -1) {
+        // freeReco + makeFreeMask (GeneUtils.cpp:322-360): two draws per block
+        const uint32_t n0 = 2 + (uint32_t)(s * 2 * nbk + 2 * b);
+        uint64_t v = ((uint64_t)D(n0) << 32) + D(n0 + 1);
+        v &= 0x5555555555555555ull;
+        v += v << 1;
+        const uint64_t a0 = par[b], a1 = par[nbk + b];
+        word = (sel == 0) ? ((v & a0) | (~v & a1)) : ((v & a1) | (~v & a0));
+        nmut = 2 + 4 * (uint32_t)nbk;
+    } else {
+        word = par[sel * nbk + b];
+        nmut = 2;
+    }
+    if (p.nbino > 0) {
+        // BinomialDist::getN then GeneUtils::mutateNucs (GeneUtils.cpp:112-141)
+        uint32_t dm[4];
+        const uint32_t mb = nmut >> 2;
+        draw4(x.seed, step, QHG_STREAM_GENETICS, nid, mb, dm);
+        auto M = [&](uint32_t n) -> uint32_t {
+            if ((n >> 2) == mb) {
+                uint32_t v = dm[0];
+#pragma unroll
+                for (int q = 1; q < 4; q++) if ((uint32_t)q == (n & 3u)) v = dm[q];
+                return v;
+            }
+            return draw(x.seed, step, QHG_STREAM_GENETICS, nid, n);
+        };
+        const double um = wrandd(M(nmut));
+        int k = 0;
+        while (k < p.nbino && um > p.bino[k]) k++;
+        const uint32_t nbits_all = 2u * 2u * (uint32_t)p.gen_size;
+        for (int j = 0; j < k; j++) {
+            const uint32_t bp = wrandi(0, nbits_all, 2, M(nmut + 1 + 2 * (uint32_t)j));
+            if ((int)(bp >> 6) == w) {
+                const uint64_t mk = (uint64_t)wrandi(1, 4, 1, M(nmut + 2 + 2 * (uint32_t)j));
+                word ^= mk << (bp & 63u);
+            }
+        }
+    }
+    return word;
+}
+
+// k_genome: the step's births, one lane per genome word (64 / gwords births
+// per wave when a genome has at most 64 words)
+__global__ void __launch_bounds__(kWave) k_genome(Ctx x, Prog p, StepArgs sa) {
     if (x.st->error) return;
+    const int lane = threadIdx.x;
     const int pr = (int)sa.parity;
     const int64_t B = x.st->births_last;
     const int64_t max_id = x.st->max_id[pr];
     const int64_t gbase = x.st->gpool[pr];
-    for (int64_t rg = (int64_t)blockIdx.x * 256 + threadIdx.x; rg < B; rg += (int64_t)gridDim.x * 256) {
+    const int gw = x.gwords, nbk = gw / 2;
+    const int per = (gw <= kWave) ? kWave / gw : 1;
+    const int pb = (gw <= kWave) ? lane / gw : 0;
+    for (int64_t base = (int64_t)blockIdx.x * per; base < B; base += (int64_t)gridDim.x * per) {
+        const int64_t rg = base + pb;
+        if (pb >= per || rg >= B) continue;
         const int32_t ms = x.brec[2 * rg], fs = x.brec[2 * rg + 1];
-        make_genome(x, p, sa.step, (uint64_t)(max_id + 1 + rg), gbase + rg, x.cur.ghandle[ms], x.cur.ghandle[fs]);
+        const int64_t hm = x.cur.ghandle[ms], hf = x.cur.ghandle[fs];
+        const uint64_t nid = (uint64_t)(max_id + 1 + rg);
+        uint64_t *out = x.gpool + (gbase + rg) * gw;
+        for (int w = (gw <= kWave) ? lane % gw : lane; w < gw; w += (gw <= kWave) ? gw : kWave) {
+            const int s = w / nbk, b = w % nbk;
+            const uint64_t *par = x.gpool + (s ? hf : hm) * gw;
+            out[w] = child_word(x, p, sa.step, nid, par, s, b, nbk, w);
+        }
     }
 }
 

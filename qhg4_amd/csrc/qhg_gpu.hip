@@ -632,7 +632,7 @@ int phase_finalize(qhg_pop *p) {
     rc = launch_bs(p, KB_SCATTER, k_cell_scatter, p->scatter_grid + 1, kCellBlock, x, sa);
     if (rc) return rc;
     if (p->gwords) {
-        rc = launch(p, KB_GENOME, k_genome, p->num_cu * 4, x, p->prog, sa);
+        rc = launch_bs(p, KB_GENOME, k_genome, p->num_cu * 16, kWave, x, p->prog, sa);
         if (rc) return rc;
     }
     p->acted = false;

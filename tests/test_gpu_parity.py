@@ -79,6 +79,26 @@ def test_c3_small_genetics():
     _lockstep(sc, 80, 20, use_run=True)
 
 
+@pytest.mark.parametrize("crossovers,genome_size,rate", [(7, 256, 1e-2), (-1, 100, 5e-3), (0, 256, 2e-2),
+                                                       (3, 1000, 1e-3)])
+def test_genetics_variants(crossovers, genome_size, rate):
+    """Several breaks per chromosome (unsorted and repeated within a block),
+    free recombination, plain strand copies, genomes that do not fill their
+    last word, and mutation counts above one."""
+    print("case: genetics crossovers=%d genome=%d rate=%g, ico level 4, 3k agents, 40 steps"
+          % (crossovers, genome_size, rate), flush=True)
+    sc = scenarios.c2(seed=21 + genome_size, level=4, n_agents=3000, patch_cells=150, genetics=True)
+    p = sc["params"]
+    p.gen_num_crossover = crossovers
+    p.gen_genome_size = genome_size
+    p.gen_mutation_rate = rate
+    gw = 2 * abi.genome_words(genome_size)
+    rng = np.random.default_rng(genome_size)
+    n = len(sc["agents"]["id"])
+    sc["agents"]["genome"] = rng.integers(0, 2**63, n * gw, dtype=np.uint64)
+    _lockstep(sc, 40, 10, use_run=True)
+
+
 def test_compaction_keeps_state():
     print("case: forced compaction, C1 tutorial rates", flush=True)
     sc = tut_verhulst(scenarios.c1(seed=11, width=64, height=64, n_agents=3000))

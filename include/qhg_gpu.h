@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define QHG_ABI_VERSION 2
+#define QHG_ABI_VERSION 3
 
 /* error codes */
 #define QHG_OK              0
@@ -236,6 +236,28 @@ int qhg_gpu_compact(qhg_pop *pop);   /* LBController::compactData: here the geno
 int qhg_gpu_set_timing(qhg_pop *pop, int32_t enable);
 int qhg_gpu_kernel_times(qhg_pop *pop, double *avg_ms, int64_t *launches, int32_t max, int32_t *count);
 const char *qhg_gpu_kernel_name(int32_t index);
+
+/* Batched stepping.  No counterpart in the reference (QHG4 steps one
+ * population per process, PopLooper.cpp:166-202); it serves callers that run
+ * many independent scenarios on one GPU (ensembles, a scenario server).  A
+ * batch step advances every member by one step (the same as its
+ * initialize/doActions/finalize) with one launch per phase for all members;
+ * each member keeps its own buffers, parameters, seed and step count, and
+ * ends in exactly the state it would reach stepped alone.  While a member,
+ * a population takes no step calls of its own (they return QHG_ERR_STATE);
+ * its observables (stats, download, cell counts) stay available.  At most
+ * 256 members; all on the batch's device.  Destroying a member removes it. */
+typedef struct qhg_batch qhg_batch;
+int qhg_batch_create(qhg_batch **out);
+int qhg_batch_destroy(qhg_batch *batch);
+int qhg_batch_add(qhg_batch *batch, qhg_pop *pop);
+int qhg_batch_remove(qhg_batch *batch, qhg_pop *pop);
+int qhg_batch_size(qhg_batch *batch, int32_t *n);
+int qhg_batch_run(qhg_batch *batch, int32_t nsteps);
+int qhg_batch_sync(qhg_batch *batch);
+int qhg_batch_set_timing(qhg_batch *batch, int32_t enable);
+int qhg_batch_kernel_times(qhg_batch *batch, double *avg_ms, int64_t *launches, int32_t max, int32_t *count);
+const char *qhg_batch_kernel_name(int32_t index);
 
 #ifdef __cplusplus
 }

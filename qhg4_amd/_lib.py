@@ -20,6 +20,9 @@ EXPORTS = [
     "qhg_gpu_finalize_step", "qhg_gpu_run", "qhg_gpu_set_env", "qhg_gpu_sync", "qhg_gpu_stats",
     "qhg_gpu_cell_counts", "qhg_gpu_capacities", "qhg_gpu_env_weights", "qhg_gpu_compact",
     "qhg_gpu_set_timing", "qhg_gpu_kernel_times", "qhg_gpu_kernel_name",
+    "qhg_batch_create", "qhg_batch_destroy", "qhg_batch_add", "qhg_batch_remove", "qhg_batch_size",
+    "qhg_batch_run", "qhg_batch_sync", "qhg_batch_set_timing", "qhg_batch_kernel_times",
+    "qhg_batch_kernel_name",
 ]
 
 
@@ -61,6 +64,16 @@ def lib():
         "qhg_gpu_set_timing": ([h, C.c_int32], C.c_int),
         "qhg_gpu_kernel_times": ([h, P(C.c_double), P(C.c_int64), C.c_int32, P(C.c_int32)], C.c_int),
         "qhg_gpu_kernel_name": ([C.c_int32], C.c_char_p),
+        "qhg_batch_create": ([P(h)], C.c_int),
+        "qhg_batch_destroy": ([h], C.c_int),
+        "qhg_batch_add": ([h, h], C.c_int),
+        "qhg_batch_remove": ([h, h], C.c_int),
+        "qhg_batch_size": ([h, P(C.c_int32)], C.c_int),
+        "qhg_batch_run": ([h, C.c_int32], C.c_int),
+        "qhg_batch_sync": ([h], C.c_int),
+        "qhg_batch_set_timing": ([h, C.c_int32], C.c_int),
+        "qhg_batch_kernel_times": ([h, P(C.c_double), P(C.c_int64), C.c_int32, P(C.c_int32)], C.c_int),
+        "qhg_batch_kernel_name": ([C.c_int32], C.c_char_p),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -76,6 +76,7 @@ struct DevStatus {
     int64_t births_last, deaths_last, moves_last;
     int64_t births_total, deaths_total, moves_total;
     int64_t agent_steps, fertile_total, occupied_total;
+    int64_t acc_deaths, acc_moves;   // batched steps: this step's counts so far
     int32_t error, pad;
 };
 
@@ -959,112 +960,120 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs
 // newborns.  The block of the last tile closes the step's bookkeeping in the
 // other status half.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kScanBlock) k_cell_scan(Ctx x, StepArgs sa) {
-    __shared__ uint64_t s_w[kScanBlock / 64], s_w2[kScanBlock / 64];
-    __shared__ uint64_t s_excl, s_excl2;
+struct ScanLds {
+    uint64_t w[kScanBlock / 64], w2[kScanBlock / 64];
+    uint64_t excl, excl2;
+};
+
+// One look-back tile of k_cell_scan (kCellTile cells), all kScanBlock threads.
+__device__ __forceinline__ void scan_tile(const Ctx &x, const StepArgs &sa, uint32_t tile, ScanLds &L) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t nc = x.ncells;
+    const int64_t base = (int64_t)tile * kCellTile;
+    uint32_t stay[kScanPer];
+    int32_t e[kScanPer][kMaxIn];
+    uint32_t cm[kScanPer][kMaxIn];
+    uint64_t v[kScanPer];
+    // one round trip: stayers, births, arrivals per incoming run and the
+    // runs' source slots; the counts go back to zero once read
+    uint64_t sum = 0, osum = 0;
+#pragma unroll
+    for (int q = 0; q < kScanPer; q++) {
+        const int64_t c = base + (int64_t)threadIdx.x * kScanPer + q;
+        const bool ok = c < nc;
+        stay[q] = ok ? x.cstay[c] : 0u;
+        const uint32_t b = ok ? x.cbirth[c] : 0u;
+#pragma unroll
+        for (int k = 0; k < kMaxIn; k++) {
+            const bool in = ok && k < x.in_w;
+            e[q][k] = in ? x.in_e[c * x.in_w + k] : -1;
+            cm[q][k] = in ? x.cin[c * x.in_w + k] : 0u;
+        }
+        uint32_t inc = 0;
+#pragma unroll
+        for (int k = 0; k < kMaxIn; k++) {
+            inc += cm[q][k];
+            if (cm[q][k]) x.cin[c * x.in_w + k] = 0;
+        }
+        if (stay[q]) x.cstay[c] = 0;
+        if (b) x.cbirth[c] = 0;
+        v[q] = ok ? ((uint64_t)(stay[q] + inc + b) | ((uint64_t)b << 32)) : 0ull;
+        sum += v[q];
+        osum += ((uint32_t)v[q] != 0u) ? 1u : 0u;
+    }
+    // block exclusive scan of the per-thread sums
+    const uint64_t incl = wave_incl_scan_u64(sum);
+    const uint64_t oincl = wave_incl_scan_u64(osum);
+    if (lane == 63) { L.w[wave] = incl; L.w2[wave] = oincl; }
+    __syncthreads();
+    uint64_t wpre = 0, tot = 0, owpre = 0, otot = 0;
+#pragma unroll
+    for (int w = 0; w < kScanBlock / 64; w++) {
+        if (w < wave) { wpre += L.w[w]; owpre += L.w2[w]; }
+        tot += L.w[w];
+        otot += L.w2[w];
+    }
+    if (threadIdx.x < 64) {
+        uint64_t ex, ex2;
+        lookback2(x.tiles, x.tiles2, tile, tot, otot, &x.st->error, ex, ex2);
+        if (threadIdx.x == 0) { L.excl = ex; L.excl2 = ex2; }
+    }
+    __syncthreads();
+    uint64_t run = L.excl + wpre + incl - sum;
+    uint64_t orun = L.excl2 + owpre + oincl - osum;
+#pragma unroll
+    for (int q = 0; q < kScanPer; q++) {
+        const int64_t c = base + (int64_t)threadIdx.x * kScanPer + q;
+        if (c < nc) {
+            const uint32_t o = (uint32_t)run;
+            x.off_nxt[c] = o;
+            x.birth_off[c] = (uint32_t)(run >> 32);
+            uint32_t pos = o + stay[q];
+#pragma unroll
+            for (int k = 0; k < kMaxIn; k++) {
+                if (e[q][k] >= 0) x.rpos[e[q][k]] = pos;
+                pos += cm[q][k];
+            }
+            x.bpos[c] = pos;
+            const uint32_t nn = (uint32_t)v[q];
+            if (nn != 0) {
+                x.occ_nxt[orun] = OccCell{(int32_t)c, o, nn, 0u};
+                orun++;
+            }
+        }
+        run += v[q];
+    }
+    if (base + kCellTile >= nc && threadIdx.x == kScanBlock - 1) {
+        // last tile: the next step's sizes (the totals the step kernels
+        // need; deaths, moves and the host snapshot follow in k_cell_scatter)
+        const int64_t total = (int64_t)(uint32_t)run;
+        const int64_t B = (int64_t)(run >> 32);
+        x.off_nxt[nc] = (uint32_t)total;
+        x.birth_off[nc] = (uint32_t)B;
+        DevStatus *st = x.st;
+        const int p = (int)sa.parity, q = 1 - p;
+        if (total > x.cap || (x.gwords && st->gpool[p] + B > x.gcap)) st->error |= kErrCapacity;
+        st->agent_steps += st->n_alive[p];
+        st->occupied_total += st->n_occ[p];
+        st->n_alive[q] = total;
+        st->n_occ[q] = (int64_t)orun;
+        st->max_id[q] = st->max_id[p] + B;
+        st->gpool[q] = st->gpool[p] + (x.gwords ? B : 0);
+        st->births_last = B;
+        st->births_total += B;
+        st->step = (int64_t)sa.step + 1;
+        st->t = sa.t + 1.0;
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(kScanBlock) k_cell_scan(Ctx x, StepArgs sa) {
+    __shared__ ScanLds lds;
+    const int lane = threadIdx.x & 63;
     if (x.st->error) return;
     // tiles in increasing order per block: a block only ever waits on tiles
     // that resident blocks hold, whatever the grid size
-    for (uint32_t tile = blockIdx.x; tile < (uint32_t)x.ntiles; tile += gridDim.x) {
-        const int64_t base = (int64_t)tile * kCellTile;
-        uint32_t stay[kScanPer];
-        int32_t e[kScanPer][kMaxIn];
-        uint32_t cm[kScanPer][kMaxIn];
-        uint64_t v[kScanPer];
-        // one round trip: stayers, births, arrivals per incoming run and the
-        // runs' source slots; the counts go back to zero once read
-        uint64_t sum = 0, osum = 0;
-#pragma unroll
-        for (int q = 0; q < kScanPer; q++) {
-            const int64_t c = base + (int64_t)threadIdx.x * kScanPer + q;
-            const bool ok = c < nc;
-            stay[q] = ok ? x.cstay[c] : 0u;
-            const uint32_t b = ok ? x.cbirth[c] : 0u;
-#pragma unroll
-            for (int k = 0; k < kMaxIn; k++) {
-                const bool in = ok && k < x.in_w;
-                e[q][k] = in ? x.in_e[c * x.in_w + k] : -1;
-                cm[q][k] = in ? x.cin[c * x.in_w + k] : 0u;
-            }
-            uint32_t inc = 0;
-#pragma unroll
-            for (int k = 0; k < kMaxIn; k++) {
-                inc += cm[q][k];
-                if (cm[q][k]) x.cin[c * x.in_w + k] = 0;
-            }
-            if (stay[q]) x.cstay[c] = 0;
-            if (b) x.cbirth[c] = 0;
-            v[q] = ok ? ((uint64_t)(stay[q] + inc + b) | ((uint64_t)b << 32)) : 0ull;
-            sum += v[q];
-            osum += ((uint32_t)v[q] != 0u) ? 1u : 0u;
-        }
-        // block exclusive scan of the per-thread sums
-        const uint64_t incl = wave_incl_scan_u64(sum);
-        const uint64_t oincl = wave_incl_scan_u64(osum);
-        if (lane == 63) { s_w[wave] = incl; s_w2[wave] = oincl; }
-        __syncthreads();
-        uint64_t wpre = 0, tot = 0, owpre = 0, otot = 0;
-#pragma unroll
-        for (int w = 0; w < kScanBlock / 64; w++) {
-            if (w < wave) { wpre += s_w[w]; owpre += s_w2[w]; }
-            tot += s_w[w];
-            otot += s_w2[w];
-        }
-        if (threadIdx.x < 64) {
-            uint64_t ex, ex2;
-            lookback2(x.tiles, x.tiles2, tile, tot, otot, &x.st->error, ex, ex2);
-            if (threadIdx.x == 0) { s_excl = ex; s_excl2 = ex2; }
-        }
-        __syncthreads();
-        uint64_t run = s_excl + wpre + incl - sum;
-        uint64_t orun = s_excl2 + owpre + oincl - osum;
-#pragma unroll
-        for (int q = 0; q < kScanPer; q++) {
-            const int64_t c = base + (int64_t)threadIdx.x * kScanPer + q;
-            if (c < nc) {
-                const uint32_t o = (uint32_t)run;
-                x.off_nxt[c] = o;
-                x.birth_off[c] = (uint32_t)(run >> 32);
-                uint32_t pos = o + stay[q];
-#pragma unroll
-                for (int k = 0; k < kMaxIn; k++) {
-                    if (e[q][k] >= 0) x.rpos[e[q][k]] = pos;
-                    pos += cm[q][k];
-                }
-                x.bpos[c] = pos;
-                const uint32_t nn = (uint32_t)v[q];
-                if (nn != 0) {
-                    x.occ_nxt[orun] = OccCell{(int32_t)c, o, nn, 0u};
-                    orun++;
-                }
-            }
-            run += v[q];
-        }
-        if (base + kCellTile >= nc && threadIdx.x == kScanBlock - 1) {
-            // last tile: the next step's sizes (the totals the step kernels
-            // need; deaths, moves and the host snapshot follow in k_cell_scatter)
-            const int64_t total = (int64_t)(uint32_t)run;
-            const int64_t B = (int64_t)(run >> 32);
-            x.off_nxt[nc] = (uint32_t)total;
-            x.birth_off[nc] = (uint32_t)B;
-            DevStatus *st = x.st;
-            const int p = (int)sa.parity, q = 1 - p;
-            if (total > x.cap || (x.gwords && st->gpool[p] + B > x.gcap)) st->error |= kErrCapacity;
-            st->agent_steps += st->n_alive[p];
-            st->occupied_total += st->n_occ[p];
-            st->n_alive[q] = total;
-            st->n_occ[q] = (int64_t)orun;
-            st->max_id[q] = st->max_id[p] + B;
-            st->gpool[q] = st->gpool[p] + (x.gwords ? B : 0);
-            st->births_last = B;
-            st->births_total += B;
-            st->step = (int64_t)sa.step + 1;
-            st->t = sa.t + 1.0;
-        }
-        __syncthreads();
-    }
+    for (uint32_t tile = blockIdx.x; tile < (uint32_t)x.ntiles; tile += gridDim.x) scan_tile(x, sa, tile, lds);
     {
         // off the critical path: this block's slice of the k_cell_act
         // partials (deaths, moves, FERTILE counts), one atomic each
@@ -1527,6 +1536,319 @@ __global__ void __launch_bounds__(kWave) k_genome(Ctx x, Prog p, StepArgs sa) {
             const int s = w / nbk, b = w % nbk;
             const uint64_t *par = x.gpool + (s ? hf : hm) * gw;
             out[w] = child_word(x, p, sa.step, nid, par, s, b, nbk, w);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched steps: many independent populations per launch.  Each member keeps
+// its own buffers, status and parameters; a device table holds its context
+// for both step parities.  Work items (groups of occupied cells, scan tiles,
+// births) are numbered across the members by prefixes computed in every block
+// from the members' own status, so one launch per phase steps every member
+// exactly as it would step alone.
+// ---------------------------------------------------------------------------
+constexpr int kMaxPops = 256;          // members per batch (4 per lane)
+
+struct PopDesc {
+    Ctx ctx[2];               // by the member's step parity
+    Prog prog;
+    int64_t step0;            // the member's step and time when it joined,
+    int64_t join;             // at batch step `join`
+    double t0;
+    int64_t pad;
+};
+
+struct BatchArgs {
+    const PopDesc *pops;
+    int32_t npops;
+    int32_t total_tiles;      // scan tiles of all members
+    const int32_t *tile_base; // npops + 1: first global tile of each member
+    uint64_t *tiles;          // look-back words of all members (both words)
+    uint64_t *tiles2;
+    int64_t bstep;            // index of this batch step
+};
+
+// A member's step arguments: step and time advance by one per batch step
+// from where the member joined (fixed for the whole batch step, whatever the
+// kernels write into its status); the group size follows its occupancy.
+__device__ __forceinline__ StepArgs pop_step_args(const PopDesc &d, int64_t bstep) {
+    StepArgs a{};
+    const int64_t step = d.step0 + (bstep - d.join);
+    a.step = (uint32_t)step;
+    a.parity = (uint32_t)(step & 1);
+    a.t = d.t0 + (double)(bstep - d.join);
+    const DevStatus *st = d.ctx[0].st;
+    const int64_t occ = st->n_occ[a.parity] > 0 ? st->n_occ[a.parity] : 1;
+    double mean = (double)st->n_alive[a.parity] / (double)occ;
+    mean = mean > 1.0 ? mean : 1.0;
+    const double g = floor(64.0 / mean);
+    a.gcells = (int32_t)(g < 1.0 ? 1.0 : (g > (double)kGroupCells ? (double)kGroupCells : g));
+    return a;
+}
+
+// Exclusive prefix over the members of per-member counts (lane l holds
+// members l, l+64, l+128, l+192); returns the total.
+__device__ __forceinline__ int64_t pop_prefix(const int64_t (&cnt)[4], int64_t (&base)[4]) {
+    int64_t carry = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint64_t incl = wave_incl_scan_u64((uint64_t)cnt[k]);
+        base[k] = carry + (int64_t)incl - cnt[k];
+        carry += (int64_t)__shfl((long long)incl, 63, 64);
+    }
+    return carry;
+}
+
+// the member owning global item g: the last member whose base is <= g
+__device__ __forceinline__ int pop_of(const int64_t (&base)[4], int npops, int64_t g) {
+    const int lane = threadIdx.x & 63;
+    int n = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) n += __popcll(__ballot(lane + 64 * k < npops && base[k] <= g));
+    return n - 1;
+}
+
+__device__ __forceinline__ int64_t lane_value(const int64_t (&v)[4], int r) {
+    const int k = r >> 6, l = r & 63;
+    int64_t out = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        if (q == k) out = (int64_t)__shfl((long long)v[q], l, 64);
+    return out;
+}
+
+// per-member group counts of the current step
+__device__ __forceinline__ void batch_groups(const BatchArgs &B, int64_t (&cnt)[4]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int r = lane + 64 * k;
+        cnt[k] = 0;
+        if (r < B.npops) {
+            const DevStatus *st = B.pops[r].ctx[0].st;
+            if (!st->error) {
+                const StepArgs a = pop_step_args(B.pops[r], B.bstep);
+                const int64_t nocc = st->n_occ[a.parity];
+                cnt[k] = (nocc + a.gcells - 1) / a.gcells;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kCellBlock) k_batch_act(BatchArgs B) {
+    __shared__ ActLds lds;
+    const int lane = threadIdx.x;
+    // every member's look-back words start at zero
+    for (int32_t i = blockIdx.x * kCellBlock + lane; i < B.total_tiles; i += gridDim.x * kCellBlock) {
+        B.tiles[i] = 0;
+        B.tiles2[i] = 0;
+    }
+    int64_t cnt[4], base[4];
+    batch_groups(B, cnt);
+    const int64_t total = pop_prefix(cnt, base);
+    // a contiguous run of groups per block, so a block stays with one member
+    // for long stretches and adds its counts to that member once per stretch
+    const int64_t per = (total + gridDim.x - 1) / gridDim.x;
+    const int64_t g0 = (int64_t)blockIdx.x * per;
+    const int64_t g1 = (g0 + per < total) ? g0 + per : total;
+    int r = -1;
+    int64_t rbase = 0, rend = 0;
+    uint32_t deaths = 0, moves = 0, fert = 0;
+    StepArgs sa{};
+    const Ctx *x = nullptr;
+    const Prog *pg = nullptr;
+    int64_t nocc = 0;
+    for (int64_t g = g0; g < g1; g++) {
+        if (g >= rend) {
+            if (r >= 0) {
+                const uint64_t dm = wave_sum_u64(((uint64_t)moves << 32) | deaths);
+                const uint32_t fe = (uint32_t)wave_sum_u64(fert);
+                if (lane == 0) {
+                    atomicAdd((unsigned long long *)&x->st->acc_deaths, (unsigned long long)(uint32_t)dm);
+                    atomicAdd((unsigned long long *)&x->st->acc_moves, (unsigned long long)(dm >> 32));
+                    atomicAdd((unsigned long long *)&x->st->fertile_total, (unsigned long long)fe);
+                }
+                deaths = moves = fert = 0;
+            }
+            r = pop_of(base, B.npops, g);
+            rbase = lane_value(base, r);
+            rend = rbase + lane_value(cnt, r);
+            const DevStatus *st = B.pops[r].ctx[0].st;
+            sa = pop_step_args(B.pops[r], B.bstep);
+            x = &B.pops[r].ctx[sa.parity];
+            pg = &B.pops[r].prog;
+            nocc = st->n_occ[sa.parity];
+        }
+        const int64_t gc = sa.gcells;
+        const int64_t i0 = (g - rbase) * gc;
+        const int ng = (int)((nocc - i0 < gc) ? (nocc - i0) : gc);
+        OccCell e{0, 0, 0, 0};
+        if (lane < ng) e = x->occ_cur[i0 + lane];
+        const uint32_t S0 = readlane_u32(e.s, 0);
+        const uint32_t T = readlane_u32(e.s, ng - 1) + readlane_u32(e.n, ng - 1) - S0;
+        const uint32_t nmax = wave_max_u32(lane < ng ? e.n : 0u);
+        if (T <= (uint32_t)kGroupSlots && nmax <= (uint32_t)kGroupCellMax) {
+            group_act(*x, *pg, sa, ng, e, e.s - S0, S0, T, lds.g, deaths, moves, fert);
+            continue;
+        }
+        for (int k = 0; k < ng; k++) {
+            const int32_t c = (int32_t)readlane_u32((uint32_t)e.c, k);
+            const uint32_t s = readlane_u32(e.s, k);
+            const int32_t n = (int32_t)readlane_u32(e.n, k);
+            PairLists L;
+            if (n > kLdsCell) {
+                L = PairLists{x->gfu + s, x->gmu + s, x->gfid + s, x->gmid + s, x->gfs + s, x->gms + s,
+                              x->gmate + s, x->gflag + s};
+            } else {
+                L = PairLists{lds.c.fu, lds.c.mu, lds.c.fid, lds.c.mid, lds.c.fs, lds.c.ms, lds.c.mate,
+                              lds.c.flag};
+            }
+            cell_act(*x, *pg, sa, c, s, n, L, deaths, moves, fert);
+        }
+    }
+    if (r >= 0) {
+        const uint64_t dm = wave_sum_u64(((uint64_t)moves << 32) | deaths);
+        const uint32_t fe = (uint32_t)wave_sum_u64(fert);
+        if (lane == 0) {
+            atomicAdd((unsigned long long *)&x->st->acc_deaths, (unsigned long long)(uint32_t)dm);
+            atomicAdd((unsigned long long *)&x->st->acc_moves, (unsigned long long)(dm >> 32));
+            atomicAdd((unsigned long long *)&x->st->fertile_total, (unsigned long long)fe);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kScanBlock) k_batch_scan(BatchArgs B) {
+    __shared__ ScanLds lds;
+    const int lane = threadIdx.x & 63;
+    // static tile prefix; every wave of the block computes it the same way
+    int64_t tb[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int r = lane + 64 * k;
+        tb[k] = (r < B.npops) ? B.tile_base[r] : 0;
+    }
+    for (int64_t t = blockIdx.x; t < B.total_tiles; t += gridDim.x) {
+        const int r = pop_of(tb, B.npops, t);
+        const int64_t lt = t - lane_value(tb, r);
+        const DevStatus *st = B.pops[r].ctx[0].st;
+        if (st->error) continue;
+        const StepArgs sa = pop_step_args(B.pops[r], B.bstep);
+        scan_tile(B.pops[r].ctx[sa.parity], sa, (uint32_t)lt, lds);
+    }
+}
+
+__global__ void __launch_bounds__(kCellBlock) k_batch_scatter(BatchArgs B) {
+    __shared__ ScatLds lds;
+    const int lane = threadIdx.x;
+    if (blockIdx.x == 0) {
+        // close every member's step and copy its status into host memory
+        for (int r = 0; r < B.npops; r++) {
+            DevStatus *st = B.pops[r].ctx[0].st;
+            if (lane == 0 && !st->error) {
+                st->deaths_last = st->acc_deaths;
+                st->moves_last = st->acc_moves;
+                st->deaths_total += st->acc_deaths;
+                st->moves_total += st->acc_moves;
+                st->acc_deaths = 0;
+                st->acc_moves = 0;
+            }
+            wave_sync();
+            DevStatus *hr = B.pops[r].ctx[0].hring;
+            if (hr) {
+                const uint64_t *srcw = reinterpret_cast<const uint64_t *>(st);
+                uint64_t *dstw = reinterpret_cast<uint64_t *>(hr + (st->step % kStatusRing));
+                if (lane < (int)(sizeof(DevStatus) / 8)) dstw[lane] = srcw[lane];
+            }
+            wave_sync();
+        }
+        return;
+    }
+    // the same group numbering as k_batch_act
+    int64_t cnt[4], base[4];
+    batch_groups(B, cnt);
+    const int64_t total = pop_prefix(cnt, base);
+    const int64_t G = gridDim.x - 1;
+    const int64_t per = (total + G - 1) / G;
+    const int64_t g0 = (int64_t)(blockIdx.x - 1) * per;
+    const int64_t g1 = (g0 + per < total) ? g0 + per : total;
+    int r = -1;
+    int64_t rbase = 0, rend = 0, nocc = 0, max_id = 0, gbase = 0;
+    StepArgs sa{};
+    const Ctx *x = nullptr;
+    for (int64_t g = g0; g < g1; g++) {
+        if (g >= rend) {
+            r = pop_of(base, B.npops, g);
+            rbase = lane_value(base, r);
+            rend = rbase + lane_value(cnt, r);
+            const DevStatus *st = B.pops[r].ctx[0].st;
+            sa = pop_step_args(B.pops[r], B.bstep);
+            x = &B.pops[r].ctx[sa.parity];
+            nocc = st->n_occ[sa.parity];
+            max_id = st->max_id[sa.parity];
+            gbase = st->gpool[sa.parity];
+        }
+        const int64_t gc = sa.gcells;
+        const int64_t i0 = (g - rbase) * gc;
+        const int ng = (int)((nocc - i0 < gc) ? (nocc - i0) : gc);
+        OccCell e{0, 0, 0, 0};
+        if (lane < ng) e = x->occ_cur[i0 + lane];
+        const uint32_t S0 = readlane_u32(e.s, 0);
+        const uint32_t T = readlane_u32(e.s, ng - 1) + readlane_u32(e.n, ng - 1) - S0;
+        if (T <= (uint32_t)kGroupSlots) {
+            group_scatter(*x, sa, ng, e, e.s - S0, S0, T, lds.g, max_id, gbase);
+            continue;
+        }
+        for (int k = 0; k < ng; k++) {
+            const int32_t c = (int32_t)readlane_u32((uint32_t)e.c, k);
+            const uint32_t s = readlane_u32(e.s, k);
+            const int32_t n = (int32_t)readlane_u32(e.n, k);
+            int32_t *mli = (n > kLdsCell) ? x->gfu + s : lds.c.mli;
+            int64_t *mid = (n > kLdsCell) ? x->gfid + s : lds.c.mid;
+            cell_scatter(*x, sa, c, s, n, mli, mid, max_id, gbase);
+        }
+    }
+}
+
+// births of every member with Genetics, one lane per genome word
+__global__ void __launch_bounds__(kWave) k_batch_genome(BatchArgs B) {
+    const int lane = threadIdx.x;
+    int64_t cnt[4], base[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int r = lane + 64 * k;
+        cnt[k] = 0;
+        if (r < B.npops) {
+            const Ctx &x0 = B.pops[r].ctx[0];
+            if (x0.gwords && !x0.st->error) {
+                const int per = (x0.gwords <= kWave) ? kWave / x0.gwords : 1;
+                cnt[k] = (x0.st->births_last + per - 1) / per;   // births of the step just scanned
+            }
+        }
+    }
+    const int64_t total = pop_prefix(cnt, base);
+    for (int64_t g = blockIdx.x; g < total; g += gridDim.x) {
+        const int r = pop_of(base, B.npops, g);
+        const int64_t gl = g - lane_value(base, r);
+        const DevStatus *st = B.pops[r].ctx[0].st;
+        const StepArgs sa = pop_step_args(B.pops[r], B.bstep);
+        const int par = (int)sa.parity;
+        const Ctx &x = B.pops[r].ctx[par];
+        const Prog &pg = B.pops[r].prog;
+        const int gw = x.gwords, nbk = gw / 2;
+        const int per = (gw <= kWave) ? kWave / gw : 1;
+        const int pb = (gw <= kWave) ? lane / gw : 0;
+        const int64_t rg = gl * per + pb;
+        if (pb >= per || rg >= st->births_last) continue;
+        const int64_t max_id = st->max_id[par], gbase = st->gpool[par];
+        const int32_t ms = x.brec[2 * rg], fs = x.brec[2 * rg + 1];
+        const int64_t hm = x.cur.ghandle[ms], hf = x.cur.ghandle[fs];
+        const uint64_t nid = (uint64_t)(max_id + 1 + rg);
+        uint64_t *out = x.gpool + (gbase + rg) * gw;
+        for (int w = (gw <= kWave) ? lane % gw : lane; w < gw; w += (gw <= kWave) ? gw : kWave) {
+            const int s = w / nbk, b = w % nbk;
+            const uint64_t *par_g = x.gpool + (s ? hf : hm) * gw;
+            out[w] = child_word(x, pg, sa.step, nid, par_g, s, b, nbk, w);
         }
     }
 }

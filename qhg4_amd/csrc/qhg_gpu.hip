@@ -143,8 +143,16 @@ struct qhg_pop {
 
     int cur() const { return (int)(steps_enqueued & 1); }
 
-    Ctx ctx() const {
-        const int b = cur();
+    // batched stepping: the batch this population belongs to, and what it
+    // replaced while a member (its stream and status events)
+    qhg_batch *batch = nullptr;
+    hipStream_t own_stream = nullptr;
+    hipEvent_t own_ev[kRing] = {};
+    int64_t gen = 0;          // bumped whenever a device buffer moves
+
+    Ctx ctx() const { return ctx_for(cur()); }
+
+    Ctx ctx_for(int b) const {
         Ctx x{};
         x.cur = buf[b];
         x.nxt = buf[1 - b];
@@ -451,6 +459,7 @@ int compact_pool(qhg_pop *p, int64_t new_gcap) {
         p->gcap = new_gcap;
     }
     p->gp_cur = other;
+    p->gen++;
     p->pool_reset_step = p->steps_enqueued;
     p->compactions++;
     return 0;
@@ -486,6 +495,7 @@ int grow_agents(qhg_pop *p, int64_t new_cap, int64_t n) {
     p->buf[0] = nb[0];
     p->buf[1] = nb[1];
     p->cap = new_cap;
+    p->gen++;
     return alloc_slot_scratch(p, new_cap);
 }
 
@@ -795,6 +805,7 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
 
 int qhg_gpu_destroy(qhg_pop *p) {
     if (!p) return 0;
+    if (p->batch) (void)qhg_batch_remove(p->batch, p);
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     free_agentbuf(p->buf[0]);
     free_agentbuf(p->buf[1]);
@@ -935,6 +946,7 @@ int qhg_gpu_upload_agents(qhg_pop *p, const qhg_agents *a) {
     p->known = st;
     p->known_step = p->steps_enqueued;
     p->pool_reset_step = -1;
+    p->gen++;
     return 0;
 }
 
@@ -1029,6 +1041,7 @@ int qhg_gpu_pre_loop(qhg_pop *p, double t) {
 
 int qhg_gpu_initialize_step(qhg_pop *p, double t) {
     if (!p) return fail(QHG_ERR_ARG, "null handle");
+    if (p->batch) return fail(QHG_ERR_STATE, "population is stepped by its batch");
     int rc = phase_initialize(p);
     if (rc) return rc;
     p->t_next = t;
@@ -1039,6 +1052,7 @@ int qhg_gpu_do_actions(qhg_pop *p, uint32_t prio, double t) {
     (void)prio;
     (void)t;
     if (!p) return fail(QHG_ERR_ARG, "null handle");
+    if (p->batch) return fail(QHG_ERR_STATE, "population is stepped by its batch");
     // all priority levels run fused in the first call of a step
     if (p->acted) return 0;
     return phase_act(p);
@@ -1046,6 +1060,7 @@ int qhg_gpu_do_actions(qhg_pop *p, uint32_t prio, double t) {
 
 int qhg_gpu_finalize_step(qhg_pop *p) {
     if (!p) return fail(QHG_ERR_ARG, "null handle");
+    if (p->batch) return fail(QHG_ERR_STATE, "population is stepped by its batch");
     int rc = phase_finalize(p);
     if (rc) return rc;
     return p->timing ? collect_timing(p) : 0;
@@ -1053,6 +1068,7 @@ int qhg_gpu_finalize_step(qhg_pop *p) {
 
 int qhg_gpu_run(qhg_pop *p, int32_t nsteps) {
     if (!p) return fail(QHG_ERR_ARG, "null handle");
+    if (p->batch) return fail(QHG_ERR_STATE, "population is stepped by its batch");
     for (int32_t s = 0; s < nsteps; s++) {
         int rc = phase_initialize(p);
         if (rc) return rc;
@@ -1192,6 +1208,331 @@ int qhg_gpu_kernel_times(qhg_pop *p, double *avg_ms, int64_t *launches, int32_t 
 const char *qhg_gpu_kernel_name(int32_t index) {
     if (index < 0 || index >= KB_COUNT) return "";
     return kKernelNames[index];
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batched stepping (many populations per launch)
+// ===========================================================================
+enum BatchKernelId { BK_ACT, BK_SCAN, BK_SCATTER, BK_GENOME, BK_COUNT };
+
+struct qhg_batch {
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int num_cu = 256;
+    int act_grid = 1, scan_grid = 1, scatter_grid = 1, genome_grid = 1;
+    std::vector<qhg_pop *> pops;
+    std::vector<int64_t> gens, step0, join;
+    std::vector<double> t0;
+    bool dirty = true;
+    int64_t bstep = 0;
+    PopDesc *d_desc = nullptr;
+    size_t desc_cap = 0;
+    int32_t *d_tile_base = nullptr;
+    uint64_t *d_tiles = nullptr, *d_tiles2 = nullptr;
+    size_t tiles_cap = 0;
+    int32_t total_tiles = 0;
+    bool any_genetics = false;
+    hipEvent_t ev[kRing] = {};
+    int64_t ev_step[kRing];
+    // timing
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
+    std::vector<std::pair<int, size_t>> ev_used;
+    double t_sum[BK_COUNT] = {};
+    int64_t t_cnt[BK_COUNT] = {};
+};
+
+namespace {
+
+template <typename... KArgs, typename... Args>
+int blaunch(qhg_batch *b, int kid, void (*kern)(KArgs...), int grid, int block, Args... args) {
+    if (b->timing) {
+        const size_t slot = b->ev_used.size();
+        if (slot >= b->ev_pool.size()) {
+            hipEvent_t e0, e1;
+            HIPCHK(hipEventCreate(&e0));
+            HIPCHK(hipEventCreate(&e1));
+            b->ev_pool.push_back({e0, e1});
+        }
+        b->ev_used.push_back({kid, slot});
+        hipExtLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, b->stream, b->ev_pool[slot].first,
+                              b->ev_pool[slot].second, 0, args...);
+    } else {
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, b->stream, args...);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int batch_collect_timing(qhg_batch *b) {
+    if (b->ev_used.empty()) return 0;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (auto &u : b->ev_used) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, b->ev_pool[u.second].first, b->ev_pool[u.second].second));
+        b->t_sum[u.first] += ms;
+        b->t_cnt[u.first] += 1;
+    }
+    b->ev_used.clear();
+    return 0;
+}
+
+// device table of the members' contexts (both parities), their scan tiles
+// laid out one after another in the batch's look-back arrays
+int batch_upload_table(qhg_batch *b) {
+    const size_t n = b->pops.size();
+    std::vector<int32_t> tile_base(n + 1, 0);
+    for (size_t i = 0; i < n; i++) tile_base[i + 1] = tile_base[i] + b->pops[i]->ntiles;
+    const size_t tiles = std::max<size_t>(1, (size_t)tile_base[n]);
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (tiles > b->tiles_cap) {
+        if (b->d_tiles) (void)hipFree(b->d_tiles);
+        if (b->d_tiles2) (void)hipFree(b->d_tiles2);
+        b->d_tiles = b->d_tiles2 = nullptr;
+        HIPCHK(hipMalloc(&b->d_tiles, tiles * 8));
+        HIPCHK(hipMalloc(&b->d_tiles2, tiles * 8));
+        HIPCHK(hipMemset(b->d_tiles, 0, tiles * 8));
+        HIPCHK(hipMemset(b->d_tiles2, 0, tiles * 8));
+        b->tiles_cap = tiles;
+    }
+    if (n + 1 > b->desc_cap) {
+        if (b->d_desc) (void)hipFree(b->d_desc);
+        if (b->d_tile_base) (void)hipFree(b->d_tile_base);
+        b->d_desc = nullptr;
+        b->d_tile_base = nullptr;
+        const size_t c = std::max<size_t>(16, 2 * (n + 1));
+        HIPCHK(hipMalloc(&b->d_desc, c * sizeof(PopDesc)));
+        HIPCHK(hipMalloc(&b->d_tile_base, c * sizeof(int32_t)));
+        b->desc_cap = c;
+    }
+    std::vector<PopDesc> desc(std::max<size_t>(1, n));
+    b->any_genetics = false;
+    for (size_t i = 0; i < n; i++) {
+        qhg_pop *p = b->pops[i];
+        for (int par = 0; par < 2; par++) {
+            desc[i].ctx[par] = p->ctx_for(par);
+            desc[i].ctx[par].tiles = b->d_tiles + tile_base[i];
+            desc[i].ctx[par].tiles2 = b->d_tiles2 + tile_base[i];
+        }
+        desc[i].prog = p->prog;
+        desc[i].step0 = b->step0[i];
+        desc[i].join = b->join[i];
+        desc[i].t0 = b->t0[i];
+        b->gens[i] = p->gen;
+        if (p->gwords) b->any_genetics = true;
+    }
+    if (n) HIPCHK(hipMemcpy(b->d_desc, desc.data(), n * sizeof(PopDesc), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->d_tile_base, tile_base.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+    b->total_tiles = tile_base[n];
+    b->dirty = false;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qhg_batch_create(qhg_batch **out) {
+    if (!out) return fail(QHG_ERR_ARG, "null argument");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(QHG_ERR_HIP, "no HIP device");
+    qhg_batch *b = new qhg_batch();
+    if (hipGetDevice(&b->device) != hipSuccess) { delete b; return fail(QHG_ERR_HIP, "hipGetDevice"); }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, b->device) == hipSuccess && prop.multiProcessorCount > 0)
+        b->num_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return fail(QHG_ERR_HIP, "stream");
+    }
+    for (int k = 0; k < kRing; k++) {
+        if (hipEventCreateWithFlags(&b->ev[k], hipEventDisableTiming) != hipSuccess) {
+            qhg_batch_destroy(b);
+            return fail(QHG_ERR_HIP, "event");
+        }
+        b->ev_step[k] = -1;
+    }
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_scan, kScanBlock, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    b->scan_grid = per_cu * b->num_cu;
+    b->act_grid = b->num_cu * QHG_CELL_GRID_PER_CU;
+    b->scatter_grid = b->act_grid;
+    b->genome_grid = b->num_cu * 16;
+    *out = b;
+    return 0;
+}
+
+int qhg_batch_destroy(qhg_batch *b) {
+    if (!b) return 0;
+    while (!b->pops.empty()) (void)qhg_batch_remove(b, b->pops.back());
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    void *ptrs[] = {b->d_desc, b->d_tile_base, b->d_tiles, b->d_tiles2};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    for (int k = 0; k < kRing; k++)
+        if (b->ev[k]) (void)hipEventDestroy(b->ev[k]);
+    for (auto &e : b->ev_pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return 0;
+}
+
+int qhg_batch_add(qhg_batch *b, qhg_pop *p) {
+    if (!b || !p) return fail(QHG_ERR_ARG, "null argument");
+    if (p->batch) return fail(QHG_ERR_STATE, "population already belongs to a batch");
+    if (p->device != b->device) return fail(QHG_ERR_ARG, "population and batch are on different devices");
+    if ((int)b->pops.size() >= kMaxPops) return fail(QHG_ERR_CAPACITY, "batch is full (256 populations)");
+    if (p->acted || p->initialized) return fail(QHG_ERR_STATE, "population is inside a step");
+    // hand over: everything the population enqueued so far has finished
+    HIPCHK(hipStreamSynchronize(p->stream));
+    poll_ring(p);
+    int rc = refresh_known(p);
+    if (rc) return rc;
+    if (p->has_me && !p->me_done) {
+        rc = run_evaluators(p);
+        if (rc) return rc;
+        p->me_done = true;
+        HIPCHK(hipStreamSynchronize(p->stream));
+    }
+    p->own_stream = p->stream;
+    p->stream = b->stream;
+    for (int k = 0; k < kRing; k++) {
+        p->own_ev[k] = p->ring_ev[k];
+        p->ring_ev[k] = nullptr;
+        p->ring_step[k] = -1;
+    }
+    p->batch = b;
+    b->pops.push_back(p);
+    b->gens.push_back(p->gen);
+    b->step0.push_back(p->steps_enqueued);
+    b->join.push_back(b->bstep);
+    b->t0.push_back(p->t_next);
+    b->dirty = true;
+    return 0;
+}
+
+int qhg_batch_remove(qhg_batch *b, qhg_pop *p) {
+    if (!b || !p) return fail(QHG_ERR_ARG, "null argument");
+    size_t i = 0;
+    while (i < b->pops.size() && b->pops[i] != p) i++;
+    if (i == b->pops.size()) return fail(QHG_ERR_ARG, "population is not in this batch");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    poll_ring(p);
+    p->stream = p->own_stream;
+    for (int k = 0; k < kRing; k++) {
+        p->ring_ev[k] = p->own_ev[k];
+        p->ring_step[k] = -1;
+    }
+    p->batch = nullptr;
+    int rc = refresh_known(p);
+    b->pops.erase(b->pops.begin() + i);
+    b->gens.erase(b->gens.begin() + i);
+    b->step0.erase(b->step0.begin() + i);
+    b->join.erase(b->join.begin() + i);
+    b->t0.erase(b->t0.begin() + i);
+    b->dirty = true;
+    return rc;
+}
+
+int qhg_batch_size(qhg_batch *b, int32_t *n) {
+    if (!b || !n) return fail(QHG_ERR_ARG, "null argument");
+    *n = (int32_t)b->pops.size();
+    return 0;
+}
+
+int qhg_batch_run(qhg_batch *b, int32_t nsteps) {
+    if (!b) return fail(QHG_ERR_ARG, "null handle");
+    for (int32_t s = 0; s < nsteps; s++) {
+        if (b->pops.empty()) return 0;
+        for (qhg_pop *p : b->pops) {
+            int rc = ensure_capacity(p);
+            if (rc) return rc;
+        }
+        bool stale = b->dirty;
+        for (size_t i = 0; i < b->pops.size(); i++) stale = stale || b->gens[i] != b->pops[i]->gen;
+        if (stale) {
+            int rc = batch_upload_table(b);
+            if (rc) return rc;
+        }
+        BatchArgs A{};
+        A.pops = b->d_desc;
+        A.npops = (int32_t)b->pops.size();
+        A.total_tiles = b->total_tiles;
+        A.tile_base = b->d_tile_base;
+        A.tiles = b->d_tiles;
+        A.tiles2 = b->d_tiles2;
+        A.bstep = b->bstep;
+        int rc = blaunch(b, BK_ACT, k_batch_act, b->act_grid, kCellBlock, A);
+        if (rc) return rc;
+        rc = blaunch(b, BK_SCAN, k_batch_scan, std::max(1, std::min(b->scan_grid, b->total_tiles)), kScanBlock, A);
+        if (rc) return rc;
+        rc = blaunch(b, BK_SCATTER, k_batch_scatter, b->scatter_grid + 1, kCellBlock, A);
+        if (rc) return rc;
+        if (b->any_genetics) {
+            rc = blaunch(b, BK_GENOME, k_batch_genome, b->genome_grid, kWave, A);
+            if (rc) return rc;
+        }
+        b->bstep++;
+        for (qhg_pop *p : b->pops) {
+            p->steps_enqueued++;
+            p->t_next += 1.0;
+        }
+        // one event per batch step stands for every member's status snapshot
+        const int slot = (int)(b->bstep % kRing);
+        if (b->ev_step[slot] >= 0) {
+            HIPCHK(hipEventSynchronize(b->ev[slot]));
+            for (qhg_pop *p : b->pops) poll_ring(p);
+        }
+        HIPCHK(hipEventRecord(b->ev[slot], b->stream));
+        b->ev_step[slot] = b->bstep;
+        for (qhg_pop *p : b->pops) {
+            const int k = (int)(p->steps_enqueued % kRing);
+            p->ring_ev[k] = b->ev[slot];
+            p->ring_step[k] = p->steps_enqueued;
+        }
+        if (b->timing && b->ev_used.size() > 4096) {
+            rc = batch_collect_timing(b);
+            if (rc) return rc;
+        }
+    }
+    return 0;
+}
+
+int qhg_batch_sync(qhg_batch *b) {
+    if (!b) return fail(QHG_ERR_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int qhg_batch_set_timing(qhg_batch *b, int32_t enable) {
+    if (!b) return fail(QHG_ERR_ARG, "null handle");
+    int rc = batch_collect_timing(b);
+    if (rc) return rc;
+    b->timing = enable != 0;
+    for (int k = 0; k < BK_COUNT; k++) { b->t_sum[k] = 0; b->t_cnt[k] = 0; }
+    return 0;
+}
+
+const char *qhg_batch_kernel_name(int32_t index) {
+    static const char *names[BK_COUNT] = {"k_batch_act", "k_batch_scan", "k_batch_scatter", "k_batch_genome"};
+    if (index < 0 || index >= BK_COUNT) return "";
+    return names[index];
+}
+
+int qhg_batch_kernel_times(qhg_batch *b, double *avg_ms, int64_t *launches, int32_t max, int32_t *count) {
+    if (!b) return fail(QHG_ERR_ARG, "null handle");
+    int rc = batch_collect_timing(b);
+    if (rc) return rc;
+    const int n = std::min<int>(max, BK_COUNT);
+    for (int k = 0; k < n; k++) {
+        if (avg_ms) avg_ms[k] = b->t_cnt[k] ? b->t_sum[k] / (double)b->t_cnt[k] : 0.0;
+        if (launches) launches[k] = b->t_cnt[k];
+    }
+    if (count) *count = n;
+    return 0;
 }
 
 }  // extern "C"

@@ -123,3 +123,55 @@ class GpuPopulation:
         got = C.c_int32()
         check(lib().qhg_gpu_kernel_times(self.h, avg, cnt, n, C.byref(got)), "qhg_gpu_kernel_times")
         return {lib().qhg_gpu_kernel_name(i).decode(): (avg[i], cnt[i]) for i in range(got.value)}
+
+
+class GpuBatch:
+    """Many populations stepped together: one launch per phase for every
+    member (qhg_batch_* in include/qhg_gpu.h).  Each member ends in the state
+    it would reach stepped alone; while a member it takes no step calls of
+    its own, its observables stay available."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().qhg_batch_create(C.byref(h)), "qhg_batch_create")
+        self.h = h
+        self.members = []
+
+    def add(self, pop):
+        check(lib().qhg_batch_add(self.h, pop.h), "qhg_batch_add")
+        self.members.append(pop)
+
+    def remove(self, pop):
+        check(lib().qhg_batch_remove(self.h, pop.h), "qhg_batch_remove")
+        self.members.remove(pop)
+
+    def __len__(self):
+        n = C.c_int32()
+        check(lib().qhg_batch_size(self.h, C.byref(n)), "qhg_batch_size")
+        return n.value
+
+    def run(self, nsteps):
+        check(lib().qhg_batch_run(self.h, int(nsteps)), "qhg_batch_run")
+
+    def sync(self):
+        check(lib().qhg_batch_sync(self.h), "qhg_batch_sync")
+
+    def set_timing(self, on):
+        check(lib().qhg_batch_set_timing(self.h, 1 if on else 0), "qhg_batch_set_timing")
+
+    def kernel_times(self):
+        n = 8
+        avg = (C.c_double * n)()
+        cnt = (C.c_int64 * n)()
+        got = C.c_int32()
+        check(lib().qhg_batch_kernel_times(self.h, avg, cnt, n, C.byref(got)), "qhg_batch_kernel_times")
+        return {lib().qhg_batch_kernel_name(i).decode(): (avg[i], cnt[i]) for i in range(got.value)}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().qhg_batch_destroy(self.h)
+            self.h = None
+            self.members = []
+
+    def __del__(self):
+        self.close()

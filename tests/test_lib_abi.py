@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_words():
     L = _lib.lib()
-    assert L.qhg_gpu_abi_version() == 2
+    assert L.qhg_gpu_abi_version() == 3
     assert L.qhg_gpu_genome_words(256) == 8 == abi.genome_words(256)
     assert L.qhg_gpu_genome_words(1) == 1
 

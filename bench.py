@@ -105,6 +105,156 @@ def make_workload(name, seed):
     return sc, desc
 
 
+C4_DESC = ("C4: ensemble of C2-style replicas (grids of 2,562-163,842 cells, 10k-2M initial agents "
+           "log-uniform, per-replica move/birth/death/K-scale parameters), each run alone through the "
+           "plain path vs all as members of one batch")
+
+
+def run_c4(args):
+    """The ensemble leg: plain one-at-a-time vs batched, same scenarios,
+    same steps; the per-replica results must be identical."""
+    from qhg4_amd import abi, scenarios, ensemble
+
+    def make(r):
+        return scenarios.c4_replica(r, base_seed=args.seed, steps=args.steps)
+
+    n, steps = args.replicas, args.steps
+    print("bench: C4 %d replicas x %d steps, plain path one at a time" % (n, steps), file=sys.stderr, flush=True)
+    t_plain, plain = ensemble.run_plain(
+        make, n, steps, log=lambda r, t: print("bench: plain %d done, %.2f s" % (r, t), file=sys.stderr, flush=True))
+    print("bench: plain %.2f s; batched" % t_plain, file=sys.stderr, flush=True)
+    run = ensemble.BatchedRun(make, n, graph=not args.no_graph)
+    dev_bytes = run.device_bytes()
+    t_batch, tuning = run.run(steps, tune_steps=0 if args.no_tune else 2)
+    batched = run.summaries()
+    identical = batched == plain
+    agent_steps = sum(s["agent_steps"] for s in batched)
+    print("bench: batched %.2f s, identical=%s" % (t_batch, identical), file=sys.stderr, flush=True)
+    # per-kernel device time over a few more steps, timed per dispatch
+    roof, kernels = None, {}
+    if not args.no_kernel_timing:
+        ks = 5
+        s0 = [p.stats() for p in run.pops]
+        run.batch.set_timing(True)
+        run.batch.run(ks)
+        times = run.batch.kernel_times()
+        s1 = [p.stats() for p in run.pops]
+        bm = bytes_model()
+        tot = {}
+        for p, a, b in zip(run.pops, s0, s1):
+            prm = p.params
+            ops = {prm.action_kind[i] for i in range(prm.num_actions)}
+            for k, v in kernel_bytes(bm, delta(b, a, ks), p.grid, ops, 0).items():
+                kk = k.replace("k_cell_", "k_batch_")
+                tot[kk] = tot.get(kk, 0) + v
+        kernels, roof = kernel_table(tot, times)
+    run.close()
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(make(0), 0, steps, args.cpu_seconds)
+        cpu["sample"] = "replica 0 of the ensemble: " + cpu["sample"]
+    speedup = t_plain / t_batch
+    out = base_line(args, agent_steps / t_batch, t_batch * 1e3 / steps, 0)
+    out["config"] = {"workload": C4_DESC, "replicas": n, "steps_per_replica": steps, "base_seed": args.seed,
+                     "initial_agents_total": int(sum(scenarios.c4_draw(r, args.seed)["n0"] for r in range(n))),
+                     "agent_steps_total": int(agent_steps), "device_bytes_resident": int(dev_bytes)}
+    out["ensemble"] = {"plain_value": round(agent_steps / t_plain, 1), "plain_seconds": round(t_plain, 3),
+                       "batched_seconds": round(t_batch, 3), "speedup_vs_plain": round(speedup, 3),
+                       "identical_to_plain": identical, "graph": not args.no_graph, "tuning": tuning}
+    out["roofline"] = roof
+    out["cpu_baseline"] = cpu
+    out["kernels"] = kernels
+    return out
+
+
+C5_DESC = ("C5: served stream of C2-style scenario requests (Poisson arrivals at 20/s, grids of 2,562-655,362 "
+           "cells, 10k-5M initial agents log-uniform, horizons 200-5,000 steps, per-request parameters), "
+           "continuous batching under an HBM budget")
+
+
+def run_c5(args):
+    """The serving leg: requests arrive in real time and are served by
+    continuous batching; a sample of them is also run alone through the plain
+    path, for the plain rate and to check that their results are the same
+    alone and in the mix."""
+    import numpy as np
+    from qhg4_amd import scenarios, ensemble
+    from qhg4_amd.server import ScenarioServer
+
+    reqs = scenarios.c5_requests(args.requests, base_seed=args.seed)
+    for lv in sorted({r["level"] for r in reqs}):
+        scenarios.c4_world(lv, args.seed)          # the server's grids and fields, built before it starts
+
+    def make(req):
+        return scenarios.c5_scenario(req, world_seed=args.seed)
+
+    def log(msg):
+        print("bench: " + msg, file=sys.stderr, flush=True)
+
+    log("C5 %d requests, budget %.0f GB" % (len(reqs), args.budget_gb))
+    srv = ScenarioServer(make, budget_bytes=args.budget_gb * 1e9, chunk=args.chunk, graph=not args.no_graph)
+    results, rep = srv.serve(reqs, log=log)
+    log("served %d requests in %.1f s, p50 %.2f s p99 %.2f s" % (rep["requests"], rep["seconds"], rep["p50_s"],
+                                                                 rep["p99_s"]))
+    # plain path on a sample: every k-th request by id
+    by_id = {r["id"]: r for r in results}
+    sample = [reqs[i] for i in np.linspace(0, len(reqs) - 1, min(args.plain_sample, len(reqs))).astype(int)]
+    t_plain, steps_plain, same = 0.0, 0, True
+    for q in sample:
+        sec, summ = ensemble.run_plain(lambda _r, q=q: make(q), 1, q["horizon"])
+        t_plain += sec
+        steps_plain += summ[0]["agent_steps"]
+        same = same and summ[0] == by_id[q["id"]]["summary"]
+    log("plain sample of %d: %.2f s, identical=%s" % (len(sample), t_plain, same))
+    plain_value = steps_plain / t_plain if t_plain > 0 else None
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(make(reqs[0]), 0, reqs[0]["horizon"], args.cpu_seconds)
+        cpu["sample"] = "request 0 of the stream: " + cpu["sample"]
+    out = base_line(args, rep["value"], float("nan"), 0)
+    out["steps"] = None
+    out["ms_per_step"] = None
+    out["config"] = {"workload": C5_DESC, "requests": len(reqs), "budget_gb": args.budget_gb,
+                     "chunk_steps": args.chunk, "base_seed": args.seed}
+    out["serving"] = {"p50_latency_s": round(rep["p50_s"], 3), "p99_latency_s": round(rep["p99_s"], 3),
+                      "mean_latency_s": round(rep["mean_s"], 3), "seconds": rep["seconds"],
+                      "agent_steps": rep["agent_steps"], "peak_members": rep["peak_members"],
+                      "peak_device_bytes": rep["peak_device_bytes"], "tunings": rep["tunings"],
+                      "plain_sample": len(sample), "plain_value": round(plain_value, 1) if plain_value else None,
+                      "speedup_vs_plain": round(rep["value"] / plain_value, 3) if plain_value else None,
+                      "sample_identical_alone_and_in_mix": same}
+    out["roofline"] = None
+    out["cpu_baseline"] = cpu
+    return out
+
+
+def kernel_table(total_bytes, times):
+    kernels = {}
+    for name, bts in total_bytes.items():
+        avg_ms, n = times.get(name, (0.0, 0))
+        if not n:
+            continue
+        per_launch = bts / n
+        gbps = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+        kernels[name] = {"avg_us": round(avg_ms * 1e3, 3), "launches": n,
+                         "bytes_per_launch": int(per_launch), "GBps": round(gbps, 1)}
+    if not kernels:
+        return kernels, None
+    dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
+    kd = kernels[dom]
+    roof = {"bound": "hbm", "achieved": kd["GBps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": round(kd["GBps"] / HBM_PEAK_GBPS, 5), "traffic": None, "kernel": dom,
+            "bytes_per_launch": kd["bytes_per_launch"], "avg_launch_us": kd["avg_us"]}
+    return kernels, roof
+
+
+def base_line(args, value, ms_per_step, warmup):
+    return {"metric": METRIC, "value": round(value, 1), "unit": "agent-steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": warmup, "ms_per_step": round(ms_per_step, 5),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic"}
+
+
 def delta(s1, s0, steps):
     return {"live": s1["agent_steps"] - s0["agent_steps"],
             "occupied": s1["occupied_cell_steps"] - s0["occupied_cell_steps"],
@@ -161,7 +311,14 @@ def main():
     ap.add_argument("--steps", type=int, default=1900)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--requests", type=int, default=2000, help="C5: requests in the stream")
+    ap.add_argument("--budget-gb", type=float, default=250.0, help="C5: HBM budget for resident state")
+    ap.add_argument("--chunk", type=int, default=16, help="C5: steps per batch chunk")
+    ap.add_argument("--plain-sample", type=int, default=16, help="C5: requests also run alone")
+    ap.add_argument("--replicas", type=int, default=256, help="C4: replicas in the ensemble")
+    ap.add_argument("--no-graph", action="store_true", help="C4: launch batch steps without graph replay")
+    ap.add_argument("--no-tune", action="store_true", help="C4: keep the default launch shapes")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
@@ -173,6 +330,19 @@ def main():
         print("bench.py: this build runs on one GPU only (the population step has no multi-GPU path); "
               "use --gpus 1", file=sys.stderr)
         return 2
+
+    if args.workload == "c5":
+        if "--seed" not in sys.argv:
+            args.seed = 2000
+        print(json.dumps(run_c5(args)))
+        return 0
+    if args.workload == "c4":
+        if "--steps" not in sys.argv:
+            args.steps = 1000
+        if "--seed" not in sys.argv:
+            args.seed = 1000
+        print(json.dumps(run_c4(args)))
+        return 0
 
     from qhg4_amd import abi
     sc, desc = make_workload(args.workload, args.seed)

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define QHG_ABI_VERSION 3
+#define QHG_ABI_VERSION 4
 
 /* error codes */
 #define QHG_OK              0
@@ -237,6 +237,28 @@ int qhg_gpu_set_timing(qhg_pop *pop, int32_t enable);
 int qhg_gpu_kernel_times(qhg_pop *pop, double *avg_ms, int64_t *launches, int32_t max, int32_t *count);
 const char *qhg_gpu_kernel_name(int32_t index);
 
+/* Device memory held by the population (agent buffers at the current
+ * capacity, genome pools, per-cell tables, scratch), in bytes.  A capacity
+ * growth briefly holds the old and the new agent buffers.  No counterpart
+ * in the reference; for callers that admit work against a memory budget. */
+int qhg_gpu_device_bytes(qhg_pop *pop, int64_t *bytes);
+
+/* Launch shapes of the step kernels.  group_agents: target agents per wave
+ * group of occupied cells (1..4096); blocks_per_cu: act and scatter blocks
+ * per compute unit (1..32).  Results never depend on them.  No counterpart
+ * in the reference (OpenMP schedules its loops itself). */
+typedef struct qhg_tuning {
+    double group_agents;
+    int32_t blocks_per_cu;
+    int32_t pad;
+} qhg_tuning;
+int qhg_gpu_set_tuning(qhg_pop *pop, const qhg_tuning *t);
+int qhg_gpu_get_tuning(qhg_pop *pop, qhg_tuning *t);
+/* Autotuner: times candidate shapes on the population's own steps
+ * (trial_steps each; the population advances by *steps_taken steps), keeps
+ * the one with the most agent-steps per unit of device time and returns it. */
+int qhg_gpu_autotune(qhg_pop *pop, int32_t trial_steps, qhg_tuning *chosen, int32_t *steps_taken);
+
 /* Batched stepping.  No counterpart in the reference (QHG4 steps one
  * population per process, PopLooper.cpp:166-202); it serves callers that run
  * many independent scenarios on one GPU (ensembles, a scenario server).  A
@@ -258,6 +280,12 @@ int qhg_batch_sync(qhg_batch *batch);
 int qhg_batch_set_timing(qhg_batch *batch, int32_t enable);
 int qhg_batch_kernel_times(qhg_batch *batch, double *avg_ms, int64_t *launches, int32_t max, int32_t *count);
 const char *qhg_batch_kernel_name(int32_t index);
+/* Replay each batch step from a captured HIP graph (recaptured when the
+ * members or shapes change; not while timing is on). */
+int qhg_batch_set_graph(qhg_batch *batch, int32_t enable);
+int qhg_batch_set_tuning(qhg_batch *batch, const qhg_tuning *t);
+int qhg_batch_get_tuning(qhg_batch *batch, qhg_tuning *t);
+int qhg_batch_autotune(qhg_batch *batch, int32_t trial_steps, qhg_tuning *chosen, int32_t *steps_taken);
 
 #ifdef __cplusplus
 }

@@ -19,10 +19,12 @@ EXPORTS = [
     "qhg_gpu_download_agents", "qhg_gpu_pre_loop", "qhg_gpu_initialize_step", "qhg_gpu_do_actions",
     "qhg_gpu_finalize_step", "qhg_gpu_run", "qhg_gpu_set_env", "qhg_gpu_sync", "qhg_gpu_stats",
     "qhg_gpu_cell_counts", "qhg_gpu_capacities", "qhg_gpu_env_weights", "qhg_gpu_compact",
-    "qhg_gpu_set_timing", "qhg_gpu_kernel_times", "qhg_gpu_kernel_name",
+    "qhg_gpu_set_timing", "qhg_gpu_kernel_times", "qhg_gpu_kernel_name", "qhg_gpu_device_bytes",
+    "qhg_gpu_set_tuning", "qhg_gpu_get_tuning", "qhg_gpu_autotune",
     "qhg_batch_create", "qhg_batch_destroy", "qhg_batch_add", "qhg_batch_remove", "qhg_batch_size",
     "qhg_batch_run", "qhg_batch_sync", "qhg_batch_set_timing", "qhg_batch_kernel_times",
-    "qhg_batch_kernel_name",
+    "qhg_batch_kernel_name", "qhg_batch_set_graph", "qhg_batch_set_tuning", "qhg_batch_get_tuning",
+    "qhg_batch_autotune",
 ]
 
 
@@ -64,6 +66,10 @@ def lib():
         "qhg_gpu_set_timing": ([h, C.c_int32], C.c_int),
         "qhg_gpu_kernel_times": ([h, P(C.c_double), P(C.c_int64), C.c_int32, P(C.c_int32)], C.c_int),
         "qhg_gpu_kernel_name": ([C.c_int32], C.c_char_p),
+        "qhg_gpu_device_bytes": ([h, P(C.c_int64)], C.c_int),
+        "qhg_gpu_set_tuning": ([h, P(abi.Tuning)], C.c_int),
+        "qhg_gpu_get_tuning": ([h, P(abi.Tuning)], C.c_int),
+        "qhg_gpu_autotune": ([h, C.c_int32, P(abi.Tuning), P(C.c_int32)], C.c_int),
         "qhg_batch_create": ([P(h)], C.c_int),
         "qhg_batch_destroy": ([h], C.c_int),
         "qhg_batch_add": ([h, h], C.c_int),
@@ -74,6 +80,10 @@ def lib():
         "qhg_batch_set_timing": ([h, C.c_int32], C.c_int),
         "qhg_batch_kernel_times": ([h, P(C.c_double), P(C.c_int64), C.c_int32, P(C.c_int32)], C.c_int),
         "qhg_batch_kernel_name": ([C.c_int32], C.c_char_p),
+        "qhg_batch_set_graph": ([h, C.c_int32], C.c_int),
+        "qhg_batch_set_tuning": ([h, P(abi.Tuning)], C.c_int),
+        "qhg_batch_get_tuning": ([h, P(abi.Tuning)], C.c_int),
+        "qhg_batch_autotune": ([h, C.c_int32, P(abi.Tuning), P(C.c_int32)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -153,6 +153,18 @@ class Stats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class Tuning(C.Structure):
+    """qhg_tuning: launch shapes of the step kernels (results never depend on them)."""
+    _fields_ = [
+        ("group_agents", C.c_double),
+        ("blocks_per_cu", C.c_int32),
+        ("pad", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {"group_agents": self.group_agents, "blocks_per_cu": self.blocks_per_cu}
+
+
 def ptr(arr, ctype):
     """Pointer to a contiguous numpy array (or NULL for None)."""
     if arr is None:

@@ -1559,6 +1559,14 @@ struct PopDesc {
     int64_t pad;
 };
 
+// The batch step index lives in device memory, one copy per kernel of the
+// step (act, scan, scatter, genome), so the launch arguments stay the same
+// from step to step and one step can be captured in a graph and replayed.
+// Every kernel reads its own copy; block 0 sets the next kernel's copy, which
+// no running kernel reads, and the last kernel of the step advances the act
+// kernel's copy for the next step.
+enum BatchCounter { BC_ACT, BC_SCAN, BC_SCATTER, BC_GENOME, BC_COUNT };
+
 struct BatchArgs {
     const PopDesc *pops;
     int32_t npops;
@@ -1566,24 +1574,42 @@ struct BatchArgs {
     const int32_t *tile_base; // npops + 1: first global tile of each member
     uint64_t *tiles;          // look-back words of all members (both words)
     uint64_t *tiles2;
-    int64_t bstep;            // index of this batch step
+    int64_t *bctr;            // BC_COUNT copies of the batch step index
+    double group_agents;      // target agents per wave group
+    int32_t genome;           // k_batch_genome runs in this step
+    int32_t pad;
 };
+
+__device__ __forceinline__ int64_t batch_step_in(const BatchArgs &B, int k, int next) {
+    const int64_t b = B.bctr[k];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (next >= 0) B.bctr[next] = b;
+        if (next < 0 || (k == BC_SCATTER && !B.genome)) B.bctr[BC_ACT] = b + 1;
+    }
+    return b;
+}
+
+// cells per wave group: about `target` agents per group from the mean
+// occupancy (results do not depend on it)
+__host__ __device__ __forceinline__ int32_t group_cells(double target, int64_t alive, int64_t occ) {
+    const double o = (double)(occ > 0 ? occ : 1);
+    double mean = (double)alive / o;
+    mean = mean > 1.0 ? mean : 1.0;
+    const double g = floor(target / mean);
+    return (int32_t)(g < 1.0 ? 1.0 : (g > (double)kGroupCells ? (double)kGroupCells : g));
+}
 
 // A member's step arguments: step and time advance by one per batch step
 // from where the member joined (fixed for the whole batch step, whatever the
 // kernels write into its status); the group size follows its occupancy.
-__device__ __forceinline__ StepArgs pop_step_args(const PopDesc &d, int64_t bstep) {
+__device__ __forceinline__ StepArgs pop_step_args(const PopDesc &d, int64_t bstep, double group_agents) {
     StepArgs a{};
     const int64_t step = d.step0 + (bstep - d.join);
     a.step = (uint32_t)step;
     a.parity = (uint32_t)(step & 1);
     a.t = d.t0 + (double)(bstep - d.join);
     const DevStatus *st = d.ctx[0].st;
-    const int64_t occ = st->n_occ[a.parity] > 0 ? st->n_occ[a.parity] : 1;
-    double mean = (double)st->n_alive[a.parity] / (double)occ;
-    mean = mean > 1.0 ? mean : 1.0;
-    const double g = floor(64.0 / mean);
-    a.gcells = (int32_t)(g < 1.0 ? 1.0 : (g > (double)kGroupCells ? (double)kGroupCells : g));
+    a.gcells = group_cells(group_agents, st->n_alive[a.parity], st->n_occ[a.parity]);
     return a;
 }
 
@@ -1619,7 +1645,7 @@ __device__ __forceinline__ int64_t lane_value(const int64_t (&v)[4], int r) {
 }
 
 // per-member group counts of the current step
-__device__ __forceinline__ void batch_groups(const BatchArgs &B, int64_t (&cnt)[4]) {
+__device__ __forceinline__ void batch_groups(const BatchArgs &B, int64_t bstep, int64_t (&cnt)[4]) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -1628,7 +1654,7 @@ __device__ __forceinline__ void batch_groups(const BatchArgs &B, int64_t (&cnt)[
         if (r < B.npops) {
             const DevStatus *st = B.pops[r].ctx[0].st;
             if (!st->error) {
-                const StepArgs a = pop_step_args(B.pops[r], B.bstep);
+                const StepArgs a = pop_step_args(B.pops[r], bstep, B.group_agents);
                 const int64_t nocc = st->n_occ[a.parity];
                 cnt[k] = (nocc + a.gcells - 1) / a.gcells;
             }
@@ -1639,13 +1665,14 @@ __device__ __forceinline__ void batch_groups(const BatchArgs &B, int64_t (&cnt)[
 __global__ void __launch_bounds__(kCellBlock) k_batch_act(BatchArgs B) {
     __shared__ ActLds lds;
     const int lane = threadIdx.x;
+    const int64_t bstep = batch_step_in(B, BC_ACT, BC_SCAN);
     // every member's look-back words start at zero
     for (int32_t i = blockIdx.x * kCellBlock + lane; i < B.total_tiles; i += gridDim.x * kCellBlock) {
         B.tiles[i] = 0;
         B.tiles2[i] = 0;
     }
     int64_t cnt[4], base[4];
-    batch_groups(B, cnt);
+    batch_groups(B, bstep, cnt);
     const int64_t total = pop_prefix(cnt, base);
     // a contiguous run of groups per block, so a block stays with one member
     // for long stretches and adds its counts to that member once per stretch
@@ -1675,7 +1702,7 @@ __global__ void __launch_bounds__(kCellBlock) k_batch_act(BatchArgs B) {
             rbase = lane_value(base, r);
             rend = rbase + lane_value(cnt, r);
             const DevStatus *st = B.pops[r].ctx[0].st;
-            sa = pop_step_args(B.pops[r], B.bstep);
+            sa = pop_step_args(B.pops[r], bstep, B.group_agents);
             x = &B.pops[r].ctx[sa.parity];
             pg = &B.pops[r].prog;
             nocc = st->n_occ[sa.parity];
@@ -1721,6 +1748,7 @@ __global__ void __launch_bounds__(kCellBlock) k_batch_act(BatchArgs B) {
 __global__ void __launch_bounds__(kScanBlock) k_batch_scan(BatchArgs B) {
     __shared__ ScanLds lds;
     const int lane = threadIdx.x & 63;
+    const int64_t bstep = batch_step_in(B, BC_SCAN, BC_SCATTER);
     // static tile prefix; every wave of the block computes it the same way
     int64_t tb[4];
 #pragma unroll
@@ -1733,7 +1761,7 @@ __global__ void __launch_bounds__(kScanBlock) k_batch_scan(BatchArgs B) {
         const int64_t lt = t - lane_value(tb, r);
         const DevStatus *st = B.pops[r].ctx[0].st;
         if (st->error) continue;
-        const StepArgs sa = pop_step_args(B.pops[r], B.bstep);
+        const StepArgs sa = pop_step_args(B.pops[r], bstep, B.group_agents);
         scan_tile(B.pops[r].ctx[sa.parity], sa, (uint32_t)lt, lds);
     }
 }
@@ -1741,6 +1769,7 @@ __global__ void __launch_bounds__(kScanBlock) k_batch_scan(BatchArgs B) {
 __global__ void __launch_bounds__(kCellBlock) k_batch_scatter(BatchArgs B) {
     __shared__ ScatLds lds;
     const int lane = threadIdx.x;
+    const int64_t bstep = batch_step_in(B, BC_SCATTER, BC_GENOME);
     if (blockIdx.x == 0) {
         // close every member's step and copy its status into host memory
         for (int r = 0; r < B.npops; r++) {
@@ -1766,7 +1795,7 @@ __global__ void __launch_bounds__(kCellBlock) k_batch_scatter(BatchArgs B) {
     }
     // the same group numbering as k_batch_act
     int64_t cnt[4], base[4];
-    batch_groups(B, cnt);
+    batch_groups(B, bstep, cnt);
     const int64_t total = pop_prefix(cnt, base);
     const int64_t G = gridDim.x - 1;
     const int64_t per = (total + G - 1) / G;
@@ -1782,7 +1811,7 @@ __global__ void __launch_bounds__(kCellBlock) k_batch_scatter(BatchArgs B) {
             rbase = lane_value(base, r);
             rend = rbase + lane_value(cnt, r);
             const DevStatus *st = B.pops[r].ctx[0].st;
-            sa = pop_step_args(B.pops[r], B.bstep);
+            sa = pop_step_args(B.pops[r], bstep, B.group_agents);
             x = &B.pops[r].ctx[sa.parity];
             nocc = st->n_occ[sa.parity];
             max_id = st->max_id[sa.parity];
@@ -1813,6 +1842,7 @@ __global__ void __launch_bounds__(kCellBlock) k_batch_scatter(BatchArgs B) {
 // births of every member with Genetics, one lane per genome word
 __global__ void __launch_bounds__(kWave) k_batch_genome(BatchArgs B) {
     const int lane = threadIdx.x;
+    const int64_t bstep = batch_step_in(B, BC_GENOME, -1);
     int64_t cnt[4], base[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -1831,7 +1861,7 @@ __global__ void __launch_bounds__(kWave) k_batch_genome(BatchArgs B) {
         const int r = pop_of(base, B.npops, g);
         const int64_t gl = g - lane_value(base, r);
         const DevStatus *st = B.pops[r].ctx[0].st;
-        const StepArgs sa = pop_step_args(B.pops[r], B.bstep);
+        const StepArgs sa = pop_step_args(B.pops[r], bstep, B.group_agents);
         const int par = (int)sa.parity;
         const Ctx &x = B.pops[r].ctx[par];
         const Prog &pg = B.pops[r].prog;

@@ -86,6 +86,9 @@ struct qhg_pop {
     int device = 0;
     int num_cu = 256;
     int act_grid = 1, scatter_grid = 1, scan_grid = 1, ntiles = 1;
+    // launch shapes (results do not depend on them)
+    double group_agents = QHG_GROUP_AGENTS;
+    int blocks_per_cu = QHG_CELL_GRID_PER_CU;
 
     // agents: two cell-sorted buffers of cap slots, segment offsets per buffer
     AgentBuf buf[2];
@@ -148,6 +151,8 @@ struct qhg_pop {
     qhg_batch *batch = nullptr;
     hipStream_t own_stream = nullptr;
     hipEvent_t own_ev[kRing] = {};
+    int64_t ring_bstep[kRing];  // batch step each ring entry's event closes
+    int64_t *bdone = nullptr;   // newest batch step known complete
     int64_t gen = 0;          // bumped whenever a device buffer moves
 
     Ctx ctx() const { return ctx_for(cur()); }
@@ -210,14 +215,98 @@ struct qhg_pop {
         // cells per wave group from the newest known mean occupancy: about
         // one wave of agents per group (results do not depend on it)
         const int kb = (int)(known_step & 1);
-        const double occ = (double)std::max<int64_t>(1, known.n_occ[kb]);
-        const double mean = std::max(1.0, (double)known.n_alive[kb] / occ);
-        a.gcells = (int32_t)std::max(1.0, std::min((double)kGroupCells, std::floor(QHG_GROUP_AGENTS / mean)));
+        a.gcells = group_cells(group_agents, known.n_alive[kb], known.n_occ[kb]);
         return a;
     }
 };
 
 namespace {
+
+constexpr int kMaxBlocksPerCu = 32;
+
+void set_shape(qhg_pop *p) {
+    p->act_grid = (int)std::max<int64_t>(1, std::min<int64_t>(p->ncells, (int64_t)p->num_cu * p->blocks_per_cu));
+    p->scatter_grid = p->act_grid;
+}
+
+// device memory held by a population: agent buffers and slot scratch (cap
+// slots), genome pools, per-cell tables, fixed-size scratch
+int64_t device_bytes(const qhg_pop *p) {
+    const int64_t g = p->gwords ? 1 : 0;
+    const int64_t per_slot = 2 * (46 + 4 * g) + 38 + 12 * g;
+    int64_t b = p->cap * per_slot;
+    for (int k = 0; k < 2; k++)
+        if (p->d_gpool[k]) b += p->gcap * p->gwords * 8;
+    const int64_t nc = p->ncells, cn = p->conn;
+    int64_t per_cell = cn * 4 + 1 + p->in_w * 4 + cn * 4 + 8 + (cn + 1) * 8 + 8 + 2 * 4 + 4 + 4 + p->in_w * 4 + 4 + 4 +
+                       cn * 4 + 2 * (int64_t)sizeof(OccCell);
+    for (int k = 0; k < 7; k++)
+        if (p->d_env[k]) per_cell += 8;
+    if (p->d_ice) per_cell += 1;
+    if (p->d_coastal) per_cell += 1;
+    b += nc * per_cell + 3 * 4;
+    b += 2 * (int64_t)p->ntiles * 8 + (int64_t)p->num_cu * kMaxBlocksPerCu * kParts * 4 + (int64_t)sizeof(DevStatus);
+    return b;
+}
+
+int check_tuning(const qhg_tuning *t) {
+    if (!(t->group_agents >= 1.0 && t->group_agents <= 4096.0))
+        return fail(QHG_ERR_ARG, "group_agents must be in [1, 4096]");
+    if (t->blocks_per_cu < 1 || t->blocks_per_cu > kMaxBlocksPerCu)
+        return fail(QHG_ERR_ARG, "blocks_per_cu must be in [1, 32]");
+    return 0;
+}
+
+struct Timer {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int init() {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        return 0;
+    }
+    ~Timer() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
+// Launch-shape search on the live workload: every trial runs trial_steps
+// real steps (results do not depend on the shape) and is scored in agent-
+// steps per ms of device time.  Coordinate search: the group target at the
+// starting blocks per CU, then blocks per CU at the best group target.
+template <typename Apply, typename Trial>
+int tune_search(const qhg_tuning &start, int32_t trial_steps, Apply apply, Trial trial, qhg_tuning *best,
+                int32_t *taken) {
+    static const double groups[] = {32.0, 48.0, 64.0, 96.0, 128.0, 192.0};
+    static const int blocks[] = {4, 8, 16, 32};
+    qhg_tuning cur = start, win = start;
+    double win_rate = -1.0;
+    *taken = 0;
+    auto score = [&](const qhg_tuning &t) -> int {
+        int rc = apply(t);
+        if (rc) return rc;
+        double r = 0.0;
+        rc = trial(&r);
+        if (rc) return rc;
+        *taken += trial_steps;
+        if (r > win_rate) { win_rate = r; win = t; }
+        return 0;
+    };
+    for (double g : groups) {
+        cur.group_agents = g;
+        int rc = score(cur);
+        if (rc) return rc;
+    }
+    cur = win;
+    for (int bpc : blocks) {
+        if (bpc == start.blocks_per_cu) continue;
+        cur.blocks_per_cu = bpc;
+        int rc = score(cur);
+        if (rc) return rc;
+    }
+    *best = win;
+    return apply(win);
+}
 
 int grid_for(int64_t n, int max_blocks) {
     int64_t b = (n + kBlock - 1) / kBlock;
@@ -502,9 +591,11 @@ int grow_agents(qhg_pop *p, int64_t new_cap, int64_t n) {
 void poll_ring(qhg_pop *p) {
     // newest completed snapshot wins; its own step field says which step it
     // precedes (a slot may already hold a later step's snapshot)
+    // (a batch member asks its batch, which polls its events once per step)
     for (int k = 0; k < kRing; k++) {
         if (p->ring_step[k] < 0) continue;
-        if (hipEventQuery(p->ring_ev[k]) != hipSuccess) continue;
+        const bool done = p->bdone ? p->ring_bstep[k] <= *p->bdone : hipEventQuery(p->ring_ev[k]) == hipSuccess;
+        if (!done) continue;
         DevStatus snap;
         std::memcpy(&snap, (const void *)&p->h_ring[k], sizeof(snap));
         if (snap.step > p->known_step) {
@@ -564,6 +655,7 @@ int ensure_capacity(qhg_pop *p) {
             if (rc) return rc;
         } else {
             HIPCHK(hipEventSynchronize(p->ring_ev[k]));
+            if (p->bdone) *p->bdone = std::max(*p->bdone, p->ring_bstep[k]);
             poll_ring(p);
         }
     }
@@ -733,8 +825,7 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
             per_cu = 1;
         p->scan_grid = (int)std::min<int64_t>(p->ntiles, (int64_t)per_cu * p->num_cu);
     }
-    p->act_grid = (int)std::min<int64_t>(p->ncells, (int64_t)p->num_cu * QHG_CELL_GRID_PER_CU);
-    p->scatter_grid = p->act_grid;
+    set_shape(p);
     if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup_fail(fail(QHG_ERR_HIP, "stream"));
     const size_t nc = (size_t)p->ncells;
@@ -764,7 +855,7 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
         hipMalloc(&p->d_tiles2, (size_t)p->ntiles * 8) != hipSuccess ||
         hipMalloc(&p->d_occ[0], nc * sizeof(OccCell)) != hipSuccess ||
         hipMalloc(&p->d_occ[1], nc * sizeof(OccCell)) != hipSuccess ||
-        hipMalloc(&p->d_wave_part, (size_t)p->act_grid * kParts * 4) != hipSuccess ||
+        hipMalloc(&p->d_wave_part, (size_t)p->num_cu * kMaxBlocksPerCu * kParts * 4) != hipSuccess ||
         hipMalloc(&p->d_st, sizeof(DevStatus)) != hipSuccess)
         return cleanup_fail(fail(QHG_ERR_NOMEM, "device allocation failed"));
     (void)hipMemsetAsync(p->d_K, 0, nc * 8, p->stream);
@@ -1210,6 +1301,63 @@ const char *qhg_gpu_kernel_name(int32_t index) {
     return kKernelNames[index];
 }
 
+int qhg_gpu_device_bytes(qhg_pop *p, int64_t *bytes) {
+    if (!p || !bytes) return fail(QHG_ERR_ARG, "null argument");
+    *bytes = device_bytes(p);
+    return 0;
+}
+
+int qhg_gpu_set_tuning(qhg_pop *p, const qhg_tuning *t) {
+    if (!p || !t) return fail(QHG_ERR_ARG, "null argument");
+    int rc = check_tuning(t);
+    if (rc) return rc;
+    if (p->acted || p->initialized) return fail(QHG_ERR_STATE, "population is inside a step");
+    p->group_agents = t->group_agents;
+    p->blocks_per_cu = t->blocks_per_cu;
+    set_shape(p);
+    return 0;
+}
+
+int qhg_gpu_get_tuning(qhg_pop *p, qhg_tuning *t) {
+    if (!p || !t) return fail(QHG_ERR_ARG, "null argument");
+    *t = qhg_tuning{p->group_agents, p->blocks_per_cu, 0};
+    return 0;
+}
+
+int qhg_gpu_autotune(qhg_pop *p, int32_t trial_steps, qhg_tuning *chosen, int32_t *steps_taken) {
+    if (!p) return fail(QHG_ERR_ARG, "null handle");
+    if (p->batch) return fail(QHG_ERR_STATE, "population is stepped by its batch");
+    if (trial_steps < 1) return fail(QHG_ERR_ARG, "trial_steps < 1");
+    int rc = qhg_gpu_set_timing(p, 0);
+    if (rc) return rc;
+    Timer tm;
+    rc = tm.init();
+    if (rc) return rc;
+    auto trial = [&](double *rate) -> int {
+        int r = refresh_known(p);
+        if (r) return r;
+        const int64_t a0 = p->known.agent_steps;
+        HIPCHK(hipEventRecord(tm.e0, p->stream));
+        r = qhg_gpu_run(p, trial_steps);
+        if (r) return r;
+        HIPCHK(hipEventRecord(tm.e1, p->stream));
+        r = refresh_known(p);
+        if (r) return r;
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, tm.e0, tm.e1));
+        *rate = ms > 0 ? (double)(p->known.agent_steps - a0) / ms : 0.0;
+        return 0;
+    };
+    auto apply = [&](const qhg_tuning &t) { return qhg_gpu_set_tuning(p, &t); };
+    qhg_tuning t0{p->group_agents, p->blocks_per_cu, 0}, best{};
+    int32_t taken = 0;
+    rc = tune_search(t0, trial_steps, apply, trial, &best, &taken);
+    if (steps_taken) *steps_taken = taken;
+    if (rc) return rc;
+    if (chosen) *chosen = best;
+    return 0;
+}
+
 }  // extern "C"
 
 // ===========================================================================
@@ -1221,7 +1369,7 @@ struct qhg_batch {
     hipStream_t stream = nullptr;
     int device = 0;
     int num_cu = 256;
-    int act_grid = 1, scan_grid = 1, scatter_grid = 1, genome_grid = 1;
+    int scan_grid = 1, genome_grid = 1;
     std::vector<qhg_pop *> pops;
     std::vector<int64_t> gens, step0, join;
     std::vector<double> t0;
@@ -1236,6 +1384,17 @@ struct qhg_batch {
     bool any_genetics = false;
     hipEvent_t ev[kRing] = {};
     int64_t ev_step[kRing];
+    int64_t done_bstep = 0;        // every batch step below this one is complete
+    int64_t *d_bctr = nullptr;     // the kernels' copies of the batch step index
+    // launch shapes (results do not depend on them)
+    double group_agents = QHG_GROUP_AGENTS;
+    int blocks_per_cu = QHG_CELL_GRID_PER_CU;
+    // one step captured as a graph, replayed while the launch set is unchanged
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int64_t graph_key[4] = {-1, -1, -1, -1};
+    int64_t table_version = 0;
     // timing
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
@@ -1263,6 +1422,79 @@ int blaunch(qhg_batch *b, int kid, void (*kern)(KArgs...), int grid, int block, 
         hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, b->stream, args...);
     }
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// newest completed batch step from the batch's own events (in stream order,
+// so the newest completed event covers every older one)
+void batch_poll(qhg_batch *b) {
+    int64_t best = b->done_bstep;
+    for (int k = 0; k < kRing; k++)
+        if (b->ev_step[k] > best && hipEventQuery(b->ev[k]) == hipSuccess) best = b->ev_step[k];
+    b->done_bstep = best;
+}
+
+BatchArgs batch_args(const qhg_batch *b) {
+    BatchArgs A{};
+    A.pops = b->d_desc;
+    A.npops = (int32_t)b->pops.size();
+    A.total_tiles = b->total_tiles;
+    A.tile_base = b->d_tile_base;
+    A.tiles = b->d_tiles;
+    A.tiles2 = b->d_tiles2;
+    A.bctr = b->d_bctr;
+    A.group_agents = b->group_agents;
+    A.genome = b->any_genetics ? 1 : 0;
+    return A;
+}
+
+int batch_act_grid(const qhg_batch *b) { return b->num_cu * b->blocks_per_cu; }
+
+// the kernels of one batch step, in order
+int batch_launch_step(qhg_batch *b) {
+    const BatchArgs A = batch_args(b);
+    const int ag = batch_act_grid(b);
+    int rc = blaunch(b, BK_ACT, k_batch_act, ag, kCellBlock, A);
+    if (rc) return rc;
+    rc = blaunch(b, BK_SCAN, k_batch_scan, std::max(1, std::min(b->scan_grid, b->total_tiles)), kScanBlock, A);
+    if (rc) return rc;
+    // one more block than the groups use: the members' bookkeeping
+    rc = blaunch(b, BK_SCATTER, k_batch_scatter, ag + 1, kCellBlock, A);
+    if (rc) return rc;
+    if (b->any_genetics) return blaunch(b, BK_GENOME, k_batch_genome, b->genome_grid, kWave, A);
+    return 0;
+}
+
+void batch_drop_graph(qhg_batch *b) {
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+    for (auto &k : b->graph_key) k = -1;
+}
+
+// one batch step: eagerly (and timed, when timing is on) or as a replay of
+// the captured step, recaptured whenever the launch set changes
+int batch_step(qhg_batch *b) {
+    if (!b->use_graph || b->timing) return batch_launch_step(b);
+    const int64_t key[4] = {b->table_version, (int64_t)(b->group_agents * 1024.0), b->blocks_per_cu,
+                            b->any_genetics ? 1 : 0};
+    if (!b->graph_exec || !std::equal(key, key + 4, b->graph_key)) {
+        batch_drop_graph(b);
+        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = batch_launch_step(b);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(QHG_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        b->graph = g;
+        HIPCHK(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+        std::copy(key, key + 4, b->graph_key);
+    }
+    HIPCHK(hipGraphLaunch(b->graph_exec, b->stream));
     return 0;
 }
 
@@ -1327,6 +1559,7 @@ int batch_upload_table(qhg_batch *b) {
     HIPCHK(hipMemcpy(b->d_tile_base, tile_base.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
     b->total_tiles = tile_base[n];
     b->dirty = false;
+    b->table_version++;
     return 0;
 }
 
@@ -1354,12 +1587,15 @@ int qhg_batch_create(qhg_batch **out) {
         }
         b->ev_step[k] = -1;
     }
+    if (hipMalloc(&b->d_bctr, BC_COUNT * sizeof(int64_t)) != hipSuccess ||
+        hipMemset(b->d_bctr, 0, BC_COUNT * sizeof(int64_t)) != hipSuccess) {
+        qhg_batch_destroy(b);
+        return fail(QHG_ERR_NOMEM, "batch counters");
+    }
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_scan, kScanBlock, 0) != hipSuccess || per_cu < 1)
         per_cu = 1;
     b->scan_grid = per_cu * b->num_cu;
-    b->act_grid = b->num_cu * QHG_CELL_GRID_PER_CU;
-    b->scatter_grid = b->act_grid;
     b->genome_grid = b->num_cu * 16;
     *out = b;
     return 0;
@@ -1369,7 +1605,8 @@ int qhg_batch_destroy(qhg_batch *b) {
     if (!b) return 0;
     while (!b->pops.empty()) (void)qhg_batch_remove(b, b->pops.back());
     if (b->stream) (void)hipStreamSynchronize(b->stream);
-    void *ptrs[] = {b->d_desc, b->d_tile_base, b->d_tiles, b->d_tiles2};
+    batch_drop_graph(b);
+    void *ptrs[] = {b->d_desc, b->d_tile_base, b->d_tiles, b->d_tiles2, b->d_bctr};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     for (int k = 0; k < kRing; k++)
@@ -1405,6 +1642,7 @@ int qhg_batch_add(qhg_batch *b, qhg_pop *p) {
         p->ring_step[k] = -1;
     }
     p->batch = b;
+    p->bdone = &b->done_bstep;
     b->pops.push_back(p);
     b->gens.push_back(p->gen);
     b->step0.push_back(p->steps_enqueued);
@@ -1427,6 +1665,7 @@ int qhg_batch_remove(qhg_batch *b, qhg_pop *p) {
         p->ring_step[k] = -1;
     }
     p->batch = nullptr;
+    p->bdone = nullptr;
     int rc = refresh_known(p);
     b->pops.erase(b->pops.begin() + i);
     b->gens.erase(b->gens.begin() + i);
@@ -1447,6 +1686,7 @@ int qhg_batch_run(qhg_batch *b, int32_t nsteps) {
     if (!b) return fail(QHG_ERR_ARG, "null handle");
     for (int32_t s = 0; s < nsteps; s++) {
         if (b->pops.empty()) return 0;
+        batch_poll(b);
         for (qhg_pop *p : b->pops) {
             int rc = ensure_capacity(p);
             if (rc) return rc;
@@ -1457,24 +1697,8 @@ int qhg_batch_run(qhg_batch *b, int32_t nsteps) {
             int rc = batch_upload_table(b);
             if (rc) return rc;
         }
-        BatchArgs A{};
-        A.pops = b->d_desc;
-        A.npops = (int32_t)b->pops.size();
-        A.total_tiles = b->total_tiles;
-        A.tile_base = b->d_tile_base;
-        A.tiles = b->d_tiles;
-        A.tiles2 = b->d_tiles2;
-        A.bstep = b->bstep;
-        int rc = blaunch(b, BK_ACT, k_batch_act, b->act_grid, kCellBlock, A);
+        int rc = batch_step(b);
         if (rc) return rc;
-        rc = blaunch(b, BK_SCAN, k_batch_scan, std::max(1, std::min(b->scan_grid, b->total_tiles)), kScanBlock, A);
-        if (rc) return rc;
-        rc = blaunch(b, BK_SCATTER, k_batch_scatter, b->scatter_grid + 1, kCellBlock, A);
-        if (rc) return rc;
-        if (b->any_genetics) {
-            rc = blaunch(b, BK_GENOME, k_batch_genome, b->genome_grid, kWave, A);
-            if (rc) return rc;
-        }
         b->bstep++;
         for (qhg_pop *p : b->pops) {
             p->steps_enqueued++;
@@ -1484,6 +1708,7 @@ int qhg_batch_run(qhg_batch *b, int32_t nsteps) {
         const int slot = (int)(b->bstep % kRing);
         if (b->ev_step[slot] >= 0) {
             HIPCHK(hipEventSynchronize(b->ev[slot]));
+            b->done_bstep = std::max(b->done_bstep, b->ev_step[slot]);
             for (qhg_pop *p : b->pops) poll_ring(p);
         }
         HIPCHK(hipEventRecord(b->ev[slot], b->stream));
@@ -1492,6 +1717,7 @@ int qhg_batch_run(qhg_batch *b, int32_t nsteps) {
             const int k = (int)(p->steps_enqueued % kRing);
             p->ring_ev[k] = b->ev[slot];
             p->ring_step[k] = p->steps_enqueued;
+            p->ring_bstep[k] = b->bstep;
         }
         if (b->timing && b->ev_used.size() > 4096) {
             rc = batch_collect_timing(b);
@@ -1504,6 +1730,72 @@ int qhg_batch_run(qhg_batch *b, int32_t nsteps) {
 int qhg_batch_sync(qhg_batch *b) {
     if (!b) return fail(QHG_ERR_ARG, "null handle");
     HIPCHK(hipStreamSynchronize(b->stream));
+    b->done_bstep = b->bstep;
+    for (qhg_pop *p : b->pops) poll_ring(p);
+    return 0;
+}
+
+int qhg_batch_set_graph(qhg_batch *b, int32_t enable) {
+    if (!b) return fail(QHG_ERR_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->use_graph = enable != 0;
+    if (!b->use_graph) batch_drop_graph(b);
+    return 0;
+}
+
+int qhg_batch_set_tuning(qhg_batch *b, const qhg_tuning *t) {
+    if (!b || !t) return fail(QHG_ERR_ARG, "null argument");
+    int rc = check_tuning(t);
+    if (rc) return rc;
+    b->group_agents = t->group_agents;
+    b->blocks_per_cu = t->blocks_per_cu;
+    return 0;
+}
+
+int qhg_batch_get_tuning(qhg_batch *b, qhg_tuning *t) {
+    if (!b || !t) return fail(QHG_ERR_ARG, "null argument");
+    *t = qhg_tuning{b->group_agents, b->blocks_per_cu, 0};
+    return 0;
+}
+
+int qhg_batch_autotune(qhg_batch *b, int32_t trial_steps, qhg_tuning *chosen, int32_t *steps_taken) {
+    if (!b) return fail(QHG_ERR_ARG, "null handle");
+    if (trial_steps < 1) return fail(QHG_ERR_ARG, "trial_steps < 1");
+    int rc = qhg_batch_set_timing(b, 0);
+    if (rc) return rc;
+    Timer tm;
+    rc = tm.init();
+    if (rc) return rc;
+    auto agent_steps = [&](int64_t *sum) -> int {
+        int r = qhg_batch_sync(b);
+        if (r) return r;
+        int64_t a = 0;
+        for (qhg_pop *p : b->pops) a += p->known.agent_steps;
+        *sum = a;
+        return 0;
+    };
+    auto trial = [&](double *rate) -> int {
+        int64_t a0 = 0, a1 = 0;
+        int r = agent_steps(&a0);
+        if (r) return r;
+        HIPCHK(hipEventRecord(tm.e0, b->stream));
+        r = qhg_batch_run(b, trial_steps);
+        if (r) return r;
+        HIPCHK(hipEventRecord(tm.e1, b->stream));
+        r = agent_steps(&a1);
+        if (r) return r;
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, tm.e0, tm.e1));
+        *rate = ms > 0 ? (double)(a1 - a0) / ms : 0.0;
+        return 0;
+    };
+    auto apply = [&](const qhg_tuning &t) { return qhg_batch_set_tuning(b, &t); };
+    qhg_tuning t0{b->group_agents, b->blocks_per_cu, 0}, best{};
+    int32_t taken = 0;
+    rc = tune_search(t0, trial_steps, apply, trial, &best, &taken);
+    if (steps_taken) *steps_taken = taken;
+    if (rc) return rc;
+    if (chosen) *chosen = best;
     return 0;
 }
 

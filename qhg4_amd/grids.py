@@ -160,3 +160,25 @@ def bfs_patch(grid, start, ncells, allowed):
                 if len(out) >= ncells:
                     break
     return np.array(out, dtype=np.int32)
+
+
+def bfs_levels_patch(grid, start, ncells, allowed):
+    """``ncells`` allowed cells nearest ``start`` by hop count: whole BFS
+    levels, the last one cut in cell-index order (vectorised; for large
+    patches)."""
+    nbr = grid["neighbors"]
+    seen = np.zeros(grid["num_cells"], dtype=bool)
+    seen[start] = True
+    out = [np.array([start], dtype=np.int32)]
+    total = 1
+    frontier = out[0]
+    while total < ncells and frontier.size:
+        cand = nbr[frontier].ravel()
+        cand = np.unique(cand[cand >= 0])
+        cand = cand[~seen[cand] & allowed[cand]]
+        seen[cand] = True
+        take = cand[:ncells - total].astype(np.int32)
+        out.append(take)
+        total += take.size
+        frontier = cand
+    return np.concatenate(out)

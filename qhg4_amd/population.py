@@ -116,6 +116,29 @@ class GpuPopulation:
     def set_timing(self, on):
         check(lib().qhg_gpu_set_timing(self.h, 1 if on else 0), "qhg_gpu_set_timing")
 
+    def device_bytes(self):
+        n = C.c_int64()
+        check(lib().qhg_gpu_device_bytes(self.h, C.byref(n)), "qhg_gpu_device_bytes")
+        return n.value
+
+    # -- launch shapes (results never depend on them) -----------------------
+    def set_tuning(self, group_agents=64.0, blocks_per_cu=16):
+        t = abi.Tuning(float(group_agents), int(blocks_per_cu), 0)
+        check(lib().qhg_gpu_set_tuning(self.h, C.byref(t)), "qhg_gpu_set_tuning")
+
+    def get_tuning(self):
+        t = abi.Tuning()
+        check(lib().qhg_gpu_get_tuning(self.h, C.byref(t)), "qhg_gpu_get_tuning")
+        return t.as_dict()
+
+    def autotune(self, trial_steps=4):
+        """Pick launch shapes on this population's own steps; returns the
+        chosen shape and the number of steps the population advanced."""
+        t = abi.Tuning()
+        n = C.c_int32()
+        check(lib().qhg_gpu_autotune(self.h, int(trial_steps), C.byref(t), C.byref(n)), "qhg_gpu_autotune")
+        return t.as_dict(), n.value
+
     def kernel_times(self):
         n = 16
         avg = (C.c_double * n)()
@@ -158,6 +181,26 @@ class GpuBatch:
 
     def set_timing(self, on):
         check(lib().qhg_batch_set_timing(self.h, 1 if on else 0), "qhg_batch_set_timing")
+
+    def set_graph(self, on):
+        check(lib().qhg_batch_set_graph(self.h, 1 if on else 0), "qhg_batch_set_graph")
+
+    def set_tuning(self, group_agents=64.0, blocks_per_cu=16):
+        t = abi.Tuning(float(group_agents), int(blocks_per_cu), 0)
+        check(lib().qhg_batch_set_tuning(self.h, C.byref(t)), "qhg_batch_set_tuning")
+
+    def get_tuning(self):
+        t = abi.Tuning()
+        check(lib().qhg_batch_get_tuning(self.h, C.byref(t)), "qhg_batch_get_tuning")
+        return t.as_dict()
+
+    def autotune(self, trial_steps=2):
+        """Pick launch shapes on the members' own steps; returns the chosen
+        shape and the number of steps every member advanced."""
+        t = abi.Tuning()
+        n = C.c_int32()
+        check(lib().qhg_batch_autotune(self.h, int(trial_steps), C.byref(t), C.byref(n)), "qhg_batch_autotune")
+        return t.as_dict(), n.value
 
     def kernel_times(self):
         n = 8

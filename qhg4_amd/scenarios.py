@@ -9,6 +9,9 @@ C2  OoA-style model on a 40,962-cell icosahedral grid (OoAPop action set,
     multiplicative mode, no OldAgeDeath).
 C3  C2 plus 2-bit diploid genetics (OoANavGen2bitPop,
     QHG4/populations/OoANavGen2bitPop.cpp:75-166, 226-241).
+C4  an ensemble of C2-style replicas of uneven grid and population size with
+    per-replica parameters (c4_replica).
+C5  a stream of C2-style scenario requests (c5_requests / c5_scenario).
 
 All inputs are synthetic and seeded; the same dict feeds the MI355X path and
 the oracle.
@@ -16,7 +19,7 @@ the oracle.
 import numpy as np
 
 from . import abi
-from .grids import rect_grid, ico_grid, value_noise, bfs_patch
+from .grids import rect_grid, ico_grid, value_noise, bfs_patch, bfs_levels_patch
 
 ALT_SIGMOID_THRESHOLD = 2500.0
 ALT_SIGMOID_WIDTH = 400.0
@@ -173,6 +176,102 @@ def c3(seed=42, level=8, n_agents=2_000_000):
     params = ooa_params(seed, genetics=True)
     return {"name": "C3", "grid": grid, "env": env, "params": params, "agents": agents,
             "steps": 5000}
+
+
+# -- C4: ensemble of C2-style replicas ----------------------------------------
+C4_LEVELS = (4, 5, 6, 7)          # 2,562 / 10,242 / 40,962 / 163,842 cells
+C4_N0 = (10_000, 2_000_000)       # initial agents, log-uniform
+C4_DENSITY = 25.0                 # initial agents per land cell (as C2)
+_c4_worlds = {}
+
+
+def c4_world(level, base_seed):
+    """Grid and environment of one C4 grid size, shared by every replica of
+    that size (the fields are seeded by the ensemble's base seed)."""
+    key = (level, base_seed)
+    if key not in _c4_worlds:
+        grid = ico_grid(level)
+        env = ico_env(grid, base_seed + level)
+        alt = env["altitude"]
+        land = alt > 0
+        good = np.flatnonzero(land & (alt < 1500) & (env["base_npp"] > 0.05)).astype(np.int32)
+        _c4_worlds[key] = (grid, env, land, good)
+    return _c4_worlds[key]
+
+
+def c4_draw(r, base_seed=1000):
+    """The drawn sizes and parameters of replica r (seed base_seed + r):
+    N0 log-uniform in [10k, 2M]; the grid uniformly among the C4 sizes whose
+    land holds N0 at 25 agents per cell (so no replica starts far above its
+    carrying capacity and dies out in its first step); move probability
+    U(0.05, 0.5), b0 U(0.05, 0.3), d0 U(0.01, 0.1), K scale U(20, 80)."""
+    rng = np.random.default_rng(base_seed + r)
+    n0 = int(round(np.exp(rng.uniform(np.log(C4_N0[0]), np.log(C4_N0[1])))))
+    fits = [lv for lv in C4_LEVELS if 0.8 * (10 * 4 ** lv + 2) * C4_DENSITY >= n0] or [C4_LEVELS[-1]]
+    level = fits[int(rng.integers(0, len(fits)))]
+    return {
+        "n0": n0, "level": level,
+        "wmove_prob": float(rng.uniform(0.05, 0.5)), "b0": float(rng.uniform(0.05, 0.3)),
+        "d0": float(rng.uniform(0.01, 0.1)), "k_scale": float(rng.uniform(20.0, 80.0)),
+        "start": float(rng.random()), "seed": base_seed + r,
+    }
+
+
+def c4_replica(r, base_seed=1000, steps=1000):
+    """Replica r of the C4 ensemble: the C2 model (same actions, priorities
+    and evaluators) with the drawn sizes and parameters; N0 agents in a BFS
+    patch of land cells at 25 agents per cell around a random habitable cell."""
+    sc = scenario_from_draw(c4_draw(r, base_seed), base_seed)
+    sc["name"] = "C4[%d]" % r
+    sc["steps"] = steps
+    return sc
+
+
+# -- C5: served scenario stream ------------------------------------------------
+C5_LEVELS = (4, 5, 6, 7, 8)       # 2,562 ... 655,362 cells
+C5_N0 = (10_000, 5_000_000)
+C5_HORIZON = (200, 5000)
+C5_RATE = 20.0                    # requests per second (Poisson)
+
+
+def c5_requests(n, base_seed=2000, rate=C5_RATE):
+    """n requests: arrival times of a Poisson process and each request's
+    drawn sizes, horizon and parameters (seed base_seed + i)."""
+    rng = np.random.default_rng(base_seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    out = []
+    for i in range(n):
+        r = np.random.default_rng(base_seed + 1 + i)
+        n0 = int(round(np.exp(r.uniform(np.log(C5_N0[0]), np.log(C5_N0[1])))))
+        fits = [lv for lv in C5_LEVELS if 0.8 * (10 * 4 ** lv + 2) * C4_DENSITY >= n0] or [C5_LEVELS[-1]]
+        out.append({
+            "id": i, "arrival": float(arrivals[i]), "n0": n0, "level": fits[int(r.integers(0, len(fits)))],
+            "horizon": int(r.integers(C5_HORIZON[0], C5_HORIZON[1] + 1)),
+            "wmove_prob": float(r.uniform(0.05, 0.5)), "b0": float(r.uniform(0.05, 0.3)),
+            "d0": float(r.uniform(0.01, 0.1)), "k_scale": float(r.uniform(20.0, 80.0)),
+            "start": float(r.random()), "seed": base_seed + 1 + i,
+        })
+    return out
+
+
+def scenario_from_draw(d, world_seed):
+    """A C2-model scenario from a drawn request/replica: N0 agents at 25 per
+    land cell in a BFS patch around a habitable cell of the drawn grid size."""
+    grid, env, land, good = c4_world(d["level"], world_seed)
+    start = int(good[int(d["start"] * len(good))])
+    patch = bfs_levels_patch(grid, start, int(np.ceil(d["n0"] / C4_DENSITY)), land)
+    rng = np.random.default_rng(d["seed"] + 7)
+    cells = patch[rng.integers(0, len(patch), d["n0"])]
+    agents = _agents(rng, cells, d["n0"], abi.QHG_LIFE_ALIVE, -1.0)
+    params = ooa_params(d["seed"], wmove_prob=d["wmove_prob"], b0=d["b0"], d0=d["d0"], k_scale=d["k_scale"])
+    return {"grid": grid, "env": env, "params": params, "agents": agents, "draw": d}
+
+
+def c5_scenario(req, world_seed=2000):
+    sc = scenario_from_draw(req, world_seed)
+    sc["name"] = "C5[%d]" % req["id"]
+    sc["steps"] = req["horizon"]
+    return sc
 
 
 def grid_desc(grid):

@@ -1,0 +1,59 @@
+"""The scenario server on the GPU: every request's result is the same served
+in the mix as run alone through the plain path."""
+import numpy as np
+import pytest
+
+from qhg4_amd import ensemble, scenarios
+from qhg4_amd.server import ScenarioServer
+
+pytestmark = pytest.mark.gpu
+
+WORLD = 77
+
+
+def small_requests(n, seed, horizon=(5, 60), spread=0.02):
+    rng = np.random.default_rng(seed)
+    arr = np.cumsum(rng.exponential(spread, n))
+    out = []
+    for i in range(n):
+        out.append({"id": i, "arrival": float(arr[i]), "n0": int(rng.integers(300, 20000)),
+                    "level": int(rng.choice([4, 5])), "horizon": int(rng.integers(horizon[0], horizon[1] + 1)),
+                    "wmove_prob": float(rng.uniform(0.05, 0.5)), "b0": float(rng.uniform(0.05, 0.3)),
+                    "d0": float(rng.uniform(0.01, 0.1)), "k_scale": float(rng.uniform(20, 80)),
+                    "start": float(rng.random()), "seed": seed * 100 + i})
+    return out
+
+
+def make(req):
+    return scenarios.c5_scenario(req, world_seed=WORLD)
+
+
+def alone(req):
+    _, s = ensemble.run_plain(lambda _r: make(req), 1, req["horizon"])
+    return s[0]
+
+
+def test_served_results_equal_alone_under_memory_pressure():
+    """A budget that holds about three runs: requests queue, join as others
+    finish, and still end exactly as alone."""
+    print("case: 14 requests, budget 60 MB, at most 4 members", flush=True)
+    reqs = small_requests(14, 5)
+    srv = ScenarioServer(make, budget_bytes=60e6, max_members=4, chunk=8, workers=2, tune_every_s=None)
+    res, rep = srv.serve(reqs)
+    assert len(res) == len(reqs)
+    assert rep["peak_members"] <= 4
+    by = {r["id"]: r for r in res}
+    for q in reqs:
+        assert by[q["id"]]["summary"] == alone(q), q["id"]
+
+
+def test_served_results_equal_alone_with_tuning_and_graphs():
+    """Many members at once, the autotuner running inside the serve loop."""
+    print("case: 12 requests arriving together, autotune in the loop", flush=True)
+    reqs = small_requests(12, 6, horizon=(60, 90), spread=0.0005)
+    srv = ScenarioServer(make, budget_bytes=8e9, max_members=16, chunk=5, workers=4, tune_every_s=0.0)
+    res, rep = srv.serve(reqs)
+    assert len(res) == len(reqs)
+    by = {r["id"]: r for r in res}
+    for q in reqs:
+        assert by[q["id"]]["summary"] == alone(q), q["id"]

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_words():
     L = _lib.lib()
-    assert L.qhg_gpu_abi_version() == 3
+    assert L.qhg_gpu_abi_version() == 4
     assert L.qhg_gpu_genome_words(256) == 8 == abi.genome_words(256)
     assert L.qhg_gpu_genome_words(1) == 1
 
@@ -43,6 +43,16 @@ def test_struct_sizes_match_header():
     assert C.sizeof(abi.Params) == 8 + 4 + 64 + 64 + 16 * 48 + 9 * 8 + 4 + 4 + 16 + 16 + 32 + 16 + \
         2 * 4 * 64 * 8 + 9 * 8 + 4 + 4 + 8 + 4  # + 4 bytes tail padding to 8
     assert C.sizeof(abi.Stats) == 17 * 8
+    assert C.sizeof(abi.Tuning) == 16
+
+
+def test_tuning_entry_points_validate_without_a_gpu():
+    L = _lib.lib()
+    t = abi.Tuning(64.0, 16, 0)
+    assert L.qhg_gpu_set_tuning(None, C.byref(t)) == -1
+    assert L.qhg_batch_set_tuning(None, C.byref(t)) == -1
+    assert L.qhg_batch_autotune(None, 4, None, None) == -1
+    assert L.qhg_gpu_device_bytes(None, None) == -1
 
 
 @pytest.mark.parametrize("p,n", [(1e-3, 512), (1e-2, 512), (0.2, 64)])

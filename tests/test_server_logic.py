@@ -1,0 +1,145 @@
+"""The scenario server's host logic (admission order, slot and memory
+limits, horizons), with the device stepping replaced by a stand-in that
+counts steps: no GPU needed."""
+import time
+
+import numpy as np
+import pytest
+
+from qhg4_amd import server
+
+
+class FakePop:
+    live = []
+
+    def __init__(self, sc):
+        self.sc = sc
+        self.steps = 0
+        self.bytes = server.estimate_bytes(sc) - 4096
+        FakePop.live.append(self)
+
+    @classmethod
+    def from_scenario(cls, sc, capacity=0):
+        return cls(sc)
+
+    def preLoop(self, t=0.0):
+        pass
+
+    def device_bytes(self):
+        return self.bytes
+
+    def stats(self):
+        n = len(self.sc["agents"]["id"])
+        return {k: 0 for k in server.summary.__globals__["SUMMARY_KEYS"]} | {"step": self.steps,
+                                                                              "agent_steps": n * self.steps}
+
+    def close(self):
+        if self in FakePop.live:
+            FakePop.live.remove(self)
+
+
+class FakeBatch:
+    def __init__(self):
+        self.members = []
+        self.max_seen = 0
+        self.tuned = 0
+
+    def set_graph(self, on):
+        pass
+
+    def add(self, p):
+        self.members.append(p)
+        self.max_seen = max(self.max_seen, len(self.members))
+
+    def remove(self, p):
+        self.members.remove(p)
+
+    def run(self, k):
+        for p in self.members:
+            p.steps += k
+
+    def autotune(self, trial):
+        self.tuned += 1
+        self.run(9 * trial)
+        return {"group_agents": 64.0, "blocks_per_cu": 16}, 9 * trial
+
+    def sync(self):
+        time.sleep(2e-4)     # a device wait releases the GIL to the preparing threads
+
+    def close(self):
+        pass
+
+
+class Clock:
+    """Virtual time: every call advances 1 ms."""
+
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        self.t += 1e-3
+        return self.t
+
+
+def fake_scenario(req):
+    n = req["n0"]
+    return {"grid": {"num_cells": 1000}, "agents": {"id": np.arange(n)}}
+
+
+@pytest.fixture
+def fakes(monkeypatch):
+    made = []
+
+    def batch():
+        b = FakeBatch()
+        made.append(b)
+        return b
+    monkeypatch.setattr(server, "GpuPopulation", FakePop)
+    monkeypatch.setattr(server, "GpuBatch", batch)
+    FakePop.live = []
+    return made
+
+
+def requests(n, seed=1):
+    rng = np.random.default_rng(seed)
+    arr = np.cumsum(rng.exponential(0.005, n))
+    return [{"id": i, "arrival": float(arr[i]), "n0": int(rng.integers(1000, 200000)),
+             "horizon": int(rng.integers(1, 120))} for i in range(n)]
+
+
+@pytest.mark.parametrize("max_members,budget_mb,chunk", [(4, 1e6, 16), (256, 60.0, 8), (3, 40.0, 1)])
+def test_every_request_runs_exactly_its_horizon(fakes, max_members, budget_mb, chunk):
+    reqs = requests(60)
+    srv = server.ScenarioServer(fake_scenario, budget_bytes=budget_mb * 1e6, max_members=max_members,
+                                chunk=chunk, workers=2, tune_every_s=None, clock=Clock())
+    res, rep = srv.serve(reqs)
+    assert sorted(r["id"] for r in res) == list(range(60))
+    by = {r["id"]: r for r in res}
+    for q in reqs:
+        r = by[q["id"]]
+        assert r["summary"]["step"] == q["horizon"]
+        assert r["admitted"] >= q["arrival"]
+        assert r["done"] >= r["admitted"]
+    # first come, first admitted
+    adm = [by[q["id"]]["admitted"] for q in sorted(reqs, key=lambda q: q["arrival"])]
+    assert adm == sorted(adm)
+    assert fakes[0].max_seen <= max_members
+    one = max(server.estimate_bytes(fake_scenario(q)) for q in reqs)
+    assert rep["peak_device_bytes"] <= max(budget_mb * 1e6, one)
+    assert rep["p99_s"] >= rep["p50_s"] > 0
+    assert not FakePop.live                     # every population was freed
+
+
+def test_autotune_never_overshoots_a_horizon(fakes):
+    reqs = [dict(q, arrival=0.0, horizon=1000 + 7 * q["id"]) for q in requests(12)]
+    srv = server.ScenarioServer(fake_scenario, budget_bytes=1e12, max_members=16, chunk=16, workers=2,
+                                tune_every_s=0.0, clock=Clock())
+    res, rep = srv.serve(reqs)
+    assert fakes[0].tuned >= 1 and rep["tunings"]
+    for r in res:
+        assert r["summary"]["step"] == reqs[r["id"]]["horizon"]
+
+
+def test_server_rejects_bad_member_limit():
+    with pytest.raises(ValueError):
+        server.ScenarioServer(fake_scenario, max_members=300)

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define QHG_ABI_VERSION 4
+#define QHG_ABI_VERSION 5
 
 /* error codes */
 #define QHG_OK              0
@@ -59,6 +59,8 @@ extern "C" {
 #define QHG_ACT_MULTIEVAL     8   /* QHG4/actions/MultiEvaluator.cpp:142-478  */
 #define QHG_ACT_NPPCAP        9   /* QHG4/actions/NPPCapacity.cpp:138-217     */
 #define QHG_ACT_GENETICS     10   /* QHG4/actions/Genetics.cpp:285-337        */
+#define QHG_ACT_OLDAGEDEATH  11   /* QHG4/actions/OldAgeDeath.cpp (execute)   */
+#define QHG_ACT_ATANDEATH    12   /* QHG4/actions/ATanDeath.cpp:48-90         */
 
 /* MultiEvaluator combination modes, QHG4/actions/MultiEvaluator.h:20-25 */
 #define QHG_ME_ADD_SIMPLE 0
@@ -92,6 +94,8 @@ extern "C" {
 #define QHG_ACTW_LDEATH     3   /* LinearDeath                                  */
 #define QHG_ACTW_RMOVE_GO   4   /* RandomMove: move?                            */
 #define QHG_ACTW_RMOVE_DIR  5   /* RandomMove: target slot                      */
+#define QHG_ACTW_OADEATH    6   /* OldAgeDeath: age threshold jitter            */
+#define QHG_ACTW_ATANDEATH  7   /* ATanDeath: death draw                        */
 
 typedef struct qhg_grid_desc {
     int32_t        num_cells;
@@ -150,6 +154,11 @@ typedef struct qhg_params {
     int32_t gen_genome_size;        /* loci per strand                       */
     int32_t gen_num_crossover;      /* >0 fixed count, 0 none, -1 free reco   */
     double  gen_mutation_rate;
+
+    /* OAD_max_age, OAD_uncertainty */
+    double oad_max_age, oad_uncertainty;
+    /* ATanDeath_max_age, ATanDeath_range, ATanDeath_slope */
+    double atan_max_age, atan_range, atan_slope;
 } qhg_params;
 
 /* Agent records as structure-of-arrays.  Arrays the caller does not need may
@@ -258,6 +267,36 @@ int qhg_gpu_get_tuning(qhg_pop *pop, qhg_tuning *t);
  * (trial_steps each; the population advances by *steps_taken steps), keeps
  * the one with the most agent-steps per unit of device time and returns it. */
 int qhg_gpu_autotune(qhg_pop *pop, int32_t trial_steps, qhg_tuning *chosen, int32_t *steps_taken);
+
+/* Species configuration from the reference's XML.  A species block names a
+ * population class and lists a <module> per action (nested modules for a
+ * MultiEvaluator's SingleEvaluators) with <param name value> entries, and a
+ * <priorities> map.  qhg_params_from_modules applies the reference's rules
+ * (SPopulation::readSpeciesData, QHG4/core/SPopulation.cpp:1108-1142;
+ * Prioritizer::getActionAttributes, QHG4/core/Prioritizer.cpp:440-462;
+ * Action::checkAttributes, QHG4/actions/Action.cpp:54-75): the class fixes
+ * its actions and evaluator wiring ("tut_SexualPop", "tut_EnvironCapAltPop",
+ * "OoAPop", "OoANavGen2bitPop"); every action's module must hold exactly its
+ * attribute names; actions without a priority are not scheduled; a
+ * priority for an action the class lacks, or one with no device
+ * implementation (Navigate), is an error.  Module names are those of the
+ * Prioritizer: "Verhulst", "MultiEvaluator[Alt+NPP]", "SingleEvaluator[Alt]". */
+typedef struct qhg_module_attrs {
+    const char *name;
+    int32_t num_params;
+    const char *const *keys;
+    const char *const *values;
+} qhg_module_attrs;
+int qhg_params_from_modules(const char *pop_class, const qhg_module_attrs *mods, int32_t nmods,
+                            const char *const *prio_names, const int32_t *prio_values, int32_t nprios,
+                            uint64_t seed, qhg_params *out);
+/* A numeric attribute by its reference name (e.g. "Verhulst_b0"). */
+int qhg_params_get_attribute(const qhg_params *p, const char *name, double *value);
+int qhg_params_set_attribute(qhg_params *p, const char *name, double value);
+/* PopBase::modifyAttributes (QHG4/core/PopBase.h:75; e.g.
+ * QHG4/actions/Verhulst.cpp:235-254) for attributes read every step; the
+ * capacity and evaluator attributes are refused. */
+int qhg_gpu_modify_attribute(qhg_pop *pop, const char *name, double value);
 
 /* Batched stepping.  No counterpart in the reference (QHG4 steps one
  * population per process, PopLooper.cpp:166-202); it serves callers that run

@@ -42,6 +42,7 @@ def lib():
         L.orc_num_alive.restype = C.c_int64
         L.orc_get_agents.argtypes = [C.c_void_p, P(abi.Agents)]
         L.orc_pre_loop.argtypes = [C.c_void_p]
+        L.orc_set_params.argtypes = [C.c_void_p, P(abi.Params)]
         L.orc_step.argtypes = [C.c_void_p]
         L.orc_run.argtypes = [C.c_void_p, C.c_int32]
         L.orc_stats.argtypes = [C.c_void_p, P(abi.Stats)]
@@ -107,6 +108,11 @@ class Oracle:
 
     def pre_loop(self):
         lib().orc_pre_loop(self.h)
+
+    def set_params(self, params):
+        if lib().orc_set_params(self.h, C.byref(params)) != 0:
+            raise RuntimeError("orc_set_params: the action set differs")
+        self.params = params
 
     def step(self):
         lib().orc_step(self.h)

@@ -584,6 +584,15 @@ static void multi_eval(orc *o) {
     }
 }
 
+/* PopBase::modifyAttributes restated for the tests: the parameter block is
+ * replaced between steps (same actions; per-step attributes only) */
+int orc_set_params(orc *o, const qhg_params *p) {
+    if (p->num_actions != o->P.num_actions) return -1;
+    o->P = *p;
+    sort_actions(o);
+    return 0;
+}
+
 int orc_pre_loop(orc *o) {
     if (has_action(o, QHG_ACT_NPPCAP)) npp_capacity(o);
     update_counts(o);
@@ -730,6 +739,80 @@ static void act_verhulst(orc *o) {
             double r = orc_wrandd(orc_draw(seed, step, QHG_STREAM_ACT, (uint64_t)o->id[i], QHG_ACTW_LDEATH));
             if (r < o->dc[c]) { o->life[i] = 0; deaths++; }
         }
+    }
+    o->deaths_last += deaths;
+}
+
+/* A deterministic arctangent built from IEEE +,-,*,/ only, so that a
+ * restatement and the device agree to the bit (libm's atan differs between
+ * implementations in the last place, which can flip a death decision):
+ * odd symmetry, atan(x) = pi/2 - atan(1/x) above 1, then
+ * atan(x) = atan(k/8) + atan((x - k/8) / (1 + x k/8)) with k/8 nearest x,
+ * and the Taylor series of the small remainder through its 17th power.
+ * The atan(k/8) values are the doubles nearest the true values (50-digit
+ * evaluation).  Pinned against libm atan to a few ulp by the tests. */
+double orc_atan(double x) {
+    static const double at8[9] = {0.0, 0.12435499454676144, 0.24497866312686414, 0.35877067027057225,
+                                  0.4636476090008061, 0.5585993153435624, 0.6435011087932844,
+                                  0.7188299996216245, 0.7853981633974483};
+    int neg = x < 0.0;
+    double a = neg ? -x : x;
+    int inv = a > 1.0;
+    if (inv) a = 1.0 / a;
+    int k = (int)(a * 8.0 + 0.5);
+    double c = (double)k * 0.125;
+    double t = (a - c) / (1.0 + a * c);
+    double z = t * t;
+    /* Horner from the t^17 term down */
+    double s = 1.0 / 17.0;
+    s = -1.0 / 15.0 + z * s;
+    s = 1.0 / 13.0 + z * s;
+    s = -1.0 / 11.0 + z * s;
+    s = 1.0 / 9.0 + z * s;
+    s = -1.0 / 7.0 + z * s;
+    s = 1.0 / 5.0 + z * s;
+    s = -1.0 / 3.0 + z * s;
+    s = 1.0 + z * s;
+    double r = at8[k] + t * s;
+    if (inv) r = 1.5707963267948966 - r;
+    return neg ? -r : r;
+}
+
+/* OldAgeDeath::execute (QHG4/actions/OldAgeDeath.cpp, execute): the age is
+ * set, then the agent dies when older than max_age + U(1 - u*max_age,
+ * 1 + u*max_age) */
+static void act_oldagedeath(orc *o, double t) {
+    uint32_t step = o->step;
+    uint64_t seed = o->P.seed;
+    double M = o->P.oad_max_age, u = o->P.oad_uncertainty;
+    int64_t deaths = 0;
+#pragma omp parallel for schedule(static) reduction(+:deaths)
+    for (int64_t i = 0; i < o->n; i++) {
+        if (o->life[i] == 0) continue;
+        o->age[i] = t - o->birth[i];
+        double r = orc_wrandr(1 - u * M, 1 + u * M,
+                              orc_draw(seed, step, QHG_STREAM_ACT, (uint64_t)o->id[i], QHG_ACTW_OADEATH));
+        if (o->age[i] > M + r) { o->life[i] = 0; deaths++; }
+    }
+    o->deaths_last += deaths;
+}
+
+/* ATanDeath::execute (QHG4/actions/ATanDeath.cpp:66-90) with the scale of
+ * ATanDeath::preLoop (:50-51): death with probability
+ * 1/2 + scale * atan(slope * (age - max_age)) / pi */
+static void act_atandeath(orc *o, double t) {
+    uint32_t step = o->step;
+    uint64_t seed = o->P.seed;
+    const double PI = 3.141592653589793;
+    double scale = (PI / 2 - 0.001) / atan(o->P.atan_slope * o->P.atan_range);
+    int64_t deaths = 0;
+#pragma omp parallel for schedule(static) reduction(+:deaths)
+    for (int64_t i = 0; i < o->n; i++) {
+        if (o->life[i] == 0) continue;
+        o->age[i] = t - o->birth[i];
+        double pd = 0.5 + scale * orc_atan(o->P.atan_slope * (o->age[i] - o->P.atan_max_age)) / PI;
+        double r = orc_wrandd(orc_draw(seed, step, QHG_STREAM_ACT, (uint64_t)o->id[i], QHG_ACTW_ATANDEATH));
+        if (r < pd) { o->life[i] = 0; deaths++; }
     }
     o->deaths_last += deaths;
 }
@@ -925,6 +1008,8 @@ int orc_step(orc *o) {
         case QHG_ACT_VERHULSTVARK: act_verhulst(o); break;
         case QHG_ACT_RANDOMMOVE: act_randommove(o); break;
         case QHG_ACT_WEIGHTEDMOVE: act_weightedmove(o); break;
+        case QHG_ACT_OLDAGEDEATH: act_oldagedeath(o, t); break;
+        case QHG_ACT_ATANDEATH: act_atandeath(o, t); break;
         default: break;
         }
     }

@@ -29,6 +29,7 @@ int      orc_set_agents(orc *o, const qhg_agents *a);
 int64_t  orc_num_alive(orc *o);
 int      orc_get_agents(orc *o, qhg_agents *a);
 int      orc_pre_loop(orc *o);
+int      orc_set_params(orc *o, const qhg_params *p);
 int      orc_step(orc *o);
 int      orc_run(orc *o, int32_t nsteps);
 void     orc_stats(orc *o, qhg_stats *s);
@@ -55,6 +56,7 @@ int32_t  orc_bino_table(double p, int32_t n, double eps, double *cdf, int32_t ma
 int32_t  orc_bino_getn(const double *cdf, int32_t nbins, double u);
 double   orc_betai(double a, double b, double x);
 double   orc_polyline(int32_t npts, const double *x, const double *v, double X);
+double   orc_atan(double x);
 int32_t  orc_genome_words(int32_t genome_size);
 
 #ifdef __cplusplus

@@ -21,6 +21,7 @@ EXPORTS = [
     "qhg_gpu_cell_counts", "qhg_gpu_capacities", "qhg_gpu_env_weights", "qhg_gpu_compact",
     "qhg_gpu_set_timing", "qhg_gpu_kernel_times", "qhg_gpu_kernel_name", "qhg_gpu_device_bytes",
     "qhg_gpu_set_tuning", "qhg_gpu_get_tuning", "qhg_gpu_autotune",
+    "qhg_params_from_modules", "qhg_params_get_attribute", "qhg_params_set_attribute", "qhg_gpu_modify_attribute",
     "qhg_batch_create", "qhg_batch_destroy", "qhg_batch_add", "qhg_batch_remove", "qhg_batch_size",
     "qhg_batch_run", "qhg_batch_sync", "qhg_batch_set_timing", "qhg_batch_kernel_times",
     "qhg_batch_kernel_name", "qhg_batch_set_graph", "qhg_batch_set_tuning", "qhg_batch_get_tuning",
@@ -70,6 +71,11 @@ def lib():
         "qhg_gpu_set_tuning": ([h, P(abi.Tuning)], C.c_int),
         "qhg_gpu_get_tuning": ([h, P(abi.Tuning)], C.c_int),
         "qhg_gpu_autotune": ([h, C.c_int32, P(abi.Tuning), P(C.c_int32)], C.c_int),
+        "qhg_params_from_modules": ([C.c_char_p, P(abi.ModuleAttrs), C.c_int32, P(C.c_char_p), P(C.c_int32),
+                                     C.c_int32, C.c_uint64, P(abi.Params)], C.c_int),
+        "qhg_params_get_attribute": ([P(abi.Params), C.c_char_p, P(C.c_double)], C.c_int),
+        "qhg_params_set_attribute": ([P(abi.Params), C.c_char_p, C.c_double], C.c_int),
+        "qhg_gpu_modify_attribute": ([h, C.c_char_p, C.c_double], C.c_int),
         "qhg_batch_create": ([P(h)], C.c_int),
         "qhg_batch_destroy": ([h], C.c_int),
         "qhg_batch_add": ([h, h], C.c_int),

@@ -21,6 +21,8 @@ ACT_WEIGHTEDMOVE = 7
 ACT_MULTIEVAL = 8
 ACT_NPPCAP = 9
 ACT_GENETICS = 10
+ACT_OLDAGEDEATH = 11
+ACT_ATANDEATH = 12
 
 # reference action names (Action<T>::getActionName), used for same-prio order
 ACTION_NAMES = {
@@ -34,6 +36,8 @@ ACTION_NAMES = {
     ACT_MULTIEVAL: "MultiEvaluator",
     ACT_NPPCAP: "NPPCapacity",
     ACT_GENETICS: "Genetics",
+    ACT_OLDAGEDEATH: "OldAgeDeath",
+    ACT_ATANDEATH: "ATanDeath",
 }
 
 ME_ADD_SIMPLE, ME_MUL_SIMPLE, ME_MAX_SIMPLE, ME_MIN_SIMPLE = 0, 2, 3, 5
@@ -108,6 +112,11 @@ class Params(C.Structure):
         ("gen_genome_size", C.c_int32),
         ("gen_num_crossover", C.c_int32),
         ("gen_mutation_rate", C.c_double),
+        ("oad_max_age", C.c_double),
+        ("oad_uncertainty", C.c_double),
+        ("atan_max_age", C.c_double),
+        ("atan_range", C.c_double),
+        ("atan_slope", C.c_double),
     ]
 
 
@@ -151,6 +160,16 @@ class Stats(C.Structure):
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class ModuleAttrs(C.Structure):
+    """qhg_module_attrs: one <module> of a species block and its <param>s."""
+    _fields_ = [
+        ("name", C.c_char_p),
+        ("num_params", C.c_int32),
+        ("keys", C.POINTER(C.c_char_p)),
+        ("values", C.POINTER(C.c_char_p)),
+    ]
 
 
 class Tuning(C.Structure):

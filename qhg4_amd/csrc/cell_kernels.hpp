@@ -37,6 +37,7 @@
 #include <stdint.h>
 
 #include "philox.hpp"
+#include "detmath.hpp"
 #include "../../include/qhg_gpu.h"
 
 namespace qhg {
@@ -149,6 +150,8 @@ struct StepArgs {
     int32_t pad;
 };
 
+constexpr double kPi = 3.141592653589793;   // Q_PI, QHG4/utils/qhg_consts.h:71
+
 struct Prog {
     int32_t nops;
     int32_t ops[QHG_MAX_ACTIONS];
@@ -159,6 +162,8 @@ struct Prog {
     double fert_min, fert_max, fert_inter;
     double b0, d0, theta, Kconst;
     double rmove_p, wmove_p;
+    double oad_max, oad_unc;                // OldAgeDeath
+    double atan_max, atan_slope, atan_scale;  // ATanDeath (scale from ATanDeath::preLoop)
     int32_t gen_size, gen_cross, nbino, pad;
     double bino[QHG_MAX_BINO];
 };
@@ -388,6 +393,19 @@ __device__ __forceinline__ void act_program(const Ctx &x, const Prog &p, const S
                 if (a.npend == 0) { a.pk0 = op; a.pu0 = dir; a.npend = 1; }
                 else if (a.npend == 1) { a.pk1 = op; a.pu1 = dir; a.npend = 2; }
             }
+        } else if (op == QHG_ACT_OLDAGEDEATH) {
+            // OldAgeDeath::execute: age, then death past max age + U(1-uM, 1+uM)
+            a.age = t - a.birth;
+            a.age_dirty = true;
+            const double r = wrandr(1 - p.oad_unc * p.oad_max, 1 + p.oad_unc * p.oad_max, a.u[QHG_ACTW_OADEATH]);
+            if (a.age > p.oad_max + r) { a.life = 0; deaths++; break; }
+        } else if (op == QHG_ACT_ATANDEATH) {
+            // ATanDeath::execute (ATanDeath.cpp:66-90): age, then death with
+            // probability 1/2 + scale * atan(slope * (age - max age)) / pi
+            a.age = t - a.birth;
+            a.age_dirty = true;
+            const double pd = 0.5 + p.atan_scale * det_atan(p.atan_slope * (a.age - p.atan_max)) / kPi;
+            if (wrandd(a.u[QHG_ACTW_ATANDEATH]) < pd) { a.life = 0; deaths++; break; }
         }
         // RandomPair, MultiEvaluator, NPPCapacity, Genetics: no execute
     }

@@ -35,6 +35,13 @@ int fail(int code, const std::string &msg) {
     return code;
 }
 
+}  // namespace
+
+// the error hook of the other host sources of the library (config.cpp)
+int qhg_internal_fail(int code, const std::string &msg) { return fail(code, msg); }
+
+namespace {
+
 #define HIPCHK(expr)                                                                  \
     do {                                                                              \
         hipError_t e_ = (expr);                                                       \
@@ -420,10 +427,11 @@ int build_prog(qhg_pop *p) {
     bool seen_getold = false;
     for (int q = 0; q < P.num_actions; q++) {
         const int k = P.action_kind[idx[q]];
-        if (k < QHG_ACT_GETOLD || k > QHG_ACT_GENETICS) return fail(QHG_ERR_ARG, "unknown action kind");
+        if (k < QHG_ACT_GETOLD || k > QHG_ACT_ATANDEATH) return fail(QHG_ERR_ARG, "unknown action kind");
         g.ops[q] = k;
         g.opword |= (uint64_t)k << (4 * q);
-        if (k == QHG_ACT_GETOLD) seen_getold = true;
+        // actions that set the age before Fertility reads it
+        if (k == QHG_ACT_GETOLD || k == QHG_ACT_OLDAGEDEATH || k == QHG_ACT_ATANDEATH) seen_getold = true;
         if (k == QHG_ACT_FERTILITY && !seen_getold) g.load_age = 1;
     }
     g.fert_min = P.fert_min_age;
@@ -435,6 +443,16 @@ int build_prog(qhg_pop *p) {
     g.Kconst = P.verh_K;
     g.rmove_p = P.rmove_prob;
     g.wmove_p = P.wmove_prob;
+    g.oad_max = P.oad_max_age;
+    g.oad_unc = P.oad_uncertainty;
+    g.atan_max = P.atan_max_age;
+    g.atan_slope = P.atan_slope;
+    if (has_kind(P, QHG_ACT_ATANDEATH)) {
+        // ATanDeath::preLoop (QHG4/actions/ATanDeath.cpp:50-51); EPS = 0.001 (ATanDeath.h:14)
+        const double q = std::atan(P.atan_slope * P.atan_range);
+        if (!(q != 0.0)) return fail(QHG_ERR_ARG, "ATanDeath_slope * ATanDeath_range must not be 0");
+        g.atan_scale = (kPi / 2 - 0.001) / q;
+    }
     p->has_pair = has_kind(P, QHG_ACT_RANDOMPAIR);
     p->has_me = has_kind(P, QHG_ACT_MULTIEVAL);
     p->has_cap = has_kind(P, QHG_ACT_NPPCAP);
@@ -447,6 +465,8 @@ int build_prog(qhg_pop *p) {
         case QHG_ACT_VERHULSTVARK: g.need_gender = 1; g.need_act0 = 1; break;
         case QHG_ACT_RANDOMMOVE: g.need_act1 = 1; break;
         case QHG_ACT_WEIGHTEDMOVE: g.need_act0 = 1; break;
+        case QHG_ACT_OLDAGEDEATH:
+        case QHG_ACT_ATANDEATH: g.need_birth = 1; g.need_act1 = 1; break;
         default: break;
         }
     }
@@ -1299,6 +1319,28 @@ int qhg_gpu_kernel_times(qhg_pop *p, double *avg_ms, int64_t *launches, int32_t 
 const char *qhg_gpu_kernel_name(int32_t index) {
     if (index < 0 || index >= KB_COUNT) return "";
     return kKernelNames[index];
+}
+
+int qhg_gpu_modify_attribute(qhg_pop *p, const char *name, double value) {
+    if (!p || !name) return fail(QHG_ERR_ARG, "null argument");
+    if (p->acted || p->initialized) return fail(QHG_ERR_STATE, "population is inside a step");
+    if (std::strncmp(name, "NPPCap_", 7) == 0 || std::strncmp(name, "Multi_weight_", 13) == 0)
+        return fail(QHG_ERR_ARG, std::string(name) + " feeds the capacity and weight tables, which change only "
+                                 "with environment events");
+    qhg_params P = p->P;
+    int rc = qhg_params_set_attribute(&P, name, value);
+    if (rc) return rc;
+    const qhg_params old = p->P;
+    const int64_t gw = p->gwords;
+    p->P = P;
+    rc = build_prog(p);
+    if (rc || p->gwords != gw) {
+        p->P = old;
+        (void)build_prog(p);
+        return rc ? rc : fail(QHG_ERR_ARG, "the genome size cannot change");
+    }
+    p->gen++;   // a batch re-reads the member's program
+    return 0;
 }
 
 int qhg_gpu_device_bytes(qhg_pop *p, int64_t *bytes) {

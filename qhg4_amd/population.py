@@ -13,6 +13,17 @@ from ._lib import lib, check
 from .scenarios import grid_desc, env_desc
 
 
+# reference attribute name -> qhg_params field, for the host copy of the parameters
+_FIELD_OF = {
+    "Fertility_min_age": "fert_min_age", "Fertility_max_age": "fert_max_age",
+    "Fertility_interbirth": "fert_interbirth", "Verhulst_b0": "verh_b0", "Verhulst_d0": "verh_d0",
+    "Verhulst_theta": "verh_theta", "Verhulst_K": "verh_K", "RandomMove_prob": "rmove_prob",
+    "WeightedMove_prob": "wmove_prob", "OAD_max_age": "oad_max_age", "OAD_uncertainty": "oad_uncertainty",
+    "ATanDeath_max_age": "atan_max_age", "ATanDeath_range": "atan_range", "ATanDeath_slope": "atan_slope",
+    "Genetics_mutation_rate": "gen_mutation_rate",
+}
+
+
 class GpuPopulation:
     def __init__(self, grid, env, params, agents=None, capacity=0):
         self.grid = grid
@@ -86,6 +97,13 @@ class GpuPopulation:
 
     def compactData(self):
         check(lib().qhg_gpu_compact(self.h), "qhg_gpu_compact")
+
+    def modifyAttributes(self, name, value):
+        """PopBase::modifyAttributes for an attribute read every step."""
+        check(lib().qhg_gpu_modify_attribute(self.h, name.encode(), float(value)), "qhg_gpu_modify_attribute")
+        field = _FIELD_OF.get(name)
+        if field:
+            setattr(self.params, field, float(value))
 
     def set_env(self, env):
         self.env.update(env)

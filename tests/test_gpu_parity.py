@@ -132,3 +132,75 @@ def test_empty_and_single_agent():
     gpu.preLoop()
     gpu.run(5)
     assert gpu.stats()["num_alive"] == 0 and gpu.agents()["id"].size == 0
+
+
+def with_action(sc, kind, prio, **kw):
+    p = sc["params"]
+    n = p.num_actions
+    p.action_kind[n] = kind
+    p.action_prio[n] = prio
+    p.action_name[n].value = abi.ACTION_NAMES[kind].encode()
+    p.num_actions = n + 1
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return sc
+
+
+def test_c1_with_atandeath():
+    """tut_Sexual-style model with ATanDeath at priority 3 (its slot in
+    QHG4/tutorial_data/xmldat/tut_Sexual.xml); ages spread over the death
+    ramp so both branches and deaths before pairing/moves occur."""
+    sc = tut_verhulst(scenarios.c1(seed=41, width=40, height=40, n_agents=6000))
+    sc["agents"]["birth_time"] = -np.linspace(0.0, 80.0, 6000)
+    sc["agents"]["age"] = np.linspace(0.0, 80.0, 6000)
+    with_action(sc, abi.ACT_ATANDEATH, 3, atan_max_age=45.0, atan_range=6.0, atan_slope=0.5)
+    _lockstep(sc, 40, 10)
+
+
+def test_c2_with_oldagedeath():
+    """OoAPop's action set including OldAgeDeath (QHG4/populations/OoAPop.cpp:32-88)."""
+    sc = scenarios.c2(seed=42, level=5, n_agents=20000, patch_cells=800)
+    sc["agents"]["birth_time"] = -np.linspace(0.0, 70.0, 20000)
+    sc["agents"]["age"] = np.linspace(0.0, 70.0, 20000)
+    with_action(sc, abi.ACT_OLDAGEDEATH, 3, oad_max_age=55.0, oad_uncertainty=0.05)
+    _lockstep(sc, 40, 10, use_run=True)
+
+
+def test_species_xml_model():
+    """A model configured from a species XML block (OoAPop with OldAgeDeath
+    and the additive MultiEvaluator) runs bit-exact against the oracle."""
+    import os
+    from qhg4_amd import config
+    xml = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "species_ooa.xml")
+    sc = scenarios.c2(seed=43, level=5, n_agents=20000, patch_cells=800)
+    sc["params"] = config.params_from_xml(xml, seed=43)
+    sc["name"] = "species_ooa.xml"
+    _lockstep(sc, 30, 10)
+
+
+def test_modify_attributes_mid_run():
+    """modifyAttributes between steps (RandomMove_prob, Verhulst_b0,
+    ATanDeath_slope) takes effect from the next step on, as in the oracle."""
+    import copy
+    sc = tut_verhulst(scenarios.c1(seed=44, width=40, height=40, n_agents=5000))
+    with_action(sc, abi.ACT_ATANDEATH, 10, atan_max_age=60.0, atan_range=6.0, atan_slope=1.0)
+    gpu = GpuPopulation.from_scenario(sc)
+    ora = Oracle(sc)
+    gpu.preLoop()
+    ora.pre_loop()
+    p = copy.copy(sc["params"])
+    for k, (name, field, value) in enumerate([("RandomMove_prob", "rmove_prob", 0.4),
+                                              ("Verhulst_b0", "verh_b0", 0.3),
+                                              ("ATanDeath_slope", "atan_slope", 0.2)]):
+        gpu.run(8)
+        ora.run(8)
+        gpu.modifyAttributes(name, value)
+        setattr(p, field, value)
+        ora.set_params(p)
+        assert_same_state(gpu.agents(), ora.agents(), "before change %d" % k)
+    gpu.run(8)
+    ora.run(8)
+    assert_same_state(gpu.agents(), ora.agents(), "after the changes")
+    with pytest.raises(Exception, match="environment events"):
+        gpu.modifyAttributes("NPPCap_K_max", 10.0)
+    gpu.close()

@@ -268,6 +268,16 @@ int qhg_gpu_get_tuning(qhg_pop *pop, qhg_tuning *t);
  * the one with the most agent-steps per unit of device time and returns it. */
 int qhg_gpu_autotune(qhg_pop *pop, int32_t trial_steps, qhg_tuning *chosen, int32_t *steps_taken);
 
+/* The population's clock: the index of its next step, that step's time
+ * and the highest agent id so far (new ids continue from it; the
+ * reference's IDGen state, QHG4/core/IDGen.h:28).  set_clock restores a
+ * clock read from a dump (SPopulation::restoreSpeciesDataQDF,
+ * QHG4/core/SPopulation.cpp:2138-2207) before stepping resumes; max_id
+ * never decreases below the uploaded agents' ids. */
+int qhg_gpu_set_clock(qhg_pop *pop, int64_t step, double t, int64_t max_id);
+int qhg_gpu_get_clock(qhg_pop *pop, int64_t *step, double *t, int64_t *max_id);
+int qhg_gpu_get_params(qhg_pop *pop, qhg_params *out);
+
 /* Species configuration from the reference's XML.  A species block names a
  * population class and lists a <module> per action (nested modules for a
  * MultiEvaluator's SingleEvaluators) with <param name value> entries, and a
@@ -290,6 +300,19 @@ typedef struct qhg_module_attrs {
 int qhg_params_from_modules(const char *pop_class, const qhg_module_attrs *mods, int32_t nmods,
                             const char *const *prio_names, const int32_t *prio_values, int32_t nprios,
                             uint64_t seed, qhg_params *out);
+/* The same from one flat name -> value map (the attributes of a species
+ * group of a QDF file, Prioritizer::writeActionParamsQDF): each name goes to
+ * the class action that owns it; names no action of the class owns are
+ * ignored.  PolyLine values are strings "x0 v0 x1 v1 ...". */
+int qhg_params_from_attribute_map(const char *pop_class, int32_t n, const char *const *keys,
+                                  const char *const *values, const char *const *prio_names,
+                                  const int32_t *prio_values, int32_t nprios, uint64_t seed, qhg_params *out);
+/* Attribute names: of an action kind by index (NULL past the last), and of
+ * a class's MultiEvaluator weights and SingleEvaluator PolyLines by
+ * evaluator index (NULL when none). */
+const char *qhg_params_attribute_name(int32_t kind, int32_t index);
+const char *qhg_params_weight_name(const char *pop_class, int32_t eval);
+const char *qhg_params_polyline_name(const char *pop_class, int32_t eval);
 /* A numeric attribute by its reference name (e.g. "Verhulst_b0"). */
 int qhg_params_get_attribute(const qhg_params *p, const char *name, double *value);
 int qhg_params_set_attribute(qhg_params *p, const char *name, double value);

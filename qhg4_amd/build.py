@@ -22,16 +22,47 @@ FLAGS = [
 ]
 
 
-def build(verbose=False):
-    cmd = [HIPCC] + FLAGS + ["-o", LIB + ".tmp"] + [os.path.join(CSRC, s) for s in SOURCES]
+# QDF (HDF5) I/O: a host library of its own, linked to the image's HDF5.
+# The HDF5 prefix also holds an older libstdc++ than the ROCm libraries
+# need, so the library's search path is a private directory holding only a
+# link to libhdf5 (qhg4_amd/_hdf5), not the prefix's lib directory.
+QDF_LIB = os.path.join(HERE, "libqhg4_qdf.so")
+HDF5 = os.environ.get("QHG_HDF5_PREFIX", "/opt/conda")
+HDF5_SONAME = "libhdf5.so.103"
+HDF5_LINK_DIR = os.path.join(HERE, "_hdf5")
+QDF_CMD = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I" + os.path.join(HDF5, "include"),
+           os.path.join(CSRC, "qdf_io.cpp"), "-L" + HDF5_LINK_DIR, "-l:" + HDF5_SONAME, "-L" + HERE,
+           "-lqhg4_amd", "-Wl,-rpath,$ORIGIN/_hdf5", "-Wl,-rpath,$ORIGIN"]
+
+
+def _hdf5_link():
+    os.makedirs(HDF5_LINK_DIR, exist_ok=True)
+    link = os.path.join(HDF5_LINK_DIR, HDF5_SONAME)
+    target = os.path.join(HDF5, "lib", HDF5_SONAME)
+    if os.path.islink(link) and os.readlink(link) == target:
+        return
+    if os.path.lexists(link):
+        os.remove(link)
+    os.symlink(target, link)
+
+
+def _run(cmd, what, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     msgs = [ln for ln in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in ln]
     if r.returncode != 0:
         sys.stderr.write("\n".join(msgs) + "\n")
-        raise RuntimeError("hipcc failed (%d)" % r.returncode)
+        raise RuntimeError("%s failed (%d)" % (what, r.returncode))
+
+
+def build(verbose=False):
+    _run([HIPCC] + FLAGS + ["-o", LIB + ".tmp"] + [os.path.join(CSRC, s) for s in SOURCES], "hipcc", verbose)
     os.replace(LIB + ".tmp", LIB)
+    if os.path.exists(os.path.join(HDF5, "include", "hdf5.h")):
+        _hdf5_link()
+        _run(QDF_CMD + ["-o", QDF_LIB + ".tmp"], "g++ (QDF library)", verbose)
+        os.replace(QDF_LIB + ".tmp", QDF_LIB)
     return LIB
 
 

@@ -325,6 +325,74 @@ int qhg_params_from_modules(const char *pop_class, const qhg_module_attrs *mods,
     return 0;
 }
 
+const char *qhg_params_attribute_name(int32_t kind, int32_t index) {
+    const std::vector<Attr> &a = attrs(kind);
+    return (index >= 0 && index < (int32_t)a.size()) ? a[index].name : nullptr;
+}
+
+const char *qhg_params_weight_name(const char *pop_class, int32_t eval) {
+    for (const ClassSpec &c : classes())
+        if (pop_class && std::strcmp(c.name, pop_class) == 0)
+            return (eval >= 0 && eval < (int32_t)c.evals.size()) ? c.evals[eval].weight : nullptr;
+    return nullptr;
+}
+
+const char *qhg_params_polyline_name(const char *pop_class, int32_t eval) {
+    for (const ClassSpec &c : classes())
+        if (pop_class && std::strcmp(c.name, pop_class) == 0)
+            return (eval >= 0 && eval < (int32_t)c.evals.size() && *c.evals[eval].pl_attr) ? c.evals[eval].pl_attr
+                                                                                              : nullptr;
+    return nullptr;
+}
+
+// A species' attributes as one flat map (the attributes of a QDF species
+// group, written by Prioritizer::writeActionParamsQDF): every name goes to
+// the module of the class's action that owns it; names no action of the
+// class owns (ClassName, NumCells, other actions' attributes) are ignored.
+int qhg_params_from_attribute_map(const char *pop_class, int32_t n, const char *const *keys,
+                                  const char *const *values, const char *const *prio_names,
+                                  const int32_t *prio_values, int32_t nprios, uint64_t seed, qhg_params *out) {
+    if (!pop_class || (n > 0 && (!keys || !values))) return qhg_internal_fail(QHG_ERR_ARG, "null argument");
+    const ClassSpec *cs = nullptr;
+    for (const ClassSpec &c : classes())
+        if (std::strcmp(c.name, pop_class) == 0) cs = &c;
+    if (!cs) return qhg_internal_fail(QHG_ERR_ARG, std::string("unknown population class ") + pop_class);
+    std::map<std::string, std::string> kv;
+    for (int32_t k = 0; k < n; k++)
+        if (keys[k] && values[k]) kv[keys[k]] = values[k];
+    std::vector<std::pair<std::string, std::vector<std::pair<std::string, std::string>>>> mods;
+    auto add = [&](const std::string &module, const char *key) {
+        auto it = kv.find(key);
+        if (it == kv.end()) return;
+        for (auto &m : mods)
+            if (m.first == module) { m.second.push_back({key, it->second}); return; }
+        mods.push_back({module, {{key, it->second}}});
+    };
+    for (int kind : cs->kinds) {
+        const std::string an = action_name(*cs, kind);
+        if (kind == QHG_ACT_MULTIEVAL) {
+            mods.push_back({an, {}});
+            for (const EvalSpec &ev : cs->evals) {
+                add(an, ev.weight);
+                if (*ev.pl_attr) add(std::string("SingleEvaluator[") + ev.id + "]", ev.pl_attr);
+            }
+            continue;
+        }
+        for (const Attr &a : attrs(kind)) add(an, a.name);
+    }
+    std::vector<qhg_module_attrs> ma(mods.size());
+    std::vector<std::vector<const char *>> kp(mods.size()), vp(mods.size());
+    for (size_t m = 0; m < mods.size(); m++) {
+        for (auto &e : mods[m].second) {
+            kp[m].push_back(e.first.c_str());
+            vp[m].push_back(e.second.c_str());
+        }
+        ma[m] = qhg_module_attrs{mods[m].first.c_str(), (int32_t)kp[m].size(), kp[m].data(), vp[m].data()};
+    }
+    return qhg_params_from_modules(pop_class, ma.data(), (int32_t)ma.size(), prio_names, prio_values, nprios, seed,
+                                   out);
+}
+
 int qhg_params_get_attribute(const qhg_params *p, const char *name, double *value) {
     if (!p || !name || !value) return qhg_internal_fail(QHG_ERR_ARG, "null argument");
     for (int kind = QHG_ACT_GETOLD; kind <= QHG_ACT_ATANDEATH; kind++)

@@ -1061,6 +1061,59 @@ int qhg_gpu_upload_agents(qhg_pop *p, const qhg_agents *a) {
     return 0;
 }
 
+int qhg_gpu_set_clock(qhg_pop *p, int64_t step, double t, int64_t max_id) {
+    if (!p) return fail(QHG_ERR_ARG, "null handle");
+    if (p->batch) return fail(QHG_ERR_STATE, "population is stepped by its batch");
+    if (p->acted || p->initialized) return fail(QHG_ERR_STATE, "population is inside a step");
+    if (step < 0 || step > (int64_t)UINT32_MAX || max_id < 0) return fail(QHG_ERR_ARG, "step or max_id out of range");
+    HIPCHK(hipStreamSynchronize(p->stream));
+    poll_ring(p);
+    for (int k = 0; k < kRing; k++) p->ring_step[k] = -1;
+    DevStatus st;
+    int rc = read_status_sync(p, &st);
+    if (rc) return rc;
+    const int nb = (int)(step & 1);
+    if (nb != p->cur()) {
+        // the agents stay where they are: the buffers change roles
+        std::swap(p->buf[0], p->buf[1]);
+        std::swap(p->d_off[0], p->d_off[1]);
+        std::swap(p->d_occ[0], p->d_occ[1]);
+        std::swap(st.n_alive[0], st.n_alive[1]);
+        std::swap(st.n_occ[0], st.n_occ[1]);
+        std::swap(st.max_id[0], st.max_id[1]);
+        std::swap(st.gpool[0], st.gpool[1]);
+    }
+    st.max_id[nb] = std::max(st.max_id[nb], max_id);
+    st.step = step;
+    HIPCHK(hipMemcpyAsync(p->d_st, &st, sizeof(st), hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    p->steps_enqueued = step;
+    p->t_next = t;
+    p->known = st;
+    p->known_step = step;
+    p->pool_reset_step = -1;
+    p->gen++;
+    return 0;
+}
+
+int qhg_gpu_get_clock(qhg_pop *p, int64_t *step, double *t, int64_t *max_id) {
+    if (!p) return fail(QHG_ERR_ARG, "null handle");
+    if (p->acted || p->initialized) return fail(QHG_ERR_STATE, "population is inside a step");
+    DevStatus st;
+    int rc = read_status_sync(p, &st);
+    if (rc) return rc;
+    if (step) *step = p->steps_enqueued;
+    if (t) *t = p->t_next;
+    if (max_id) *max_id = st.max_id[p->cur()];
+    return 0;
+}
+
+int qhg_gpu_get_params(qhg_pop *p, qhg_params *out) {
+    if (!p || !out) return fail(QHG_ERR_ARG, "null argument");
+    *out = p->P;
+    return 0;
+}
+
 int qhg_gpu_num_alive(qhg_pop *p, int64_t *n) {
     if (!p || !n) return fail(QHG_ERR_ARG, "null argument");
     DevStatus s;

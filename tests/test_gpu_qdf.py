@@ -1,0 +1,47 @@
+"""Dump and resume on the GPU: a population written to a QDF file at step k
+and restored into a fresh population continues exactly as the
+uninterrupted run (agents, genomes, ids of later newborns)."""
+import pytest
+
+from qhg4_amd import qdf, scenarios
+from qhg4_amd.population import GpuPopulation
+from tests.helpers import assert_same_state
+
+pytestmark = pytest.mark.gpu
+
+
+def run(sc, steps):
+    pop = GpuPopulation.from_scenario(sc)
+    pop.preLoop()
+    pop.run(steps)
+    a, s = pop.agents(), pop.stats()
+    pop.close()
+    return a, s
+
+
+@pytest.mark.parametrize("case,k,m", [("c2", 15, 15), ("c2", 8, 9), ("c3", 7, 10)])
+def test_dump_and_resume(tmp_path, case, k, m):
+    print("case: %s dump at %d, resume for %d" % (case, k, m), flush=True)
+    if case == "c2":
+        sc = scenarios.c2(seed=51, level=5, n_agents=20000, patch_cells=800)
+    else:
+        sc = scenarios.c2(seed=52, level=4, n_agents=6000, patch_cells=300, genetics=True)
+    ref_a, ref_s = run(sc, k + m)
+    f = str(tmp_path / "dump.qdf")
+    a = GpuPopulation.from_scenario(sc)
+    a.preLoop()
+    a.run(k)
+    qdf.write_population(a, f, pop_class="OoAPop")
+    a.close()
+    info = qdf.read_info(f)
+    assert info["step"] == k and info["time"] == float(k)
+    params = qdf.read_params(f)
+    assert bytes(params) == bytes(sc["params"])
+    b = GpuPopulation(sc["grid"], sc["env"], params)
+    b.preLoop()
+    qdf.restore_population(b, f)
+    b.run(m)
+    assert_same_state(b.agents(), ref_a, "resumed %s" % case)
+    s = b.stats()
+    assert (s["num_alive"], s["max_id"], s["step"]) == (ref_s["num_alive"], ref_s["max_id"], ref_s["step"])
+    b.close()

@@ -13,22 +13,32 @@ from oracle.oracle import lib as olib
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_symbols():
-    names = set()
-    for fn in os.listdir(os.path.join(ROOT, "include")):
-        if fn.endswith(".h"):
-            txt = open(os.path.join(ROOT, "include", fn)).read()
-            names |= set(re.findall(r"\b(qhg_[a-z0-9_]+)\s*\(", txt))
-    return names
+def declared_symbols(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    return set(re.findall(r"\b(qhg_[a-z0-9_]+)\s*\(", txt))
+
+
+def test_every_header_is_covered():
+    assert sorted(fn for fn in os.listdir(os.path.join(ROOT, "include")) if fn.endswith(".h")) == \
+        ["qhg_gpu.h", "qhg_qdf.h"]
 
 
 def test_library_exports_every_declared_symbol():
     L = _lib.lib()
-    decl = declared_symbols()
+    decl = declared_symbols("qhg_gpu.h")
     assert decl, "no declarations found"
     for name in sorted(decl):
         assert hasattr(L, name), name
     assert set(_lib.EXPORTS) == decl
+
+
+def test_qdf_library_exports_every_declared_symbol():
+    from qhg4_amd import qdf
+    L = qdf.lib()
+    decl = declared_symbols("qhg_qdf.h")
+    for name in sorted(decl):
+        assert hasattr(L, name), name
+    assert set(qdf.EXPORTS) == decl
 
 
 def test_abi_version_and_words():

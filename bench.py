@@ -52,15 +52,6 @@ def bytes_model():
     return out
 
 
-def grid_edges(grid):
-    """Incoming (source cell, neighbour slot) pairs a move can take."""
-    import numpy as np
-    nbr = np.asarray(grid["neighbors"]).reshape(grid["num_cells"], grid["connectivity"])
-    nn = np.asarray(grid["num_neighbors"])
-    slot = np.arange(grid["connectivity"])[None, :]
-    return int(((nbr >= 0) & (slot < nn[:, None])).sum())
-
-
 def kernel_bytes(bm, d, grid, ops, gwords):
     """Algorithmic bytes of each step kernel summed over the measured steps.
     d: deltas of live agent-steps, occupied cell-steps, births, deaths."""
@@ -77,7 +68,7 @@ def kernel_bytes(bm, d, grid, ops, gwords):
     per_cell = bm["ACT_CELL"] + conn * bm["ACT_CELL_PER_CONN"]
     per_occ = bm["ACT_OCC"] + conn * bm["ACT_OCC_PER_CONN"] + bm["ACT_OCC_W_EXTRA"]
     act = steps * ncells * per_cell + occ * per_occ + live * per_live + births * bm["ACT_BIRTH"]
-    scan = steps * (ncells * bm["SCAN_CELL"] + grid_edges(grid) * bm["SCAN_EDGE"])
+    scan = steps * ncells * bm["SCAN_CELL"]
     survivors = live - deaths
     per_surv = bm["SCAT_SURVIVOR"] + (bm["SCAT_SURVIVOR_GEN"] if gwords else 0)
     per_birth = bm["SCAT_BIRTH"] + (bm["SCAT_BIRTH_GEN"] if gwords else 0)

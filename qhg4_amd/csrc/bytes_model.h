@@ -28,10 +28,11 @@
 #define QHG_B_ACT_LIVE_FERT     10  /* gender r 1 + last birth r 8 + life w 1                */
 #define QHG_B_ACT_BIRTH         12  /* mother's NumBabies r/w 8 + father slot w 4            */
 
-/* k_cell_scan (next counts -> segment and birth offsets) */
-#define QHG_B_SCAN_CELL         24  /* stayers r 4 + births r 4 + in-list offset r 4 + next
-                                       offset w 4 + birth offset w 4 + cell-tile share 4    */
-#define QHG_B_SCAN_EDGE          9  /* source r 4 + slot r 1 + mover count r 4               */
+/* k_cell_scan (next counts -> segment and birth offsets).  The incoming-run
+   rows of cells that receive movers (count r 4 + zero w 4 + source r 4 +
+   position w 4 per moving run) are not counted: a lower bound. */
+#define QHG_B_SCAN_CELL         20  /* stayers r 4 + births r 4 + arrivals r 4 + next
+                                       offset w 4 + cell-tile share 4                       */
 
 /* k_cell_scatter (performMoves + createAgentAtIndex + makeOffspring) */
 #define QHG_B_SCAT_CELL          8  /* next offset r 4 (+ neighbour's) + birth count r 4     */

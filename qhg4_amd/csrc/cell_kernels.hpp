@@ -124,6 +124,8 @@ struct Ctx {
     uint8_t *gflag;
     uint32_t *cstay, *cbirth;  // per cell: stayers, births (zero outside a step)
     uint32_t *cin;            // ncells * in_w: movers arriving per incoming run
+    uint32_t *carr;           // per cell: movers arriving in all (zero outside a step)
+    uint32_t *tact;           // per scan tile: cells of it hold agents next step (zero outside a step)
     const int32_t *outk;      // ncells * conn: cin index of each (cell, slot) run,
                               // -1 where the slot takes no moves
     uint32_t *birth_off;      // ncells + 1: births before each cell
@@ -147,7 +149,7 @@ struct StepArgs {
     uint32_t parity;          // step & 1: which status halves are current
     double t;                 // time of the step
     int32_t gcells;           // occupied cells per wave group (1..kGroupCells)
-    int32_t pad;
+    int32_t dense;            // k_cell_scan reads every cell's incoming runs (many cells occupied)
 };
 
 constexpr double kPi = 3.141592653589793;   // Q_PI, QHG4/utils/qhg_consts.h:71
@@ -340,7 +342,10 @@ struct CellView {             // the per-cell values every agent of the cell rea
 // loaded every field).  A dead agent stops.  Moves are only registered here
 // (their decision draws are taken now) and resolved afterwards, as
 // performMoves applies them after all actions.
-__device__ __forceinline__ void act_program(const Ctx &x, const Prog &p, const StepArgs &sa, const CellView &cv,
+// kDeaths: the program may hold OldAgeDeath / ATanDeath (a separate
+// instantiation keeps their code out of the common kernel)
+template <bool kDeaths, typename CtxT, typename ProgT>
+__device__ __forceinline__ void act_program(const CtxT &x, const ProgT &p, const StepArgs &sa, const CellView &cv,
                                             ActItem &a, uint32_t &deaths) {
     const double t = sa.t;
     if (p.need_act0) draw4(x.seed, sa.step, QHG_STREAM_ACT, (uint64_t)a.id, 0, &a.u[0]);
@@ -393,13 +398,13 @@ __device__ __forceinline__ void act_program(const Ctx &x, const Prog &p, const S
                 if (a.npend == 0) { a.pk0 = op; a.pu0 = dir; a.npend = 1; }
                 else if (a.npend == 1) { a.pk1 = op; a.pu1 = dir; a.npend = 2; }
             }
-        } else if (op == QHG_ACT_OLDAGEDEATH) {
+        } else if (kDeaths && op == QHG_ACT_OLDAGEDEATH) {
             // OldAgeDeath::execute: age, then death past max age + U(1-uM, 1+uM)
             a.age = t - a.birth;
             a.age_dirty = true;
             const double r = wrandr(1 - p.oad_unc * p.oad_max, 1 + p.oad_unc * p.oad_max, a.u[QHG_ACTW_OADEATH]);
             if (a.age > p.oad_max + r) { a.life = 0; deaths++; break; }
-        } else if (op == QHG_ACT_ATANDEATH) {
+        } else if (kDeaths && op == QHG_ACT_ATANDEATH) {
             // ATanDeath::execute (ATanDeath.cpp:66-90): age, then death with
             // probability 1/2 + scale * atan(slope * (age - max age)) / pi
             a.age = t - a.birth;
@@ -414,7 +419,8 @@ __device__ __forceinline__ void act_program(const Ctx &x, const Prog &p, const S
 // Neighbour slot a registered move lands on, or -1 when the agent stays
 // (slot 0, a -1 hole, or ice for WeightedMove).  w: the cell's cumulative
 // weight row, nbr: its neighbour row (registers, compile-time indices only).
-__device__ __forceinline__ int move_dir(const Ctx &x, int kind, uint32_t u, int nn, const double (&w)[7],
+template <typename CtxT>
+__device__ __forceinline__ int move_dir(const CtxT &x, int kind, uint32_t u, int nn, const double (&w)[7],
                                         const int32_t (&nbr)[6]) {
     int idx = -1;
     if (kind == QHG_ACT_RANDOMMOVE) {
@@ -464,7 +470,8 @@ __device__ __forceinline__ void id_order(const int64_t *ids, int32_t n, int32_t 
     }
 }
 
-__device__ __forceinline__ void cell_act(const Ctx &x, const Prog &p, const StepArgs &sa, int32_t c, uint32_t s,
+template <bool kDeaths, typename CtxT, typename ProgT>
+__device__ __forceinline__ void cell_act(const CtxT &x, const ProgT &p, const StepArgs &sa, int32_t c, uint32_t s,
                                          int32_t n, const PairLists &L, uint32_t &my_deaths, uint32_t &my_moves,
                                          uint32_t &my_fert) {
     const int lane = threadIdx.x;
@@ -584,7 +591,7 @@ __device__ __forceinline__ void cell_act(const Ctx &x, const Prog &p, const Step
         const uint8_t lifeS = a.life;
         int dir = -1;
         if (valid && a.life != 0) {
-            act_program(x, p, sa, cv, a, my_deaths);
+            act_program<kDeaths>(x, p, sa, cv, a, my_deaths);
             if (a.npend > 0) {
                 // the cell's neighbour and weight rows, for movers only
                 int32_t nbr[6];
@@ -624,10 +631,16 @@ __device__ __forceinline__ void cell_act(const Ctx &x, const Prog &p, const Step
     if (lane == 0) {
         x.cstay[c] = stay;
         if (births) x.cbirth[c] = births;
+        if (stay | births) x.tact[c / kCellTile] = 1u;
     }
 #pragma unroll
     for (int d = 0; d < 6; d++)
-        if (lane == d && d < x.conn && mv[d]) x.cin[x.outk[(int64_t)c * x.conn + d]] = mv[d];
+        if (lane == d && d < x.conn && mv[d]) {
+            const int32_t dest = x.nbr[(int64_t)c * x.conn + d];
+            x.cin[x.outk[(int64_t)c * x.conn + d]] = mv[d];
+            atomicAdd(&x.carr[dest], mv[d]);
+            x.tact[dest / kCellTile] = 1u;
+        }
     wave_sync();
 }
 
@@ -667,7 +680,8 @@ union ActLds {
 // The cells of a group are consecutive occupied cells, so their segments
 // form one slot range [S0, S0+T).  Lane k < ng holds cell k's entry e and
 // eb = e.s - S0.  Each cell's lists live at its own positions [eb, eb+n).
-__device__ __forceinline__ void group_act(const Ctx &x, const Prog &p, const StepArgs &sa, int ng, const OccCell &e,
+template <bool kDeaths, typename CtxT, typename ProgT>
+__device__ __forceinline__ void group_act(const CtxT &x, const ProgT &p, const StepArgs &sa, int ng, const OccCell &e,
                                           uint32_t eb, uint32_t S0, uint32_t T, GroupActLds &G,
                                           uint32_t &my_deaths, uint32_t &my_moves, uint32_t &my_fert) {
     const int lane = threadIdx.x;
@@ -854,7 +868,7 @@ __device__ __forceinline__ void group_act(const Ctx &x, const Prog &p, const Ste
         const uint8_t lifeS = a.life;
         int dir = -1;
         if (valid && a.life != 0) {
-            act_program(x, p, sa, cv, a, my_deaths);
+            act_program<kDeaths>(x, p, sa, cv, a, my_deaths);
             if (a.npend > 0) {
                 int32_t nbr[6];
                 double w[7];
@@ -892,14 +906,21 @@ __device__ __forceinline__ void group_act(const Ctx &x, const Prog &p, const Ste
     if (lane < ng) {
         x.cstay[e.c] = G.cnt[lane][kCntStay];
         if (G.cnt[lane][kCntBirth]) x.cbirth[e.c] = G.cnt[lane][kCntBirth];
+        if (G.cnt[lane][kCntStay] | G.cnt[lane][kCntBirth]) x.tact[e.c / kCellTile] = 1u;
         for (int d = 0; d < x.conn; d++) {
             const uint32_t m = G.cnt[lane][kCntMv + d];
-            if (m) x.cin[G.outk[lane][d]] = m;
+            if (m) {
+                const int32_t dest = G.nbr[lane][d];
+                x.cin[G.outk[lane][d]] = m;
+                atomicAdd(&x.carr[dest], m);
+                x.tact[dest / kCellTile] = 1u;
+            }
         }
     }
     wave_sync();
 }
 
+template <bool kDeaths>
 __global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs sa) {
     __shared__ ActLds lds;
     const int lane = threadIdx.x;
@@ -941,7 +962,7 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs
         const uint32_t T = readlane_u32(e.s, ng - 1) + readlane_u32(e.n, ng - 1) - S0;
         const uint32_t nmax = wave_max_u32(lane < ng ? e.n : 0u);
         if (T <= (uint32_t)kGroupSlots && nmax <= (uint32_t)kGroupCellMax) {
-            group_act(x, p, sa, ng, e, e.s - S0, S0, T, lds.g, my_deaths, my_moves, my_fert);
+            group_act<kDeaths>(x, p, sa, ng, e, e.s - S0, S0, T, lds.g, my_deaths, my_moves, my_fert);
             continue;
         }
         for (int k = 0; k < ng; k++) {
@@ -956,7 +977,7 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs
                 L = PairLists{lds.c.fu, lds.c.mu, lds.c.fid, lds.c.mid, lds.c.fs, lds.c.ms, lds.c.mate,
                               lds.c.flag};
             }
-            cell_act(x, p, sa, c, s, n, L, my_deaths, my_moves, my_fert);
+            cell_act<kDeaths>(x, p, sa, c, s, n, L, my_deaths, my_moves, my_fert);
         }
     }
     const uint64_t dm = wave_sum_u64(((uint64_t)my_moves << 32) | my_deaths);
@@ -984,16 +1005,31 @@ struct ScanLds {
 };
 
 // One look-back tile of k_cell_scan (kCellTile cells), all kScanBlock threads.
-__device__ __forceinline__ void scan_tile(const Ctx &x, const StepArgs &sa, uint32_t tile, ScanLds &L) {
+template <typename CtxT>
+__device__ __forceinline__ void scan_tile(const CtxT &x, const StepArgs &sa, uint32_t tile, ScanLds &L) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t nc = x.ncells;
     const int64_t base = (int64_t)tile * kCellTile;
-    uint32_t stay[kScanPer];
-    int32_t e[kScanPer][kMaxIn];
+    const bool last = base + kCellTile >= nc;
+    if (!last && x.tact[tile] == 0u) {
+        // no cell of the tile holds agents next step: its sums are zero and
+        // none of its per-cell outputs is read (the step totals come from
+        // the last tile, which always runs)
+        if (threadIdx.x == 0) {
+            const uint64_t f = (tile == 0) ? kFlagInc : kFlagAgg;
+            tile_publish(&x.tiles[tile], f);
+            tile_publish(&x.tiles2[tile], f);
+        }
+        return;
+    }
+    uint32_t stay[kScanPer], arr[kScanPer];
     uint32_t cm[kScanPer][kMaxIn];
     uint64_t v[kScanPer];
-    // one round trip: stayers, births, arrivals per incoming run and the
-    // runs' source slots; the counts go back to zero once read
+    // stayers, births and arrivals of every cell.  With many cells occupied
+    // (sa.dense) the incoming-run counts of every cell are loaded in the same
+    // round trip; otherwise only those of cells that receive movers, later.
+    // The counts go back to zero once read.
+    const bool dense = sa.dense != 0;
     uint64_t sum = 0, osum = 0;
 #pragma unroll
     for (int q = 0; q < kScanPer; q++) {
@@ -1001,21 +1037,13 @@ __device__ __forceinline__ void scan_tile(const Ctx &x, const StepArgs &sa, uint
         const bool ok = c < nc;
         stay[q] = ok ? x.cstay[c] : 0u;
         const uint32_t b = ok ? x.cbirth[c] : 0u;
+        arr[q] = ok ? x.carr[c] : 0u;
 #pragma unroll
-        for (int k = 0; k < kMaxIn; k++) {
-            const bool in = ok && k < x.in_w;
-            e[q][k] = in ? x.in_e[c * x.in_w + k] : -1;
-            cm[q][k] = in ? x.cin[c * x.in_w + k] : 0u;
-        }
-        uint32_t inc = 0;
-#pragma unroll
-        for (int k = 0; k < kMaxIn; k++) {
-            inc += cm[q][k];
-            if (cm[q][k]) x.cin[c * x.in_w + k] = 0;
-        }
+        for (int k = 0; k < kMaxIn; k++) cm[q][k] = (dense && ok && k < x.in_w) ? x.cin[c * x.in_w + k] : 0u;
         if (stay[q]) x.cstay[c] = 0;
         if (b) x.cbirth[c] = 0;
-        v[q] = ok ? ((uint64_t)(stay[q] + inc + b) | ((uint64_t)b << 32)) : 0ull;
+        if (arr[q]) x.carr[c] = 0;
+        v[q] = ok ? ((uint64_t)(stay[q] + arr[q] + b) | ((uint64_t)b << 32)) : 0ull;
         sum += v[q];
         osum += ((uint32_t)v[q] != 0u) ? 1u : 0u;
     }
@@ -1045,14 +1073,24 @@ __device__ __forceinline__ void scan_tile(const Ctx &x, const StepArgs &sa, uint
         if (c < nc) {
             const uint32_t o = (uint32_t)run;
             x.off_nxt[c] = o;
-            x.birth_off[c] = (uint32_t)(run >> 32);
             uint32_t pos = o + stay[q];
+            if (arr[q]) {
+                // arrivals: one block per incoming run, in run order
 #pragma unroll
-            for (int k = 0; k < kMaxIn; k++) {
-                if (e[q][k] >= 0) x.rpos[e[q][k]] = pos;
-                pos += cm[q][k];
+                for (int k = 0; k < kMaxIn; k++) {
+                    if (k >= x.in_w) break;
+                    const uint32_t m = dense ? cm[q][k] : x.cin[c * x.in_w + k];
+                    if (m) {
+                        x.rpos[x.in_e[c * x.in_w + k]] = pos;
+                        x.cin[c * x.in_w + k] = 0;
+                        pos += m;
+                    }
+                }
             }
-            x.bpos[c] = pos;
+            if ((uint32_t)(v[q] >> 32)) {
+                x.bpos[c] = pos;
+                x.birth_off[c] = (uint32_t)(run >> 32);
+            }
             const uint32_t nn = (uint32_t)v[q];
             if (nn != 0) {
                 x.occ_nxt[orun] = OccCell{(int32_t)c, o, nn, 0u};
@@ -1061,7 +1099,8 @@ __device__ __forceinline__ void scan_tile(const Ctx &x, const StepArgs &sa, uint
         }
         run += v[q];
     }
-    if (base + kCellTile >= nc && threadIdx.x == kScanBlock - 1) {
+    if (threadIdx.x == 0) x.tact[tile] = 0u;
+    if (last && threadIdx.x == kScanBlock - 1) {
         // last tile: the next step's sizes (the totals the step kernels
         // need; deaths, moves and the host snapshot follow in k_cell_scatter)
         const int64_t total = (int64_t)(uint32_t)run;
@@ -1124,7 +1163,8 @@ __global__ void __launch_bounds__(kScanBlock) k_cell_scan(Ctx x, StepArgs sa) {
 // ---------------------------------------------------------------------------
 // k_cell_scatter: agents of one occupied source cell to the next buffer
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void cell_scatter(const Ctx &x, const StepArgs &sa, int32_t c, uint32_t s, int32_t n,
+template <typename CtxT>
+__device__ __forceinline__ void cell_scatter(const CtxT &x, const StepArgs &sa, int32_t c, uint32_t s, int32_t n,
                                              int32_t *mli, int64_t *mid, int64_t max_id, int64_t gbase) {
     const int lane = threadIdx.x;
     const uint64_t lt = lanemask_lt();
@@ -1241,7 +1281,8 @@ union ScatLds {
 
 // all cells of a group (see group_act): stayers and movers ranked per cell
 // and run by masked ballots, running counts across chunks in LDS
-__device__ __forceinline__ void group_scatter(const Ctx &x, const StepArgs &sa, int ng, const OccCell &e,
+template <typename CtxT>
+__device__ __forceinline__ void group_scatter(const CtxT &x, const StepArgs &sa, int ng, const OccCell &e,
                                               uint32_t eb, uint32_t S0, uint32_t T, GroupScatLds &G,
                                               int64_t max_id, int64_t gbase) {
     const int lane = threadIdx.x;
@@ -1436,7 +1477,8 @@ __device__ __forceinline__ uint64_t bit_range(int lo, int hi) {
 // crossover draws, the father's, the mutation count, then (position, mask)
 // pairs; mutations that hit this word are applied here.  Draws 0-3 and the
 // block holding the mutation count are computed once.
-__device__ uint64_t child_word(const Ctx &x, const Prog &p, uint32_t step, uint64_t nid, const uint64_t *par,
+template <typename CtxT, typename ProgT>
+__device__ uint64_t child_word(const CtxT &x, const ProgT &p, uint32_t step, uint64_t nid, const uint64_t *par,
                                int s, int b, int nbk, int w) {
     uint32_t d0[4];
     draw4(x.seed, step, QHG_STREAM_GENETICS, nid, 0, d0);
@@ -1598,13 +1640,33 @@ struct BatchArgs {
     int32_t pad;
 };
 
+// values every lane holds alike (ballot and shuffle results), as such
+__device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uniform_i64(int64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+template <typename T>
+__device__ __forceinline__ T *uniform_ptr(T *p) {
+    return (T *)(uintptr_t)uniform_i64((int64_t)(uintptr_t)p);
+}
+
 __device__ __forceinline__ int64_t batch_step_in(const BatchArgs &B, int k, int next) {
-    const int64_t b = B.bctr[k];
+    const int64_t b = uniform_i64(B.bctr[k]);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (next >= 0) B.bctr[next] = b;
         if (next < 0 || (k == BC_SCATTER && !B.genome)) B.bctr[BC_ACT] = b + 1;
     }
     return b;
+}
+
+// the scan loads every cell's incoming runs at once when at least one cell
+// in 2^shift is occupied: worth it where the scan is latency-bound (one
+// population: shift 3), less so where many tiles keep it bandwidth-bound
+// (a batch: shift 1).  Results do not depend on it.
+__host__ __device__ __forceinline__ int32_t scan_dense(int64_t occ, int64_t ncells, int shift) {
+    return ((occ << shift) >= ncells) ? 1 : 0;
 }
 
 // cells per wave group: about `target` agents per group from the mean
@@ -1617,10 +1679,22 @@ __host__ __device__ __forceinline__ int32_t group_cells(double target, int64_t a
     return (int32_t)(g < 1.0 ? 1.0 : (g > (double)kGroupCells ? (double)kGroupCells : g));
 }
 
+// The descriptor table is written by the host before a launch and by no
+// kernel: read through the constant address space, its fields go to scalar
+// registers (and stay there across the kernels' own stores).
+typedef __attribute__((address_space(4))) const PopDesc ConstPopDesc;
+typedef __attribute__((address_space(4))) const Ctx ConstCtx;
+typedef __attribute__((address_space(4))) const Prog ConstProg;
+
+__device__ __forceinline__ ConstPopDesc &pop_desc(const BatchArgs &B, int r) {
+    return ((ConstPopDesc *)B.pops)[r];
+}
+
 // A member's step arguments: step and time advance by one per batch step
 // from where the member joined (fixed for the whole batch step, whatever the
 // kernels write into its status); the group size follows its occupancy.
-__device__ __forceinline__ StepArgs pop_step_args(const PopDesc &d, int64_t bstep, double group_agents) {
+template <typename Desc>
+__device__ __forceinline__ StepArgs pop_step_args(const Desc &d, int64_t bstep, double group_agents) {
     StepArgs a{};
     const int64_t step = d.step0 + (bstep - d.join);
     a.step = (uint32_t)step;
@@ -1628,6 +1702,16 @@ __device__ __forceinline__ StepArgs pop_step_args(const PopDesc &d, int64_t bste
     a.t = d.t0 + (double)(bstep - d.join);
     const DevStatus *st = d.ctx[0].st;
     a.gcells = group_cells(group_agents, st->n_alive[a.parity], st->n_occ[a.parity]);
+    a.dense = scan_dense(st->n_occ[a.parity], d.ctx[0].ncells, 1);
+    return a;
+}
+
+// the same for a member every lane of the wave works on (its values as such)
+template <typename Desc>
+__device__ __forceinline__ StepArgs pop_step_args_uniform(const Desc &d, int64_t bstep, double group_agents) {
+    StepArgs a = pop_step_args(d, bstep, group_agents);
+    a.gcells = uniform_i32(a.gcells);
+    a.dense = uniform_i32(a.dense);
     return a;
 }
 
@@ -1650,7 +1734,7 @@ __device__ __forceinline__ int pop_of(const int64_t (&base)[4], int npops, int64
     int n = 0;
 #pragma unroll
     for (int k = 0; k < 4; k++) n += __popcll(__ballot(lane + 64 * k < npops && base[k] <= g));
-    return n - 1;
+    return uniform_i32(n - 1);
 }
 
 __device__ __forceinline__ int64_t lane_value(const int64_t (&v)[4], int r) {
@@ -1659,7 +1743,7 @@ __device__ __forceinline__ int64_t lane_value(const int64_t (&v)[4], int r) {
 #pragma unroll
     for (int q = 0; q < 4; q++)
         if (q == k) out = (int64_t)__shfl((long long)v[q], l, 64);
-    return out;
+    return uniform_i64(out);
 }
 
 // per-member group counts of the current step
@@ -1670,9 +1754,9 @@ __device__ __forceinline__ void batch_groups(const BatchArgs &B, int64_t bstep, 
         const int r = lane + 64 * k;
         cnt[k] = 0;
         if (r < B.npops) {
-            const DevStatus *st = B.pops[r].ctx[0].st;
+            const DevStatus *st = pop_desc(B, r).ctx[0].st;
             if (!st->error) {
-                const StepArgs a = pop_step_args(B.pops[r], bstep, B.group_agents);
+                const StepArgs a = pop_step_args(pop_desc(B, r), bstep, B.group_agents);
                 const int64_t nocc = st->n_occ[a.parity];
                 cnt[k] = (nocc + a.gcells - 1) / a.gcells;
             }
@@ -1680,6 +1764,7 @@ __device__ __forceinline__ void batch_groups(const BatchArgs &B, int64_t bstep, 
     }
 }
 
+template <bool kDeaths>
 __global__ void __launch_bounds__(kCellBlock) k_batch_act(BatchArgs B) {
     __shared__ ActLds lds;
     const int lane = threadIdx.x;
@@ -1697,68 +1782,51 @@ __global__ void __launch_bounds__(kCellBlock) k_batch_act(BatchArgs B) {
     const int64_t per = (total + gridDim.x - 1) / gridDim.x;
     const int64_t g0 = (int64_t)blockIdx.x * per;
     const int64_t g1 = (g0 + per < total) ? g0 + per : total;
-    int r = -1;
-    int64_t rbase = 0, rend = 0;
-    uint32_t deaths = 0, moves = 0, fert = 0;
-    StepArgs sa{};
-    const Ctx *x = nullptr;
-    const Prog *pg = nullptr;
-    int64_t nocc = 0;
-    for (int64_t g = g0; g < g1; g++) {
-        if (g >= rend) {
-            if (r >= 0) {
-                const uint64_t dm = wave_sum_u64(((uint64_t)moves << 32) | deaths);
-                const uint32_t fe = (uint32_t)wave_sum_u64(fert);
-                if (lane == 0) {
-                    atomicAdd((unsigned long long *)&x->st->acc_deaths, (unsigned long long)(uint32_t)dm);
-                    atomicAdd((unsigned long long *)&x->st->acc_moves, (unsigned long long)(dm >> 32));
-                    atomicAdd((unsigned long long *)&x->st->fertile_total, (unsigned long long)fe);
-                }
-                deaths = moves = fert = 0;
-            }
-            r = pop_of(base, B.npops, g);
-            rbase = lane_value(base, r);
-            rend = rbase + lane_value(cnt, r);
-            const DevStatus *st = B.pops[r].ctx[0].st;
-            sa = pop_step_args(B.pops[r], bstep, B.group_agents);
-            x = &B.pops[r].ctx[sa.parity];
-            pg = &B.pops[r].prog;
-            nocc = st->n_occ[sa.parity];
-        }
+    // one stretch per member: its context is fixed for the inner loop
+    for (int64_t g = g0; g < g1;) {
+        const int r = pop_of(base, B.npops, g);
+        const int64_t rbase = lane_value(base, r);
+        const int64_t rend0 = rbase + lane_value(cnt, r);
+        const int64_t rend = rend0 < g1 ? rend0 : g1;
+        const StepArgs sa = pop_step_args_uniform(pop_desc(B, r), bstep, B.group_agents);
+        ConstCtx &x = *uniform_ptr(&pop_desc(B, r).ctx[sa.parity]);
+        ConstProg &pg = *uniform_ptr(&pop_desc(B, r).prog);
+        const int64_t nocc = uniform_i64(x.st->n_occ[sa.parity]);
         const int64_t gc = sa.gcells;
-        const int64_t i0 = (g - rbase) * gc;
-        const int ng = (int)((nocc - i0 < gc) ? (nocc - i0) : gc);
-        OccCell e{0, 0, 0, 0};
-        if (lane < ng) e = x->occ_cur[i0 + lane];
-        const uint32_t S0 = readlane_u32(e.s, 0);
-        const uint32_t T = readlane_u32(e.s, ng - 1) + readlane_u32(e.n, ng - 1) - S0;
-        const uint32_t nmax = wave_max_u32(lane < ng ? e.n : 0u);
-        if (T <= (uint32_t)kGroupSlots && nmax <= (uint32_t)kGroupCellMax) {
-            group_act(*x, *pg, sa, ng, e, e.s - S0, S0, T, lds.g, deaths, moves, fert);
-            continue;
-        }
-        for (int k = 0; k < ng; k++) {
-            const int32_t c = (int32_t)readlane_u32((uint32_t)e.c, k);
-            const uint32_t s = readlane_u32(e.s, k);
-            const int32_t n = (int32_t)readlane_u32(e.n, k);
-            PairLists L;
-            if (n > kLdsCell) {
-                L = PairLists{x->gfu + s, x->gmu + s, x->gfid + s, x->gmid + s, x->gfs + s, x->gms + s,
-                              x->gmate + s, x->gflag + s};
-            } else {
-                L = PairLists{lds.c.fu, lds.c.mu, lds.c.fid, lds.c.mid, lds.c.fs, lds.c.ms, lds.c.mate,
-                              lds.c.flag};
+        uint32_t deaths = 0, moves = 0, fert = 0;
+        for (; g < rend; g++) {
+            const int64_t i0 = (g - rbase) * gc;
+            const int ng = (int)((nocc - i0 < gc) ? (nocc - i0) : gc);
+            OccCell e{0, 0, 0, 0};
+            if (lane < ng) e = x.occ_cur[i0 + lane];
+            const uint32_t S0 = readlane_u32(e.s, 0);
+            const uint32_t T = readlane_u32(e.s, ng - 1) + readlane_u32(e.n, ng - 1) - S0;
+            const uint32_t nmax = wave_max_u32(lane < ng ? e.n : 0u);
+            if (T <= (uint32_t)kGroupSlots && nmax <= (uint32_t)kGroupCellMax) {
+                group_act<kDeaths>(x, pg, sa, ng, e, e.s - S0, S0, T, lds.g, deaths, moves, fert);
+                continue;
             }
-            cell_act(*x, *pg, sa, c, s, n, L, deaths, moves, fert);
+            for (int k = 0; k < ng; k++) {
+                const int32_t c = (int32_t)readlane_u32((uint32_t)e.c, k);
+                const uint32_t s = readlane_u32(e.s, k);
+                const int32_t n = (int32_t)readlane_u32(e.n, k);
+                PairLists L;
+                if (n > kLdsCell) {
+                    L = PairLists{x.gfu + s, x.gmu + s, x.gfid + s, x.gmid + s, x.gfs + s, x.gms + s,
+                                  x.gmate + s, x.gflag + s};
+                } else {
+                    L = PairLists{lds.c.fu, lds.c.mu, lds.c.fid, lds.c.mid, lds.c.fs, lds.c.ms, lds.c.mate,
+                                  lds.c.flag};
+                }
+                cell_act<kDeaths>(x, pg, sa, c, s, n, L, deaths, moves, fert);
+            }
         }
-    }
-    if (r >= 0) {
         const uint64_t dm = wave_sum_u64(((uint64_t)moves << 32) | deaths);
         const uint32_t fe = (uint32_t)wave_sum_u64(fert);
         if (lane == 0) {
-            atomicAdd((unsigned long long *)&x->st->acc_deaths, (unsigned long long)(uint32_t)dm);
-            atomicAdd((unsigned long long *)&x->st->acc_moves, (unsigned long long)(dm >> 32));
-            atomicAdd((unsigned long long *)&x->st->fertile_total, (unsigned long long)fe);
+            atomicAdd((unsigned long long *)&x.st->acc_deaths, (unsigned long long)(uint32_t)dm);
+            atomicAdd((unsigned long long *)&x.st->acc_moves, (unsigned long long)(dm >> 32));
+            atomicAdd((unsigned long long *)&x.st->fertile_total, (unsigned long long)fe);
         }
     }
 }
@@ -1777,10 +1845,10 @@ __global__ void __launch_bounds__(kScanBlock) k_batch_scan(BatchArgs B) {
     for (int64_t t = blockIdx.x; t < B.total_tiles; t += gridDim.x) {
         const int r = pop_of(tb, B.npops, t);
         const int64_t lt = t - lane_value(tb, r);
-        const DevStatus *st = B.pops[r].ctx[0].st;
+        const DevStatus *st = pop_desc(B, r).ctx[0].st;
         if (st->error) continue;
-        const StepArgs sa = pop_step_args(B.pops[r], bstep, B.group_agents);
-        scan_tile(B.pops[r].ctx[sa.parity], sa, (uint32_t)lt, lds);
+        const StepArgs sa = pop_step_args_uniform(pop_desc(B, r), bstep, B.group_agents);
+        scan_tile(*uniform_ptr(&pop_desc(B, r).ctx[sa.parity]), sa, (uint32_t)lt, lds);
     }
 }
 
@@ -1789,25 +1857,25 @@ __global__ void __launch_bounds__(kCellBlock) k_batch_scatter(BatchArgs B) {
     const int lane = threadIdx.x;
     const int64_t bstep = batch_step_in(B, BC_SCATTER, BC_GENOME);
     if (blockIdx.x == 0) {
-        // close every member's step and copy its status into host memory
-        for (int r = 0; r < B.npops; r++) {
-            DevStatus *st = B.pops[r].ctx[0].st;
-            if (lane == 0 && !st->error) {
-                st->deaths_last = st->acc_deaths;
-                st->moves_last = st->acc_moves;
-                st->deaths_total += st->acc_deaths;
-                st->moves_total += st->acc_moves;
+        // close every member's step and copy its status into host memory,
+        // one lane per member
+        for (int r = lane; r < B.npops; r += kWave) {
+            DevStatus *st = pop_desc(B, r).ctx[0].st;
+            if (!st->error) {
+                const int64_t dd = st->acc_deaths, dm = st->acc_moves;
+                st->deaths_last = dd;
+                st->moves_last = dm;
+                st->deaths_total += dd;
+                st->moves_total += dm;
                 st->acc_deaths = 0;
                 st->acc_moves = 0;
             }
-            wave_sync();
-            DevStatus *hr = B.pops[r].ctx[0].hring;
+            DevStatus *hr = pop_desc(B, r).ctx[0].hring;
             if (hr) {
                 const uint64_t *srcw = reinterpret_cast<const uint64_t *>(st);
                 uint64_t *dstw = reinterpret_cast<uint64_t *>(hr + (st->step % kStatusRing));
-                if (lane < (int)(sizeof(DevStatus) / 8)) dstw[lane] = srcw[lane];
+                for (int w = 0; w < (int)(sizeof(DevStatus) / 8); w++) dstw[w] = srcw[w];
             }
-            wave_sync();
         }
         return;
     }
@@ -1819,40 +1887,38 @@ __global__ void __launch_bounds__(kCellBlock) k_batch_scatter(BatchArgs B) {
     const int64_t per = (total + G - 1) / G;
     const int64_t g0 = (int64_t)(blockIdx.x - 1) * per;
     const int64_t g1 = (g0 + per < total) ? g0 + per : total;
-    int r = -1;
-    int64_t rbase = 0, rend = 0, nocc = 0, max_id = 0, gbase = 0;
-    StepArgs sa{};
-    const Ctx *x = nullptr;
-    for (int64_t g = g0; g < g1; g++) {
-        if (g >= rend) {
-            r = pop_of(base, B.npops, g);
-            rbase = lane_value(base, r);
-            rend = rbase + lane_value(cnt, r);
-            const DevStatus *st = B.pops[r].ctx[0].st;
-            sa = pop_step_args(B.pops[r], bstep, B.group_agents);
-            x = &B.pops[r].ctx[sa.parity];
-            nocc = st->n_occ[sa.parity];
-            max_id = st->max_id[sa.parity];
-            gbase = st->gpool[sa.parity];
-        }
+    // one stretch per member: its context is fixed for the inner loop
+    for (int64_t g = g0; g < g1;) {
+        const int r = pop_of(base, B.npops, g);
+        const int64_t rbase = lane_value(base, r);
+        const int64_t rend0 = rbase + lane_value(cnt, r);
+        const int64_t rend = rend0 < g1 ? rend0 : g1;
+        const StepArgs sa = pop_step_args_uniform(pop_desc(B, r), bstep, B.group_agents);
+        ConstCtx &x = *uniform_ptr(&pop_desc(B, r).ctx[sa.parity]);
+        const DevStatus *st = x.st;
+        const int64_t nocc = uniform_i64(st->n_occ[sa.parity]);
+        const int64_t max_id = uniform_i64(st->max_id[sa.parity]);
+        const int64_t gbase = uniform_i64(st->gpool[sa.parity]);
         const int64_t gc = sa.gcells;
-        const int64_t i0 = (g - rbase) * gc;
-        const int ng = (int)((nocc - i0 < gc) ? (nocc - i0) : gc);
-        OccCell e{0, 0, 0, 0};
-        if (lane < ng) e = x->occ_cur[i0 + lane];
-        const uint32_t S0 = readlane_u32(e.s, 0);
-        const uint32_t T = readlane_u32(e.s, ng - 1) + readlane_u32(e.n, ng - 1) - S0;
-        if (T <= (uint32_t)kGroupSlots) {
-            group_scatter(*x, sa, ng, e, e.s - S0, S0, T, lds.g, max_id, gbase);
-            continue;
-        }
-        for (int k = 0; k < ng; k++) {
-            const int32_t c = (int32_t)readlane_u32((uint32_t)e.c, k);
-            const uint32_t s = readlane_u32(e.s, k);
-            const int32_t n = (int32_t)readlane_u32(e.n, k);
-            int32_t *mli = (n > kLdsCell) ? x->gfu + s : lds.c.mli;
-            int64_t *mid = (n > kLdsCell) ? x->gfid + s : lds.c.mid;
-            cell_scatter(*x, sa, c, s, n, mli, mid, max_id, gbase);
+        for (; g < rend; g++) {
+            const int64_t i0 = (g - rbase) * gc;
+            const int ng = (int)((nocc - i0 < gc) ? (nocc - i0) : gc);
+            OccCell e{0, 0, 0, 0};
+            if (lane < ng) e = x.occ_cur[i0 + lane];
+            const uint32_t S0 = readlane_u32(e.s, 0);
+            const uint32_t T = readlane_u32(e.s, ng - 1) + readlane_u32(e.n, ng - 1) - S0;
+            if (T <= (uint32_t)kGroupSlots) {
+                group_scatter(x, sa, ng, e, e.s - S0, S0, T, lds.g, max_id, gbase);
+                continue;
+            }
+            for (int k = 0; k < ng; k++) {
+                const int32_t c = (int32_t)readlane_u32((uint32_t)e.c, k);
+                const uint32_t s = readlane_u32(e.s, k);
+                const int32_t n = (int32_t)readlane_u32(e.n, k);
+                int32_t *mli = (n > kLdsCell) ? x.gfu + s : lds.c.mli;
+                int64_t *mid = (n > kLdsCell) ? x.gfid + s : lds.c.mid;
+                cell_scatter(x, sa, c, s, n, mli, mid, max_id, gbase);
+            }
         }
     }
 }
@@ -1867,7 +1933,7 @@ __global__ void __launch_bounds__(kWave) k_batch_genome(BatchArgs B) {
         const int r = lane + 64 * k;
         cnt[k] = 0;
         if (r < B.npops) {
-            const Ctx &x0 = B.pops[r].ctx[0];
+            ConstCtx &x0 = pop_desc(B, r).ctx[0];
             if (x0.gwords && !x0.st->error) {
                 const int per = (x0.gwords <= kWave) ? kWave / x0.gwords : 1;
                 cnt[k] = (x0.st->births_last + per - 1) / per;   // births of the step just scanned
@@ -1878,11 +1944,11 @@ __global__ void __launch_bounds__(kWave) k_batch_genome(BatchArgs B) {
     for (int64_t g = blockIdx.x; g < total; g += gridDim.x) {
         const int r = pop_of(base, B.npops, g);
         const int64_t gl = g - lane_value(base, r);
-        const DevStatus *st = B.pops[r].ctx[0].st;
-        const StepArgs sa = pop_step_args(B.pops[r], bstep, B.group_agents);
+        const DevStatus *st = pop_desc(B, r).ctx[0].st;
+        const StepArgs sa = pop_step_args_uniform(pop_desc(B, r), bstep, B.group_agents);
         const int par = (int)sa.parity;
-        const Ctx &x = B.pops[r].ctx[par];
-        const Prog &pg = B.pops[r].prog;
+        ConstCtx &x = *uniform_ptr(&pop_desc(B, r).ctx[par]);
+        ConstProg &pg = *uniform_ptr(&pop_desc(B, r).prog);
         const int gw = x.gwords, nbk = gw / 2;
         const int per = (gw <= kWave) ? kWave / gw : 1;
         const int pb = (gw <= kWave) ? lane / gw : 0;

@@ -119,7 +119,8 @@ struct qhg_pop {
     double *d_K = nullptr, *d_W = nullptr, *d_tmp = nullptr;
     uint8_t *d_ice = nullptr, *d_coastal = nullptr;
     double *d_env[7] = {};   // alt lon lat water npp temp rain
-    uint32_t *d_cstay = nullptr, *d_cbirth = nullptr, *d_cin = nullptr, *d_birth_off = nullptr;
+    uint32_t *d_cstay = nullptr, *d_cbirth = nullptr, *d_cin = nullptr, *d_carr = nullptr, *d_birth_off = nullptr;
+    uint32_t *d_tact = nullptr;   // per scan tile
     int32_t *d_outk = nullptr;
     uint32_t *d_bpos = nullptr, *d_rpos = nullptr;
     uint64_t *d_tiles = nullptr, *d_tiles2 = nullptr;
@@ -127,7 +128,7 @@ struct qhg_pop {
     DevStatus *d_st = nullptr;
 
     Prog prog{};
-    bool has_pair = false, has_me = false, has_cap = false;
+    bool has_pair = false, has_me = false, has_cap = false, has_death = false;
     bool me_done = false;
     bool acted = false;       // k_cell_act launched for the current step
     bool initialized = false; // initialize phase done for the current step
@@ -196,6 +197,8 @@ struct qhg_pop {
         x.cstay = d_cstay;
         x.cbirth = d_cbirth;
         x.cin = d_cin;
+        x.carr = d_carr;
+        x.tact = d_tact;
         x.outk = d_outk;
         x.birth_off = d_birth_off;
         x.bpos = d_bpos;
@@ -223,6 +226,7 @@ struct qhg_pop {
         // one wave of agents per group (results do not depend on it)
         const int kb = (int)(known_step & 1);
         a.gcells = group_cells(group_agents, known.n_alive[kb], known.n_occ[kb]);
+        a.dense = scan_dense(known.n_occ[kb], ncells, 3);
         return a;
     }
 };
@@ -245,14 +249,15 @@ int64_t device_bytes(const qhg_pop *p) {
     for (int k = 0; k < 2; k++)
         if (p->d_gpool[k]) b += p->gcap * p->gwords * 8;
     const int64_t nc = p->ncells, cn = p->conn;
-    int64_t per_cell = cn * 4 + 1 + p->in_w * 4 + cn * 4 + 8 + (cn + 1) * 8 + 8 + 2 * 4 + 4 + 4 + p->in_w * 4 + 4 + 4 +
+    int64_t per_cell = cn * 4 + 1 + p->in_w * 4 + cn * 4 + 8 + (cn + 1) * 8 + 8 + 2 * 4 + 4 + 4 + p->in_w * 4 + 4 + 4 + 4 +
                        cn * 4 + 2 * (int64_t)sizeof(OccCell);
     for (int k = 0; k < 7; k++)
         if (p->d_env[k]) per_cell += 8;
     if (p->d_ice) per_cell += 1;
     if (p->d_coastal) per_cell += 1;
     b += nc * per_cell + 3 * 4;
-    b += 2 * (int64_t)p->ntiles * 8 + (int64_t)p->num_cu * kMaxBlocksPerCu * kParts * 4 + (int64_t)sizeof(DevStatus);
+    b += 2 * (int64_t)p->ntiles * 8 + (int64_t)p->ntiles * 4 + (int64_t)p->num_cu * kMaxBlocksPerCu * kParts * 4 +
+         (int64_t)sizeof(DevStatus);
     return b;
 }
 
@@ -456,6 +461,7 @@ int build_prog(qhg_pop *p) {
     p->has_pair = has_kind(P, QHG_ACT_RANDOMPAIR);
     p->has_me = has_kind(P, QHG_ACT_MULTIEVAL);
     p->has_cap = has_kind(P, QHG_ACT_NPPCAP);
+    p->has_death = has_kind(P, QHG_ACT_OLDAGEDEATH) || has_kind(P, QHG_ACT_ATANDEATH);
     g.has_pair = p->has_pair;
     for (int q = 0; q < g.nops; q++) {
         switch (g.ops[q]) {
@@ -735,7 +741,9 @@ int phase_act(qhg_pop *p) {
         int rc = phase_initialize(p);
         if (rc) return rc;
     }
-    int rc = launch_bs(p, KB_ACT, k_cell_act, p->act_grid, kCellBlock, p->ctx(), p->prog, p->step_args());
+    int rc = p->has_death
+                 ? launch_bs(p, KB_ACT, k_cell_act<true>, p->act_grid, kCellBlock, p->ctx(), p->prog, p->step_args())
+                 : launch_bs(p, KB_ACT, k_cell_act<false>, p->act_grid, kCellBlock, p->ctx(), p->prog, p->step_args());
     if (rc) return rc;
     p->acted = true;
     return 0;
@@ -869,6 +877,7 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
         hipMalloc(&p->d_tmp, nc * 8) != hipSuccess || hipMalloc(&p->d_off[0], (nc + 1) * 4) != hipSuccess ||
         hipMalloc(&p->d_off[1], (nc + 1) * 4) != hipSuccess || hipMalloc(&p->d_cstay, nc * 4) != hipSuccess ||
         hipMalloc(&p->d_cbirth, nc * 4) != hipSuccess || hipMalloc(&p->d_cin, nc * p->in_w * 4) != hipSuccess ||
+        hipMalloc(&p->d_carr, nc * 4) != hipSuccess || hipMalloc(&p->d_tact, (size_t)p->ntiles * 4) != hipSuccess ||
         hipMalloc(&p->d_birth_off, (nc + 1) * 4) != hipSuccess || hipMalloc(&p->d_bpos, nc * 4) != hipSuccess ||
         hipMalloc(&p->d_rpos, nc * p->conn * 4) != hipSuccess ||
         hipMalloc(&p->d_tiles, (size_t)p->ntiles * 8) != hipSuccess ||
@@ -888,6 +897,8 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
     (void)hipMemsetAsync(p->d_cstay, 0, nc * 4, p->stream);
     (void)hipMemsetAsync(p->d_cbirth, 0, nc * 4, p->stream);
     (void)hipMemsetAsync(p->d_cin, 0, nc * p->in_w * 4, p->stream);
+    (void)hipMemsetAsync(p->d_carr, 0, nc * 4, p->stream);
+    (void)hipMemsetAsync(p->d_tact, 0, (size_t)p->ntiles * 4, p->stream);
     (void)hipMemsetAsync(p->d_st, 0, sizeof(DevStatus), p->stream);
     if (env) {
         const double *src[7] = {env->altitude, env->longitude, env->latitude, env->water,
@@ -922,7 +933,7 @@ int qhg_gpu_destroy(qhg_pop *p) {
     free_agentbuf(p->buf[1]);
     free_slot_scratch(p);
     void *ptrs[] = {p->d_off[0], p->d_off[1], p->d_gpool[0], p->d_gpool[1], p->d_nbr, p->d_nn, p->d_in_e,
-                    p->d_K, p->d_W, p->d_tmp, p->d_ice, p->d_coastal, p->d_cstay, p->d_cbirth, p->d_cin, p->d_outk,
+                    p->d_K, p->d_W, p->d_tmp, p->d_ice, p->d_coastal, p->d_cstay, p->d_cbirth, p->d_cin, p->d_carr, p->d_tact, p->d_outk,
                     p->d_birth_off, p->d_bpos, p->d_rpos, p->d_tiles, p->d_tiles2, p->d_occ[0], p->d_occ[1],
                     p->d_wave_part, p->d_st};
     for (void *q : ptrs)
@@ -1022,6 +1033,8 @@ int qhg_gpu_upload_agents(qhg_pop *p, const qhg_agents *a) {
     HIPCHK(hipMemsetAsync(p->d_cstay, 0, ncz * 4, p->stream));
     HIPCHK(hipMemsetAsync(p->d_cbirth, 0, ncz * 4, p->stream));
     HIPCHK(hipMemsetAsync(p->d_cin, 0, ncz * p->in_w * 4, p->stream));
+    HIPCHK(hipMemsetAsync(p->d_carr, 0, ncz * 4, p->stream));
+    HIPCHK(hipMemsetAsync(p->d_tact, 0, (size_t)p->ntiles * 4, p->stream));
     if (!occ.empty())
         HIPCHK(hipMemcpyAsync(p->d_occ[b], occ.data(), occ.size() * sizeof(OccCell), hipMemcpyHostToDevice,
                               p->stream));
@@ -1134,12 +1147,13 @@ int qhg_gpu_download_agents(qhg_pop *p, qhg_agents *a) {
     const AgentBuf &d = p->buf[b];
     std::vector<int64_t> id(n), mate(n);
     std::vector<int32_t> nbab(n), gh(n);
-    std::vector<uint32_t> off((size_t)p->ncells + 1);
+    std::vector<OccCell> occ((size_t)std::max<int64_t>(0, s.n_occ[b]));
     std::vector<uint8_t> life(n), gender(n);
     std::vector<double> birth(n), age(n), lastb(n);
     std::vector<uint64_t> pool;
     hipStream_t st = p->stream;
-    HIPCHK(hipMemcpyAsync(off.data(), p->d_off[b], off.size() * 4, hipMemcpyDeviceToHost, st));
+    if (!occ.empty())
+        HIPCHK(hipMemcpyAsync(occ.data(), p->d_occ[b], occ.size() * sizeof(OccCell), hipMemcpyDeviceToHost, st));
     if (n > 0) {
         HIPCHK(hipMemcpyAsync(id.data(), d.id, n * 8, hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(mate.data(), d.mate_id, n * 8, hipMemcpyDeviceToHost, st));
@@ -1158,10 +1172,15 @@ int qhg_gpu_download_agents(qhg_pop *p, qhg_agents *a) {
         }
     }
     HIPCHK(hipStreamSynchronize(st));
-    if ((int64_t)off[p->ncells] != n) return fail(QHG_ERR_STATE, "download: segment offsets disagree with the count");
-    std::vector<int32_t> cell(n);
-    for (int32_t c = 0; c < p->ncells; c++)
-        for (uint32_t i = off[c]; i < off[(size_t)c + 1]; i++) cell[i] = c;
+    // every agent's cell from the occupied-cell list (cell, first slot, count)
+    std::vector<int32_t> cell(n, -1);
+    int64_t covered = 0;
+    for (const OccCell &e : occ) {
+        if (e.c < 0 || e.c >= p->ncells || (int64_t)e.s + e.n > n) return fail(QHG_ERR_STATE, "download: bad cell list");
+        for (uint32_t i = e.s; i < e.s + e.n; i++) cell[i] = e.c;
+        covered += e.n;
+    }
+    if (covered != n) return fail(QHG_ERR_STATE, "download: the cell list disagrees with the count");
     std::vector<int64_t> order(n);
     std::iota(order.begin(), order.end(), 0);
     std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return id[x] < id[y]; });
@@ -1315,10 +1334,20 @@ int qhg_gpu_stats(qhg_pop *p, qhg_stats *o) {
 
 int qhg_gpu_cell_counts(qhg_pop *p, uint32_t *out) {
     if (!p || !out) return fail(QHG_ERR_ARG, "null argument");
-    std::vector<uint32_t> off((size_t)p->ncells + 1);
-    HIPCHK(hipMemcpyAsync(off.data(), p->d_off[p->cur()], off.size() * 4, hipMemcpyDeviceToHost, p->stream));
+    DevStatus s;
+    int rc = read_status_sync(p, &s);
+    if (rc) return rc;
+    const int b = p->cur();
+    std::vector<OccCell> occ((size_t)std::max<int64_t>(0, s.n_occ[b]));
+    if (!occ.empty())
+        HIPCHK(hipMemcpyAsync(occ.data(), p->d_occ[b], occ.size() * sizeof(OccCell), hipMemcpyDeviceToHost,
+                              p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
-    for (int32_t c = 0; c < p->ncells; c++) out[c] = off[(size_t)c + 1] - off[c];
+    std::memset(out, 0, (size_t)p->ncells * 4);
+    for (const OccCell &e : occ) {
+        if (e.c < 0 || e.c >= p->ncells) return fail(QHG_ERR_STATE, "cell counts: bad cell list");
+        out[e.c] = e.n;
+    }
     return 0;
 }
 
@@ -1476,7 +1505,7 @@ struct qhg_batch {
     uint64_t *d_tiles = nullptr, *d_tiles2 = nullptr;
     size_t tiles_cap = 0;
     int32_t total_tiles = 0;
-    bool any_genetics = false;
+    bool any_genetics = false, any_death = false;
     hipEvent_t ev[kRing] = {};
     int64_t ev_step[kRing];
     int64_t done_bstep = 0;        // every batch step below this one is complete
@@ -1549,7 +1578,8 @@ int batch_act_grid(const qhg_batch *b) { return b->num_cu * b->blocks_per_cu; }
 int batch_launch_step(qhg_batch *b) {
     const BatchArgs A = batch_args(b);
     const int ag = batch_act_grid(b);
-    int rc = blaunch(b, BK_ACT, k_batch_act, ag, kCellBlock, A);
+    int rc = b->any_death ? blaunch(b, BK_ACT, k_batch_act<true>, ag, kCellBlock, A)
+                          : blaunch(b, BK_ACT, k_batch_act<false>, ag, kCellBlock, A);
     if (rc) return rc;
     rc = blaunch(b, BK_SCAN, k_batch_scan, std::max(1, std::min(b->scan_grid, b->total_tiles)), kScanBlock, A);
     if (rc) return rc;
@@ -1573,7 +1603,7 @@ void batch_drop_graph(qhg_batch *b) {
 int batch_step(qhg_batch *b) {
     if (!b->use_graph || b->timing) return batch_launch_step(b);
     const int64_t key[4] = {b->table_version, (int64_t)(b->group_agents * 1024.0), b->blocks_per_cu,
-                            b->any_genetics ? 1 : 0};
+                            (b->any_genetics ? 1 : 0) | (b->any_death ? 2 : 0)};
     if (!b->graph_exec || !std::equal(key, key + 4, b->graph_key)) {
         batch_drop_graph(b);
         HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -1636,6 +1666,7 @@ int batch_upload_table(qhg_batch *b) {
     }
     std::vector<PopDesc> desc(std::max<size_t>(1, n));
     b->any_genetics = false;
+    b->any_death = false;
     for (size_t i = 0; i < n; i++) {
         qhg_pop *p = b->pops[i];
         for (int par = 0; par < 2; par++) {
@@ -1649,6 +1680,7 @@ int batch_upload_table(qhg_batch *b) {
         desc[i].t0 = b->t0[i];
         b->gens[i] = p->gen;
         if (p->gwords) b->any_genetics = true;
+        if (p->has_death) b->any_death = true;
     }
     if (n) HIPCHK(hipMemcpy(b->d_desc, desc.data(), n * sizeof(PopDesc), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b->d_tile_base, tile_base.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice));

@@ -20,6 +20,18 @@ roofline.traffic = its measured HBM bytes per launch (rocprofv3 counters).
 cpu_baseline: the CPU oracle (the reference algorithm restated in C with the
 same Philox draws, OpenMP) on the box's host cores, timed on the same steps,
 bounded to about 20 s.
+
+The same line carries two more objects:
+  ensemble  C4: 256 C2-style replicas (1,000 steps each) run one at a time
+            through the plain path and as members of one batch; value is
+            the batched agent-steps/s, speedup_vs_plain the ratio of the two
+            wall times, identical_to_plain the per-replica comparison.
+  serving   C5: a Poisson(20/s) stream of C2-style requests (400 by default,
+            --serve-requests) served by continuous batching under a 250 GB
+            budget; p50/p99 completion latency, aggregate agent-steps/s over
+            first arrival .. last completion, and a sample run alone.
+--workload c4 / c5 print those measurements as lines of their own (c5 with
+the full 2,000-request stream by default).
 """
 import argparse
 import json
@@ -38,7 +50,7 @@ HBM_PEAK_GBPS = 8000.0
 # what it reads for a 256 MiB device copy (FETCH x2, WRITE x1); see
 # profiles/step_kernels_c2_counters.md.  Reported as roofline.traffic only for
 # that exact workload.
-MEASURED_TRAFFIC = {("c2", 100, 1900): {"k_cell_act": 22.28e6, "k_cell_scatter": 41.93e6, "k_cell_scan": 7.14e6}}
+MEASURED_TRAFFIC = {("c2", 100, 1900): {"k_cell_act": 23.26e6, "k_cell_scatter": 41.87e6, "k_cell_scan": 4.21e6}}
 METRIC = "agent-steps/sec at 1 MI355X + %HBM peak; ensemble vs plain speedup, p50/p99"
 
 
@@ -101,29 +113,29 @@ C4_DESC = ("C4: ensemble of C2-style replicas (grids of 2,562-163,842 cells, 10k
            "plain path vs all as members of one batch")
 
 
-def run_c4(args):
-    """The ensemble leg: plain one-at-a-time vs batched, same scenarios,
-    same steps; the per-replica results must be identical."""
-    from qhg4_amd import abi, scenarios, ensemble
+def c4_leg(seed, n, steps, graph=True, tune=True, kernel_timing=True):
+    """The ensemble measurement: the same n replicas run one at a time
+    through the plain path and all together as members of one batch; the
+    per-replica results must be identical."""
+    from qhg4_amd import scenarios, ensemble
 
     def make(r):
-        return scenarios.c4_replica(r, base_seed=args.seed, steps=args.steps)
+        return scenarios.c4_replica(r, base_seed=seed, steps=steps)
 
-    n, steps = args.replicas, args.steps
     print("bench: C4 %d replicas x %d steps, plain path one at a time" % (n, steps), file=sys.stderr, flush=True)
     t_plain, plain = ensemble.run_plain(
         make, n, steps, log=lambda r, t: print("bench: plain %d done, %.2f s" % (r, t), file=sys.stderr, flush=True))
     print("bench: plain %.2f s; batched" % t_plain, file=sys.stderr, flush=True)
-    run = ensemble.BatchedRun(make, n, graph=not args.no_graph)
+    run = ensemble.BatchedRun(make, n, graph=graph)
     dev_bytes = run.device_bytes()
-    t_batch, tuning = run.run(steps, tune_steps=0 if args.no_tune else 2)
+    t_batch, tuning = run.run(steps, tune_steps=2 if tune else 0)
     batched = run.summaries()
     identical = batched == plain
     agent_steps = sum(s["agent_steps"] for s in batched)
     print("bench: batched %.2f s, identical=%s" % (t_batch, identical), file=sys.stderr, flush=True)
     # per-kernel device time over a few more steps, timed per dispatch
     roof, kernels = None, {}
-    if not args.no_kernel_timing:
+    if kernel_timing:
         ks = 5
         s0 = [p.stats() for p in run.pops]
         run.batch.set_timing(True)
@@ -140,21 +152,32 @@ def run_c4(args):
                 tot[kk] = tot.get(kk, 0) + v
         kernels, roof = kernel_table(tot, times)
     run.close()
+    return {"workload": C4_DESC, "replicas": n, "steps_per_replica": steps, "base_seed": seed,
+            "initial_agents_total": int(sum(scenarios.c4_draw(r, seed)["n0"] for r in range(n))),
+            "agent_steps_total": int(agent_steps), "device_bytes_resident": int(dev_bytes),
+            "value": round(agent_steps / t_batch, 1), "plain_value": round(agent_steps / t_plain, 1),
+            "batched_seconds": round(t_batch, 3), "plain_seconds": round(t_plain, 3),
+            "speedup_vs_plain": round(t_plain / t_batch, 3), "identical_to_plain": identical, "graph": graph,
+            "tuning": tuning, "kernels_after_run": kernels, "roofline_after_run": roof,
+            "make": make}
+
+
+def run_c4(args):
+    e = c4_leg(args.seed, args.replicas, args.steps, graph=not args.no_graph, tune=not args.no_tune,
+               kernel_timing=not args.no_kernel_timing)
+    make = e.pop("make")
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(make(0), 0, steps, args.cpu_seconds)
+        cpu = cpu_baseline(make(0), 0, args.steps, args.cpu_seconds)
         cpu["sample"] = "replica 0 of the ensemble: " + cpu["sample"]
-    speedup = t_plain / t_batch
-    out = base_line(args, agent_steps / t_batch, t_batch * 1e3 / steps, 0)
-    out["config"] = {"workload": C4_DESC, "replicas": n, "steps_per_replica": steps, "base_seed": args.seed,
-                     "initial_agents_total": int(sum(scenarios.c4_draw(r, args.seed)["n0"] for r in range(n))),
-                     "agent_steps_total": int(agent_steps), "device_bytes_resident": int(dev_bytes)}
-    out["ensemble"] = {"plain_value": round(agent_steps / t_plain, 1), "plain_seconds": round(t_plain, 3),
-                       "batched_seconds": round(t_batch, 3), "speedup_vs_plain": round(speedup, 3),
-                       "identical_to_plain": identical, "graph": not args.no_graph, "tuning": tuning}
-    out["roofline"] = roof
+    out = base_line(args, e["value"], e["batched_seconds"] * 1e3 / args.steps, 0)
+    out["config"] = {k: e[k] for k in ("workload", "replicas", "steps_per_replica", "base_seed",
+                                        "initial_agents_total", "agent_steps_total", "device_bytes_resident")}
+    out["ensemble"] = {k: e[k] for k in ("plain_value", "plain_seconds", "batched_seconds", "speedup_vs_plain",
+                                          "identical_to_plain", "graph", "tuning")}
+    out["roofline"] = e["roofline_after_run"]
     out["cpu_baseline"] = cpu
-    out["kernels"] = kernels
+    out["kernels"] = e["kernels_after_run"]
     return out
 
 
@@ -163,33 +186,32 @@ C5_DESC = ("C5: served stream of C2-style scenario requests (Poisson arrivals at
            "continuous batching under an HBM budget")
 
 
-def run_c5(args):
-    """The serving leg: requests arrive in real time and are served by
-    continuous batching; a sample of them is also run alone through the plain
-    path, for the plain rate and to check that their results are the same
-    alone and in the mix."""
+def c5_leg(seed, nreq, budget_gb=250.0, chunk=16, plain_sample=16, graph=True):
+    """The serving measurement: nreq requests arrive in real time and are
+    served by continuous batching; a sample of them is also run alone
+    through the plain path, for the plain rate and to check that their
+    results are the same alone and in the mix."""
     import numpy as np
     from qhg4_amd import scenarios, ensemble
     from qhg4_amd.server import ScenarioServer
 
-    reqs = scenarios.c5_requests(args.requests, base_seed=args.seed)
+    reqs = scenarios.c5_requests(nreq, base_seed=seed)
     for lv in sorted({r["level"] for r in reqs}):
-        scenarios.c4_world(lv, args.seed)          # the server's grids and fields, built before it starts
+        scenarios.c4_world(lv, seed)          # the server's grids and fields, built before it starts
 
     def make(req):
-        return scenarios.c5_scenario(req, world_seed=args.seed)
+        return scenarios.c5_scenario(req, world_seed=seed)
 
     def log(msg):
         print("bench: " + msg, file=sys.stderr, flush=True)
 
-    log("C5 %d requests, budget %.0f GB" % (len(reqs), args.budget_gb))
-    srv = ScenarioServer(make, budget_bytes=args.budget_gb * 1e9, chunk=args.chunk, graph=not args.no_graph)
+    log("C5 %d requests, budget %.0f GB" % (len(reqs), budget_gb))
+    srv = ScenarioServer(make, budget_bytes=budget_gb * 1e9, chunk=chunk, graph=graph)
     results, rep = srv.serve(reqs, log=log)
     log("served %d requests in %.1f s, p50 %.2f s p99 %.2f s" % (rep["requests"], rep["seconds"], rep["p50_s"],
                                                                  rep["p99_s"]))
-    # plain path on a sample: every k-th request by id
     by_id = {r["id"]: r for r in results}
-    sample = [reqs[i] for i in np.linspace(0, len(reqs) - 1, min(args.plain_sample, len(reqs))).astype(int)]
+    sample = [reqs[i] for i in np.linspace(0, len(reqs) - 1, min(plain_sample, len(reqs))).astype(int)]
     t_plain, steps_plain, same = 0.0, 0, True
     for q in sample:
         sec, summ = ensemble.run_plain(lambda _r, q=q: make(q), 1, q["horizon"])
@@ -198,22 +220,30 @@ def run_c5(args):
         same = same and summ[0] == by_id[q["id"]]["summary"]
     log("plain sample of %d: %.2f s, identical=%s" % (len(sample), t_plain, same))
     plain_value = steps_plain / t_plain if t_plain > 0 else None
+    return {"workload": C5_DESC, "requests": len(reqs), "budget_gb": budget_gb, "chunk_steps": chunk,
+            "base_seed": seed, "value": round(rep["value"], 1),
+            "p50_latency_s": round(rep["p50_s"], 3), "p99_latency_s": round(rep["p99_s"], 3),
+            "mean_latency_s": round(rep["mean_s"], 3), "seconds": rep["seconds"],
+            "arrival_window_s": round(reqs[-1]["arrival"], 3), "agent_steps": rep["agent_steps"],
+            "peak_members": rep["peak_members"], "peak_device_bytes": rep["peak_device_bytes"],
+            "tunings": rep["tunings"], "plain_sample": len(sample),
+            "plain_value": round(plain_value, 1) if plain_value else None,
+            "speedup_vs_plain": round(rep["value"] / plain_value, 3) if plain_value else None,
+            "sample_identical_alone_and_in_mix": same, "make": make, "first": reqs[0]}
+
+
+def run_c5(args):
+    e = c5_leg(args.seed, args.requests, args.budget_gb, args.chunk, args.plain_sample, graph=not args.no_graph)
+    make, first = e.pop("make"), e.pop("first")
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(make(reqs[0]), 0, reqs[0]["horizon"], args.cpu_seconds)
+        cpu = cpu_baseline(make(first), 0, first["horizon"], args.cpu_seconds)
         cpu["sample"] = "request 0 of the stream: " + cpu["sample"]
-    out = base_line(args, rep["value"], float("nan"), 0)
+    out = base_line(args, e["value"], float("nan"), 0)
     out["steps"] = None
     out["ms_per_step"] = None
-    out["config"] = {"workload": C5_DESC, "requests": len(reqs), "budget_gb": args.budget_gb,
-                     "chunk_steps": args.chunk, "base_seed": args.seed}
-    out["serving"] = {"p50_latency_s": round(rep["p50_s"], 3), "p99_latency_s": round(rep["p99_s"], 3),
-                      "mean_latency_s": round(rep["mean_s"], 3), "seconds": rep["seconds"],
-                      "agent_steps": rep["agent_steps"], "peak_members": rep["peak_members"],
-                      "peak_device_bytes": rep["peak_device_bytes"], "tunings": rep["tunings"],
-                      "plain_sample": len(sample), "plain_value": round(plain_value, 1) if plain_value else None,
-                      "speedup_vs_plain": round(rep["value"] / plain_value, 3) if plain_value else None,
-                      "sample_identical_alone_and_in_mix": same}
+    out["config"] = {k: e[k] for k in ("workload", "requests", "budget_gb", "chunk_steps", "base_seed")}
+    out["serving"] = {k: v for k, v in e.items() if k not in out["config"] and k != "value"}
     out["roofline"] = None
     out["cpu_baseline"] = cpu
     return out
@@ -308,6 +338,10 @@ def main():
     ap.add_argument("--chunk", type=int, default=16, help="C5: steps per batch chunk")
     ap.add_argument("--plain-sample", type=int, default=16, help="C5: requests also run alone")
     ap.add_argument("--replicas", type=int, default=256, help="C4: replicas in the ensemble")
+    ap.add_argument("--no-ensemble", action="store_true", help="C2 line without the C4 ensemble object")
+    ap.add_argument("--no-serving", action="store_true", help="C2 line without the C5 serving object")
+    ap.add_argument("--serve-requests", type=int, default=400,
+                    help="C5 requests for the serving object of the C2 line (the full stream: --workload c5)")
     ap.add_argument("--no-graph", action="store_true", help="C4: launch batch steps without graph replay")
     ap.add_argument("--no-tune", action="store_true", help="C4: keep the default launch shapes")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -393,6 +427,17 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kernels,
     }
+    if not args.no_ensemble:
+        e = c4_leg(1000, args.replicas, 1000, graph=not args.no_graph, tune=not args.no_tune,
+                   kernel_timing=not args.no_kernel_timing)
+        e.pop("make")
+        out["ensemble"] = e
+    if not args.no_serving:
+        e = c5_leg(2000, args.serve_requests, args.budget_gb, args.chunk, args.plain_sample,
+                   graph=not args.no_graph)
+        e.pop("make")
+        e.pop("first")
+        out["serving"] = e
     print(json.dumps(out))
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)

@@ -134,6 +134,7 @@ struct Ctx {
     uint64_t *tiles;          // look-back words of k_cell_scan: agents | births,
     uint64_t *tiles2;         //   and occupied cells
     int32_t ntiles;
+    uint32_t *tick;           // k_cell_scan's tile tickets (zeroed by k_cell_act)
     int32_t nact_blocks;      // grid of k_cell_act
     uint32_t *wave_part;      // kParts per k_cell_act block: deaths, moves, fertile
     uint64_t *gpool;          // genome pool, gwords u64 per handle
@@ -948,6 +949,7 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs
     if (blockIdx.x == 0 && lane == 0) {
         x.st->deaths_last = 0;
         x.st->moves_last = 0;
+        *x.tick = 0u;
     }
 
     // groups of gcells consecutive occupied cells, round-robin over the
@@ -1002,7 +1004,21 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs
 struct ScanLds {
     uint64_t w[kScanBlock / 64], w2[kScanBlock / 64];
     uint64_t excl, excl2;
+    uint32_t ticket;
 };
+
+// The next tile for this block.  Tiles are handed out in increasing order by
+// an atomic counter, so every tile a block waits on in the look-back has
+// been taken by a block that is already running: the scan makes progress
+// however few of its blocks are resident (other streams' kernels may hold
+// compute units meanwhile).
+__device__ __forceinline__ uint32_t next_tile(uint32_t *ticket, ScanLds &L) {
+    if (threadIdx.x == 0) L.ticket = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t t = (uint32_t)__builtin_amdgcn_readfirstlane((int)L.ticket);
+    __syncthreads();
+    return t;
+}
 
 // One look-back tile of k_cell_scan (kCellTile cells), all kScanBlock threads.
 template <typename CtxT>
@@ -1128,9 +1144,8 @@ __global__ void __launch_bounds__(kScanBlock) k_cell_scan(Ctx x, StepArgs sa) {
     __shared__ ScanLds lds;
     const int lane = threadIdx.x & 63;
     if (x.st->error) return;
-    // tiles in increasing order per block: a block only ever waits on tiles
-    // that resident blocks hold, whatever the grid size
-    for (uint32_t tile = blockIdx.x; tile < (uint32_t)x.ntiles; tile += gridDim.x) scan_tile(x, sa, tile, lds);
+    for (uint32_t tile = next_tile(x.tick, lds); tile < (uint32_t)x.ntiles; tile = next_tile(x.tick, lds))
+        scan_tile(x, sa, tile, lds);
     {
         // off the critical path: this block's slice of the k_cell_act
         // partials (deaths, moves, FERTILE counts), one atomic each
@@ -1635,6 +1650,7 @@ struct BatchArgs {
     uint64_t *tiles;          // look-back words of all members (both words)
     uint64_t *tiles2;
     int64_t *bctr;            // BC_COUNT copies of the batch step index
+    uint32_t *ticket;         // k_batch_scan's tile tickets (zeroed by k_batch_act)
     double group_agents;      // target agents per wave group
     int32_t genome;           // k_batch_genome runs in this step
     int32_t pad;
@@ -1769,11 +1785,12 @@ __global__ void __launch_bounds__(kCellBlock) k_batch_act(BatchArgs B) {
     __shared__ ActLds lds;
     const int lane = threadIdx.x;
     const int64_t bstep = batch_step_in(B, BC_ACT, BC_SCAN);
-    // every member's look-back words start at zero
+    // every member's look-back words and the scan's tile tickets start at zero
     for (int32_t i = blockIdx.x * kCellBlock + lane; i < B.total_tiles; i += gridDim.x * kCellBlock) {
         B.tiles[i] = 0;
         B.tiles2[i] = 0;
     }
+    if (blockIdx.x == 0 && lane == 0) *B.ticket = 0u;
     int64_t cnt[4], base[4];
     batch_groups(B, bstep, cnt);
     const int64_t total = pop_prefix(cnt, base);
@@ -1842,7 +1859,7 @@ __global__ void __launch_bounds__(kScanBlock) k_batch_scan(BatchArgs B) {
         const int r = lane + 64 * k;
         tb[k] = (r < B.npops) ? B.tile_base[r] : 0;
     }
-    for (int64_t t = blockIdx.x; t < B.total_tiles; t += gridDim.x) {
+    for (int64_t t = next_tile(B.ticket, lds); t < B.total_tiles; t = next_tile(B.ticket, lds)) {
         const int r = pop_of(tb, B.npops, t);
         const int64_t lt = t - lane_value(tb, r);
         const DevStatus *st = pop_desc(B, r).ctx[0].st;

@@ -121,6 +121,7 @@ struct qhg_pop {
     double *d_env[7] = {};   // alt lon lat water npp temp rain
     uint32_t *d_cstay = nullptr, *d_cbirth = nullptr, *d_cin = nullptr, *d_carr = nullptr, *d_birth_off = nullptr;
     uint32_t *d_tact = nullptr;   // per scan tile
+    uint32_t *d_tick = nullptr;   // the scan's tile tickets
     int32_t *d_outk = nullptr;
     uint32_t *d_bpos = nullptr, *d_rpos = nullptr;
     uint64_t *d_tiles = nullptr, *d_tiles2 = nullptr;
@@ -199,6 +200,7 @@ struct qhg_pop {
         x.cin = d_cin;
         x.carr = d_carr;
         x.tact = d_tact;
+        x.tick = d_tick;
         x.outk = d_outk;
         x.birth_off = d_birth_off;
         x.bpos = d_bpos;
@@ -878,6 +880,7 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
         hipMalloc(&p->d_off[1], (nc + 1) * 4) != hipSuccess || hipMalloc(&p->d_cstay, nc * 4) != hipSuccess ||
         hipMalloc(&p->d_cbirth, nc * 4) != hipSuccess || hipMalloc(&p->d_cin, nc * p->in_w * 4) != hipSuccess ||
         hipMalloc(&p->d_carr, nc * 4) != hipSuccess || hipMalloc(&p->d_tact, (size_t)p->ntiles * 4) != hipSuccess ||
+        hipMalloc(&p->d_tick, 4) != hipSuccess ||
         hipMalloc(&p->d_birth_off, (nc + 1) * 4) != hipSuccess || hipMalloc(&p->d_bpos, nc * 4) != hipSuccess ||
         hipMalloc(&p->d_rpos, nc * p->conn * 4) != hipSuccess ||
         hipMalloc(&p->d_tiles, (size_t)p->ntiles * 8) != hipSuccess ||
@@ -899,6 +902,7 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
     (void)hipMemsetAsync(p->d_cin, 0, nc * p->in_w * 4, p->stream);
     (void)hipMemsetAsync(p->d_carr, 0, nc * 4, p->stream);
     (void)hipMemsetAsync(p->d_tact, 0, (size_t)p->ntiles * 4, p->stream);
+    (void)hipMemsetAsync(p->d_tick, 0, 4, p->stream);
     (void)hipMemsetAsync(p->d_st, 0, sizeof(DevStatus), p->stream);
     if (env) {
         const double *src[7] = {env->altitude, env->longitude, env->latitude, env->water,
@@ -933,7 +937,7 @@ int qhg_gpu_destroy(qhg_pop *p) {
     free_agentbuf(p->buf[1]);
     free_slot_scratch(p);
     void *ptrs[] = {p->d_off[0], p->d_off[1], p->d_gpool[0], p->d_gpool[1], p->d_nbr, p->d_nn, p->d_in_e,
-                    p->d_K, p->d_W, p->d_tmp, p->d_ice, p->d_coastal, p->d_cstay, p->d_cbirth, p->d_cin, p->d_carr, p->d_tact, p->d_outk,
+                    p->d_K, p->d_W, p->d_tmp, p->d_ice, p->d_coastal, p->d_cstay, p->d_cbirth, p->d_cin, p->d_carr, p->d_tact, p->d_tick, p->d_outk,
                     p->d_birth_off, p->d_bpos, p->d_rpos, p->d_tiles, p->d_tiles2, p->d_occ[0], p->d_occ[1],
                     p->d_wave_part, p->d_st};
     for (void *q : ptrs)
@@ -961,11 +965,12 @@ int qhg_gpu_upload_agents(qhg_pop *p, const qhg_agents *a) {
         if (a->id[i] < 0) return fail(QHG_ERR_ARG, "upload: negative agent id");
         order.push_back(i);
     }
-    // cell-sorted layout: by cell, then id
-    std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+    // cell-sorted layout: by cell, then id (input already in that order is kept)
+    auto before = [&](int64_t x, int64_t y) {
         if (a->cell[x] != a->cell[y]) return a->cell[x] < a->cell[y];
         return a->id[x] < a->id[y];
-    });
+    };
+    if (!std::is_sorted(order.begin(), order.end(), before)) std::sort(order.begin(), order.end(), before);
     {
         std::vector<int64_t> ids;
         ids.reserve(order.size());
@@ -1510,6 +1515,7 @@ struct qhg_batch {
     int64_t ev_step[kRing];
     int64_t done_bstep = 0;        // every batch step below this one is complete
     int64_t *d_bctr = nullptr;     // the kernels' copies of the batch step index
+    uint32_t *d_ticket = nullptr;  // the batch scan's tile tickets
     // launch shapes (results do not depend on them)
     double group_agents = QHG_GROUP_AGENTS;
     int blocks_per_cu = QHG_CELL_GRID_PER_CU;
@@ -1567,6 +1573,7 @@ BatchArgs batch_args(const qhg_batch *b) {
     A.tiles = b->d_tiles;
     A.tiles2 = b->d_tiles2;
     A.bctr = b->d_bctr;
+    A.ticket = b->d_ticket;
     A.group_agents = b->group_agents;
     A.genome = b->any_genetics ? 1 : 0;
     return A;
@@ -1715,7 +1722,9 @@ int qhg_batch_create(qhg_batch **out) {
         b->ev_step[k] = -1;
     }
     if (hipMalloc(&b->d_bctr, BC_COUNT * sizeof(int64_t)) != hipSuccess ||
-        hipMemset(b->d_bctr, 0, BC_COUNT * sizeof(int64_t)) != hipSuccess) {
+        hipMemset(b->d_bctr, 0, BC_COUNT * sizeof(int64_t)) != hipSuccess ||
+        hipMalloc(&b->d_ticket, sizeof(uint32_t)) != hipSuccess ||
+        hipMemset(b->d_ticket, 0, sizeof(uint32_t)) != hipSuccess) {
         qhg_batch_destroy(b);
         return fail(QHG_ERR_NOMEM, "batch counters");
     }
@@ -1733,7 +1742,7 @@ int qhg_batch_destroy(qhg_batch *b) {
     while (!b->pops.empty()) (void)qhg_batch_remove(b, b->pops.back());
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     batch_drop_graph(b);
-    void *ptrs[] = {b->d_desc, b->d_tile_base, b->d_tiles, b->d_tiles2, b->d_bctr};
+    void *ptrs[] = {b->d_desc, b->d_tile_base, b->d_tiles, b->d_tiles2, b->d_bctr, b->d_ticket};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     for (int k = 0; k < kRing; k++)

@@ -263,6 +263,10 @@ def scenario_from_draw(d, world_seed):
     rng = np.random.default_rng(d["seed"] + 7)
     cells = patch[rng.integers(0, len(patch), d["n0"])]
     agents = _agents(rng, cells, d["n0"], abi.QHG_LIFE_ALIVE, -1.0)
+    # listed by cell (ids ascending within a cell): the same agents, in the
+    # order the device keeps them, so the upload has nothing to sort
+    order = np.argsort(agents["cell"], kind="stable")
+    agents = {k: v[order] for k, v in agents.items()}
     params = ooa_params(d["seed"], wmove_prob=d["wmove_prob"], b0=d["b0"], d0=d["d0"], k_scale=d["k_scale"])
     return {"grid": grid, "env": env, "params": params, "agents": agents, "draw": d}
 

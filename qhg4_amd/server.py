@@ -9,8 +9,9 @@ is one launch per phase for all members, so the runs overlap on the device.
 A member leaves the batch when it reaches its horizon; its final statistics
 are read back and its device memory is freed.
 
-While the device works on a chunk, the host admits and creates the next runs
-on their own streams; they join the batch between chunks.  Every run ends in
+Admission reserves the request's estimated device memory; a worker thread
+then builds its initial state, creates the population on its own stream and
+uploads it while the device works on the batch; it joins between chunks.  Every run ends in
 exactly the state it reaches alone (the batch kernels use each member's own
 Philox key and step counter).
 
@@ -28,6 +29,7 @@ from .population import GpuBatch, GpuPopulation
 
 # device bytes of a population without genetics: agent buffers (two of 46 B
 # per slot) and slot scratch (38 B); per-cell tables at most 340 B per cell
+# (the population reports its exact figure once it exists)
 PER_SLOT_BYTES = 130
 PER_CELL_BYTES = 340
 FIXED_BYTES = 1 << 20
@@ -39,13 +41,23 @@ def estimate_bytes(sc):
     return capacity_for(sc) * PER_SLOT_BYTES + int(sc["grid"]["num_cells"]) * PER_CELL_BYTES + FIXED_BYTES
 
 
+def estimate_request(req):
+    """Device bytes of a C5-style request (its "n0" agents on a level-"level"
+    icosahedral grid) before its state exists."""
+    ncells = 10 * 4 ** int(req["level"]) + 2
+    return (2 * int(req["n0"]) + 65536) * PER_SLOT_BYTES + ncells * PER_CELL_BYTES + FIXED_BYTES
+
+
 class ScenarioServer:
-    def __init__(self, make, budget_bytes=250e9, max_members=MAX_MEMBERS, chunk=16, workers=8, graph=True,
-                 tune_every_s=60.0, clock=time.perf_counter):
-        """make(request) -> scenario dict (runs on a worker thread)."""
+    def __init__(self, make, estimate=estimate_request, budget_bytes=250e9, max_members=MAX_MEMBERS, chunk=16,
+                 workers=8, graph=True, tune_every_s=60.0, clock=time.perf_counter):
+        """make(request) -> scenario dict; estimate(request) -> device bytes
+        its state will take (reserved at admission).  Both run before the
+        request's state exists; make runs on a worker thread."""
         if not 1 <= max_members <= MAX_MEMBERS:
             raise ValueError("max_members must be in [1, %d]" % MAX_MEMBERS)
         self.make = make
+        self.estimate = estimate
         self.budget = float(budget_bytes)
         self.max_members = max_members
         self.chunk = int(chunk)
@@ -54,75 +66,91 @@ class ScenarioServer:
         self.tune_every_s = tune_every_s
         self.clock = clock
 
+    def _prepare(self, req):
+        """Worker thread: the request's scenario, its population on the
+        device (uploaded, pre-loop done), ready to join the batch."""
+        sc = self.make(req)
+        pop = GpuPopulation.from_scenario(sc, capacity=capacity_for(sc))
+        pop.preLoop(0.0)
+        return pop
+
     def serve(self, requests, log=None):
         """Serve requests (dicts with id, arrival [s from start], horizon)
         in real time.  Returns (results, report)."""
         pending = collections.deque(sorted(requests, key=lambda r: r["arrival"]))
-        waiting = collections.deque()       # (request, future scenario)
-        active = {}                         # pop -> [request, steps left, admitted]
-        joining = []                        # created, not yet members
+        waiting = collections.deque()       # arrived, not yet admitted
+        preparing = []                      # [request, future population, reserved bytes, admitted at]
+        active = {}                         # pop -> [request, steps left, admitted at, joined at]
         results = []
         batch = GpuBatch()
         batch.set_graph(self.graph)
         pool = cf.ThreadPoolExecutor(self.workers)
         t0 = self.clock()
-        used = 0
+        used = 0                            # device bytes of the members
         peak_used = 0
         peak_members = 0
         last_tune = None
         tunings = []
         last_log = t0
         try:
-            while pending or waiting or active or joining:
+            while pending or waiting or preparing or active:
                 now = self.clock() - t0
                 while pending and pending[0]["arrival"] <= now:
-                    r = pending.popleft()
-                    waiting.append((r, pool.submit(self.make, r)))
-                # members join between chunks
-                for pop, r, adm in joining:
+                    waiting.append(pending.popleft())
+                # admission in arrival order while slots and the budget allow
+                reserved = sum(j[2] for j in preparing)
+                while waiting and len(active) + len(preparing) < self.max_members:
+                    need = int(self.estimate(waiting[0]))
+                    if used + reserved + need > self.budget and (active or preparing):
+                        break
+                    r = waiting.popleft()
+                    preparing.append([r, pool.submit(self._prepare, r), need, self.clock() - t0])
+                    reserved += need
+                # prepared populations join between chunks
+                for job in [j for j in preparing if j[1].done()]:
+                    preparing.remove(job)
+                    pop = job[1].result()
                     batch.add(pop)
-                    active[pop] = [r, int(r["horizon"]), adm]
-                joining = []
+                    active[pop] = [job[0], int(job[0]["horizon"]), job[3], self.clock() - t0]
+                    used += pop.device_bytes()
                 peak_members = max(peak_members, len(active))
+                peak_used = max(peak_used, used + sum(j[2] for j in preparing))
                 if not active:
-                    if not self._admit(waiting, joining, active, used, t0):
-                        self._idle(pending, waiting, t0)
-                    used = self._used(active, joining)
+                    self._idle(pending, preparing, t0)
                     continue
                 left = min(e[1] for e in active.values())
                 tuned = False
                 if (self.tune_every_s is not None and left >= 40 and len(active) >= 8 and
                         (last_tune is None or now - last_tune >= self.tune_every_s)):
-                    tuning, taken = batch.autotune(2)
+                    tuning, k = batch.autotune(2)
                     tunings.append(dict(tuning, at_s=round(now, 3), members=len(active)))
                     last_tune = now
-                    k = taken
                     tuned = True
                 else:
                     k = min(self.chunk, left)
                     batch.run(k)
-                # while the device works: admit what fits
-                self._admit(waiting, joining, active, used, t0)
                 batch.sync()
                 for e in active.values():
                     e[1] -= k
                 done_at = self.clock() - t0
                 for pop in [p for p, e in active.items() if e[1] == 0]:
-                    r, _, adm = active.pop(pop)
+                    r, _, adm, joined = active.pop(pop)
                     batch.remove(pop)
                     s = summary(pop.stats())
+                    used -= pop.device_bytes()
                     pop.close()
-                    results.append({"id": r["id"], "arrival": r["arrival"], "admitted": adm, "done": done_at,
-                                    "latency": done_at - r["arrival"], "summary": s})
-                used = self._used(active, joining)
-                peak_used = max(peak_used, used)
+                    results.append({"id": r["id"], "arrival": r["arrival"], "admitted": adm, "joined": joined,
+                                    "done": done_at, "latency": done_at - r["arrival"], "summary": s})
                 if log and (self.clock() - last_log > 10.0 or tuned):
                     last_log = self.clock()
-                    log("t=%.1fs done %d running %d queued %d used %.1f GB" %
-                        (self.clock() - t0, len(results), len(active), len(waiting), used / 1e9))
+                    log("t=%.1fs done %d running %d preparing %d queued %d used %.1f GB" %
+                        (self.clock() - t0, len(results), len(active), len(preparing), len(waiting), used / 1e9))
         finally:
-            for pop, _, _ in joining:
-                pop.close()
+            for job in preparing:
+                try:
+                    job[1].result().close()
+                except Exception:
+                    pass
             for pop in list(active):
                 pop.close()
             batch.close()
@@ -131,33 +159,11 @@ class ScenarioServer:
         return results, self._report(results, t_end, peak_used, peak_members, tunings)
 
     # -- helpers ------------------------------------------------------------
-    def _admit(self, waiting, joining, active, used, t0):
-        """Create populations for the waiting requests, in arrival order,
-        while slots and budget allow.  Returns whether any was created."""
-        made = False
-        while waiting and waiting[0][1].done() and len(active) + len(joining) < self.max_members:
-            r, fut = waiting[0]
-            sc = fut.result()
-            need = estimate_bytes(sc)
-            if used + need > self.budget and (active or joining):
-                break
-            waiting.popleft()
-            pop = GpuPopulation.from_scenario(sc, capacity=capacity_for(sc))
-            pop.preLoop(0.0)
-            joining.append((pop, r, self.clock() - t0))
-            used += pop.device_bytes()
-            made = True
-        return made
-
-    def _idle(self, pending, waiting, t0):
-        if waiting:
-            cf.wait([waiting[0][1]], timeout=0.05)
+    def _idle(self, pending, preparing, t0):
+        if preparing:
+            cf.wait([j[1] for j in preparing], timeout=0.05, return_when=cf.FIRST_COMPLETED)
         elif pending:
             time.sleep(max(0.0, min(0.05, pending[0]["arrival"] - (self.clock() - t0))))
-
-    @staticmethod
-    def _used(active, joining):
-        return sum(p.device_bytes() for p in active) + sum(p.device_bytes() for p, _, _ in joining)
 
     @staticmethod
     def _report(results, t_end, peak_used, peak_members, tunings):

@@ -86,6 +86,10 @@ def fake_scenario(req):
     return {"grid": {"num_cells": 1000}, "agents": {"id": np.arange(n)}}
 
 
+def fake_estimate(req):
+    return server.estimate_bytes(fake_scenario(req))
+
+
 @pytest.fixture
 def fakes(monkeypatch):
     made = []
@@ -110,8 +114,8 @@ def requests(n, seed=1):
 @pytest.mark.parametrize("max_members,budget_mb,chunk", [(4, 1e6, 16), (256, 60.0, 8), (3, 40.0, 1)])
 def test_every_request_runs_exactly_its_horizon(fakes, max_members, budget_mb, chunk):
     reqs = requests(60)
-    srv = server.ScenarioServer(fake_scenario, budget_bytes=budget_mb * 1e6, max_members=max_members,
-                                chunk=chunk, workers=2, tune_every_s=None, clock=Clock())
+    srv = server.ScenarioServer(fake_scenario, estimate=fake_estimate, budget_bytes=budget_mb * 1e6,
+                                max_members=max_members, chunk=chunk, workers=2, tune_every_s=None, clock=Clock())
     res, rep = srv.serve(reqs)
     assert sorted(r["id"] for r in res) == list(range(60))
     by = {r["id"]: r for r in res}
@@ -120,9 +124,10 @@ def test_every_request_runs_exactly_its_horizon(fakes, max_members, budget_mb, c
         assert r["summary"]["step"] == q["horizon"]
         assert r["admitted"] >= q["arrival"]
         assert r["done"] >= r["admitted"]
-    # first come, first admitted
+    # first come, first admitted (joining the batch follows preparation)
     adm = [by[q["id"]]["admitted"] for q in sorted(reqs, key=lambda q: q["arrival"])]
     assert adm == sorted(adm)
+    assert all(by[q["id"]]["joined"] >= by[q["id"]]["admitted"] for q in reqs)
     assert fakes[0].max_seen <= max_members
     one = max(server.estimate_bytes(fake_scenario(q)) for q in reqs)
     assert rep["peak_device_bytes"] <= max(budget_mb * 1e6, one)
@@ -132,8 +137,8 @@ def test_every_request_runs_exactly_its_horizon(fakes, max_members, budget_mb, c
 
 def test_autotune_never_overshoots_a_horizon(fakes):
     reqs = [dict(q, arrival=0.0, horizon=1000 + 7 * q["id"]) for q in requests(12)]
-    srv = server.ScenarioServer(fake_scenario, budget_bytes=1e12, max_members=16, chunk=16, workers=2,
-                                tune_every_s=0.0, clock=Clock())
+    srv = server.ScenarioServer(fake_scenario, estimate=fake_estimate, budget_bytes=1e12, max_members=16, chunk=16,
+                                workers=2, tune_every_s=0.0, clock=Clock())
     res, rep = srv.serve(reqs)
     assert fakes[0].tuned >= 1 and rep["tunings"]
     for r in res:

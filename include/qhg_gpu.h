@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define QHG_ABI_VERSION 5
+#define QHG_ABI_VERSION 6
 
 /* error codes */
 #define QHG_OK              0
@@ -232,6 +232,25 @@ int qhg_gpu_run(qhg_pop *pop, int32_t nsteps);
  * :164-209, NPPCapacity.cpp:121-131) */
 int qhg_gpu_set_env(qhg_pop *pop, const qhg_env_desc *env);
 
+/* Environment events between steps (Simulator::runLoop's checkEvents ->
+ * updateEvent -> flushEvents, QHG4/app/Simulator.cpp:335-374): the fields
+ * the env descriptor holds (non-NULL) replace the current ones; with
+ * QHG_EVENT_KILL_WET every agent in a cell with altitude < 0 or ice is
+ * removed at once, the geography-event rule of OoAPop::updateEvent
+ * (QHG4/populations/OoAPop.cpp:164-196), OoANavGen2bitPop::updateEvent
+ * (QHG4/populations/OoANavGen2bitPop.cpp:177-210) and
+ * tut_EnvironCapAltPop::updateEvent (QHG4/populations/tut_EnvironCapAltPop.cpp
+ * :120-150); then capacities and evaluator weights are recomputed (the
+ * flush, NPPCapacity.cpp:121-131, MultiEvaluator.cpp:142-210).  The removed
+ * agents are not step deaths (stats count none).  Event bits follow
+ * QHG4/utils_qhg/EventConsts.h; qhg_params_event_flags adds KILL_WET for the
+ * classes whose updateEvent applies it. */
+#define QHG_EVENT_GEO       0x01
+#define QHG_EVENT_CLIMATE   0x02
+#define QHG_EVENT_VEG       0x04
+#define QHG_EVENT_KILL_WET  0x100
+int qhg_gpu_env_event(qhg_pop *pop, const qhg_env_desc *env, int32_t events);
+
 /* observables */
 int qhg_gpu_sync(qhg_pop *pop);
 int qhg_gpu_stats(qhg_pop *pop, qhg_stats *out);
@@ -320,6 +339,12 @@ int qhg_params_set_attribute(qhg_params *p, const char *name, double value);
  * QHG4/actions/Verhulst.cpp:235-254) for attributes read every step; the
  * capacity and evaluator attributes are refused. */
 int qhg_gpu_modify_attribute(qhg_pop *pop, const char *name, double value);
+/* The event bits a class reacts to: events | QHG_EVENT_KILL_WET when events
+ * holds QHG_EVENT_GEO and the class drowns agents on geography events
+ * (tut_EnvironCapAltPop, OoAPop, OoANavGen2bitPop); events unchanged for
+ * the others (SPopulation::updateEvent does nothing,
+ * QHG4/core/SPopulation.h:116); negative for an unknown class. */
+int qhg_params_event_flags(const char *pop_class, int32_t events);
 
 /* Batched stepping.  No counterpart in the reference (QHG4 steps one
  * population per process, PopLooper.cpp:166-202); it serves callers that run

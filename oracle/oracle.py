@@ -43,6 +43,7 @@ def lib():
         L.orc_get_agents.argtypes = [C.c_void_p, P(abi.Agents)]
         L.orc_pre_loop.argtypes = [C.c_void_p]
         L.orc_set_params.argtypes = [C.c_void_p, P(abi.Params)]
+        L.orc_env_event.argtypes = [C.c_void_p, P(abi.EnvDesc), C.c_int32]
         L.orc_step.argtypes = [C.c_void_p]
         L.orc_run.argtypes = [C.c_void_p, C.c_int32]
         L.orc_stats.argtypes = [C.c_void_p, P(abi.Stats)]
@@ -113,6 +114,14 @@ class Oracle:
         if lib().orc_set_params(self.h, C.byref(params)) != 0:
             raise RuntimeError("orc_set_params: the action set differs")
         self.params = params
+
+    def env_event(self, env, events):
+        """Environment event between steps (see orc_env_event)."""
+        self.env = dict(getattr(self, "env", self.sc["env"]))
+        self.env.update(env)
+        self.edesc = env_desc(self.env)
+        if lib().orc_env_event(self.h, C.byref(self.edesc), int(events)) != 0:
+            raise RuntimeError("orc_env_event failed")
 
     def step(self):
         lib().orc_step(self.h)

@@ -593,6 +593,61 @@ int orc_set_params(orc *o, const qhg_params *p) {
     return 0;
 }
 
+/* Environment events between steps: replace the given fields; with
+ * QHG_EVENT_KILL_WET remove every agent in a cell with altitude < 0 or ice
+ * (OoAPop::updateEvent, QHG4/populations/OoAPop.cpp:164-196: registerDeath
+ * then performDeaths before the next step; not counted as step deaths);
+ * then recompute capacities and evaluator weights (the flush,
+ * QHG4/actions/NPPCapacity.cpp:121-131, MultiEvaluator.cpp:142-210). */
+static void replace_field(double **dst, const double *src, size_t n) {
+    if (!src) return;
+    if (!*dst) *dst = (double *)malloc(n * 8);
+    memcpy(*dst, src, n * 8);
+}
+
+static void replace_flags(uint8_t **dst, const uint8_t *src, size_t n) {
+    if (!src) return;
+    if (!*dst) *dst = (uint8_t *)malloc(n);
+    memcpy(*dst, src, n);
+}
+
+int orc_env_event(orc *o, const qhg_env_desc *env, int32_t events) {
+    if (!env) return QHG_ERR_ARG;
+    size_t nc = (size_t)o->ncells;
+    replace_field(&o->alt, env->altitude, nc);
+    replace_field(&o->lon, env->longitude, nc);
+    replace_field(&o->lat, env->latitude, nc);
+    replace_field(&o->water, env->water, nc);
+    replace_field(&o->npp, env->base_npp, nc);
+    replace_field(&o->temp, env->temperature, nc);
+    replace_field(&o->rain, env->rainfall, nc);
+    replace_flags(&o->ice, env->ice, nc);
+    replace_flags(&o->coastal, env->coastal, nc);
+    if (events & QHG_EVENT_KILL_WET) {
+        int64_t w = 0;
+        for (int64_t i = 0; i < o->n; i++) {
+            int32_t c = o->cell[i];
+            int wet = (o->alt && o->alt[c] < 0) || (o->ice && o->ice[c] > 0);
+            if (wet || o->life[i] == 0) continue;
+            if (w != i) {
+                o->id[w] = o->id[i]; o->cell[w] = o->cell[i]; o->mate[w] = o->mate[i];
+                o->nbabies[w] = o->nbabies[i]; o->life[w] = o->life[i]; o->gender[w] = o->gender[i];
+                o->birth[w] = o->birth[i]; o->age[w] = o->age[i]; o->lastb[w] = o->lastb[i];
+                if (o->gw) memmove(o->genome + w * o->gw, o->genome + i * o->gw, (size_t)o->gw * 8);
+            }
+            w++;
+        }
+        o->n = w;
+        /* mate indices of the last pairing no longer hold: read back as
+         * unpaired until the next step pairs again */
+        for (int64_t i = 0; i < o->n; i++) o->mate[i] = -3;
+        update_counts(o);
+    }
+    if (has_action(o, QHG_ACT_NPPCAP)) npp_capacity(o);
+    if (has_action(o, QHG_ACT_MULTIEVAL)) { multi_eval(o); o->me_first = 0; }
+    return 0;
+}
+
 int orc_pre_loop(orc *o) {
     if (has_action(o, QHG_ACT_NPPCAP)) npp_capacity(o);
     update_counts(o);

@@ -30,6 +30,7 @@ int64_t  orc_num_alive(orc *o);
 int      orc_get_agents(orc *o, qhg_agents *a);
 int      orc_pre_loop(orc *o);
 int      orc_set_params(orc *o, const qhg_params *p);
+int      orc_env_event(orc *o, const qhg_env_desc *env, int32_t events);
 int      orc_step(orc *o);
 int      orc_run(orc *o, int32_t nsteps);
 void     orc_stats(orc *o, qhg_stats *s);

@@ -17,13 +17,13 @@ EXPORTS = [
     "qhg_gpu_abi_version", "qhg_gpu_last_error", "qhg_gpu_genome_words", "qhg_gpu_binomial_table",
     "qhg_gpu_create", "qhg_gpu_destroy", "qhg_gpu_upload_agents", "qhg_gpu_num_alive",
     "qhg_gpu_download_agents", "qhg_gpu_pre_loop", "qhg_gpu_initialize_step", "qhg_gpu_do_actions",
-    "qhg_gpu_finalize_step", "qhg_gpu_run", "qhg_gpu_set_env", "qhg_gpu_sync", "qhg_gpu_stats",
+    "qhg_gpu_finalize_step", "qhg_gpu_run", "qhg_gpu_set_env", "qhg_gpu_env_event", "qhg_gpu_sync", "qhg_gpu_stats",
     "qhg_gpu_cell_counts", "qhg_gpu_capacities", "qhg_gpu_env_weights", "qhg_gpu_compact",
     "qhg_gpu_set_timing", "qhg_gpu_kernel_times", "qhg_gpu_kernel_name", "qhg_gpu_device_bytes",
     "qhg_gpu_set_tuning", "qhg_gpu_get_tuning", "qhg_gpu_autotune",
     "qhg_params_from_modules", "qhg_params_get_attribute", "qhg_params_set_attribute", "qhg_gpu_modify_attribute",
     "qhg_gpu_set_clock", "qhg_gpu_get_clock", "qhg_gpu_get_params", "qhg_params_from_attribute_map",
-    "qhg_params_attribute_name", "qhg_params_weight_name", "qhg_params_polyline_name",
+    "qhg_params_attribute_name", "qhg_params_weight_name", "qhg_params_polyline_name", "qhg_params_event_flags",
     "qhg_batch_create", "qhg_batch_destroy", "qhg_batch_add", "qhg_batch_remove", "qhg_batch_size",
     "qhg_batch_run", "qhg_batch_sync", "qhg_batch_set_timing", "qhg_batch_kernel_times",
     "qhg_batch_kernel_name", "qhg_batch_set_graph", "qhg_batch_set_tuning", "qhg_batch_get_tuning",
@@ -60,6 +60,8 @@ def lib():
         "qhg_gpu_finalize_step": ([h], C.c_int),
         "qhg_gpu_run": ([h, C.c_int32], C.c_int),
         "qhg_gpu_set_env": ([h, P(abi.EnvDesc)], C.c_int),
+        "qhg_gpu_env_event": ([h, P(abi.EnvDesc), C.c_int32], C.c_int),
+        "qhg_params_event_flags": ([C.c_char_p, C.c_int32], C.c_int),
         "qhg_gpu_sync": ([h], C.c_int),
         "qhg_gpu_stats": ([h, P(abi.Stats)], C.c_int),
         "qhg_gpu_cell_counts": ([h, P(C.c_uint32)], C.c_int),

@@ -24,6 +24,10 @@ ACT_GENETICS = 10
 ACT_OLDAGEDEATH = 11
 ACT_ATANDEATH = 12
 
+# environment events (QHG4/utils_qhg/EventConsts.h) and the drowning rule
+EVENT_GEO, EVENT_CLIMATE, EVENT_VEG = 0x01, 0x02, 0x04
+EVENT_KILL_WET = 0x100
+
 # reference action names (Action<T>::getActionName), used for same-prio order
 ACTION_NAMES = {
     ACT_GETOLD: "GetOld",

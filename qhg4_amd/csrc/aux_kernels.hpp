@@ -130,4 +130,94 @@ __global__ void __launch_bounds__(kBlock) k_eval_combine(Ctx x, const double *tm
     }
 }
 
+// The geography-event kill of OoAPop::updateEvent
+// (QHG4/populations/OoAPop.cpp:164-196): every agent in a cell with
+// altitude < 0 or ice is removed before the next step.  The buffer is cell
+// sorted, so whole cells go.  Pass 1 (one block) lists the kept occupied
+// cells of buffer b, in order, in occ_dst with dense segment starts, their
+// old starts in src_s, and the resulting counts in the other parity's
+// status; pass 2 moves the kept agents into the other buffer, which then
+// takes the current buffer's role.
+constexpr int kKillBlock = 1024;
+
+__global__ void __launch_bounds__(kKillBlock) k_env_kill(const OccCell *occ, int64_t nocc, const double *alt,
+                                                         const uint8_t *ice, OccCell *occ_dst, uint32_t *src_s,
+                                                         DevStatus *st, int b) {
+    constexpr int kWaves = kKillBlock / kWave;
+    __shared__ uint32_t wkeep[kWaves], wslots[kWaves];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    int64_t out = 0;      // kept cells so far
+    uint32_t slots = 0;   // their agents
+    for (int64_t base = 0; base < nocc; base += kKillBlock) {
+        const int64_t i = base + threadIdx.x;
+        OccCell e{0, 0, 0, 0};
+        bool keep = false;
+        if (i < nocc) {
+            e = occ[i];
+            keep = !((alt && alt[e.c] < 0) || (ice && ice[e.c] > 0));
+        }
+        const uint32_t v = keep ? e.n : 0u;
+        uint32_t incl = v;
+        for (int d = 1; d < kWave; d <<= 1) {
+            const uint32_t y = __shfl_up(incl, d, kWave);
+            if (lane >= d) incl += y;
+        }
+        const uint64_t m = __ballot(keep);
+        const uint32_t below = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wkeep[w] = (uint32_t)__popcll(m);
+        if (lane == kWave - 1) wslots[w] = incl;
+        __syncthreads();
+        uint32_t pre = 0, total = 0, spre = 0, stotal = 0;
+        for (int k = 0; k < kWaves; k++) {
+            if (k < w) {
+                pre += wkeep[k];
+                spre += wslots[k];
+            }
+            total += wkeep[k];
+            stotal += wslots[k];
+        }
+        if (keep) {
+            const int64_t o = out + pre + below;
+            occ_dst[o] = OccCell{e.c, slots + spre + incl - v, e.n, 0u};
+            src_s[o] = e.s;
+        }
+        out += total;
+        slots += stotal;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int nb = 1 - b;
+        st->n_occ[nb] = out;
+        st->n_alive[nb] = slots;
+        st->max_id[nb] = st->max_id[b];
+        st->gpool[nb] = st->gpool[b];
+    }
+}
+
+// Pass 2: one wave per kept cell copies its agents.  The last pairing no
+// longer holds, so mate ids read back as unpaired (-3) until the next step
+// pairs again.
+__global__ void __launch_bounds__(kBlock) k_env_move(AgentBuf from, AgentBuf to, const OccCell *occ,
+                                                     const uint32_t *src_s, const DevStatus *st, int nb) {
+    const int64_t nocc = st->n_occ[nb];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t nwaves = (int64_t)gridDim.x * (kBlock / kWave);
+    for (int64_t k = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave; k < nocc; k += nwaves) {
+        const OccCell e = occ[k];
+        const uint32_t s = src_s[k];
+        for (uint32_t j = lane; j < e.n; j += kWave) {
+            const uint32_t f = s + j, t = e.s + j;
+            to.id[t] = from.id[f];
+            to.mate_id[t] = -3;
+            to.life[t] = from.life[f];
+            to.gender[t] = from.gender[f];
+            to.birth[t] = from.birth[f];
+            to.age[t] = from.age[f];
+            to.lastb[t] = from.lastb[f];
+            to.nbabies[t] = from.nbabies[f];
+            if (to.ghandle) to.ghandle[t] = from.ghandle[f];
+        }
+    }
+}
+
 }  // namespace qhg

@@ -423,4 +423,18 @@ int qhg_params_set_attribute(qhg_params *p, const char *name, double value) {
     return qhg_internal_fail(QHG_ERR_ARG, std::string("cannot modify attribute ") + name);
 }
 
+// The updateEvent rules of the classes: the three environment-driven ones
+// drown agents on geography events (OoAPop.cpp:164-196,
+// OoANavGen2bitPop.cpp:177-210, tut_EnvironCapAltPop.cpp:120-150);
+// tut_SexualPop keeps SPopulation's no-op (SPopulation.h:116).
+int qhg_params_event_flags(const char *pop_class, int32_t events) {
+    if (!pop_class) return qhg_internal_fail(QHG_ERR_ARG, "null class name");
+    for (const ClassSpec &c : classes()) {
+        if (std::strcmp(c.name, pop_class) != 0) continue;
+        const bool drowns = std::strcmp(c.name, "tut_SexualPop") != 0;
+        return (drowns && (events & QHG_EVENT_GEO)) ? (events | QHG_EVENT_KILL_WET) : events;
+    }
+    return qhg_internal_fail(QHG_ERR_ARG, std::string("unknown population class ") + pop_class);
+}
+
 }  // extern "C"

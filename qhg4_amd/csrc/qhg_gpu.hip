@@ -1272,8 +1272,14 @@ int qhg_gpu_run(qhg_pop *p, int32_t nsteps) {
     return 0;
 }
 
-int qhg_gpu_set_env(qhg_pop *p, const qhg_env_desc *env) {
+int qhg_gpu_set_env(qhg_pop *p, const qhg_env_desc *env) { return qhg_gpu_env_event(p, env, 0); }
+
+int qhg_gpu_env_event(qhg_pop *p, const qhg_env_desc *env, int32_t events) {
     if (!p || !env) return fail(QHG_ERR_ARG, "null argument");
+    if (events & ~(QHG_EVENT_GEO | QHG_EVENT_CLIMATE | QHG_EVENT_VEG | QHG_EVENT_KILL_WET))
+        return fail(QHG_ERR_ARG, "unknown event bits");
+    if ((events & QHG_EVENT_KILL_WET) && p->batch) return fail(QHG_ERR_STATE, "population is stepped by its batch");
+    if (p->acted || p->initialized) return fail(QHG_ERR_STATE, "population is inside a step");
     const size_t nc = (size_t)p->ncells;
     HIPCHK(hipStreamSynchronize(p->stream));
     const double *src[7] = {env->altitude, env->longitude, env->latitude, env->water,
@@ -1290,6 +1296,40 @@ int qhg_gpu_set_env(qhg_pop *p, const qhg_env_desc *env) {
     if (env->coastal) {
         if (!p->d_coastal) HIPCHK(hipMalloc(&p->d_coastal, nc));
         HIPCHK(hipMemcpyAsync(p->d_coastal, env->coastal, nc, hipMemcpyHostToDevice, p->stream));
+    }
+    if ((events & QHG_EVENT_KILL_WET) && (p->d_env[0] || p->d_ice)) {
+        poll_ring(p);
+        DevStatus st;
+        int rc = read_status_sync(p, &st);
+        if (rc) return rc;
+        const int b = p->cur();
+        if (st.n_occ[b] > 0) {
+            // d_tmp (ncells doubles) holds the kept cells' old segment starts
+            uint32_t *src_s = reinterpret_cast<uint32_t *>(p->d_tmp);
+            rc = launch_bs(p, KB_CAPACITY, k_env_kill, 1, kKillBlock, (const OccCell *)p->d_occ[b], st.n_occ[b],
+                           (const double *)p->d_env[0], (const uint8_t *)p->d_ice, p->d_occ[1 - b], src_s,
+                           p->d_st, b);
+            if (rc) return rc;
+            rc = launch(p, KB_CAPACITY, k_env_move, grid_for(st.n_occ[b] * kWave, 4096), p->buf[b], p->buf[1 - b],
+                        (const OccCell *)p->d_occ[1 - b], (const uint32_t *)src_s, (const DevStatus *)p->d_st,
+                        1 - b);
+            if (rc) return rc;
+            rc = read_status_sync(p, &st);
+            if (rc) return rc;
+            // the repacked buffer takes the current buffer's role
+            std::swap(p->buf[0], p->buf[1]);
+            std::swap(p->d_off[0], p->d_off[1]);
+            std::swap(p->d_occ[0], p->d_occ[1]);
+            std::swap(st.n_alive[0], st.n_alive[1]);
+            std::swap(st.n_occ[0], st.n_occ[1]);
+            std::swap(st.max_id[0], st.max_id[1]);
+            std::swap(st.gpool[0], st.gpool[1]);
+            HIPCHK(hipMemcpyAsync(p->d_st, &st, sizeof(st), hipMemcpyHostToDevice, p->stream));
+            HIPCHK(hipStreamSynchronize(p->stream));
+            p->known = st;
+            p->known_step = p->steps_enqueued;
+            p->gen++;
+        }
     }
     if (p->has_cap) {
         int rc = run_capacity(p);

@@ -110,6 +110,14 @@ class GpuPopulation:
         self._edesc = env_desc(self.env)
         check(lib().qhg_gpu_set_env(self.h, C.byref(self._edesc)), "qhg_gpu_set_env")
 
+    def env_event(self, env, events):
+        """updateEvent + flushEvents between steps: the given fields replace
+        the current ones; with abi.EVENT_KILL_WET the agents of cells under
+        water or ice are removed; capacities and weights are recomputed."""
+        self.env.update(env)
+        self._edesc = env_desc(self.env)
+        check(lib().qhg_gpu_env_event(self.h, C.byref(self._edesc), int(events)), "qhg_gpu_env_event")
+
     # -- observables -------------------------------------------------------
     def stats(self):
         s = abi.Stats()

@@ -204,3 +204,54 @@ def test_modify_attributes_mid_run():
     with pytest.raises(Exception, match="environment events"):
         gpu.modifyAttributes("NPPCap_K_max", 10.0)
     gpu.close()
+
+
+@pytest.mark.parametrize("genetics", [False, True])
+def test_environment_events(genetics):
+    """updateEvent/flushEvents between steps: a climate event (NPP falls)
+    recomputes capacities and weights; two geography events drown the agents
+    of cells under water or ice (the second after the first repacked the
+    buffers); stepping goes on bit-exact with the oracle."""
+    print("case: environment events, ico level 4, genetics=%s" % genetics, flush=True)
+    sc = scenarios.c2(seed=17, level=4, n_agents=5000, patch_cells=250, genetics=genetics)
+    gpu = GpuPopulation.from_scenario(sc)
+    ora = Oracle(sc)
+    gpu.preLoop()
+    ora.pre_loop()
+    gpu.run(6)
+    ora.run(6)
+    env = {k: np.array(v, copy=True) for k, v in sc["env"].items()}
+    env["base_npp"] = env["base_npp"] * 0.7
+    gpu.env_event({"base_npp": env["base_npp"]}, abi.EVENT_CLIMATE)
+    ora.env_event({"base_npp": env["base_npp"]}, abi.EVENT_CLIMATE)
+    assert np.array_equal(gpu.capacities().view(np.uint64), ora.capacities().view(np.uint64))
+    assert np.array_equal(gpu.env_weights().view(np.uint64), ora.env_weights().view(np.uint64))
+    rng = np.random.default_rng(8)
+    for k in range(2):
+        gpu.run(5)
+        ora.run(5)
+        occupied = np.unique(ora.agents()["cell"])
+        pick = rng.permutation(occupied)
+        env["altitude"][pick[:len(pick) // 5]] = -40.0
+        ice = np.zeros(sc["grid"]["num_cells"], dtype=np.uint8)
+        ice[pick[len(pick) // 5: len(pick) // 4]] = 1
+        ev = {"altitude": env["altitude"], "ice": ice}
+        n0 = ora.stats()["num_alive"]
+        gpu.env_event(ev, abi.EVENT_GEO | abi.EVENT_KILL_WET)
+        ora.env_event(ev, abi.EVENT_GEO | abi.EVENT_KILL_WET)
+        where = "after geography event %d" % k
+        assert ora.stats()["num_alive"] < n0, where
+        assert_same_state(gpu.agents(), ora.agents(), where)
+        assert np.array_equal(gpu.getNumAgentsArray(), ora.cell_counts()), where
+        assert gpu.stats()["num_alive"] == ora.stats()["num_alive"], where
+        assert np.array_equal(gpu.capacities().view(np.uint64), ora.capacities().view(np.uint64)), where
+        assert np.array_equal(gpu.env_weights().view(np.uint64), ora.env_weights().view(np.uint64)), where
+    for k in range(3):
+        gpu.run(6)
+        ora.run(6)
+        assert_same_state(gpu.agents(), ora.agents(), "step block %d after the events" % k)
+        gs, os_ = gpu.stats(), ora.stats()
+        for key in ("num_alive", "max_id", "births_total", "deaths_total", "moves_total"):
+            assert gs[key] == os_[key], key
+        assert gs["device_error"] == 0
+    gpu.close()

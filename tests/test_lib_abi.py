@@ -43,7 +43,7 @@ def test_qdf_library_exports_every_declared_symbol():
 
 def test_abi_version_and_words():
     L = _lib.lib()
-    assert L.qhg_gpu_abi_version() == 5
+    assert L.qhg_gpu_abi_version() == 6
     assert L.qhg_gpu_genome_words(256) == 8 == abi.genome_words(256)
     assert L.qhg_gpu_genome_words(1) == 1
 

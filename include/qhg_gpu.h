@@ -243,7 +243,8 @@ int qhg_gpu_set_env(qhg_pop *pop, const qhg_env_desc *env);
  * :120-150); then capacities and evaluator weights are recomputed (the
  * flush, NPPCapacity.cpp:121-131, MultiEvaluator.cpp:142-210).  The removed
  * agents are not step deaths (stats count none).  Event bits follow
- * QHG4/utils_qhg/EventConsts.h; qhg_params_event_flags adds KILL_WET for the
+ * QHG4/utils_qhg/EventConsts.h (EVENT_ID_GEO 2, EVENT_ID_CLIMATE 3, EVENT_ID_VEG 4
+ * map to the bits below); qhg_params_event_flags adds KILL_WET for the
  * classes whose updateEvent applies it. */
 #define QHG_EVENT_GEO       0x01
 #define QHG_EVENT_CLIMATE   0x02

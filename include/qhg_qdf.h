@@ -78,6 +78,35 @@ int qhg_qdf_read_agents(const char *path, const char *species, qhg_agents *a);
  * dumped run stopped. */
 int qhg_qdf_restore_population(const char *path, const char *species, qhg_pop *pop);
 
+/* The same pieces on groups and datasets the caller opened (HDF5 hid_t
+ * handles passed as int64_t), for a population inside the reference's
+ * writer and reader, which open the species group and create the
+ * AgentDataSet themselves (PopWriter::write, QHG4/io/PopWriter.cpp:57-130;
+ * PopReader::read, QHG4/io/PopReader.h:34):
+ *   qhg_qdf_agent_type           createAgentDataTypeQDF (SPopulation.cpp:1356-1372);
+ *                                the caller closes the type
+ *   qhg_qdf_write_species_group  writeSpeciesDataQDF / dumpSpeciesDataQDF
+ *                                (SPopulation.cpp:1748-1778)
+ *   qhg_qdf_read_species_group   readSpeciesDataQDF / restoreSpeciesDataQDF
+ *                                (SPopulation.cpp:1784-1830); info may be NULL
+ *                                (its step/time/max_id come from the group only)
+ *   qhg_qdf_write_agent_dataset  writeAgentDataQDF / dumpAgentDataQDF: every row
+ *                                of the dataset (its extent must equal a->n)
+ *   qhg_qdf_read_agent_dataset   readAgentDataQDF / restoreAgentDataQDF
+ *   qhg_qdf_write_genome / read_genome
+ *                                writeAdditionalDataQDF / readAdditionalDataQDF of
+ *                                Genetics (QHG4/actions/Genetics.cpp, Genome
+ *                                dataset n x words; words < 0 on read: any) */
+int64_t qhg_qdf_agent_type(void);
+int qhg_qdf_write_species_group(int64_t group, const char *species, const char *pop_class, const qhg_params *params,
+                                int32_t num_cells, int64_t step, double time, int64_t max_id);
+int qhg_qdf_read_species_group(int64_t group, uint64_t seed, qhg_params *out, qhg_qdf_info *info);
+int qhg_qdf_dataset_rows(int64_t dataset, int64_t *n);
+int qhg_qdf_write_agent_dataset(int64_t dataset, const qhg_agents *agents, const int64_t *cell_ids);
+int qhg_qdf_read_agent_dataset(int64_t dataset, qhg_agents *agents);
+int qhg_qdf_write_genome(int64_t group, const uint64_t *genome, int64_t n, int32_t words);
+int qhg_qdf_read_genome(int64_t group, uint64_t *genome, int64_t n, int32_t words);
+
 /* Grids (the reference's Grid/CellDataSet {CellID, NumNeighbors,
  * Neighbors[max]}, Geography, Climate and Vegetation groups). */
 typedef struct qhg_qdf_grid_info {

@@ -234,6 +234,174 @@ herr_t collect_attr(hid_t loc, const char *name, const H5A_info_t *, void *op) {
     return 0;
 }
 
+// AgentDataSet rows from agent arrays (all rows of the dataset at once)
+int write_records(hid_t ds, const qhg_agents *agents, const int64_t *cell_ids) {
+    const int64_t n = agents->n;
+    Hid sp(H5Dget_space(ds), H5Sclose);
+    if (!sp.ok() || (int64_t)H5Sget_simple_extent_npoints(sp) != n)
+        return qfail(QHG_ERR_ARG, "AgentDataSet rows differ from the agent count");
+    std::vector<QAgent> rec((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        QAgent &r = rec[(size_t)i];
+        std::memset(&r, 0, sizeof(r));
+        r.life = agents->life ? agents->life[i] : QHG_LIFE_ALIVE;
+        r.cell_idx = agents->cell ? agents->cell[i] : 0;
+        r.cell_id = (cell_ids && agents->cell) ? (int32_t)cell_ids[agents->cell[i]] : r.cell_idx;
+        r.agent_id = agents->id[i];
+        r.birth = agents->birth_time ? agents->birth_time[i] : 0.0;
+        r.gender = agents->gender ? agents->gender[i] : 0;
+        r.age = agents->age ? agents->age[i] : 0.0;
+        r.lastb = agents->last_birth ? agents->last_birth[i] : -1.0;
+        r.mate = agents->mate_id ? agents->mate_id[i] : -3;
+        r.nbabies = agents->num_babies ? agents->num_babies[i] : 0;
+    }
+    Hid t(agent_type(), H5Tclose);
+    if (n > 0 && H5Dwrite(ds, t, H5S_ALL, H5S_ALL, H5P_DEFAULT, rec.data()) < 0)
+        return qfail(QHG_ERR_ARG, "writing AgentDataSet");
+    return 0;
+}
+
+// agent arrays from AgentDataSet rows: the members the file has, defaults
+// for the others
+int read_records(hid_t ds, qhg_agents *a) {
+    Hid sp(H5Dget_space(ds), H5Sclose);
+    if (!sp.ok()) return qfail(QHG_ERR_ARG, "no dataspace");
+    const int64_t n = (int64_t)H5Sget_simple_extent_npoints(sp);
+    if (n != a->n) return qfail(QHG_ERR_ARG, "agent count differs from the arrays given");
+    Hid ft(H5Dget_type(ds), H5Tclose);
+    for (const char *need : {"LifeState", "CellIdx", "AgentID", "BirthTime", "Gender"})
+        if (H5Tget_member_index(ft, need) < 0) return qfail(QHG_ERR_ARG, std::string("AgentDataSet lacks ") + need);
+    std::vector<QAgent> rec((size_t)n);
+    for (QAgent &r : rec) {
+        std::memset(&r, 0, sizeof(r));
+        r.lastb = -1.0;
+        r.mate = -3;
+    }
+    const bool has_age = H5Tget_member_index(ft, "Age") >= 0;
+    Hid mt(agent_type(ft), H5Tclose);
+    if (n > 0 && H5Dread(ds, mt, H5S_ALL, H5S_ALL, H5P_DEFAULT, rec.data()) < 0)
+        return qfail(QHG_ERR_ARG, "reading AgentDataSet");
+    for (int64_t i = 0; i < n; i++) {
+        const QAgent &r = rec[(size_t)i];
+        a->id[i] = r.agent_id;
+        if (a->cell) a->cell[i] = r.cell_idx;
+        if (a->life) a->life[i] = (uint8_t)r.life;
+        if (a->gender) a->gender[i] = r.gender;
+        if (a->birth_time) a->birth_time[i] = r.birth;
+        if (a->age) a->age[i] = has_age ? r.age : 0.0;
+        if (a->last_birth) a->last_birth[i] = r.lastb;
+        if (a->mate_id) a->mate_id[i] = r.mate;
+        if (a->num_babies) a->num_babies[i] = r.nbabies;
+    }
+    return 0;
+}
+
+int write_genome(hid_t g, const uint64_t *genome, int64_t n, int32_t words) {
+    if (H5Lexists(g, "Genome", H5P_DEFAULT) > 0) H5Ldelete(g, "Genome", H5P_DEFAULT);
+    hsize_t d[2] = {(hsize_t)n, (hsize_t)words};
+    Hid sp(H5Screate_simple(2, d, nullptr), H5Sclose);
+    Hid ds(H5Dcreate2(g, "Genome", H5T_NATIVE_UINT64, sp, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Dclose);
+    if (!ds.ok() || (n > 0 && H5Dwrite(ds, H5T_NATIVE_UINT64, H5S_ALL, H5S_ALL, H5P_DEFAULT, genome) < 0))
+        return qfail(QHG_ERR_ARG, "writing Genome");
+    return 0;
+}
+
+int read_genome(hid_t g, uint64_t *genome, int64_t n, int32_t words) {
+    if (H5Lexists(g, "Genome", H5P_DEFAULT) <= 0) return qfail(QHG_ERR_ARG, "no Genome dataset");
+    Hid gd(H5Dopen2(g, "Genome", H5P_DEFAULT), H5Dclose);
+    Hid gs(H5Dget_space(gd), H5Sclose);
+    hsize_t d[2] = {0, 0};
+    if (H5Sget_simple_extent_ndims(gs) != 2) return qfail(QHG_ERR_ARG, "Genome is not two-dimensional");
+    H5Sget_simple_extent_dims(gs, d, nullptr);
+    if ((int64_t)d[0] != n) return qfail(QHG_ERR_ARG, "Genome rows differ from the agent count");
+    if (words >= 0 && (int64_t)d[1] != words) return qfail(QHG_ERR_ARG, "Genome words differ from the population's");
+    if (n > 0 && H5Dread(gd, H5T_NATIVE_UINT64, H5S_ALL, H5S_ALL, H5P_DEFAULT, genome) < 0)
+        return qfail(QHG_ERR_ARG, "reading Genome");
+    return 0;
+}
+
+// the parameter block of a species group: QHGGpuParams when present, else
+// rebuilt from ClassName, PrioInfo and the action attributes
+int params_from_group(hid_t g, uint64_t seed, qhg_params *out) {
+    if (get_num(g, "QHGGpuParams", H5T_NATIVE_UINT8, out, sizeof(qhg_params))) return 0;
+    std::string cls;
+    if (!get_str(g, "ClassName", cls)) return qfail(QHG_ERR_ARG, "no ClassName attribute");
+    std::vector<std::string> pn;
+    std::vector<int32_t> pv;
+    if (H5Aexists(g, "PrioInfo") > 0) {
+        Hid a(H5Aopen(g, "PrioInfo", H5P_DEFAULT), H5Aclose);
+        Hid sp(H5Aget_space(a), H5Sclose);
+        const hssize_t n = H5Sget_simple_extent_npoints(sp);
+        std::vector<PrioRec> pr((size_t)std::max<hssize_t>(n, 1));
+        Hid st(H5Tcopy(H5T_C_S1), H5Tclose);
+        H5Tset_size(st, 128);
+        Hid ct(H5Tcreate(H5T_COMPOUND, sizeof(PrioRec)), H5Tclose);
+        H5Tinsert(ct, "FuncName", HOFFSET(PrioRec, name), st);
+        H5Tinsert(ct, "FuncPrio", HOFFSET(PrioRec, prio), H5T_NATIVE_INT32);
+        if (H5Aread(a, ct, pr.data()) < 0) return qfail(QHG_ERR_ARG, "reading PrioInfo");
+        for (hssize_t k = 0; k < n; k++) {
+            pr[k].name[127] = 0;
+            pn.push_back(pr[k].name);
+            pv.push_back(pr[k].prio);
+        }
+    }
+    std::map<std::string, std::string> kv;
+    hsize_t idx = 0;
+    H5Aiterate2(g, H5_INDEX_NAME, H5_ITER_INC, &idx, collect_attr, &kv);
+    std::vector<const char *> k, v, p;
+    for (auto &e : kv) {
+        k.push_back(e.first.c_str());
+        v.push_back(e.second.c_str());
+    }
+    for (auto &str : pn) p.push_back(str.c_str());
+    int rc = qhg_params_from_attribute_map(cls.c_str(), (int32_t)k.size(), k.data(), v.data(), p.data(), pv.data(),
+                                           (int32_t)p.size(), seed, out);
+    return rc ? qfail(rc, qhg_gpu_last_error()) : 0;
+}
+
+// what a species group says of itself; root (or -1) supplies the file's
+// Step / StartTime when the group has no clock of its own
+void info_from_group(hid_t g, hid_t root, qhg_qdf_info *info) {
+    qhg_qdf_info I;
+    std::memset(&I, 0, sizeof(I));
+    hid_t ds = -1;
+    if (H5Lexists(g, "AgentDataSet", H5P_DEFAULT) > 0) {
+        ds = H5Dopen2(g, "AgentDataSet", H5P_DEFAULT);
+        Hid sp(H5Dget_space(ds), H5Sclose);
+        I.num_agents = (int64_t)H5Sget_simple_extent_npoints(sp);
+    }
+    Hid dsh(ds, H5Dclose);
+    if (H5Lexists(g, "Genome", H5P_DEFAULT) > 0) {
+        Hid gd(H5Dopen2(g, "Genome", H5P_DEFAULT), H5Dclose);
+        Hid gs(H5Dget_space(gd), H5Sclose);
+        hsize_t d[2] = {0, 0};
+        if (H5Sget_simple_extent_ndims(gs) == 2) {
+            H5Sget_simple_extent_dims(gs, d, nullptr);
+            I.genome_words = (int32_t)d[1];
+        }
+    }
+    std::string s;
+    if (get_str(g, "ClassName", s)) std::strncpy(I.class_name, s.c_str(), sizeof(I.class_name) - 1);
+    if (get_str(g, "SpeciesName", s)) std::strncpy(I.species_name, s.c_str(), sizeof(I.species_name) - 1);
+    get_num(g, "NumCells", H5T_NATIVE_INT32, &I.num_cells, 1);
+    if (!get_num(g, "StepCount", H5T_NATIVE_INT64, &I.step, 1) && root >= 0 && get_str(root, "Step", s))
+        I.step = std::strtoll(s.c_str(), nullptr, 10);
+    if (!get_num(g, "NextTime", H5T_NATIVE_DOUBLE, &I.time, 1) && root >= 0 && get_str(root, "StartTime", s))
+        I.time = std::strtod(s.c_str(), nullptr);
+    get_num(g, "PhiloxSeed", H5T_NATIVE_UINT64, &I.seed, 1);
+    I.has_params = H5Aexists(g, "QHGGpuParams") > 0;
+    if (!get_num(g, "MaxAgentID", H5T_NATIVE_INT64, &I.max_id, 1) && I.num_agents > 0) {
+        std::vector<int64_t> ids((size_t)I.num_agents);
+        Hid t(H5Tcreate(H5T_COMPOUND, sizeof(int64_t)), H5Tclose);
+        H5Tinsert(t, "AgentID", 0, H5T_NATIVE_INT64);
+        if (H5Dread(dsh, t, H5S_ALL, H5S_ALL, H5P_DEFAULT, ids.data()) >= 0)
+            I.max_id = *std::max_element(ids.begin(), ids.end());
+    }
+    *info = I;
+}
+
+static_assert(sizeof(hid_t) == sizeof(int64_t), "HDF5 handles travel as int64_t");
+
 }  // namespace
 
 extern "C" {
@@ -272,36 +440,16 @@ int qhg_qdf_write_agents(const char *path, int32_t create, const char *species, 
     int rc = write_species_attrs(g, species, pop_class, params, num_cells, step, time, max_id);
     if (rc) return rc;
     const int64_t n = agents->n;
-    std::vector<QAgent> rec((size_t)n);
-    for (int64_t i = 0; i < n; i++) {
-        QAgent &r = rec[(size_t)i];
-        std::memset(&r, 0, sizeof(r));
-        r.life = agents->life ? agents->life[i] : QHG_LIFE_ALIVE;
-        r.cell_idx = agents->cell ? agents->cell[i] : 0;
-        r.cell_id = (cell_ids && agents->cell) ? (int32_t)cell_ids[agents->cell[i]] : r.cell_idx;
-        r.agent_id = agents->id[i];
-        r.birth = agents->birth_time ? agents->birth_time[i] : 0.0;
-        r.gender = agents->gender ? agents->gender[i] : 0;
-        r.age = agents->age ? agents->age[i] : 0.0;
-        r.lastb = agents->last_birth ? agents->last_birth[i] : -1.0;
-        r.mate = agents->mate_id ? agents->mate_id[i] : -3;
-        r.nbabies = agents->num_babies ? agents->num_babies[i] : 0;
-    }
     {
         Hid t(agent_type(), H5Tclose);
         hsize_t d = (hsize_t)n;
         Hid sp(H5Screate_simple(1, &d, nullptr), H5Sclose);
         Hid ds(H5Dcreate2(g, "AgentDataSet", t, sp, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Dclose);
-        if (!ds.ok() || (n > 0 && H5Dwrite(ds, t, H5S_ALL, H5S_ALL, H5P_DEFAULT, rec.data()) < 0))
-            return qfail(QHG_ERR_ARG, "writing AgentDataSet");
+        if (!ds.ok()) return qfail(QHG_ERR_ARG, "creating AgentDataSet");
+        rc = write_records(ds, agents, cell_ids);
+        if (rc) return rc;
     }
-    if (genome_words > 0) {
-        hsize_t d[2] = {(hsize_t)n, (hsize_t)genome_words};
-        Hid sp(H5Screate_simple(2, d, nullptr), H5Sclose);
-        Hid ds(H5Dcreate2(g, "Genome", H5T_NATIVE_UINT64, sp, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Dclose);
-        if (!ds.ok() || (n > 0 && H5Dwrite(ds, H5T_NATIVE_UINT64, H5S_ALL, H5S_ALL, H5P_DEFAULT, agents->genome) < 0))
-            return qfail(QHG_ERR_ARG, "writing Genome");
-    }
+    if (genome_words > 0) return write_genome(g, agents->genome, n, genome_words);
     return 0;
 }
 
@@ -332,40 +480,9 @@ int qhg_qdf_read_info(const char *path, const char *species, qhg_qdf_info *info)
     if (!f.ok()) return qfail(QHG_ERR_ARG, std::string("cannot open ") + path);
     Hid g(open_species(f, species), H5Gclose);
     if (!g.ok()) return qfail(QHG_ERR_ARG, std::string("no population group ") + species);
-    qhg_qdf_info I;
-    std::memset(&I, 0, sizeof(I));
     if (H5Lexists(g, "AgentDataSet", H5P_DEFAULT) <= 0) return qfail(QHG_ERR_ARG, "no AgentDataSet");
-    Hid ds(H5Dopen2(g, "AgentDataSet", H5P_DEFAULT), H5Dclose);
-    Hid sp(H5Dget_space(ds), H5Sclose);
-    I.num_agents = (int64_t)H5Sget_simple_extent_npoints(sp);
-    if (H5Lexists(g, "Genome", H5P_DEFAULT) > 0) {
-        Hid gd(H5Dopen2(g, "Genome", H5P_DEFAULT), H5Dclose);
-        Hid gs(H5Dget_space(gd), H5Sclose);
-        hsize_t d[2] = {0, 0};
-        if (H5Sget_simple_extent_ndims(gs) == 2) {
-            H5Sget_simple_extent_dims(gs, d, nullptr);
-            I.genome_words = (int32_t)d[1];
-        }
-    }
-    std::string s;
-    if (get_str(g, "ClassName", s)) std::strncpy(I.class_name, s.c_str(), sizeof(I.class_name) - 1);
-    if (get_str(g, "SpeciesName", s)) std::strncpy(I.species_name, s.c_str(), sizeof(I.species_name) - 1);
-    get_num(g, "NumCells", H5T_NATIVE_INT32, &I.num_cells, 1);
     Hid root(H5Gopen2(f, "/", H5P_DEFAULT), H5Gclose);
-    if (!get_num(g, "StepCount", H5T_NATIVE_INT64, &I.step, 1) && get_str(root, "Step", s))
-        I.step = std::strtoll(s.c_str(), nullptr, 10);
-    if (!get_num(g, "NextTime", H5T_NATIVE_DOUBLE, &I.time, 1) && get_str(root, "StartTime", s))
-        I.time = std::strtod(s.c_str(), nullptr);
-    get_num(g, "PhiloxSeed", H5T_NATIVE_UINT64, &I.seed, 1);
-    I.has_params = H5Aexists(g, "QHGGpuParams") > 0;
-    if (!get_num(g, "MaxAgentID", H5T_NATIVE_INT64, &I.max_id, 1) && I.num_agents > 0) {
-        std::vector<int64_t> ids((size_t)I.num_agents);
-        Hid t(H5Tcreate(H5T_COMPOUND, sizeof(int64_t)), H5Tclose);
-        H5Tinsert(t, "AgentID", 0, H5T_NATIVE_INT64);
-        if (H5Dread(ds, t, H5S_ALL, H5S_ALL, H5P_DEFAULT, ids.data()) >= 0)
-            I.max_id = *std::max_element(ids.begin(), ids.end());
-    }
-    *info = I;
+    info_from_group(g, root, info);
     return 0;
 }
 
@@ -376,41 +493,7 @@ int qhg_qdf_read_params(const char *path, const char *species, uint64_t seed, qh
     if (!f.ok()) return qfail(QHG_ERR_ARG, std::string("cannot open ") + path);
     Hid g(open_species(f, species), H5Gclose);
     if (!g.ok()) return qfail(QHG_ERR_ARG, std::string("no population group ") + species);
-    if (get_num(g, "QHGGpuParams", H5T_NATIVE_UINT8, out, sizeof(qhg_params))) return 0;
-    // a reference-written group: class, priorities and action attributes
-    std::string cls;
-    if (!get_str(g, "ClassName", cls)) return qfail(QHG_ERR_ARG, "no ClassName attribute");
-    std::vector<std::string> pn;
-    std::vector<int32_t> pv;
-    if (H5Aexists(g, "PrioInfo") > 0) {
-        Hid a(H5Aopen(g, "PrioInfo", H5P_DEFAULT), H5Aclose);
-        Hid sp(H5Aget_space(a), H5Sclose);
-        const hssize_t n = H5Sget_simple_extent_npoints(sp);
-        std::vector<PrioRec> pr((size_t)std::max<hssize_t>(n, 1));
-        Hid st(H5Tcopy(H5T_C_S1), H5Tclose);
-        H5Tset_size(st, 128);
-        Hid ct(H5Tcreate(H5T_COMPOUND, sizeof(PrioRec)), H5Tclose);
-        H5Tinsert(ct, "FuncName", HOFFSET(PrioRec, name), st);
-        H5Tinsert(ct, "FuncPrio", HOFFSET(PrioRec, prio), H5T_NATIVE_INT32);
-        if (H5Aread(a, ct, pr.data()) < 0) return qfail(QHG_ERR_ARG, "reading PrioInfo");
-        for (hssize_t k = 0; k < n; k++) {
-            pr[k].name[127] = 0;
-            pn.push_back(pr[k].name);
-            pv.push_back(pr[k].prio);
-        }
-    }
-    std::map<std::string, std::string> kv;
-    hsize_t idx = 0;
-    H5Aiterate2(g, H5_INDEX_NAME, H5_ITER_INC, &idx, collect_attr, &kv);
-    std::vector<const char *> k, v, p;
-    for (auto &e : kv) {
-        k.push_back(e.first.c_str());
-        v.push_back(e.second.c_str());
-    }
-    for (auto &s : pn) p.push_back(s.c_str());
-    int rc = qhg_params_from_attribute_map(cls.c_str(), (int32_t)k.size(), k.data(), v.data(), p.data(), pv.data(),
-                                           (int32_t)p.size(), seed, out);
-    return rc ? qfail(rc, qhg_gpu_last_error()) : 0;
+    return params_from_group(g, seed, out);
 }
 
 int qhg_qdf_read_agents(const char *path, const char *species, qhg_agents *a) {
@@ -422,44 +505,9 @@ int qhg_qdf_read_agents(const char *path, const char *species, qhg_agents *a) {
     if (!g.ok()) return qfail(QHG_ERR_ARG, std::string("no population group ") + species);
     Hid ds(H5Dopen2(g, "AgentDataSet", H5P_DEFAULT), H5Dclose);
     if (!ds.ok()) return qfail(QHG_ERR_ARG, "no AgentDataSet");
-    Hid sp(H5Dget_space(ds), H5Sclose);
-    const int64_t n = (int64_t)H5Sget_simple_extent_npoints(sp);
-    if (n != a->n) return qfail(QHG_ERR_ARG, "agent count differs from the arrays given");
-    Hid ft(H5Dget_type(ds), H5Tclose);
-    for (const char *need : {"LifeState", "CellIdx", "AgentID", "BirthTime", "Gender"})
-        if (H5Tget_member_index(ft, need) < 0) return qfail(QHG_ERR_ARG, std::string("AgentDataSet lacks ") + need);
-    std::vector<QAgent> rec((size_t)n);
-    for (QAgent &r : rec) {
-        std::memset(&r, 0, sizeof(r));
-        r.lastb = -1.0;
-        r.mate = -3;
-    }
-    const bool has_age = H5Tget_member_index(ft, "Age") >= 0;
-    Hid mt(agent_type(ft), H5Tclose);
-    if (n > 0 && H5Dread(ds, mt, H5S_ALL, H5S_ALL, H5P_DEFAULT, rec.data()) < 0)
-        return qfail(QHG_ERR_ARG, "reading AgentDataSet");
-    for (int64_t i = 0; i < n; i++) {
-        const QAgent &r = rec[(size_t)i];
-        a->id[i] = r.agent_id;
-        if (a->cell) a->cell[i] = r.cell_idx;
-        if (a->life) a->life[i] = (uint8_t)r.life;
-        if (a->gender) a->gender[i] = r.gender;
-        if (a->birth_time) a->birth_time[i] = r.birth;
-        if (a->age) a->age[i] = has_age ? r.age : 0.0;
-        if (a->last_birth) a->last_birth[i] = r.lastb;
-        if (a->mate_id) a->mate_id[i] = r.mate;
-        if (a->num_babies) a->num_babies[i] = r.nbabies;
-    }
-    if (a->genome && H5Lexists(g, "Genome", H5P_DEFAULT) > 0) {
-        Hid gd(H5Dopen2(g, "Genome", H5P_DEFAULT), H5Dclose);
-        Hid gs(H5Dget_space(gd), H5Sclose);
-        hsize_t d[2] = {0, 0};
-        if (H5Sget_simple_extent_ndims(gs) != 2) return qfail(QHG_ERR_ARG, "Genome is not two-dimensional");
-        H5Sget_simple_extent_dims(gs, d, nullptr);
-        if ((int64_t)d[0] != n) return qfail(QHG_ERR_ARG, "Genome rows differ from the agent count");
-        if (n > 0 && H5Dread(gd, H5T_NATIVE_UINT64, H5S_ALL, H5S_ALL, H5P_DEFAULT, a->genome) < 0)
-            return qfail(QHG_ERR_ARG, "reading Genome");
-    }
+    int rc = read_records(ds, a);
+    if (rc) return rc;
+    if (a->genome && H5Lexists(g, "Genome", H5P_DEFAULT) > 0) return read_genome(g, a->genome, a->n, -1);
     return 0;
 }
 
@@ -485,6 +533,57 @@ int qhg_qdf_restore_population(const char *path, const char *species, qhg_pop *p
     if (qhg_gpu_upload_agents(pop, &a) || qhg_gpu_set_clock(pop, I.step, I.time, I.max_id))
         return qfail(QHG_ERR_STATE, qhg_gpu_last_error());
     return 0;
+}
+
+// -- species groups and datasets opened by the caller ---------------------
+
+int64_t qhg_qdf_agent_type(void) { return (int64_t)agent_type(); }
+
+int qhg_qdf_write_species_group(int64_t group, const char *species, const char *pop_class, const qhg_params *params,
+                                int32_t num_cells, int64_t step, double time, int64_t max_id) {
+    if (group < 0 || !species || !params) return qfail(QHG_ERR_ARG, "null argument");
+    quiet();
+    return write_species_attrs((hid_t)group, species, pop_class, params, num_cells, step, time, max_id);
+}
+
+int qhg_qdf_read_species_group(int64_t group, uint64_t seed, qhg_params *out, qhg_qdf_info *info) {
+    if (group < 0) return qfail(QHG_ERR_ARG, "invalid group");
+    quiet();
+    if (info) info_from_group((hid_t)group, -1, info);
+    return out ? params_from_group((hid_t)group, seed, out) : 0;
+}
+
+int qhg_qdf_dataset_rows(int64_t dataset, int64_t *n) {
+    if (dataset < 0 || !n) return qfail(QHG_ERR_ARG, "null argument");
+    Hid sp(H5Dget_space((hid_t)dataset), H5Sclose);
+    if (!sp.ok()) return qfail(QHG_ERR_ARG, "no dataspace");
+    *n = (int64_t)H5Sget_simple_extent_npoints(sp);
+    return 0;
+}
+
+int qhg_qdf_write_agent_dataset(int64_t dataset, const qhg_agents *agents, const int64_t *cell_ids) {
+    if (dataset < 0 || !agents || agents->n < 0 || (agents->n > 0 && !agents->id))
+        return qfail(QHG_ERR_ARG, "null argument");
+    quiet();
+    return write_records((hid_t)dataset, agents, cell_ids);
+}
+
+int qhg_qdf_read_agent_dataset(int64_t dataset, qhg_agents *agents) {
+    if (dataset < 0 || !agents || !agents->id) return qfail(QHG_ERR_ARG, "null argument");
+    quiet();
+    return read_records((hid_t)dataset, agents);
+}
+
+int qhg_qdf_write_genome(int64_t group, const uint64_t *genome, int64_t n, int32_t words) {
+    if (group < 0 || n < 0 || words <= 0 || (n > 0 && !genome)) return qfail(QHG_ERR_ARG, "bad argument");
+    quiet();
+    return write_genome((hid_t)group, genome, n, words);
+}
+
+int qhg_qdf_read_genome(int64_t group, uint64_t *genome, int64_t n, int32_t words) {
+    if (group < 0 || n < 0 || (n > 0 && !genome)) return qfail(QHG_ERR_ARG, "bad argument");
+    quiet();
+    return read_genome((hid_t)group, genome, n, words);
 }
 
 int qhg_qdf_read_grid_info(const char *path, qhg_qdf_grid_info *info) {

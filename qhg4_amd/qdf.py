@@ -15,7 +15,9 @@ _lib = None
 # every symbol declared in include/qhg_qdf.h
 EXPORTS = ["qhg_qdf_last_error", "qhg_qdf_write_agents", "qhg_qdf_write_population", "qhg_qdf_read_info",
            "qhg_qdf_read_params", "qhg_qdf_read_agents", "qhg_qdf_restore_population", "qhg_qdf_read_grid_info",
-           "qhg_qdf_read_grid"]
+           "qhg_qdf_read_grid", "qhg_qdf_agent_type", "qhg_qdf_write_species_group",
+           "qhg_qdf_read_species_group", "qhg_qdf_dataset_rows", "qhg_qdf_write_agent_dataset",
+           "qhg_qdf_read_agent_dataset", "qhg_qdf_write_genome", "qhg_qdf_read_genome"]
 
 
 class QdfInfo(C.Structure):
@@ -52,6 +54,16 @@ def lib():
             "qhg_qdf_read_grid_info": ([C.c_char_p, P(QdfGridInfo)], C.c_int),
             "qhg_qdf_read_grid": ([C.c_char_p, P(C.c_int32), P(C.c_uint8), P(C.c_int32)] + [P(C.c_double)] * 4 +
                                   [P(C.c_uint8)] * 2 + [P(C.c_double)] * 3 + [P(C.c_int32)], C.c_int),
+            # on groups / datasets opened by the caller (HDF5 handles as int64)
+            "qhg_qdf_agent_type": ([], C.c_int64),
+            "qhg_qdf_write_species_group": ([C.c_int64, C.c_char_p, C.c_char_p, P(abi.Params), C.c_int32, C.c_int64,
+                                             C.c_double, C.c_int64], C.c_int),
+            "qhg_qdf_read_species_group": ([C.c_int64, C.c_uint64, P(abi.Params), P(QdfInfo)], C.c_int),
+            "qhg_qdf_dataset_rows": ([C.c_int64, P(C.c_int64)], C.c_int),
+            "qhg_qdf_write_agent_dataset": ([C.c_int64, P(abi.Agents), P(C.c_int64)], C.c_int),
+            "qhg_qdf_read_agent_dataset": ([C.c_int64, P(abi.Agents)], C.c_int),
+            "qhg_qdf_write_genome": ([C.c_int64, P(C.c_uint64), C.c_int64, C.c_int32], C.c_int),
+            "qhg_qdf_read_genome": ([C.c_int64, P(C.c_uint64), C.c_int64, C.c_int32], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

@@ -68,7 +68,7 @@ def kernel_bytes(bm, d, grid, ops, gwords):
     """Algorithmic bytes of each step kernel summed over the measured steps.
     d: deltas of live agent-steps, occupied cell-steps, births, deaths."""
     from qhg4_amd import abi
-    ncells, conn = grid["num_cells"], grid["connectivity"]
+    conn = grid["connectivity"]
     live, occ, births, deaths, steps = d["live"], d["occupied"], d["births"], d["deaths"], d["steps"]
     per_live = bm["ACT_LIVE_BASE"]
     if abi.ACT_RANDOMPAIR in ops:
@@ -77,14 +77,13 @@ def kernel_bytes(bm, d, grid, ops, gwords):
         per_live += bm["ACT_LIVE_GETOLD"]
     if abi.ACT_FERTILITY in ops:
         per_live += bm["ACT_LIVE_FERT"]
-    per_cell = bm["ACT_CELL"] + conn * bm["ACT_CELL_PER_CONN"]
     per_occ = bm["ACT_OCC"] + conn * bm["ACT_OCC_PER_CONN"] + bm["ACT_OCC_W_EXTRA"]
-    act = steps * ncells * per_cell + occ * per_occ + live * per_live + births * bm["ACT_BIRTH"]
-    scan = steps * ncells * bm["SCAN_CELL"]
+    act = occ * per_occ + live * per_live + births * bm["ACT_BIRTH"]
+    scan = occ * bm["SCAN_OCC"]
     survivors = live - deaths
     per_surv = bm["SCAT_SURVIVOR"] + (bm["SCAT_SURVIVOR_GEN"] if gwords else 0)
     per_birth = bm["SCAT_BIRTH"] + (bm["SCAT_BIRTH_GEN"] if gwords else 0)
-    scat = steps * ncells * bm["SCAT_CELL"] + live * bm["SCAT_LIVE"] + survivors * per_surv + births * per_birth
+    scat = occ * bm["SCAT_OCC"] + live * bm["SCAT_LIVE"] + survivors * per_surv + births * per_birth
     out = {"k_cell_act": act, "k_cell_scan": scan, "k_cell_scatter": scat}
     if gwords:
         out["k_genome"] = births * bm["SCAT_GENOME_WORDS"] * gwords * 8
@@ -147,9 +146,11 @@ def c4_leg(seed, n, steps, graph=True, tune=True, kernel_timing=True):
         for p, a, b in zip(run.pops, s0, s1):
             prm = p.params
             ops = {prm.action_kind[i] for i in range(prm.num_actions)}
-            for k, v in kernel_bytes(bm, delta(b, a, ks), p.grid, ops, 0).items():
+            dd = delta(b, a, ks)
+            for k, v in kernel_bytes(bm, dd, p.grid, ops, 0).items():
                 kk = k.replace("k_cell_", "k_batch_")
                 tot[kk] = tot.get(kk, 0) + v
+            tot["k_batch_scan_sums"] = tot.get("k_batch_scan_sums", 0) + dd["occupied"] * bm["SCAN_SUMS_OCC"]
         kernels, roof = kernel_table(tot, times)
     run.close()
     return {"workload": C4_DESC, "replicas": n, "steps_per_replica": steps, "base_seed": seed,

@@ -1020,82 +1020,103 @@ __device__ __forceinline__ uint32_t next_tile(uint32_t *ticket, ScanLds &L) {
     return t;
 }
 
-// One look-back tile of k_cell_scan (kCellTile cells), all kScanBlock threads.
-template <typename CtxT>
-__device__ __forceinline__ void scan_tile(const CtxT &x, const StepArgs &sa, uint32_t tile, ScanLds &L) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t nc = x.ncells;
-    const int64_t base = (int64_t)tile * kCellTile;
-    const bool last = base + kCellTile >= nc;
-    if (!last && x.tact[tile] == 0u) {
-        // no cell of the tile holds agents next step: its sums are zero and
-        // none of its per-cell outputs is read (the step totals come from
-        // the last tile, which always runs)
-        if (threadIdx.x == 0) {
-            const uint64_t f = (tile == 0) ? kFlagInc : kFlagAgg;
-            tile_publish(&x.tiles[tile], f);
-            tile_publish(&x.tiles2[tile], f);
-        }
-        return;
-    }
+// The inputs of one scan tile, per thread (kScanPer cells each): stayers,
+// arrivals, incoming-run counts (dense steps), and next count | births << 32.
+struct TileIn {
     uint32_t stay[kScanPer], arr[kScanPer];
     uint32_t cm[kScanPer][kMaxIn];
     uint64_t v[kScanPer];
-    // stayers, births and arrivals of every cell.  With many cells occupied
-    // (sa.dense) the incoming-run counts of every cell are loaded in the same
-    // round trip; otherwise only those of cells that receive movers, later.
-    // The counts go back to zero once read.
-    const bool dense = sa.dense != 0;
-    uint64_t sum = 0, osum = 0;
+    uint64_t sum, osum;   // this thread's next agents | births, and occupied cells
+};
+
+// A tile none of whose cells holds agents next step (the step's last tile
+// always runs: it closes the bookkeeping).
+template <typename CtxT>
+__device__ __forceinline__ bool tile_idle(const CtxT &x, uint32_t tile) {
+    const int64_t base = (int64_t)tile * kCellTile;
+    return base + kCellTile < x.ncells && x.tact[tile] == 0u;
+}
+
+// Reads a tile's inputs.  With clear, the counts go back to zero once read
+// (the scan proper); without, they stay (a pre-pass that only sums them).
+// With many cells occupied (sa.dense) the incoming-run counts of every cell
+// are loaded in the same round trip; otherwise only those of cells that
+// receive movers, at the write.
+template <typename CtxT>
+__device__ __forceinline__ void tile_load(const CtxT &x, const StepArgs &sa, uint32_t tile, bool clear, TileIn &I) {
+    const int64_t nc = x.ncells;
+    const int64_t base = (int64_t)tile * kCellTile;
+    const bool dense = clear && sa.dense != 0;
+    I.sum = 0;
+    I.osum = 0;
 #pragma unroll
     for (int q = 0; q < kScanPer; q++) {
         const int64_t c = base + (int64_t)threadIdx.x * kScanPer + q;
         const bool ok = c < nc;
-        stay[q] = ok ? x.cstay[c] : 0u;
+        I.stay[q] = ok ? x.cstay[c] : 0u;
         const uint32_t b = ok ? x.cbirth[c] : 0u;
-        arr[q] = ok ? x.carr[c] : 0u;
+        I.arr[q] = ok ? x.carr[c] : 0u;
 #pragma unroll
-        for (int k = 0; k < kMaxIn; k++) cm[q][k] = (dense && ok && k < x.in_w) ? x.cin[c * x.in_w + k] : 0u;
-        if (stay[q]) x.cstay[c] = 0;
-        if (b) x.cbirth[c] = 0;
-        if (arr[q]) x.carr[c] = 0;
-        v[q] = ok ? ((uint64_t)(stay[q] + arr[q] + b) | ((uint64_t)b << 32)) : 0ull;
-        sum += v[q];
-        osum += ((uint32_t)v[q] != 0u) ? 1u : 0u;
+        for (int k = 0; k < kMaxIn; k++) I.cm[q][k] = (dense && ok && k < x.in_w) ? x.cin[c * x.in_w + k] : 0u;
+        if (clear) {
+            if (I.stay[q]) x.cstay[c] = 0;
+            if (b) x.cbirth[c] = 0;
+            if (I.arr[q]) x.carr[c] = 0;
+        }
+        I.v[q] = ok ? ((uint64_t)(I.stay[q] + I.arr[q] + b) | ((uint64_t)b << 32)) : 0ull;
+        I.sum += I.v[q];
+        I.osum += ((uint32_t)I.v[q] != 0u) ? 1u : 0u;
     }
-    // block exclusive scan of the per-thread sums
-    const uint64_t incl = wave_incl_scan_u64(sum);
-    const uint64_t oincl = wave_incl_scan_u64(osum);
+}
+
+// Block sums of a tile's inputs: the tile totals, and this thread's
+// exclusive offsets inside the tile.
+struct TileSums {
+    uint64_t tot, otot, pre, opre;
+};
+
+__device__ __forceinline__ TileSums tile_sums(const TileIn &I, ScanLds &L) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t incl = wave_incl_scan_u64(I.sum);
+    const uint64_t oincl = wave_incl_scan_u64(I.osum);
     if (lane == 63) { L.w[wave] = incl; L.w2[wave] = oincl; }
     __syncthreads();
-    uint64_t wpre = 0, tot = 0, owpre = 0, otot = 0;
+    TileSums S{0, 0, 0, 0};
+    uint64_t wpre = 0, owpre = 0;
 #pragma unroll
     for (int w = 0; w < kScanBlock / 64; w++) {
         if (w < wave) { wpre += L.w[w]; owpre += L.w2[w]; }
-        tot += L.w[w];
-        otot += L.w2[w];
+        S.tot += L.w[w];
+        S.otot += L.w2[w];
     }
-    if (threadIdx.x < 64) {
-        uint64_t ex, ex2;
-        lookback2(x.tiles, x.tiles2, tile, tot, otot, &x.st->error, ex, ex2);
-        if (threadIdx.x == 0) { L.excl = ex; L.excl2 = ex2; }
-    }
-    __syncthreads();
-    uint64_t run = L.excl + wpre + incl - sum;
-    uint64_t orun = L.excl2 + owpre + oincl - osum;
+    S.pre = wpre + incl - I.sum;
+    S.opre = owpre + oincl - I.osum;
+    return S;
+}
+
+// Writes a tile's outputs from its exclusive prefix (ex agents | births,
+// ex2 occupied cells): next offsets, run and newborn positions, the next
+// occupied-cell list; the step's last tile closes the bookkeeping.
+template <typename CtxT>
+__device__ __forceinline__ void tile_write(const CtxT &x, const StepArgs &sa, uint32_t tile, const TileIn &I,
+                                           uint64_t run, uint64_t orun) {
+    const int64_t nc = x.ncells;
+    const int64_t base = (int64_t)tile * kCellTile;
+    const bool last = base + kCellTile >= nc;
+    const bool dense = sa.dense != 0;
 #pragma unroll
     for (int q = 0; q < kScanPer; q++) {
         const int64_t c = base + (int64_t)threadIdx.x * kScanPer + q;
         if (c < nc) {
             const uint32_t o = (uint32_t)run;
             x.off_nxt[c] = o;
-            uint32_t pos = o + stay[q];
-            if (arr[q]) {
+            uint32_t pos = o + I.stay[q];
+            if (I.arr[q]) {
                 // arrivals: one block per incoming run, in run order
 #pragma unroll
                 for (int k = 0; k < kMaxIn; k++) {
                     if (k >= x.in_w) break;
-                    const uint32_t m = dense ? cm[q][k] : x.cin[c * x.in_w + k];
+                    const uint32_t m = dense ? I.cm[q][k] : x.cin[c * x.in_w + k];
                     if (m) {
                         x.rpos[x.in_e[c * x.in_w + k]] = pos;
                         x.cin[c * x.in_w + k] = 0;
@@ -1103,17 +1124,17 @@ __device__ __forceinline__ void scan_tile(const CtxT &x, const StepArgs &sa, uin
                     }
                 }
             }
-            if ((uint32_t)(v[q] >> 32)) {
+            if ((uint32_t)(I.v[q] >> 32)) {
                 x.bpos[c] = pos;
                 x.birth_off[c] = (uint32_t)(run >> 32);
             }
-            const uint32_t nn = (uint32_t)v[q];
+            const uint32_t nn = (uint32_t)I.v[q];
             if (nn != 0) {
                 x.occ_nxt[orun] = OccCell{(int32_t)c, o, nn, 0u};
                 orun++;
             }
         }
-        run += v[q];
+        run += I.v[q];
     }
     if (threadIdx.x == 0) x.tact[tile] = 0u;
     if (last && threadIdx.x == kScanBlock - 1) {
@@ -1137,6 +1158,32 @@ __device__ __forceinline__ void scan_tile(const CtxT &x, const StepArgs &sa, uin
         st->step = (int64_t)sa.step + 1;
         st->t = sa.t + 1.0;
     }
+}
+
+// One look-back tile of k_cell_scan (kCellTile cells), all kScanBlock threads.
+template <typename CtxT>
+__device__ __forceinline__ void scan_tile(const CtxT &x, const StepArgs &sa, uint32_t tile, ScanLds &L) {
+    if (tile_idle(x, tile)) {
+        // no cell of the tile holds agents next step: its sums are zero and
+        // none of its per-cell outputs is read (the step totals come from
+        // the last tile, which always runs)
+        if (threadIdx.x == 0) {
+            const uint64_t f = (tile == 0) ? kFlagInc : kFlagAgg;
+            tile_publish(&x.tiles[tile], f);
+            tile_publish(&x.tiles2[tile], f);
+        }
+        return;
+    }
+    TileIn I;
+    tile_load(x, sa, tile, true, I);
+    const TileSums S = tile_sums(I, L);
+    if (threadIdx.x < 64) {
+        uint64_t ex, ex2;
+        lookback2(x.tiles, x.tiles2, tile, S.tot, S.otot, &x.st->error, ex, ex2);
+        if (threadIdx.x == 0) { L.excl = ex; L.excl2 = ex2; }
+    }
+    __syncthreads();
+    tile_write(x, sa, tile, I, L.excl + S.pre, L.excl2 + S.opre);
     __syncthreads();
 }
 
@@ -1647,10 +1694,9 @@ struct BatchArgs {
     int32_t npops;
     int32_t total_tiles;      // scan tiles of all members
     const int32_t *tile_base; // npops + 1: first global tile of each member
-    uint64_t *tiles;          // look-back words of all members (both words)
-    uint64_t *tiles2;
+    uint64_t *tiles;          // per tile of every member: agents | births totals
+    uint64_t *tiles2;         //   ... and occupied-cell totals (k_batch_scan_sums)
     int64_t *bctr;            // BC_COUNT copies of the batch step index
-    uint32_t *ticket;         // k_batch_scan's tile tickets (zeroed by k_batch_act)
     double group_agents;      // target agents per wave group
     int32_t genome;           // k_batch_genome runs in this step
     int32_t pad;
@@ -1785,12 +1831,6 @@ __global__ void __launch_bounds__(kCellBlock) k_batch_act(BatchArgs B) {
     __shared__ ActLds lds;
     const int lane = threadIdx.x;
     const int64_t bstep = batch_step_in(B, BC_ACT, BC_SCAN);
-    // every member's look-back words and the scan's tile tickets start at zero
-    for (int32_t i = blockIdx.x * kCellBlock + lane; i < B.total_tiles; i += gridDim.x * kCellBlock) {
-        B.tiles[i] = 0;
-        B.tiles2[i] = 0;
-    }
-    if (blockIdx.x == 0 && lane == 0) *B.ticket = 0u;
     int64_t cnt[4], base[4];
     batch_groups(B, bstep, cnt);
     const int64_t total = pop_prefix(cnt, base);
@@ -1848,24 +1888,83 @@ __global__ void __launch_bounds__(kCellBlock) k_batch_act(BatchArgs B) {
     }
 }
 
-__global__ void __launch_bounds__(kScanBlock) k_batch_scan(BatchArgs B) {
+// The batched scan runs in two passes, so no block ever waits on another
+// (a look-back across thousands of member tiles would serialise on its tile
+// handout, and static tile ownership could wait on blocks that other
+// streams keep from running).  Pass 1: every tile's totals (agents |
+// births, occupied cells) into the member's tile words; idle tiles are zero.
+__global__ void __launch_bounds__(kScanBlock) k_batch_scan_sums(BatchArgs B) {
     __shared__ ScanLds lds;
     const int lane = threadIdx.x & 63;
-    const int64_t bstep = batch_step_in(B, BC_SCAN, BC_SCATTER);
-    // static tile prefix; every wave of the block computes it the same way
+    const int64_t bstep = uniform_i64(B.bctr[BC_SCAN]);
     int64_t tb[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const int r = lane + 64 * k;
         tb[k] = (r < B.npops) ? B.tile_base[r] : 0;
     }
-    for (int64_t t = next_tile(B.ticket, lds); t < B.total_tiles; t = next_tile(B.ticket, lds)) {
+    for (int64_t t = blockIdx.x; t < B.total_tiles; t += gridDim.x) {
         const int r = pop_of(tb, B.npops, t);
         const int64_t lt = t - lane_value(tb, r);
         const DevStatus *st = pop_desc(B, r).ctx[0].st;
         if (st->error) continue;
         const StepArgs sa = pop_step_args_uniform(pop_desc(B, r), bstep, B.group_agents);
-        scan_tile(*uniform_ptr(&pop_desc(B, r).ctx[sa.parity]), sa, (uint32_t)lt, lds);
+        const ConstCtx &x = *uniform_ptr(&pop_desc(B, r).ctx[sa.parity]);
+        uint64_t tot = 0, otot = 0;
+        if (!tile_idle(x, (uint32_t)lt)) {
+            TileIn I;
+            tile_load(x, sa, (uint32_t)lt, false, I);
+            const TileSums S = tile_sums(I, lds);
+            tot = S.tot;
+            otot = S.otot;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            B.tiles[t] = tot;
+            B.tiles2[t] = otot;
+        }
+    }
+}
+
+// Pass 2: each busy tile sums its member's earlier tile totals for its
+// exclusive prefix, then scans and writes as k_cell_scan does.
+__global__ void __launch_bounds__(kScanBlock) k_batch_scan(BatchArgs B) {
+    __shared__ ScanLds lds;
+    const int lane = threadIdx.x & 63;
+    const int64_t bstep = batch_step_in(B, BC_SCAN, BC_SCATTER);
+    int64_t tb[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int r = lane + 64 * k;
+        tb[k] = (r < B.npops) ? B.tile_base[r] : 0;
+    }
+    for (int64_t t = blockIdx.x; t < B.total_tiles; t += gridDim.x) {
+        const int r = pop_of(tb, B.npops, t);
+        const int64_t first = lane_value(tb, r);
+        const int64_t lt = t - first;
+        const DevStatus *st = pop_desc(B, r).ctx[0].st;
+        if (st->error) continue;
+        const StepArgs sa = pop_step_args_uniform(pop_desc(B, r), bstep, B.group_agents);
+        const ConstCtx &x = *uniform_ptr(&pop_desc(B, r).ctx[sa.parity]);
+        if (tile_idle(x, (uint32_t)lt)) continue;
+        if (threadIdx.x < 64) {
+            uint64_t a = 0, o = 0;
+            for (int64_t u = first + lane; u < t; u += 64) {
+                a += B.tiles[u];
+                o += B.tiles2[u];
+            }
+            a = wave_sum_u64(a);
+            o = wave_sum_u64(o);
+            if (lane == 0) {
+                lds.excl = a;
+                lds.excl2 = o;
+            }
+        }
+        TileIn I;
+        tile_load(x, sa, (uint32_t)lt, true, I);
+        const TileSums S = tile_sums(I, lds);
+        tile_write(x, sa, (uint32_t)lt, I, lds.excl + S.pre, lds.excl2 + S.opre);
+        __syncthreads();
     }
 }
 

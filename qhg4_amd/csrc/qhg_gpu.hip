@@ -1532,7 +1532,7 @@ int qhg_gpu_autotune(qhg_pop *p, int32_t trial_steps, qhg_tuning *chosen, int32_
 // ===========================================================================
 // Batched stepping (many populations per launch)
 // ===========================================================================
-enum BatchKernelId { BK_ACT, BK_SCAN, BK_SCATTER, BK_GENOME, BK_COUNT };
+enum BatchKernelId { BK_ACT, BK_SCAN, BK_SCATTER, BK_GENOME, BK_SCAN_SUMS, BK_COUNT };
 
 struct qhg_batch {
     hipStream_t stream = nullptr;
@@ -1555,7 +1555,6 @@ struct qhg_batch {
     int64_t ev_step[kRing];
     int64_t done_bstep = 0;        // every batch step below this one is complete
     int64_t *d_bctr = nullptr;     // the kernels' copies of the batch step index
-    uint32_t *d_ticket = nullptr;  // the batch scan's tile tickets
     // launch shapes (results do not depend on them)
     double group_agents = QHG_GROUP_AGENTS;
     int blocks_per_cu = QHG_CELL_GRID_PER_CU;
@@ -1613,7 +1612,6 @@ BatchArgs batch_args(const qhg_batch *b) {
     A.tiles = b->d_tiles;
     A.tiles2 = b->d_tiles2;
     A.bctr = b->d_bctr;
-    A.ticket = b->d_ticket;
     A.group_agents = b->group_agents;
     A.genome = b->any_genetics ? 1 : 0;
     return A;
@@ -1628,7 +1626,10 @@ int batch_launch_step(qhg_batch *b) {
     int rc = b->any_death ? blaunch(b, BK_ACT, k_batch_act<true>, ag, kCellBlock, A)
                           : blaunch(b, BK_ACT, k_batch_act<false>, ag, kCellBlock, A);
     if (rc) return rc;
-    rc = blaunch(b, BK_SCAN, k_batch_scan, std::max(1, std::min(b->scan_grid, b->total_tiles)), kScanBlock, A);
+    const int sg = std::max(1, std::min(b->scan_grid, b->total_tiles));
+    rc = blaunch(b, BK_SCAN_SUMS, k_batch_scan_sums, sg, kScanBlock, A);
+    if (rc) return rc;
+    rc = blaunch(b, BK_SCAN, k_batch_scan, sg, kScanBlock, A);
     if (rc) return rc;
     // one more block than the groups use: the members' bookkeeping
     rc = blaunch(b, BK_SCATTER, k_batch_scatter, ag + 1, kCellBlock, A);
@@ -1762,9 +1763,7 @@ int qhg_batch_create(qhg_batch **out) {
         b->ev_step[k] = -1;
     }
     if (hipMalloc(&b->d_bctr, BC_COUNT * sizeof(int64_t)) != hipSuccess ||
-        hipMemset(b->d_bctr, 0, BC_COUNT * sizeof(int64_t)) != hipSuccess ||
-        hipMalloc(&b->d_ticket, sizeof(uint32_t)) != hipSuccess ||
-        hipMemset(b->d_ticket, 0, sizeof(uint32_t)) != hipSuccess) {
+        hipMemset(b->d_bctr, 0, BC_COUNT * sizeof(int64_t)) != hipSuccess) {
         qhg_batch_destroy(b);
         return fail(QHG_ERR_NOMEM, "batch counters");
     }
@@ -1782,7 +1781,7 @@ int qhg_batch_destroy(qhg_batch *b) {
     while (!b->pops.empty()) (void)qhg_batch_remove(b, b->pops.back());
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     batch_drop_graph(b);
-    void *ptrs[] = {b->d_desc, b->d_tile_base, b->d_tiles, b->d_tiles2, b->d_bctr, b->d_ticket};
+    void *ptrs[] = {b->d_desc, b->d_tile_base, b->d_tiles, b->d_tiles2, b->d_bctr};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     for (int k = 0; k < kRing; k++)
@@ -1985,7 +1984,8 @@ int qhg_batch_set_timing(qhg_batch *b, int32_t enable) {
 }
 
 const char *qhg_batch_kernel_name(int32_t index) {
-    static const char *names[BK_COUNT] = {"k_batch_act", "k_batch_scan", "k_batch_scatter", "k_batch_genome"};
+    static const char *names[BK_COUNT] = {"k_batch_act", "k_batch_scan", "k_batch_scatter", "k_batch_genome",
+                                          "k_batch_scan_sums"};
     if (index < 0 || index >= BK_COUNT) return "";
     return names[index];
 }

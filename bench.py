@@ -127,23 +127,19 @@ def c4_leg(seed, n, steps, graph=True, tune=True, kernel_timing=True):
     print("bench: plain %.2f s; batched" % t_plain, file=sys.stderr, flush=True)
     run = ensemble.BatchedRun(make, n, graph=graph)
     dev_bytes = run.device_bytes()
-    t_batch, tuning = run.run(steps, tune_steps=2 if tune else 0)
+    t_batch, tuning = run.run(steps, tune_steps=2 if tune else 0, timing_steps=5 if kernel_timing else 0)
     batched = run.summaries()
     identical = batched == plain
     agent_steps = sum(s["agent_steps"] for s in batched)
     print("bench: batched %.2f s, identical=%s" % (t_batch, identical), file=sys.stderr, flush=True)
-    # per-kernel device time over a few more steps, timed per dispatch
+    # per-kernel device time over the steps timed per dispatch right after
+    # tuning, while the members hold their early (largest) populations
     roof, kernels = None, {}
-    if kernel_timing:
-        ks = 5
-        s0 = [p.stats() for p in run.pops]
-        run.batch.set_timing(True)
-        run.batch.run(ks)
-        times = run.batch.kernel_times()
-        s1 = [p.stats() for p in run.pops]
+    if run.timed:
+        ks = run.timed["steps"]
         bm = bytes_model()
         tot = {}
-        for p, a, b in zip(run.pops, s0, s1):
+        for p, a, b in zip(run.pops, run.timed["before"], run.timed["after"]):
             prm = p.params
             ops = {prm.action_kind[i] for i in range(prm.num_actions)}
             dd = delta(b, a, ks)
@@ -151,7 +147,7 @@ def c4_leg(seed, n, steps, graph=True, tune=True, kernel_timing=True):
                 kk = k.replace("k_cell_", "k_batch_")
                 tot[kk] = tot.get(kk, 0) + v
             tot["k_batch_scan_sums"] = tot.get("k_batch_scan_sums", 0) + dd["occupied"] * bm["SCAN_SUMS_OCC"]
-        kernels, roof = kernel_table(tot, times)
+        kernels, roof = kernel_table(tot, run.timed["times"])
     run.close()
     return {"workload": C4_DESC, "replicas": n, "steps_per_replica": steps, "base_seed": seed,
             "initial_agents_total": int(sum(scenarios.c4_draw(r, seed)["n0"] for r in range(n))),
@@ -159,7 +155,7 @@ def c4_leg(seed, n, steps, graph=True, tune=True, kernel_timing=True):
             "value": round(agent_steps / t_batch, 1), "plain_value": round(agent_steps / t_plain, 1),
             "batched_seconds": round(t_batch, 3), "plain_seconds": round(t_plain, 3),
             "speedup_vs_plain": round(t_plain / t_batch, 3), "identical_to_plain": identical, "graph": graph,
-            "tuning": tuning, "kernels_after_run": kernels, "roofline_after_run": roof,
+            "tuning": tuning, "kernels_early": kernels, "roofline_early": roof,
             "make": make}
 
 
@@ -176,9 +172,9 @@ def run_c4(args):
                                         "initial_agents_total", "agent_steps_total", "device_bytes_resident")}
     out["ensemble"] = {k: e[k] for k in ("plain_value", "plain_seconds", "batched_seconds", "speedup_vs_plain",
                                           "identical_to_plain", "graph", "tuning")}
-    out["roofline"] = e["roofline_after_run"]
+    out["roofline"] = e["roofline_early"]
     out["cpu_baseline"] = cpu
-    out["kernels"] = e["kernels_after_run"]
+    out["kernels"] = e["kernels_early"]
     return out
 
 
@@ -428,12 +424,13 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kernels,
     }
-    if not args.no_ensemble:
+    # the ensemble and serving objects ride on the headline (C2) line only
+    if args.workload == "c2" and not args.no_ensemble:
         e = c4_leg(1000, args.replicas, 1000, graph=not args.no_graph, tune=not args.no_tune,
                    kernel_timing=not args.no_kernel_timing)
         e.pop("make")
         out["ensemble"] = e
-    if not args.no_serving:
+    if args.workload == "c2" and not args.no_serving:
         e = c5_leg(2000, args.serve_requests, args.budget_gb, args.chunk, args.plain_sample,
                    graph=not args.no_graph)
         e.pop("make")

@@ -2087,19 +2087,48 @@ __global__ void __launch_bounds__(kWave) k_batch_genome(BatchArgs B) {
 // genome pool compaction: the live agents' genomes to handles 0..n-1 in slot
 // order (into a second pool); the handles are rewritten
 // ---------------------------------------------------------------------------
+// Genome pool compaction: the genomes of the buffer's n live agents move to
+// rows 0..n-1 of dst.  Lanes take consecutive words of a row (several rows
+// per wave when a row is shorter than a wave), so every row moves as one
+// coalesced read and write; each wave keeps kPoolUnroll row groups in flight.
+// The handles are renumbered afterwards, by k_gpool_commit.
+constexpr int kPoolUnroll = 4;
+
 __global__ void __launch_bounds__(256) k_gpool_compact(Ctx x, uint64_t *dst, int32_t parity) {
     const int64_t n = x.st->n_alive[parity];
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-        const int64_t h = x.cur.ghandle[i];
-        const uint64_t *src = x.gpool + h * x.gwords;
-        uint64_t *d = dst + i * x.gwords;
-        for (int q = 0; q < x.gwords; q++) d[q] = src[q];
-        x.cur.ghandle[i] = (int32_t)i;
+    const int64_t gw = x.gwords;
+    const int lane = threadIdx.x & 63;
+    const int rpw = gw <= 64 ? (int)(64 / gw) : 1;          // rows per wave pass
+    const int r = gw <= 64 ? lane / (int)gw : 0;            // this lane's row
+    const int q0 = gw <= 64 ? lane - r * (int)gw : lane;    // ... and first word
+    const bool on = r < rpw;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / 64);
+    const int64_t span = (int64_t)rpw * kPoolUnroll;
+    for (int64_t base = wave * span; base < n; base += nwaves * span) {
+        int64_t h[kPoolUnroll];
+#pragma unroll
+        for (int u = 0; u < kPoolUnroll; u++) {
+            const int64_t i = base + (int64_t)u * rpw + r;
+            h[u] = (on && i < n) ? (int64_t)x.cur.ghandle[i] : -1;
+        }
+        for (int64_t q = q0; q < gw; q += 64) {
+            uint64_t v[kPoolUnroll];
+#pragma unroll
+            for (int u = 0; u < kPoolUnroll; u++) v[u] = (h[u] >= 0) ? x.gpool[h[u] * gw + q] : 0ull;
+#pragma unroll
+            for (int u = 0; u < kPoolUnroll; u++)
+                if (h[u] >= 0) dst[(base + (int64_t)u * rpw + r) * gw + q] = v[u];
+        }
     }
 }
 
-__global__ void k_gpool_commit(DevStatus *st, int32_t parity) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) st->gpool[parity] = st->n_alive[parity];
+// after the copy: agent i's genome is row i, and the pool holds n rows
+__global__ void __launch_bounds__(256) k_gpool_commit(Ctx x, int32_t parity) {
+    const int64_t n = x.st->n_alive[parity];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x.cur.ghandle[i] = (int32_t)i;
+    if (blockIdx.x == 0 && threadIdx.x == 0) x.st->gpool[parity] = n;
 }
 
 }  // namespace qhg

@@ -564,9 +564,9 @@ int compact_pool(qhg_pop *p, int64_t new_gcap) {
         HIPCHK(hipMalloc(&p->d_gpool[other], (size_t)new_gcap * p->gwords * 8));
     }
     const int32_t par = p->cur();
-    int rc = launch(p, KB_GPOOL, k_gpool_compact, p->num_cu * 4, p->ctx(), p->d_gpool[other], par);
+    int rc = launch(p, KB_GPOOL, k_gpool_compact, p->num_cu * 8, p->ctx(), p->d_gpool[other], par);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_gpool_commit, dim3(1), dim3(64), 0, p->stream, p->d_st, par);
+    hipLaunchKernelGGL(k_gpool_commit, dim3(p->num_cu * 4), dim3(256), 0, p->stream, p->ctx(), par);
     HIPCHK(hipGetLastError());
     if (new_gcap != p->gcap) {
         // the old pool is no longer referenced once the copy has run

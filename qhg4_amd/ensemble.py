@@ -66,14 +66,29 @@ class BatchedRun:
         self.batch.set_graph(graph)
         self.batch.sync()
 
-    def run(self, steps, tune_steps=2):
+    def run(self, steps, tune_steps=2, timing_steps=0):
         """Advance every member by `steps` steps, the autotuner's trial steps
-        included.  Returns (seconds, chosen launch shape)."""
+        included.  Returns (seconds, chosen launch shape).  With
+        timing_steps, that many of the steps right after the tuning run with
+        per-dispatch timing (eagerly, not replayed from the graph), while the
+        members hold their early populations; self.timed then holds the
+        kernel times and the members' stats before and after them."""
         t0 = time.perf_counter()
         taken = 0
         tuning = self.batch.get_tuning()
         if tune_steps and steps >= 20 * tune_steps:   # the search takes 9 trials
             tuning, taken = self.batch.autotune(tune_steps)
+        self.timed = None
+        if timing_steps and steps - taken > timing_steps:
+            before = [p.stats() for p in self.pops]
+            self.batch.set_timing(True)
+            self.batch.run(timing_steps)
+            self.batch.sync()
+            times = self.batch.kernel_times()
+            self.batch.set_timing(False)
+            self.timed = {"steps": timing_steps, "times": times, "before": before,
+                          "after": [p.stats() for p in self.pops]}
+            taken += timing_steps
         if steps > taken:
             self.batch.run(steps - taken)
         self.batch.sync()

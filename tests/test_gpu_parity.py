@@ -105,6 +105,22 @@ def test_compaction_keeps_state():
     _lockstep(sc, 90, 15, compact_at=(15, 45, 60))
 
 
+@pytest.mark.parametrize("genome_size", [100, 256, 300, 1000, 2048])
+def test_genome_pool_compaction(genome_size):
+    """Forced genome-pool compaction with genome rows shorter than a wave,
+    rows that do not divide it, and rows longer than a wave (2048 loci:
+    128 words)."""
+    print("case: genome pool compaction, genome size %d, ico level 4, 3k agents, 36 steps" % genome_size,
+          flush=True)
+    sc = scenarios.c2(seed=31 + genome_size, level=4, n_agents=3000, patch_cells=150, genetics=True)
+    sc["params"].gen_genome_size = genome_size
+    gw = 2 * abi.genome_words(genome_size)
+    rng = np.random.default_rng(genome_size)
+    n = len(sc["agents"]["id"])
+    sc["agents"]["genome"] = rng.integers(0, 2**63, n * gw, dtype=np.uint64)
+    _lockstep(sc, 36, 6, use_run=True, compact_at=(6, 12, 24, 30))
+
+
 def test_growth_beyond_capacity():
     print("case: capacity growth from a tiny initial capacity", flush=True)
     sc = tut_verhulst(scenarios.c1(seed=13, width=64, height=64, n_agents=3000))

@@ -1191,7 +1191,12 @@ __global__ void __launch_bounds__(kScanBlock) k_cell_scan(Ctx x, StepArgs sa) {
     __shared__ ScanLds lds;
     const int lane = threadIdx.x & 63;
     if (x.st->error) return;
-    for (uint32_t tile = next_tile(x.tick, lds); tile < (uint32_t)x.ntiles; tile = next_tile(x.tick, lds))
+    // when the grid has a block per tile, the first round of tickets hands
+    // out every tile: no block asks again (that round trip would only end
+    // the kernel later)
+    const uint32_t nt = (uint32_t)x.ntiles;
+    const bool one_round = gridDim.x >= nt;
+    for (uint32_t tile = next_tile(x.tick, lds); tile < nt; tile = one_round ? nt : next_tile(x.tick, lds))
         scan_tile(x, sa, tile, lds);
     {
         // off the critical path: this block's slice of the k_cell_act

@@ -729,6 +729,23 @@ __device__ __forceinline__ void group_act(const CtxT &x, const ProgT &p, const S
         if (p.need_lastb) lastb0 = x.cur.lastb[slot0];
         if (p.load_age) age0 = x.cur.age[slot0];
     }
+    // the second chunk's fields in the same round trip (groups hold up to
+    // two chunks at the usual target; later chunks load where they are used)
+    const uint32_t q1 = (uint32_t)(kWave + lane);
+    const bool v1 = q1 < T;
+    uint8_t life1 = 0, g1 = 2;
+    int64_t id1 = 0;
+    double birth1 = 0, lastb1 = 0, age1 = 0;
+    int32_t nbab1 = 0;
+    if (v1) {
+        life1 = x.cur.life[S0 + q1];
+        g1 = x.cur.gender[S0 + q1];
+        id1 = x.cur.id[S0 + q1];
+        nbab1 = x.cur.nbabies[S0 + q1];
+        if (p.need_birth) birth1 = x.cur.birth[S0 + q1];
+        if (p.need_lastb) lastb1 = x.cur.lastb[S0 + q1];
+        if (p.load_age) age1 = x.cur.age[S0 + q1];
+    }
     // per-cell tables to LDS: their loads were issued before the fields', so
     // waiting for them leaves the field loads in flight
     if (lane < ng) {
@@ -751,9 +768,9 @@ __device__ __forceinline__ void group_act(const CtxT &x, const ProgT &p, const S
             const int k = (ch == 0) ? k0 : group_cell_of(q, eb, ng);
             const uint32_t base = (uint32_t)__shfl((int)eb, k, 64);
             const uint32_t nk = (uint32_t)__shfl((int)e.n, k, 64);
-            uint8_t l = life0, g = g0;
-            int64_t id = id0;
-            if (ch > 0 && valid) {
+            uint8_t l = (ch == 0) ? life0 : life1, g = (ch == 0) ? g0 : g1;
+            int64_t id = (ch == 0) ? id0 : id1;
+            if (ch > 1 && valid) {
                 l = x.cur.life[S0 + q];
                 g = x.cur.gender[S0 + q];
                 id = x.cur.id[S0 + q];
@@ -836,20 +853,20 @@ __device__ __forceinline__ void group_act(const CtxT &x, const ProgT &p, const S
         const int k = (ch == 0) ? k0 : group_cell_of(q, eb, ng);
         const uint32_t nk = (uint32_t)__shfl((int)e.n, k, 64);
         ActItem a;
-        a.life = life0;
-        a.g = g0;
-        a.id = id0;
-        a.birth = birth0;
-        a.lastb = lastb0;
-        a.age = age0;
+        a.life = (ch == 0) ? life0 : life1;
+        a.g = (ch == 0) ? g0 : g1;
+        a.id = (ch == 0) ? id0 : id1;
+        a.birth = (ch == 0) ? birth0 : birth1;
+        a.lastb = (ch == 0) ? lastb0 : lastb1;
+        a.age = (ch == 0) ? age0 : age1;
         a.mate = -1;
         a.age_dirty = false;
         a.bflag = 0;
         a.npend = 0;
         a.pk0 = a.pk1 = 0;
         a.pu0 = a.pu1 = 0;
-        int32_t nbab = nbab0;
-        if (ch > 0) {
+        int32_t nbab = (ch == 0) ? nbab0 : nbab1;
+        if (ch > 1) {
             a.life = 0;
             if (valid) {
                 a.life = x.cur.life[slot];

@@ -1042,6 +1042,7 @@ __device__ __forceinline__ uint32_t next_tile(uint32_t *ticket, ScanLds &L) {
 struct TileIn {
     uint32_t stay[kScanPer], arr[kScanPer];
     uint32_t cm[kScanPer][kMaxIn];
+    int32_t ie[kScanPer][kMaxIn];   // dense steps: the incoming runs' source slots
     uint64_t v[kScanPer];
     uint64_t sum, osum;   // this thread's next agents | births, and occupied cells
 };
@@ -1074,7 +1075,11 @@ __device__ __forceinline__ void tile_load(const CtxT &x, const StepArgs &sa, uin
         const uint32_t b = ok ? x.cbirth[c] : 0u;
         I.arr[q] = ok ? x.carr[c] : 0u;
 #pragma unroll
-        for (int k = 0; k < kMaxIn; k++) I.cm[q][k] = (dense && ok && k < x.in_w) ? x.cin[c * x.in_w + k] : 0u;
+        for (int k = 0; k < kMaxIn; k++) {
+            const bool rd = dense && ok && k < x.in_w;
+            I.cm[q][k] = rd ? x.cin[c * x.in_w + k] : 0u;
+            I.ie[q][k] = rd ? x.in_e[c * x.in_w + k] : -1;
+        }
         if (clear) {
             if (I.stay[q]) x.cstay[c] = 0;
             if (b) x.cbirth[c] = 0;
@@ -1135,7 +1140,7 @@ __device__ __forceinline__ void tile_write(const CtxT &x, const StepArgs &sa, ui
                     if (k >= x.in_w) break;
                     const uint32_t m = dense ? I.cm[q][k] : x.cin[c * x.in_w + k];
                     if (m) {
-                        x.rpos[x.in_e[c * x.in_w + k]] = pos;
+                        x.rpos[dense ? I.ie[q][k] : x.in_e[c * x.in_w + k]] = pos;
                         x.cin[c * x.in_w + k] = 0;
                         pos += m;
                     }

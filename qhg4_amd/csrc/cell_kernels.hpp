@@ -1377,9 +1377,15 @@ __device__ __forceinline__ void group_scatter(const CtxT &x, const StepArgs &sa,
     const int lane = threadIdx.x;
     const uint64_t lt = lanemask_lt();
     const int nchunks = (int)((T + kWave - 1) / kWave);
+    // the cells' newborn positions and birth ranks, read now so the newborn
+    // pass below needs no round trip of its own (values of cells without
+    // births are stale and unused)
+    uint32_t cb_pos = 0, cb_off = 0;
     if (lane < ng) {
 #pragma unroll
         for (int q = 0; q < kScN; q++) G.cnt[lane][q] = 0;
+        cb_pos = x.bpos[e.c];
+        cb_off = x.birth_off[e.c];
     }
     wave_sync();
     for (int ch = 0; ch < nchunks; ch++) {
@@ -1464,8 +1470,9 @@ __device__ __forceinline__ void group_scatter(const CtxT &x, const StepArgs &sa,
         const uint32_t q = (uint32_t)(ch * kWave + lane);
         const bool valid = q < T;
         const int k = group_cell_of(q, eb, ng);
-        const int32_t c = __shfl(e.c, k, 64);
         const uint32_t base = (uint32_t)__shfl((int)eb, k, 64);
+        const uint32_t kpos = (uint32_t)__shfl((int)cb_pos, k, 64);
+        const uint32_t koff = (uint32_t)__shfl((int)cb_off, k, 64);
         if (!valid) continue;
         const uint32_t nm = G.cnt[k][kScMoth];
         const uint32_t t = q - base;
@@ -1473,8 +1480,8 @@ __device__ __forceinline__ void group_scatter(const CtxT &x, const StepArgs &sa,
         const int64_t my = G.mid[q];
         uint32_t r = 0;
         for (uint32_t u = 0; u < nm; u++) r += G.mid[base + u] < my;
-        const int64_t to = (int64_t)x.bpos[c] + r;
-        const int64_t rg = (int64_t)x.birth_off[c] + r;
+        const int64_t to = (int64_t)kpos + r;
+        const int64_t rg = (int64_t)koff + r;
         const int64_t nid = max_id + 1 + rg;
         x.nxt.id[to] = nid;
         x.nxt.mate_id[to] = -3;

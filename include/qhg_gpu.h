@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define QHG_ABI_VERSION 6
+#define QHG_ABI_VERSION 7
 
 /* error codes */
 #define QHG_OK              0
@@ -61,6 +61,9 @@ extern "C" {
 #define QHG_ACT_GENETICS     10   /* QHG4/actions/Genetics.cpp:285-337        */
 #define QHG_ACT_OLDAGEDEATH  11   /* QHG4/actions/OldAgeDeath.cpp (execute)   */
 #define QHG_ACT_ATANDEATH    12   /* QHG4/actions/ATanDeath.cpp:48-90         */
+#define QHG_ACT_ASSORTPAIR   13   /* QHG4/actions/AssortativePairHam.cpp:93-238
+                                     + QHG4/utils/RankTable.cpp:119-278        */
+#define QHG_ACT_LAST         QHG_ACT_ASSORTPAIR
 
 /* MultiEvaluator combination modes, QHG4/actions/MultiEvaluator.h:20-25 */
 #define QHG_ME_ADD_SIMPLE 0
@@ -86,6 +89,8 @@ extern "C" {
 #define QHG_STREAM_ACT      2   /* per-agent action words, QHG_ACTW_*           */
 #define QHG_STREAM_NEWBORN  6   /* word 0: gender of the newborn                */
 #define QHG_STREAM_GENETICS 7   /* Genetics::makeOffspring draws, in order      */
+#define QHG_STREAM_ASSORT   8   /* AssortativePairHamming: one cell's draws in order
+                                   (key: cell index | (draw >> 16) << 32, word: draw & 0xffff) */
 
 /* words of QHG_STREAM_ACT (one Philox block covers words 0-3) */
 #define QHG_ACTW_WMOVE_GO   0   /* WeightedMove: move?                          */
@@ -159,6 +164,11 @@ typedef struct qhg_params {
     double oad_max_age, oad_uncertainty;
     /* ATanDeath_max_age, ATanDeath_range, ATanDeath_slope */
     double atan_max_age, atan_range, atan_slope;
+    /* AssortativePairHamming_cutoff (a float in the reference),
+       AssortativePairHamming_permute (0 / 1) */
+    double  assort_cutoff;
+    int32_t assort_permute;
+    int32_t assort_pad;
 } qhg_params;
 
 /* Agent records as structure-of-arrays.  Arrays the caller does not need may

@@ -14,6 +14,7 @@
  *   moves              QHG4/core/SPopulation.cpp:1014-1092
  *   counts             QHG4/core/SPopulation.cpp:509-536
  *   actions            QHG4/actions/{GetOld,Fertility,RandomPair,Verhulst,
+ *                      AssortativePairHam (+ QHG4/utils/RankTable.cpp),
  *                      VerhulstVarK,LinearBirth,LinearDeath,RandomMove,
  *                      WeightedMove,SingleEvaluator,MultiEvaluator,
  *                      NPPCapacity,Genetics}.cpp
@@ -730,6 +731,169 @@ static void random_pair(orc *o) {
     }
 }
 
+/* GeneUtils::bitcount2 (QHG4/genes/GeneUtils.cpp:63-68): the 2-bit
+ * nucleotides of x that are not zero */
+static inline int orc_bitcount2(uint64_t x) {
+    return __builtin_popcountll((x | (x >> 1)) & 0x5555555555555555ull);
+}
+
+/* GeneUtils::calcDist (QHG4/genes/GeneUtils.cpp:414-444).  `size_arg` is
+ * read as a genome size in nucleotides: the distance covers
+ * numNucs2Blocks(size_arg) = ceil(size_arg / 32) words per strand position. */
+int32_t orc_calc_dist(const uint64_t *g1, const uint64_t *g2, int32_t size_arg) {
+    const int32_t nb = (int32_t)(((uint32_t)size_arg + 31u) / 32u);
+    int32_t c1 = 0, c2 = 0;
+    for (int32_t k = 0; k < 2 * nb; k++) c1 += orc_bitcount2(g1[k] ^ g2[k]);
+    for (int32_t k = 0; k < nb; k++) c2 += orc_bitcount2(g1[k + nb] ^ g2[k]) + orc_bitcount2(g1[k] ^ g2[k + nb]);
+    return c1 < c2 ? c1 : c2;
+}
+
+/* RankTable::makeAllPairings, pairHighestRankers, randomPair, pair and
+ * clearCross (QHG4/utils/RankTable.cpp:119-278) over the nf x nm table R
+ * (row-major, changed in place).  next(ctx) gives the draws in order.
+ * perm (nf) and midx (nm) are scratch; the pairs go to pairs[2k] (row) and
+ * pairs[2k+1] (column).  Returns the number of pairs. */
+typedef uint32_t (*orc_next_fn)(void *ctx);
+static int32_t rank_pairs(int32_t nf, int32_t nm, float *R, float cutoff, int32_t permute, orc_next_fn next,
+                          void *ctx, int32_t *perm, int32_t *midx, int32_t *pairs) {
+    int32_t npairs = 0;
+    for (;;) {
+        /* current maximum at or above the cutoff; entries below it become 0 */
+        float fmax = 0;
+        for (int32_t i = 0; i < nf; i++)
+            for (int32_t j = 0; j < nm; j++) {
+                float *r = &R[(size_t)i * nm + j];
+                if (*r < cutoff) *r = 0;
+                else if (*r >= fmax) fmax = *r;
+            }
+        if (!(fmax > 0)) break;
+        /* rows that reach the maximum, in row order */
+        uint32_t np = 0;
+        for (int32_t i = 0; i < nf; i++)
+            for (int32_t j = 0; j < nm; j++)
+                if (R[(size_t)i * nm + j] >= fmax) { perm[np++] = i; break; }
+        if (permute) {
+            for (uint32_t i = np - 1; i > 1; i--) {
+                uint32_t k = orc_wrandi(0, i, 1, next(ctx));
+                int32_t t = perm[k]; perm[k] = perm[i]; perm[i] = t;
+            }
+        }
+        int have = 0;
+        for (uint32_t q = 0; q < np; q++) {
+            int32_t i = perm[q], cnt = 0;
+            for (int32_t j = 0; j < nm; j++)
+                if (R[(size_t)i * nm + j] >= fmax) midx[cnt++] = j;
+            if (cnt > 0) {
+                int32_t j = midx[orc_wrandi(0, (uint32_t)cnt, 1, next(ctx))];
+                pairs[2 * npairs] = i;
+                pairs[2 * npairs + 1] = j;
+                npairs++;
+                for (int32_t jj = 0; jj < nm; jj++) R[(size_t)i * nm + jj] = 0;
+                for (int32_t ii = 0; ii < nf; ii++) R[(size_t)ii * nm + j] = 0;
+                have = 1;
+            }
+        }
+        if (!have) break;
+    }
+    return npairs;
+}
+
+typedef struct { const uint32_t *u; int32_t n, used; } orc_seq;
+static uint32_t seq_next(void *ctx) {
+    orc_seq *s = (orc_seq *)ctx;
+    uint32_t v = (s->used < s->n) ? s->u[s->used] : 0u;
+    s->used++;
+    return v;
+}
+
+int32_t orc_rank_pairs(int32_t nf, int32_t nm, float *ranks, float cutoff, int32_t permute, const uint32_t *u,
+                       int32_t nu, int32_t *pairs, int32_t *used) {
+    int32_t *perm = (int32_t *)malloc((size_t)(nf + 1) * 4), *midx = (int32_t *)malloc((size_t)(nm + 1) * 4);
+    orc_seq s = { u, nu, 0 };
+    int32_t n = rank_pairs(nf, nm, ranks, cutoff, permute, seq_next, &s, perm, midx, pairs);
+    free(perm); free(midx);
+    if (used) *used = s.used;
+    return (s.used > nu) ? -1 : n;
+}
+
+/* a cell's draws of AssortativePairHamming, in order */
+typedef struct { uint64_t seed; uint32_t step; int32_t cell; uint32_t j; } orc_cell_draws;
+static uint32_t cell_next(void *ctx) {
+    orc_cell_draws *d = (orc_cell_draws *)ctx;
+    uint32_t j = d->j++;
+    return orc_draw(d->seed, d->step, QHG_STREAM_ASSORT, (uint64_t)(uint32_t)d->cell | ((uint64_t)(j >> 16) << 32),
+                    j & 0xffffu);
+}
+
+/* AssortativePairHam::initialize + findMates
+ * (QHG4/actions/AssortativePairHam.cpp:93-238): per cell, the FERTILE
+ * females and males (id order), the match table
+ * 1 - calcDist(female, male, numBlocks) / genomeSize as a float, and the
+ * RankTable pairing with the cutoff and permute attributes.  The reference
+ * passes the block count where calcDist reads a genome size; that call is
+ * kept. */
+static void assort_pair(orc *o) {
+    int64_t n = o->n;
+    int32_t nc = o->ncells;
+    for (int64_t i = 0; i < n; i++) o->mate[i] = -3;
+    int32_t *off = o->pc_off;
+    memset(off, 0, (size_t)(nc + 1) * 4);
+    for (int64_t i = 0; i < n; i++)
+        if (o->life[i] == QHG_LIFE_FERTILE && o->gender[i] <= 1) off[o->cell[i] + 1]++;
+    for (int32_t c = 0; c < nc; c++) off[c + 1] += off[c];
+    int32_t *fill = (int32_t *)malloc((size_t)nc * 4);
+    memcpy(fill, off, (size_t)nc * 4);
+    for (int64_t i = 0; i < n; i++)
+        if (o->life[i] == QHG_LIFE_FERTILE && o->gender[i] <= 1) o->pc_buf[fill[o->cell[i]]++] = (int32_t)i;
+    free(fill);
+    const int32_t gsize = o->P.gen_genome_size;
+    const int32_t nblocks = (int32_t)(((uint32_t)gsize + 31u) / 32u);
+    const float cutoff = (float)o->P.assort_cutoff;
+    const int32_t permute = o->P.assort_permute != 0;
+#pragma omp parallel
+    {
+        int32_t tcap = 0;
+        int32_t *fem = NULL, *mal = NULL, *perm = NULL, *midx = NULL, *pairs = NULL;
+        float *R = NULL;
+        size_t rcap = 0;
+#pragma omp for schedule(dynamic, 16)
+        for (int32_t c = 0; c < nc; c++) {
+            int32_t s = off[c], e = off[c + 1];
+            if (e - s < 2) continue;
+            if (e - s > tcap) {
+                tcap = e - s;
+                fem = (int32_t *)realloc(fem, (size_t)tcap * 4);
+                mal = (int32_t *)realloc(mal, (size_t)tcap * 4);
+                perm = (int32_t *)realloc(perm, (size_t)tcap * 4);
+                midx = (int32_t *)realloc(midx, (size_t)tcap * 4);
+                pairs = (int32_t *)realloc(pairs, (size_t)tcap * 8);
+            }
+            int32_t F = 0, M = 0;
+            for (int32_t q = s; q < e; q++) {
+                int32_t a = o->pc_buf[q];
+                if (o->gender[a] == 0) fem[F++] = a; else mal[M++] = a;
+            }
+            if (F == 0 || M == 0) continue;
+            if ((size_t)F * M > rcap) { rcap = (size_t)F * M; R = (float *)realloc(R, rcap * 4); }
+            for (int32_t f = 0; f < F; f++)
+                for (int32_t m = 0; m < M; m++) {
+                    int32_t d = 0;
+                    if (o->gw) d = orc_calc_dist(o->genome + (size_t)fem[f] * o->gw, o->genome + (size_t)mal[m] * o->gw,
+                                                 nblocks);
+                    R[(size_t)f * M + m] = (float)(1.0 - (1.0 * d) / gsize);
+                }
+            orc_cell_draws dr = { o->P.seed, o->step, c, 0 };
+            int32_t np = rank_pairs(F, M, R, cutoff, permute, cell_next, &dr, perm, midx, pairs);
+            for (int32_t k = 0; k < np; k++) {
+                int32_t a = fem[pairs[2 * k]], b = mal[pairs[2 * k + 1]];
+                o->mate[a] = b;
+                o->mate[b] = a;
+            }
+        }
+        free(fem); free(mal); free(perm); free(midx); free(pairs); free(R);
+    }
+}
+
 /* LinearBirth::initialize / LinearDeath::initialize,
  * QHG4/actions/LinearBirth.cpp:90-115, LinearDeath.cpp:92-123 */
 static void verhulst_init(orc *o, int varK) {
@@ -1048,6 +1212,7 @@ int orc_step(orc *o) {
     for (int a = 0; a < o->nact; a++) {
         switch (o->act[a]) {
         case QHG_ACT_RANDOMPAIR: random_pair(o); break;
+        case QHG_ACT_ASSORTPAIR: assort_pair(o); break;
         case QHG_ACT_VERHULST: verhulst_init(o, 0); break;
         case QHG_ACT_VERHULSTVARK: verhulst_init(o, 1); break;
         case QHG_ACT_MULTIEVAL: if (o->me_first) { multi_eval(o); o->me_first = 0; } break;

@@ -46,6 +46,11 @@ uint32_t orc_draw(uint64_t seed, uint32_t step, uint32_t stream, uint64_t id, ui
 double   orc_wrandd(uint32_t u);
 double   orc_wrandr(double a, double b, uint32_t u);
 uint32_t orc_wrandi(uint32_t a, uint32_t b, uint32_t s, uint32_t u);
+/* AssortativePairHamming pieces: GeneUtils::calcDist and the RankTable
+ * pairing over a row-major nf x nm table with the draws u[0..nu) */
+int32_t  orc_calc_dist(const uint64_t *g1, const uint64_t *g2, int32_t size_arg);
+int32_t  orc_rank_pairs(int32_t nf, int32_t nm, float *ranks, float cutoff, int32_t permute, const uint32_t *u,
+                        int32_t nu, int32_t *pairs, int32_t *used);
 uint64_t orc_multimask(const uint32_t *breaks, int32_t n);
 void     orc_crossover(uint64_t *out, const uint64_t *parent, int32_t genome_size,
                        const uint32_t *break_pos, int32_t nbreaks);

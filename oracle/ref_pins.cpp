@@ -15,6 +15,7 @@
 
 #include "BinomialDist.h"
 #include "GeneUtils.h"
+#include "RankTable.h"
 #include "WELL512.h"
 #include "bino_tools.h"
 
@@ -103,5 +104,36 @@ int ref_bino_getn(double p, int n, double eps, const double *u, int m, int32_t *
 // bino_tools.cpp:13 and :94
 double ref_gammaln(double x) { return gammaln(x); }
 double ref_beta(double a, double b, double x) { return beta(a, b, x); }
+
+// GeneUtils::calcDist (QHG4/genes/GeneUtils.cpp:414-444)
+int ref_calc_dist(uint64_t *g1, uint64_t *g2, int size_arg) {
+    return GeneUtils::calcDist((ulong *)g1, (ulong *)g2, size_arg);
+}
+
+// RankTable::createInstance + makeAllPairings (QHG4/utils/RankTable.cpp:17-150)
+// over a row-major nf x nm table, with one WELL512 generator (thread 0's).
+// Returns the number of pairs; *nraw = raw words consumed (copied to raw).
+int ref_rank_pairs(int nf, int nm, const float *ranks, float cutoff, int permute, const uint32_t *state,
+                   int32_t *pairs, uint32_t *raw, int maxraw, int *nraw) {
+    uint32_t st[16];
+    std::memcpy(st, state, sizeof(st));
+    WELL512 w(st, 0);
+    WELL512 *aw[1] = {&w};
+    std::vector<float *> rows((size_t)nf);
+    for (int i = 0; i < nf; i++) rows[(size_t)i] = const_cast<float *>(ranks + (size_t)i * nm);
+    RankTable *rt = RankTable::createInstance((uint)nf, (uint)nm, aw, rows.data());
+    if (!rt) return -1;
+    rt->makeAllPairings(cutoff, permute != 0);
+    const couples &c = rt->getPairs();
+    for (size_t k = 0; k < c.size(); k++) {
+        pairs[2 * k] = (int32_t)c[k].first;
+        pairs[2 * k + 1] = (int32_t)c[k].second;
+    }
+    const int n = (int)c.size();
+    delete rt;
+    *nraw = (int)w.getCount();
+    if (*nraw <= maxraw) raw_words(state, 0, *nraw, raw);
+    return n;
+}
 
 }  // extern "C"

@@ -23,6 +23,8 @@ ACT_NPPCAP = 9
 ACT_GENETICS = 10
 ACT_OLDAGEDEATH = 11
 ACT_ATANDEATH = 12
+ACT_ASSORTPAIR = 13
+ACT_LAST = ACT_ASSORTPAIR
 
 # environment events (QHG4/utils_qhg/EventConsts.h) and the drowning rule
 EVENT_GEO, EVENT_CLIMATE, EVENT_VEG = 0x01, 0x02, 0x04
@@ -42,6 +44,7 @@ ACTION_NAMES = {
     ACT_GENETICS: "Genetics",
     ACT_OLDAGEDEATH: "OldAgeDeath",
     ACT_ATANDEATH: "ATanDeath",
+    ACT_ASSORTPAIR: "AssortativePairHamming",
 }
 
 ME_ADD_SIMPLE, ME_MUL_SIMPLE, ME_MAX_SIMPLE, ME_MIN_SIMPLE = 0, 2, 3, 5
@@ -52,7 +55,7 @@ MAX_EVALS = 4
 MAX_PL_POINTS = 64
 NAME_LEN = 48
 
-STREAM_PAIR, STREAM_ACT, STREAM_NEWBORN, STREAM_GENETICS = 1, 2, 6, 7
+STREAM_PAIR, STREAM_ACT, STREAM_NEWBORN, STREAM_GENETICS, STREAM_ASSORT = 1, 2, 6, 7, 8
 ACTW_WMOVE_GO, ACTW_WMOVE_DIR, ACTW_LBIRTH, ACTW_LDEATH, ACTW_RMOVE_GO, ACTW_RMOVE_DIR = 0, 1, 2, 3, 4, 5
 
 
@@ -121,6 +124,9 @@ class Params(C.Structure):
         ("atan_max_age", C.c_double),
         ("atan_range", C.c_double),
         ("atan_slope", C.c_double),
+        ("assort_cutoff", C.c_double),
+        ("assort_permute", C.c_int32),
+        ("assort_pad", C.c_int32),
     ]
 
 

@@ -168,6 +168,10 @@ struct Prog {
     double oad_max, oad_unc;                // OldAgeDeath
     double atan_max, atan_slope, atan_scale;  // ATanDeath (scale from ATanDeath::preLoop)
     int32_t gen_size, gen_cross, nbino, pad;
+    // AssortativePairHamming: mates come from k_assort_pair (x.gmate)
+    int32_t has_assort, assort_permute;
+    int32_t assort_words;     // numNucs2Blocks(numBlocks): words per strand position compared
+    float assort_cutoff;
     double bino[QHG_MAX_BINO];
 };
 
@@ -589,6 +593,10 @@ __device__ __forceinline__ void cell_act(const CtxT &x, const ProgT &p, const St
             if (p.load_age) a.age = x.cur.age[slot];
         }
         if (valid && p.has_pair) a.mate = L.mate[li];
+        if (valid && p.has_assort) {
+            const int32_t ps = x.gmate[slot];
+            a.mate = (ps >= 0) ? (int32_t)((uint32_t)ps - s) : -1;
+        }
         const uint8_t lifeS = a.life;
         int dir = -1;
         if (valid && a.life != 0) {
@@ -883,6 +891,11 @@ __device__ __forceinline__ void group_act(const CtxT &x, const ProgT &p, const S
         cv.K = G.K[k];
         cv.nn = G.nn[k];
         if (valid && p.has_pair) a.mate = G.mate[q];
+        if (valid && p.has_assort) {
+            // the partner's slot from k_assort_pair, as a group position
+            const int32_t ps = x.gmate[slot];
+            a.mate = (ps >= 0) ? (int32_t)((uint32_t)ps - S0) : -1;
+        }
         const uint8_t lifeS = a.life;
         int dir = -1;
         if (valid && a.life != 0) {
@@ -1007,6 +1020,206 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs
         wp[1] = (uint32_t)(dm >> 32);
         wp[2] = fe;
     }
+}
+
+// ---------------------------------------------------------------------------
+// k_assort_pair: AssortativePairHamming (AssortativePairHam.cpp:93-238 with
+// RankTable.cpp:119-278), one wave per occupied cell, before k_cell_act.
+//   - the cell's FERTILE females and males, each in id order;
+//   - the match of female i and male j, 1 - calcDist(i, j, numBlocks) /
+//     genomeSize as a float (GeneUtils.cpp:414-444), read from the genomes
+//     when needed rather than held in a table: a couple's value in the
+//     reference's table is that match while both are unpaired and it is at
+//     or above the cutoff, else 0;
+//   - rounds of RankTable::pairHighestRankers: the largest value, the
+//     females that reach it (permuted when asked), each female in turn
+//     picking at random among the unpaired males at that value.
+// The cell's draws come in the reference's order from QHG_STREAM_ASSORT.
+// Output: x.gmate[slot] = partner slot (-1 none) and mate_id; k_cell_act
+// reads gmate instead of pairing.  Scratch in the cell's own slot range
+// [s, s+n): gfu/gfid, gmu/gmid (gather), gfs/gms (slots in id order),
+// gfu/gmu (then their genome handles in id order), gfid (the round's female
+// order), gflag (paired: female r at s+r, male r at s+F+r).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int bitcount2(uint64_t v) {   // GeneUtils::bitcount2
+    return __popcll((v | (v >> 1)) & 0x5555555555555555ull);
+}
+
+__device__ __forceinline__ float wave_max_f32(float v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const float o = __shfl_xor(v, d, 64);
+        v = (o > v) ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t assort_draw(uint64_t seed, uint32_t step, int32_t c, uint32_t j) {
+    return draw(seed, step, QHG_STREAM_ASSORT, (uint64_t)(uint32_t)c | ((uint64_t)(j >> 16) << 32), j & 0xffffu);
+}
+
+template <typename CtxT>
+__device__ __forceinline__ float assort_match(const CtxT &x, const Prog &p, int32_t hf, int32_t hm) {
+    const uint64_t *a = x.gpool + (int64_t)hf * x.gwords;
+    const uint64_t *b = x.gpool + (int64_t)hm * x.gwords;
+    const int D = p.assort_words;
+    int c1 = 0, c2 = 0;
+    for (int k = 0; k < 2 * D; k++) c1 += bitcount2(a[k] ^ b[k]);
+    for (int k = 0; k < D; k++) c2 += bitcount2(a[k + D] ^ b[k]) + bitcount2(a[k] ^ b[k + D]);
+    const int d = (c1 < c2) ? c1 : c2;
+    return (float)(1.0 - (1.0 * d) / (double)p.gen_size);
+}
+
+template <typename CtxT>
+__device__ void assort_cell(const CtxT &x, const Prog &p, const StepArgs &sa, int32_t c, uint32_t s, int32_t n,
+                            uint32_t &fert) {
+    const int lane = threadIdx.x;
+    const uint64_t lt = lanemask_lt();
+    int32_t F = 0, M = 0;
+    for (int32_t b = 0; b < n; b += kWave) {
+        const int32_t li = b + lane;
+        const bool valid = li < n;
+        const uint32_t slot = s + (uint32_t)li;
+        uint8_t l = 0, g = 2;
+        int64_t id = 0;
+        if (valid) {
+            l = x.cur.life[slot];
+            g = x.cur.gender[slot];
+            id = x.cur.id[slot];
+            x.gmate[slot] = -1;
+            x.cur.mate_id[slot] = -3;
+        }
+        const bool isF = valid && l == QHG_LIFE_FERTILE && g == 0;
+        const bool isM = valid && l == QHG_LIFE_FERTILE && g == 1;
+        const uint64_t mF = __ballot(isF), mM = __ballot(isM);
+        if (isF) { const int32_t at = F + __popcll(mF & lt); x.gfu[s + at] = (int32_t)slot; x.gfid[s + at] = id; }
+        if (isM) { const int32_t at = M + __popcll(mM & lt); x.gmu[s + at] = (int32_t)slot; x.gmid[s + at] = id; }
+        F += __popcll(mF);
+        M += __popcll(mM);
+    }
+    fert += (lane == 0) ? (uint32_t)(F + M) : 0u;
+    if (F == 0 || M == 0) return;
+    wave_sync();
+    // id order (ids are distinct): slots to gfs / gms by rank
+    for (int32_t j = lane; j < F; j += kWave) {
+        const int64_t my = x.gfid[s + j];
+        int32_t r = 0;
+        for (int32_t k = 0; k < F; k++) r += x.gfid[s + k] < my;
+        x.gfs[s + r] = x.gfu[s + j];
+    }
+    for (int32_t j = lane; j < M; j += kWave) {
+        const int64_t my = x.gmid[s + j];
+        int32_t r = 0;
+        for (int32_t k = 0; k < M; k++) r += x.gmid[s + k] < my;
+        x.gms[s + r] = x.gmu[s + j];
+    }
+    wave_sync();
+    for (int32_t j = lane; j < F; j += kWave) x.gfu[s + j] = x.cur.ghandle[x.gfs[s + j]];
+    for (int32_t j = lane; j < M; j += kWave) x.gmu[s + j] = x.cur.ghandle[x.gms[s + j]];
+    for (int32_t j = lane; j < F + M; j += kWave) x.gflag[s + j] = 0;
+    wave_sync();
+    const int32_t *hf = x.gfu + s, *hm = x.gmu + s;
+    uint8_t *pf = x.gflag + s, *pm = x.gflag + s + F;
+    const float cut = p.assort_cutoff;
+    uint32_t dj = 0;   // the cell's draws so far
+    const int64_t FM = (int64_t)F * M;
+    for (int32_t round = 0; round < F; round++) {
+        // the largest value at or above the cutoff (0 when none)
+        float vmax = 0.f;
+        for (int64_t t = lane; t < FM; t += kWave) {
+            const int32_t i = (int32_t)(t / M), j = (int32_t)(t - (int64_t)i * M);
+            if (pf[i] || pm[j]) continue;
+            const float v = assort_match(x, p, hf[i], hm[j]);
+            if (v >= cut && v >= vmax) vmax = v;
+        }
+        vmax = wave_max_f32(vmax);
+        if (!(vmax > 0.f)) break;
+        // females reaching it, in id order (a value at vmax > 0 is at or above the cutoff)
+        uint32_t np = 0;
+        for (int32_t b = 0; b < F; b += kWave) {
+            const int32_t i = b + lane;
+            bool reach = false;
+            if (i < F && !pf[i]) {
+                for (int32_t j = 0; j < M && !reach; j++)
+                    reach = !pm[j] && assort_match(x, p, hf[i], hm[j]) >= vmax;
+            }
+            const uint64_t mk = __ballot(reach);
+            if (reach) x.gfid[s + np + (uint32_t)__popcll(mk & lt)] = i;
+            np += (uint32_t)__popcll(mk);
+        }
+        wave_sync();
+        int64_t *ord = x.gfid + s;
+        if (p.assort_permute && np > 2) {
+            // for i = np-1 down to 2: swap entry i with entry wrandi(0, i)
+            const uint32_t nperm = np - 2;
+            for (uint32_t base = 0; base < nperm; base += kWave) {
+                const uint32_t u = (base + lane < nperm) ? assort_draw(x.seed, sa.step, c, dj + base + lane) : 0u;
+                const uint32_t cnt = (nperm - base < (uint32_t)kWave) ? nperm - base : (uint32_t)kWave;
+                for (uint32_t q = 0; q < cnt; q++) {
+                    const uint32_t i = np - 1 - (base + q);
+                    const uint32_t k = wrandi(0, i, 1, readlane_u32(u, (int)q));
+                    if (lane == 0) {
+                        const int64_t tmp = ord[k];
+                        ord[k] = ord[i];
+                        ord[i] = tmp;
+                    }
+                }
+            }
+            dj += nperm;
+            wave_sync();
+        }
+        // each female in turn: a random unpaired male at vmax
+        bool paired = false;
+        for (uint32_t q = 0; q < np; q++) {
+            const int32_t i = (int32_t)ord[q];
+            uint32_t cnt = 0;
+            for (int32_t b = 0; b < M; b += kWave) {
+                const int32_t j = b + lane;
+                const bool ok = j < M && !pm[j] && assort_match(x, p, hf[i], hm[j]) >= vmax;
+                cnt += (uint32_t)__popcll(__ballot(ok));
+            }
+            if (cnt == 0) continue;
+            int32_t sel = (int32_t)wrandi(0, cnt, 1, assort_draw(x.seed, sa.step, c, dj++));
+            int32_t jm = -1;
+            for (int32_t b = 0; b < M; b += kWave) {
+                const int32_t j = b + lane;
+                const bool ok = j < M && !pm[j] && assort_match(x, p, hf[i], hm[j]) >= vmax;
+                const uint64_t mk = __ballot(ok);
+                const int pc = __popcll(mk);
+                if (sel < pc) { jm = b + select_bit(mk, sel); break; }
+                sel -= pc;
+            }
+            if (jm < 0) {   // cannot happen: the second pass sees what the first counted
+                if (lane == 0) atomicOr(&x.st->error, kErrGather);
+                return;
+            }
+            if (lane == 0) {
+                const int32_t sf = x.gfs[s + i], sm = x.gms[s + jm];
+                pf[i] = 1;
+                pm[jm] = 1;
+                x.gmate[sf] = sm;
+                x.gmate[sm] = sf;
+                x.cur.mate_id[sf] = x.cur.id[sm];
+                x.cur.mate_id[sm] = x.cur.id[sf];
+            }
+            paired = true;
+            wave_sync();
+        }
+        if (!paired) break;   // makeAllPairings stops at a round without pairs
+    }
+}
+
+__global__ void __launch_bounds__(kWave) k_assort_pair(Ctx x, Prog p, StepArgs sa) {
+    if (x.st->error) return;
+    const int64_t nocc = x.st->n_occ[sa.parity];
+    uint32_t fert = 0;
+    for (int64_t i = blockIdx.x; i < nocc; i += gridDim.x) {
+        const OccCell e = x.occ_cur[i];
+        assort_cell(x, p, sa, e.c, e.s, (int32_t)e.n, fert);
+    }
+    // the FERTILE agents considered, one atomic per wave
+    const uint64_t fe = wave_sum_u64(fert);
+    if (threadIdx.x == 0 && fe) atomicAdd((unsigned long long *)&x.st->fertile_total, (unsigned long long)fe);
 }
 
 // ---------------------------------------------------------------------------

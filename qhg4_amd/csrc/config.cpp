@@ -10,7 +10,8 @@
 // :249-257).  This file applies the same rules to fill a qhg_params:
 //   - the class fixes the actions, their ids and the evaluator wiring
 //     (QHG4/populations/tut_SexualPop.cpp:24-43, tut_EnvironCapAltPop.cpp,
-//     OoAPop.cpp:32-88, OoANavGen2bitPop.cpp:30-100);
+//     OoAPop.cpp:32-88, OoANavGen2bitPop.cpp:30-100,
+//     OoANavGen2bitAssHamPop.cpp:31-93);
 //   - an action's module must hold exactly its attribute names
 //     (Action::checkAttributes, QHG4/actions/Action.cpp:54-75 and
 //     QHG4/core/clsutils.cpp:75-103): a missing or unknown name is an error;
@@ -75,6 +76,13 @@ const std::vector<ClassSpec> &classes() {
          {{"Alt", QHG_EV_ALTITUDE, "AltCapPref", "Multi_weight_alt"},
           {"NPP", QHG_EV_CAPACITY, "NPPPref", "Multi_weight_npp"}},
          {"Navigate"}},
+        {"OoANavGen2bitAssHamPop",
+         {QHG_ACT_MULTIEVAL, QHG_ACT_WEIGHTEDMOVE, QHG_ACT_VERHULSTVARK, QHG_ACT_ASSORTPAIR, QHG_ACT_GETOLD,
+          QHG_ACT_OLDAGEDEATH, QHG_ACT_FERTILITY, QHG_ACT_NPPCAP, QHG_ACT_GENETICS},
+         "Alt+NPP", QHG_ME_ADD_SIMPLE,
+         {{"Alt", QHG_EV_ALTITUDE, "AltCapPref", "Multi_weight_alt"},
+          {"NPP", QHG_EV_CAPACITY, "NPPPref", "Multi_weight_npp"}},
+         {"Navigate"}},
     };
     return c;
 }
@@ -93,6 +101,7 @@ const char *base_name(int kind) {
     case QHG_ACT_GENETICS: return "Genetics";
     case QHG_ACT_OLDAGEDEATH: return "OldAgeDeath";
     case QHG_ACT_ATANDEATH: return "ATanDeath";
+    case QHG_ACT_ASSORTPAIR: return "AssortativePairHamming";
     default: return "";
     }
 }
@@ -139,6 +148,9 @@ const std::vector<Attr> &attrs(int kind) {
                                           {"Genetics_initial_muts", nullptr, nullptr},
                                           {"Genetics_create_new_genome", nullptr, nullptr},
                                           {"Genetics_bits_per_nuc", nullptr, nullptr}};
+    // AssortativePairHam.h:11-13 (the cutoff is read as a float)
+    static const std::vector<Attr> asp = {{"AssortativePairHamming_cutoff", &qhg_params::assort_cutoff, nullptr},
+                                          {"AssortativePairHamming_permute", nullptr, &qhg_params::assort_permute}};
     switch (kind) {
     case QHG_ACT_FERTILITY: return fert;
     case QHG_ACT_VERHULST: return verh;
@@ -149,6 +161,7 @@ const std::vector<Attr> &attrs(int kind) {
     case QHG_ACT_OLDAGEDEATH: return oad;
     case QHG_ACT_ATANDEATH: return atd;
     case QHG_ACT_GENETICS: return gen;
+    case QHG_ACT_ASSORTPAIR: return asp;
     default: return none;
     }
 }
@@ -395,7 +408,7 @@ int qhg_params_from_attribute_map(const char *pop_class, int32_t n, const char *
 
 int qhg_params_get_attribute(const qhg_params *p, const char *name, double *value) {
     if (!p || !name || !value) return qhg_internal_fail(QHG_ERR_ARG, "null argument");
-    for (int kind = QHG_ACT_GETOLD; kind <= QHG_ACT_ATANDEATH; kind++)
+    for (int kind = QHG_ACT_GETOLD; kind <= QHG_ACT_LAST; kind++)
         for (const Attr &a : attrs(kind)) {
             if (std::strcmp(a.name, name) != 0) continue;
             if (a.d) { *value = p->*(a.d); return 0; }
@@ -411,7 +424,7 @@ int qhg_params_get_attribute(const qhg_params *p, const char *name, double *valu
 // double attribute takes the value; int attributes and unknown names fail
 int qhg_params_set_attribute(qhg_params *p, const char *name, double value) {
     if (!p || !name) return qhg_internal_fail(QHG_ERR_ARG, "null argument");
-    for (int kind = QHG_ACT_GETOLD; kind <= QHG_ACT_ATANDEATH; kind++)
+    for (int kind = QHG_ACT_GETOLD; kind <= QHG_ACT_LAST; kind++)
         for (const Attr &a : attrs(kind))
             if (std::strcmp(a.name, name) == 0 && a.d) {
                 p->*(a.d) = value;

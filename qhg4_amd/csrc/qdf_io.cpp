@@ -167,7 +167,7 @@ int write_species_attrs(hid_t g, const char *species, const char *pop_class, con
         if (rc) return rc;
     }
     // the action attributes by their reference names (writeActionParamsQDF)
-    for (int kind = QHG_ACT_GETOLD; kind <= QHG_ACT_ATANDEATH; kind++) {
+    for (int kind = QHG_ACT_GETOLD; kind <= QHG_ACT_LAST; kind++) {
         if (!has_kind(P, kind)) continue;
         for (int i = 0;; i++) {
             const char *n = qhg_params_attribute_name(kind, i);

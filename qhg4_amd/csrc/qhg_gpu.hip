@@ -49,9 +49,9 @@ namespace {
             return fail(QHG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-enum KernelId { KB_ACT, KB_SCAN, KB_SCATTER, KB_GENOME, KB_CAPACITY, KB_EVAL, KB_GPOOL, KB_COUNT };
-const char *kKernelNames[KB_COUNT] = {"k_cell_act", "k_cell_scan", "k_cell_scatter", "k_genome",
-                                      "k_capacity", "k_eval", "k_gpool_compact"};
+enum KernelId { KB_ACT, KB_SCAN, KB_SCATTER, KB_GENOME, KB_CAPACITY, KB_EVAL, KB_GPOOL, KB_ASSORT, KB_COUNT };
+const char *kKernelNames[KB_COUNT] = {"k_cell_act", "k_cell_scan",     "k_cell_scatter", "k_genome",
+                                      "k_capacity", "k_eval", "k_gpool_compact", "k_assort_pair"};
 
 constexpr int kRing = kStatusRing;
 #ifndef QHG_GROUP_AGENTS
@@ -129,7 +129,7 @@ struct qhg_pop {
     DevStatus *d_st = nullptr;
 
     Prog prog{};
-    bool has_pair = false, has_me = false, has_cap = false, has_death = false;
+    bool has_pair = false, has_me = false, has_cap = false, has_death = false, has_assort = false;
     bool me_done = false;
     bool acted = false;       // k_cell_act launched for the current step
     bool initialized = false; // initialize phase done for the current step
@@ -434,7 +434,7 @@ int build_prog(qhg_pop *p) {
     bool seen_getold = false;
     for (int q = 0; q < P.num_actions; q++) {
         const int k = P.action_kind[idx[q]];
-        if (k < QHG_ACT_GETOLD || k > QHG_ACT_ATANDEATH) return fail(QHG_ERR_ARG, "unknown action kind");
+        if (k < QHG_ACT_GETOLD || k > QHG_ACT_LAST) return fail(QHG_ERR_ARG, "unknown action kind");
         g.ops[q] = k;
         g.opword |= (uint64_t)k << (4 * q);
         // actions that set the age before Fertility reads it
@@ -461,6 +461,18 @@ int build_prog(qhg_pop *p) {
         g.atan_scale = (kPi / 2 - 0.001) / q;
     }
     p->has_pair = has_kind(P, QHG_ACT_RANDOMPAIR);
+    p->has_assort = has_kind(P, QHG_ACT_ASSORTPAIR);
+    if (p->has_pair && p->has_assort) return fail(QHG_ERR_ARG, "RandomPair and AssortativePairHamming both pair");
+    if (p->has_assort) {
+        if (!has_kind(P, QHG_ACT_GENETICS)) return fail(QHG_ERR_ARG, "AssortativePairHamming needs Genetics");
+        // AssortativePairHam::preLoop / findMates (AssortativePairHam.cpp:79-90,
+        // :199): calcDist gets the block count where it reads a genome size
+        const int32_t blocks = qhg_gpu_genome_words(P.gen_genome_size);
+        g.assort_words = qhg_gpu_genome_words(blocks);
+        g.assort_cutoff = (float)P.assort_cutoff;
+        g.assort_permute = P.assort_permute != 0;
+        g.has_assort = 1;
+    }
     p->has_me = has_kind(P, QHG_ACT_MULTIEVAL);
     p->has_cap = has_kind(P, QHG_ACT_NPPCAP);
     p->has_death = has_kind(P, QHG_ACT_OLDAGEDEATH) || has_kind(P, QHG_ACT_ATANDEATH);
@@ -733,6 +745,11 @@ int phase_initialize(qhg_pop *p) {
         rc = run_evaluators(p);
         if (rc) return rc;
         p->me_done = true;
+    }
+    if (p->has_assort) {
+        // AssortativePairHam::initialize pairs in the initialize phase
+        rc = launch_bs(p, KB_ASSORT, k_assort_pair, p->act_grid, kWave, p->ctx(), p->prog, p->step_args());
+        if (rc) return rc;
     }
     p->initialized = true;
     return 0;
@@ -1798,6 +1815,7 @@ int qhg_batch_add(qhg_batch *b, qhg_pop *p) {
     if (p->device != b->device) return fail(QHG_ERR_ARG, "population and batch are on different devices");
     if ((int)b->pops.size() >= kMaxPops) return fail(QHG_ERR_CAPACITY, "batch is full (256 populations)");
     if (p->acted || p->initialized) return fail(QHG_ERR_STATE, "population is inside a step");
+    if (p->has_assort) return fail(QHG_ERR_ARG, "AssortativePairHamming populations do not run in batches");
     // hand over: everything the population enqueued so far has finished
     HIPCHK(hipStreamSynchronize(p->stream));
     poll_ring(p);

@@ -20,7 +20,8 @@ _FIELD_OF = {
     "Verhulst_theta": "verh_theta", "Verhulst_K": "verh_K", "RandomMove_prob": "rmove_prob",
     "WeightedMove_prob": "wmove_prob", "OAD_max_age": "oad_max_age", "OAD_uncertainty": "oad_uncertainty",
     "ATanDeath_max_age": "atan_max_age", "ATanDeath_range": "atan_range", "ATanDeath_slope": "atan_slope",
-    "Genetics_mutation_rate": "gen_mutation_rate",
+    "Genetics_mutation_rate": "gen_mutation_rate", "AssortativePairHamming_cutoff": "assort_cutoff",
+    "AssortativePairHamming_permute": "assort_permute",
 }
 
 
@@ -103,7 +104,8 @@ class GpuPopulation:
         check(lib().qhg_gpu_modify_attribute(self.h, name.encode(), float(value)), "qhg_gpu_modify_attribute")
         field = _FIELD_OF.get(name)
         if field:
-            setattr(self.params, field, float(value))
+            cur = getattr(self.params, field)
+            setattr(self.params, field, int(value) if isinstance(cur, int) else float(value))
 
     def set_env(self, env):
         self.env.update(env)

@@ -101,17 +101,23 @@ def ico_env(grid, seed):
     return {"altitude": alt, "base_npp": npp, "longitude": grid["lon"], "latitude": grid["lat"]}
 
 
-def ooa_params(seed, genetics=False, wmove_prob=0.2, b0=0.8, d0=0.001, theta=0.1, k_scale=50.0):
+def ooa_params(seed, genetics=False, wmove_prob=0.2, b0=0.8, d0=0.001, theta=0.1, k_scale=50.0, assort=None,
+               genome_size=256):
     """VerhulstVarK values are those of QHG4/tutorial_data/xmldat/tut_EnvironCapAlt.xml
     (C2 names none; with C1's b0=0.1, d0=0.05 the paired-female birth rate
-    cannot outpace deaths and every run dies out)."""
+    cannot outpace deaths and every run dies out).  assort=(cutoff, permute)
+    pairs with AssortativePairHamming instead of RandomPair, as
+    OoANavGen2bitAssHamPop does (needs genetics)."""
+    pair = abi.ACT_RANDOMPAIR if assort is None else abi.ACT_ASSORTPAIR
     actions = [(abi.ACT_NPPCAP, 1), (abi.ACT_GETOLD, 2), (abi.ACT_WEIGHTEDMOVE, 4),
                (abi.ACT_MULTIEVAL, 5, "MultiEvaluator[Alt+NPP]"), (abi.ACT_FERTILITY, 6),
-               (abi.ACT_RANDOMPAIR, 7), (abi.ACT_VERHULSTVARK, 8)]
+               (pair, 7), (abi.ACT_VERHULSTVARK, 8)]
     kw = {}
     if genetics:
         actions.append((abi.ACT_GENETICS, 9))
-        kw = dict(gen_genome_size=256, gen_num_crossover=1, gen_mutation_rate=1e-3)
+        kw = dict(gen_genome_size=genome_size, gen_num_crossover=1, gen_mutation_rate=1e-3)
+    if assort is not None:
+        kw.update(assort_cutoff=float(assort[0]), assort_permute=int(assort[1]))
     return make_params(
         seed, actions,
         fert_min_age=15.0, fert_max_age=45.0, fert_interbirth=3.0,
@@ -124,7 +130,7 @@ def ooa_params(seed, genetics=False, wmove_prob=0.2, b0=0.8, d0=0.001, theta=0.1
         **kw)
 
 
-def c2(seed=42, level=6, n_agents=100000, patch_cells=4000, genetics=False):
+def c2(seed=42, level=6, n_agents=100000, patch_cells=4000, genetics=False, assort=None, genome_size=256):
     """100k agents in a BFS patch of land cells.  The patch is 4,000 cells
     (~25 agents per cell, about half of K): a 200-cell patch puts 500 agents
     in cells with K <= 75, where b = b0 + (theta-b0)*N/K < -1 and the
@@ -139,11 +145,11 @@ def c2(seed=42, level=6, n_agents=100000, patch_cells=4000, genetics=False):
     rng = np.random.default_rng(seed + 7)
     cells = patch[rng.integers(0, len(patch), n_agents)]
     agents = _agents(rng, cells, n_agents, abi.QHG_LIFE_ALIVE, -1.0)
-    params = ooa_params(seed, genetics=genetics)
+    params = ooa_params(seed, genetics=genetics, assort=assort, genome_size=genome_size)
     out = {"name": "C2", "grid": grid, "env": env, "params": params, "agents": agents,
            "steps": 2000}
     if genetics:
-        out["agents"]["genome"] = initial_genomes(rng, n_agents, 256)
+        out["agents"]["genome"] = initial_genomes(rng, n_agents, genome_size)
     return out
 
 

@@ -271,3 +271,27 @@ def test_environment_events(genetics):
             assert gs[key] == os_[key], key
         assert gs["device_error"] == 0
     gpu.close()
+
+
+@pytest.mark.parametrize("cutoff,permute,genome_size,cells,k_max", [
+    (0.9, 1, 256, 150, None),     # ~20 agents per cell, many ties, permuted
+    (0.0, 0, 256, 150, None),     # every couple eligible, no permutation draws
+    (0.97, 1, 1100, 150, None),   # 35 blocks: the distance spans two words per strand position
+    (0.9, 1, 256, 12, 1500.0),    # 250-450 agents per cell: cells past the grouped path and LDS
+])
+def test_assortative_pairing(cutoff, permute, genome_size, cells, k_max):
+    """OoANavGen2bitAssHamPop's AssortativePairHamming
+    (QHG4/actions/AssortativePairHam.cpp:93-238, QHG4/utils/RankTable.cpp)
+    on k_assort_pair, bit-exact with the oracle; such a population does not
+    join a batch."""
+    from qhg4_amd._lib import QhgError
+    from qhg4_amd.population import GpuBatch
+    sc = scenarios.c2(seed=50 + cells + genome_size, level=4, n_agents=3000, patch_cells=cells, genetics=True,
+                      assort=(cutoff, permute), genome_size=genome_size)
+    if k_max is not None:
+        sc["params"].npp_k_max = k_max   # room for the crowded cells to live and breed
+    gpu, ora = _lockstep(sc, 20, 5)
+    assert gpu.stats()["births_total"] > 0
+    assert (gpu.agents()["mate_id"] >= 0).any() or gpu.stats()["num_alive"] == 0
+    with pytest.raises(QhgError, match="AssortativePairHamming"):
+        GpuBatch().add(gpu)

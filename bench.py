@@ -31,7 +31,8 @@ The same line carries two more objects:
             budget; p50/p99 completion latency, aggregate agent-steps/s over
             first arrival .. last completion, and a sample run alone.
 --workload c4 / c5 print those measurements as lines of their own (c5 with
-the full 2,000-request stream by default).
+the full 2,000-request stream by default).  --workload c2a times the C2 model
+with genetics and AssortativePairHamming pairing (k_assort_pair in kernels).
 """
 import argparse
 import json
@@ -50,7 +51,7 @@ HBM_PEAK_GBPS = 8000.0
 # what it reads for a 256 MiB device copy (FETCH x2, WRITE x1); see
 # profiles/step_kernels_c2_counters.md.  Reported as roofline.traffic only for
 # that exact workload.
-MEASURED_TRAFFIC = {("c2", 100, 1900): {"k_cell_act": 23.26e6, "k_cell_scatter": 41.87e6, "k_cell_scan": 4.21e6}}
+MEASURED_TRAFFIC = {("c2", 100, 1900): {"k_cell_act": 23.25e6, "k_cell_scatter": 41.45e6, "k_cell_scan": 4.71e6}}
 METRIC = "agent-steps/sec at 1 MI355X + %HBM peak; ensemble vs plain speedup, p50/p99"
 
 
@@ -87,6 +88,11 @@ def kernel_bytes(bm, d, grid, ops, gwords):
     out = {"k_cell_act": act, "k_cell_scan": scan, "k_cell_scatter": scat}
     if gwords:
         out["k_genome"] = births * bm["SCAT_GENOME_WORDS"] * gwords * 8
+    if abi.ACT_ASSORTPAIR in ops and gwords:
+        # words per strand position the distance compares (the block count
+        # read as a genome size, as AssortativePairHam passes it)
+        D = ((gwords // 2) + 31) // 32
+        out["k_assort_pair"] = live * bm["ASSORT_LIVE"] + d["fertile"] * (bm["ASSORT_FERTILE"] + 2 * D * 8)
     return out
 
 
@@ -99,6 +105,10 @@ def make_workload(name, seed):
     elif name == "c1":
         sc = scenarios.c1(seed=seed)
         desc = "C1: plain model on a 128x128 lattice, 10,000 initial agents"
+    elif name == "c2a":
+        sc = scenarios.c2(seed=seed, genetics=True, assort=(0.9, 1))
+        desc = "C2 with 2x256-locus 2-bit genetics and AssortativePairHamming (cutoff 0.9, permuted) " \
+               "in place of RandomPair (OoANavGen2bitAssHamPop's pairing), 100,000 initial agents"
     elif name == "c3":
         sc = scenarios.c3(seed=seed)
         desc = "C3: C2 model + 2x256-locus 2-bit genetics on a 655,362-cell icosahedral grid, 2M agents"
@@ -277,7 +287,8 @@ def delta(s1, s0, steps):
     return {"live": s1["agent_steps"] - s0["agent_steps"],
             "occupied": s1["occupied_cell_steps"] - s0["occupied_cell_steps"],
             "births": s1["births_total"] - s0["births_total"], "deaths": s1["deaths_total"] - s0["deaths_total"],
-            "moves": s1["moves_total"] - s0["moves_total"], "steps": steps}
+            "moves": s1["moves_total"] - s0["moves_total"], "fertile": s1["fertile_total"] - s0["fertile_total"],
+            "steps": steps}
 
 
 def gpu_run(sc, warmup, steps, timing):
@@ -329,7 +340,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1900)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c2a", "c3", "c4", "c5"])
     ap.add_argument("--requests", type=int, default=2000, help="C5: requests in the stream")
     ap.add_argument("--budget-gb", type=float, default=250.0, help="C5: HBM budget for resident state")
     ap.add_argument("--chunk", type=int, default=16, help="C5: steps per batch chunk")

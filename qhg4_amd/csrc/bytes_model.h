@@ -49,6 +49,13 @@
 #define QHG_B_SCAT_BIRTH        54  /* newborn record w 46 + birth offset r 4 + father r 4   */
 #define QHG_B_SCAT_BIRTH_GEN    12  /* handle w 4 + birth record w 8                         */
 
+/* k_assort_pair (AssortativePairHamming), per live agent and per FERTILE
+   agent of a cell holding both sexes; each couple's genome words are read
+   through the caches, counted once per agent (a lower bound) */
+#define QHG_B_ASSORT_LIVE       22  /* life r 1 + gender r 1 + id r 8 + partner slot w 4
+                                       + mate id w 8                                         */
+#define QHG_B_ASSORT_FERTILE     4  /* genome handle r 4, + 2 x D x 8 genome words r          */
+
 /* k_genome (Genetics::makeOffspring), per birth */
 #define QHG_B_SCAT_GENOME_WORDS  3  /* x gwords x 8: two parent genomes r + child w          */
 

@@ -1209,13 +1209,158 @@ __device__ void assort_cell(const CtxT &x, const Prog &p, const StepArgs &sa, in
     }
 }
 
+// A cell of at most one wave of agents, everything in LDS: the sorted
+// lists, the compared genome words and the whole match table (F + M <= 64,
+// so F * M <= 1024), read from HBM once.  Same rounds and draws as
+// assort_cell.
+struct AssortLds {
+    int64_t fid[kWave], mid[kWave];       // ids, gather order
+    int64_t fsid[kWave], msid[kWave];     // ids, id order
+    int32_t fslot[kWave], mslot[kWave];   // slots, id order
+    int32_t ord[kWave];                   // the round's female order
+    uint64_t fw[kWave][4], mw[kWave][4];  // compared genome words (2 * D <= 4), id order
+    float T[kWave * kWave / 4];           // match of female i and male j at i * M + j
+    uint8_t pf[kWave], pm[kWave];         // paired
+};
+
+__device__ __forceinline__ float assort_match_w(const uint64_t *a, const uint64_t *b, int D, int gen_size) {
+    int c1 = 0, c2 = 0;
+    for (int k = 0; k < 2 * D; k++) c1 += bitcount2(a[k] ^ b[k]);
+    for (int k = 0; k < D; k++) c2 += bitcount2(a[k + D] ^ b[k]) + bitcount2(a[k] ^ b[k + D]);
+    const int d = (c1 < c2) ? c1 : c2;
+    return (float)(1.0 - (1.0 * d) / (double)gen_size);
+}
+
+template <typename CtxT>
+__device__ void assort_cell_lds(const CtxT &x, const Prog &p, const StepArgs &sa, int32_t c, uint32_t s, int32_t n,
+                                AssortLds &L, uint32_t &fert) {
+    const int lane = threadIdx.x;
+    const uint64_t lt = lanemask_lt();
+    const bool valid = lane < n;
+    const uint32_t slot = s + (uint32_t)lane;
+    uint8_t l = 0, g = 2;
+    int64_t id = 0;
+    int32_t h = 0;
+    if (valid) {
+        l = x.cur.life[slot];
+        g = x.cur.gender[slot];
+        id = x.cur.id[slot];
+        h = x.cur.ghandle[slot];
+        x.gmate[slot] = -1;
+        x.cur.mate_id[slot] = -3;
+    }
+    const bool isF = valid && l == QHG_LIFE_FERTILE && g == 0;
+    const bool isM = valid && l == QHG_LIFE_FERTILE && g == 1;
+    const uint64_t mF = __ballot(isF), mM = __ballot(isM);
+    const int F = __popcll(mF), M = __popcll(mM);
+    fert += (lane == 0) ? (uint32_t)(F + M) : 0u;
+    if (F == 0 || M == 0) return;
+    if (isF) L.fid[__popcll(mF & lt)] = id;
+    if (isM) L.mid[__popcll(mM & lt)] = id;
+    wave_sync();
+    // id order (ids are distinct), then the compared genome words
+    int r = 0;
+    if (isF) for (int k = 0; k < F; k++) r += L.fid[k] < id;
+    if (isM) for (int k = 0; k < M; k++) r += L.mid[k] < id;
+    const int D = p.assort_words;
+    const uint64_t *gw = x.gpool + (int64_t)h * x.gwords;
+    uint64_t w[4] = {0, 0, 0, 0};   // constant indices only: stays in registers
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if ((isF || isM) && k < 2 * D) w[k] = gw[k];
+    if (isF) {
+        L.fslot[r] = (int32_t)slot;
+        L.fsid[r] = id;
+#pragma unroll
+        for (int k = 0; k < 4; k++) L.fw[r][k] = w[k];
+        L.pf[r] = 0;
+    }
+    if (isM) {
+        L.mslot[r] = (int32_t)slot;
+        L.msid[r] = id;
+#pragma unroll
+        for (int k = 0; k < 4; k++) L.mw[r][k] = w[k];
+        L.pm[r] = 0;
+    }
+    wave_sync();
+    for (int t = lane; t < F * M; t += kWave) {
+        const int i = t / M, j = t - i * M;
+        L.T[t] = assort_match_w(L.fw[i], L.mw[j], D, p.gen_size);
+    }
+    wave_sync();
+    const float cut = p.assort_cutoff;
+    uint32_t dj = 0;   // the cell's draws so far
+    for (int round = 0; round < F; round++) {
+        // the largest value at or above the cutoff (0 when none)
+        float vmax = 0.f;
+        for (int t = lane; t < F * M; t += kWave) {
+            const int i = t / M, j = t - i * M;
+            if (L.pf[i] || L.pm[j]) continue;
+            const float v = L.T[t];
+            if (v >= cut && v >= vmax) vmax = v;
+        }
+        vmax = wave_max_f32(vmax);
+        if (!(vmax > 0.f)) break;
+        // females reaching it, in id order
+        bool reach = false;
+        if (lane < F && !L.pf[lane])
+            for (int j = 0; j < M && !reach; j++) reach = !L.pm[j] && L.T[lane * M + j] >= vmax;
+        const uint64_t mk = __ballot(reach);
+        const uint32_t np = (uint32_t)__popcll(mk);
+        if (reach) L.ord[__popcll(mk & lt)] = lane;
+        wave_sync();
+        if (p.assort_permute && np > 2) {
+            const uint32_t nperm = np - 2;   // np <= 63: one batch of draws
+            const uint32_t u = ((uint32_t)lane < nperm) ? assort_draw(x.seed, sa.step, c, dj + (uint32_t)lane) : 0u;
+            for (uint32_t q = 0; q < nperm; q++) {
+                const uint32_t i = np - 1 - q;
+                const uint32_t k = wrandi(0, i, 1, readlane_u32(u, (int)q));
+                if (lane == 0) {
+                    const int32_t tmp = L.ord[k];
+                    L.ord[k] = L.ord[i];
+                    L.ord[i] = tmp;
+                }
+            }
+            dj += nperm;
+            wave_sync();
+        }
+        // each female in turn: a random unpaired male at vmax
+        bool paired = false;
+        for (uint32_t q = 0; q < np; q++) {
+            const int i = L.ord[q];
+            const bool ok = lane < M && !L.pm[lane] && L.T[i * M + lane] >= vmax;
+            const uint64_t cm = __ballot(ok);
+            const uint32_t cnt = (uint32_t)__popcll(cm);
+            if (cnt == 0) continue;
+            const int jm = select_bit(cm, (int)wrandi(0, cnt, 1, assort_draw(x.seed, sa.step, c, dj++)));
+            if (lane == 0) {
+                const int32_t sf = L.fslot[i], sm = L.mslot[jm];
+                L.pf[i] = 1;
+                L.pm[jm] = 1;
+                x.gmate[sf] = sm;
+                x.gmate[sm] = sf;
+                x.cur.mate_id[sf] = L.msid[jm];
+                x.cur.mate_id[sm] = L.fsid[i];
+            }
+            paired = true;
+            wave_sync();
+        }
+        if (!paired) break;   // makeAllPairings stops at a round without pairs
+    }
+    wave_sync();
+}
+
 __global__ void __launch_bounds__(kWave) k_assort_pair(Ctx x, Prog p, StepArgs sa) {
+    __shared__ AssortLds lds;
     if (x.st->error) return;
     const int64_t nocc = x.st->n_occ[sa.parity];
     uint32_t fert = 0;
     for (int64_t i = blockIdx.x; i < nocc; i += gridDim.x) {
         const OccCell e = x.occ_cur[i];
-        assort_cell(x, p, sa, e.c, e.s, (int32_t)e.n, fert);
+        if (e.n <= (uint32_t)kWave)
+            assort_cell_lds(x, p, sa, e.c, e.s, (int32_t)e.n, lds, fert);
+        else
+            assort_cell(x, p, sa, e.c, e.s, (int32_t)e.n, fert);
     }
     // the FERTILE agents considered, one atomic per wave
     const uint64_t fe = wave_sum_u64(fert);

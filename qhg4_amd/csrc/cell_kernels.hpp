@@ -596,6 +596,7 @@ __device__ __forceinline__ void cell_act(const CtxT &x, const ProgT &p, const St
         if (valid && p.has_assort) {
             const int32_t ps = x.gmate[slot];
             a.mate = (ps >= 0) ? (int32_t)((uint32_t)ps - s) : -1;
+            my_fert += (a.life == QHG_LIFE_FERTILE && a.g <= 1) ? 1u : 0u;
         }
         const uint8_t lifeS = a.life;
         int dir = -1;
@@ -892,9 +893,11 @@ __device__ __forceinline__ void group_act(const CtxT &x, const ProgT &p, const S
         cv.nn = G.nn[k];
         if (valid && p.has_pair) a.mate = G.mate[q];
         if (valid && p.has_assort) {
-            // the partner's slot from k_assort_pair, as a group position
+            // the partner's slot from k_assort_pair, as a group position;
+            // the FERTILE agents it considered are counted here
             const int32_t ps = x.gmate[slot];
             a.mate = (ps >= 0) ? (int32_t)((uint32_t)ps - S0) : -1;
+            my_fert += (a.life == QHG_LIFE_FERTILE && a.g <= 1) ? 1u : 0u;
         }
         const uint8_t lifeS = a.life;
         int dir = -1;
@@ -1071,8 +1074,7 @@ __device__ __forceinline__ float assort_match(const CtxT &x, const Prog &p, int3
 }
 
 template <typename CtxT>
-__device__ void assort_cell(const CtxT &x, const Prog &p, const StepArgs &sa, int32_t c, uint32_t s, int32_t n,
-                            uint32_t &fert) {
+__device__ void assort_cell(const CtxT &x, const Prog &p, const StepArgs &sa, int32_t c, uint32_t s, int32_t n) {
     const int lane = threadIdx.x;
     const uint64_t lt = lanemask_lt();
     int32_t F = 0, M = 0;
@@ -1097,7 +1099,6 @@ __device__ void assort_cell(const CtxT &x, const Prog &p, const StepArgs &sa, in
         F += __popcll(mF);
         M += __popcll(mM);
     }
-    fert += (lane == 0) ? (uint32_t)(F + M) : 0u;
     if (F == 0 || M == 0) return;
     wave_sync();
     // id order (ids are distinct): slots to gfs / gms by rank
@@ -1210,30 +1211,32 @@ __device__ void assort_cell(const CtxT &x, const Prog &p, const StepArgs &sa, in
 }
 
 // A cell of at most one wave of agents, everything in LDS: the sorted
-// lists, the compared genome words and the whole match table (F + M <= 64,
-// so F * M <= 1024), read from HBM once.  Same rounds and draws as
-// assort_cell.
+// lists, the compared genome words and the whole distance table (F + M <=
+// 64, so F * M <= 1024), read from HBM once.  A couple's match is the
+// block's table value of its distance (d <= 2 * D * 32 = 128), the same
+// float as 1 - d / genomeSize.  Same rounds and draws as assort_cell.
+constexpr int kAssortMaxDist = 128;
 struct AssortLds {
     int64_t fid[kWave], mid[kWave];       // ids, gather order
     int64_t fsid[kWave], msid[kWave];     // ids, id order
     int32_t fslot[kWave], mslot[kWave];   // slots, id order
     int32_t ord[kWave];                   // the round's female order
     uint64_t fw[kWave][4], mw[kWave][4];  // compared genome words (2 * D <= 4), id order
-    float T[kWave * kWave / 4];           // match of female i and male j at i * M + j
+    float val[kAssortMaxDist + 1];        // match of each distance
+    uint8_t T[kWave * kWave / 4];         // distance of female i and male j at i * M + j
     uint8_t pf[kWave], pm[kWave];         // paired
 };
 
-__device__ __forceinline__ float assort_match_w(const uint64_t *a, const uint64_t *b, int D, int gen_size) {
+__device__ __forceinline__ int assort_dist_w(const uint64_t *a, const uint64_t *b, int D) {
     int c1 = 0, c2 = 0;
     for (int k = 0; k < 2 * D; k++) c1 += bitcount2(a[k] ^ b[k]);
     for (int k = 0; k < D; k++) c2 += bitcount2(a[k + D] ^ b[k]) + bitcount2(a[k] ^ b[k + D]);
-    const int d = (c1 < c2) ? c1 : c2;
-    return (float)(1.0 - (1.0 * d) / (double)gen_size);
+    return (c1 < c2) ? c1 : c2;
 }
 
 template <typename CtxT>
 __device__ void assort_cell_lds(const CtxT &x, const Prog &p, const StepArgs &sa, int32_t c, uint32_t s, int32_t n,
-                                AssortLds &L, uint32_t &fert) {
+                                AssortLds &L) {
     const int lane = threadIdx.x;
     const uint64_t lt = lanemask_lt();
     const bool valid = lane < n;
@@ -1253,7 +1256,6 @@ __device__ void assort_cell_lds(const CtxT &x, const Prog &p, const StepArgs &sa
     const bool isM = valid && l == QHG_LIFE_FERTILE && g == 1;
     const uint64_t mF = __ballot(isF), mM = __ballot(isM);
     const int F = __popcll(mF), M = __popcll(mM);
-    fert += (lane == 0) ? (uint32_t)(F + M) : 0u;
     if (F == 0 || M == 0) return;
     if (isF) L.fid[__popcll(mF & lt)] = id;
     if (isM) L.mid[__popcll(mM & lt)] = id;
@@ -1285,7 +1287,7 @@ __device__ void assort_cell_lds(const CtxT &x, const Prog &p, const StepArgs &sa
     wave_sync();
     for (int t = lane; t < F * M; t += kWave) {
         const int i = t / M, j = t - i * M;
-        L.T[t] = assort_match_w(L.fw[i], L.mw[j], D, p.gen_size);
+        L.T[t] = (uint8_t)assort_dist_w(L.fw[i], L.mw[j], D);
     }
     wave_sync();
     const float cut = p.assort_cutoff;
@@ -1296,7 +1298,7 @@ __device__ void assort_cell_lds(const CtxT &x, const Prog &p, const StepArgs &sa
         for (int t = lane; t < F * M; t += kWave) {
             const int i = t / M, j = t - i * M;
             if (L.pf[i] || L.pm[j]) continue;
-            const float v = L.T[t];
+            const float v = L.val[L.T[t]];
             if (v >= cut && v >= vmax) vmax = v;
         }
         vmax = wave_max_f32(vmax);
@@ -1304,7 +1306,7 @@ __device__ void assort_cell_lds(const CtxT &x, const Prog &p, const StepArgs &sa
         // females reaching it, in id order
         bool reach = false;
         if (lane < F && !L.pf[lane])
-            for (int j = 0; j < M && !reach; j++) reach = !L.pm[j] && L.T[lane * M + j] >= vmax;
+            for (int j = 0; j < M && !reach; j++) reach = !L.pm[j] && L.val[L.T[lane * M + j]] >= vmax;
         const uint64_t mk = __ballot(reach);
         const uint32_t np = (uint32_t)__popcll(mk);
         if (reach) L.ord[__popcll(mk & lt)] = lane;
@@ -1328,7 +1330,7 @@ __device__ void assort_cell_lds(const CtxT &x, const Prog &p, const StepArgs &sa
         bool paired = false;
         for (uint32_t q = 0; q < np; q++) {
             const int i = L.ord[q];
-            const bool ok = lane < M && !L.pm[lane] && L.T[i * M + lane] >= vmax;
+            const bool ok = lane < M && !L.pm[lane] && L.val[L.T[i * M + lane]] >= vmax;
             const uint64_t cm = __ballot(ok);
             const uint32_t cnt = (uint32_t)__popcll(cm);
             if (cnt == 0) continue;
@@ -1354,17 +1356,16 @@ __global__ void __launch_bounds__(kWave) k_assort_pair(Ctx x, Prog p, StepArgs s
     __shared__ AssortLds lds;
     if (x.st->error) return;
     const int64_t nocc = x.st->n_occ[sa.parity];
-    uint32_t fert = 0;
+    for (int k = threadIdx.x; k <= kAssortMaxDist; k += kWave)
+        lds.val[k] = (float)(1.0 - (1.0 * k) / (double)p.gen_size);
+    wave_sync();
     for (int64_t i = blockIdx.x; i < nocc; i += gridDim.x) {
         const OccCell e = x.occ_cur[i];
         if (e.n <= (uint32_t)kWave)
-            assort_cell_lds(x, p, sa, e.c, e.s, (int32_t)e.n, lds, fert);
+            assort_cell_lds(x, p, sa, e.c, e.s, (int32_t)e.n, lds);
         else
-            assort_cell(x, p, sa, e.c, e.s, (int32_t)e.n, fert);
+            assort_cell(x, p, sa, e.c, e.s, (int32_t)e.n);
     }
-    // the FERTILE agents considered, one atomic per wave
-    const uint64_t fe = wave_sum_u64(fert);
-    if (threadIdx.x == 0 && fe) atomicAdd((unsigned long long *)&x.st->fertile_total, (unsigned long long)fe);
 }
 
 // ---------------------------------------------------------------------------

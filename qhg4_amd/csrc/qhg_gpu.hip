@@ -472,6 +472,7 @@ int build_prog(qhg_pop *p) {
         g.assort_cutoff = (float)P.assort_cutoff;
         g.assort_permute = P.assort_permute != 0;
         g.has_assort = 1;
+        g.need_gender = 1;   // k_cell_act counts the FERTILE agents by sex
     }
     p->has_me = has_kind(P, QHG_ACT_MULTIEVAL);
     p->has_cap = has_kind(P, QHG_ACT_NPPCAP);

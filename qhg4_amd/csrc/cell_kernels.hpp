@@ -475,7 +475,7 @@ __device__ __forceinline__ void id_order(const int64_t *ids, int32_t n, int32_t 
     }
 }
 
-template <bool kDeaths, typename CtxT, typename ProgT>
+template <bool kDeaths, bool kAssort = false, typename CtxT, typename ProgT>
 __device__ __forceinline__ void cell_act(const CtxT &x, const ProgT &p, const StepArgs &sa, int32_t c, uint32_t s,
                                          int32_t n, const PairLists &L, uint32_t &my_deaths, uint32_t &my_moves,
                                          uint32_t &my_fert) {
@@ -593,7 +593,7 @@ __device__ __forceinline__ void cell_act(const CtxT &x, const ProgT &p, const St
             if (p.load_age) a.age = x.cur.age[slot];
         }
         if (valid && p.has_pair) a.mate = L.mate[li];
-        if (valid && p.has_assort) {
+        if (kAssort && valid) {
             const int32_t ps = x.gmate[slot];
             a.mate = (ps >= 0) ? (int32_t)((uint32_t)ps - s) : -1;
             my_fert += (a.life == QHG_LIFE_FERTILE && a.g <= 1) ? 1u : 0u;
@@ -690,7 +690,7 @@ union ActLds {
 // The cells of a group are consecutive occupied cells, so their segments
 // form one slot range [S0, S0+T).  Lane k < ng holds cell k's entry e and
 // eb = e.s - S0.  Each cell's lists live at its own positions [eb, eb+n).
-template <bool kDeaths, typename CtxT, typename ProgT>
+template <bool kDeaths, bool kAssort = false, typename CtxT, typename ProgT>
 __device__ __forceinline__ void group_act(const CtxT &x, const ProgT &p, const StepArgs &sa, int ng, const OccCell &e,
                                           uint32_t eb, uint32_t S0, uint32_t T, GroupActLds &G,
                                           uint32_t &my_deaths, uint32_t &my_moves, uint32_t &my_fert) {
@@ -892,7 +892,7 @@ __device__ __forceinline__ void group_act(const CtxT &x, const ProgT &p, const S
         cv.K = G.K[k];
         cv.nn = G.nn[k];
         if (valid && p.has_pair) a.mate = G.mate[q];
-        if (valid && p.has_assort) {
+        if (kAssort && valid) {
             // the partner's slot from k_assort_pair, as a group position;
             // the FERTILE agents it considered are counted here
             const int32_t ps = x.gmate[slot];
@@ -954,7 +954,7 @@ __device__ __forceinline__ void group_act(const CtxT &x, const ProgT &p, const S
     wave_sync();
 }
 
-template <bool kDeaths>
+template <bool kDeaths, bool kAssort>
 __global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs sa) {
     __shared__ ActLds lds;
     const int lane = threadIdx.x;
@@ -997,7 +997,7 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs
         const uint32_t T = readlane_u32(e.s, ng - 1) + readlane_u32(e.n, ng - 1) - S0;
         const uint32_t nmax = wave_max_u32(lane < ng ? e.n : 0u);
         if (T <= (uint32_t)kGroupSlots && nmax <= (uint32_t)kGroupCellMax) {
-            group_act<kDeaths>(x, p, sa, ng, e, e.s - S0, S0, T, lds.g, my_deaths, my_moves, my_fert);
+            group_act<kDeaths, kAssort>(x, p, sa, ng, e, e.s - S0, S0, T, lds.g, my_deaths, my_moves, my_fert);
             continue;
         }
         for (int k = 0; k < ng; k++) {
@@ -1012,7 +1012,7 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs
                 L = PairLists{lds.c.fu, lds.c.mu, lds.c.fid, lds.c.mid, lds.c.fs, lds.c.ms, lds.c.mate,
                               lds.c.flag};
             }
-            cell_act<kDeaths>(x, p, sa, c, s, n, L, my_deaths, my_moves, my_fert);
+            cell_act<kDeaths, kAssort>(x, p, sa, c, s, n, L, my_deaths, my_moves, my_fert);
         }
     }
     const uint64_t dm = wave_sum_u64(((uint64_t)my_moves << 32) | my_deaths);

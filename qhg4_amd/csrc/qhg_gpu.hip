@@ -761,9 +761,10 @@ int phase_act(qhg_pop *p) {
         int rc = phase_initialize(p);
         if (rc) return rc;
     }
-    int rc = p->has_death
-                 ? launch_bs(p, KB_ACT, k_cell_act<true>, p->act_grid, kCellBlock, p->ctx(), p->prog, p->step_args())
-                 : launch_bs(p, KB_ACT, k_cell_act<false>, p->act_grid, kCellBlock, p->ctx(), p->prog, p->step_args());
+    // the age-death branches and the assortative mates compile in only where used
+    auto kern = p->has_death ? (p->has_assort ? k_cell_act<true, true> : k_cell_act<true, false>)
+                             : (p->has_assort ? k_cell_act<false, true> : k_cell_act<false, false>);
+    int rc = launch_bs(p, KB_ACT, kern, p->act_grid, kCellBlock, p->ctx(), p->prog, p->step_args());
     if (rc) return rc;
     p->acted = true;
     return 0;

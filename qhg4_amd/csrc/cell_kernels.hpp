@@ -1223,6 +1223,7 @@ struct AssortLds {
     int32_t ord[kWave];                   // the round's female order
     uint64_t fw[kWave][4], mw[kWave][4];  // compared genome words (2 * D <= 4), id order
     float val[kAssortMaxDist + 1];        // match of each distance
+    uint32_t rmax[kWave];                 // a round's largest value per female (float bits)
     uint8_t T[kWave * kWave / 4];         // distance of female i and male j at i * M + j
     uint8_t pf[kWave], pm[kWave];         // paired
 };
@@ -1291,50 +1292,56 @@ __device__ void assort_cell_lds(const CtxT &x, const Prog &p, const StepArgs &sa
     }
     wave_sync();
     const float cut = p.assort_cutoff;
-    uint32_t dj = 0;   // the cell's draws so far
+    // the cell's draws, computed 64 at a time and taken in order
+    uint32_t dj = 0, ubase = 0;
+    uint32_t U = assort_draw(x.seed, sa.step, c, (uint32_t)lane);
+    auto next_u = [&]() -> uint32_t {
+        if (dj - ubase >= (uint32_t)kWave) {
+            ubase = dj;
+            U = assort_draw(x.seed, sa.step, c, ubase + (uint32_t)lane);
+        }
+        return readlane_u32(U, (int)(dj++ - ubase));
+    };
     for (int round = 0; round < F; round++) {
-        // the largest value at or above the cutoff (0 when none)
-        float vmax = 0.f;
+        // each unpaired female's largest value at or above the cutoff among
+        // the unpaired males (0 when none; positive floats order as their bits)
+        if (lane < F) L.rmax[lane] = 0u;
+        wave_sync();
         for (int t = lane; t < F * M; t += kWave) {
             const int i = t / M, j = t - i * M;
             if (L.pf[i] || L.pm[j]) continue;
             const float v = L.val[L.T[t]];
-            if (v >= cut && v >= vmax) vmax = v;
+            if (v >= cut && v > 0.f) atomicMax(&L.rmax[i], __float_as_uint(v));
         }
-        vmax = wave_max_f32(vmax);
+        wave_sync();
+        const float rm = (lane < F) ? __uint_as_float(L.rmax[lane]) : 0.f;
+        const float vmax = wave_max_f32(rm);
         if (!(vmax > 0.f)) break;
-        // females reaching it, in id order
-        bool reach = false;
-        if (lane < F && !L.pf[lane])
-            for (int j = 0; j < M && !reach; j++) reach = !L.pm[j] && L.val[L.T[lane * M + j]] >= vmax;
+        // females reaching it, in id order; lane q holds the round's entry q
+        const bool reach = lane < F && rm >= vmax;
         const uint64_t mk = __ballot(reach);
         const uint32_t np = (uint32_t)__popcll(mk);
         if (reach) L.ord[__popcll(mk & lt)] = lane;
         wave_sync();
+        int32_t ord = ((uint32_t)lane < np) ? L.ord[lane] : 0;
         if (p.assort_permute && np > 2) {
-            const uint32_t nperm = np - 2;   // np <= 63: one batch of draws
-            const uint32_t u = ((uint32_t)lane < nperm) ? assort_draw(x.seed, sa.step, c, dj + (uint32_t)lane) : 0u;
-            for (uint32_t q = 0; q < nperm; q++) {
-                const uint32_t i = np - 1 - q;
-                const uint32_t k = wrandi(0, i, 1, readlane_u32(u, (int)q));
-                if (lane == 0) {
-                    const int32_t tmp = L.ord[k];
-                    L.ord[k] = L.ord[i];
-                    L.ord[i] = tmp;
-                }
+            // for i = np-1 down to 2: swap entry i with entry wrandi(0, i)
+            for (uint32_t i = np - 1; i > 1; i--) {
+                const uint32_t k = wrandi(0, i, 1, next_u());
+                const int32_t vk = (int32_t)readlane_u32((uint32_t)ord, (int)k);
+                const int32_t vi = (int32_t)readlane_u32((uint32_t)ord, (int)i);
+                ord = (lane == (int)k) ? vi : ((lane == (int)i) ? vk : ord);
             }
-            dj += nperm;
-            wave_sync();
         }
         // each female in turn: a random unpaired male at vmax
         bool paired = false;
         for (uint32_t q = 0; q < np; q++) {
-            const int i = L.ord[q];
+            const int i = (int)readlane_u32((uint32_t)ord, (int)q);
             const bool ok = lane < M && !L.pm[lane] && L.val[L.T[i * M + lane]] >= vmax;
             const uint64_t cm = __ballot(ok);
             const uint32_t cnt = (uint32_t)__popcll(cm);
             if (cnt == 0) continue;
-            const int jm = select_bit(cm, (int)wrandi(0, cnt, 1, assort_draw(x.seed, sa.step, c, dj++)));
+            const int jm = select_bit(cm, (int)wrandi(0, cnt, 1, next_u()));
             if (lane == 0) {
                 const int32_t sf = L.fslot[i], sm = L.mslot[jm];
                 L.pf[i] = 1;

@@ -222,14 +222,16 @@ def test_modify_attributes_mid_run():
     gpu.close()
 
 
-@pytest.mark.parametrize("genetics", [False, True])
+@pytest.mark.parametrize("genetics", [False, True, "assort"])
 def test_environment_events(genetics):
     """updateEvent/flushEvents between steps: a climate event (NPP falls)
     recomputes capacities and weights; two geography events drown the agents
     of cells under water or ice (the second after the first repacked the
-    buffers); stepping goes on bit-exact with the oracle."""
+    buffers); stepping goes on bit-exact with the oracle.  "assort": genetics
+    with AssortativePairHamming pairing."""
     print("case: environment events, ico level 4, genetics=%s" % genetics, flush=True)
-    sc = scenarios.c2(seed=17, level=4, n_agents=5000, patch_cells=250, genetics=genetics)
+    sc = scenarios.c2(seed=17, level=4, n_agents=5000, patch_cells=250, genetics=bool(genetics),
+                      assort=(0.9, 1) if genetics == "assort" else None)
     gpu = GpuPopulation.from_scenario(sc)
     ora = Oracle(sc)
     gpu.preLoop()

@@ -19,11 +19,15 @@ def run(sc, steps):
     return a, s
 
 
-@pytest.mark.parametrize("case,k,m", [("c2", 15, 15), ("c2", 8, 9), ("c3", 7, 10)])
+@pytest.mark.parametrize("case,k,m", [("c2", 15, 15), ("c2", 8, 9), ("c3", 7, 10), ("assort", 9, 8)])
 def test_dump_and_resume(tmp_path, case, k, m):
+    """case "assort": OoANavGen2bitAssHamPop's AssortativePairHamming pairing,
+    whose cutoff and permute flag travel in the file."""
     print("case: %s dump at %d, resume for %d" % (case, k, m), flush=True)
     if case == "c2":
         sc = scenarios.c2(seed=51, level=5, n_agents=20000, patch_cells=800)
+    elif case == "assort":
+        sc = scenarios.c2(seed=53, level=4, n_agents=6000, patch_cells=300, genetics=True, assort=(0.92, 1))
     else:
         sc = scenarios.c2(seed=52, level=4, n_agents=6000, patch_cells=300, genetics=True)
     ref_a, ref_s = run(sc, k + m)
@@ -31,7 +35,7 @@ def test_dump_and_resume(tmp_path, case, k, m):
     a = GpuPopulation.from_scenario(sc)
     a.preLoop()
     a.run(k)
-    qdf.write_population(a, f, pop_class="OoAPop")
+    qdf.write_population(a, f, pop_class="OoANavGen2bitAssHamPop" if case == "assort" else "OoAPop")
     a.close()
     info = qdf.read_info(f)
     assert info["step"] == k and info["time"] == float(k)

@@ -137,6 +137,8 @@ struct Ctx {
     uint32_t *tick;           // k_cell_scan's tile tickets (zeroed by k_cell_act)
     int32_t nact_blocks;      // grid of k_cell_act
     uint32_t *wave_part;      // kParts per k_cell_act block: deaths, moves, fertile
+    uint64_t *scan_part;      // 3 per k_cell_scan wave: the sums of its slice of wave_part
+    int32_t scan_waves;       // waves of k_cell_scan's grid
     uint64_t *gpool;          // genome pool, gwords u64 per handle
     int32_t gwords;
     int64_t gcap;
@@ -1587,8 +1589,9 @@ __global__ void __launch_bounds__(kScanBlock) k_cell_scan(Ctx x, StepArgs sa) {
     for (uint32_t tile = next_tile(x.tick, lds); tile < nt; tile = one_round ? nt : next_tile(x.tick, lds))
         scan_tile(x, sa, tile, lds);
     {
-        // off the critical path: this block's slice of the k_cell_act
-        // partials (deaths, moves, FERTILE counts), one atomic each
+        // off the critical path: this wave's slice of the k_cell_act
+        // partials (deaths, moves, FERTILE counts), stored for k_cell_scatter's
+        // bookkeeping block (plain stores: no contended atomics)
         uint64_t D = 0, Mv = 0, Fe = 0;
         for (int32_t j = blockIdx.x * kScanBlock + threadIdx.x; j < x.nact_blocks; j += gridDim.x * kScanBlock) {
             D += x.wave_part[kParts * j + 0];
@@ -1598,12 +1601,11 @@ __global__ void __launch_bounds__(kScanBlock) k_cell_scan(Ctx x, StepArgs sa) {
         D = wave_sum_u64(D);
         Mv = wave_sum_u64(Mv);
         Fe = wave_sum_u64(Fe);
-        if (lane == 0 && (D | Mv | Fe) != 0) {
-            atomicAdd((unsigned long long *)&x.st->deaths_last, (unsigned long long)D);
-            atomicAdd((unsigned long long *)&x.st->deaths_total, (unsigned long long)D);
-            atomicAdd((unsigned long long *)&x.st->moves_last, (unsigned long long)Mv);
-            atomicAdd((unsigned long long *)&x.st->moves_total, (unsigned long long)Mv);
-            atomicAdd((unsigned long long *)&x.st->fertile_total, (unsigned long long)Fe);
+        if (lane == 0) {
+            uint64_t *sp = x.scan_part + 3 * ((int64_t)blockIdx.x * (kScanBlock / kWave) + (threadIdx.x / kWave));
+            sp[0] = D;
+            sp[1] = Mv;
+            sp[2] = Fe;
         }
     }
 }
@@ -1900,6 +1902,25 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_scatter(Ctx x, StepArgs sa)
     }
     if (err) return;
     if (blockIdx.x == 0) {
+        // the step's deaths, moves and FERTILE counts from k_cell_scan's
+        // per-wave sums, then the host's status snapshot
+        uint64_t D = 0, Mv = 0, Fe = 0;
+        for (int32_t j = lane; j < x.scan_waves; j += kWave) {
+            D += x.scan_part[3 * j + 0];
+            Mv += x.scan_part[3 * j + 1];
+            Fe += x.scan_part[3 * j + 2];
+        }
+        D = wave_sum_u64(D);
+        Mv = wave_sum_u64(Mv);
+        Fe = wave_sum_u64(Fe);
+        if (lane == 0) {
+            DevStatus *st = x.st;
+            st->deaths_last = (int64_t)D;
+            st->deaths_total += (int64_t)D;
+            st->moves_last = (int64_t)Mv;
+            st->moves_total += (int64_t)Mv;
+            st->fertile_total += (int64_t)Fe;
+        }
         status_snapshot(x);
         return;
     }

@@ -126,6 +126,7 @@ struct qhg_pop {
     uint32_t *d_bpos = nullptr, *d_rpos = nullptr;
     uint64_t *d_tiles = nullptr, *d_tiles2 = nullptr;
     uint32_t *d_wave_part = nullptr;
+    uint64_t *d_scan_part = nullptr;   // 3 per k_cell_scan wave (grid <= ntiles blocks)
     DevStatus *d_st = nullptr;
 
     Prog prog{};
@@ -210,6 +211,8 @@ struct qhg_pop {
         x.ntiles = ntiles;
         x.nact_blocks = act_grid;
         x.wave_part = d_wave_part;
+        x.scan_part = d_scan_part;
+        x.scan_waves = scan_grid * (kScanBlock / kWave);
         x.gpool = d_gpool[gp_cur];
         x.gwords = gwords;
         x.gcap = gcap;
@@ -259,6 +262,7 @@ int64_t device_bytes(const qhg_pop *p) {
     if (p->d_coastal) per_cell += 1;
     b += nc * per_cell + 3 * 4;
     b += 2 * (int64_t)p->ntiles * 8 + (int64_t)p->ntiles * 4 + (int64_t)p->num_cu * kMaxBlocksPerCu * kParts * 4 +
+         (int64_t)p->ntiles * (kScanBlock / kWave) * 3 * 8 +
          (int64_t)sizeof(DevStatus);
     return b;
 }
@@ -907,6 +911,7 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
         hipMalloc(&p->d_occ[0], nc * sizeof(OccCell)) != hipSuccess ||
         hipMalloc(&p->d_occ[1], nc * sizeof(OccCell)) != hipSuccess ||
         hipMalloc(&p->d_wave_part, (size_t)p->num_cu * kMaxBlocksPerCu * kParts * 4) != hipSuccess ||
+        hipMalloc(&p->d_scan_part, (size_t)p->ntiles * (kScanBlock / kWave) * 3 * 8) != hipSuccess ||
         hipMalloc(&p->d_st, sizeof(DevStatus)) != hipSuccess)
         return cleanup_fail(fail(QHG_ERR_NOMEM, "device allocation failed"));
     (void)hipMemsetAsync(p->d_K, 0, nc * 8, p->stream);
@@ -958,7 +963,7 @@ int qhg_gpu_destroy(qhg_pop *p) {
     void *ptrs[] = {p->d_off[0], p->d_off[1], p->d_gpool[0], p->d_gpool[1], p->d_nbr, p->d_nn, p->d_in_e,
                     p->d_K, p->d_W, p->d_tmp, p->d_ice, p->d_coastal, p->d_cstay, p->d_cbirth, p->d_cin, p->d_carr, p->d_tact, p->d_tick, p->d_outk,
                     p->d_birth_off, p->d_bpos, p->d_rpos, p->d_tiles, p->d_tiles2, p->d_occ[0], p->d_occ[1],
-                    p->d_wave_part, p->d_st};
+                    p->d_wave_part, p->d_scan_part, p->d_st};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     for (int k = 0; k < 7; k++)

@@ -33,6 +33,7 @@
 // Genomes live in a pool addressed by a per-agent handle, so they are only
 // touched at births and at occasional pool compactions.
 #pragma once
+#include <cstddef>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -1871,17 +1872,6 @@ __device__ __forceinline__ void group_scatter(const CtxT &x, const StepArgs &sa,
     wave_sync();
 }
 
-// Block 0 of the k_cell_scatter grid (dispatched first, so it overlaps the
-// scatter work) copies the finished step's status into host memory.
-__device__ void status_snapshot(const Ctx &x) {
-    if (threadIdx.x != 0 || !x.hring) return;
-    const DevStatus *st = x.st;
-    DevStatus *dst = x.hring + (st->step % kStatusRing);
-    const uint64_t *srcw = reinterpret_cast<const uint64_t *>(st);
-    uint64_t *dstw = reinterpret_cast<uint64_t *>(dst);
-    for (int k = 0; k < (int)(sizeof(DevStatus) / 8); k++) dstw[k] = srcw[k];
-}
-
 __global__ void __launch_bounds__(kCellBlock) k_cell_scatter(Ctx x, StepArgs sa) {
     __shared__ ScatLds lds;
     const int lane = threadIdx.x;
@@ -1902,8 +1892,13 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_scatter(Ctx x, StepArgs sa)
     }
     if (err) return;
     if (blockIdx.x == 0) {
-        // the step's deaths, moves and FERTILE counts from k_cell_scan's
-        // per-wave sums, then the host's status snapshot
+        // the step's deaths, moves and FERTILE counts (k_cell_scan's per-wave
+        // sums) go into the status and the host's snapshot; the status words
+        // and the sums load in the same round trip, one word per lane
+        constexpr int kWords = (int)(sizeof(DevStatus) / 8);
+        static_assert(kWords <= kWave, "one status word per lane");
+        uint64_t *sw = reinterpret_cast<uint64_t *>(x.st);
+        uint64_t w = (lane < kWords) ? sw[lane] : 0ull;
         uint64_t D = 0, Mv = 0, Fe = 0;
         for (int32_t j = lane; j < x.scan_waves; j += kWave) {
             D += x.scan_part[3 * j + 0];
@@ -1913,15 +1908,21 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_scatter(Ctx x, StepArgs sa)
         D = wave_sum_u64(D);
         Mv = wave_sum_u64(Mv);
         Fe = wave_sum_u64(Fe);
-        if (lane == 0) {
-            DevStatus *st = x.st;
-            st->deaths_last = (int64_t)D;
-            st->deaths_total += (int64_t)D;
-            st->moves_last = (int64_t)Mv;
-            st->moves_total += (int64_t)Mv;
-            st->fertile_total += (int64_t)Fe;
+        bool upd = true;
+        if (lane == (int)(offsetof(DevStatus, deaths_last) / 8)) w = D;
+        else if (lane == (int)(offsetof(DevStatus, deaths_total) / 8)) w += D;
+        else if (lane == (int)(offsetof(DevStatus, moves_last) / 8)) w = Mv;
+        else if (lane == (int)(offsetof(DevStatus, moves_total) / 8)) w += Mv;
+        else if (lane == (int)(offsetof(DevStatus, fertile_total) / 8)) w += Fe;
+        else upd = false;
+        if (upd) sw[lane] = w;
+        if (x.hring) {
+            constexpr int kStep = (int)(offsetof(DevStatus, step) / 8);
+            const uint64_t step = ((uint64_t)readlane_u32((uint32_t)(w >> 32), kStep) << 32) |
+                                  readlane_u32((uint32_t)w, kStep);
+            uint64_t *dst = reinterpret_cast<uint64_t *>(x.hring + (int64_t)(step % (uint64_t)kStatusRing));
+            if (lane < kWords) dst[lane] = w;
         }
-        status_snapshot(x);
         return;
     }
     const int64_t ngroups = (nocc + gc - 1) / gc;

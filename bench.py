@@ -51,7 +51,7 @@ HBM_PEAK_GBPS = 8000.0
 # what it reads for a 256 MiB device copy (FETCH x2, WRITE x1); see
 # profiles/step_kernels_c2_counters.md.  Reported as roofline.traffic only for
 # that exact workload.
-MEASURED_TRAFFIC = {("c2", 100, 1900): {"k_cell_act": 23.26e6, "k_cell_scatter": 41.33e6, "k_cell_scan": 4.71e6}}
+MEASURED_TRAFFIC = {("c2", 100, 1900): {"k_cell_act": 23.26e6, "k_cell_scatter": 41.35e6, "k_cell_scan": 4.70e6}}
 METRIC = "agent-steps/sec at 1 MI355X + %HBM peak; ensemble vs plain speedup, p50/p99"
 
 

@@ -42,6 +42,7 @@ extern "C" {
 #define QHG_ERR_CAPACITY   -3
 #define QHG_ERR_STATE      -4
 #define QHG_ERR_NOMEM      -5
+#define QHG_ERR_MEMBER     -6   /* batch: member(s) failed and left the batch; the others completed the call */
 
 /* life states, QHG4/core/SPopulation.h:70-74 */
 #define QHG_LIFE_DEAD     0
@@ -198,7 +199,7 @@ typedef struct qhg_stats {
     int64_t  births_total, deaths_total, moves_total;
     int64_t  compactions;
     int32_t  device_error;
-    int32_t  pad;
+    int32_t  host_error;     /* code with which a batch step of this population failed, 0 if none */
     int64_t  agent_steps;    /* sum over steps of the live count at step start */
     int64_t  slot_steps;     /* sum over steps of the slots in use at step start */
     int64_t  fertile_total;  /* FERTILE agents considered for pairing, all steps */
@@ -269,6 +270,14 @@ int qhg_gpu_cell_counts(qhg_pop *pop, uint32_t *out);        /* num_cells */
 int qhg_gpu_capacities(qhg_pop *pop, double *out);           /* num_cells */
 int qhg_gpu_env_weights(qhg_pop *pop, double *out);          /* num_cells*(conn+1) */
 int qhg_gpu_compact(qhg_pop *pop);   /* LBController::compactData: here the genome pool */
+/* Digest of the whole population state: out[0] = live agents, out[1..2] =
+ * two order-independent sums of a 64-bit hash of every live agent's fields
+ * (id, cell, life state, gender, birth time, age, last birth, stored partner
+ * id, number of babies, genome words).  Two populations in the same state
+ * have the same digest whatever order their buffers hold the agents in.  No
+ * counterpart in the reference; it lets ensembles and servers compare full
+ * states without downloading them. */
+int qhg_gpu_state_digest(qhg_pop *pop, uint64_t out[3]);
 
 /* profiling: average device time (ms) per launch of each kernel since the
  * last reset, measured with HIP events on the handle's stream. */
@@ -373,6 +382,10 @@ int qhg_batch_destroy(qhg_batch *batch);
 int qhg_batch_add(qhg_batch *batch, qhg_pop *pop);
 int qhg_batch_remove(qhg_batch *batch, qhg_pop *pop);
 int qhg_batch_size(qhg_batch *batch, int32_t *n);
+/* A member whose step cannot be prepared (its device reported an error,
+ * its buffers could not grow) leaves the batch with its code in
+ * qhg_stats.host_error; the other members complete the call's steps and the
+ * call returns QHG_ERR_MEMBER (qhg_batch_autotune likewise). */
 int qhg_batch_run(qhg_batch *batch, int32_t nsteps);
 int qhg_batch_sync(qhg_batch *batch);
 int qhg_batch_set_timing(qhg_batch *batch, int32_t enable);

@@ -479,17 +479,37 @@ int orc_get_agents(orc *o, qhg_agents *a) {
     return 0;
 }
 
+/* An exponential built from IEEE +,-,*,/ and ldexp only, so that the
+ * restatement and the device agree to the bit where the reference calls
+ * libm exp (the Miami NPP below).  Cody-Waite reduction x = k ln2 + r with
+ * ln2 = head (32 significant bits, so k * head is exact) + tail, then
+ * e^r = 1 + r(1 + r/2(1 + r/3(... (1 + r/13)))), then 2^k e^r.  Below
+ * 2^-1022 the result is 0, above 2^1023 it is +inf.  Pinned against libm exp
+ * to a few ulp by the tests. */
+double orc_exp(double x) {
+    if (isnan(x)) return x;
+    if (x > 709.0) return INFINITY;
+    if (x < -708.0) return 0.0;
+    double kr = x * 1.44269504088896338700e+00;
+    int k = (int)(kr + (kr >= 0.0 ? 0.5 : -0.5));
+    double r = (x - (double)k * 6.93147180369123816490e-01) - (double)k * 1.90821492927058770002e-10;
+    double s = 1.0;
+    for (int n = 13; n >= 2; n--) s = 1.0 + (r / (double)n) * s;
+    s = 1.0 + r * s;
+    return ldexp(s, k);
+}
+
 /* ------------------------------------------------------------------ */
 /* NPPCapacity::recalculate, QHG4/actions/NPPCapacity.cpp:138-217;     */
-/* Miami NPP, QHG4/core/NPPCalcMiami.cpp                               */
+/* Miami NPP, QHG4/core/NPPCalcMiami.cpp:28-41 (exp via orc_exp)       */
 /* ------------------------------------------------------------------ */
 static void npp_capacity(orc *o) {
     const qhg_params *P = &o->P;
     for (int32_t i = 0; i < o->ncells; i++) {
         double miami = 0;
         if (o->temp && o->rain) {
-            double nT = 3000.0 / (1 + exp(1.315 - 0.119 * o->temp[i]));
-            double nP = 3000.0 * (1 - exp(-0.000664 * o->rain[i]));
+            double nT = 3000.0 / (1 + orc_exp(1.315 - 0.119 * o->temp[i]));
+            double nP = 3000.0 * (1 - orc_exp(-0.000664 * o->rain[i]));
             miami = 0.000475 * ((nT < nP) ? nT : nP);
         }
         double cap;
@@ -659,6 +679,26 @@ int orc_pre_loop(orc *o) {
 /* initialize phase                                                    */
 /* ------------------------------------------------------------------ */
 
+/* The pairing rule of RandomPair::findMates (QHG4/actions/RandomPair.cpp
+ * :212-235 and, with the sexes swapped, :249-272): the side with fewer
+ * candidates chooses in list order; chooser j takes
+ * k = (int)wrandr(0, left, u[j]), left = candidates still unpaired, and is
+ * matched with the k-th unpaired candidate (0-based) in list order.
+ * pick[j] receives the chosen candidate's list index; taken (ncand bytes)
+ * is scratch.  Requires nchoose <= ncand. */
+void orc_pair_cell(int32_t nchoose, int32_t ncand, const uint32_t *u, uint8_t *taken, int32_t *pick) {
+    memset(taken, 0, (size_t)ncand);
+    int32_t left = ncand;
+    for (int32_t j = 0; j < nchoose && left > 0; j++) {
+        int32_t choose = (int32_t)orc_wrandr(0, left, u[j]);
+        int32_t found = -1, m = -1;
+        while (found < choose) { m++; if (!taken[m]) found++; }
+        taken[m] = 1;
+        pick[j] = m;
+        left--;
+    }
+}
+
 /* RandomPair::initialize + findMates, QHG4/actions/RandomPair.cpp:107-279 */
 static void random_pair(orc *o) {
     int64_t n = o->n;
@@ -681,7 +721,8 @@ static void random_pair(orc *o) {
 #pragma omp parallel
     {
         int32_t tcap = 0;
-        int32_t *fem = NULL, *mal = NULL;
+        int32_t *fem = NULL, *mal = NULL, *pick = NULL;
+        uint32_t *u = NULL;
         uint8_t *taken = NULL;
 #pragma omp for schedule(dynamic, 64)
         for (int32_t c = 0; c < nc; c++) {
@@ -691,6 +732,8 @@ static void random_pair(orc *o) {
                 tcap = e - s;
                 fem = (int32_t *)realloc(fem, (size_t)tcap * 4);
                 mal = (int32_t *)realloc(mal, (size_t)tcap * 4);
+                pick = (int32_t *)realloc(pick, (size_t)tcap * 4);
+                u = (uint32_t *)realloc(u, (size_t)tcap * 4);
                 taken = (uint8_t *)realloc(taken, (size_t)tcap);
             }
             int32_t F = 0, M = 0;
@@ -699,35 +742,19 @@ static void random_pair(orc *o) {
                 if (o->gender[a] == 0) fem[F++] = a; else mal[M++] = a;
             }
             if (F == 0 || M == 0) continue;
-            if (F <= M) {
-                memset(taken, 0, (size_t)M);
-                int32_t left = M;
-                for (int32_t f = 0; f < F && left > 0; f++) {
-                    uint32_t u = orc_draw(seed, step, QHG_STREAM_PAIR, (uint64_t)o->id[fem[f]], 0);
-                    int32_t choose = (int32_t)orc_wrandr(0, left, u);
-                    int32_t found = -1, m = -1;
-                    while (found < choose) { m++; if (!taken[m]) found++; }
-                    taken[m] = 1;
-                    o->mate[fem[f]] = mal[m];
-                    o->mate[mal[m]] = fem[f];
-                    left--;
-                }
-            } else {
-                memset(taken, 0, (size_t)F);
-                int32_t left = F;
-                for (int32_t m = 0; m < M && left > 0; m++) {
-                    uint32_t u = orc_draw(seed, step, QHG_STREAM_PAIR, (uint64_t)o->id[mal[m]], 0);
-                    int32_t choose = (int32_t)orc_wrandr(0, left, u);
-                    int32_t found = -1, f = -1;
-                    while (found < choose) { f++; if (!taken[f]) found++; }
-                    taken[f] = 1;
-                    o->mate[fem[f]] = mal[m];
-                    o->mate[mal[m]] = fem[f];
-                    left--;
-                }
+            /* females choose among males when F <= M (:212-235), else males
+             * among females (:249-272); each chooser's draw is keyed by its id */
+            int32_t *chooser = (F <= M) ? fem : mal, *cand = (F <= M) ? mal : fem;
+            int32_t nch = (F <= M) ? F : M, nca = (F <= M) ? M : F;
+            for (int32_t j = 0; j < nch; j++)
+                u[j] = orc_draw(seed, step, QHG_STREAM_PAIR, (uint64_t)o->id[chooser[j]], 0);
+            orc_pair_cell(nch, nca, u, taken, pick);
+            for (int32_t j = 0; j < nch; j++) {
+                o->mate[chooser[j]] = cand[pick[j]];
+                o->mate[cand[pick[j]]] = chooser[j];
             }
         }
-        free(fem); free(mal); free(taken);
+        free(fem); free(mal); free(pick); free(u); free(taken);
     }
 }
 

@@ -46,6 +46,10 @@ uint32_t orc_draw(uint64_t seed, uint32_t step, uint32_t stream, uint64_t id, ui
 double   orc_wrandd(uint32_t u);
 double   orc_wrandr(double a, double b, uint32_t u);
 uint32_t orc_wrandi(uint32_t a, uint32_t b, uint32_t s, uint32_t u);
+/* RandomPair's k-th-unpaired rule for one cell: nchoose choosers (the
+ * smaller sex, id order) with draws u[], ncand candidates; pick[j] = the
+ * list index chooser j takes.  taken is ncand bytes of scratch. */
+void     orc_pair_cell(int32_t nchoose, int32_t ncand, const uint32_t *u, uint8_t *taken, int32_t *pick);
 /* AssortativePairHamming pieces: GeneUtils::calcDist and the RankTable
  * pairing over a row-major nf x nm table with the draws u[0..nu) */
 int32_t  orc_calc_dist(const uint64_t *g1, const uint64_t *g2, int32_t size_arg);
@@ -63,6 +67,7 @@ int32_t  orc_bino_getn(const double *cdf, int32_t nbins, double u);
 double   orc_betai(double a, double b, double x);
 double   orc_polyline(int32_t npts, const double *x, const double *v, double X);
 double   orc_atan(double x);
+double   orc_exp(double x);
 int32_t  orc_genome_words(int32_t genome_size);
 
 #ifdef __cplusplus

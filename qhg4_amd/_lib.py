@@ -18,7 +18,7 @@ EXPORTS = [
     "qhg_gpu_create", "qhg_gpu_destroy", "qhg_gpu_upload_agents", "qhg_gpu_num_alive",
     "qhg_gpu_download_agents", "qhg_gpu_pre_loop", "qhg_gpu_initialize_step", "qhg_gpu_do_actions",
     "qhg_gpu_finalize_step", "qhg_gpu_run", "qhg_gpu_set_env", "qhg_gpu_env_event", "qhg_gpu_sync", "qhg_gpu_stats",
-    "qhg_gpu_cell_counts", "qhg_gpu_capacities", "qhg_gpu_env_weights", "qhg_gpu_compact",
+    "qhg_gpu_cell_counts", "qhg_gpu_capacities", "qhg_gpu_env_weights", "qhg_gpu_compact", "qhg_gpu_state_digest",
     "qhg_gpu_set_timing", "qhg_gpu_kernel_times", "qhg_gpu_kernel_name", "qhg_gpu_device_bytes",
     "qhg_gpu_set_tuning", "qhg_gpu_get_tuning", "qhg_gpu_autotune",
     "qhg_params_from_modules", "qhg_params_get_attribute", "qhg_params_set_attribute", "qhg_gpu_modify_attribute",
@@ -32,7 +32,9 @@ EXPORTS = [
 
 
 class QhgError(RuntimeError):
-    pass
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 def lib():
@@ -68,6 +70,7 @@ def lib():
         "qhg_gpu_capacities": ([h, P(C.c_double)], C.c_int),
         "qhg_gpu_env_weights": ([h, P(C.c_double)], C.c_int),
         "qhg_gpu_compact": ([h], C.c_int),
+        "qhg_gpu_state_digest": ([h, P(C.c_uint64)], C.c_int),
         "qhg_gpu_set_timing": ([h, C.c_int32], C.c_int),
         "qhg_gpu_kernel_times": ([h, P(C.c_double), P(C.c_int64), C.c_int32, P(C.c_int32)], C.c_int),
         "qhg_gpu_kernel_name": ([C.c_int32], C.c_char_p),
@@ -113,4 +116,4 @@ def lib():
 
 def check(rc, what):
     if rc != 0:
-        raise QhgError("%s failed (%d): %s" % (what, rc, lib().qhg_gpu_last_error().decode()))
+        raise QhgError("%s failed (%d): %s" % (what, rc, lib().qhg_gpu_last_error().decode()), rc)

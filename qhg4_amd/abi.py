@@ -9,6 +9,7 @@ import ctypes as C
 import numpy as np
 
 QHG_OK = 0
+QHG_ERR_MEMBER = -6
 QHG_LIFE_DEAD, QHG_LIFE_ALIVE, QHG_LIFE_FERTILE = 0, 1, 5
 
 ACT_GETOLD = 1
@@ -161,7 +162,7 @@ class Stats(C.Structure):
         ("moves_total", C.c_int64),
         ("compactions", C.c_int64),
         ("device_error", C.c_int32),
-        ("pad", C.c_int32),
+        ("host_error", C.c_int32),
         ("agent_steps", C.c_int64),
         ("slot_steps", C.c_int64),
         ("fertile_total", C.c_int64),

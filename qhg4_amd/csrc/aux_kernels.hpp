@@ -17,14 +17,16 @@ struct CapParams {
 };
 
 // NPPCapacity::recalculate, QHG4/actions/NPPCapacity.cpp:138-217, with the
-// Miami NPP of QHG4/core/NPPCalcMiami.cpp for the Oceania substitution.
+// Miami NPP of QHG4/core/NPPCalcMiami.cpp:28-41 for the Oceania
+// substitution (its exponentials through det_exp, so every cell's K is the
+// same double on the device and on the host).
 __global__ void __launch_bounds__(kBlock) k_capacity(Ctx x, EnvPtrs e, CapParams P) {
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < x.ncells;
          i += (int64_t)gridDim.x * kBlock) {
         double miami = 0;
         if (e.temp && e.rain) {
-            const double nT = 3000.0 / (1 + exp(1.315 - 0.119 * e.temp[i]));
-            const double nP = 3000.0 * (1 - exp(-0.000664 * e.rain[i]));
+            const double nT = 3000.0 / (1 + det_exp(1.315 - 0.119 * e.temp[i]));
+            const double nP = 3000.0 * (1 - det_exp(-0.000664 * e.rain[i]));
             miami = 0.000475 * ((nT < nP) ? nT : nP);
         }
         double cap;
@@ -217,6 +219,61 @@ __global__ void __launch_bounds__(kBlock) k_env_move(AgentBuf from, AgentBuf to,
             to.nbabies[t] = from.nbabies[f];
             if (to.ghandle) to.ghandle[t] = from.ghandle[f];
         }
+    }
+}
+
+}  // namespace qhg
+
+namespace qhg {
+
+// The output function of SplitMix64 (Steele, Lea, Flood: "Fast splittable
+// pseudorandom number generators", OOPSLA 2014): a bijective 64-bit mix.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Digest of a population's state: every live agent's fields (id, cell,
+// life state, gender, birth time, age, last birth, stored partner id,
+// number of babies, genome words) are folded into one 64-bit hash, and the
+// hashes are summed in two independent ways, so the digest does not depend
+// on the order the agents are stored in.  One wave per occupied cell in
+// grid-stride order; out[0..1] (zeroed by the caller) receive the sums.
+__global__ void __launch_bounds__(kBlock) k_state_digest(AgentBuf a, const OccCell *occ, int64_t nocc,
+                                                         const uint64_t *gpool, int32_t gwords,
+                                                         unsigned long long *out) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t nwaves = ((int64_t)gridDim.x * kBlock) / kWave;
+    uint64_t s0 = 0, s1 = 0;
+    for (int64_t e = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave; e < nocc; e += nwaves) {
+        const OccCell oc = occ[e];
+        for (uint32_t i = oc.s + lane; i < oc.s + oc.n; i += kWave) {
+            uint64_t h = 0x9E3779B97F4A7C15ull;
+            h = mix64(h ^ (uint64_t)a.id[i]);
+            h = mix64(h ^ (uint64_t)(uint32_t)oc.c);
+            h = mix64(h ^ ((uint64_t)a.life[i] | ((uint64_t)a.gender[i] << 8)));
+            h = mix64(h ^ (uint64_t)__double_as_longlong(a.birth[i]));
+            h = mix64(h ^ (uint64_t)__double_as_longlong(a.age[i]));
+            h = mix64(h ^ (uint64_t)__double_as_longlong(a.lastb[i]));
+            h = mix64(h ^ (uint64_t)a.mate_id[i]);
+            h = mix64(h ^ (uint64_t)(uint32_t)a.nbabies[i]);
+            if (gwords > 0) {
+                const uint64_t *g = gpool + (size_t)a.ghandle[i] * (size_t)gwords;
+                for (int32_t k = 0; k < gwords; k++) h = mix64(h ^ g[k]);
+            }
+            s0 += h;
+            s1 += mix64(h ^ 0xD6E8FEB86659FD93ull);
+        }
+    }
+#pragma unroll
+    for (int d = kWave / 2; d >= 1; d >>= 1) {
+        s0 += (uint64_t)__shfl_xor((unsigned long long)s0, d, kWave);
+        s1 += (uint64_t)__shfl_xor((unsigned long long)s1, d, kWave);
+    }
+    if (lane == 0 && (s0 | s1)) {
+        atomicAdd(&out[0], (unsigned long long)s0);
+        atomicAdd(&out[1], (unsigned long long)s1);
     }
 }
 

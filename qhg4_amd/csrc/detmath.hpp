@@ -41,3 +41,41 @@ QHG_HD double det_atan(double x) {
     if (inv) r = kHalfPi - r;
     return neg ? -r : r;
 }
+
+// Deterministic exponential for the Miami NPP of NPPCapacity's Oceania rule
+// (QHG4/core/NPPCalcMiami.cpp:28-41 calls libm exp on the host; the device
+// and a host restatement must give the same double).  Only IEEE +, -, *, /
+// and the exact power-of-two scaling ldexp, in a fixed order:
+//   1. k = x / ln 2 rounded to the nearest integer, r = x - k ln 2, with
+//      ln 2 split into a 32-bit head (k * head is exact for |k| < 2^21) and
+//      a tail (Cody and Waite's reduction), so |r| <= ln(2)/2;
+//   2. e^r by its Taylor series through r^13 in nested form,
+//      1 + r(1 + r/2(1 + r/3(... (1 + r/13)))) (remainder below 4e-18);
+//   3. e^x = 2^k e^r.
+// Results below 2^-1022 are flushed to 0 and above 2^1023 give +inf.
+QHG_HD double det_exp(double x) {
+    const double kLn2Hi = 6.93147180369123816490e-01;   // 0x3FE62E42FEE00000
+    const double kLn2Lo = 1.90821492927058770002e-10;   // 0x3DEA39EF35793C76
+    const double kInvLn2 = 1.44269504088896338700e+00;
+    if (x != x) return x;
+    if (x > 709.0) return __builtin_inf();
+    if (x < -708.0) return 0.0;
+    const double kr = x * kInvLn2;
+    const int k = (int)(kr + (kr >= 0.0 ? 0.5 : -0.5));
+    const double r = (x - (double)k * kLn2Hi) - (double)k * kLn2Lo;
+    double s = 1.0;
+    s = 1.0 + (r / 13.0) * s;
+    s = 1.0 + (r / 12.0) * s;
+    s = 1.0 + (r / 11.0) * s;
+    s = 1.0 + (r / 10.0) * s;
+    s = 1.0 + (r / 9.0) * s;
+    s = 1.0 + (r / 8.0) * s;
+    s = 1.0 + (r / 7.0) * s;
+    s = 1.0 + (r / 6.0) * s;
+    s = 1.0 + (r / 5.0) * s;
+    s = 1.0 + (r / 4.0) * s;
+    s = 1.0 + (r / 3.0) * s;
+    s = 1.0 + (r / 2.0) * s;
+    s = 1.0 + r * s;
+    return ldexp(s, k);
+}

@@ -127,6 +127,7 @@ struct qhg_pop {
     uint64_t *d_tiles = nullptr, *d_tiles2 = nullptr;
     uint32_t *d_wave_part = nullptr;
     uint64_t *d_scan_part = nullptr;   // 3 per k_cell_scan wave (grid <= ntiles blocks)
+    unsigned long long *d_digest = nullptr;   // 2 sums of qhg_gpu_state_digest (allocated at its first call)
     DevStatus *d_st = nullptr;
 
     Prog prog{};
@@ -164,6 +165,7 @@ struct qhg_pop {
     int64_t ring_bstep[kRing];  // batch step each ring entry's event closes
     int64_t *bdone = nullptr;   // newest batch step known complete
     int64_t gen = 0;          // bumped whenever a device buffer moves
+    int32_t host_error = 0;   // code with which a batch step of this population failed (it then left its batch)
 
     Ctx ctx() const { return ctx_for(cur()); }
 
@@ -963,7 +965,7 @@ int qhg_gpu_destroy(qhg_pop *p) {
     void *ptrs[] = {p->d_off[0], p->d_off[1], p->d_gpool[0], p->d_gpool[1], p->d_nbr, p->d_nn, p->d_in_e,
                     p->d_K, p->d_W, p->d_tmp, p->d_ice, p->d_coastal, p->d_cstay, p->d_cbirth, p->d_cin, p->d_carr, p->d_tact, p->d_tick, p->d_outk,
                     p->d_birth_off, p->d_bpos, p->d_rpos, p->d_tiles, p->d_tiles2, p->d_occ[0], p->d_occ[1],
-                    p->d_wave_part, p->d_scan_part, p->d_st};
+                    p->d_wave_part, p->d_scan_part, p->d_st, p->d_digest};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     for (int k = 0; k < 7; k++)
@@ -1240,6 +1242,30 @@ int qhg_gpu_download_agents(qhg_pop *p, qhg_agents *a) {
     return 0;
 }
 
+int qhg_gpu_state_digest(qhg_pop *p, uint64_t out[3]) {
+    if (!p || !out) return fail(QHG_ERR_ARG, "null argument");
+    DevStatus s;
+    int rc = read_status_sync(p, &s);
+    if (rc) return rc;
+    const int b = p->cur();
+    if (!p->d_digest) HIPCHK(hipMalloc(&p->d_digest, 2 * sizeof(unsigned long long)));
+    HIPCHK(hipMemsetAsync(p->d_digest, 0, 2 * sizeof(unsigned long long), p->stream));
+    const int64_t nocc = s.n_occ[b];
+    if (nocc > 0) {
+        const int grid = (int)std::min<int64_t>((nocc + kBlock / kWave - 1) / (kBlock / kWave), 8 * p->num_cu);
+        hipLaunchKernelGGL(k_state_digest, dim3(grid), dim3(kBlock), 0, p->stream, p->buf[b], p->d_occ[b], nocc,
+                           p->gwords ? p->d_gpool[p->gp_cur] : nullptr, p->gwords, p->d_digest);
+        HIPCHK(hipGetLastError());
+    }
+    unsigned long long h[2];
+    HIPCHK(hipMemcpyAsync(h, p->d_digest, sizeof(h), hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    out[0] = (uint64_t)s.n_alive[b];
+    out[1] = (uint64_t)h[0];
+    out[2] = (uint64_t)h[1];
+    return 0;
+}
+
 int qhg_gpu_pre_loop(qhg_pop *p, double t) {
     if (!p) return fail(QHG_ERR_ARG, "null handle");
     p->t_next = t;
@@ -1394,6 +1420,7 @@ int qhg_gpu_stats(qhg_pop *p, qhg_stats *o) {
     o->moves_total = s.moves_total;
     o->compactions = p->compactions;
     o->device_error = s.error;
+    o->host_error = p->host_error;
     o->agent_steps = s.agent_steps;
     o->slot_steps = s.agent_steps;
     o->fertile_total = s.fertile_total;
@@ -1884,13 +1911,24 @@ int qhg_batch_size(qhg_batch *b, int32_t *n) {
 
 int qhg_batch_run(qhg_batch *b, int32_t nsteps) {
     if (!b) return fail(QHG_ERR_ARG, "null handle");
+    int nfailed = 0;
+    std::string first_msg;
     for (int32_t s = 0; s < nsteps; s++) {
-        if (b->pops.empty()) return 0;
+        if (b->pops.empty()) break;
         batch_poll(b);
-        for (qhg_pop *p : b->pops) {
+        // a member whose step cannot be prepared (its device reported an
+        // error, its buffers could not grow) leaves the batch with its error
+        // recorded; the other members go on
+        for (size_t i = 0; i < b->pops.size();) {
+            qhg_pop *p = b->pops[i];
             int rc = ensure_capacity(p);
-            if (rc) return rc;
+            if (rc == 0) { i++; continue; }
+            if (nfailed == 0) first_msg = qhg_gpu_last_error();
+            nfailed++;
+            p->host_error = rc;
+            (void)qhg_batch_remove(b, p);
         }
+        if (b->pops.empty()) break;
         bool stale = b->dirty;
         for (size_t i = 0; i < b->pops.size(); i++) stale = stale || b->gens[i] != b->pops[i]->gen;
         if (stale) {
@@ -1924,6 +1962,9 @@ int qhg_batch_run(qhg_batch *b, int32_t nsteps) {
             if (rc) return rc;
         }
     }
+    if (nfailed)
+        return fail(QHG_ERR_MEMBER, std::to_string(nfailed) + " member(s) failed and left the batch (first: " +
+                                        first_msg + ")");
     return 0;
 }
 
@@ -1966,6 +2007,7 @@ int qhg_batch_autotune(qhg_batch *b, int32_t trial_steps, qhg_tuning *chosen, in
     Timer tm;
     rc = tm.init();
     if (rc) return rc;
+    bool member_failed = false;
     auto agent_steps = [&](int64_t *sum) -> int {
         int r = qhg_batch_sync(b);
         if (r) return r;
@@ -1980,6 +2022,10 @@ int qhg_batch_autotune(qhg_batch *b, int32_t trial_steps, qhg_tuning *chosen, in
         if (r) return r;
         HIPCHK(hipEventRecord(tm.e0, b->stream));
         r = qhg_batch_run(b, trial_steps);
+        if (r == QHG_ERR_MEMBER) {   // the members still in the batch took the trial steps
+            member_failed = true;
+            r = 0;
+        }
         if (r) return r;
         HIPCHK(hipEventRecord(tm.e1, b->stream));
         r = agent_steps(&a1);
@@ -1996,6 +2042,7 @@ int qhg_batch_autotune(qhg_batch *b, int32_t trial_steps, qhg_tuning *chosen, in
     if (steps_taken) *steps_taken = taken;
     if (rc) return rc;
     if (chosen) *chosen = best;
+    if (member_failed) return fail(QHG_ERR_MEMBER, "member(s) failed during the tuning steps and left the batch");
     return 0;
 }
 

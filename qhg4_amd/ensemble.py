@@ -9,7 +9,8 @@ run_batched  every scenario as a member of one batch: one launch per phase
 
 Both give every scenario exactly the state it reaches alone (the batch kernels
 use the members' own Philox keys and step counters), so their per-scenario
-summaries are compared for equality.  There is no counterpart in the
+summaries, which carry a digest of the full agent state, are compared for
+equality.  There is no counterpart in the
 reference, which runs one population per process (QHG4/core/PopLooper.cpp).
 """
 import time
@@ -26,8 +27,13 @@ def capacity_for(sc):
     return 2 * len(sc["agents"]["id"]) + 65536
 
 
-def summary(stats):
-    return {k: int(stats[k]) for k in SUMMARY_KEYS}
+def summary(pop):
+    """A population's counters and the digest of its full state (every live
+    agent's fields and genome, qhg_gpu_state_digest)."""
+    stats = pop.stats()
+    out = {k: int(stats[k]) for k in SUMMARY_KEYS}
+    out["digest"] = pop.state_digest()
+    return out
 
 
 def create(sc):
@@ -50,7 +56,7 @@ def run_plain(make, n, steps, log=None):
         pop.run(steps)
         pop.sync()
         secs += time.perf_counter() - t0
-        out.append(summary(pop.stats()))
+        out.append(summary(pop))
         pop.close()
     return secs, out
 
@@ -95,7 +101,7 @@ class BatchedRun:
         return time.perf_counter() - t0, tuning
 
     def summaries(self):
-        return [summary(p.stats()) for p in self.pops]
+        return [summary(p) for p in self.pops]
 
     def device_bytes(self):
         return sum(p.device_bytes() for p in self.pops)

@@ -141,6 +141,14 @@ class GpuPopulation:
         check(lib().qhg_gpu_env_weights(self.h, abi.ptr(out, C.c_double)), "qhg_gpu_env_weights")
         return out
 
+    def state_digest(self):
+        """Digest of the full state (every live agent's fields and genome),
+        independent of the order the device holds the agents in:
+        "<live agents>:<hash>:<hash>"."""
+        out = (C.c_uint64 * 3)()
+        check(lib().qhg_gpu_state_digest(self.h, out), "qhg_gpu_state_digest")
+        return "%d:%016x:%016x" % (out[0], out[1], out[2])
+
     def set_timing(self, on):
         check(lib().qhg_gpu_set_timing(self.h, 1 if on else 0), "qhg_gpu_set_timing")
 
@@ -187,6 +195,24 @@ class GpuBatch:
         check(lib().qhg_batch_create(C.byref(h)), "qhg_batch_create")
         self.h = h
         self.members = []
+        self.failed = []      # members that failed a step and left the batch, not yet taken
+
+    def _members_left(self, rc, what):
+        """After a run or tuning call: members that failed left the batch
+        (their error stays in stats()["host_error"]); the others completed
+        the call.  Any other error is the batch's own."""
+        if rc == abi.QHG_ERR_MEMBER:
+            gone = [p for p in self.members if p.stats()["host_error"] != 0]
+            for p in gone:
+                self.members.remove(p)
+            self.failed.extend(gone)
+            return
+        check(rc, what)
+
+    def take_failed(self):
+        """Members that left the batch on an error since the last call."""
+        out, self.failed = self.failed, []
+        return out
 
     def add(self, pop):
         check(lib().qhg_batch_add(self.h, pop.h), "qhg_batch_add")
@@ -202,7 +228,9 @@ class GpuBatch:
         return n.value
 
     def run(self, nsteps):
-        check(lib().qhg_batch_run(self.h, int(nsteps)), "qhg_batch_run")
+        """Advance every member by nsteps steps.  A member that fails leaves
+        the batch (see take_failed); the others still take every step."""
+        self._members_left(lib().qhg_batch_run(self.h, int(nsteps)), "qhg_batch_run")
 
     def sync(self):
         check(lib().qhg_batch_sync(self.h), "qhg_batch_sync")
@@ -227,7 +255,8 @@ class GpuBatch:
         shape and the number of steps every member advanced."""
         t = abi.Tuning()
         n = C.c_int32()
-        check(lib().qhg_batch_autotune(self.h, int(trial_steps), C.byref(t), C.byref(n)), "qhg_batch_autotune")
+        self._members_left(lib().qhg_batch_autotune(self.h, int(trial_steps), C.byref(t), C.byref(n)),
+                           "qhg_batch_autotune")
         return t.as_dict(), n.value
 
     def kernel_times(self):

@@ -9,6 +9,11 @@ is one launch per phase for all members, so the runs overlap on the device.
 A member leaves the batch when it reaches its horizon; its final statistics
 are read back and its device memory is freed.
 
+A request that fails (its scenario or population cannot be built, uploaded or
+joined; its device step reports an error; its buffers cannot grow; its
+result cannot be read) is reported with an "error" and its slot and memory
+are freed; the other requests go on.
+
 Admission reserves the request's estimated device memory; a worker thread
 then builds its initial state, creates the population on its own stream and
 uploads it while the device works on the batch; it joins between chunks.  Every run ends in
@@ -109,8 +114,15 @@ class ScenarioServer:
                 # prepared populations join between chunks
                 for job in [j for j in preparing if j[1].done()]:
                     preparing.remove(job)
-                    pop = job[1].result()
-                    batch.add(pop)
+                    pop = None
+                    try:
+                        pop = job[1].result()
+                        batch.add(pop)
+                    except Exception as e:  # noqa: BLE001 -- one request's failure
+                        if pop is not None:
+                            pop.close()
+                        results.append(self._failed(job[0], job[3], None, self.clock() - t0, e))
+                        continue
                     active[pop] = [job[0], int(job[0]["horizon"]), job[3], self.clock() - t0]
                     used += pop.device_bytes()
                 peak_members = max(peak_members, len(active))
@@ -130,14 +142,25 @@ class ScenarioServer:
                     k = min(self.chunk, left)
                     batch.run(k)
                 batch.sync()
+                done_at = self.clock() - t0
+                for pop in batch.take_failed():      # left the batch on an error
+                    r, _, adm, joined = active.pop(pop)
+                    used -= pop.device_bytes()
+                    err = "step failed (code %d)" % pop.stats()["host_error"]
+                    pop.close()
+                    results.append(self._failed(r, adm, joined, done_at, err))
                 for e in active.values():
                     e[1] -= k
-                done_at = self.clock() - t0
                 for pop in [p for p, e in active.items() if e[1] == 0]:
                     r, _, adm, joined = active.pop(pop)
-                    batch.remove(pop)
-                    s = summary(pop.stats())
                     used -= pop.device_bytes()
+                    try:
+                        batch.remove(pop)
+                        s = summary(pop)
+                    except Exception as e:  # noqa: BLE001 -- one request's failure
+                        pop.close()
+                        results.append(self._failed(r, adm, joined, done_at, e))
+                        continue
                     pop.close()
                     results.append({"id": r["id"], "arrival": r["arrival"], "admitted": adm, "joined": joined,
                                     "done": done_at, "latency": done_at - r["arrival"], "summary": s})
@@ -159,6 +182,11 @@ class ScenarioServer:
         return results, self._report(results, t_end, peak_used, peak_members, tunings)
 
     # -- helpers ------------------------------------------------------------
+    @staticmethod
+    def _failed(r, admitted, joined, done, err):
+        return {"id": r["id"], "arrival": r["arrival"], "admitted": admitted, "joined": joined, "done": done,
+                "latency": done - r["arrival"], "summary": None, "error": str(err)}
+
     def _idle(self, pending, preparing, t0):
         if preparing:
             cf.wait([j[1] for j in preparing], timeout=0.05, return_when=cf.FIRST_COMPLETED)
@@ -167,11 +195,13 @@ class ScenarioServer:
 
     @staticmethod
     def _report(results, t_end, peak_used, peak_members, tunings):
-        lat = np.array([r["latency"] for r in results]) if results else np.zeros(1)
-        steps = sum(r["summary"]["agent_steps"] for r in results)
+        ok = [r for r in results if r["summary"] is not None]
+        lat = np.array([r["latency"] for r in ok]) if ok else np.zeros(1)
+        steps = sum(r["summary"]["agent_steps"] for r in ok)
         first = min((r["arrival"] for r in results), default=0.0)
         span = max(t_end - first, 1e-9)
-        return {"requests": len(results), "agent_steps": int(steps), "seconds": round(t_end, 3),
+        return {"requests": len(results), "failed": len(results) - len(ok), "agent_steps": int(steps),
+                "seconds": round(t_end, 3),
                 "value": steps / span, "p50_s": float(np.percentile(lat, 50)),
                 "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()),
                 "peak_device_bytes": int(peak_used), "peak_members": int(peak_members), "tunings": tunings}

@@ -31,7 +31,13 @@ class FakePop:
     def stats(self):
         n = len(self.sc["agents"]["id"])
         return {k: 0 for k in server.summary.__globals__["SUMMARY_KEYS"]} | {"step": self.steps,
-                                                                              "agent_steps": n * self.steps}
+                                                                              "agent_steps": n * self.steps,
+                                                                              "host_error": self.sc.get("err", 0)}
+
+    def state_digest(self):
+        if self.sc.get("bad_readback"):
+            raise RuntimeError("read-back failed")
+        return "%d:%d" % (len(self.sc["agents"]["id"]), self.steps)
 
     def close(self):
         if self in FakePop.live:
@@ -43,6 +49,7 @@ class FakeBatch:
         self.members = []
         self.max_seen = 0
         self.tuned = 0
+        self.failed = []
 
     def set_graph(self, on):
         pass
@@ -55,8 +62,20 @@ class FakeBatch:
         self.members.remove(p)
 
     def run(self, k):
-        for p in self.members:
+        for p in list(self.members):
+            fail_at = p.sc.get("fail_at")
+            if fail_at is not None and p.steps + k > fail_at:
+                # the device step of this member fails: it leaves the batch
+                p.steps = fail_at
+                p.sc["err"] = -3
+                self.members.remove(p)
+                self.failed.append(p)
+                continue
             p.steps += k
+
+    def take_failed(self):
+        out, self.failed = self.failed, []
+        return out
 
     def autotune(self, trial):
         self.tuned += 1
@@ -148,3 +167,41 @@ def test_autotune_never_overshoots_a_horizon(fakes):
 def test_server_rejects_bad_member_limit():
     with pytest.raises(ValueError):
         server.ScenarioServer(fake_scenario, max_members=300)
+
+
+def failing_scenario(req):
+    if req.get("bad_make"):
+        raise ValueError("no such grid")
+    sc = fake_scenario(req)
+    for k in ("fail_at", "bad_readback"):
+        if k in req:
+            sc[k] = req[k]
+    return sc
+
+
+@pytest.mark.parametrize("chunk", [1, 8])
+def test_a_failing_request_is_evicted_alone(fakes, chunk):
+    """A request whose scenario cannot be built, one whose device step fails
+    midway and one whose result cannot be read are reported as failed; every
+    other request still runs its full horizon, and nothing stays allocated."""
+    reqs = requests(40, seed=3)
+    reqs[3]["bad_make"] = True
+    reqs[11]["fail_at"] = max(1, reqs[11]["horizon"] // 2)
+    reqs[11]["horizon"] = max(reqs[11]["horizon"], 10)
+    reqs[20]["bad_readback"] = True
+    srv = server.ScenarioServer(failing_scenario, estimate=fake_estimate, budget_bytes=80e6, max_members=6,
+                                chunk=chunk, workers=2, tune_every_s=None, clock=Clock())
+    res, rep = srv.serve(reqs)
+    assert sorted(r["id"] for r in res) == list(range(40))
+    by = {r["id"]: r for r in res}
+    bad = {3, 11, 20}
+    for q in reqs:
+        r = by[q["id"]]
+        if q["id"] in bad:
+            assert r["summary"] is None and r["error"], q["id"]
+        else:
+            assert "error" not in r and r["summary"]["step"] == q["horizon"], q["id"]
+    assert "no such grid" in by[3]["error"] and "read-back" in by[20]["error"]
+    assert rep["failed"] == 3 and rep["requests"] == 40
+    assert not FakePop.live
+    assert not fakes[0].members

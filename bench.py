@@ -26,13 +26,18 @@ The same line carries two more objects:
             through the plain path and as members of one batch; value is
             the batched agent-steps/s, speedup_vs_plain the ratio of the two
             wall times, identical_to_plain the per-replica comparison.
-  serving   C5: a Poisson(20/s) stream of C2-style requests (400 by default,
-            --serve-requests) served by continuous batching under a 250 GB
+  serving   C5: a Poisson(20/s) stream of 2,000 C2-style requests
+            (--serve-requests) served by continuous batching under a 250 GB
             budget; p50/p99 completion latency, aggregate agent-steps/s over
             first arrival .. last completion, and a sample run alone.
---workload c4 / c5 print those measurements as lines of their own (c5 with
-the full 2,000-request stream by default).  --workload c2a times the C2 model
-with genetics and AssortativePairHamming pairing (k_assort_pair in kernels).
+and two C2 runs beside the timed window: c2_full_run, the whole 2,000-step
+run timed as one region, and c2_stated_patch, the run from the initial state
+as stated (100,000 agents in 200 cells), under which the population dies out.
+Ensemble and serving results are compared for identity through full-state
+digests (every live agent's fields and genome).
+--workload c4 / c5 print those measurements as lines of their own.
+--workload c2a times the C2 model with genetics and AssortativePairHamming
+pairing (k_assort_pair in kernels).
 """
 import argparse
 import json
@@ -46,12 +51,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
 
-# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on the
-# default workload and steps (100 warm-up + 1,900 timed), each counter scaled by
-# what it reads for a 256 MiB device copy (FETCH x2, WRITE x1); see
-# profiles/step_kernels_c2_counters.md.  Reported as roofline.traffic only for
-# that exact workload.
-MEASURED_TRAFFIC = {("c2", 100, 1900): {"k_cell_act": 23.26e6, "k_cell_scatter": 41.35e6, "k_cell_scan": 4.70e6}}
+# Measured HBM bytes per algorithmic byte of each step kernel: rocprofv3
+# FETCH_SIZE + WRITE_SIZE per dispatch on the C2 workload (each counter
+# scaled by what it reads for a 256 MiB device copy: FETCH x2, WRITE x1)
+# divided by the kernel's bytes_model.h bytes per dispatch over the same
+# steps (profiles/step_kernels_c2_counters.md).  roofline.traffic is this
+# ratio times the timed window's own algorithmic bytes per launch.
+TRAFFIC_PER_ALGO_BYTE = {"k_cell_act": 1.72, "k_cell_scatter": 1.82, "k_cell_scan": 14.2}
 METRIC = "agent-steps/sec at 1 MI355X + %HBM peak; ensemble vs plain speedup, p50/p99"
 
 
@@ -291,9 +297,14 @@ def delta(s1, s0, steps):
             "steps": steps}
 
 
+TUNING = {}   # launch shape of the plain path (--group-agents, --blocks-per-cu); results do not depend on it
+
+
 def gpu_run(sc, warmup, steps, timing):
     from qhg4_amd.population import GpuPopulation
     pop = GpuPopulation.from_scenario(sc, capacity=1 << 23)
+    if TUNING:
+        pop.set_tuning(**TUNING)
     pop.preLoop(0.0)
     if warmup:
         pop.run(warmup)
@@ -311,9 +322,18 @@ def gpu_run(sc, warmup, steps, timing):
     return t1 - t0, s0, s1, times
 
 
+def host_threads():
+    """Threads for the CPU baseline: every core this process may run on
+    (its affinity mask), bounded by OMP_NUM_THREADS where the host sets it
+    (the share of the host's cores that goes with one GPU)."""
+    cores = len(os.sched_getaffinity(0))
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return (min(cores, omp) if omp > 0 else cores), cores
+
+
 def cpu_baseline(sc, warmup, steps, budget_s):
     from oracle.oracle import Oracle, lib
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    threads, cores = host_threads()
     lib().orc_set_threads(threads)
     o = Oracle(sc)
     o.pre_loop()
@@ -330,8 +350,11 @@ def cpu_baseline(sc, warmup, steps, budget_s):
     agent_steps = s1["agent_steps"] - s0["agent_steps"]
     return {"value": agent_steps / (t1 - t0), "unit": "agent-steps/s", "cores": threads, "kind": "port",
             "sample": "oracle (reference step loop restated in C, OpenMP, same Philox draws) on steps "
-                      "%d..%d of the same workload (%d of the %d timed steps, %.1f s, %d agent-steps)"
-                      % (warmup, warmup + done, done, steps, t1 - t0, agent_steps)}
+                      "%d..%d of the same workload (%d of the %d timed steps, %.1f s, %d agent-steps); "
+                      "%d OpenMP threads = the cores in this process's affinity mask (%d) bounded by "
+                      "OMP_NUM_THREADS (%s)"
+                      % (warmup, warmup + done, done, steps, t1 - t0, agent_steps, threads, cores,
+                         os.environ.get("OMP_NUM_THREADS", "unset"))}
 
 
 def main():
@@ -348,10 +371,14 @@ def main():
     ap.add_argument("--replicas", type=int, default=256, help="C4: replicas in the ensemble")
     ap.add_argument("--no-ensemble", action="store_true", help="C2 line without the C4 ensemble object")
     ap.add_argument("--no-serving", action="store_true", help="C2 line without the C5 serving object")
-    ap.add_argument("--serve-requests", type=int, default=400,
-                    help="C5 requests for the serving object of the C2 line (the full stream: --workload c5)")
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="C5 requests for the serving object of the C2 line")
+    ap.add_argument("--no-extra-c2", action="store_true",
+                    help="C2 line without the full-run and stated-patch objects")
     ap.add_argument("--no-graph", action="store_true", help="C4: launch batch steps without graph replay")
     ap.add_argument("--no-tune", action="store_true", help="C4: keep the default launch shapes")
+    ap.add_argument("--group-agents", type=float, default=None, help="plain path: target agents per wave group")
+    ap.add_argument("--blocks-per-cu", type=int, default=None, help="plain path: act/scatter blocks per CU")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
@@ -378,6 +405,8 @@ def main():
         return 0
 
     from qhg4_amd import abi
+    if args.group_agents is not None or args.blocks_per_cu is not None:
+        TUNING.update(group_agents=args.group_agents or 64.0, blocks_per_cu=args.blocks_per_cu or 16)
     sc, desc = make_workload(args.workload, args.seed)
     print("bench: workload %s, warmup %d, steps %d" % (args.workload, args.warmup, args.steps),
           file=sys.stderr, flush=True)
@@ -406,9 +435,10 @@ def main():
                              "bytes_per_launch": int(per_launch), "GBps": round(gbps, 1)}
         dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
         kd = kernels[dom]
-        traffic = MEASURED_TRAFFIC.get((args.workload, args.warmup, args.steps), {}).get(dom)
+        ratio = TRAFFIC_PER_ALGO_BYTE.get(dom) if args.workload == "c2" else None
         roof = {"bound": "hbm", "achieved": kd["GBps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": round(kd["GBps"] / HBM_PEAK_GBPS, 5), "traffic": traffic, "kernel": dom,
+                "frac": round(kd["GBps"] / HBM_PEAK_GBPS, 5),
+                "traffic": int(ratio * kd["bytes_per_launch"]) if ratio else None, "kernel": dom,
                 "bytes_per_launch": kd["bytes_per_launch"], "avg_launch_us": kd["avg_us"]}
 
     cpu = None if args.no_cpu else cpu_baseline(sc, args.warmup, args.steps, args.cpu_seconds)
@@ -436,6 +466,9 @@ def main():
         "kernels": kernels,
     }
     # the ensemble and serving objects ride on the headline (C2) line only
+    if args.workload == "c2" and not args.no_extra_c2:
+        out["c2_full_run"] = c2_full_run(sc)
+        out["c2_stated_patch"] = c2_stated_patch(args.seed)
     if args.workload == "c2" and not args.no_ensemble:
         e = c4_leg(1000, args.replicas, 1000, graph=not args.no_graph, tune=not args.no_tune,
                    kernel_timing=not args.no_kernel_timing)
@@ -451,6 +484,42 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
     return 0
+
+
+def c2_full_run(sc):
+    """The whole 2,000-step C2 run from its initial state, timed as one
+    region (the headline value times only the requested window)."""
+    wall, s0, s1, _ = gpu_run(sc, 0, 2000, timing=False)
+    live = s1["agent_steps"] - s0["agent_steps"]
+    print("bench: C2 full run %.3f s, %d agent-steps" % (wall, live), file=sys.stderr, flush=True)
+    return {"steps": 2000, "value": round(live / wall, 1), "unit": "agent-steps/s",
+            "ms_per_step": round(wall * 1e3 / 2000, 5), "mean_live_agents": round(live / 2000, 1),
+            "final_live_agents": int(s1["num_alive"])}
+
+
+# VerhulstVarK values for the stated C2 start: C4's lowest birth and death
+# rates (b0 0.05, d0 0.01) and C1's theta (0.01).  At the stated patch (100,000
+# agents in 200 cells, N/K about 13) every value set within the configs'
+# ranges dies out; these die out last (about step 1,100).
+C2_STATED_VERHULST = (0.05, 0.01, 0.01)
+
+
+def c2_stated_patch(seed):
+    """C2 from the initial state as the configuration states it: 100,000
+    agents in the 200 land cells around the seed cell."""
+    from qhg4_amd import scenarios
+    sc = scenarios.c2(seed=seed, patch_cells=200)
+    p = sc["params"]
+    p.verh_b0, p.verh_d0, p.verh_theta = C2_STATED_VERHULST
+    wall, s0, s1, _ = gpu_run(sc, 0, 2000, timing=False)
+    live = s1["agent_steps"] - s0["agent_steps"]
+    print("bench: C2 stated patch %.3f s, %d agent-steps, %d alive at the end" % (wall, live, s1["num_alive"]),
+          file=sys.stderr, flush=True)
+    return {"workload": "C2 from 100,000 agents in a 200-cell patch, VerhulstVarK b0 %g d0 %g theta %g"
+                        % C2_STATED_VERHULST,
+            "steps": 2000, "value": round(live / wall, 1), "unit": "agent-steps/s",
+            "ms_per_step": round(wall * 1e3 / 2000, 5), "agent_steps": int(live),
+            "final_live_agents": int(s1["num_alive"])}
 
 
 def copy_calibration(nbytes):

@@ -57,3 +57,28 @@ def test_served_results_equal_alone_with_tuning_and_graphs():
     by = {r["id"]: r for r in res}
     for q in reqs:
         assert by[q["id"]]["summary"] == alone(q), q["id"]
+
+
+def test_an_invalid_request_is_evicted_alone():
+    """A request whose upload the library refuses (an agent in a cell the
+    grid lacks) is reported as failed; every other request of the mix ends
+    exactly as alone."""
+    print("case: 10 requests, one refused at upload", flush=True)
+    reqs = small_requests(10, 7, horizon=(20, 60))
+    bad = reqs[4]["id"]
+
+    def make_with_bad(req):
+        sc = make(req)
+        if req["id"] == bad:
+            cells = sc["agents"]["cell"].copy()
+            cells[0] = sc["grid"]["num_cells"] + 5
+            sc["agents"]["cell"] = cells
+        return sc
+    srv = ScenarioServer(make_with_bad, budget_bytes=8e9, max_members=8, chunk=8, workers=2, tune_every_s=None)
+    res, rep = srv.serve(reqs)
+    assert len(res) == len(reqs) and rep["failed"] == 1
+    by = {r["id"]: r for r in res}
+    assert by[bad]["summary"] is None and "cell index out of range" in by[bad]["error"]
+    for q in reqs:
+        if q["id"] != bad:
+            assert by[q["id"]]["summary"] == alone(q), q["id"]

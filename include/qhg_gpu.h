@@ -398,6 +398,26 @@ int qhg_batch_set_tuning(qhg_batch *batch, const qhg_tuning *t);
 int qhg_batch_get_tuning(qhg_batch *batch, qhg_tuning *t);
 int qhg_batch_autotune(qhg_batch *batch, int32_t trial_steps, qhg_tuning *chosen, int32_t *steps_taken);
 
+/* Capacities (NPPCapacity) and evaluator weights (MultiEvaluator) of a grid
+ * from its environment arrays, as qhg_gpu_set_env recomputes them, with no
+ * population and no allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Every pointer in
+ * grid and env is DEVICE memory (neighbors needed with MultiEvaluator);
+ * K_out (num_cells doubles) is written when params schedule NPPCapacity and
+ * read as the capacity input of evaluators otherwise; W_out (num_cells *
+ * (connectivity + 1) doubles) is written when they schedule MultiEvaluator.
+ * The kernels' per-cell scratch is a fixed device array of
+ * QHG_NOALLOC_MAX_CELLS doubles (8 MiB held by every GPU the library runs
+ * on until the process ends); larger grids are refused with
+ * QHG_ERR_CAPACITY.  The work is enqueued on `stream` (a hipStream_t, NULL
+ * for the null stream) and the call returns without waiting.  Calls that
+ * overlap on one GPU share its scratch array and race: serialise them (one
+ * stream, or an event between them). */
+#define QHG_NOALLOC_MAX_CELLS (1 << 20)
+int qhg_env_tables_noalloc(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg_params *params,
+                           double *K_out, double *W_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,7 +27,7 @@ EXPORTS = [
     "qhg_batch_create", "qhg_batch_destroy", "qhg_batch_add", "qhg_batch_remove", "qhg_batch_size",
     "qhg_batch_run", "qhg_batch_sync", "qhg_batch_set_timing", "qhg_batch_kernel_times",
     "qhg_batch_kernel_name", "qhg_batch_set_graph", "qhg_batch_set_tuning", "qhg_batch_get_tuning",
-    "qhg_batch_autotune",
+    "qhg_batch_autotune", "qhg_env_tables_noalloc",
 ]
 
 
@@ -105,6 +105,8 @@ def lib():
         "qhg_batch_set_tuning": ([h, P(abi.Tuning)], C.c_int),
         "qhg_batch_get_tuning": ([h, P(abi.Tuning)], C.c_int),
         "qhg_batch_autotune": ([h, C.c_int32, P(abi.Tuning), P(C.c_int32)], C.c_int),
+        "qhg_env_tables_noalloc": ([P(abi.GridDesc), P(abi.EnvDesc), P(abi.Params), C.c_void_p, C.c_void_p,
+                                    C.c_void_p], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

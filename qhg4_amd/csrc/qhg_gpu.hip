@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -2076,3 +2077,108 @@ int qhg_batch_kernel_times(qhg_batch *b, double *avg_ms, int64_t *launches, int3
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Environment tables without allocation (qhg_env_tables_noalloc)
+// ---------------------------------------------------------------------------
+// The evaluator kernels' per-cell scratch (one double per cell, the values
+// of the evaluator being combined) lives in this fixed device array instead
+// of a population's buffers.  Every device the library runs on holds its own
+// copy until the process ends.  Host code never passes the array's name to a
+// kernel (on the host it is only a stand-in symbol): the address of each
+// device's copy is looked up once per device and kept in g_scratch_at.
+static __device__ __align__(256) double scratch[QHG_NOALLOC_MAX_CELLS];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_at[kMaxDevices] = {};
+
+// the address of the current device's copy of `scratch` (looked up on the
+// first call on that device, with the device current)
+int scratch_address(double **out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(QHG_ERR_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_at[dev]) {
+        void *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(QHG_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_at[dev] = static_cast<double *>(ptr);
+    }
+    *out = g_scratch_at[dev];
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int qhg_env_tables_noalloc(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg_params *params,
+                                      double *K_out, double *W_out, void *stream) {
+    if (!grid || !env || !params) return fail(QHG_ERR_ARG, "null argument");
+    if (grid->num_cells <= 0 || grid->connectivity < 1 || grid->connectivity > 6)
+        return fail(QHG_ERR_ARG, "grid: num_cells must be positive and connectivity in 1..6");
+    if ((int64_t)grid->num_cells > (int64_t)QHG_NOALLOC_MAX_CELLS)
+        return fail(QHG_ERR_CAPACITY, "grid of " + std::to_string(grid->num_cells) + " cells exceeds the " +
+                                          std::to_string(QHG_NOALLOC_MAX_CELLS) +
+                                          "-cell scratch of qhg_env_tables_noalloc");
+    bool has_cap = false, has_me = false;
+    for (int i = 0; i < params->num_actions && i < QHG_MAX_ACTIONS; i++) {
+        has_cap |= params->action_kind[i] == QHG_ACT_NPPCAP;
+        has_me |= params->action_kind[i] == QHG_ACT_MULTIEVAL;
+    }
+    if (has_cap && !K_out) return fail(QHG_ERR_ARG, "NPPCapacity is scheduled: K_out is required");
+    if (has_me && !W_out) return fail(QHG_ERR_ARG, "MultiEvaluator is scheduled: W_out is required");
+    if (has_me && !grid->neighbors) return fail(QHG_ERR_ARG, "MultiEvaluator needs the grid's neighbours");
+    if (has_me && (params->me_num < 1 || params->me_num > QHG_MAX_EVALS))
+        return fail(QHG_ERR_ARG, "MultiEvaluator needs 1..4 evaluators");
+    double *tmp = nullptr;
+    int rc = scratch_address(&tmp);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int nb = grid_for(grid->num_cells, 4096);
+    Ctx x{};
+    x.ncells = grid->num_cells;
+    x.conn = grid->connectivity;
+    x.nbr = grid->neighbors;
+    x.K = K_out;
+    x.W = W_out;
+    x.ice = env->ice;
+    if (has_cap) {
+        const qhg_params &P = *params;
+        EnvPtrs e{env->altitude, env->longitude, env->latitude, env->water, env->base_npp, env->temperature,
+                  env->rainfall, env->ice, env->coastal};
+        CapParams c{P.npp_water_factor, P.npp_coastal_factor, P.npp_coastal_min_lat, P.npp_coastal_max_lat,
+                    P.npp_min, P.npp_max, P.npp_k_min, P.npp_k_max, P.npp_efficiency};
+        hipLaunchKernelGGL(k_capacity, dim3(nb), dim3(kBlock), 0, st, x, e, c);
+        HIPCHK(hipGetLastError());
+    }
+    if (has_me) {
+        const qhg_params &P = *params;
+        for (int ev = 0; ev < P.me_num; ev++) {
+            const double *in = nullptr;
+            switch (P.me_input[ev]) {
+            case QHG_EV_ALTITUDE: in = env->altitude; break;
+            case QHG_EV_CAPACITY: in = K_out; break;
+            case QHG_EV_BASE_NPP: in = env->base_npp; break;
+            }
+            if (!in) return fail(QHG_ERR_ARG, "evaluator input array missing");
+            if (P.me_pl_n[ev] < 0 || P.me_pl_n[ev] > QHG_MAX_PL_POINTS)
+                return fail(QHG_ERR_ARG, "PolyLine point count out of range");
+            EvalParams ep{};
+            ep.mode = P.me_mode;
+            ep.cumulate = P.me_cumulate[ev];
+            ep.npts = P.me_pl_n[ev];
+            ep.first = (ev == 0);
+            ep.last = (ev == P.me_num - 1);
+            ep.weight = P.me_weight[ev];
+            for (int j = 0; j < ep.npts; j++) { ep.px[j] = P.me_pl_x[ev][j]; ep.pv[j] = P.me_pl_v[ev][j]; }
+            hipLaunchKernelGGL(k_eval_values, dim3(nb), dim3(kBlock), 0, st, x, in, tmp, ep);
+            HIPCHK(hipGetLastError());
+            hipLaunchKernelGGL(k_eval_combine, dim3(nb), dim3(kBlock), 0, st, x, (const double *)tmp, ep);
+            HIPCHK(hipGetLastError());
+        }
+    }
+    return 0;
+}

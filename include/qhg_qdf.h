@@ -19,12 +19,15 @@
  *                              PrioInfo {FuncName[128], FuncPrio} per action,
  *                              every action attribute by its reference name
  *                              (doubles; PolyLines as {X, V, slopes}),
- *                              MaxAgentID, PhiloxSeed, StepCount, NextTime and
- *                              QHGGpuParams (this engine's parameter block)
+ *                              IDGenState (the last id handed out), and this
+ *                              engine's PhiloxSeed, StepCount, NextTime,
+ *                              MultiEvaluatorMode
  *     AgentDataSet             {LifeState, CellIdx, CellID, AgentID, BirthTime,
  *                              Gender, Age, LastBirth, MateID, NumBabies}
  *                              (times FP64)
  *     Genome                   num_agents x genome_words uint64 [Genetics]
+ *     DeadSpaces               int32 slots awaiting reuse (empty: the device
+ *                              buffers hold no holes)
  *
  * Every function returns 0 or a negative QHG_ERR_* code; qhg_qdf_last_error()
  * gives the message.
@@ -46,7 +49,7 @@ typedef struct qhg_qdf_info {
     double   time;            /* its time */
     int64_t  max_id;          /* highest agent id so far */
     uint64_t seed;
-    int32_t  has_params;      /* QHGGpuParams present */
+    int32_t  has_params;      /* ClassName and PrioInfo present: parameters can be rebuilt */
     int32_t  pad;
     char     class_name[64];
     char     species_name[64];
@@ -66,9 +69,9 @@ int qhg_qdf_write_population(qhg_pop *pop, const char *path, int32_t create, con
                              const char *pop_class, int32_t num_cells, const int64_t *cell_ids);
 
 int qhg_qdf_read_info(const char *path, const char *species, qhg_qdf_info *info);
-/* The parameter block: QHGGpuParams when present, else rebuilt from the
- * class name, PrioInfo and the action attributes (a file the reference
- * wrote). */
+/* The parameter block, rebuilt from the class name, PrioInfo and the action
+ * attributes (files this library or the reference wrote); the key of the
+ * draws from PhiloxSeed, or `seed` when the file has none. */
 int qhg_qdf_read_params(const char *path, const char *species, uint64_t seed, qhg_params *out);
 /* Agents into caller arrays of info.num_agents entries (genome:
  * num_agents * genome_words); any array may be NULL except id. */

@@ -1554,7 +1554,13 @@ __device__ __forceinline__ void tile_write(const CtxT &x, const StepArgs &sa, ui
 // One look-back tile of k_cell_scan (kCellTile cells), all kScanBlock threads.
 template <typename CtxT>
 __device__ __forceinline__ void scan_tile(const CtxT &x, const StepArgs &sa, uint32_t tile, ScanLds &L) {
-    if (tile_idle(x, tile)) {
+    // the activity flag and the tile's counts in one round trip (an idle
+    // tile's counts are zero, so reading them clears nothing)
+    const bool may_idle = (int64_t)tile * kCellTile + kCellTile < x.ncells;
+    const uint32_t act = may_idle ? x.tact[tile] : 1u;
+    TileIn I;
+    tile_load(x, sa, tile, true, I);
+    if (act == 0u) {
         // no cell of the tile holds agents next step: its sums are zero and
         // none of its per-cell outputs is read (the step totals come from
         // the last tile, which always runs)
@@ -1565,8 +1571,6 @@ __device__ __forceinline__ void scan_tile(const CtxT &x, const StepArgs &sa, uin
         }
         return;
     }
-    TileIn I;
-    tile_load(x, sa, tile, true, I);
     const TileSums S = tile_sums(I, L);
     if (threadIdx.x < 64) {
         uint64_t ex, ex2;
@@ -1581,6 +1585,16 @@ __device__ __forceinline__ void scan_tile(const CtxT &x, const StepArgs &sa, uin
 __global__ void __launch_bounds__(kScanBlock) k_cell_scan(Ctx x, StepArgs sa) {
     __shared__ ScanLds lds;
     const int lane = threadIdx.x & 63;
+    // this thread's slice of the k_cell_act partials (deaths, moves, FERTILE
+    // counts), loaded with the status so that it adds no round trip; its
+    // wave's sums are stored at the end for k_cell_scatter's bookkeeping
+    // block (plain stores: no contended atomics)
+    uint64_t D = 0, Mv = 0, Fe = 0;
+    for (int32_t j = blockIdx.x * kScanBlock + threadIdx.x; j < x.nact_blocks; j += gridDim.x * kScanBlock) {
+        D += x.wave_part[kParts * j + 0];
+        Mv += x.wave_part[kParts * j + 1];
+        Fe += x.wave_part[kParts * j + 2];
+    }
     if (x.st->error) return;
     // when the grid has a block per tile, the first round of tickets hands
     // out every tile: no block asks again (that round trip would only end
@@ -1589,25 +1603,14 @@ __global__ void __launch_bounds__(kScanBlock) k_cell_scan(Ctx x, StepArgs sa) {
     const bool one_round = gridDim.x >= nt;
     for (uint32_t tile = next_tile(x.tick, lds); tile < nt; tile = one_round ? nt : next_tile(x.tick, lds))
         scan_tile(x, sa, tile, lds);
-    {
-        // off the critical path: this wave's slice of the k_cell_act
-        // partials (deaths, moves, FERTILE counts), stored for k_cell_scatter's
-        // bookkeeping block (plain stores: no contended atomics)
-        uint64_t D = 0, Mv = 0, Fe = 0;
-        for (int32_t j = blockIdx.x * kScanBlock + threadIdx.x; j < x.nact_blocks; j += gridDim.x * kScanBlock) {
-            D += x.wave_part[kParts * j + 0];
-            Mv += x.wave_part[kParts * j + 1];
-            Fe += x.wave_part[kParts * j + 2];
-        }
-        D = wave_sum_u64(D);
-        Mv = wave_sum_u64(Mv);
-        Fe = wave_sum_u64(Fe);
-        if (lane == 0) {
-            uint64_t *sp = x.scan_part + 3 * ((int64_t)blockIdx.x * (kScanBlock / kWave) + (threadIdx.x / kWave));
-            sp[0] = D;
-            sp[1] = Mv;
-            sp[2] = Fe;
-        }
+    D = wave_sum_u64(D);
+    Mv = wave_sum_u64(Mv);
+    Fe = wave_sum_u64(Fe);
+    if (lane == 0) {
+        uint64_t *sp = x.scan_part + 3 * ((int64_t)blockIdx.x * (kScanBlock / kWave) + (threadIdx.x / kWave));
+        sp[0] = D;
+        sp[1] = Mv;
+        sp[2] = Fe;
     }
 }
 

@@ -166,14 +166,27 @@ int write_species_attrs(hid_t g, const char *species, const char *pop_class, con
         rc = put_num(g, "PrioInfo", ct, pr.data(), (hsize_t)P->num_actions);
         if (rc) return rc;
     }
-    // the action attributes by their reference names (writeActionParamsQDF)
+    // the action attributes by their reference names (writeActionParamsQDF),
+    // for every action kind, scheduled or not: a class builds all of its
+    // actions and each reads its attributes (Prioritizer::getActionAttributes,
+    // QHG4/core/Prioritizer.cpp:440-462); a reader ignores the names its
+    // class does not own
     for (int kind = QHG_ACT_GETOLD; kind <= QHG_ACT_LAST; kind++) {
-        if (!has_kind(P, kind)) continue;
         for (int i = 0;; i++) {
             const char *n = qhg_params_attribute_name(kind, i);
             if (!n) break;
             double v = 0;
-            if (qhg_params_get_attribute(P, n, &v) != 0) continue;   // non-numeric attribute
+            if (qhg_params_get_attribute(P, n, &v) != 0) {
+                // attributes the reference reads that steer only how genomes
+                // are made at the start (Genetics.cpp): in a dump the genomes
+                // are in the file, 2 bits per nucleotide
+                if (std::strcmp(n, "Genetics_initial_muts") == 0 || std::strcmp(n, "Genetics_create_new_genome") == 0)
+                    v = 0;
+                else if (std::strcmp(n, "Genetics_bits_per_nuc") == 0)
+                    v = 2;
+                else
+                    continue;   // non-numeric attribute
+            }
             rc = put_num(g, n, H5T_NATIVE_DOUBLE, &v, 1);
             if (rc) return rc;
         }
@@ -187,12 +200,27 @@ int write_species_attrs(hid_t g, const char *species, const char *pop_class, con
                 return rc;
         }
     }
-    // this engine's state: clock, id state, seed and the parameter block
-    if (!rc) rc = put_num(g, "MaxAgentID", H5T_NATIVE_INT64, &max_id, 1);
+    // the id generator's state under the reference's name (the last id
+    // handed out; the reference writes one entry per thread,
+    // SPopulation::dumpSpeciesDataQDF, QHG4/core/SPopulation.cpp:2184-2191)
+    if (!rc) rc = put_num(g, "IDGenState", H5T_NATIVE_INT64, &max_id, 1);
+    // this engine's own state: the draws' key and the clock, and the
+    // MultiEvaluator's combination mode (the reference classes fix it in
+    // their constructors, e.g. MODE_ADD_SIMPLE at QHG4/populations/OoAPop.cpp:56;
+    // this engine takes it from the parameters)
+    if (!rc && has_kind(P, QHG_ACT_MULTIEVAL)) rc = put_num(g, "MultiEvaluatorMode", H5T_NATIVE_INT32, &P->me_mode, 1);
     if (!rc) rc = put_num(g, "PhiloxSeed", H5T_NATIVE_UINT64, &P->seed, 1);
     if (!rc) rc = put_num(g, "StepCount", H5T_NATIVE_INT64, &step, 1);
     if (!rc) rc = put_num(g, "NextTime", H5T_NATIVE_DOUBLE, &time, 1);
-    if (!rc) rc = put_num(g, "QHGGpuParams", H5T_NATIVE_UINT8, P, sizeof(qhg_params));
+    // DeadSpaces (SPopulation::dumpDeadSpaces, SPopulation.cpp:2456-2505):
+    // the slots the previous step's deaths left for recycling.  The device
+    // buffers hold no holes, so the list is empty.
+    if (!rc) {
+        const hsize_t zero = 0;
+        Hid sp(H5Screate_simple(1, &zero, nullptr), H5Sclose);
+        Hid d(H5Dcreate2(g, "DeadSpaces", H5T_NATIVE_INT32, sp, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Dclose);
+        if (!d.ok()) rc = qfail(QHG_ERR_ARG, "cannot create DeadSpaces");
+    }
     return rc;
 }
 
@@ -320,10 +348,23 @@ int read_genome(hid_t g, uint64_t *genome, int64_t n, int32_t words) {
     return 0;
 }
 
-// the parameter block of a species group: QHGGpuParams when present, else
-// rebuilt from ClassName, PrioInfo and the action attributes
+// every entry of a 1-D integer attribute (e.g. IDGenState: one per thread
+// of the run that wrote it)
+bool get_i64s(hid_t loc, const char *name, std::vector<int64_t> &out) {
+    if (H5Aexists(loc, name) <= 0) return false;
+    Hid a(H5Aopen(loc, name, H5P_DEFAULT), H5Aclose);
+    Hid sp(H5Aget_space(a), H5Sclose);
+    const hssize_t n = H5Sget_simple_extent_npoints(sp);
+    if (n <= 0) return false;
+    out.resize((size_t)n);
+    return H5Aread(a, H5T_NATIVE_INT64, out.data()) >= 0;
+}
+
+// the parameter block of a species group, rebuilt from ClassName, PrioInfo
+// and the action attributes; the draws' key from PhiloxSeed, or `seed` when
+// the group has none (a file the reference wrote)
 int params_from_group(hid_t g, uint64_t seed, qhg_params *out) {
-    if (get_num(g, "QHGGpuParams", H5T_NATIVE_UINT8, out, sizeof(qhg_params))) return 0;
+    get_num(g, "PhiloxSeed", H5T_NATIVE_UINT64, &seed, 1);
     std::string cls;
     if (!get_str(g, "ClassName", cls)) return qfail(QHG_ERR_ARG, "no ClassName attribute");
     std::vector<std::string> pn;
@@ -356,7 +397,15 @@ int params_from_group(hid_t g, uint64_t seed, qhg_params *out) {
     for (auto &str : pn) p.push_back(str.c_str());
     int rc = qhg_params_from_attribute_map(cls.c_str(), (int32_t)k.size(), k.data(), v.data(), p.data(), pv.data(),
                                            (int32_t)p.size(), seed, out);
-    return rc ? qfail(rc, qhg_gpu_last_error()) : 0;
+    if (rc) return qfail(rc, qhg_gpu_last_error());
+    int32_t mode = 0;
+    if (get_num(g, "MultiEvaluatorMode", H5T_NATIVE_INT32, &mode, 1)) {
+        if (mode != QHG_ME_ADD_SIMPLE && mode != QHG_ME_MUL_SIMPLE && mode != QHG_ME_MAX_SIMPLE &&
+            mode != QHG_ME_MIN_SIMPLE)
+            return qfail(QHG_ERR_ARG, "unknown MultiEvaluatorMode");
+        out->me_mode = mode;
+    }
+    return 0;
 }
 
 // what a species group says of itself; root (or -1) supplies the file's
@@ -389,8 +438,13 @@ void info_from_group(hid_t g, hid_t root, qhg_qdf_info *info) {
     if (!get_num(g, "NextTime", H5T_NATIVE_DOUBLE, &I.time, 1) && root >= 0 && get_str(root, "StartTime", s))
         I.time = std::strtod(s.c_str(), nullptr);
     get_num(g, "PhiloxSeed", H5T_NATIVE_UINT64, &I.seed, 1);
-    I.has_params = H5Aexists(g, "QHGGpuParams") > 0;
-    if (!get_num(g, "MaxAgentID", H5T_NATIVE_INT64, &I.max_id, 1) && I.num_agents > 0) {
+    I.has_params = H5Aexists(g, "ClassName") > 0 && H5Aexists(g, "PrioInfo") > 0;
+    // the highest id handed out: the largest IDGenState entry, else the
+    // largest agent id in the file
+    std::vector<int64_t> idg;
+    if (get_i64s(g, "IDGenState", idg)) {
+        I.max_id = *std::max_element(idg.begin(), idg.end());
+    } else if (I.num_agents > 0) {
         std::vector<int64_t> ids((size_t)I.num_agents);
         Hid t(H5Tcreate(H5T_COMPOUND, sizeof(int64_t)), H5Tclose);
         H5Tinsert(t, "AgentID", 0, H5T_NATIVE_INT64);

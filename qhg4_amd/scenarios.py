@@ -109,6 +109,13 @@ def ooa_params(seed, genetics=False, wmove_prob=0.2, b0=0.8, d0=0.001, theta=0.1
     pairs with AssortativePairHamming instead of RandomPair, as
     OoANavGen2bitAssHamPop does (needs genetics)."""
     pair = abi.ACT_RANDOMPAIR if assort is None else abi.ACT_ASSORTPAIR
+    # the genetics classes (OoANavGen2bitPop, OoANavGen2bitAssHamPop) read
+    # their NPP evaluator through a PolyLine (NPPPref; a SingleEvaluator with
+    # a PolyLine name needs at least two points, PolyLine.cpp:92-127): a ramp
+    # through (0, 0) and (K_max, K_max) passes the capacity on, as a float;
+    # OoAPop's NPP evaluator reads the capacity directly
+    k_max = 1.5 * k_scale
+    npp_pl = [(0.0, 0.0), (k_max, k_max)] if genetics else []
     actions = [(abi.ACT_NPPCAP, 1), (abi.ACT_GETOLD, 2), (abi.ACT_WEIGHTEDMOVE, 4),
                (abi.ACT_MULTIEVAL, 5, "MultiEvaluator[Alt+NPP]"), (abi.ACT_FERTILITY, 6),
                (pair, 7), (abi.ACT_VERHULSTVARK, 8)]
@@ -125,8 +132,8 @@ def ooa_params(seed, genetics=False, wmove_prob=0.2, b0=0.8, d0=0.001, theta=0.1
         wmove_prob=wmove_prob,
         me_mode=abi.ME_MUL_SIMPLE, me_num=2,
         me_input=[abi.EV_ALTITUDE, abi.EV_CAPACITY], me_cumulate=[0, 0], me_weight=[1.0, 1.0],
-        me_pl=[sigmoid_polyline(), []],
-        npp_min=0.0, npp_max=1.5, npp_k_min=0.0, npp_k_max=1.5 * k_scale, npp_efficiency=1.0,
+        me_pl=[sigmoid_polyline(), npp_pl],
+        npp_min=0.0, npp_max=1.5, npp_k_min=0.0, npp_k_max=k_max, npp_efficiency=1.0,
         **kw)
 
 

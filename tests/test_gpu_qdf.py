@@ -5,6 +5,7 @@ import pytest
 
 from qhg4_amd import qdf, scenarios
 from qhg4_amd.population import GpuPopulation
+from oracle.oracle import Oracle
 from tests.helpers import assert_same_state
 
 pytestmark = pytest.mark.gpu
@@ -35,7 +36,8 @@ def test_dump_and_resume(tmp_path, case, k, m):
     a = GpuPopulation.from_scenario(sc)
     a.preLoop()
     a.run(k)
-    qdf.write_population(a, f, pop_class="OoANavGen2bitAssHamPop" if case == "assort" else "OoAPop")
+    cls = {"c2": "OoAPop", "c3": "OoANavGen2bitPop", "assort": "OoANavGen2bitAssHamPop"}[case]
+    qdf.write_population(a, f, pop_class=cls)
     a.close()
     info = qdf.read_info(f)
     assert info["step"] == k and info["time"] == float(k)
@@ -45,7 +47,14 @@ def test_dump_and_resume(tmp_path, case, k, m):
     b.preLoop()
     qdf.restore_population(b, f)
     b.run(m)
-    assert_same_state(b.agents(), ref_a, "resumed %s" % case)
+    got = b.agents()
+    assert_same_state(got, ref_a, "resumed %s" % case)
     s = b.stats()
     assert (s["num_alive"], s["max_id"], s["step"]) == (ref_s["num_alive"], ref_s["max_id"], ref_s["step"])
     b.close()
+    # the resumed run equals the oracle's uninterrupted run
+    ora = Oracle(sc)
+    ora.pre_loop()
+    ora.run(k + m)
+    assert_same_state(got, ora.agents(), "resumed %s vs the oracle" % case)
+    assert ora.stats()["max_id"] == s["max_id"]

@@ -75,8 +75,13 @@ def test_reference_names_in_the_file(tmp_path):
     for member in ("LifeState", "CellIdx", "CellID", "AgentID", "BirthTime", "Gender", "Age", "LastBirth"):
         assert '"%s"' % member in out, member
     for attr in ("ClassName", "SpeciesName", "NumCells", "PrioInfo", "FuncName", "FuncPrio", "Verhulst_b0",
-                 "WeightedMove_prob", "NPPCap_K_max", "OAD_max_age", "Multi_weight_alt", "AltCapPref"):
+                 "WeightedMove_prob", "NPPCap_K_max", "OAD_max_age", "Multi_weight_alt", "AltCapPref",
+                 "IDGenState"):
         assert attr in out, attr
+    # the dump extras under the reference's names (SPopulation.cpp:2184-2195), and no raw
+    # parameter struct: the parameters live in the attributes above only
+    assert 'DATASET "DeadSpaces"' in out
+    assert "QHGGpuParams" not in out and "MaxAgentID" not in out
     assert 'GROUP "Populations"' in out and 'GROUP "sapiens"' in out and 'DATASET "AgentDataSet"' in out
     assert 'ATTRIBUTE "QHG"' in out and 'ATTRIBUTE "Step"' in out and 'ATTRIBUTE "StartTime"' in out
 

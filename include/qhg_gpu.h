@@ -352,6 +352,14 @@ int qhg_params_from_attribute_map(const char *pop_class, int32_t n, const char *
 const char *qhg_params_attribute_name(int32_t kind, int32_t index);
 const char *qhg_params_weight_name(const char *pop_class, int32_t eval);
 const char *qhg_params_polyline_name(const char *pop_class, int32_t eval);
+/* The reference's names of a class's actions, as its Prioritizer keys them
+ * (Action::getActionName, QHG4/actions/Action.cpp:11-19, e.g. "Verhulst",
+ * "MultiEvaluator[Alt+NPP]"): the index-th (NULL past the last), its kind in
+ * *kind (0 for an action with no device implementation, e.g. Navigate); and
+ * of the MultiEvaluator's evaluators ("SingleEvaluator[Alt]").  The QDF
+ * writer names the per-action groups of a species group after them. */
+const char *qhg_params_class_action(const char *pop_class, int32_t index, int32_t *kind);
+const char *qhg_params_evaluator_name(const char *pop_class, int32_t eval);
 /* A numeric attribute by its reference name (e.g. "Verhulst_b0"). */
 int qhg_params_get_attribute(const qhg_params *p, const char *name, double *value);
 int qhg_params_set_attribute(qhg_params *p, const char *name, double value);

@@ -22,6 +22,16 @@
  *                              IDGenState (the last id handed out), and this
  *                              engine's PhiloxSeed, StepCount, NextTime,
  *                              MultiEvaluatorMode
+ *     <action name>            one group per action of the class, named as its
+ *                              Prioritizer keys it ("Verhulst",
+ *                              "MultiEvaluator[Alt+NPP]"), holding that
+ *                              action's attributes, as the reference's
+ *                              Prioritizer::writeActionParamsQDF writes them
+ *                              and its readers look for them
+ *                              (QHG4/core/Prioritizer.cpp:217-246); a
+ *                              MultiEvaluator group holds its weights and
+ *                              one "SingleEvaluator[<name>]" group per
+ *                              evaluator with its PolyLine
  *     AgentDataSet             {LifeState, CellIdx, CellID, AgentID, BirthTime,
  *                              Gender, Age, LastBirth, MateID, NumBabies}
  *                              (times FP64)

@@ -24,6 +24,7 @@ EXPORTS = [
     "qhg_params_from_modules", "qhg_params_get_attribute", "qhg_params_set_attribute", "qhg_gpu_modify_attribute",
     "qhg_gpu_set_clock", "qhg_gpu_get_clock", "qhg_gpu_get_params", "qhg_params_from_attribute_map",
     "qhg_params_attribute_name", "qhg_params_weight_name", "qhg_params_polyline_name", "qhg_params_event_flags",
+    "qhg_params_class_action", "qhg_params_evaluator_name",
     "qhg_batch_create", "qhg_batch_destroy", "qhg_batch_add", "qhg_batch_remove", "qhg_batch_size",
     "qhg_batch_run", "qhg_batch_sync", "qhg_batch_set_timing", "qhg_batch_kernel_times",
     "qhg_batch_kernel_name", "qhg_batch_set_graph", "qhg_batch_set_tuning", "qhg_batch_get_tuning",
@@ -91,6 +92,8 @@ def lib():
         "qhg_params_attribute_name": ([C.c_int32, C.c_int32], C.c_char_p),
         "qhg_params_weight_name": ([C.c_char_p, C.c_int32], C.c_char_p),
         "qhg_params_polyline_name": ([C.c_char_p, C.c_int32], C.c_char_p),
+        "qhg_params_class_action": ([C.c_char_p, C.c_int32, P(C.c_int32)], C.c_char_p),
+        "qhg_params_evaluator_name": ([C.c_char_p, C.c_int32], C.c_char_p),
         "qhg_batch_create": ([P(h)], C.c_int),
         "qhg_batch_destroy": ([h], C.c_int),
         "qhg_batch_add": ([h, h], C.c_int),

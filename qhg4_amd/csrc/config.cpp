@@ -222,6 +222,24 @@ std::string action_name(const ClassSpec &c, int kind) {
     return base_name(kind);
 }
 
+// The reference's names of a class's actions (Action::getActionName,
+// QHG4/actions/Action.cpp:11-19, with the ids the class constructors give
+// them), built once; then its evaluators' names.
+const std::vector<std::string> &class_names(const ClassSpec &c, bool evaluators) {
+    static const auto *tab = [] {
+        auto *t = new std::map<std::string, std::pair<std::vector<std::string>, std::vector<std::string>>>();
+        for (const ClassSpec &k : classes()) {
+            auto &e = (*t)[k.name];
+            for (int kind : k.kinds) e.first.push_back(action_name(k, kind));
+            for (const char *o : k.other) e.first.push_back(o);
+            for (const EvalSpec &ev : k.evals) e.second.push_back(std::string("SingleEvaluator[") + ev.id + "]");
+        }
+        return t;
+    }();
+    const auto &e = tab->at(c.name);
+    return evaluators ? e.second : e.first;
+}
+
 }  // namespace
 
 extern "C" {
@@ -355,6 +373,26 @@ const char *qhg_params_polyline_name(const char *pop_class, int32_t eval) {
         if (pop_class && std::strcmp(c.name, pop_class) == 0)
             return (eval >= 0 && eval < (int32_t)c.evals.size() && *c.evals[eval].pl_attr) ? c.evals[eval].pl_attr
                                                                                               : nullptr;
+    return nullptr;
+}
+
+const char *qhg_params_class_action(const char *pop_class, int32_t index, int32_t *kind) {
+    for (const ClassSpec &c : classes()) {
+        if (!pop_class || std::strcmp(c.name, pop_class) != 0) continue;
+        const std::vector<std::string> &names = class_names(c, false);
+        if (index < 0 || index >= (int32_t)names.size()) return nullptr;
+        if (kind) *kind = (index < (int32_t)c.kinds.size()) ? c.kinds[index] : 0;
+        return names[index].c_str();
+    }
+    return nullptr;
+}
+
+const char *qhg_params_evaluator_name(const char *pop_class, int32_t eval) {
+    for (const ClassSpec &c : classes()) {
+        if (!pop_class || std::strcmp(c.name, pop_class) != 0) continue;
+        const std::vector<std::string> &names = class_names(c, true);
+        return (eval >= 0 && eval < (int32_t)names.size()) ? names[eval].c_str() : nullptr;
+    }
     return nullptr;
 }
 

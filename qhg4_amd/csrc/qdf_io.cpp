@@ -144,6 +144,60 @@ bool has_kind(const qhg_params *p, int kind) {
     return false;
 }
 
+// the numeric attributes of one action kind, by their reference names
+int put_kind_attrs(hid_t loc, const qhg_params *P, int kind) {
+    for (int i = 0;; i++) {
+        const char *n = qhg_params_attribute_name(kind, i);
+        if (!n) return 0;
+        double v = 0;
+        if (qhg_params_get_attribute(P, n, &v) != 0) {
+            // attributes the reference reads that steer only how genomes
+            // are made at the start (Genetics.cpp): in a dump the genomes
+            // are in the file, 2 bits per nucleotide
+            if (std::strcmp(n, "Genetics_initial_muts") == 0 || std::strcmp(n, "Genetics_create_new_genome") == 0)
+                v = 0;
+            else if (std::strcmp(n, "Genetics_bits_per_nuc") == 0)
+                v = 2;
+            else
+                continue;   // non-numeric attribute
+        }
+        int rc = put_num(loc, n, H5T_NATIVE_DOUBLE, &v, 1);
+        if (rc) return rc;
+    }
+}
+
+// The reference's current layout (Prioritizer::writeActionParamsQDF,
+// QHG4/core/Prioritizer.cpp:340-375): one group per action of the class,
+// named as its Prioritizer names it, holding the action's attributes; the
+// MultiEvaluator's group holds its weights and one group per evaluator with
+// the evaluator's PolyLine (MultiEvaluator::writeAttributesQDF,
+// QHG4/actions/MultiEvaluator.cpp:515-540).  Actions with no device
+// implementation (Navigate) get no group.
+int put_action_groups(hid_t g, const char *pop_class, const qhg_params *P) {
+    for (int i = 0;; i++) {
+        int32_t kind = 0;
+        const char *an = qhg_params_class_action(pop_class, i, &kind);
+        if (!an) return 0;
+        if (kind == 0) continue;
+        Hid ag(H5Gcreate2(g, an, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Gclose);
+        if (!ag.ok()) return qfail(QHG_ERR_ARG, std::string("cannot create action group ") + an);
+        int rc = put_kind_attrs(ag, P, kind);
+        if (rc) return rc;
+        if (kind != QHG_ACT_MULTIEVAL) continue;
+        for (int e = 0; e < P->me_num; e++) {
+            const char *w = qhg_params_weight_name(pop_class, e);
+            if (w && (rc = put_num(ag, w, H5T_NATIVE_DOUBLE, &P->me_weight[e], 1))) return rc;
+            const char *en = qhg_params_evaluator_name(pop_class, e);
+            if (!en) continue;
+            Hid eg(H5Gcreate2(ag, en, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Gclose);
+            if (!eg.ok()) return qfail(QHG_ERR_ARG, std::string("cannot create evaluator group ") + en);
+            const char *pl = qhg_params_polyline_name(pop_class, e);
+            if (pl && P->me_pl_n[e] > 0 && (rc = put_polyline(eg, pl, P->me_pl_n[e], P->me_pl_x[e], P->me_pl_v[e])))
+                return rc;
+        }
+    }
+}
+
 int write_species_attrs(hid_t g, const char *species, const char *pop_class, const qhg_params *P, int32_t num_cells,
                         int64_t step, double time, int64_t max_id) {
     int rc = put_str(g, "ClassName", pop_class ? pop_class : "");
@@ -171,26 +225,8 @@ int write_species_attrs(hid_t g, const char *species, const char *pop_class, con
     // actions and each reads its attributes (Prioritizer::getActionAttributes,
     // QHG4/core/Prioritizer.cpp:440-462); a reader ignores the names its
     // class does not own
-    for (int kind = QHG_ACT_GETOLD; kind <= QHG_ACT_LAST; kind++) {
-        for (int i = 0;; i++) {
-            const char *n = qhg_params_attribute_name(kind, i);
-            if (!n) break;
-            double v = 0;
-            if (qhg_params_get_attribute(P, n, &v) != 0) {
-                // attributes the reference reads that steer only how genomes
-                // are made at the start (Genetics.cpp): in a dump the genomes
-                // are in the file, 2 bits per nucleotide
-                if (std::strcmp(n, "Genetics_initial_muts") == 0 || std::strcmp(n, "Genetics_create_new_genome") == 0)
-                    v = 0;
-                else if (std::strcmp(n, "Genetics_bits_per_nuc") == 0)
-                    v = 2;
-                else
-                    continue;   // non-numeric attribute
-            }
-            rc = put_num(g, n, H5T_NATIVE_DOUBLE, &v, 1);
-            if (rc) return rc;
-        }
-    }
+    for (int kind = QHG_ACT_GETOLD; kind <= QHG_ACT_LAST && !rc; kind++) rc = put_kind_attrs(g, P, kind);
+    if (rc) return rc;
     if (has_kind(P, QHG_ACT_MULTIEVAL) && pop_class) {
         for (int e = 0; e < P->me_num; e++) {
             const char *w = qhg_params_weight_name(pop_class, e);
@@ -200,6 +236,9 @@ int write_species_attrs(hid_t g, const char *species, const char *pop_class, con
                 return rc;
         }
     }
+    // ... and the same per action group (the layout the reference reads
+    // now; the flat attributes above serve readers of the older layout)
+    if (pop_class && *pop_class && (rc = put_action_groups(g, pop_class, P))) return rc;
     // the id generator's state under the reference's name (the last id
     // handed out; the reference writes one entry per thread,
     // SPopulation::dumpSpeciesDataQDF, QHG4/core/SPopulation.cpp:2184-2191)
@@ -348,6 +387,24 @@ int read_genome(hid_t g, uint64_t *genome, int64_t n, int32_t words) {
     return 0;
 }
 
+// the attributes of every subgroup of `loc`, down to `depth` levels
+void collect_groups(hid_t loc, std::map<std::string, std::string> &kv, int depth) {
+    if (depth <= 0) return;
+    H5G_info_t gi;
+    if (H5Gget_info(loc, &gi) < 0) return;
+    for (hsize_t k = 0; k < gi.nlinks; k++) {
+        char name[256];
+        if (H5Lget_name_by_idx(loc, ".", H5_INDEX_NAME, H5_ITER_INC, k, name, sizeof(name), H5P_DEFAULT) < 0) continue;
+        H5O_info_t oi;
+        if (H5Oget_info_by_name(loc, name, &oi, H5P_DEFAULT) < 0 || oi.type != H5O_TYPE_GROUP) continue;
+        Hid sub(H5Gopen2(loc, name, H5P_DEFAULT), H5Gclose);
+        if (!sub.ok()) continue;
+        hsize_t idx = 0;
+        H5Aiterate2(sub, H5_INDEX_NAME, H5_ITER_INC, &idx, collect_attr, &kv);
+        collect_groups(sub, kv, depth - 1);
+    }
+}
+
 // every entry of a 1-D integer attribute (e.g. IDGenState: one per thread
 // of the run that wrote it)
 bool get_i64s(hid_t loc, const char *name, std::vector<int64_t> &out) {
@@ -389,6 +446,9 @@ int params_from_group(hid_t g, uint64_t seed, qhg_params *out) {
     std::map<std::string, std::string> kv;
     hsize_t idx = 0;
     H5Aiterate2(g, H5_INDEX_NAME, H5_ITER_INC, &idx, collect_attr, &kv);
+    // per-action groups (and a MultiEvaluator's evaluator groups) of the
+    // reference's current layout
+    collect_groups(g, kv, 2);
     std::vector<const char *> k, v, p;
     for (auto &e : kv) {
         k.push_back(e.first.c_str());

@@ -21,11 +21,14 @@ REFERENCE_TREE = "/root/reference/QHG4"
 
 
 def build():
-    """Compile the oracle; where the reference sources are present, also the
-    reference utility library the pin tests compare against (oracle/_ref)."""
+    """Compile the oracle; where the reference sources are present, also
+    (into oracle/_ref) the reference utility library the pin tests compare
+    against, the reference population run by the statistical comparison
+    (ref_harness) and the reference host that loads the PopBase plugin
+    (plugin_host)."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
     if os.path.isdir(REFERENCE_TREE):
-        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+        subprocess.run(["make", "-s", "-C", HERE, "ref", "harness", "plugin_host"], check=True)
 
 
 def lib():

@@ -35,6 +35,12 @@ QDF_CMD = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I" + os.pa
            "-lqhg4_amd", "-Wl,-rpath,$ORIGIN/_hdf5", "-Wl,-rpath,$ORIGIN"]
 
 
+# PopBase plugins (qhg4_amd/plugin): they compile against a QHG4 source
+# tree's headers and are built where one is present.
+QHG4_TREE = os.environ.get("QHG4_TREE", "/root/reference/QHG4")
+PLUGIN_DIR = os.path.join(HERE, "plugin")
+
+
 def _hdf5_link():
     os.makedirs(HDF5_LINK_DIR, exist_ok=True)
     link = os.path.join(HDF5_LINK_DIR, HDF5_SONAME)
@@ -63,6 +69,8 @@ def build(verbose=False):
         _hdf5_link()
         _run(QDF_CMD + ["-o", QDF_LIB + ".tmp"], "g++ (QDF library)", verbose)
         os.replace(QDF_LIB + ".tmp", QDF_LIB)
+        if os.path.isdir(os.path.join(QHG4_TREE, "core")):
+            _run(["make", "-s", "-B", "-j8", "-C", PLUGIN_DIR, "QHG4=" + QHG4_TREE], "make (PopBase plugins)", verbose)
     return LIB
 
 

@@ -57,8 +57,10 @@ GpuPop::GpuPop(const std::string &sClassName, SCellGrid *pCG, PopFinder *, int, 
     if (Vegetation *v = pCG->m_pVegetation) m_env.base_npp = v->m_adBaseANPP;
 }
 
+// PopWriter closes the agent type it was handed (QHG4/io/PopWriter.cpp:26-31),
+// so the handle may be gone by now: close it only while it is still valid
 GpuPop::~GpuPop() {
-    if (m_hAgentDataType != H5P_DEFAULT) H5Tclose(m_hAgentDataType);
+    if (m_hAgentDataType != H5P_DEFAULT && H5Iis_valid(m_hAgentDataType) > 0) H5Tclose(m_hAgentDataType);
 }
 
 int GpuPop::report(int iResult, const char *pWhere) {
@@ -291,12 +293,13 @@ int GpuPop::mergePop(PopBase *pOther) {
 hid_t GpuPop::createAgentDataTypeQDF() { return (hid_t)qhg_qdf_agent_type(); }
 
 void GpuPop::setAgentDataType() {
-    if (m_hAgentDataType != H5P_DEFAULT) H5Tclose(m_hAgentDataType);
+    if (m_hAgentDataType != H5P_DEFAULT && H5Iis_valid(m_hAgentDataType) > 0) H5Tclose(m_hAgentDataType);
     m_hAgentDataType = createAgentDataTypeQDF();
 }
 
+// a fresh type when none was made yet or a writer has closed the last one
 hid_t GpuPop::getAgentQDFDataType() {
-    if (m_hAgentDataType == H5P_DEFAULT) setAgentDataType();
+    if (m_hAgentDataType == H5P_DEFAULT || H5Iis_valid(m_hAgentDataType) <= 0) setAgentDataType();
     return m_hAgentDataType;
 }
 

@@ -52,7 +52,7 @@ constexpr int kGroupCells = 16;       // cells of one packed group
 constexpr int kGroupCellMax = 64;     // largest cell the packed pairing takes
 constexpr int kMaxIn = 8;             // incoming neighbour slots per cell
 constexpr int kScanBlock = 256;       // k_cell_scan threads per block
-constexpr int kScanPer = 2;           // cells per k_cell_scan thread
+constexpr int kScanPer = 1;           // cells per k_cell_scan thread
 constexpr int kCellTile = kScanBlock * kScanPer;   // cells per look-back tile
 constexpr int kMaxCross = 32;         // device limit on Genetics_num_crossover
 constexpr int kStatusRing = 8;        // host-mapped status snapshots
@@ -274,9 +274,15 @@ __device__ __forceinline__ uint64_t tile_read(uint64_t *w) {
 
 // Look-back over a pair of words per tile (two independent sums).  Called
 // by all 64 lanes of one wave; returns the exclusive prefixes of `tile`.
-// A predecessor counts only when both of its words carry the same flag
-// (the publisher writes them one after the other).  Every wait is bounded:
-// past the spin limit the step is flagged as failed.
+// Each round trip reads kLookWin windows of 64 predecessors at once, nearest
+// first, and sums back to the nearest inclusive word (tiles before the first
+// read as inclusive zeros), so up to 64 * kLookWin predecessors resolve in
+// one read once their aggregates are out.  A predecessor counts only when
+// both of its words carry the same flag (the publisher writes them one after
+// the other).  Every wait is bounded: past the spin limit the step is
+// flagged as failed.
+constexpr int kLookWin = 4;
+
 __device__ void lookback2(uint64_t *t1, uint64_t *t2, uint32_t tile, uint64_t agg1, uint64_t agg2, int32_t *err,
                           uint64_t &ex1, uint64_t &ex2) {
     const int lane = threadIdx.x & 63;
@@ -296,16 +302,35 @@ __device__ void lookback2(uint64_t *t1, uint64_t *t2, uint32_t tile, uint64_t ag
     int64_t top = (int64_t)tile - 1;
     uint32_t spins = 0;
     while (true) {
-        const int64_t j = top - lane;
-        const uint64_t w1 = (j >= 0) ? tile_read(&t1[j]) : kFlagInc;
-        const uint64_t w2 = (j >= 0) ? tile_read(&t2[j]) : kFlagInc;
-        const uint64_t f1 = w1 & ~kValMask, f2 = w2 & ~kValMask;
-        const bool ready = f1 == f2 && f1 != 0;
-        const uint64_t inc = __ballot(ready && f1 == kFlagInc);
-        const uint64_t nready = __ballot(!ready);
-        const int first = inc ? (__ffsll((unsigned long long)inc) - 1) : 64;
-        const uint64_t need = (first >= 63) ? ~0ull : ((2ull << first) - 1ull);
-        if (nready & need) {
+        uint64_t w1[kLookWin], w2[kLookWin];
+#pragma unroll
+        for (int m = 0; m < kLookWin; m++) {
+            const int64_t j = top - lane - 64 * m;
+            w1[m] = (j >= 0) ? tile_read(&t1[j]) : kFlagInc;
+            w2[m] = (j >= 0) ? tile_read(&t2[j]) : kFlagInc;
+        }
+        uint64_t s1 = 0, s2 = 0;
+        bool wait = false, done = false;
+#pragma unroll
+        for (int m = 0; m < kLookWin; m++) {
+            const uint64_t f1 = w1[m] & ~kValMask, f2 = w2[m] & ~kValMask;
+            const bool ready = f1 == f2 && f1 != 0;
+            const uint64_t inc = __ballot(ready && f1 == kFlagInc);
+            const uint64_t nready = __ballot(!ready);
+            const int first = inc ? (__ffsll((unsigned long long)inc) - 1) : 64;
+            const uint64_t need = (first >= 63) ? ~0ull : ((2ull << first) - 1ull);
+            if (nready & need) {
+                wait = true;
+                break;
+            }
+            s1 += wave_sum_u64(lane <= first ? (w1[m] & kValMask) : 0ull);
+            s2 += wave_sum_u64(lane <= first ? (w2[m] & kValMask) : 0ull);
+            if (first < 64) {
+                done = true;
+                break;
+            }
+        }
+        if (wait) {
             if (++spins < kLookbackSpins) {
                 __builtin_amdgcn_s_sleep(1);
                 continue;
@@ -313,10 +338,10 @@ __device__ void lookback2(uint64_t *t1, uint64_t *t2, uint32_t tile, uint64_t ag
             if (lane == 0) atomicOr(err, kErrLookback);
             break;
         }
-        ex1 += wave_sum_u64(lane <= first ? (w1 & kValMask) : 0ull);
-        ex2 += wave_sum_u64(lane <= first ? (w2 & kValMask) : 0ull);
-        if (first < 64) break;
-        top -= 64;
+        ex1 += s1;
+        ex2 += s2;
+        if (done) break;
+        top -= 64 * kLookWin;
     }
     if (lane == 0) {
         tile_publish(&t1[tile], kFlagInc | (ex1 + agg1));
@@ -1381,16 +1406,26 @@ __global__ void __launch_bounds__(kWave) k_assort_pair(Ctx x, Prog p, StepArgs s
 // ---------------------------------------------------------------------------
 // k_cell_scan: next per-cell counts and birth counts, scanned together
 // (low 32 bits agents, high bits births), and the occupied cells of the next
-// step, with a decoupled look-back over tiles of kCellTile cells.  Blocks
-// take tiles blockIdx, blockIdx + grid, ...; the grid fits on the device.  Also writes the
-// absolute next-buffer position of every incoming run and of each cell's
-// newborns.  The block of the last tile closes the step's bookkeeping in the
-// other status half.
+// step, with a decoupled look-back over tiles of kCellTile cells, one cell
+// per thread.  Blocks take tiles by ticket; the grid fits on the device.
+// Also writes the absolute next-buffer position of every incoming run and
+// of each cell's newborns.  The block of the last tile closes the step's
+// bookkeeping in the other status half.
+// The scan is latency-bound (one wave per SIMD at most): every thread's
+// loads go out in one round trip, and the code on that path is kept short.
 // ---------------------------------------------------------------------------
+constexpr int kStageWords = kCellTile * kMaxIn;       // a tile's incoming-run words
+constexpr int kStagePer = kStageWords / kScanBlock;   // ... per thread
+static_assert(kScanPer == 1, "one cell per scan thread");
+
 struct ScanLds {
     uint64_t w[kScanBlock / 64], w2[kScanBlock / 64];
     uint64_t excl, excl2;
     uint32_t ticket;
+    // dense steps: the tile's incoming-run counts and source slots, loaded
+    // coalesced (cell-major, in_w words per cell)
+    uint32_t cm[kStageWords];
+    int32_t ie[kStageWords];
 };
 
 // The next tile for this block.  Tiles are handed out in increasing order by
@@ -1406,14 +1441,17 @@ __device__ __forceinline__ uint32_t next_tile(uint32_t *ticket, ScanLds &L) {
     return t;
 }
 
-// The inputs of one scan tile, per thread (kScanPer cells each): stayers,
-// arrivals, incoming-run counts (dense steps), and next count | births << 32.
+// The inputs of one scan tile for this thread's cell: stayers, arrivals,
+// births, and next count | births << 32.
 struct TileIn {
-    uint32_t stay[kScanPer], arr[kScanPer];
-    uint32_t cm[kScanPer][kMaxIn];
-    int32_t ie[kScanPer][kMaxIn];   // dense steps: the incoming runs' source slots
-    uint64_t v[kScanPer];
-    uint64_t sum, osum;   // this thread's next agents | births, and occupied cells
+    uint32_t stay, arr, birth;
+    uint64_t v;
+    bool staged;              // the tile's incoming runs go to ScanLds (dense step)
+    uint32_t sm[kStagePer];   // ... this thread's share of them
+    int32_t se[kStagePer];
+    uint32_t rst, rbi, rar;   // raw loads, until tile_ready
+    bool okc;                 // the cell exists
+    int32_t wn;               // staged words of the tile
 };
 
 // A tile none of whose cells holds agents next step (the step's last tile
@@ -1424,39 +1462,80 @@ __device__ __forceinline__ bool tile_idle(const CtxT &x, uint32_t tile) {
     return base + kCellTile < x.ncells && x.tact[tile] == 0u;
 }
 
-// Reads a tile's inputs.  With clear, the counts go back to zero once read
-// (the scan proper); without, they stay (a pre-pass that only sums them).
-// With many cells occupied (sa.dense) the incoming-run counts of every cell
-// are loaded in the same round trip; otherwise only those of cells that
-// receive movers, at the write.
+// Reads a tile's inputs into I (the counts stay in place; tile_clear zeroes
+// them once the tile is known to be this block's).  With stage and many
+// cells occupied (sa.dense), the incoming-run counts and source slots of
+// the whole tile are read too, coalesced, for tile_stage to put into LDS;
+// otherwise only those of cells that receive movers are read, at the write.
+// tile_load only issues the loads (indices past the end are clamped, so no
+// branch splits them): tile_ready then drops what lies past the end, so
+// that other loads can be issued in between and share the round trip.
 template <typename CtxT>
-__device__ __forceinline__ void tile_load(const CtxT &x, const StepArgs &sa, uint32_t tile, bool clear, TileIn &I) {
+__device__ __forceinline__ void tile_load(const CtxT &x, const StepArgs &sa, uint32_t tile, bool stage, TileIn &I) {
     const int64_t nc = x.ncells;
     const int64_t base = (int64_t)tile * kCellTile;
-    const bool dense = clear && sa.dense != 0;
-    I.sum = 0;
-    I.osum = 0;
+    const int64_t c = base + (int64_t)threadIdx.x;
+    I.okc = c < nc;
+    const int64_t cc = I.okc ? c : 0;
+    I.rst = x.cstay[cc];
+    I.rbi = x.cbirth[cc];
+    I.rar = x.carr[cc];
+    I.staged = stage && sa.dense != 0;
+    const int32_t iw = x.in_w;
+    const int64_t w0 = base * iw;
+    I.wn = (int32_t)(((nc - base < kCellTile) ? (nc - base) : kCellTile) * iw);
+    if (I.staged) {
 #pragma unroll
-    for (int q = 0; q < kScanPer; q++) {
-        const int64_t c = base + (int64_t)threadIdx.x * kScanPer + q;
-        const bool ok = c < nc;
-        I.stay[q] = ok ? x.cstay[c] : 0u;
-        const uint32_t b = ok ? x.cbirth[c] : 0u;
-        I.arr[q] = ok ? x.carr[c] : 0u;
+        for (int r = 0; r < kStagePer; r++) {
+            const int32_t j = r * kScanBlock + (int32_t)threadIdx.x;
+            const int64_t jj = w0 + ((j < I.wn) ? j : 0);
+            I.sm[r] = x.cin[jj];
+            I.se[r] = x.in_e[jj];
+        }
+    }
+}
+
+__device__ __forceinline__ void tile_ready(TileIn &I) {
+    I.stay = I.okc ? I.rst : 0u;
+    I.birth = I.okc ? I.rbi : 0u;
+    I.arr = I.okc ? I.rar : 0u;
+    I.v = (uint64_t)(I.stay + I.arr + I.birth) | ((uint64_t)I.birth << 32);
+    if (I.staged) {
 #pragma unroll
-        for (int k = 0; k < kMaxIn; k++) {
-            const bool rd = dense && ok && k < x.in_w;
-            I.cm[q][k] = rd ? x.cin[c * x.in_w + k] : 0u;
-            I.ie[q][k] = rd ? x.in_e[c * x.in_w + k] : -1;
+        for (int r = 0; r < kStagePer; r++) {
+            if (r * kScanBlock + (int32_t)threadIdx.x >= I.wn) {
+                I.sm[r] = 0u;
+                I.se[r] = -1;
+            }
         }
-        if (clear) {
-            if (I.stay[q]) x.cstay[c] = 0;
-            if (b) x.cbirth[c] = 0;
-            if (I.arr[q]) x.carr[c] = 0;
-        }
-        I.v[q] = ok ? ((uint64_t)(I.stay[q] + I.arr[q] + b) | ((uint64_t)b << 32)) : 0ull;
-        I.sum += I.v[q];
-        I.osum += ((uint32_t)I.v[q] != 0u) ? 1u : 0u;
+    }
+}
+
+// The staged incoming runs into LDS (read after the next block barrier).
+__device__ __forceinline__ void tile_stage(const TileIn &I, ScanLds &L) {
+    if (!I.staged) return;
+#pragma unroll
+    for (int r = 0; r < kStagePer; r++) {
+        const int32_t j = r * kScanBlock + (int32_t)threadIdx.x;
+        L.cm[j] = I.sm[r];
+        L.ie[j] = I.se[r];
+    }
+}
+
+// The scan proper owns the tile: the counts it read go back to zero (the
+// staged incoming-run counts too; unstaged ones are cleared at the write).
+template <typename CtxT>
+__device__ __forceinline__ void tile_clear(const CtxT &x, uint32_t tile, const TileIn &I) {
+    const int64_t base = (int64_t)tile * kCellTile;
+    const int64_t c = base + (int64_t)threadIdx.x;
+    if (I.stay) x.cstay[c] = 0u;
+    if (I.birth) x.cbirth[c] = 0u;
+    if (I.arr) x.carr[c] = 0u;
+    if (I.staged) {
+        const int64_t w0 = base * x.in_w;
+#pragma unroll
+        for (int r = 0; r < kStagePer; r++)
+            if (I.sm[r] != 0u) x.cin[w0 + r * kScanBlock + (int32_t)threadIdx.x] = 0u;
     }
 }
 
@@ -1468,8 +1547,9 @@ struct TileSums {
 
 __device__ __forceinline__ TileSums tile_sums(const TileIn &I, ScanLds &L) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint64_t incl = wave_incl_scan_u64(I.sum);
-    const uint64_t oincl = wave_incl_scan_u64(I.osum);
+    const uint64_t occ = ((uint32_t)I.v != 0u) ? 1u : 0u;
+    const uint64_t incl = wave_incl_scan_u64(I.v);
+    const uint64_t oincl = wave_incl_scan_u64(occ);
     if (lane == 63) { L.w[wave] = incl; L.w2[wave] = oincl; }
     __syncthreads();
     TileSums S{0, 0, 0, 0};
@@ -1480,57 +1560,60 @@ __device__ __forceinline__ TileSums tile_sums(const TileIn &I, ScanLds &L) {
         S.tot += L.w[w];
         S.otot += L.w2[w];
     }
-    S.pre = wpre + incl - I.sum;
-    S.opre = owpre + oincl - I.osum;
+    S.pre = wpre + incl - I.v;
+    S.opre = owpre + oincl - occ;
     return S;
 }
 
-// Writes a tile's outputs from its exclusive prefix (ex agents | births,
-// ex2 occupied cells): next offsets, run and newborn positions, the next
-// occupied-cell list; the step's last tile closes the bookkeeping.
+// Writes a tile's outputs from this cell's exclusive prefix (run agents |
+// births, orun occupied cells): next offset, run and newborn positions, the
+// next occupied-cell list; the step's last tile closes the bookkeeping.
 template <typename CtxT>
 __device__ __forceinline__ void tile_write(const CtxT &x, const StepArgs &sa, uint32_t tile, const TileIn &I,
-                                           uint64_t run, uint64_t orun) {
+                                           const ScanLds &L, uint64_t run, uint64_t orun) {
     const int64_t nc = x.ncells;
     const int64_t base = (int64_t)tile * kCellTile;
     const bool last = base + kCellTile >= nc;
-    const bool dense = sa.dense != 0;
+    const int64_t c = base + (int64_t)threadIdx.x;
+    const int32_t iw = x.in_w;
+    if (I.okc) {
+        const uint32_t o = (uint32_t)run;
+        x.off_nxt[c] = o;
+        uint32_t pos = o + I.stay;
+        if (I.arr) {
+            // arrivals: one block per incoming run, in run order (the row's
+            // counts and slots read at once)
+            uint32_t m[kMaxIn];
+            int32_t ie[kMaxIn];
+            const int32_t lrow = (int32_t)threadIdx.x * iw;
 #pragma unroll
-    for (int q = 0; q < kScanPer; q++) {
-        const int64_t c = base + (int64_t)threadIdx.x * kScanPer + q;
-        if (c < nc) {
-            const uint32_t o = (uint32_t)run;
-            x.off_nxt[c] = o;
-            uint32_t pos = o + I.stay[q];
-            if (I.arr[q]) {
-                // arrivals: one block per incoming run, in run order
+            for (int k = 0; k < kMaxIn; k++) {
+                const bool in = k < iw;
+                m[k] = !in ? 0u : (I.staged ? L.cm[lrow + k] : x.cin[c * iw + k]);
+                ie[k] = !in ? -1 : (I.staged ? L.ie[lrow + k] : x.in_e[c * iw + k]);
+            }
 #pragma unroll
-                for (int k = 0; k < kMaxIn; k++) {
-                    if (k >= x.in_w) break;
-                    const uint32_t m = dense ? I.cm[q][k] : x.cin[c * x.in_w + k];
-                    if (m) {
-                        x.rpos[dense ? I.ie[q][k] : x.in_e[c * x.in_w + k]] = pos;
-                        x.cin[c * x.in_w + k] = 0;
-                        pos += m;
-                    }
+            for (int k = 0; k < kMaxIn; k++) {
+                if (m[k]) {
+                    x.rpos[ie[k]] = pos;
+                    if (!I.staged) x.cin[c * iw + k] = 0;
+                    pos += m[k];
                 }
             }
-            if ((uint32_t)(I.v[q] >> 32)) {
-                x.bpos[c] = pos;
-                x.birth_off[c] = (uint32_t)(run >> 32);
-            }
-            const uint32_t nn = (uint32_t)I.v[q];
-            if (nn != 0) {
-                x.occ_nxt[orun] = OccCell{(int32_t)c, o, nn, 0u};
-                orun++;
-            }
         }
-        run += I.v[q];
+        if (I.birth) {
+            x.bpos[c] = pos;
+            x.birth_off[c] = (uint32_t)(run >> 32);
+        }
+        const uint32_t nn = (uint32_t)I.v;
+        if (nn != 0) x.occ_nxt[orun] = OccCell{(int32_t)c, o, nn, 0u};
     }
     if (threadIdx.x == 0) x.tact[tile] = 0u;
     if (last && threadIdx.x == kScanBlock - 1) {
         // last tile: the next step's sizes (the totals the step kernels
         // need; deaths, moves and the host snapshot follow in k_cell_scatter)
+        run += I.v;
+        orun += ((uint32_t)I.v != 0u) ? 1u : 0u;
         const int64_t total = (int64_t)(uint32_t)run;
         const int64_t B = (int64_t)(run >> 32);
         x.off_nxt[nc] = (uint32_t)total;
@@ -1551,15 +1634,12 @@ __device__ __forceinline__ void tile_write(const CtxT &x, const StepArgs &sa, ui
     }
 }
 
-// One look-back tile of k_cell_scan (kCellTile cells), all kScanBlock threads.
+// One look-back tile of k_cell_scan, all kScanBlock threads, from its
+// inputs already loaded into I (and staged into L).  more: another tile
+// follows in this block (the LDS is reused).
 template <typename CtxT>
-__device__ __forceinline__ void scan_tile(const CtxT &x, const StepArgs &sa, uint32_t tile, ScanLds &L) {
-    // the activity flag and the tile's counts in one round trip (an idle
-    // tile's counts are zero, so reading them clears nothing)
-    const bool may_idle = (int64_t)tile * kCellTile + kCellTile < x.ncells;
-    const uint32_t act = may_idle ? x.tact[tile] : 1u;
-    TileIn I;
-    tile_load(x, sa, tile, true, I);
+__device__ __forceinline__ void scan_tile(const CtxT &x, const StepArgs &sa, uint32_t tile, uint32_t act,
+                                          TileIn &I, ScanLds &L, bool more) {
     if (act == 0u) {
         // no cell of the tile holds agents next step: its sums are zero and
         // none of its per-cell outputs is read (the step totals come from
@@ -1571,6 +1651,7 @@ __device__ __forceinline__ void scan_tile(const CtxT &x, const StepArgs &sa, uin
         }
         return;
     }
+    tile_clear(x, tile, I);
     const TileSums S = tile_sums(I, L);
     if (threadIdx.x < 64) {
         uint64_t ex, ex2;
@@ -1578,31 +1659,82 @@ __device__ __forceinline__ void scan_tile(const CtxT &x, const StepArgs &sa, uin
         if (threadIdx.x == 0) { L.excl = ex; L.excl2 = ex2; }
     }
     __syncthreads();
-    tile_write(x, sa, tile, I, L.excl + S.pre, L.excl2 + S.opre);
-    __syncthreads();
+    tile_write(x, sa, tile, I, L, L.excl + S.pre, L.excl2 + S.opre);
+    if (more) __syncthreads();
 }
 
+// kOneRound: the grid has a block per tile, so the first round of tickets
+// hands out every tile and no block asks again (straight-line code; the
+// host launches the looping form otherwise).
+template <bool kOneRound>
 __global__ void __launch_bounds__(kScanBlock) k_cell_scan(Ctx x, StepArgs sa) {
     __shared__ ScanLds lds;
     const int lane = threadIdx.x & 63;
-    // this thread's slice of the k_cell_act partials (deaths, moves, FERTILE
-    // counts), loaded with the status so that it adds no round trip; its
-    // wave's sums are stored at the end for k_cell_scatter's bookkeeping
-    // block (plain stores: no contended atomics)
-    uint64_t D = 0, Mv = 0, Fe = 0;
-    for (int32_t j = blockIdx.x * kScanBlock + threadIdx.x; j < x.nact_blocks; j += gridDim.x * kScanBlock) {
+    const uint32_t nt = (uint32_t)x.ntiles;
+    // One round trip for: the status, this thread's first slice of the
+    // k_cell_act partials (deaths, moves, FERTILE counts), the inputs of tile
+    // blockIdx.x (the ticket this block most likely draws) with its activity
+    // flag, and, issued last, the first ticket: waiting for the ticket waits
+    // for them all.  The partials' wave sums are stored at the end for
+    // k_cell_scatter's bookkeeping block (plain stores: no contended atomics).
+    const int32_t err = x.st->error;
+    const int32_t j0 = blockIdx.x * kScanBlock + threadIdx.x;
+    const int32_t jstride = gridDim.x * kScanBlock;
+    const bool pok = j0 < x.nact_blocks;
+    const int32_t jc = pok ? j0 : 0;
+    const uint32_t p0 = x.wave_part[kParts * jc + 0];
+    const uint32_t p1 = x.wave_part[kParts * jc + 1];
+    const uint32_t p2 = x.wave_part[kParts * jc + 2];
+    const uint32_t spec = (blockIdx.x < nt) ? blockIdx.x : nt - 1;
+    const uint32_t ta = x.tact[spec];
+    TileIn I;
+    tile_load(x, sa, spec, true, I);
+    uint32_t tk = 0;
+    if (threadIdx.x == 0) tk = atomicAdd(x.tick, 1u);
+    tile_ready(I);
+    tile_stage(I, lds);
+    if (threadIdx.x == 0) lds.ticket = tk;
+    __syncthreads();
+    const uint32_t first = (uint32_t)__builtin_amdgcn_readfirstlane((int)lds.ticket);
+    if (err) return;
+    const auto active = [&](uint32_t t, uint32_t flag) -> uint32_t {
+        return ((int64_t)t * kCellTile + kCellTile >= x.ncells || flag != 0u) ? 1u : 0u;
+    };
+    if (kOneRound) {
+        if (first < nt) {
+            uint32_t act = active(spec, ta);
+            if (first != spec) {
+                // not the tile loaded ahead: load this one now
+                __syncthreads();   // the LDS staging may still be read
+                act = active(first, x.tact[first]);
+                tile_load(x, sa, first, true, I);
+                tile_ready(I);
+                tile_stage(I, lds);
+            }
+            scan_tile(x, sa, first, act, I, lds, false);
+        }
+    } else {
+        uint32_t tile = spec;
+        for (uint32_t t = first; t < nt; t = next_tile(x.tick, lds)) {
+            uint32_t act = active(tile, ta);
+            if (t != tile) {
+                __syncthreads();   // the LDS staging may still be read
+                tile = t;
+                act = active(tile, x.tact[tile]);
+                tile_load(x, sa, tile, true, I);
+                tile_ready(I);
+                tile_stage(I, lds);
+            }
+            scan_tile(x, sa, t, act, I, lds, true);
+            tile = nt;   // later tiles are loaded on demand
+        }
+    }
+    uint64_t D = pok ? p0 : 0u, Mv = pok ? p1 : 0u, Fe = pok ? p2 : 0u;
+    for (int32_t j = j0 + jstride; j < x.nact_blocks; j += jstride) {
         D += x.wave_part[kParts * j + 0];
         Mv += x.wave_part[kParts * j + 1];
         Fe += x.wave_part[kParts * j + 2];
     }
-    if (x.st->error) return;
-    // when the grid has a block per tile, the first round of tickets hands
-    // out every tile: no block asks again (that round trip would only end
-    // the kernel later)
-    const uint32_t nt = (uint32_t)x.ntiles;
-    const bool one_round = gridDim.x >= nt;
-    for (uint32_t tile = next_tile(x.tick, lds); tile < nt; tile = one_round ? nt : next_tile(x.tick, lds))
-        scan_tile(x, sa, tile, lds);
     D = wave_sum_u64(D);
     Mv = wave_sum_u64(Mv);
     Fe = wave_sum_u64(Fe);
@@ -2339,6 +2471,7 @@ __global__ void __launch_bounds__(kScanBlock) k_batch_scan_sums(BatchArgs B) {
         if (!tile_idle(x, (uint32_t)lt)) {
             TileIn I;
             tile_load(x, sa, (uint32_t)lt, false, I);
+            tile_ready(I);
             const TileSums S = tile_sums(I, lds);
             tot = S.tot;
             otot = S.otot;
@@ -2387,8 +2520,11 @@ __global__ void __launch_bounds__(kScanBlock) k_batch_scan(BatchArgs B) {
         }
         TileIn I;
         tile_load(x, sa, (uint32_t)lt, true, I);
+        tile_ready(I);
+        tile_stage(I, lds);
+        tile_clear(x, (uint32_t)lt, I);
         const TileSums S = tile_sums(I, lds);
-        tile_write(x, sa, (uint32_t)lt, I, lds.excl + S.pre, lds.excl2 + S.opre);
+        tile_write(x, sa, (uint32_t)lt, I, lds, lds.excl + S.pre, lds.excl2 + S.opre);
         __syncthreads();
     }
 }

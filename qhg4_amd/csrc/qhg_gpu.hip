@@ -784,7 +784,9 @@ int phase_finalize(qhg_pop *p) {
     }
     const Ctx x = p->ctx();
     const StepArgs sa = p->step_args();
-    int rc = launch_bs(p, KB_SCAN, k_cell_scan, p->scan_grid, kScanBlock, x, sa);
+    // a block per tile when they all fit: the straight-line form
+    int rc = (p->scan_grid >= p->ntiles) ? launch_bs(p, KB_SCAN, k_cell_scan<true>, p->scan_grid, kScanBlock, x, sa)
+                                         : launch_bs(p, KB_SCAN, k_cell_scan<false>, p->scan_grid, kScanBlock, x, sa);
     if (rc) return rc;
     // one more block than the groups use: the step's bookkeeping
     rc = launch_bs(p, KB_SCATTER, k_cell_scatter, p->scatter_grid + 1, kCellBlock, x, sa);
@@ -876,7 +878,7 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
     {
         // the look-back scan runs at most as many blocks as fit on the device
         int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cell_scan, kScanBlock, 0) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cell_scan<false>, kScanBlock, 0) != hipSuccess ||
             per_cu < 1)
             per_cu = 1;
         p->scan_grid = (int)std::min<int64_t>(p->ntiles, (int64_t)per_cu * p->num_cu);
@@ -903,9 +905,9 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
     CK(upload_array(&p->d_outk, outk.data(), outk.size(), p->stream));
     if (hipMalloc(&p->d_K, nc * 8) != hipSuccess || hipMalloc(&p->d_W, nc * (p->conn + 1) * 8) != hipSuccess ||
         hipMalloc(&p->d_tmp, nc * 8) != hipSuccess || hipMalloc(&p->d_off[0], (nc + 1) * 4) != hipSuccess ||
-        hipMalloc(&p->d_off[1], (nc + 1) * 4) != hipSuccess || hipMalloc(&p->d_cstay, nc * 4) != hipSuccess ||
-        hipMalloc(&p->d_cbirth, nc * 4) != hipSuccess || hipMalloc(&p->d_cin, nc * p->in_w * 4) != hipSuccess ||
-        hipMalloc(&p->d_carr, nc * 4) != hipSuccess || hipMalloc(&p->d_tact, (size_t)p->ntiles * 4) != hipSuccess ||
+        hipMalloc(&p->d_off[1], (nc + 1) * 4) != hipSuccess || hipMalloc(&p->d_cstay, (nc + 2) * 4) != hipSuccess ||
+        hipMalloc(&p->d_cbirth, (nc + 2) * 4) != hipSuccess || hipMalloc(&p->d_cin, nc * p->in_w * 4) != hipSuccess ||
+        hipMalloc(&p->d_carr, (nc + 2) * 4) != hipSuccess || hipMalloc(&p->d_tact, (size_t)p->ntiles * 4) != hipSuccess ||
         hipMalloc(&p->d_tick, 4) != hipSuccess ||
         hipMalloc(&p->d_birth_off, (nc + 1) * 4) != hipSuccess || hipMalloc(&p->d_bpos, nc * 4) != hipSuccess ||
         hipMalloc(&p->d_rpos, nc * p->conn * 4) != hipSuccess ||
@@ -924,10 +926,10 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
     (void)hipMemsetAsync(p->d_tiles, 0, (size_t)p->ntiles * 8, p->stream);
     (void)hipMemsetAsync(p->d_tiles2, 0, (size_t)p->ntiles * 8, p->stream);
     // per-cell counts start at zero; k_cell_scan restores zeros after use
-    (void)hipMemsetAsync(p->d_cstay, 0, nc * 4, p->stream);
-    (void)hipMemsetAsync(p->d_cbirth, 0, nc * 4, p->stream);
+    (void)hipMemsetAsync(p->d_cstay, 0, (nc + 2) * 4, p->stream);
+    (void)hipMemsetAsync(p->d_cbirth, 0, (nc + 2) * 4, p->stream);
     (void)hipMemsetAsync(p->d_cin, 0, nc * p->in_w * 4, p->stream);
-    (void)hipMemsetAsync(p->d_carr, 0, nc * 4, p->stream);
+    (void)hipMemsetAsync(p->d_carr, 0, (nc + 2) * 4, p->stream);
     (void)hipMemsetAsync(p->d_tact, 0, (size_t)p->ntiles * 4, p->stream);
     (void)hipMemsetAsync(p->d_tick, 0, 4, p->stream);
     (void)hipMemsetAsync(p->d_st, 0, sizeof(DevStatus), p->stream);
@@ -1062,10 +1064,10 @@ int qhg_gpu_upload_agents(qhg_pop *p, const qhg_agents *a) {
     }
     const int b = p->cur();
     const size_t ncz = (size_t)p->ncells;
-    HIPCHK(hipMemsetAsync(p->d_cstay, 0, ncz * 4, p->stream));
-    HIPCHK(hipMemsetAsync(p->d_cbirth, 0, ncz * 4, p->stream));
+    HIPCHK(hipMemsetAsync(p->d_cstay, 0, (ncz + 2) * 4, p->stream));
+    HIPCHK(hipMemsetAsync(p->d_cbirth, 0, (ncz + 2) * 4, p->stream));
     HIPCHK(hipMemsetAsync(p->d_cin, 0, ncz * p->in_w * 4, p->stream));
-    HIPCHK(hipMemsetAsync(p->d_carr, 0, ncz * 4, p->stream));
+    HIPCHK(hipMemsetAsync(p->d_carr, 0, (ncz + 2) * 4, p->stream));
     HIPCHK(hipMemsetAsync(p->d_tact, 0, (size_t)p->ntiles * 4, p->stream));
     if (!occ.empty())
         HIPCHK(hipMemcpyAsync(p->d_occ[b], occ.data(), occ.size() * sizeof(OccCell), hipMemcpyHostToDevice,

@@ -64,7 +64,13 @@ extern "C" {
 #define QHG_ACT_ATANDEATH    12   /* QHG4/actions/ATanDeath.cpp:48-90         */
 #define QHG_ACT_ASSORTPAIR   13   /* QHG4/actions/AssortativePairHam.cpp:93-238
                                      + QHG4/utils/RankTable.cpp:119-278        */
-#define QHG_ACT_LAST         QHG_ACT_ASSORTPAIR
+#define QHG_ACT_ASSORTPROT   14   /* QHG4/actions/AssortativePair.cpp:94-271:
+                                     matches counted over the proteins of
+                                     QHG4/genes/ProteinBuilder.cpp:163-220, 231-289
+                                     (MD5 per protein) and
+                                     QHG4/genes/ProteomeComparator.cpp:25-60,
+                                     then the same RankTable pairing       */
+#define QHG_ACT_LAST         QHG_ACT_ASSORTPROT
 
 /* MultiEvaluator combination modes, QHG4/actions/MultiEvaluator.h:20-25 */
 #define QHG_ME_ADD_SIMPLE 0
@@ -90,7 +96,7 @@ extern "C" {
 #define QHG_STREAM_ACT      2   /* per-agent action words, QHG_ACTW_*           */
 #define QHG_STREAM_NEWBORN  6   /* word 0: gender of the newborn                */
 #define QHG_STREAM_GENETICS 7   /* Genetics::makeOffspring draws, in order      */
-#define QHG_STREAM_ASSORT   8   /* AssortativePairHamming: one cell's draws in order
+#define QHG_STREAM_ASSORT   8   /* AssortativePair(Hamming): one cell's draws in order
                                    (key: cell index | (draw >> 16) << 32, word: draw & 0xffff) */
 
 /* words of QHG_STREAM_ACT (one Philox block covers words 0-3) */
@@ -165,8 +171,8 @@ typedef struct qhg_params {
     double oad_max_age, oad_uncertainty;
     /* ATanDeath_max_age, ATanDeath_range, ATanDeath_slope */
     double atan_max_age, atan_range, atan_slope;
-    /* AssortativePairHamming_cutoff (a float in the reference),
-       AssortativePairHamming_permute (0 / 1) */
+    /* AssortativePairHamming_cutoff / AssortativePair_cutoff (a float in
+       the reference), ..._permute (0 / 1): a class has one of the two */
     double  assort_cutoff;
     int32_t assort_permute;
     int32_t assort_pad;

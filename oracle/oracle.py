@@ -76,6 +76,11 @@ def lib():
         L.orc_betai.restype = C.c_double
         L.orc_polyline.argtypes = [C.c_int32, P(C.c_double), P(C.c_double), C.c_double]
         L.orc_polyline.restype = C.c_double
+        L.orc_md5.argtypes = [P(C.c_uint8), C.c_int64, P(C.c_uint8)]
+        L.orc_proteome.argtypes = [P(C.c_uint64), C.c_int32, C.c_int32, P(C.c_uint8), C.c_int32]
+        L.orc_proteome.restype = C.c_int32
+        L.orc_prot_matches.argtypes = [P(C.c_uint8), C.c_int32, P(C.c_uint8), C.c_int32]
+        L.orc_prot_matches.restype = C.c_int32
         _lib = L
     return _lib
 

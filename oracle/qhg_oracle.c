@@ -843,6 +843,147 @@ int32_t orc_rank_pairs(int32_t nf, int32_t nm, float *ranks, float cutoff, int32
     return (s.used > nu) ? -1 : n;
 }
 
+/* MD5 (RFC 1321 sections 3.1-3.5): message padded with 0x80 and zeros to
+ * 56 mod 64 bytes, then the bit length as a little-endian 64-bit word;
+ * 64-byte blocks as 16 little-endian words; four rounds of 16 operations
+ * with the functions F, G, H, I, the sine table T and the per-round shifts;
+ * the digest is A, B, C, D in little-endian byte order.  The table is
+ * computed here from its definition (T[i] = floor(2^32 |sin(i + 1)|)). */
+static uint32_t md5_T[64];
+static int md5_T_ready = 0;
+
+static void md5_init_table(void) {
+    if (md5_T_ready) return;
+    for (int i = 0; i < 64; i++) md5_T[i] = (uint32_t)(fabs(sin((double)(i + 1))) * 4294967296.0);
+    md5_T_ready = 1;
+}
+
+static uint32_t md5_rotl(uint32_t v, int s) { return (v << s) | (v >> (32 - s)); }
+
+static void md5_compress(uint32_t st[4], const uint8_t blk[64]) {
+    static const int S[4][4] = {{7, 12, 17, 22}, {5, 9, 14, 20}, {4, 11, 16, 23}, {6, 10, 15, 21}};
+    uint32_t X[16];
+    for (int k = 0; k < 16; k++)
+        X[k] = (uint32_t)blk[4 * k] | ((uint32_t)blk[4 * k + 1] << 8) | ((uint32_t)blk[4 * k + 2] << 16) |
+               ((uint32_t)blk[4 * k + 3] << 24);
+    uint32_t a = st[0], b = st[1], c = st[2], d = st[3];
+    for (int i = 0; i < 64; i++) {
+        uint32_t f;
+        int k;
+        switch (i / 16) {
+        case 0: f = (b & c) | (~b & d); k = i; break;
+        case 1: f = (b & d) | (c & ~d); k = (1 + 5 * i) % 16; break;
+        case 2: f = b ^ c ^ d; k = (5 + 3 * i) % 16; break;
+        default: f = c ^ (b | ~d); k = (7 * i) % 16; break;
+        }
+        uint32_t na = b + md5_rotl(a + f + X[k] + md5_T[i], S[i / 16][i % 4]);
+        a = d; d = c; c = b; b = na;
+    }
+    st[0] += a; st[1] += b; st[2] += c; st[3] += d;
+}
+
+void orc_md5(const uint8_t *msg, int64_t len, uint8_t out[16]) {
+    md5_init_table();
+    uint32_t st[4] = {0x67452301u, 0xefcdab89u, 0x98badcfeu, 0x10325476u};
+    int64_t done = 0;
+    for (; len - done >= 64; done += 64) md5_compress(st, msg + done);
+    uint8_t tail[128];
+    int64_t r = len - done;
+    memset(tail, 0, sizeof tail);
+    memcpy(tail, msg + done, (size_t)r);
+    tail[r] = 0x80;
+    int64_t tl = (r + 1 + 8 <= 64) ? 64 : 128;
+    uint64_t bits = (uint64_t)len * 8u;
+    for (int k = 0; k < 8; k++) tail[tl - 8 + k] = (uint8_t)(bits >> (8 * k));
+    md5_compress(st, tail);
+    if (tl == 128) md5_compress(st, tail + 64);
+    for (int k = 0; k < 4; k++)
+        for (int b = 0; b < 4; b++) out[4 * k + b] = (uint8_t)(st[k] >> (8 * b));
+}
+
+/* The genetic code of ProteinBuilder (aTranslationTable,
+ * QHG4/genes/ProteinBuilder.h:64-78): codon X*16 + Y*4 + Z (A=0, C=1, G=2,
+ * T=3) to the amino-acid number (Met = 10, stop = 20). */
+static const uint8_t orc_code[64] = {
+    8, 11, 8, 11, 16, 16, 16, 16, 14, 15, 14, 15, 7, 7, 10, 7,
+    13, 6, 13, 6, 12, 12, 12, 12, 14, 14, 14, 14, 9, 9, 9, 9,
+    3, 2, 3, 2, 0, 0, 0, 0, 5, 5, 5, 5, 17, 17, 17, 17,
+    20, 19, 20, 19, 15, 15, 15, 15, 20, 1, 18, 1, 9, 4, 9, 4,
+};
+enum { AA_MET = 10, AA_STP = 20, AA_END_TOTAL = 33, AA_END_ALLELE = 34 };
+
+/* ProteinBuilder's reader (getNextNuc / getNextAA,
+ * QHG4/genes/ProteinBuilder.cpp:231-289): nucleotides strand by strand,
+ * 2 bits each from the low end of 32-nucleotide words; a strand's end ends
+ * the codon being read (its nucleotides are dropped) */
+typedef struct {
+    const uint64_t *g;
+    int32_t G, nb, pos, strand;
+    uint64_t cur;
+} orc_reader;
+
+static int reader_nuc(orc_reader *r) {   /* 0..3, or -1 strand end, -2 genome end */
+    r->pos++;
+    if (r->pos >= r->G) {
+        if (r->strand < 1) { r->strand++; r->pos = -1; return -1; }
+        return -2;
+    }
+    if (r->pos % 32 == 0) r->cur = r->g[(size_t)r->strand * r->nb + r->pos / 32];
+    int v = (int)(r->cur & 3u);
+    r->cur >>= 2;
+    return v;
+}
+
+static int reader_aa(orc_reader *r) {
+    int idx = 0;
+    for (int i = 0; i < 3; i++) {
+        int v = reader_nuc(r);
+        if (v == -2) return AA_END_TOTAL;
+        if (v == -1) return AA_END_ALLELE;
+        idx = idx * 4 + v;
+    }
+    return orc_code[idx];
+}
+
+/* ProteinBuilder::translateGenomeHash (ProteinBuilder.cpp:163-220): the MD5
+ * digests of the agent's proteins, in order (the reference sorts its
+ * digest pointers by address, an order no result depends on).  Writes at
+ * most max digests; returns the number of proteins. */
+int32_t orc_proteome(const uint64_t *genome, int32_t gwords, int32_t genome_size, uint8_t *digests, int32_t max) {
+    if (genome_size <= 3) return 0;   /* ProteinBuilder::init refuses it */
+    orc_reader r = { genome, genome_size, gwords / 2, -1, 0, 0 };
+    uint8_t *buf = (uint8_t *)malloc((size_t)genome_size);
+    int32_t np = 0, n = 0, building = 0;
+    int aa = -1;
+    while (aa != AA_END_TOTAL) {
+        while (!building && aa != AA_END_TOTAL) {
+            aa = reader_aa(&r);
+            if (aa == AA_MET) { building = 1; n = 0; }
+        }
+        while (building && aa != AA_END_TOTAL) {
+            aa = reader_aa(&r);
+            if (aa == AA_STP || aa == AA_END_TOTAL || aa == AA_END_ALLELE) {
+                if (np < max) orc_md5(buf, n, digests + 16 * (size_t)np);
+                np++;
+                building = 0;
+            } else {
+                buf[n++] = (uint8_t)aa;
+            }
+        }
+    }
+    free(buf);
+    return np;
+}
+
+/* ProteomeComparator::countProtHashMatches (ProteomeComparator.cpp:47-56):
+ * every pair of equal digests counts */
+int32_t orc_prot_matches(const uint8_t *a, int32_t na, const uint8_t *b, int32_t nb) {
+    int32_t n = 0;
+    for (int32_t i = 0; i < na; i++)
+        for (int32_t j = 0; j < nb; j++) n += memcmp(a + 16 * (size_t)i, b + 16 * (size_t)j, 16) == 0;
+    return n;
+}
+
 /* a cell's draws of AssortativePairHamming, in order */
 typedef struct { uint64_t seed; uint32_t step; int32_t cell; uint32_t j; } orc_cell_draws;
 static uint32_t cell_next(void *ctx) {
@@ -858,7 +999,9 @@ static uint32_t cell_next(void *ctx) {
  * 1 - calcDist(female, male, numBlocks) / genomeSize as a float, and the
  * RankTable pairing with the cutoff and permute attributes.  The reference
  * passes the block count where calcDist reads a genome size; that call is
- * kept. */
+ * kept.  AssortativePair (QHG4/actions/AssortativePair.cpp:94-271) differs
+ * only in the table: the number of equal protein digests of the two
+ * agents (ProteinBuilder + ProteomeComparator). */
 static void assort_pair(orc *o) {
     int64_t n = o->n;
     int32_t nc = o->ncells;
@@ -877,6 +1020,10 @@ static void assort_pair(orc *o) {
     const int32_t nblocks = (int32_t)(((uint32_t)gsize + 31u) / 32u);
     const float cutoff = (float)o->P.assort_cutoff;
     const int32_t permute = o->P.assort_permute != 0;
+    const int prot = has_action(o, QHG_ACT_ASSORTPROT);
+    /* the most proteins a genome can hold: one per two codons per strand */
+    const int32_t pmax = 2 * ((gsize / 3 + 1) / 2 + 1);
+    md5_init_table();   /* before the threads share it */
 #pragma omp parallel
     {
         int32_t tcap = 0;
@@ -902,13 +1049,28 @@ static void assort_pair(orc *o) {
             }
             if (F == 0 || M == 0) continue;
             if ((size_t)F * M > rcap) { rcap = (size_t)F * M; R = (float *)realloc(R, rcap * 4); }
-            for (int32_t f = 0; f < F; f++)
-                for (int32_t m = 0; m < M; m++) {
-                    int32_t d = 0;
-                    if (o->gw) d = orc_calc_dist(o->genome + (size_t)fem[f] * o->gw, o->genome + (size_t)mal[m] * o->gw,
-                                                 nblocks);
-                    R[(size_t)f * M + m] = (float)(1.0 - (1.0 * d) / gsize);
+            if (prot) {
+                /* every listed agent's digests, then the match counts */
+                int32_t *np = (int32_t *)malloc((size_t)(F + M) * 4);
+                uint8_t *dg = (uint8_t *)malloc((size_t)(F + M) * (size_t)pmax * 16);
+                for (int32_t k = 0; k < F + M; k++) {
+                    int32_t a = (k < F) ? fem[k] : mal[k - F];
+                    np[k] = orc_proteome(o->genome + (size_t)a * o->gw, o->gw, gsize, dg + (size_t)k * pmax * 16, pmax);
                 }
+                for (int32_t f = 0; f < F; f++)
+                    for (int32_t m = 0; m < M; m++)
+                        R[(size_t)f * M + m] = (float)orc_prot_matches(dg + (size_t)f * pmax * 16, np[f],
+                                                                       dg + (size_t)(F + m) * pmax * 16, np[F + m]);
+                free(np); free(dg);
+            } else {
+                for (int32_t f = 0; f < F; f++)
+                    for (int32_t m = 0; m < M; m++) {
+                        int32_t d = 0;
+                        if (o->gw) d = orc_calc_dist(o->genome + (size_t)fem[f] * o->gw,
+                                                     o->genome + (size_t)mal[m] * o->gw, nblocks);
+                        R[(size_t)f * M + m] = (float)(1.0 - (1.0 * d) / gsize);
+                    }
+            }
             orc_cell_draws dr = { o->P.seed, o->step, c, 0 };
             int32_t np = rank_pairs(F, M, R, cutoff, permute, cell_next, &dr, perm, midx, pairs);
             for (int32_t k = 0; k < np; k++) {
@@ -1239,7 +1401,8 @@ int orc_step(orc *o) {
     for (int a = 0; a < o->nact; a++) {
         switch (o->act[a]) {
         case QHG_ACT_RANDOMPAIR: random_pair(o); break;
-        case QHG_ACT_ASSORTPAIR: assort_pair(o); break;
+        case QHG_ACT_ASSORTPAIR:
+        case QHG_ACT_ASSORTPROT: assort_pair(o); break;
         case QHG_ACT_VERHULST: verhulst_init(o, 0); break;
         case QHG_ACT_VERHULSTVARK: verhulst_init(o, 1); break;
         case QHG_ACT_MULTIEVAL: if (o->me_first) { multi_eval(o); o->me_first = 0; } break;

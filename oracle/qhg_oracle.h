@@ -53,6 +53,13 @@ void     orc_pair_cell(int32_t nchoose, int32_t ncand, const uint32_t *u, uint8_
 /* AssortativePairHamming pieces: GeneUtils::calcDist and the RankTable
  * pairing over a row-major nf x nm table with the draws u[0..nu) */
 int32_t  orc_calc_dist(const uint64_t *g1, const uint64_t *g2, int32_t size_arg);
+/* MD5 of len bytes (RFC 1321) */
+void     orc_md5(const uint8_t *msg, int64_t len, uint8_t out[16]);
+/* the MD5 digests of a genome's proteins (ProteinBuilder::translateGenomeHash):
+ * at most max written (16 bytes each); returns the number of proteins */
+int32_t  orc_proteome(const uint64_t *genome, int32_t gwords, int32_t genome_size, uint8_t *digests, int32_t max);
+/* equal digest pairs of two lists (ProteomeComparator::countProtHashMatches) */
+int32_t  orc_prot_matches(const uint8_t *a, int32_t na, const uint8_t *b, int32_t nb);
 int32_t  orc_rank_pairs(int32_t nf, int32_t nm, float *ranks, float cutoff, int32_t permute, const uint32_t *u,
                         int32_t nu, int32_t *pairs, int32_t *used);
 uint64_t orc_multimask(const uint32_t *breaks, int32_t n);

@@ -1,6 +1,6 @@
 // Test infrastructure: C entry points over the reference's own utility code
 // (WELL512 conversions, GeneUtils mutation/recombination, BinomialDist,
-// bino_tools), compiled from the reference sources by `make -C oracle ref`
+// bino_tools, RankTable, ProteinBuilder), compiled from the reference sources by `make -C oracle ref`
 // into oracle/_ref/libqhg_ref.so.  The tests feed these and the oracle the
 // same raw 32-bit draws and compare the results bit for bit; nothing outside
 // tests/ loads this library.
@@ -15,6 +15,7 @@
 
 #include "BinomialDist.h"
 #include "GeneUtils.h"
+#include "ProteinBuilder.h"
 #include "RankTable.h"
 #include "WELL512.h"
 #include "bino_tools.h"
@@ -133,6 +134,23 @@ int ref_rank_pairs(int nf, int nm, const float *ranks, float cutoff, int permute
     delete rt;
     *nraw = (int)w.getCount();
     if (*nraw <= maxraw) raw_words(state, 0, *nraw, raw);
+    return n;
+}
+
+// ProteinBuilder::translateGenomeHash on a genome (QHG4/genes/ProteinBuilder.cpp:
+// 163-220): the MD5 digests of its proteins, 16 bytes each, at most max of
+// them into out (in the builder's order: it sorts the digest pointers);
+// returns the number of proteins, -1 when the builder refuses the genome
+int ref_protein_hashes(const uint64_t *genome, int genome_size, uint8_t *out, int max) {
+    ProteinBuilder *pb = ProteinBuilder::createInstance(reinterpret_cast<const ulong *>(genome), genome_size);
+    if (!pb) return -1;
+    int n = -1;
+    if (pb->translateGenomeHash() == 0) {
+        const prothash &h = pb->getProtHash();
+        n = (int)h.size();
+        for (int i = 0; i < n && i < max; i++) std::memcpy(out + 16 * i, h[i], 16);
+    }
+    delete pb;
     return n;
 }
 

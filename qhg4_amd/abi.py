@@ -25,7 +25,8 @@ ACT_GENETICS = 10
 ACT_OLDAGEDEATH = 11
 ACT_ATANDEATH = 12
 ACT_ASSORTPAIR = 13
-ACT_LAST = ACT_ASSORTPAIR
+ACT_ASSORTPROT = 14
+ACT_LAST = ACT_ASSORTPROT
 
 # environment events (QHG4/utils_qhg/EventConsts.h) and the drowning rule
 EVENT_GEO, EVENT_CLIMATE, EVENT_VEG = 0x01, 0x02, 0x04
@@ -46,6 +47,7 @@ ACTION_NAMES = {
     ACT_OLDAGEDEATH: "OldAgeDeath",
     ACT_ATANDEATH: "ATanDeath",
     ACT_ASSORTPAIR: "AssortativePairHamming",
+    ACT_ASSORTPROT: "AssortativePair",
 }
 
 ME_ADD_SIMPLE, ME_MUL_SIMPLE, ME_MAX_SIMPLE, ME_MIN_SIMPLE = 0, 2, 3, 5

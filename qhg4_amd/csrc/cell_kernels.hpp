@@ -39,6 +39,7 @@
 
 #include "philox.hpp"
 #include "detmath.hpp"
+#include "protein.hpp"
 #include "../../include/qhg_gpu.h"
 
 namespace qhg {
@@ -123,6 +124,8 @@ struct Ctx {
     int32_t *gfu, *gmu, *gfs, *gms, *gmate;   // per-slot list scratch of large cells
     int64_t *gfid, *gmid;
     uint8_t *gflag;
+    uint32_t *gprot;          // AssortativePair: kProtBudget digests (4 words each) per slot
+    int32_t *gnprot;          // ... and the slot's protein count
     uint32_t *cstay, *cbirth;  // per cell: stayers, births (zero outside a step)
     uint32_t *cin;            // ncells * in_w: movers arriving per incoming run
     uint32_t *carr;           // per cell: movers arriving in all (zero outside a step)
@@ -175,6 +178,7 @@ struct Prog {
     int32_t has_assort, assort_permute;
     int32_t assort_words;     // numNucs2Blocks(numBlocks): words per strand position compared
     float assort_cutoff;
+    int32_t assort_prot;      // AssortativePair: matches counted over protein digests
     double bino[QHG_MAX_BINO];
 };
 
@@ -1149,6 +1153,27 @@ __device__ void assort_cell(const CtxT &x, const Prog &p, const StepArgs &sa, in
     wave_sync();
     const int32_t *hf = x.gfu + s, *hm = x.gmu + s;
     uint8_t *pf = x.gflag + s, *pm = x.gflag + s + F;
+    if (p.assort_prot) {
+        // AssortativePair: every listed agent's protein digests into its
+        // slot's share of the pool (translateGenomeHash)
+        for (int32_t j = lane; j < F + M; j += kWave) {
+            const int32_t sl = (j < F) ? x.gfs[s + j] : x.gms[s + (j - F)];
+            const int32_t h = (j < F) ? hf[j] : hm[j - F];
+            const int32_t np = protein_digests(x.gpool + (int64_t)h * x.gwords, x.gwords, p.gen_size,
+                                               x.gprot + (int64_t)sl * (4 * kProtBudget));
+            x.gnprot[sl] = np;
+            if (np > kProtBudget) atomicOr(&x.st->error, kErrProteins);
+        }
+        wave_sync();
+    }
+    // the match of female i and male j (id order)
+    const auto match = [&](int32_t i, int32_t j) -> float {
+        if (!p.assort_prot) return assort_match(x, p, hf[i], hm[j]);
+        const int32_t sf = x.gfs[s + i], sm = x.gms[s + j];
+        const int32_t nf = x.gnprot[sf], nm = x.gnprot[sm];
+        return (float)protein_matches(x.gprot + (int64_t)sf * (4 * kProtBudget), nf < kProtBudget ? nf : kProtBudget,
+                                      x.gprot + (int64_t)sm * (4 * kProtBudget), nm < kProtBudget ? nm : kProtBudget);
+    };
     const float cut = p.assort_cutoff;
     uint32_t dj = 0;   // the cell's draws so far
     const int64_t FM = (int64_t)F * M;
@@ -1158,7 +1183,7 @@ __device__ void assort_cell(const CtxT &x, const Prog &p, const StepArgs &sa, in
         for (int64_t t = lane; t < FM; t += kWave) {
             const int32_t i = (int32_t)(t / M), j = (int32_t)(t - (int64_t)i * M);
             if (pf[i] || pm[j]) continue;
-            const float v = assort_match(x, p, hf[i], hm[j]);
+            const float v = match(i, j);
             if (v >= cut && v >= vmax) vmax = v;
         }
         vmax = wave_max_f32(vmax);
@@ -1170,7 +1195,7 @@ __device__ void assort_cell(const CtxT &x, const Prog &p, const StepArgs &sa, in
             bool reach = false;
             if (i < F && !pf[i]) {
                 for (int32_t j = 0; j < M && !reach; j++)
-                    reach = !pm[j] && assort_match(x, p, hf[i], hm[j]) >= vmax;
+                    reach = !pm[j] && match(i, j) >= vmax;
             }
             const uint64_t mk = __ballot(reach);
             if (reach) x.gfid[s + np + (uint32_t)__popcll(mk & lt)] = i;
@@ -1204,7 +1229,7 @@ __device__ void assort_cell(const CtxT &x, const Prog &p, const StepArgs &sa, in
             uint32_t cnt = 0;
             for (int32_t b = 0; b < M; b += kWave) {
                 const int32_t j = b + lane;
-                const bool ok = j < M && !pm[j] && assort_match(x, p, hf[i], hm[j]) >= vmax;
+                const bool ok = j < M && !pm[j] && match(i, j) >= vmax;
                 cnt += (uint32_t)__popcll(__ballot(ok));
             }
             if (cnt == 0) continue;
@@ -1212,7 +1237,7 @@ __device__ void assort_cell(const CtxT &x, const Prog &p, const StepArgs &sa, in
             int32_t jm = -1;
             for (int32_t b = 0; b < M; b += kWave) {
                 const int32_t j = b + lane;
-                const bool ok = j < M && !pm[j] && assort_match(x, p, hf[i], hm[j]) >= vmax;
+                const bool ok = j < M && !pm[j] && match(i, j) >= vmax;
                 const uint64_t mk = __ballot(ok);
                 const int pc = __popcll(mk);
                 if (sel < pc) { jm = b + select_bit(mk, sel); break; }
@@ -1396,7 +1421,9 @@ __global__ void __launch_bounds__(kWave) k_assort_pair(Ctx x, Prog p, StepArgs s
     wave_sync();
     for (int64_t i = blockIdx.x; i < nocc; i += gridDim.x) {
         const OccCell e = x.occ_cur[i];
-        if (e.n <= (uint32_t)kWave)
+        // the LDS form tabulates Hamming distances; AssortativePair's
+        // protein matches go through the general form
+        if (e.n <= (uint32_t)kWave && !p.assort_prot)
             assort_cell_lds(x, p, sa, e.c, e.s, (int32_t)e.n, lds);
         else
             assort_cell(x, p, sa, e.c, e.s, (int32_t)e.n);

@@ -11,7 +11,7 @@
 //   - the class fixes the actions, their ids and the evaluator wiring
 //     (QHG4/populations/tut_SexualPop.cpp:24-43, tut_EnvironCapAltPop.cpp,
 //     OoAPop.cpp:32-88, OoANavGen2bitPop.cpp:30-100,
-//     OoANavGen2bitAssHamPop.cpp:31-93);
+//     OoANavGen2bitAssHamPop.cpp:31-93, OoANavGen2bitAssPop.cpp:33-95);
 //   - an action's module must hold exactly its attribute names
 //     (Action::checkAttributes, QHG4/actions/Action.cpp:54-75 and
 //     QHG4/core/clsutils.cpp:75-103): a missing or unknown name is an error;
@@ -83,6 +83,13 @@ const std::vector<ClassSpec> &classes() {
          {{"Alt", QHG_EV_ALTITUDE, "AltCapPref", "Multi_weight_alt"},
           {"NPP", QHG_EV_CAPACITY, "NPPPref", "Multi_weight_npp"}},
          {"Navigate"}},
+        {"OoANavGen2bitAssPop",
+         {QHG_ACT_MULTIEVAL, QHG_ACT_WEIGHTEDMOVE, QHG_ACT_VERHULSTVARK, QHG_ACT_ASSORTPROT, QHG_ACT_GETOLD,
+          QHG_ACT_OLDAGEDEATH, QHG_ACT_FERTILITY, QHG_ACT_NPPCAP, QHG_ACT_GENETICS},
+         "Alt+NPP", QHG_ME_ADD_SIMPLE,
+         {{"Alt", QHG_EV_ALTITUDE, "AltCapPref", "Multi_weight_alt"},
+          {"NPP", QHG_EV_CAPACITY, "NPPPref", "Multi_weight_npp"}},
+         {"Navigate"}},
     };
     return c;
 }
@@ -102,6 +109,7 @@ const char *base_name(int kind) {
     case QHG_ACT_OLDAGEDEATH: return "OldAgeDeath";
     case QHG_ACT_ATANDEATH: return "ATanDeath";
     case QHG_ACT_ASSORTPAIR: return "AssortativePairHamming";
+    case QHG_ACT_ASSORTPROT: return "AssortativePair";
     default: return "";
     }
 }
@@ -151,6 +159,9 @@ const std::vector<Attr> &attrs(int kind) {
     // AssortativePairHam.h:11-13 (the cutoff is read as a float)
     static const std::vector<Attr> asp = {{"AssortativePairHamming_cutoff", &qhg_params::assort_cutoff, nullptr},
                                           {"AssortativePairHamming_permute", nullptr, &qhg_params::assort_permute}};
+    // AssortativePair.h:11-13 (the same two values)
+    static const std::vector<Attr> aspp = {{"AssortativePair_cutoff", &qhg_params::assort_cutoff, nullptr},
+                                           {"AssortativePair_permute", nullptr, &qhg_params::assort_permute}};
     switch (kind) {
     case QHG_ACT_FERTILITY: return fert;
     case QHG_ACT_VERHULST: return verh;
@@ -162,6 +173,7 @@ const std::vector<Attr> &attrs(int kind) {
     case QHG_ACT_ATANDEATH: return atd;
     case QHG_ACT_GENETICS: return gen;
     case QHG_ACT_ASSORTPAIR: return asp;
+    case QHG_ACT_ASSORTPROT: return aspp;
     default: return none;
     }
 }

@@ -108,6 +108,8 @@ struct qhg_pop {
     int32_t *d_bfather = nullptr, *d_brec = nullptr;
     int32_t *d_gfu = nullptr, *d_gmu = nullptr, *d_gfs = nullptr, *d_gms = nullptr, *d_gmate = nullptr;
     int64_t *d_gfid = nullptr, *d_gmid = nullptr;
+    uint32_t *d_gprot = nullptr;   // AssortativePair: kProtBudget digests per slot
+    int32_t *d_gnprot = nullptr;
     // genome pool (gcap handles), a second pool for compaction
     uint64_t *d_gpool[2] = {nullptr, nullptr};
     int gp_cur = 0;
@@ -133,6 +135,7 @@ struct qhg_pop {
 
     Prog prog{};
     bool has_pair = false, has_me = false, has_cap = false, has_death = false, has_assort = false;
+    bool has_prot = false;   // the assortative pairing counts protein matches
     bool me_done = false;
     bool acted = false;       // k_cell_act launched for the current step
     bool initialized = false; // initialize phase done for the current step
@@ -199,6 +202,8 @@ struct qhg_pop {
         x.gfid = d_gfid;
         x.gmid = d_gmid;
         x.gflag = d_gflag;
+        x.gprot = d_gprot;
+        x.gnprot = d_gnprot;
         x.cstay = d_cstay;
         x.cbirth = d_cbirth;
         x.cin = d_cin;
@@ -252,7 +257,7 @@ void set_shape(qhg_pop *p) {
 // slots), genome pools, per-cell tables, fixed-size scratch
 int64_t device_bytes(const qhg_pop *p) {
     const int64_t g = p->gwords ? 1 : 0;
-    const int64_t per_slot = 2 * (46 + 4 * g) + 38 + 12 * g;
+    const int64_t per_slot = 2 * (46 + 4 * g) + 38 + 12 * g + (p->has_prot ? 16 * kProtBudget + 4 : 0);
     int64_t b = p->cap * per_slot;
     for (int k = 0; k < 2; k++)
         if (p->d_gpool[k]) b += p->gcap * p->gwords * 8;
@@ -381,12 +386,14 @@ int collect_timing(qhg_pop *p) {
 
 void free_slot_scratch(qhg_pop *p) {
     void *ptrs[] = {p->d_dec, p->d_gflag, p->d_bfather, p->d_brec, p->d_gfu, p->d_gmu, p->d_gfs, p->d_gms,
-                    p->d_gmate, p->d_gfid, p->d_gmid};
+                    p->d_gmate, p->d_gfid, p->d_gmid, p->d_gprot, p->d_gnprot};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     p->d_dec = p->d_gflag = nullptr;
     p->d_bfather = p->d_brec = p->d_gfu = p->d_gmu = p->d_gfs = p->d_gms = p->d_gmate = nullptr;
     p->d_gfid = p->d_gmid = nullptr;
+    p->d_gprot = nullptr;
+    p->d_gnprot = nullptr;
 }
 
 // per-slot scratch: outcome bytes, birth records, and the list scratch of
@@ -406,6 +413,10 @@ int alloc_slot_scratch(qhg_pop *p, int64_t cap) {
     if (p->gwords) {
         HIPCHK(hipMalloc(&p->d_bfather, n * 4));
         HIPCHK(hipMalloc(&p->d_brec, n * 8));
+    }
+    if (p->has_prot) {
+        HIPCHK(hipMalloc(&p->d_gprot, n * 16 * kProtBudget));
+        HIPCHK(hipMalloc(&p->d_gnprot, n * 4));
     }
     return 0;
 }
@@ -468,16 +479,20 @@ int build_prog(qhg_pop *p) {
         g.atan_scale = (kPi / 2 - 0.001) / q;
     }
     p->has_pair = has_kind(P, QHG_ACT_RANDOMPAIR);
-    p->has_assort = has_kind(P, QHG_ACT_ASSORTPAIR);
-    if (p->has_pair && p->has_assort) return fail(QHG_ERR_ARG, "RandomPair and AssortativePairHamming both pair");
+    p->has_prot = has_kind(P, QHG_ACT_ASSORTPROT);
+    p->has_assort = has_kind(P, QHG_ACT_ASSORTPAIR) || p->has_prot;
+    if ((p->has_pair && p->has_assort) || (p->has_prot && has_kind(P, QHG_ACT_ASSORTPAIR)))
+        return fail(QHG_ERR_ARG, "RandomPair, AssortativePairHamming and AssortativePair each pair on their own");
     if (p->has_assort) {
-        if (!has_kind(P, QHG_ACT_GENETICS)) return fail(QHG_ERR_ARG, "AssortativePairHamming needs Genetics");
+        if (!has_kind(P, QHG_ACT_GENETICS))
+            return fail(QHG_ERR_ARG, "AssortativePairHamming and AssortativePair need Genetics");
         // AssortativePairHam::preLoop / findMates (AssortativePairHam.cpp:79-90,
         // :199): calcDist gets the block count where it reads a genome size
         const int32_t blocks = qhg_gpu_genome_words(P.gen_genome_size);
         g.assort_words = qhg_gpu_genome_words(blocks);
         g.assort_cutoff = (float)P.assort_cutoff;
         g.assort_permute = P.assort_permute != 0;
+        g.assort_prot = p->has_prot ? 1 : 0;
         g.has_assort = 1;
         g.need_gender = 1;   // k_cell_act counts the FERTILE agents by sex
     }
@@ -707,6 +722,9 @@ int ensure_capacity(qhg_pop *p) {
             poll_ring(p);
         }
     }
+    if (p->known.error & kErrProteins)
+        return fail(QHG_ERR_CAPACITY, "an agent has more than " + std::to_string(kProtBudget) +
+                                          " proteins (AssortativePair keeps that many digests per agent)");
     if (p->known.error)
         return fail(QHG_ERR_STATE, "device reported an error (flags 0x" + std::to_string(p->known.error) + ")");
     const int kb = (int)(p->known_step & 1);
@@ -1852,7 +1870,8 @@ int qhg_batch_add(qhg_batch *b, qhg_pop *p) {
     if (p->device != b->device) return fail(QHG_ERR_ARG, "population and batch are on different devices");
     if ((int)b->pops.size() >= kMaxPops) return fail(QHG_ERR_CAPACITY, "batch is full (256 populations)");
     if (p->acted || p->initialized) return fail(QHG_ERR_STATE, "population is inside a step");
-    if (p->has_assort) return fail(QHG_ERR_ARG, "AssortativePairHamming populations do not run in batches");
+    if (p->has_assort)
+        return fail(QHG_ERR_ARG, "AssortativePairHamming and AssortativePair populations do not run in batches");
     // hand over: everything the population enqueued so far has finished
     HIPCHK(hipStreamSynchronize(p->stream));
     poll_ring(p);

@@ -21,7 +21,8 @@ _FIELD_OF = {
     "WeightedMove_prob": "wmove_prob", "OAD_max_age": "oad_max_age", "OAD_uncertainty": "oad_uncertainty",
     "ATanDeath_max_age": "atan_max_age", "ATanDeath_range": "atan_range", "ATanDeath_slope": "atan_slope",
     "Genetics_mutation_rate": "gen_mutation_rate", "AssortativePairHamming_cutoff": "assort_cutoff",
-    "AssortativePairHamming_permute": "assort_permute",
+    "AssortativePairHamming_permute": "assort_permute", "AssortativePair_cutoff": "assort_cutoff",
+    "AssortativePair_permute": "assort_permute",
 }
 
 

@@ -102,13 +102,14 @@ def ico_env(grid, seed):
 
 
 def ooa_params(seed, genetics=False, wmove_prob=0.2, b0=0.8, d0=0.001, theta=0.1, k_scale=50.0, assort=None,
-               genome_size=256):
+               genome_size=256, protein=False):
     """VerhulstVarK values are those of QHG4/tutorial_data/xmldat/tut_EnvironCapAlt.xml
     (C2 names none; with C1's b0=0.1, d0=0.05 the paired-female birth rate
     cannot outpace deaths and every run dies out).  assort=(cutoff, permute)
     pairs with AssortativePairHamming instead of RandomPair, as
-    OoANavGen2bitAssHamPop does (needs genetics)."""
-    pair = abi.ACT_RANDOMPAIR if assort is None else abi.ACT_ASSORTPAIR
+    OoANavGen2bitAssHamPop does (needs genetics); with protein, with
+    AssortativePair (protein matches), as OoANavGen2bitAssPop does."""
+    pair = abi.ACT_RANDOMPAIR if assort is None else (abi.ACT_ASSORTPROT if protein else abi.ACT_ASSORTPAIR)
     # the genetics classes (OoANavGen2bitPop, OoANavGen2bitAssHamPop) read
     # their NPP evaluator through a PolyLine (NPPPref; a SingleEvaluator with
     # a PolyLine name needs at least two points, PolyLine.cpp:92-127): a ramp
@@ -137,7 +138,8 @@ def ooa_params(seed, genetics=False, wmove_prob=0.2, b0=0.8, d0=0.001, theta=0.1
         **kw)
 
 
-def c2(seed=42, level=6, n_agents=100000, patch_cells=4000, genetics=False, assort=None, genome_size=256):
+def c2(seed=42, level=6, n_agents=100000, patch_cells=4000, genetics=False, assort=None, genome_size=256,
+       protein=False):
     """100k agents in a BFS patch of land cells.  The patch is 4,000 cells
     (~25 agents per cell, about half of K): a 200-cell patch puts 500 agents
     in cells with K <= 75, where b = b0 + (theta-b0)*N/K < -1 and the
@@ -152,7 +154,7 @@ def c2(seed=42, level=6, n_agents=100000, patch_cells=4000, genetics=False, asso
     rng = np.random.default_rng(seed + 7)
     cells = patch[rng.integers(0, len(patch), n_agents)]
     agents = _agents(rng, cells, n_agents, abi.QHG_LIFE_ALIVE, -1.0)
-    params = ooa_params(seed, genetics=genetics, assort=assort, genome_size=genome_size)
+    params = ooa_params(seed, genetics=genetics, assort=assort, genome_size=genome_size, protein=protein)
     out = {"name": "C2", "grid": grid, "env": env, "params": params, "agents": agents,
            "steps": 2000}
     if genetics:

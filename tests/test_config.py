@@ -148,30 +148,36 @@ def test_navigate_has_no_gpu_implementation():
                                2)
 
 
-def test_assortative_species():
+@pytest.mark.parametrize("cls,act,kind,cutoff", [
+    ("OoANavGen2bitAssHamPop", "AssortativePairHamming", abi.ACT_ASSORTPAIR, 0.95),
+    ("OoANavGen2bitAssPop", "AssortativePair", abi.ACT_ASSORTPROT, 1.0),
+])
+def test_assortative_species(cls, act, kind, cutoff):
     """OoANavGen2bitAssHamPop (QHG4/populations/OoANavGen2bitAssHamPop.cpp:31-93)
-    pairs with AssortativePairHamming, whose module holds exactly its two
-    attributes (QHG4/actions/AssortativePairHam.h:11-13)."""
+    pairs with AssortativePairHamming and OoANavGen2bitAssPop
+    (QHG4/populations/OoANavGen2bitAssPop.cpp:33-95) with AssortativePair;
+    each module holds exactly its two attributes
+    (QHG4/actions/AssortativePairHam.h:11-13, AssortativePair.h:11-13)."""
     xml = open(os.path.join(HERE, "golden", "species_ooa.xml")).read()
-    xml = xml.replace('name="OoAPop"', 'name="OoANavGen2bitAssHamPop"')
-    xml = xml.replace('<prio name="RandomPair" value="7"/>', '<prio name="AssortativePairHamming" value="7"/>')
+    xml = xml.replace('name="OoAPop"', 'name="%s"' % cls)
+    xml = xml.replace('<prio name="RandomPair" value="7"/>', '<prio name="%s" value="7"/>' % act)
     xml = xml.replace("<priorities>", '''<module name="Genetics">
     <param name="Genetics_genome_size" value="256"/><param name="Genetics_num_crossover" value="1"/>
     <param name="Genetics_mutation_rate" value="0.001"/><param name="Genetics_initial_muts" value="random"/>
     <param name="Genetics_create_new_genome" value="1"/><param name="Genetics_bits_per_nuc" value="2"/>
   </module>
-  <module name="AssortativePairHamming">
-    <param name="AssortativePairHamming_cutoff" value="0.95"/><param name="AssortativePairHamming_permute" value="1"/>
+  <module name="%s">
+    <param name="%s_cutoff" value="%s"/><param name="%s_permute" value="1"/>
   </module>
-  <priorities><prio name="Genetics" value="9"/>''')
+  <priorities><prio name="Genetics" value="9"/>''' % (act, act, cutoff, act))
     xml = xml.replace('id="NPP">', 'id="NPP"><param name="NPPPref" value="0 0 1.5 1"/>')
     p = config.params_from_xml(xml, seed=2)
     a = actions(p)
-    assert a["AssortativePairHamming"] == (abi.ACT_ASSORTPAIR, 7) and "RandomPair" not in a
-    assert (p.assort_cutoff, p.assort_permute) == (0.95, 1)
-    assert config.get_attribute(p, "AssortativePairHamming_cutoff") == 0.95
-    with pytest.raises(QhgError, match="missing parameter AssortativePairHamming_permute"):
-        config.params_from_xml(xml.replace('<param name="AssortativePairHamming_permute" value="1"/>', ""), 2)
+    assert a[act] == (kind, 7) and "RandomPair" not in a
+    assert (p.assort_cutoff, p.assort_permute) == (cutoff, 1)
+    assert config.get_attribute(p, act + "_cutoff") == cutoff
+    with pytest.raises(QhgError, match="missing parameter %s_permute" % act):
+        config.params_from_xml(xml.replace('<param name="%s_permute" value="1"/>' % act, ""), 2)
     with pytest.raises(QhgError, match="does not have"):
         config.params_from_xml(xml.replace("<priorities>", '<priorities><prio name="RandomPair" value="3"/>'), 2)
     # the model runs on the oracle

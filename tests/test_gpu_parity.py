@@ -297,3 +297,58 @@ def test_assortative_pairing(cutoff, permute, genome_size, cells, k_max):
     assert (gpu.agents()["mate_id"] >= 0).any() or gpu.stats()["num_alive"] == 0
     with pytest.raises(QhgError, match="AssortativePairHamming"):
         GpuBatch().add(gpu)
+
+
+@pytest.mark.parametrize("cutoff,permute,cells", [
+    (1.0, 1, 150),    # at least one shared protein; ~20 agents per cell, permuted
+    (0.0, 0, 150),    # every couple eligible, no permutation draws
+    (2.0, 1, 40),     # crowded cells (~75 agents): cells past one wave of agents
+])
+def test_protein_assortative_pairing(cutoff, permute, cells):
+    """OoANavGen2bitAssPop's AssortativePair
+    (QHG4/actions/AssortativePair.cpp:94-271): the match of a couple is the
+    number of equal MD5 digests of their proteins (ProteinBuilder,
+    ProteomeComparator), then RankTable pairing; bit-exact with the oracle."""
+    print("case: AssortativePair (protein matches), cutoff %s permute %d, %d patch cells" % (cutoff, permute, cells),
+          flush=True)
+    # seed 110: a base genome of three proteins, so that couples share 0-3
+    sc = scenarios.c2(seed=110, level=4, n_agents=3000, patch_cells=cells, genetics=True,
+                      assort=(cutoff, permute), protein=True)
+    gpu, ora = _lockstep(sc, 20, 5)
+    assert gpu.stats()["births_total"] > 0
+    assert (gpu.agents()["mate_id"] >= 0).any() or gpu.stats()["num_alive"] == 0
+    assert gpu.stats()["device_error"] == 0
+    gpu.close()
+
+
+def test_protein_budget_exceeded_fails_the_step():
+    """An agent with more than 16 proteins (a genome of Met-stop codon pairs)
+    exceeds the digests kept per agent: the step fails with a capacity error
+    naming the limit, instead of pairing on a truncated proteome."""
+    from qhg4_amd._lib import QhgError
+    print("case: AssortativePair with a genome of 86 proteins", flush=True)
+    sc = scenarios.c2(seed=5, level=4, n_agents=2000, patch_cells=60, genetics=True, assort=(0.0, 0), protein=True)
+    # ATG TAA repeated: A=0 C=1 G=2 T=3, nucleotide i in bits 2i of its word
+    codons = [(0, 3, 2), (3, 0, 0)] * 43
+    nucs = [v for c in codons for v in c][:256]
+    words = np.zeros(8, dtype=np.uint64)
+    for i, v in enumerate(nucs):
+        words[i // 32] |= np.uint64(v) << np.uint64(2 * (i % 32))
+    g = sc["agents"]["genome"].reshape(len(sc["agents"]["id"]), -1)
+    g[:, :8] = words
+    g[:, 8:] = words
+    sc["agents"]["genome"] = np.ascontiguousarray(g.ravel())
+    a = sc["agents"]
+    a["life"][:] = abi.QHG_LIFE_FERTILE
+    a["age"][:] = 20.0
+    gpu = GpuPopulation.from_scenario(sc)
+    gpu.preLoop()
+    n0 = gpu.getNumAgentsEffective()
+    gpu.run(2)
+    st = gpu.stats()
+    assert st["device_error"] & 16, st["device_error"]   # the step froze the population
+    assert st["num_alive"] == n0 and st["births_total"] == 0
+    # the host reports it once its status catches up (the status ring holds 8 steps)
+    with pytest.raises(QhgError, match="more than 16 proteins"):
+        gpu.run(20)
+    gpu.close()

@@ -143,13 +143,15 @@ def test_reads_the_references_grid_file():
     assert o.stats()["step"] == 5
 
 
-def test_assortative_params_round_trip(tmp_path):
-    """OoANavGen2bitAssHamPop's parameters, AssortativePairHamming's cutoff
-    and permute flag included, come back byte for byte."""
-    sc = scenarios.c2(seed=6, level=3, n_agents=200, patch_cells=40, genetics=True, assort=(0.93, 1))
+@pytest.mark.parametrize("cls,protein", [("OoANavGen2bitAssHamPop", False), ("OoANavGen2bitAssPop", True)])
+def test_assortative_params_round_trip(tmp_path, cls, protein):
+    """OoANavGen2bitAssHamPop's and OoANavGen2bitAssPop's parameters, the
+    pairing's cutoff and permute flag included, come back byte for byte."""
+    sc = scenarios.c2(seed=6, level=3, n_agents=200, patch_cells=40, genetics=True, assort=(0.93, 1),
+                      protein=protein)
     p = sc["params"]
     f = str(tmp_path / "ass.qdf")
-    qdf.write_agents(f, agents(50, 16), p, pop_class="OoANavGen2bitAssHamPop", num_cells=500, genome_words=16)
+    qdf.write_agents(f, agents(50, 16), p, pop_class=cls, num_cells=500, genome_words=16)
     q = qdf.read_params(f)
     assert (q.assort_cutoff, q.assort_permute) == (0.93, 1)
     assert bytes(q) == bytes(p)

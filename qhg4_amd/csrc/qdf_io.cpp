@@ -611,7 +611,7 @@ int qhg_qdf_read_params(const char *path, const char *species, uint64_t seed, qh
 }
 
 int qhg_qdf_read_agents(const char *path, const char *species, qhg_agents *a) {
-    if (!path || !species || !a || !a->id) return qfail(QHG_ERR_ARG, "null argument");
+    if (!path || !species || !a || (!a->id && a->n > 0)) return qfail(QHG_ERR_ARG, "null argument");
     quiet();
     Hid f(H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose);
     if (!f.ok()) return qfail(QHG_ERR_ARG, std::string("cannot open ") + path);
@@ -683,7 +683,7 @@ int qhg_qdf_write_agent_dataset(int64_t dataset, const qhg_agents *agents, const
 }
 
 int qhg_qdf_read_agent_dataset(int64_t dataset, qhg_agents *agents) {
-    if (dataset < 0 || !agents || !agents->id) return qfail(QHG_ERR_ARG, "null argument");
+    if (dataset < 0 || !agents || (!agents->id && agents->n > 0)) return qfail(QHG_ERR_ARG, "null argument");
     quiet();
     return read_records((hid_t)dataset, agents);
 }

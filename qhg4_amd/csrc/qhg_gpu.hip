@@ -1001,9 +1001,9 @@ int qhg_gpu_destroy(qhg_pop *p) {
 }
 
 int qhg_gpu_upload_agents(qhg_pop *p, const qhg_agents *a) {
-    if (!p || !a || a->n < 0 || !a->id || !a->cell || !a->life || !a->gender)
+    if (!p || !a || a->n < 0 || (a->n > 0 && (!a->id || !a->cell || !a->life || !a->gender)))
         return fail(QHG_ERR_ARG, "upload: id, cell, life and gender are required");
-    if (p->gwords && !a->genome) return fail(QHG_ERR_ARG, "upload: genome required with Genetics");
+    if (p->gwords && a->n > 0 && !a->genome) return fail(QHG_ERR_ARG, "upload: genome required with Genetics");
     std::vector<int64_t> order;
     order.reserve((size_t)a->n);
     for (int64_t i = 0; i < a->n; i++) {

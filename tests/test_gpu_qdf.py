@@ -58,3 +58,41 @@ def test_dump_and_resume(tmp_path, case, k, m):
     ora.run(k + m)
     assert_same_state(got, ora.agents(), "resumed %s vs the oracle" % case)
     assert ora.stats()["max_id"] == s["max_id"]
+
+
+def test_extinction_then_dump_and_resume(tmp_path):
+    """A population that dies out (the reference removes it and stops the run,
+    QHG4/app/Simulator.cpp:303-314, 378-382): the device path reaches zero
+    agents on the oracle's step, further steps are no-ops that still advance
+    the clock, and the empty population dumps and restores."""
+    print("case: C1 48x48 lattice, 800 agents, until extinct", flush=True)
+    sc = scenarios.c1(seed=1, width=48, height=48, n_agents=800)
+    gpu = GpuPopulation.from_scenario(sc)
+    ora = Oracle(sc)
+    gpu.preLoop()
+    ora.pre_loop()
+    gone = None
+    for step in range(1, 301):
+        gpu.run(1)
+        ora.step()
+        n = gpu.getNumAgentsEffective()
+        assert n == ora.stats()["num_alive"], step
+        if n == 0:
+            gone = step
+            break
+    assert gone is not None and gone == 147
+    gpu.run(10)
+    s = gpu.stats()
+    assert s["num_alive"] == 0 and s["step"] == gone + 10 and s["device_error"] == 0
+    assert int(gpu.getNumAgentsArray().sum()) == 0
+    f = str(tmp_path / "empty.qdf")
+    qdf.write_population(gpu, f, pop_class="tut_SexualPop")
+    info = qdf.read_info(f)
+    assert info["num_agents"] == 0 and info["step"] == gone + 10
+    b = GpuPopulation(sc["grid"], sc["env"], qdf.read_params(f))
+    b.preLoop()
+    qdf.restore_population(b, f)
+    b.run(5)
+    assert b.stats()["num_alive"] == 0 and b.stats()["step"] == gone + 15
+    gpu.close()
+    b.close()

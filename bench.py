@@ -57,7 +57,7 @@ HBM_PEAK_GBPS = 8000.0
 # divided by the kernel's bytes_model.h bytes per dispatch over the same
 # steps (profiles/step_kernels_c2_counters.md).  roofline.traffic is this
 # ratio times the timed window's own algorithmic bytes per launch.
-TRAFFIC_PER_ALGO_BYTE = {"k_cell_act": 1.72, "k_cell_scatter": 1.82, "k_cell_scan": 14.2}
+TRAFFIC_PER_ALGO_BYTE = {"k_cell_act": 1.72, "k_cell_scatter": 1.82, "k_cell_scan": 21.0}
 METRIC = "agent-steps/sec at 1 MI355X + %HBM peak; ensemble vs plain speedup, p50/p99"
 
 

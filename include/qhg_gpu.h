@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define QHG_ABI_VERSION 7
+#define QHG_ABI_VERSION 8
 
 /* error codes */
 #define QHG_OK              0
@@ -70,7 +70,11 @@ extern "C" {
                                      (MD5 per protein) and
                                      QHG4/genes/ProteomeComparator.cpp:25-60,
                                      then the same RankTable pairing       */
-#define QHG_ACT_LAST         QHG_ACT_ASSORTPROT
+#define QHG_ACT_NAVIGATE     15   /* QHG4/actions/Navigate.cpp:99-244: sea-way
+                                     jumps from ports and bridge crossings,
+                                     over the tables qhg_gpu_set_navigation
+                                     gives                                  */
+#define QHG_ACT_LAST         QHG_ACT_NAVIGATE
 
 /* MultiEvaluator combination modes, QHG4/actions/MultiEvaluator.h:20-25 */
 #define QHG_ME_ADD_SIMPLE 0
@@ -98,6 +102,8 @@ extern "C" {
 #define QHG_STREAM_GENETICS 7   /* Genetics::makeOffspring draws, in order      */
 #define QHG_STREAM_ASSORT   8   /* AssortativePair(Hamming): one cell's draws in order
                                    (key: cell index | (draw >> 16) << 32, word: draw & 0xffff) */
+#define QHG_STREAM_NAV      9   /* Navigate: word 0 the jump draw, word 1 + b the
+                                   draw of current bridge b                     */
 
 /* words of QHG_STREAM_ACT (one Philox block covers words 0-3) */
 #define QHG_ACTW_WMOVE_GO   0   /* WeightedMove: move?                          */
@@ -176,6 +182,10 @@ typedef struct qhg_params {
     double  assort_cutoff;
     int32_t assort_permute;
     int32_t assort_pad;
+    /* Navigate_decay, Navigate_dist0, Navigate_prob0, Navigate_min_dens
+       (read, not used by the action, as in the reference),
+       Navigate_bridge_prob (0 when a QDF species group lacks it) */
+    double nav_decay, nav_dist0, nav_prob0, nav_min_dens, nav_bridge_prob;
 } qhg_params;
 
 /* Agent records as structure-of-arrays.  Arrays the caller does not need may
@@ -269,6 +279,35 @@ int qhg_gpu_set_env(qhg_pop *pop, const qhg_env_desc *env);
 #define QHG_EVENT_KILL_WET  0x100
 int qhg_gpu_env_event(qhg_pop *pop, const qhg_env_desc *env, int32_t events);
 
+/* Sea-ways and bridges for Navigate: the grid's navigation group as
+ * NavGroupReader::readData / readBridges load it into Navigation::setData /
+ * setBridges (QHG4/io/NavGroupReader.cpp:96-205, QHG4/core/Navigation.cpp
+ * :24-60).  Each port lists its destination cells with their distances;
+ * a bridge joins two cells.  Setting them is EVENT_ID_NAV with its flush:
+ * Navigate's jump tables are rebuilt at once (Navigate::recalculate,
+ * QHG4/actions/Navigate.cpp:99-150), as are its current bridges (both
+ * ends above sea level) after every geography event.  Ports and each
+ * port's destinations may come in any order (the reference keeps them in
+ * ordered maps); a port or a port's destination listed twice is an error,
+ * as NavGroupReader's size check makes it.  A port whose destination
+ * probabilities add up to 1 or more stops the table build there, as in the
+ * reference: ports after it (in cell order) get no table.  The population
+ * keeps its own copy of the arrays.  qhg_gpu_pre_loop fails when Navigate
+ * is scheduled and no navigation was set (Navigate::preLoop,
+ * QHG4/actions/Navigate.cpp:157-183). */
+typedef struct qhg_nav_desc {
+    int32_t        num_ports;
+    const int32_t *ports;          /* num_ports cell indices                  */
+    const int32_t *dest_offsets;   /* num_ports + 1: port k's destinations are
+                                      entries dest_offsets[k] .. [k+1] - 1   */
+    const int32_t *dest_cells;     /* destination cell indices                */
+    const double  *dest_dists;     /* their distances                         */
+    double         sample_dist;    /* Navigation::m_dSampleDist (carried)     */
+    int32_t        num_bridges;    /* at most 131072                          */
+    const int32_t *bridges;        /* num_bridges * 2 cell indices            */
+} qhg_nav_desc;
+int qhg_gpu_set_navigation(qhg_pop *pop, const qhg_nav_desc *nav);
+
 /* observables */
 int qhg_gpu_sync(qhg_pop *pop);
 int qhg_gpu_stats(qhg_pop *pop, qhg_stats *out);
@@ -331,10 +370,10 @@ int qhg_gpu_get_params(qhg_pop *pop, qhg_params *out);
  * Prioritizer::getActionAttributes, QHG4/core/Prioritizer.cpp:440-462;
  * Action::checkAttributes, QHG4/actions/Action.cpp:54-75): the class fixes
  * its actions and evaluator wiring ("tut_SexualPop", "tut_EnvironCapAltPop",
- * "OoAPop", "OoANavGen2bitPop"); every action's module must hold exactly its
- * attribute names; actions without a priority are not scheduled; a
- * priority for an action the class lacks, or one with no device
- * implementation (Navigate), is an error.  Module names are those of the
+ * "OoAPop", "OoANavGen2bitPop", ...); every action's module must hold
+ * exactly its attribute names; actions without a priority are not
+ * scheduled; a priority for an action the class lacks is an error.  Module
+ * names are those of the
  * Prioritizer: "Verhulst", "MultiEvaluator[Alt+NPP]", "SingleEvaluator[Alt]". */
 typedef struct qhg_module_attrs {
     const char *name;
@@ -361,7 +400,7 @@ const char *qhg_params_polyline_name(const char *pop_class, int32_t eval);
 /* The reference's names of a class's actions, as its Prioritizer keys them
  * (Action::getActionName, QHG4/actions/Action.cpp:11-19, e.g. "Verhulst",
  * "MultiEvaluator[Alt+NPP]"): the index-th (NULL past the last), its kind in
- * *kind (0 for an action with no device implementation, e.g. Navigate); and
+ * *kind; and
  * of the MultiEvaluator's evaluators ("SingleEvaluator[Alt]").  The QDF
  * writer names the per-action groups of a species group after them. */
 const char *qhg_params_class_action(const char *pop_class, int32_t index, int32_t *kind);

@@ -24,11 +24,12 @@ def build():
     """Compile the oracle; where the reference sources are present, also
     (into oracle/_ref) the reference utility library the pin tests compare
     against, the reference population run by the statistical comparison
-    (ref_harness) and the reference host that loads the PopBase plugin
-    (plugin_host)."""
+    (ref_harness), the reference host that loads the PopBase plugin
+    (plugin_host) and the reference's Navigate on small scenes
+    (ref_navigate)."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
     if os.path.isdir(REFERENCE_TREE):
-        subprocess.run(["make", "-s", "-C", HERE, "ref", "harness", "plugin_host"], check=True)
+        subprocess.run(["make", "-s", "-C", HERE, "ref", "harness", "plugin_host", "navigate"], check=True)
 
 
 def lib():
@@ -81,6 +82,14 @@ def lib():
         L.orc_proteome.restype = C.c_int32
         L.orc_prot_matches.argtypes = [P(C.c_uint8), C.c_int32, P(C.c_uint8), C.c_int32]
         L.orc_prot_matches.restype = C.c_int32
+        L.orc_set_navigation.argtypes = [C.c_void_p, P(abi.NavDesc)]
+        L.orc_nav_table.argtypes = [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_double), C.c_int32]
+        L.orc_nav_table.restype = C.c_int32
+        L.orc_nav_bridges.argtypes = [C.c_void_p, P(C.c_int32), C.c_int32]
+        L.orc_nav_bridges.restype = C.c_int32
+        L.orc_nav_execute.argtypes = [C.c_void_p, C.c_int32, P(C.c_double), C.c_int32, P(C.c_int32), P(C.c_int32),
+                                      C.c_int32]
+        L.orc_nav_execute.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -106,6 +115,8 @@ class Oracle:
         rc = L.orc_set_agents(self.h, C.byref(s))
         if rc != 0:
             raise RuntimeError("orc_set_agents failed %d" % rc)
+        if sc.get("nav") is not None:
+            self.set_navigation(sc["nav"])
 
     def _has_genetics(self):
         return any(self.params.action_kind[i] == abi.ACT_GENETICS for i in range(self.params.num_actions))
@@ -116,7 +127,37 @@ class Oracle:
             self.h = None
 
     def pre_loop(self):
-        lib().orc_pre_loop(self.h)
+        if lib().orc_pre_loop(self.h) != 0:
+            raise RuntimeError("orc_pre_loop failed")
+
+    def set_navigation(self, nav):
+        """Sea-ways and bridges (see orc_set_navigation)."""
+        self.nav = nav
+        d = abi.nav_struct(nav)
+        if lib().orc_set_navigation(self.h, C.byref(d)) != 0:
+            raise RuntimeError("orc_set_navigation refused the navigation")
+
+    def nav_table(self, cell):
+        """(targets, cumulative probabilities) of the port at cell, or None."""
+        t = np.zeros(4096, dtype=np.int32)
+        p = np.zeros(4096, dtype=np.float64)
+        n = lib().orc_nav_table(self.h, int(cell), abi.ptr(t, C.c_int32), abi.ptr(p, C.c_double), 4096)
+        return None if n < 0 else (t[:n].copy(), p[:n].copy())
+
+    def nav_bridges(self):
+        out = np.zeros(2 * 4096, dtype=np.int32)
+        n = lib().orc_nav_bridges(self.h, abi.ptr(out, C.c_int32), 4096)
+        return [tuple(out[2 * b:2 * b + 2]) for b in range(n)]
+
+    def nav_execute(self, cell, draws):
+        """Navigate::execute at cell over the given draws: (registered
+        destinations, draws taken)."""
+        d = np.ascontiguousarray(np.asarray(draws, dtype=np.float64).reshape(-1))
+        dst = np.zeros(4096, dtype=np.int32)
+        used = C.c_int32()
+        n = lib().orc_nav_execute(self.h, int(cell), abi.ptr(d, C.c_double) if len(d) else None, len(d),
+                                  C.byref(used), abi.ptr(dst, C.c_int32), 4096)
+        return list(dst[:n]), used.value
 
     def set_params(self, params):
         if lib().orc_set_params(self.h, C.byref(params)) != 0:

@@ -17,7 +17,7 @@
  *                      AssortativePairHam (+ QHG4/utils/RankTable.cpp),
  *                      VerhulstVarK,LinearBirth,LinearDeath,RandomMove,
  *                      WeightedMove,SingleEvaluator,MultiEvaluator,
- *                      NPPCapacity,Genetics}.cpp
+ *                      NPPCapacity,Genetics,Navigate}.cpp
  *   genome ops         QHG4/genes/GeneUtils.cpp:112-360
  *   binomial table     QHG4/utils/BinomialDist.cpp:60-104, bino_tools.cpp
  *   PolyLine           QHG4/utils/PolyLine.cpp:60-89
@@ -291,6 +291,21 @@ struct orc {
     int64_t  agent_steps;
     /* pairing scratch */
     int32_t *pc_off, *pc_buf;
+    /* navigation: Navigation::m_mDestinations (ports and each port's
+     * destinations in increasing cell order) and m_vBridges */
+    int32_t  nav_set, nav_nports, nav_nbridges;
+    int32_t *nav_port, *nav_doff, *nav_dcell, *nav_bridge;
+    double  *nav_ddist;
+    /* Navigate's state: m_mJumpProbs (a table per port, reached through the
+     * port's cell) and m_vCurBridges */
+    int32_t  nav_need_update, nav_preloop_done;
+    int32_t *jp_index;            /* ncells: the port's table, -1 for none */
+    int32_t *jp_n, *jp_off;       /* per table: destinations, first entry  */
+    int32_t *jp_targ;             /* entries: n + 1 per table, stay first  */
+    double  *jp_prob;
+    int32_t  njp, njp_entries;
+    int32_t *cur_bridge;          /* 2 per current bridge */
+    int32_t  ncur_bridges;
 };
 
 static int g_threads = 0;
@@ -361,6 +376,9 @@ int orc_create(const qhg_grid_desc *g, const qhg_env_desc *env, const qhg_params
     o->count = (uint32_t *)calloc(nc, 4);
     o->pc_off = (int32_t *)calloc(nc + 1, 4);
     o->me_first = 1;
+    o->nav_need_update = 1;
+    o->jp_index = (int32_t *)malloc(nc * sizeof(int32_t));
+    for (size_t c = 0; c < nc; c++) o->jp_index[c] = -1;
     sort_actions(o);
     o->gw = 0;
     if (has_action(o, QHG_ACT_GENETICS)) {
@@ -381,7 +399,8 @@ void orc_destroy(orc *o) {
                      o->temp, o->rain, o->ice, o->coastal, o->K, o->W, o->single, o->bc,
                      o->dc, o->count, o->pc_off, o->pc_buf, o->id, o->cell, o->mate,
                      o->nbabies, o->dest, o->life, o->gender, o->bflag, o->birth, o->age,
-                     o->lastb, o->genome };
+                     o->lastb, o->genome, o->nav_port, o->nav_doff, o->nav_dcell, o->nav_bridge,
+                     o->nav_ddist, o->jp_index, o->jp_n, o->jp_off, o->jp_targ, o->jp_prob, o->cur_bridge };
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     free(o);
 }
@@ -614,6 +633,251 @@ int orc_set_params(orc *o, const qhg_params *p) {
     return 0;
 }
 
+/* ------------------------------------------------------------------ */
+/* Navigate (QHG4/actions/Navigate.cpp) over the sea-ways and bridges  */
+/* of Navigation (QHG4/core/Navigation.h:12-40)                        */
+/* ------------------------------------------------------------------ */
+
+/* Navigate::cleanup (Navigate.cpp:63-72): drop the tables and bridges */
+static void nav_cleanup(orc *o) {
+    for (int32_t c = 0; c < o->ncells; c++) o->jp_index[c] = -1;
+    o->njp = 0;
+    o->njp_entries = 0;
+    o->ncur_bridges = 0;
+}
+
+/* Navigate::recalculate (Navigate.cpp:95-148), run when an update is due:
+ * A = prob0 / exp(decay * dist0); per port (increasing cell index, the
+ * distancemap's order) the probabilities A * exp(decay * dist) of its
+ * destinations (increasing cell index, the distlist's order), summed in that
+ * order; with a sum below 1 the table is [stay = 1 - sum, destinations...]
+ * made cumulative, otherwise the build stops at that port.  The current
+ * bridges are those with both ends above sea level. */
+static void nav_recalculate(orc *o) {
+    if (!o->nav_need_update) return;
+    const qhg_params *P = &o->P;
+    const double A = P->nav_prob0 / exp(P->nav_decay * P->nav_dist0);
+    nav_cleanup(o);
+    int32_t total = o->nav_nports + (o->nav_nports ? o->nav_doff[o->nav_nports] : 0);
+    o->jp_n = (int32_t *)realloc(o->jp_n, (size_t)(o->nav_nports + 1) * sizeof(int32_t));
+    o->jp_off = (int32_t *)realloc(o->jp_off, (size_t)(o->nav_nports + 1) * sizeof(int32_t));
+    o->jp_targ = (int32_t *)realloc(o->jp_targ, (size_t)(total + 1) * sizeof(int32_t));
+    o->jp_prob = (double *)realloc(o->jp_prob, (size_t)(total + 1) * sizeof(double));
+    for (int32_t k = 0; k < o->nav_nports; k++) {
+        const int32_t d0 = o->nav_doff[k], n = o->nav_doff[k + 1] - d0;
+        int32_t *targ = o->jp_targ + o->njp_entries;
+        double *prob = o->jp_prob + o->njp_entries;
+        double sum = 0;
+        for (int32_t j = 0; j < n; j++) {
+            double pr = A * exp(P->nav_decay * o->nav_ddist[d0 + j]);
+            sum += pr;
+            targ[j + 1] = o->nav_dcell[d0 + j];
+            prob[j + 1] = pr;
+        }
+        if (sum < 1) {
+            targ[0] = -1;
+            prob[0] = 1 - sum;
+            for (int32_t j = 1; j < n + 1; j++) prob[j] += prob[j - 1];
+            o->jp_n[o->njp] = n;
+            o->jp_off[o->njp] = o->njp_entries;
+            o->jp_index[o->nav_port[k]] = o->njp;
+            o->njp++;
+            o->njp_entries += n + 1;
+        } else {
+            fprintf(stderr, "[Navigate] probabilities for port [%d] add up to %f\n", o->nav_port[k], sum);
+            break;
+        }
+    }
+    o->cur_bridge = (int32_t *)realloc(o->cur_bridge, (size_t)(2 * o->nav_nbridges + 2) * sizeof(int32_t));
+    for (int32_t b = 0; b < o->nav_nbridges; b++) {
+        const int32_t a0 = o->nav_bridge[2 * b], a1 = o->nav_bridge[2 * b + 1];
+        if (o->alt && o->alt[a0] > 0 && o->alt[a1] > 0) {
+            o->cur_bridge[2 * o->ncur_bridges] = a0;
+            o->cur_bridge[2 * o->ncur_bridges + 1] = a1;
+            o->ncur_bridges++;
+        }
+    }
+    o->nav_need_update = 0;
+}
+
+/* Navigation::setData / setBridges (QHG4/core/Navigation.cpp:24-60) as
+ * NavGroupReader fills them: ports and destinations sorted as the maps
+ * hold them; a repeated port or destination fails (NavGroupReader's size
+ * check).  Before the population's pre-loop this is the grid's data;
+ * afterwards it is EVENT_ID_NAV with its flush (Navigate::notify,
+ * Navigate.cpp:79-89): the tables are rebuilt at once. */
+int orc_set_navigation(orc *o, const qhg_nav_desc *nav) {
+    if (!nav || nav->num_ports < 0 || nav->num_bridges < 0 || nav->num_bridges > 131072) return QHG_ERR_ARG;
+    const int32_t np = nav->num_ports;
+    if (np && (!nav->ports || !nav->dest_offsets)) return QHG_ERR_ARG;
+    const int32_t nd = np ? nav->dest_offsets[np] : 0;
+    if (nd && (!nav->dest_cells || !nav->dest_dists)) return QHG_ERR_ARG;
+    if (nav->num_bridges && !nav->bridges) return QHG_ERR_ARG;
+    int64_t *key = (int64_t *)malloc((size_t)(np + 1) * sizeof(int64_t));
+    for (int32_t k = 0; k < np; k++) {
+        if (nav->ports[k] < 0 || nav->ports[k] >= o->ncells || nav->dest_offsets[k] < 0 ||
+            nav->dest_offsets[k + 1] < nav->dest_offsets[k]) { free(key); return QHG_ERR_ARG; }
+        key[k] = ((int64_t)nav->ports[k] << 32) | k;
+    }
+    /* ports in cell order (insertion sort of the keys) */
+    for (int32_t i = 1; i < np; i++) {
+        int64_t v = key[i]; int32_t j = i - 1;
+        while (j >= 0 && key[j] > v) { key[j + 1] = key[j]; j--; }
+        key[j + 1] = v;
+    }
+    for (int32_t k = 1; k < np; k++)
+        if ((key[k] >> 32) == (key[k - 1] >> 32)) { free(key); return QHG_ERR_ARG; }
+    int32_t *port = (int32_t *)malloc((size_t)(np + 1) * sizeof(int32_t));
+    int32_t *doff = (int32_t *)malloc((size_t)(np + 1) * sizeof(int32_t));
+    int32_t *dcell = (int32_t *)malloc((size_t)(nd + 1) * sizeof(int32_t));
+    double *ddist = (double *)malloc((size_t)(nd + 1) * sizeof(double));
+    int32_t w = 0, bad = 0;
+    for (int32_t i = 0; i < np && !bad; i++) {
+        const int32_t k = (int32_t)(key[i] & 0xffffffff);
+        port[i] = nav->ports[k];
+        doff[i] = w;
+        const int32_t a = nav->dest_offsets[k], n = nav->dest_offsets[k + 1] - a;
+        /* destinations in cell order: selection of the next larger cell */
+        int32_t last = -1;
+        for (int32_t j = 0; j < n && !bad; j++) {
+            int32_t best = -1;
+            for (int32_t q = 0; q < n; q++) {
+                const int32_t c = nav->dest_cells[a + q];
+                if (c > last && (best < 0 || c < nav->dest_cells[a + best])) best = q;
+            }
+            if (best < 0) { bad = 1; break; }    /* a destination listed twice */
+            const int32_t c = nav->dest_cells[a + best];
+            if (c < 0 || c >= o->ncells) { bad = 1; break; }
+            dcell[w] = c;
+            ddist[w] = nav->dest_dists[a + best];
+            w++;
+            last = c;
+        }
+    }
+    doff[np] = w;
+    for (int32_t b = 0; b < 2 * nav->num_bridges && !bad; b++)
+        if (nav->bridges[b] < 0 || nav->bridges[b] >= o->ncells) bad = 1;
+    free(key);
+    if (bad) { free(port); free(doff); free(dcell); free(ddist); return QHG_ERR_ARG; }
+    free(o->nav_port); free(o->nav_doff); free(o->nav_dcell); free(o->nav_ddist); free(o->nav_bridge);
+    o->nav_port = port;
+    o->nav_doff = doff;
+    o->nav_dcell = dcell;
+    o->nav_ddist = ddist;
+    o->nav_nports = np;
+    o->nav_nbridges = nav->num_bridges;
+    o->nav_bridge = (int32_t *)malloc((size_t)(2 * nav->num_bridges + 2) * sizeof(int32_t));
+    if (nav->num_bridges) memcpy(o->nav_bridge, nav->bridges, (size_t)nav->num_bridges * 2 * sizeof(int32_t));
+    o->nav_set = 1;
+    o->nav_need_update = 1;
+    if (o->nav_preloop_done) nav_recalculate(o);
+    return 0;
+}
+
+/* Navigate::execute (Navigate.cpp:187-244) for an agent at cell c that is
+ * alive and has no move registered: from a port, draw dR and walk the
+ * cumulative table while dR exceeds the entry, at most up to index c (the
+ * reference bounds the walk by the map key, the port's cell index, where
+ * the destination count was meant) and at most to the last destination;
+ * entry i > 0 registers a move to its cell unless that cell is ice.  Then
+ * for every current bridge with an end at c, in list order: one draw, and a
+ * move to the other end when it is below Navigate_bridge_prob.  draw(ctx, b)
+ * gives the jump draw (b = -1) or the draw of current bridge b.  Writes the
+ * first max registered destinations to dst and the last one to *last;
+ * returns the number of registrations. */
+static int32_t nav_execute(const orc *o, int32_t c, double (*draw)(void *, int32_t), void *ctx, int32_t *last,
+                           int32_t *dst, int32_t max) {
+    int32_t nreg = 0;
+    const int32_t t = o->jp_index[c];
+    if (t >= 0) {
+        const int32_t n = o->jp_n[t];
+        const double *cum = o->jp_prob + o->jp_off[t];
+        const double dR = draw(ctx, -1);
+        int32_t i = 0;
+        while (i < c && i < n && dR > cum[i]) i++;
+        if (i > 0) {
+            const int32_t nc = o->jp_targ[o->jp_off[t] + i];
+            if (!o->ice || !o->ice[nc]) {
+                if (nreg < max) dst[nreg] = nc;
+                nreg++;
+                *last = nc;
+            }
+        }
+    }
+    for (int32_t b = 0; b < o->ncur_bridges; b++) {
+        const int32_t a0 = o->cur_bridge[2 * b], a1 = o->cur_bridge[2 * b + 1];
+        int32_t nc = -1;
+        if (a0 == c) nc = a1;
+        else if (a1 == c) nc = a0;
+        if (nc >= 0) {
+            const double dR = draw(ctx, b);
+            if (dR < o->P.nav_bridge_prob) {
+                if (nreg < max) dst[nreg] = nc;
+                nreg++;
+                *last = nc;
+            }
+        }
+    }
+    return nreg;
+}
+
+typedef struct { uint64_t seed; uint32_t step; uint64_t id; } nav_philox;
+
+static double nav_draw_philox(void *ctx, int32_t b) {
+    const nav_philox *q = (const nav_philox *)ctx;
+    return orc_wrandd(orc_draw(q->seed, q->step, QHG_STREAM_NAV, q->id, (uint32_t)(b + 1)));
+}
+
+typedef struct { const double *seq; int32_t n, used; } nav_seq;
+
+static double nav_draw_seq(void *ctx, int32_t b) {
+    nav_seq *q = (nav_seq *)ctx;
+    (void)b;
+    return (q->used < q->n) ? q->seq[q->used++] : 2.0;
+}
+
+static void act_navigate(orc *o) {
+    int64_t moves = 0;
+#pragma omp parallel for schedule(static) reduction(+:moves)
+    for (int64_t i = 0; i < o->n; i++) {
+        /* alive and not already moving (LIFE_STATE_MOVING: a move registered) */
+        if (o->life[i] == 0 || o->dest[i] >= 0) continue;
+        nav_philox q = { o->P.seed, o->step, (uint64_t)o->id[i] };
+        int32_t last = -1;
+        const int32_t k = nav_execute(o, o->cell[i], nav_draw_philox, &q, &last, NULL, 0);
+        if (k) { o->dest[i] = last; moves += k; }
+    }
+    o->moves_last += moves;
+}
+
+/* the pieces above, for the tests */
+int32_t orc_nav_table(orc *o, int32_t cell, int32_t *targ, double *cum, int32_t max) {
+    if (cell < 0 || cell >= o->ncells || o->jp_index[cell] < 0) return -1;
+    const int32_t t = o->jp_index[cell], n = o->jp_n[t] + 1;
+    for (int32_t j = 0; j < n && j < max; j++) {
+        targ[j] = o->jp_targ[o->jp_off[t] + j];
+        cum[j] = o->jp_prob[o->jp_off[t] + j];
+    }
+    return n;
+}
+
+int32_t orc_nav_bridges(orc *o, int32_t *pairs, int32_t max) {
+    for (int32_t b = 0; b < o->ncur_bridges && b < max; b++) {
+        pairs[2 * b] = o->cur_bridge[2 * b];
+        pairs[2 * b + 1] = o->cur_bridge[2 * b + 1];
+    }
+    return o->ncur_bridges;
+}
+
+int32_t orc_nav_execute(orc *o, int32_t cell, const double *draws, int32_t ndraws, int32_t *used, int32_t *dst,
+                        int32_t max) {
+    nav_seq q = { draws, ndraws, 0 };
+    int32_t last = -1;
+    const int32_t k = nav_execute(o, cell, nav_draw_seq, &q, &last, dst, max);
+    *used = q.used;
+    return k;
+}
+
 /* Environment events between steps: replace the given fields; with
  * QHG_EVENT_KILL_WET remove every agent in a cell with altitude < 0 or ice
  * (OoAPop::updateEvent, QHG4/populations/OoAPop.cpp:164-196: registerDeath
@@ -666,11 +930,25 @@ int orc_env_event(orc *o, const qhg_env_desc *env, int32_t events) {
     }
     if (has_action(o, QHG_ACT_NPPCAP)) npp_capacity(o);
     if (has_action(o, QHG_ACT_MULTIEVAL)) { multi_eval(o); o->me_first = 0; }
+    /* Navigate::notify: a geography event makes an update due, the flush
+     * runs it (once the pre-loop has built the first tables) */
+    if ((events & QHG_EVENT_GEO) || env->altitude) {
+        o->nav_need_update = 1;
+        if (o->nav_preloop_done) nav_recalculate(o);
+    }
     return 0;
 }
 
 int orc_pre_loop(orc *o) {
     if (has_action(o, QHG_ACT_NPPCAP)) npp_capacity(o);
+    if (has_action(o, QHG_ACT_NAVIGATE)) {
+        /* Navigate::preLoop (Navigate.cpp:155-183): needs the geography and
+         * the navigation; the tables are built here */
+        if (!o->nav_set || !o->alt) return QHG_ERR_ARG;
+        o->nav_need_update = 1;
+        nav_recalculate(o);
+    }
+    o->nav_preloop_done = 1;
     update_counts(o);
     return 0;
 }
@@ -1420,6 +1698,7 @@ int orc_step(orc *o) {
         case QHG_ACT_WEIGHTEDMOVE: act_weightedmove(o); break;
         case QHG_ACT_OLDAGEDEATH: act_oldagedeath(o, t); break;
         case QHG_ACT_ATANDEATH: act_atandeath(o, t); break;
+        case QHG_ACT_NAVIGATE: act_navigate(o); break;
         default: break;
         }
     }

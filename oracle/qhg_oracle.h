@@ -31,6 +31,7 @@ int      orc_get_agents(orc *o, qhg_agents *a);
 int      orc_pre_loop(orc *o);
 int      orc_set_params(orc *o, const qhg_params *p);
 int      orc_env_event(orc *o, const qhg_env_desc *env, int32_t events);
+int      orc_set_navigation(orc *o, const qhg_nav_desc *nav);
 int      orc_step(orc *o);
 int      orc_run(orc *o, int32_t nsteps);
 void     orc_stats(orc *o, qhg_stats *s);
@@ -76,6 +77,16 @@ double   orc_polyline(int32_t npts, const double *x, const double *v, double X);
 double   orc_atan(double x);
 double   orc_exp(double x);
 int32_t  orc_genome_words(int32_t genome_size);
+/* Navigate pieces: the jump table of the port at cell (n + 1 entries, the
+ * stay-home entry first, cumulative; -1 when the cell has none), the
+ * current bridges (pairs), and Navigate::execute for an agent at cell over
+ * the given draws in the order it takes them (jump first, then one per
+ * bridge at the cell): returns the number of registered moves, their
+ * destinations in dst (at most max) and the draws taken in *used */
+int32_t  orc_nav_table(orc *o, int32_t cell, int32_t *targ, double *cum, int32_t max);
+int32_t  orc_nav_bridges(orc *o, int32_t *pairs, int32_t max);
+int32_t  orc_nav_execute(orc *o, int32_t cell, const double *draws, int32_t ndraws, int32_t *used, int32_t *dst,
+                         int32_t max);
 
 #ifdef __cplusplus
 }

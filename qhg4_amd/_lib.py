@@ -17,7 +17,7 @@ EXPORTS = [
     "qhg_gpu_abi_version", "qhg_gpu_last_error", "qhg_gpu_genome_words", "qhg_gpu_binomial_table",
     "qhg_gpu_create", "qhg_gpu_destroy", "qhg_gpu_upload_agents", "qhg_gpu_num_alive",
     "qhg_gpu_download_agents", "qhg_gpu_pre_loop", "qhg_gpu_initialize_step", "qhg_gpu_do_actions",
-    "qhg_gpu_finalize_step", "qhg_gpu_run", "qhg_gpu_set_env", "qhg_gpu_env_event", "qhg_gpu_sync", "qhg_gpu_stats",
+    "qhg_gpu_finalize_step", "qhg_gpu_run", "qhg_gpu_set_env", "qhg_gpu_env_event", "qhg_gpu_set_navigation", "qhg_gpu_sync", "qhg_gpu_stats",
     "qhg_gpu_cell_counts", "qhg_gpu_capacities", "qhg_gpu_env_weights", "qhg_gpu_compact", "qhg_gpu_state_digest",
     "qhg_gpu_set_timing", "qhg_gpu_kernel_times", "qhg_gpu_kernel_name", "qhg_gpu_device_bytes",
     "qhg_gpu_set_tuning", "qhg_gpu_get_tuning", "qhg_gpu_autotune",
@@ -64,6 +64,7 @@ def lib():
         "qhg_gpu_run": ([h, C.c_int32], C.c_int),
         "qhg_gpu_set_env": ([h, P(abi.EnvDesc)], C.c_int),
         "qhg_gpu_env_event": ([h, P(abi.EnvDesc), C.c_int32], C.c_int),
+        "qhg_gpu_set_navigation": ([h, P(abi.NavDesc)], C.c_int),
         "qhg_params_event_flags": ([C.c_char_p, C.c_int32], C.c_int),
         "qhg_gpu_sync": ([h], C.c_int),
         "qhg_gpu_stats": ([h, P(abi.Stats)], C.c_int),

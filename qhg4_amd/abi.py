@@ -26,7 +26,8 @@ ACT_OLDAGEDEATH = 11
 ACT_ATANDEATH = 12
 ACT_ASSORTPAIR = 13
 ACT_ASSORTPROT = 14
-ACT_LAST = ACT_ASSORTPROT
+ACT_NAVIGATE = 15
+ACT_LAST = ACT_NAVIGATE
 
 # environment events (QHG4/utils_qhg/EventConsts.h) and the drowning rule
 EVENT_GEO, EVENT_CLIMATE, EVENT_VEG = 0x01, 0x02, 0x04
@@ -48,6 +49,7 @@ ACTION_NAMES = {
     ACT_ATANDEATH: "ATanDeath",
     ACT_ASSORTPAIR: "AssortativePairHamming",
     ACT_ASSORTPROT: "AssortativePair",
+    ACT_NAVIGATE: "Navigate",
 }
 
 ME_ADD_SIMPLE, ME_MUL_SIMPLE, ME_MAX_SIMPLE, ME_MIN_SIMPLE = 0, 2, 3, 5
@@ -58,7 +60,7 @@ MAX_EVALS = 4
 MAX_PL_POINTS = 64
 NAME_LEN = 48
 
-STREAM_PAIR, STREAM_ACT, STREAM_NEWBORN, STREAM_GENETICS, STREAM_ASSORT = 1, 2, 6, 7, 8
+STREAM_PAIR, STREAM_ACT, STREAM_NEWBORN, STREAM_GENETICS, STREAM_ASSORT, STREAM_NAV = 1, 2, 6, 7, 8, 9
 ACTW_WMOVE_GO, ACTW_WMOVE_DIR, ACTW_LBIRTH, ACTW_LDEATH, ACTW_RMOVE_GO, ACTW_RMOVE_DIR = 0, 1, 2, 3, 4, 5
 
 
@@ -130,7 +132,59 @@ class Params(C.Structure):
         ("assort_cutoff", C.c_double),
         ("assort_permute", C.c_int32),
         ("assort_pad", C.c_int32),
+        ("nav_decay", C.c_double),
+        ("nav_dist0", C.c_double),
+        ("nav_prob0", C.c_double),
+        ("nav_min_dens", C.c_double),
+        ("nav_bridge_prob", C.c_double),
     ]
+
+
+class NavDesc(C.Structure):
+    """qhg_nav_desc: ports with their destinations and distances, bridges."""
+    _fields_ = [
+        ("num_ports", C.c_int32),
+        ("ports", C.POINTER(C.c_int32)),
+        ("dest_offsets", C.POINTER(C.c_int32)),
+        ("dest_cells", C.POINTER(C.c_int32)),
+        ("dest_dists", C.POINTER(C.c_double)),
+        ("sample_dist", C.c_double),
+        ("num_bridges", C.c_int32),
+        ("bridges", C.POINTER(C.c_int32)),
+    ]
+
+
+def nav_struct(nav):
+    """A NavDesc pointing into a navigation dict (ports, dest_offsets,
+    dest_cells, dest_dists: numpy arrays; bridges: (B, 2) int32; sample_dist).
+    The dict's arrays must outlive the struct."""
+    d = NavDesc()
+    d.num_ports = len(nav["ports"])
+    d.ports = ptr(nav["ports"], C.c_int32)
+    d.dest_offsets = ptr(nav["dest_offsets"], C.c_int32)
+    d.dest_cells = ptr(nav["dest_cells"], C.c_int32)
+    d.dest_dists = ptr(nav["dest_dists"], C.c_double)
+    d.sample_dist = float(nav.get("sample_dist", 0.0))
+    d.num_bridges = len(nav["bridges"])
+    d.bridges = ptr(nav["bridges"], C.c_int32)
+    return d
+
+
+def nav_arrays(ports, bridges=(), sample_dist=0.0):
+    """Navigation dict from {port cell: [(dest cell, distance), ...]} and a
+    list of (cell, cell) bridges."""
+    keys = list(ports)
+    off = np.zeros(len(keys) + 1, dtype=np.int32)
+    cells, dists = [], []
+    for k, c in enumerate(keys):
+        for d, dist in ports[c]:
+            cells.append(d)
+            dists.append(dist)
+        off[k + 1] = len(cells)
+    return {"ports": np.array(keys, dtype=np.int32), "dest_offsets": off,
+            "dest_cells": np.array(cells, dtype=np.int32), "dest_dists": np.array(dists, dtype=np.float64),
+            "sample_dist": float(sample_dist),
+            "bridges": np.ascontiguousarray(np.array(bridges, dtype=np.int32).reshape(-1, 2))}
 
 
 class Agents(C.Structure):

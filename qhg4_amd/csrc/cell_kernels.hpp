@@ -67,7 +67,9 @@ constexpr int kParts = 3;             // per-block partial sums of k_cell_act
 // outcome byte of an agent in a step
 constexpr uint8_t kDecBirth = 0x80;   // registered a birth
 constexpr uint8_t kDecDead = 0x40;    // died
-constexpr uint8_t kDecDirMask = 0x07; // 0 stays, d+1 moves to neighbour slot d
+constexpr uint8_t kDecDirMask = 0x07; // 0 stays, d+1 moves to neighbour slot d,
+constexpr uint8_t kDecNav = 0x07;     //   kDecNav moves along a Navigate edge
+constexpr int kNavDir = kDecNav - 1;  // the move direction of a navigating mover
 
 struct DevStatus {
     int64_t n_alive[2];      // live agents at the start of a step of this parity
@@ -101,6 +103,24 @@ struct AgentBuf {
     double *lastb;
     int32_t *nbabies;
     int32_t *ghandle;        // genome pool slot (Genetics only)
+};
+
+// Navigate's tables on the device (n == 0: no Navigate, or no edge).  Every
+// cell's edges are its sea-way jumps (the port's table, destination order)
+// followed by one edge per current bridge end at the cell (bridge order).
+struct NavDev {
+    int32_t n;                // edges
+    const int32_t *off;       // ncells + 1: each cell's edges
+    const int32_t *njump;     // ncells: the jump edges among them
+    const double *stay;       // ncells: stay-home entry of the port's cumulative table
+    const int32_t *dst;       // n: target cell
+    const double *cum;        // n: cumulative probability (jump edges)
+    const int32_t *bid;       // n: index of the current bridge (bridge edges)
+    const int32_t *in_off;    // ncells + 1: edges into each cell
+    const int32_t *in_e;      // ... their indices, increasing
+    uint32_t *cnt;            // n: movers per edge this step (zero outside a step)
+    uint32_t *rpos;           // n: next-buffer slot of each edge's run
+    uint64_t *sel;            // per slot: edge | rank << 32 of a navigating mover
 };
 
 struct Ctx {
@@ -149,6 +169,7 @@ struct Ctx {
     DevStatus *st;
     DevStatus *hring;         // host-mapped ring of kStatusRing snapshots
     uint64_t seed;
+    NavDev nav;
 };
 
 struct StepArgs {
@@ -179,6 +200,7 @@ struct Prog {
     int32_t assort_words;     // numNucs2Blocks(numBlocks): words per strand position compared
     float assort_cutoff;
     int32_t assort_prot;      // AssortativePair: matches counted over protein digests
+    double nav_bridge_p;      // Navigate_bridge_prob
     double bino[QHG_MAX_BINO];
 };
 
@@ -364,6 +386,7 @@ struct ActItem {
     bool age_dirty;
     uint32_t bflag;
     int32_t npend;            // registered moves (<= 2), resolved after the program
+    int32_t navq;             // 1 + moves registered before Navigate ran, 0: it did not
     int32_t pk0, pk1;         // their action kinds
     uint32_t pu0, pu1;        // their target draws
     uint32_t u[8];            // words 0..7 of the agent's QHG_STREAM_ACT counter
@@ -379,9 +402,9 @@ struct CellView {             // the per-cell values every agent of the cell rea
 // loaded every field).  A dead agent stops.  Moves are only registered here
 // (their decision draws are taken now) and resolved afterwards, as
 // performMoves applies them after all actions.
-// kDeaths: the program may hold OldAgeDeath / ATanDeath (a separate
-// instantiation keeps their code out of the common kernel)
-template <bool kDeaths, typename CtxT, typename ProgT>
+// kRare: the program may hold OldAgeDeath or ATanDeath; kNav: Navigate
+// (separate instantiations keep their code out of the common kernel)
+template <bool kRare, bool kNav = false, typename CtxT, typename ProgT>
 __device__ __forceinline__ void act_program(const CtxT &x, const ProgT &p, const StepArgs &sa, const CellView &cv,
                                             ActItem &a, uint32_t &deaths) {
     const double t = sa.t;
@@ -435,13 +458,17 @@ __device__ __forceinline__ void act_program(const CtxT &x, const ProgT &p, const
                 if (a.npend == 0) { a.pk0 = op; a.pu0 = dir; a.npend = 1; }
                 else if (a.npend == 1) { a.pk1 = op; a.pu1 = dir; a.npend = 2; }
             }
-        } else if (kDeaths && op == QHG_ACT_OLDAGEDEATH) {
+        } else if (kRare && op == QHG_ACT_OLDAGEDEATH) {
             // OldAgeDeath::execute: age, then death past max age + U(1-uM, 1+uM)
             a.age = t - a.birth;
             a.age_dirty = true;
             const double r = wrandr(1 - p.oad_unc * p.oad_max, 1 + p.oad_unc * p.oad_max, a.u[QHG_ACTW_OADEATH]);
             if (a.age > p.oad_max + r) { a.life = 0; deaths++; break; }
-        } else if (kDeaths && op == QHG_ACT_ATANDEATH) {
+        } else if (kNav && op == QHG_ACT_NAVIGATE) {
+            // Navigate::execute runs once the moves registered so far are
+            // known (below): only an agent none of them moves navigates
+            a.navq = a.npend + 1;
+        } else if (kRare && op == QHG_ACT_ATANDEATH) {
             // ATanDeath::execute (ATanDeath.cpp:66-90): age, then death with
             // probability 1/2 + scale * atan(slope * (age - max age)) / pi
             a.age = t - a.birth;
@@ -485,6 +512,107 @@ __device__ __forceinline__ int move_dir(const CtxT &x, int kind, uint32_t u, int
     return idx - 1;
 }
 
+// Navigate::execute (QHG4/actions/Navigate.cpp:187-244) for an agent at
+// cell c, alive and with no move registered: from a port, walk the
+// cumulative table while the draw exceeds the entry, at most up to index c
+// (the reference bounds the walk by the map key, the port's cell index) and
+// at most to the last destination; entry i > 0 registers a move to its cell
+// unless that cell is ice.  Then every current bridge with an end at c
+// takes a draw and registers a move to the other end when it is below
+// Navigate_bridge_prob.  Each registration counts as a move; the last wins.
+template <typename CtxT, typename ProgT>
+__device__ void nav_execute(const CtxT &x, const ProgT &p, const StepArgs &sa, int64_t id, int32_t c, int &dir,
+                            int32_t &nave, uint32_t &moves) {
+    const int32_t e0 = x.nav.off[c], e1 = x.nav.off[c + 1];
+    if (e0 == e1) return;
+    const int32_t nj = x.nav.njump[c];
+    if (nj > 0) {
+        const double r = wrandd(draw(x.seed, sa.step, QHG_STREAM_NAV, (uint64_t)id, 0));
+        int32_t i = 0;
+        double cum = x.nav.stay[c];
+        while (i < c && i < nj && r > cum) {
+            cum = x.nav.cum[e0 + i];
+            i++;
+        }
+        if (i > 0) {
+            const int32_t e = e0 + i - 1;
+            const int32_t d = x.nav.dst[e];
+            if (x.ice == nullptr || !x.ice[d]) {
+                dir = kNavDir;
+                nave = e;
+                moves++;
+            }
+        }
+    }
+    for (int32_t e = e0 + nj; e < e1; e++) {
+        const double r = wrandd(draw(x.seed, sa.step, QHG_STREAM_NAV, (uint64_t)id, 1u + (uint32_t)x.nav.bid[e]));
+        if (r < p.nav_bridge_p) {
+            dir = kNavDir;
+            nave = e;
+            moves++;
+        }
+    }
+}
+
+// The registered moves in program order, each counting as a move and the
+// last one winning (performMoves applies them in order,
+// QHG4/core/SPopulation.cpp:1058-1092).  Navigate takes its place in the
+// program and acts only when no move was registered before it
+// (LIFE_STATE_MOVING, Navigate.cpp:191).  w, nbr: the cell's weight and
+// neighbour rows.
+template <bool kNav, typename CtxT, typename ProgT>
+__device__ __forceinline__ void resolve_moves(const CtxT &x, const ProgT &p, const StepArgs &sa, const ActItem &a,
+                                              int32_t c, int nn, const double (&w)[7], const int32_t (&nbr)[6],
+                                              int &dir, int32_t &nave, uint32_t &moves) {
+    if constexpr (!kNav) {
+        int d = move_dir(x, a.pk0, a.pu0, nn, w, nbr);
+        if (d >= 0) { dir = d; moves++; }
+        if (a.npend > 1) {
+            d = move_dir(x, a.pk1, a.pu1, nn, w, nbr);
+            if (d >= 0) { dir = d; moves++; }
+        }
+    } else {
+        for (int q = 0; q <= a.npend; q++) {
+            if (a.navq == q + 1 && dir < 0 && x.nav.n) nav_execute(x, p, sa, a.id, c, dir, nave, moves);
+            if (q < a.npend) {
+                const int d = move_dir(x, q == 0 ? a.pk0 : a.pk1, q == 0 ? a.pu0 : a.pu1, nn, w, nbr);
+                if (d >= 0) { dir = d; moves++; }
+            }
+        }
+    }
+}
+
+// Places one chunk's navigating movers in their edges' runs (called by all
+// lanes).  One wave owns every edge of its cells and meets a cell's agents
+// in slot order, chunk by chunk, so the movers of an edge rank in slot
+// order: after those of earlier chunks (the edge's count so far), then in
+// lane order.  Each mover's edge and rank go to nav.sel for k_cell_scatter;
+// the edge's count (nav.cnt, read by k_cell_scan) and the target cell's
+// arrivals grow by the chunk's movers.
+template <typename CtxT>
+__device__ __forceinline__ void nav_movers(const CtxT &x, bool mine, int32_t nave, uint32_t slot) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lanemask_lt();
+    uint64_t left = __ballot(mine);
+    while (left) {
+        const int l0 = __ffsll((unsigned long long)left) - 1;
+        const int32_t e = __shfl(nave, l0, 64);
+        const uint64_t same = __ballot(mine && nave == e);
+        uint32_t base = 0;
+        if (lane == l0) {
+            const uint32_t k = (uint32_t)__popcll(same);
+            base = atomicAdd(&x.nav.cnt[e], k);
+            const int32_t d = x.nav.dst[e];
+            atomicAdd(&x.carr[d], k);
+            x.tact[d / kCellTile] = 1u;
+        }
+        base = (uint32_t)__shfl((int)base, l0, 64);
+        if ((same >> lane) & 1ull)
+            x.nav.sel[slot] = (uint64_t)(uint32_t)e | ((uint64_t)(base + (uint32_t)__popcll(same & lt)) << 32);
+        left &= ~same;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // k_cell_act: pairing + fused actions, one wave per occupied cell
 // ---------------------------------------------------------------------------
@@ -507,7 +635,7 @@ __device__ __forceinline__ void id_order(const int64_t *ids, int32_t n, int32_t 
     }
 }
 
-template <bool kDeaths, bool kAssort = false, typename CtxT, typename ProgT>
+template <bool kRare, bool kAssort = false, bool kNav = false, typename CtxT, typename ProgT>
 __device__ __forceinline__ void cell_act(const CtxT &x, const ProgT &p, const StepArgs &sa, int32_t c, uint32_t s,
                                          int32_t n, const PairLists &L, uint32_t &my_deaths, uint32_t &my_moves,
                                          uint32_t &my_fert) {
@@ -614,6 +742,7 @@ __device__ __forceinline__ void cell_act(const CtxT &x, const ProgT &p, const St
         a.age_dirty = false;
         a.bflag = 0;
         a.npend = 0;
+        a.navq = 0;
         a.pk0 = a.pk1 = 0;
         a.pu0 = a.pu1 = 0;
         if (valid) {
@@ -632,9 +761,10 @@ __device__ __forceinline__ void cell_act(const CtxT &x, const ProgT &p, const St
         }
         const uint8_t lifeS = a.life;
         int dir = -1;
+        int32_t nave = 0;
         if (valid && a.life != 0) {
-            act_program<kDeaths>(x, p, sa, cv, a, my_deaths);
-            if (a.npend > 0) {
+            act_program<kRare, kNav>(x, p, sa, cv, a, my_deaths);
+            if (a.npend > 0 || (kNav && a.navq)) {
                 // the cell's neighbour and weight rows, for movers only
                 int32_t nbr[6];
                 double w[7];
@@ -642,16 +772,11 @@ __device__ __forceinline__ void cell_act(const CtxT &x, const ProgT &p, const St
                 for (int qq = 0; qq < 6; qq++) nbr[qq] = (qq < x.conn) ? x.nbr[(int64_t)c * x.conn + qq] : -1;
 #pragma unroll
                 for (int qq = 0; qq < 7; qq++) w[qq] = (qq < R) ? x.W[(int64_t)c * R + qq] : 0.0;
-                // every registered move counts; the last one wins
-                int d = move_dir(x, a.pk0, a.pu0, cv.nn, w, nbr);
-                if (d >= 0) { dir = d; my_moves++; }
-                if (a.npend > 1) {
-                    d = move_dir(x, a.pk1, a.pu1, cv.nn, w, nbr);
-                    if (d >= 0) { dir = d; my_moves++; }
-                }
+                resolve_moves<kNav>(x, p, sa, a, c, cv.nn, w, nbr, dir, nave, my_moves);
             }
         }
         const bool alive = valid && a.life != 0;
+        if (kNav && x.nav.n) nav_movers(x, alive && dir == kNavDir, nave, slot);
         if (valid) {
             uint8_t code = (uint8_t)(dir >= 0 ? dir + 1 : 0);
             if (!alive) code |= kDecDead;
@@ -722,7 +847,7 @@ union ActLds {
 // The cells of a group are consecutive occupied cells, so their segments
 // form one slot range [S0, S0+T).  Lane k < ng holds cell k's entry e and
 // eb = e.s - S0.  Each cell's lists live at its own positions [eb, eb+n).
-template <bool kDeaths, bool kAssort = false, typename CtxT, typename ProgT>
+template <bool kRare, bool kAssort = false, bool kNav = false, typename CtxT, typename ProgT>
 __device__ __forceinline__ void group_act(const CtxT &x, const ProgT &p, const StepArgs &sa, int ng, const OccCell &e,
                                           uint32_t eb, uint32_t S0, uint32_t T, GroupActLds &G,
                                           uint32_t &my_deaths, uint32_t &my_moves, uint32_t &my_fert) {
@@ -904,6 +1029,7 @@ __device__ __forceinline__ void group_act(const CtxT &x, const ProgT &p, const S
         a.age_dirty = false;
         a.bflag = 0;
         a.npend = 0;
+        a.navq = 0;
         a.pk0 = a.pk1 = 0;
         a.pu0 = a.pu1 = 0;
         int32_t nbab = (ch == 0) ? nbab0 : nbab1;
@@ -933,25 +1059,22 @@ __device__ __forceinline__ void group_act(const CtxT &x, const ProgT &p, const S
         }
         const uint8_t lifeS = a.life;
         int dir = -1;
+        int32_t nave = 0;
+        const int32_t cc = kNav ? __shfl(e.c, k, 64) : 0;   // the agent's cell (Navigate)
         if (valid && a.life != 0) {
-            act_program<kDeaths>(x, p, sa, cv, a, my_deaths);
-            if (a.npend > 0) {
+            act_program<kRare, kNav>(x, p, sa, cv, a, my_deaths);
+            if (a.npend > 0 || (kNav && a.navq)) {
                 int32_t nbr[6];
                 double w[7];
 #pragma unroll
                 for (int qq = 0; qq < 6; qq++) nbr[qq] = (qq < x.conn) ? G.nbr[k][qq] : -1;
 #pragma unroll
                 for (int qq = 0; qq < 7; qq++) w[qq] = (qq < R) ? G.W[k][qq] : 0.0;
-                // every registered move counts; the last one wins
-                int d = move_dir(x, a.pk0, a.pu0, cv.nn, w, nbr);
-                if (d >= 0) { dir = d; my_moves++; }
-                if (a.npend > 1) {
-                    d = move_dir(x, a.pk1, a.pu1, cv.nn, w, nbr);
-                    if (d >= 0) { dir = d; my_moves++; }
-                }
+                resolve_moves<kNav>(x, p, sa, a, cc, cv.nn, w, nbr, dir, nave, my_moves);
             }
         }
         const bool alive = valid && a.life != 0;
+        if (kNav && x.nav.n) nav_movers(x, alive && dir == kNavDir, nave, slot);
         if (valid) {
             uint8_t code = (uint8_t)(dir >= 0 ? dir + 1 : 0);
             if (!alive) code |= kDecDead;
@@ -965,7 +1088,7 @@ __device__ __forceinline__ void group_act(const CtxT &x, const ProgT &p, const S
                 if (x.gwords) x.bfather[slot] = (int32_t)(S0 + a.mate);
                 atomicAdd(&G.cnt[k][kCntBirth], 1u);
             }
-            if (alive) atomicAdd(&G.cnt[k][dir < 0 ? kCntStay : kCntMv + dir], 1u);
+            if (alive && (!kNav || dir != kNavDir)) atomicAdd(&G.cnt[k][dir < 0 ? kCntStay : kCntMv + dir], 1u);
         }
     }
     wave_sync();
@@ -986,7 +1109,7 @@ __device__ __forceinline__ void group_act(const CtxT &x, const ProgT &p, const S
     wave_sync();
 }
 
-template <bool kDeaths, bool kAssort>
+template <bool kRare, bool kAssort, bool kNav = false>
 __global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs sa) {
     __shared__ ActLds lds;
     const int lane = threadIdx.x;
@@ -1029,7 +1152,7 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs
         const uint32_t T = readlane_u32(e.s, ng - 1) + readlane_u32(e.n, ng - 1) - S0;
         const uint32_t nmax = wave_max_u32(lane < ng ? e.n : 0u);
         if (T <= (uint32_t)kGroupSlots && nmax <= (uint32_t)kGroupCellMax) {
-            group_act<kDeaths, kAssort>(x, p, sa, ng, e, e.s - S0, S0, T, lds.g, my_deaths, my_moves, my_fert);
+            group_act<kRare, kAssort, kNav>(x, p, sa, ng, e, e.s - S0, S0, T, lds.g, my_deaths, my_moves, my_fert);
             continue;
         }
         for (int k = 0; k < ng; k++) {
@@ -1044,7 +1167,7 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs
                 L = PairLists{lds.c.fu, lds.c.mu, lds.c.fid, lds.c.mid, lds.c.fs, lds.c.ms, lds.c.mate,
                               lds.c.flag};
             }
-            cell_act<kDeaths, kAssort>(x, p, sa, c, s, n, L, my_deaths, my_moves, my_fert);
+            cell_act<kRare, kAssort, kNav>(x, p, sa, c, s, n, L, my_deaths, my_moves, my_fert);
         }
     }
     const uint64_t dm = wave_sum_u64(((uint64_t)my_moves << 32) | my_deaths);
@@ -1627,6 +1750,19 @@ __device__ __forceinline__ void tile_write(const CtxT &x, const StepArgs &sa, ui
                     pos += m[k];
                 }
             }
+            if (x.nav.n) {
+                // Navigate's runs follow, in edge order
+                const int32_t j1 = x.nav.in_off[c + 1];
+                for (int32_t j = x.nav.in_off[c]; j < j1; j++) {
+                    const int32_t ne = x.nav.in_e[j];
+                    const uint32_t mn = x.nav.cnt[ne];
+                    if (mn) {
+                        x.nav.rpos[ne] = pos;
+                        x.nav.cnt[ne] = 0;
+                        pos += mn;
+                    }
+                }
+            }
         }
         if (I.birth) {
             x.bpos[c] = pos;
@@ -1830,6 +1966,11 @@ __device__ __forceinline__ void cell_scatter(const CtxT &x, const StepArgs &sa, 
             if (mine) to = (int64_t)rp[d] + nrun[d] + __popcll(mm & lt);
             nrun[d] += __popcll(mm);
         }
+        if (alive && dir == kDecNav) {
+            // a Navigate edge's run: its slot and the rank k_cell_act gave it
+            const uint64_t ns = x.nav.sel[from];
+            to = (int64_t)x.nav.rpos[(uint32_t)ns] + (uint32_t)(ns >> 32);
+        }
         if (to >= 0) {
             x.nxt.id[to] = id;
             x.nxt.mate_id[to] = mate;
@@ -1970,6 +2111,10 @@ __device__ __forceinline__ void group_scatter(const CtxT &x, const StepArgs &sa,
             const uint64_t mm = __ballot(mine);
             if (mine) to = (int64_t)rp[d] + G.cnt[k][kScMv + d] + __popcll(mm & lt & cm);
             add[1 + d] = __popcll(mm & cm);
+        }
+        if (alive && dir == kDecNav) {
+            const uint64_t ns = x.nav.sel[from];
+            to = (int64_t)x.nav.rpos[(uint32_t)ns] + (uint32_t)(ns >> 32);
         }
         if (to >= 0) {
             x.nxt.id[to] = id;
@@ -2410,7 +2555,7 @@ __device__ __forceinline__ void batch_groups(const BatchArgs &B, int64_t bstep, 
     }
 }
 
-template <bool kDeaths>
+template <bool kRare>
 __global__ void __launch_bounds__(kCellBlock) k_batch_act(BatchArgs B) {
     __shared__ ActLds lds;
     const int lane = threadIdx.x;
@@ -2444,7 +2589,7 @@ __global__ void __launch_bounds__(kCellBlock) k_batch_act(BatchArgs B) {
             const uint32_t T = readlane_u32(e.s, ng - 1) + readlane_u32(e.n, ng - 1) - S0;
             const uint32_t nmax = wave_max_u32(lane < ng ? e.n : 0u);
             if (T <= (uint32_t)kGroupSlots && nmax <= (uint32_t)kGroupCellMax) {
-                group_act<kDeaths>(x, pg, sa, ng, e, e.s - S0, S0, T, lds.g, deaths, moves, fert);
+                group_act<kRare>(x, pg, sa, ng, e, e.s - S0, S0, T, lds.g, deaths, moves, fert);
                 continue;
             }
             for (int k = 0; k < ng; k++) {
@@ -2459,7 +2604,7 @@ __global__ void __launch_bounds__(kCellBlock) k_batch_act(BatchArgs B) {
                     L = PairLists{lds.c.fu, lds.c.mu, lds.c.fid, lds.c.mid, lds.c.fs, lds.c.ms, lds.c.mate,
                                   lds.c.flag};
                 }
-                cell_act<kDeaths>(x, pg, sa, c, s, n, L, deaths, moves, fert);
+                cell_act<kRare>(x, pg, sa, c, s, n, L, deaths, moves, fert);
             }
         }
         const uint64_t dm = wave_sum_u64(((uint64_t)moves << 32) | deaths);

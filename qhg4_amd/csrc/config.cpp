@@ -71,25 +71,25 @@ const std::vector<ClassSpec> &classes() {
          {}},
         {"OoANavGen2bitPop",
          {QHG_ACT_MULTIEVAL, QHG_ACT_WEIGHTEDMOVE, QHG_ACT_VERHULSTVARK, QHG_ACT_RANDOMPAIR, QHG_ACT_GETOLD,
-          QHG_ACT_OLDAGEDEATH, QHG_ACT_FERTILITY, QHG_ACT_NPPCAP, QHG_ACT_GENETICS},
+          QHG_ACT_OLDAGEDEATH, QHG_ACT_FERTILITY, QHG_ACT_NPPCAP, QHG_ACT_GENETICS, QHG_ACT_NAVIGATE},
          "Alt+NPP", QHG_ME_ADD_SIMPLE,
          {{"Alt", QHG_EV_ALTITUDE, "AltCapPref", "Multi_weight_alt"},
           {"NPP", QHG_EV_CAPACITY, "NPPPref", "Multi_weight_npp"}},
-         {"Navigate"}},
+         {}},
         {"OoANavGen2bitAssHamPop",
          {QHG_ACT_MULTIEVAL, QHG_ACT_WEIGHTEDMOVE, QHG_ACT_VERHULSTVARK, QHG_ACT_ASSORTPAIR, QHG_ACT_GETOLD,
-          QHG_ACT_OLDAGEDEATH, QHG_ACT_FERTILITY, QHG_ACT_NPPCAP, QHG_ACT_GENETICS},
+          QHG_ACT_OLDAGEDEATH, QHG_ACT_FERTILITY, QHG_ACT_NPPCAP, QHG_ACT_GENETICS, QHG_ACT_NAVIGATE},
          "Alt+NPP", QHG_ME_ADD_SIMPLE,
          {{"Alt", QHG_EV_ALTITUDE, "AltCapPref", "Multi_weight_alt"},
           {"NPP", QHG_EV_CAPACITY, "NPPPref", "Multi_weight_npp"}},
-         {"Navigate"}},
+         {}},
         {"OoANavGen2bitAssPop",
          {QHG_ACT_MULTIEVAL, QHG_ACT_WEIGHTEDMOVE, QHG_ACT_VERHULSTVARK, QHG_ACT_ASSORTPROT, QHG_ACT_GETOLD,
-          QHG_ACT_OLDAGEDEATH, QHG_ACT_FERTILITY, QHG_ACT_NPPCAP, QHG_ACT_GENETICS},
+          QHG_ACT_OLDAGEDEATH, QHG_ACT_FERTILITY, QHG_ACT_NPPCAP, QHG_ACT_GENETICS, QHG_ACT_NAVIGATE},
          "Alt+NPP", QHG_ME_ADD_SIMPLE,
          {{"Alt", QHG_EV_ALTITUDE, "AltCapPref", "Multi_weight_alt"},
           {"NPP", QHG_EV_CAPACITY, "NPPPref", "Multi_weight_npp"}},
-         {"Navigate"}},
+         {}},
     };
     return c;
 }
@@ -110,6 +110,7 @@ const char *base_name(int kind) {
     case QHG_ACT_ATANDEATH: return "ATanDeath";
     case QHG_ACT_ASSORTPAIR: return "AssortativePairHamming";
     case QHG_ACT_ASSORTPROT: return "AssortativePair";
+    case QHG_ACT_NAVIGATE: return "Navigate";
     default: return "";
     }
 }
@@ -162,6 +163,14 @@ const std::vector<Attr> &attrs(int kind) {
     // AssortativePair.h:11-13 (the same two values)
     static const std::vector<Attr> aspp = {{"AssortativePair_cutoff", &qhg_params::assort_cutoff, nullptr},
                                            {"AssortativePair_permute", nullptr, &qhg_params::assort_permute}};
+    // Navigate.cpp:18-24 (tryGetAttributes requires all five through
+    // checkAttributes; a QDF file may lack the bridge probability, 0 then,
+    // extractAttributesQDF, Navigate.cpp:263-302)
+    static const std::vector<Attr> nav = {{"Navigate_decay", &qhg_params::nav_decay, nullptr},
+                                          {"Navigate_dist0", &qhg_params::nav_dist0, nullptr},
+                                          {"Navigate_prob0", &qhg_params::nav_prob0, nullptr},
+                                          {"Navigate_min_dens", &qhg_params::nav_min_dens, nullptr},
+                                          {"Navigate_bridge_prob", &qhg_params::nav_bridge_prob, nullptr}};
     switch (kind) {
     case QHG_ACT_FERTILITY: return fert;
     case QHG_ACT_VERHULST: return verh;
@@ -174,6 +183,7 @@ const std::vector<Attr> &attrs(int kind) {
     case QHG_ACT_GENETICS: return gen;
     case QHG_ACT_ASSORTPAIR: return asp;
     case QHG_ACT_ASSORTPROT: return aspp;
+    case QHG_ACT_NAVIGATE: return nav;
     default: return none;
     }
 }
@@ -423,6 +433,7 @@ int qhg_params_from_attribute_map(const char *pop_class, int32_t n, const char *
     std::map<std::string, std::string> kv;
     for (int32_t k = 0; k < n; k++)
         if (keys[k] && values[k]) kv[keys[k]] = values[k];
+    if (kv.count("Navigate_decay") && !kv.count("Navigate_bridge_prob")) kv["Navigate_bridge_prob"] = "0";
     std::vector<std::pair<std::string, std::vector<std::pair<std::string, std::string>>>> mods;
     auto add = [&](const std::string &module, const char *key) {
         auto it = kv.find(key);

@@ -171,8 +171,7 @@ int put_kind_attrs(hid_t loc, const qhg_params *P, int kind) {
 // named as its Prioritizer names it, holding the action's attributes; the
 // MultiEvaluator's group holds its weights and one group per evaluator with
 // the evaluator's PolyLine (MultiEvaluator::writeAttributesQDF,
-// QHG4/actions/MultiEvaluator.cpp:515-540).  Actions with no device
-// implementation (Navigate) get no group.
+// QHG4/actions/MultiEvaluator.cpp:515-540).
 int put_action_groups(hid_t g, const char *pop_class, const qhg_params *P) {
     for (int i = 0;; i++) {
         int32_t kind = 0;

@@ -133,8 +133,23 @@ struct qhg_pop {
     unsigned long long *d_digest = nullptr;   // 2 sums of qhg_gpu_state_digest (allocated at its first call)
     DevStatus *d_st = nullptr;
 
+    // Navigate: the navigation as set (ports in cell order, each port's
+    // destinations in cell order), the altitudes the current bridges are
+    // chosen by, and the device tables built from them (NavDev)
+    bool nav_set = false, nav_preloop_done = false;
+    std::vector<int32_t> nav_port, nav_doff, nav_dcell, nav_bridge;
+    std::vector<double> nav_ddist, h_alt;
+    int32_t nnav = 0;
+    int32_t *d_nav_off = nullptr, *d_nav_njump = nullptr, *d_nav_dst = nullptr, *d_nav_bid = nullptr;
+    int32_t *d_nav_in_off = nullptr, *d_nav_in_e = nullptr;
+    double *d_nav_stay = nullptr, *d_nav_cum = nullptr;
+    uint32_t *d_nav_cnt = nullptr, *d_nav_rpos = nullptr;
+    uint64_t *d_nav_sel = nullptr;   // per slot (slot scratch)
+
     Prog prog{};
-    bool has_pair = false, has_me = false, has_cap = false, has_death = false, has_assort = false;
+    bool has_pair = false, has_me = false, has_cap = false, has_assort = false;
+    bool has_rare = false;   // OldAgeDeath or ATanDeath: k_cell_act<true, ...>
+    bool has_nav = false;    // Navigate is scheduled
     bool has_prot = false;   // the assortative pairing counts protein matches
     bool me_done = false;
     bool acted = false;       // k_cell_act launched for the current step
@@ -227,6 +242,18 @@ struct qhg_pop {
         x.st = d_st;
         x.hring = d_ring;
         x.seed = P.seed;
+        x.nav.n = nnav;
+        x.nav.off = d_nav_off;
+        x.nav.njump = d_nav_njump;
+        x.nav.stay = d_nav_stay;
+        x.nav.dst = d_nav_dst;
+        x.nav.cum = d_nav_cum;
+        x.nav.bid = d_nav_bid;
+        x.nav.in_off = d_nav_in_off;
+        x.nav.in_e = d_nav_in_e;
+        x.nav.cnt = d_nav_cnt;
+        x.nav.rpos = d_nav_rpos;
+        x.nav.sel = d_nav_sel;
         return x;
     }
 
@@ -257,7 +284,8 @@ void set_shape(qhg_pop *p) {
 // slots), genome pools, per-cell tables, fixed-size scratch
 int64_t device_bytes(const qhg_pop *p) {
     const int64_t g = p->gwords ? 1 : 0;
-    const int64_t per_slot = 2 * (46 + 4 * g) + 38 + 12 * g + (p->has_prot ? 16 * kProtBudget + 4 : 0);
+    const int64_t per_slot = 2 * (46 + 4 * g) + 38 + 12 * g + (p->has_prot ? 16 * kProtBudget + 4 : 0) +
+                             (p->has_nav ? 8 : 0);
     int64_t b = p->cap * per_slot;
     for (int k = 0; k < 2; k++)
         if (p->d_gpool[k]) b += p->gcap * p->gwords * 8;
@@ -269,6 +297,7 @@ int64_t device_bytes(const qhg_pop *p) {
     if (p->d_ice) per_cell += 1;
     if (p->d_coastal) per_cell += 1;
     b += nc * per_cell + 3 * 4;
+    if (p->has_nav) b += nc * (4 + 4 + 8 + 4) + 8 + (int64_t)p->nnav * (4 + 8 + 4 + 4 + 4 + 4);
     b += 2 * (int64_t)p->ntiles * 8 + (int64_t)p->ntiles * 4 + (int64_t)p->num_cu * kMaxBlocksPerCu * kParts * 4 +
          (int64_t)p->ntiles * (kScanBlock / kWave) * 3 * 8 +
          (int64_t)sizeof(DevStatus);
@@ -386,9 +415,10 @@ int collect_timing(qhg_pop *p) {
 
 void free_slot_scratch(qhg_pop *p) {
     void *ptrs[] = {p->d_dec, p->d_gflag, p->d_bfather, p->d_brec, p->d_gfu, p->d_gmu, p->d_gfs, p->d_gms,
-                    p->d_gmate, p->d_gfid, p->d_gmid, p->d_gprot, p->d_gnprot};
+                    p->d_gmate, p->d_gfid, p->d_gmid, p->d_gprot, p->d_gnprot, p->d_nav_sel};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
+    p->d_nav_sel = nullptr;
     p->d_dec = p->d_gflag = nullptr;
     p->d_bfather = p->d_brec = p->d_gfu = p->d_gmu = p->d_gfs = p->d_gms = p->d_gmate = nullptr;
     p->d_gfid = p->d_gmid = nullptr;
@@ -418,6 +448,7 @@ int alloc_slot_scratch(qhg_pop *p, int64_t cap) {
         HIPCHK(hipMalloc(&p->d_gprot, n * 16 * kProtBudget));
         HIPCHK(hipMalloc(&p->d_gnprot, n * 4));
     }
+    if (p->has_nav) HIPCHK(hipMalloc(&p->d_nav_sel, n * 8));
     return 0;
 }
 
@@ -498,7 +529,9 @@ int build_prog(qhg_pop *p) {
     }
     p->has_me = has_kind(P, QHG_ACT_MULTIEVAL);
     p->has_cap = has_kind(P, QHG_ACT_NPPCAP);
-    p->has_death = has_kind(P, QHG_ACT_OLDAGEDEATH) || has_kind(P, QHG_ACT_ATANDEATH);
+    p->has_nav = has_kind(P, QHG_ACT_NAVIGATE);
+    p->has_rare = has_kind(P, QHG_ACT_OLDAGEDEATH) || has_kind(P, QHG_ACT_ATANDEATH);
+    g.nav_bridge_p = P.nav_bridge_prob;
     g.has_pair = p->has_pair;
     for (int q = 0; q < g.nops; q++) {
         switch (g.ops[q]) {
@@ -533,6 +566,126 @@ int build_prog(qhg_pop *p) {
         for (int e = 0; e < P.me_num; e++)
             if (P.me_pl_n[e] < 0 || P.me_pl_n[e] > QHG_MAX_PL_POINTS) return fail(QHG_ERR_ARG, "PolyLine size");
     }
+    return 0;
+}
+
+void free_nav_tables(qhg_pop *p) {
+    void *ptrs[] = {p->d_nav_off, p->d_nav_njump, p->d_nav_dst, p->d_nav_bid, p->d_nav_in_off, p->d_nav_in_e,
+                    p->d_nav_stay, p->d_nav_cum, p->d_nav_cnt, p->d_nav_rpos};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    p->d_nav_off = p->d_nav_njump = p->d_nav_dst = p->d_nav_bid = p->d_nav_in_off = p->d_nav_in_e = nullptr;
+    p->d_nav_stay = p->d_nav_cum = nullptr;
+    p->d_nav_cnt = p->d_nav_rpos = nullptr;
+    p->nnav = 0;
+}
+
+// Navigate::recalculate (QHG4/actions/Navigate.cpp:95-148) into the device
+// tables.  A = prob0 / exp(decay * dist0); per port in cell order the
+// destinations' A * exp(decay * dist), summed in destination order; a sum
+// below 1 gives the cumulative table [1 - sum, ...], a sum of 1 or more
+// stops the build (ports after it get none).  The current bridges are those
+// with both ends above sea level.  Each cell's edges: its jumps, then one
+// per current bridge end at the cell (a bridge from a cell to itself is one
+// edge).  Called between steps with the stream idle.
+int nav_rebuild(qhg_pop *p) {
+    const qhg_params &P = p->P;
+    const int32_t nc = p->ncells;
+    const double A = P.nav_prob0 / std::exp(P.nav_decay * P.nav_dist0);
+    std::vector<int32_t> njump((size_t)nc, 0), nbr_ends((size_t)nc, 0);
+    std::vector<double> stay((size_t)nc, 0.0);
+    std::vector<int32_t> tab_of((size_t)nc, -1);   // port index with a table
+    std::vector<double> cum(p->nav_dcell.size());
+    for (size_t k = 0; k < p->nav_port.size(); k++) {
+        const int32_t d0 = p->nav_doff[k], n = p->nav_doff[k + 1] - d0;
+        double sum = 0;
+        for (int32_t j = 0; j < n; j++) {
+            const double pr = A * std::exp(P.nav_decay * p->nav_ddist[(size_t)(d0 + j)]);
+            sum += pr;
+            cum[(size_t)(d0 + j)] = pr;
+        }
+        if (!(sum < 1)) {
+            std::fprintf(stderr, "[Navigate] probabilities for port [%d] add up to %f\n", p->nav_port[k], sum);
+            break;
+        }
+        const int32_t c = p->nav_port[k];
+        stay[(size_t)c] = 1 - sum;
+        double run = stay[(size_t)c];
+        for (int32_t j = 0; j < n; j++) {
+            run = cum[(size_t)(d0 + j)] + run;
+            cum[(size_t)(d0 + j)] = run;
+        }
+        njump[(size_t)c] = n;
+        tab_of[(size_t)c] = (int32_t)k;
+    }
+    std::vector<int32_t> cur;   // current bridges, pairs
+    for (size_t b = 0; b + 1 < p->nav_bridge.size(); b += 2) {
+        const int32_t a0 = p->nav_bridge[b], a1 = p->nav_bridge[b + 1];
+        if (!p->h_alt.empty() && p->h_alt[(size_t)a0] > 0 && p->h_alt[(size_t)a1] > 0) {
+            cur.push_back(a0);
+            cur.push_back(a1);
+        }
+    }
+    for (size_t b = 0; b < cur.size(); b += 2) {
+        nbr_ends[(size_t)cur[b]]++;
+        if (cur[b + 1] != cur[b]) nbr_ends[(size_t)cur[b + 1]]++;
+    }
+    std::vector<int32_t> off((size_t)nc + 1, 0);
+    for (int32_t c = 0; c < nc; c++) off[(size_t)c + 1] = off[(size_t)c] + njump[(size_t)c] + nbr_ends[(size_t)c];
+    const int32_t E = off[(size_t)nc];
+    std::vector<int32_t> dst((size_t)E + 1, 0), bid((size_t)E + 1, -1), fill(off.begin(), off.end() - 1);
+    std::vector<double> ecum((size_t)E + 1, 0.0);
+    for (int32_t c = 0; c < nc; c++) {
+        const int32_t k = tab_of[(size_t)c];
+        if (k < 0) continue;
+        const int32_t d0 = p->nav_doff[(size_t)k];
+        for (int32_t j = 0; j < njump[(size_t)c]; j++) {
+            const int32_t e = fill[(size_t)c]++;
+            dst[(size_t)e] = p->nav_dcell[(size_t)(d0 + j)];
+            ecum[(size_t)e] = cum[(size_t)(d0 + j)];
+        }
+    }
+    for (size_t b = 0; b < cur.size(); b += 2) {
+        const int32_t a0 = cur[b], a1 = cur[b + 1];
+        int32_t e = fill[(size_t)a0]++;
+        dst[(size_t)e] = a1;
+        bid[(size_t)e] = (int32_t)(b / 2);
+        if (a1 != a0) {
+            e = fill[(size_t)a1]++;
+            dst[(size_t)e] = a0;
+            bid[(size_t)e] = (int32_t)(b / 2);
+        }
+    }
+    // edges into each cell, in edge order
+    std::vector<int32_t> in_off((size_t)nc + 1, 0), in_e((size_t)E + 1, 0);
+    for (int32_t e = 0; e < E; e++) in_off[(size_t)dst[(size_t)e] + 1]++;
+    for (int32_t c = 0; c < nc; c++) in_off[(size_t)c + 1] += in_off[(size_t)c];
+    {
+        std::vector<int32_t> f(in_off.begin(), in_off.end() - 1);
+        for (int32_t e = 0; e < E; e++) in_e[(size_t)f[(size_t)dst[(size_t)e]]++] = e;
+    }
+    free_nav_tables(p);
+    const size_t n1 = (size_t)nc + 1, ne = (size_t)E + 1;
+    if (hipMalloc(&p->d_nav_off, n1 * 4) != hipSuccess || hipMalloc(&p->d_nav_njump, (size_t)nc * 4) != hipSuccess ||
+        hipMalloc(&p->d_nav_stay, (size_t)nc * 8) != hipSuccess || hipMalloc(&p->d_nav_in_off, n1 * 4) != hipSuccess ||
+        hipMalloc(&p->d_nav_dst, ne * 4) != hipSuccess || hipMalloc(&p->d_nav_bid, ne * 4) != hipSuccess ||
+        hipMalloc(&p->d_nav_cum, ne * 8) != hipSuccess || hipMalloc(&p->d_nav_in_e, ne * 4) != hipSuccess ||
+        hipMalloc(&p->d_nav_cnt, ne * 4) != hipSuccess || hipMalloc(&p->d_nav_rpos, ne * 4) != hipSuccess) {
+        free_nav_tables(p);
+        return fail(QHG_ERR_NOMEM, "Navigate tables");
+    }
+    HIPCHK(hipMemcpyAsync(p->d_nav_off, off.data(), n1 * 4, hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipMemcpyAsync(p->d_nav_njump, njump.data(), (size_t)nc * 4, hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipMemcpyAsync(p->d_nav_stay, stay.data(), (size_t)nc * 8, hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipMemcpyAsync(p->d_nav_in_off, in_off.data(), n1 * 4, hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipMemcpyAsync(p->d_nav_dst, dst.data(), ne * 4, hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipMemcpyAsync(p->d_nav_bid, bid.data(), ne * 4, hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipMemcpyAsync(p->d_nav_cum, ecum.data(), ne * 8, hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipMemcpyAsync(p->d_nav_in_e, in_e.data(), ne * 4, hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipMemsetAsync(p->d_nav_cnt, 0, ne * 4, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    p->nnav = E;
+    p->gen++;
     return 0;
 }
 
@@ -786,9 +939,11 @@ int phase_act(qhg_pop *p) {
         int rc = phase_initialize(p);
         if (rc) return rc;
     }
-    // the age-death branches and the assortative mates compile in only where used
-    auto kern = p->has_death ? (p->has_assort ? k_cell_act<true, true> : k_cell_act<true, false>)
-                             : (p->has_assort ? k_cell_act<false, true> : k_cell_act<false, false>);
+    // the age-death branches, the assortative mates and Navigate compile in
+    // only where used (Navigate's instantiations carry the age deaths too)
+    auto kern = p->has_nav    ? (p->has_assort ? k_cell_act<true, true, true> : k_cell_act<true, false, true>)
+                : p->has_rare ? (p->has_assort ? k_cell_act<true, true> : k_cell_act<true, false>)
+                              : (p->has_assort ? k_cell_act<false, true> : k_cell_act<false, false>);
     int rc = launch_bs(p, KB_ACT, kern, p->act_grid, kCellBlock, p->ctx(), p->prog, p->step_args());
     if (rc) return rc;
     p->acted = true;
@@ -955,6 +1110,7 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
         const double *src[7] = {env->altitude, env->longitude, env->latitude, env->water,
                                 env->base_npp, env->temperature, env->rainfall};
         for (int k = 0; k < 7; k++) CK(upload_array(&p->d_env[k], src[k], nc, p->stream));
+        if (p->has_nav && env->altitude) p->h_alt.assign(env->altitude, env->altitude + nc);
         CK(upload_array(&p->d_ice, env->ice, nc, p->stream));
         CK(upload_array(&p->d_coastal, env->coastal, nc, p->stream));
     }
@@ -983,6 +1139,7 @@ int qhg_gpu_destroy(qhg_pop *p) {
     free_agentbuf(p->buf[0]);
     free_agentbuf(p->buf[1]);
     free_slot_scratch(p);
+    free_nav_tables(p);
     void *ptrs[] = {p->d_off[0], p->d_off[1], p->d_gpool[0], p->d_gpool[1], p->d_nbr, p->d_nn, p->d_in_e,
                     p->d_K, p->d_W, p->d_tmp, p->d_ice, p->d_coastal, p->d_cstay, p->d_cbirth, p->d_cin, p->d_carr, p->d_tact, p->d_tick, p->d_outk,
                     p->d_birth_off, p->d_bpos, p->d_rpos, p->d_tiles, p->d_tiles2, p->d_occ[0], p->d_occ[1],
@@ -1294,6 +1451,16 @@ int qhg_gpu_pre_loop(qhg_pop *p, double t) {
         int rc = run_capacity(p);
         if (rc) return rc;
     }
+    if (p->has_nav) {
+        // Navigate::preLoop (QHG4/actions/Navigate.cpp:155-183): it needs the
+        // geography and the navigation, and builds the tables
+        if (!p->nav_set) return fail(QHG_ERR_STATE, "Navigate needs the navigation (qhg_gpu_set_navigation)");
+        if (p->h_alt.empty()) return fail(QHG_ERR_STATE, "Navigate needs the altitudes");
+        HIPCHK(hipStreamSynchronize(p->stream));
+        int rc = nav_rebuild(p);
+        if (rc) return rc;
+        p->nav_preloop_done = true;
+    }
     HIPCHK(hipStreamSynchronize(p->stream));
     return 0;
 }
@@ -1411,7 +1578,72 @@ int qhg_gpu_env_event(qhg_pop *p, const qhg_env_desc *env, int32_t events) {
         if (rc) return rc;
         p->me_done = true;
     }
+    if (p->has_nav) {
+        // Navigate::notify: a geography event makes an update due, the
+        // flush runs it (once the pre-loop has built the first tables)
+        if (env->altitude) p->h_alt.assign(env->altitude, env->altitude + nc);
+        if (((events & QHG_EVENT_GEO) || env->altitude) && p->nav_preloop_done) {
+            HIPCHK(hipStreamSynchronize(p->stream));
+            int rc = nav_rebuild(p);
+            if (rc) return rc;
+        }
+    }
     HIPCHK(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+int qhg_gpu_set_navigation(qhg_pop *p, const qhg_nav_desc *nav) {
+    if (!p || !nav) return fail(QHG_ERR_ARG, "null argument");
+    if (p->acted || p->initialized) return fail(QHG_ERR_STATE, "population is inside a step");
+    const int32_t np = nav->num_ports, nc = p->ncells;
+    if (np < 0 || nav->num_bridges < 0 || nav->num_bridges > 131072) return fail(QHG_ERR_ARG, "navigation sizes");
+    if (np && (!nav->ports || !nav->dest_offsets)) return fail(QHG_ERR_ARG, "navigation: null ports");
+    if (nav->num_bridges && !nav->bridges) return fail(QHG_ERR_ARG, "navigation: null bridges");
+    std::vector<int32_t> order((size_t)np);
+    for (int32_t k = 0; k < np; k++) {
+        order[(size_t)k] = k;
+        if (nav->ports[k] < 0 || nav->ports[k] >= nc) return fail(QHG_ERR_ARG, "navigation: port out of range");
+        if (nav->dest_offsets[k] < 0 || nav->dest_offsets[k + 1] < nav->dest_offsets[k])
+            return fail(QHG_ERR_ARG, "navigation: destination offsets");
+    }
+    if (np && nav->dest_offsets[np] > 0 && (!nav->dest_cells || !nav->dest_dists))
+        return fail(QHG_ERR_ARG, "navigation: null destinations");
+    // the maps' order: ports by cell, each port's destinations by cell
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return nav->ports[a] < nav->ports[b]; });
+    std::vector<int32_t> port, doff(1, 0), dcell;
+    std::vector<double> ddist;
+    for (int32_t i = 0; i < np; i++) {
+        const int32_t k = order[(size_t)i];
+        if (i > 0 && nav->ports[k] == port.back()) return fail(QHG_ERR_ARG, "navigation: a port is listed twice");
+        port.push_back(nav->ports[k]);
+        const int32_t a = nav->dest_offsets[k], n = nav->dest_offsets[k + 1] - a;
+        std::vector<int32_t> dj((size_t)n);
+        std::iota(dj.begin(), dj.end(), a);
+        std::stable_sort(dj.begin(), dj.end(), [&](int32_t x, int32_t y) { return nav->dest_cells[x] < nav->dest_cells[y]; });
+        for (int32_t j = 0; j < n; j++) {
+            const int32_t c = nav->dest_cells[dj[(size_t)j]];
+            if (c < 0 || c >= nc) return fail(QHG_ERR_ARG, "navigation: destination out of range");
+            if (j > 0 && c == dcell.back()) return fail(QHG_ERR_ARG, "navigation: a destination is listed twice");
+            dcell.push_back(c);
+            ddist.push_back(nav->dest_dists[dj[(size_t)j]]);
+        }
+        doff.push_back((int32_t)dcell.size());
+    }
+    std::vector<int32_t> br((size_t)2 * nav->num_bridges);
+    for (size_t b = 0; b < br.size(); b++) {
+        br[b] = nav->bridges[b];
+        if (br[b] < 0 || br[b] >= nc) return fail(QHG_ERR_ARG, "navigation: bridge end out of range");
+    }
+    HIPCHK(hipStreamSynchronize(p->stream));
+    p->nav_port.swap(port);
+    p->nav_doff.swap(doff);
+    p->nav_dcell.swap(dcell);
+    p->nav_ddist.swap(ddist);
+    p->nav_bridge.swap(br);
+    p->nav_set = true;
+    // before the pre-loop this is the grid's data; afterwards EVENT_ID_NAV
+    // and its flush (Navigate::notify, QHG4/actions/Navigate.cpp:79-89)
+    if (p->has_nav && p->nav_preloop_done) return nav_rebuild(p);
     return 0;
 }
 
@@ -1800,7 +2032,7 @@ int batch_upload_table(qhg_batch *b) {
         desc[i].t0 = b->t0[i];
         b->gens[i] = p->gen;
         if (p->gwords) b->any_genetics = true;
-        if (p->has_death) b->any_death = true;
+        if (p->has_rare) b->any_death = true;
     }
     if (n) HIPCHK(hipMemcpy(b->d_desc, desc.data(), n * sizeof(PopDesc), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b->d_tile_base, tile_base.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -1872,6 +2104,7 @@ int qhg_batch_add(qhg_batch *b, qhg_pop *p) {
     if (p->acted || p->initialized) return fail(QHG_ERR_STATE, "population is inside a step");
     if (p->has_assort)
         return fail(QHG_ERR_ARG, "AssortativePairHamming and AssortativePair populations do not run in batches");
+    if (p->has_nav) return fail(QHG_ERR_ARG, "Navigate populations do not run in batches");
     // hand over: everything the population enqueued so far has finished
     HIPCHK(hipStreamSynchronize(p->stream));
     poll_ring(p);

@@ -22,7 +22,8 @@ _FIELD_OF = {
     "ATanDeath_max_age": "atan_max_age", "ATanDeath_range": "atan_range", "ATanDeath_slope": "atan_slope",
     "Genetics_mutation_rate": "gen_mutation_rate", "AssortativePairHamming_cutoff": "assort_cutoff",
     "AssortativePairHamming_permute": "assort_permute", "AssortativePair_cutoff": "assort_cutoff",
-    "AssortativePair_permute": "assort_permute",
+    "AssortativePair_permute": "assort_permute", "Navigate_decay": "nav_decay", "Navigate_dist0": "nav_dist0",
+    "Navigate_prob0": "nav_prob0", "Navigate_min_dens": "nav_min_dens", "Navigate_bridge_prob": "nav_bridge_prob",
 }
 
 
@@ -48,7 +49,18 @@ class GpuPopulation:
 
     @classmethod
     def from_scenario(cls, sc, capacity=0):
-        return cls(sc["grid"], sc["env"], sc["params"], sc["agents"], capacity)
+        pop = cls(sc["grid"], sc["env"], sc["params"], sc["agents"], capacity)
+        if sc.get("nav") is not None:
+            pop.set_navigation(sc["nav"])
+        return pop
+
+    def set_navigation(self, nav):
+        """The grid's sea-ways and bridges (Navigation::setData / setBridges):
+        before preLoop the grid's data, afterwards EVENT_ID_NAV with its
+        flush.  nav: a dict as abi.nav_arrays builds it."""
+        self.nav = nav
+        d = abi.nav_struct(nav)
+        check(lib().qhg_gpu_set_navigation(self.h, C.byref(d)), "qhg_gpu_set_navigation")
 
     def close(self):
         if getattr(self, "h", None):

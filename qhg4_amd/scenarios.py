@@ -102,7 +102,7 @@ def ico_env(grid, seed):
 
 
 def ooa_params(seed, genetics=False, wmove_prob=0.2, b0=0.8, d0=0.001, theta=0.1, k_scale=50.0, assort=None,
-               genome_size=256, protein=False):
+               genome_size=256, protein=False, navigate=None):
     """VerhulstVarK values are those of QHG4/tutorial_data/xmldat/tut_EnvironCapAlt.xml
     (C2 names none; with C1's b0=0.1, d0=0.05 the paired-female birth rate
     cannot outpace deaths and every run dies out).  assort=(cutoff, permute)
@@ -126,6 +126,10 @@ def ooa_params(seed, genetics=False, wmove_prob=0.2, b0=0.8, d0=0.001, theta=0.1
         kw = dict(gen_genome_size=genome_size, gen_num_crossover=1, gen_mutation_rate=1e-3)
     if assort is not None:
         kw.update(assort_cutoff=float(assort[0]), assort_permute=int(assort[1]))
+    if navigate is not None:
+        prio, decay, dist0, prob0, bridge = navigate
+        actions.append((abi.ACT_NAVIGATE, prio))
+        kw.update(nav_decay=decay, nav_dist0=dist0, nav_prob0=prob0, nav_min_dens=0.0, nav_bridge_prob=bridge)
     return make_params(
         seed, actions,
         fert_min_age=15.0, fert_max_age=45.0, fert_interbirth=3.0,
@@ -159,6 +163,66 @@ def c2(seed=42, level=6, n_agents=100000, patch_cells=4000, genetics=False, asso
            "steps": 2000}
     if genetics:
         out["agents"]["genome"] = initial_genomes(rng, n_agents, genome_size)
+    return out
+
+
+EARTH_RADIUS_KM = 6371.3
+
+
+def c2nav(seed=42, level=5, n_agents=20000, patch_cells=800, nav_first=False, prob0=0.1, bridge_prob=0.3,
+          genetics=True):
+    """C2-style model with Navigate (OoANavGen2bitPop schedules it,
+    QHG4/populations/OoANavGen2bitPop.cpp:77-93), on a coast: the agents
+    start in a patch of land around the coastal cell with the best NPP.
+    Ports are the patch's coastal cells plus random coastal cells elsewhere;
+    each port gets 0-6 destinations among the 40 coastal cells nearest to it,
+    at great-circle distances in km.  Bridges join ports to patch cells, to
+    water cells (never current while the water stays) and one cell to
+    itself.  3 % of the cells are ice.  With nav_first, Navigate shares
+    WeightedMove's priority and runs before it (name order), so a later
+    WeightedMove registration can override a sea-way jump; otherwise it runs
+    after WeightedMove and only agents that did not move navigate."""
+    grid = ico_grid(level)
+    env = dict(ico_env(grid, seed))
+    alt = env["altitude"]
+    nbr = grid["neighbors"]
+    land = alt > 0
+    wet_nb = np.array([any(nb >= 0 and alt[nb] <= 0 for nb in row) for row in nbr])
+    coastal = np.flatnonzero(land & wet_nb)
+    score = np.where(np.isin(np.arange(grid["num_cells"]), coastal) & (alt < 1500), env["base_npp"], -1.0)
+    start = int(np.argmax(score))
+    patch = bfs_patch(grid, start, patch_cells, land)
+    rng = np.random.default_rng(seed + 23)
+    in_patch = np.zeros(grid["num_cells"], dtype=bool)
+    in_patch[patch] = True
+    ports = sorted(set(coastal[in_patch[coastal]].tolist()) |
+                   set(rng.choice(coastal, size=min(len(coastal), 40), replace=False).tolist()))
+    xyz = grid["xyz"]
+    xyz_c = xyz[coastal] / np.linalg.norm(xyz[coastal], axis=1)[:, None]
+    dests = {}
+    for c in ports:
+        u = xyz[c] / np.linalg.norm(xyz[c])
+        ang = np.arccos(np.clip(xyz_c @ u, -1.0, 1.0))
+        near = coastal[np.argsort(ang)[1:41]]
+        k = int(rng.integers(0, 7))
+        pick = rng.choice(near, size=k, replace=False)
+        dests[int(c)] = [(int(d), float(EARTH_RADIUS_KM * np.arccos(np.clip(
+            (xyz[d] / np.linalg.norm(xyz[d])) @ u, -1.0, 1.0)))) for d in pick]
+    water = np.flatnonzero(~land)
+    bridges = [(int(rng.choice(ports)), int(rng.choice(patch))) for _ in range(20)]
+    bridges += [(int(rng.choice(patch)), int(rng.choice(water))) for _ in range(4)]
+    bridges.append((int(patch[0]), int(patch[0])))
+    ice = np.zeros(grid["num_cells"], dtype=np.uint8)
+    ice[rng.random(grid["num_cells"]) < 0.03] = 1
+    env["ice"] = ice
+    cells = patch[rng.integers(0, len(patch), n_agents)]
+    agents = _agents(rng, cells, n_agents, abi.QHG_LIFE_ALIVE, -1.0)
+    nav_prio = 4 if nav_first else 5
+    params = ooa_params(seed, genetics=genetics, navigate=(nav_prio, -0.001, 150.0, prob0, bridge_prob))
+    out = {"name": "C2-Navigate", "grid": grid, "env": env, "params": params, "agents": agents,
+           "nav": abi.nav_arrays(dests, bridges, sample_dist=1.0), "steps": 200}
+    if genetics:
+        out["agents"]["genome"] = initial_genomes(rng, n_agents, 256)
     return out
 
 

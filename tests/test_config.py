@@ -128,7 +128,19 @@ def test_unscheduled_actions_are_not_run():
     assert "ATanDeath" not in actions(p) and p.atan_max_age == 60.0
 
 
-def test_navigate_has_no_gpu_implementation():
+NAV_MODULE = """<module name="Navigate">
+    <param name="Navigate_decay" value="-0.001"/><param name="Navigate_dist0" value="150.0"/>
+    <param name="Navigate_prob0" value="0.1"/><param name="Navigate_min_dens" value="0.0"/>
+    <param name="Navigate_bridge_prob" value="0.25"/>
+  </module>
+  """
+
+
+def test_navigate_species():
+    """OoANavGen2bitPop holds Navigate (QHG4/populations/OoANavGen2bitPop.cpp
+    :77-93): its module must be there with all five attributes, as
+    Action::checkAttributes demands them (QHG4/actions/Action.cpp:54-75,
+    Navigate.cpp:18-24); it runs only with a priority."""
     xml = open(os.path.join(HERE, "golden", "species_ooa.xml")).read()
     xml = xml.replace('name="OoAPop"', 'name="OoANavGen2bitPop"')
     xml = xml.replace("<priorities>", '''<module name="Genetics">
@@ -136,13 +148,20 @@ def test_navigate_has_no_gpu_implementation():
     <param name="Genetics_mutation_rate" value="0.001"/><param name="Genetics_initial_muts" value="random"/>
     <param name="Genetics_create_new_genome" value="1"/><param name="Genetics_bits_per_nuc" value="2"/>
   </module>
-  <priorities><prio name="Genetics" value="9"/>''')
+  ''' + NAV_MODULE + '''<priorities><prio name="Genetics" value="9"/>''')
     xml = xml.replace('id="NPP">', 'id="NPP"><param name="NPPPref" value="0 0 1.5 1"/>')
     p = config.params_from_xml(xml, seed=2)
     assert actions(p)["Genetics"] == (abi.ACT_GENETICS, 9) and p.gen_genome_size == 256
     assert p.me_pl_n[1] == 2
-    with pytest.raises(QhgError, match="Navigate has no GPU implementation"):
-        config.params_from_xml(xml.replace("<priorities>", '<priorities><prio name="Navigate" value="4"/>'), 2)
+    assert "Navigate" not in actions(p)
+    assert (p.nav_decay, p.nav_dist0, p.nav_prob0, p.nav_min_dens, p.nav_bridge_prob) == (-0.001, 150.0, 0.1, 0.0, 0.25)
+    q = config.params_from_xml(xml.replace("<priorities>", '<priorities><prio name="Navigate" value="4"/>'), 2)
+    assert actions(q)["Navigate"] == (abi.ACT_NAVIGATE, 4)
+    assert config.get_attribute(q, "Navigate_prob0") == 0.1
+    with pytest.raises(QhgError, match="no module for Navigate"):
+        config.params_from_xml(xml.replace(NAV_MODULE, ""), 2)
+    with pytest.raises(QhgError, match="missing parameter Navigate_bridge_prob"):
+        config.params_from_xml(xml.replace('<param name="Navigate_bridge_prob" value="0.25"/>', ""), 2)
     with pytest.raises(QhgError, match="2-bit"):
         config.params_from_xml(xml.replace('"Genetics_bits_per_nuc" value="2"', '"Genetics_bits_per_nuc" value="1"'),
                                2)
@@ -169,7 +188,7 @@ def test_assortative_species(cls, act, kind, cutoff):
   <module name="%s">
     <param name="%s_cutoff" value="%s"/><param name="%s_permute" value="1"/>
   </module>
-  <priorities><prio name="Genetics" value="9"/>''' % (act, act, cutoff, act))
+  ''' % (act, act, cutoff, act) + NAV_MODULE + '<priorities><prio name="Genetics" value="9"/>')
     xml = xml.replace('id="NPP">', 'id="NPP"><param name="NPPPref" value="0 0 1.5 1"/>')
     p = config.params_from_xml(xml, seed=2)
     a = actions(p)

@@ -43,7 +43,7 @@ def test_qdf_library_exports_every_declared_symbol():
 
 def test_abi_version_and_words():
     L = _lib.lib()
-    assert L.qhg_gpu_abi_version() == 7
+    assert L.qhg_gpu_abi_version() == 8
     assert L.qhg_gpu_genome_words(256) == 8 == abi.genome_words(256)
     assert L.qhg_gpu_genome_words(1) == 1
 
@@ -51,7 +51,9 @@ def test_abi_version_and_words():
 def test_struct_sizes_match_header():
     # field order / padding of the ctypes mirrors (qhg_params is the largest)
     assert C.sizeof(abi.Params) == 8 + 4 + 64 + 64 + 16 * 48 + 9 * 8 + 4 + 4 + 16 + 16 + 32 + 16 + \
-        2 * 4 * 64 * 8 + 9 * 8 + 4 + 4 + 8 + 4 + 5 * 8 + 8 + 4 + 4  # 4 bytes of padding before the first double
+        2 * 4 * 64 * 8 + 9 * 8 + 4 + 4 + 8 + 4 + 5 * 8 + 8 + 4 + 4 + 5 * 8  # 4 bytes of padding before the first double
+    # qhg_nav_desc: int32 (+4 padding), four pointers, a double, int32 (+4), a pointer
+    assert C.sizeof(abi.NavDesc) == 8 + 4 * 8 + 8 + 8 + 8
     assert C.sizeof(abi.Stats) == 17 * 8
     assert C.sizeof(abi.Tuning) == 16
 

@@ -42,6 +42,7 @@ namespace qhg {
 constexpr int kEventIdGeo = 2;
 constexpr int kEventIdClimate = 3;
 constexpr int kEventIdVeg = 4;
+constexpr int kEventIdNav = 5;
 constexpr int kEventIdFlush = 20;
 
 // one module's attributes (ParamProvider2's ModuleComplex::getAttributes)
@@ -295,12 +296,28 @@ public:
         return 0;
     }
     // EVENT_ID_GEO / CLIMATE / VEG mark the environment changed; the
-    // population reacts at flushEvents (other ids are not environment events)
+    // population reacts at flushEvents.  EVENT_ID_NAV marks the navigation
+    // changed: the caller passes the new one with setNavigation
+    // (takeNavEvent tells it so).  Other ids are not environment events.
     int updateEvent(int event_id, float) {
         if (event_id == kEventIdGeo) pending_ |= QHG_EVENT_GEO;
         if (event_id == kEventIdClimate) pending_ |= QHG_EVENT_CLIMATE;
         if (event_id == kEventIdVeg) pending_ |= QHG_EVENT_VEG;
+        if (event_id == kEventIdNav) nav_event_ = true;
         return 0;
+    }
+    bool takeNavEvent() {
+        const bool e = nav_event_;
+        nav_event_ = false;
+        return e;
+    }
+    // the grid's sea-ways and bridges (Navigation::setData / setBridges), for
+    // Navigate: before preLoop the grid's data, afterwards a navigation
+    // event with its flush (the arrays are copied)
+    int setNavigation(const qhg_nav_desc &nav) {
+        if (int rc = need()) return rc;
+        const int rc = qhg_gpu_set_navigation(pop_, &nav);
+        return rc ? fail(rc, "qhg_gpu_set_navigation") : 0;
     }
     int flushEvents(float) {
         if (!pending_ || !pop_) return 0;
@@ -390,6 +407,7 @@ private:
     Agents staged_;
     bool looping_ = false;
     int pending_ = 0;
+    bool nav_event_ = false;
     std::vector<uint32_t> counts_;
     int64_t total_ = 0;
     bool counts_fresh_ = false;

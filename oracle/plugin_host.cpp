@@ -31,7 +31,13 @@
 //   <dir>/agents.txt    "<cell>\t<agent data line>"
 //   <dir>/plan.txt      "seed <n>", "steps <n>", "geo <step> <altitude file> <ice file>",
 //                       "dump <step>" (dump there, restore into a new population
-//                       through PopReader and go on with that one)
+//                       through PopReader and go on with that one),
+//                       "nav <file>" (the grid's Navigation, as NavGroupReader
+//                       fills it: Navigation::setData / setBridges), "navev
+//                       <step> <file>" (the Navigation replaced before that
+//                       step, then updateEvent(EVENT_ID_NAV) and flushEvents)
+//   navigation file     "ports K", K lines "cell n dest dist ...", "bridges B",
+//                       B lines "cell cell"
 // writes <dir>/out.qdf (PopWriter, after the last step), <dir>/snap.txt (the
 // per-agent accessors and per-cell counts) and <dir>/log.txt.
 #include <cstdio>
@@ -47,6 +53,7 @@
 #include "EventConsts.h"
 #include "Geography.h"
 #include "IDGen.h"
+#include "Navigation.h"
 #include "ParamProvider2.h"
 #include "PopBase.h"
 #include "PopLooper.h"
@@ -70,6 +77,37 @@ bool read_raw(const std::string &path, T *out, size_t n) {
 int die(const char *what, const std::string &detail) {
     std::fprintf(stderr, "plugin_host: %s %s\n", what, detail.c_str());
     return 1;
+}
+
+// a navigation file into the grid's Navigation (setData, setBridges)
+bool read_navigation(const std::string &path, Navigation *nav) {
+    std::ifstream in(path);
+    distancemap dests;
+    bridgelist bridges;
+    std::string w;
+    int K = 0, B = 0;
+    if (!(in >> w) || w != "ports" || !(in >> K)) return false;
+    for (int k = 0; k < K; k++) {
+        int c = 0, n = 0;
+        if (!(in >> c >> n)) return false;
+        distlist dl;
+        for (int j = 0; j < n; j++) {
+            int d = 0;
+            double dist = 0;
+            if (!(in >> d >> dist)) return false;
+            dl[d] = dist;
+        }
+        dests[c] = dl;
+    }
+    if (!(in >> w) || w != "bridges" || !(in >> B)) return false;
+    for (int b = 0; b < B; b++) {
+        int x = 0, y = 0;
+        if (!(in >> x >> y)) return false;
+        bridges.push_back(bridgedef(x, y));
+    }
+    nav->setData(dests, 1.0);
+    nav->setBridges(bridges);
+    return true;
 }
 
 }  // namespace
@@ -115,14 +153,21 @@ int main(int argc, char **argv) {
 
     // plan
     uint64_t seed = 0;
-    int steps = 0, geo_step = -1, dump_step = -1;
-    std::string geo_alt, geo_ice;
+    int steps = 0, geo_step = -1, dump_step = -1, nav_step = -1;
+    std::string geo_alt, geo_ice, nav_file, nav_file2;
     std::ifstream pl(dir + "/plan.txt");
     for (std::string w; pl >> w;) {
         if (w == "seed") pl >> seed;
         else if (w == "steps") pl >> steps;
         else if (w == "geo") pl >> geo_step >> geo_alt >> geo_ice;
         else if (w == "dump") pl >> dump_step;
+        else if (w == "nav") pl >> nav_file;
+        else if (w == "navev") pl >> nav_step >> nav_file2;
+    }
+    if (!nav_file.empty()) {
+        Navigation *navi = new Navigation(cg);
+        if (!read_navigation(dir + "/" + nav_file, navi)) return die("cannot read", nav_file);
+        cg->setNavigation(navi);
     }
 
     // the application's WELL seed state (the plugin keys its draws on its
@@ -180,6 +225,14 @@ int main(int argc, char **argv) {
             pop->updateEvent(EVENT_ID_GEO, ev, (float)(s + 1));
             pop->flushEvents((float)(s + 1));
             std::fprintf(log, "geo event before step %d alive %lu\n", s + 1, pop->getNumAgentsEffective());
+        }
+        if (s + 1 == nav_step) {
+            if (!cg->m_pNavigation || !read_navigation(dir + "/" + nav_file2, cg->m_pNavigation))
+                return die("cannot read", nav_file2);
+            char ev[] = "nav";
+            pop->updateEvent(EVENT_ID_NAV, ev, (float)(s + 1));
+            pop->flushEvents((float)(s + 1));
+            std::fprintf(log, "nav event before step %d alive %lu\n", s + 1, pop->getNumAgentsEffective());
         }
         if (s + 1 == dump_step) {
             const std::string f = dir + "/dump.qdf";

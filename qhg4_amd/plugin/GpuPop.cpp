@@ -135,9 +135,36 @@ int GpuPop::uploadLoaded() {
     return report(iResult, "addAgents");
 }
 
+// Navigation::m_mDestinations (port -> destination -> distance, both maps
+// in cell order) and m_vBridges (QHG4/core/Navigation.h:12-40) as the
+// device population takes them; cell ids are used as cell indices, as
+// Navigate does (QHG4/actions/Navigate.cpp:195-230)
+int GpuPop::passNavigation() {
+    const Navigation *nav = m_pCG->m_pNavigation;
+    if (!nav) return 0;
+    std::vector<int32_t> ports, off(1, 0), cells, bridges;
+    std::vector<double> dists;
+    for (const auto &p : nav->m_mDestinations) {
+        ports.push_back(p.first);
+        for (const auto &d : p.second) {
+            cells.push_back(d.first);
+            dists.push_back(d.second);
+        }
+        off.push_back((int32_t)cells.size());
+    }
+    for (const auto &b : nav->m_vBridges) {
+        bridges.push_back(b.first);
+        bridges.push_back(b.second);
+    }
+    const qhg_nav_desc d{(int32_t)ports.size(), ports.data(), off.data(), cells.data(), dists.data(),
+                         nav->m_dSampleDist, (int32_t)nav->m_vBridges.size(), bridges.data()};
+    return report(m_pPop->setNavigation(d), "setNavigation");
+}
+
 int GpuPop::preLoop() {
     if (!m_pPop) return report(-1, "preLoop");
     int iResult = uploadLoaded();
+    if (iResult == 0) iResult = passNavigation();
     if (iResult == 0) iResult = report(m_pPop->preLoop(), "preLoop");
     if (iResult == 0 && m_bRestored) {
         iResult = report(m_pPop->setClock(m_iRestoreStep, m_dRestoreTime, m_iRestoreMaxID), "preLoop (clock)");
@@ -228,6 +255,7 @@ void GpuPop::flushEvents(float fT) {
     if (Vegetation *v = m_pCG->m_pVegetation) env.base_npp = v->m_adBaseANPP;
     m_env = env;
     m_pPop->setEnvironment(env);
+    if (m_pPop->takeNavEvent()) passNavigation();
     report(m_pPop->flushEvents(fT), "flushEvents");
     m_bSnapValid = false;
 }

@@ -126,6 +126,8 @@ public:
 
 private:
     int report(int iResult, const char *pWhere);
+    // the grid's Navigation (sea-ways and bridges) to the device population
+    int passNavigation();
     const qhg::Agents &snapshot();
     int uploadLoaded();
 

@@ -188,7 +188,8 @@ qhg_params base_params(uint64_t seed) {
 
 // one model on the oracle: steps with 1 and 4 threads must agree, an
 // environment event and a parameter change in between
-void run_model(const char *what, const World &w, qhg_params P, int n, int gw, int steps) {
+void run_model(const char *what, const World &w, qhg_params P, int n, int gw, int steps,
+               const qhg_nav_desc *nav = nullptr) {
     Pop a = make_agents(n, w.nc, gw, P.seed);
     std::vector<int64_t> alive;
     for (int th : {1, 4}) {
@@ -199,7 +200,8 @@ void run_model(const char *what, const World &w, qhg_params P, int n, int gw, in
         if (rc || !o) return;
         qhg_agents v = a.view(gw);
         CHECK(orc_set_agents(o, &v) == 0);
-        orc_pre_loop(o);
+        if (nav) CHECK(orc_set_navigation(o, nav) == 0);
+        CHECK(orc_pre_loop(o) == 0);
         orc_run(o, steps / 2);
         std::vector<double> alt2(w.alt);
         for (int c = 0; c < w.nc; c += 7) alt2[(size_t)c] = -10.0;
@@ -209,6 +211,12 @@ void run_model(const char *what, const World &w, qhg_params P, int n, int gw, in
         qhg_params P2 = P;
         P2.verh_b0 *= 0.9;
         CHECK(orc_set_params(o, &P2) == 0);
+        if (nav) {
+            // a navigation event: the same sea-ways without the last port
+            qhg_nav_desc n2 = *nav;
+            n2.num_ports = nav->num_ports > 0 ? nav->num_ports - 1 : 0;
+            CHECK(orc_set_navigation(o, &n2) == 0);
+        }
         orc_run(o, steps - steps / 2);
         qhg_stats st{};
         orc_stats(o, &st);
@@ -282,6 +290,34 @@ void oracle_models() {
         P.assort_permute = 1;
         const char *names[] = {"OoA-style + genetics", "... + AssortativePairHamming", "... + AssortativePair"};
         run_model(names[kind == 0 ? 0 : (kind == QHG_ACT_ASSORTPAIR ? 1 : 2)], w, P, 3000, 2 * 4, 24);
+        if (kind != 0) continue;
+        // ... + Navigate: ports with 0-6 destinations (one whose
+        // probabilities add up to more than 1), bridges (one from a cell to
+        // itself, some across water), a navigation event midway
+        add_action(P, QHG_ACT_NAVIGATE, 5, "Navigate");
+        P.nav_decay = -0.001;
+        P.nav_dist0 = 150;
+        P.nav_prob0 = 0.2;
+        P.nav_bridge_prob = 0.4;
+        uint64_t s = 99;
+        std::vector<int32_t> ports, off(1, 0), cells, bridges;
+        std::vector<double> dists;
+        for (int c = 3; c < w.nc; c += 11) {
+            ports.push_back(c);
+            const int k = (c == 47) ? 9 : (int)(splitmix(s) % 7);
+            for (int j = 0; j < k; j++) {
+                cells.push_back((c + 5 + 13 * j) % w.nc);
+                dists.push_back((c == 47) ? 1.0 : 10.0 + (double)(splitmix(s) % 500));
+            }
+            off.push_back((int32_t)cells.size());
+        }
+        for (int b = 0; b < 12; b++) {
+            bridges.push_back((int32_t)(splitmix(s) % (uint64_t)w.nc));
+            bridges.push_back(b == 0 ? bridges.back() : (int32_t)(splitmix(s) % (uint64_t)w.nc));
+        }
+        const qhg_nav_desc nav{(int32_t)ports.size(), ports.data(), off.data(), cells.data(), dists.data(), 1.0,
+                               12, bridges.data()};
+        run_model("... + Navigate", w, P, 3000, 2 * 4, 24, &nav);
     }
 }
 

@@ -4,10 +4,11 @@ LeakSanitizer (CPU).
 tests/cpp/sanitize_driver.cpp is compiled together with the oracle
 (oracle/qhg_oracle.c), the species configuration (qhg4_amd/csrc/config.cpp)
 and the QDF I/O (qhg4_amd/csrc/qdf_io.cpp) with
--fsanitize=address,undefined -fno-sanitize-recover=all.  It runs four action
+-fsanitize=address,undefined -fno-sanitize-recover=all.  It runs five action
 sets on the oracle with 1 and 4 OpenMP threads (the tutorial model with
-ATanDeath; the OoA model with genetics and RandomPair,
-AssortativePairHamming or AssortativePair), every MD5 padding case, the
+ATanDeath; the OoA model with genetics and RandomPair, RandomPair with
+Navigate and a navigation event, AssortativePairHamming or
+AssortativePair), every MD5 padding case, the
 proteome reader at 12 genome sizes, 200 RankTable pairings, the
 configuration's error cases, and a QDF write / read round trip with its
 error cases.  Any sanitizer report ends the program with a failure.

@@ -3,7 +3,9 @@
   include/qhg_population.hpp (linked to qhg4_amd/libqhg4_amd.so);
 - tests/cpp/_build/sanitize_driver (build_sanitize): the oracle, config.cpp
   and qdf_io.cpp compiled with -fsanitize=address,undefined, host only (no
-  HIP source is compiled)."""
+  HIP source is compiled);
+- tests/cpp/_build/dispatch_probe (build_probe): the kernel-boundary cost
+  probe, run under rocprofv3 for the step kernels' design."""
 import os
 import subprocess
 
@@ -52,6 +54,23 @@ def build_sanitize(verbose=False):
             raise RuntimeError("sanitizer build failed:\n" + r.stdout + r.stderr)
     os.replace(SAN_OUT + ".tmp", SAN_OUT)
     return SAN_OUT
+
+
+PROBE_OUT = os.path.join(HERE, "_build", "dispatch_probe")
+
+
+def build_probe(verbose=False):
+    """The dispatch-cost probe (tests/cpp/dispatch_probe.hip), for gfx950."""
+    os.makedirs(os.path.dirname(PROBE_OUT), exist_ok=True)
+    cmd = ["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O3", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", os.path.join(HERE, "dispatch_probe.hip"), "-o", PROBE_OUT + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc (dispatch_probe) failed:\n" + r.stdout + r.stderr)
+    os.replace(PROBE_OUT + ".tmp", PROBE_OUT)
+    return PROBE_OUT
 
 
 if __name__ == "__main__":

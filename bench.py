@@ -37,7 +37,8 @@ Ensemble and serving results are compared for identity through full-state
 digests (every live agent's fields and genome).
 --workload c4 / c5 print those measurements as lines of their own.
 --workload c2a times the C2 model with genetics and AssortativePairHamming
-pairing (k_assort_pair in kernels).
+pairing (k_assort_pair in kernels); --workload c2nav the C2 model with
+Navigate's sea-way jumps and bridges.
 """
 import argparse
 import json
@@ -115,6 +116,11 @@ def make_workload(name, seed):
         sc = scenarios.c2(seed=seed, genetics=True, assort=(0.9, 1))
         desc = "C2 with 2x256-locus 2-bit genetics and AssortativePairHamming (cutoff 0.9, permuted) " \
                "in place of RandomPair (OoANavGen2bitAssHamPop's pairing), 100,000 initial agents"
+    elif name == "c2nav":
+        sc = scenarios.c2nav(seed=seed, level=6, n_agents=100000, patch_cells=4000, prob0=0.15, genetics=False)
+        desc = "C2 model plus Navigate (OoANavGen2bitPop's sea-ways: ports on the coast with up to 6 " \
+               "destinations each, bridges, 3% ice) on a 40,962-cell icosahedral grid, 100,000 initial " \
+               "agents on a coast"
     elif name == "c3":
         sc = scenarios.c3(seed=seed)
         desc = "C3: C2 model + 2x256-locus 2-bit genetics on a 655,362-cell icosahedral grid, 2M agents"
@@ -363,7 +369,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1900)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c2a", "c3", "c4", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c2a", "c2nav", "c3", "c4", "c5"])
     ap.add_argument("--requests", type=int, default=2000, help="C5: requests in the stream")
     ap.add_argument("--budget-gb", type=float, default=250.0, help="C5: HBM budget for resident state")
     ap.add_argument("--chunk", type=int, default=16, help="C5: steps per batch chunk")

@@ -521,9 +521,8 @@ __device__ __forceinline__ int move_dir(const CtxT &x, int kind, uint32_t u, int
 // takes a draw and registers a move to the other end when it is below
 // Navigate_bridge_prob.  Each registration counts as a move; the last wins.
 template <typename CtxT, typename ProgT>
-__device__ void nav_execute(const CtxT &x, const ProgT &p, const StepArgs &sa, int64_t id, int32_t c, int &dir,
-                            int32_t &nave, uint32_t &moves) {
-    const int32_t e0 = x.nav.off[c], e1 = x.nav.off[c + 1];
+__device__ void nav_execute(const CtxT &x, const ProgT &p, const StepArgs &sa, int64_t id, int32_t c, int32_t e0,
+                            int32_t e1, int &dir, int32_t &nave, uint32_t &moves) {
     if (e0 == e1) return;
     const int32_t nj = x.nav.njump[c];
     if (nj > 0) {
@@ -559,11 +558,11 @@ __device__ void nav_execute(const CtxT &x, const ProgT &p, const StepArgs &sa, i
 // QHG4/core/SPopulation.cpp:1058-1092).  Navigate takes its place in the
 // program and acts only when no move was registered before it
 // (LIFE_STATE_MOVING, Navigate.cpp:191).  w, nbr: the cell's weight and
-// neighbour rows.
+// neighbour rows; ne0, ne1: its Navigate edges.
 template <bool kNav, typename CtxT, typename ProgT>
 __device__ __forceinline__ void resolve_moves(const CtxT &x, const ProgT &p, const StepArgs &sa, const ActItem &a,
-                                              int32_t c, int nn, const double (&w)[7], const int32_t (&nbr)[6],
-                                              int &dir, int32_t &nave, uint32_t &moves) {
+                                              int32_t c, int32_t ne0, int32_t ne1, int nn, const double (&w)[7],
+                                              const int32_t (&nbr)[6], int &dir, int32_t &nave, uint32_t &moves) {
     if constexpr (!kNav) {
         int d = move_dir(x, a.pk0, a.pu0, nn, w, nbr);
         if (d >= 0) { dir = d; moves++; }
@@ -572,8 +571,10 @@ __device__ __forceinline__ void resolve_moves(const CtxT &x, const ProgT &p, con
             if (d >= 0) { dir = d; moves++; }
         }
     } else {
+        // one copy of Navigate's code, not one per place it can take
+#pragma unroll 1
         for (int q = 0; q <= a.npend; q++) {
-            if (a.navq == q + 1 && dir < 0 && x.nav.n) nav_execute(x, p, sa, a.id, c, dir, nave, moves);
+            if (a.navq == q + 1 && dir < 0) nav_execute(x, p, sa, a.id, c, ne0, ne1, dir, nave, moves);
             if (q < a.npend) {
                 const int d = move_dir(x, q == 0 ? a.pk0 : a.pk1, q == 0 ? a.pu0 : a.pu1, nn, w, nbr);
                 if (d >= 0) { dir = d; moves++; }
@@ -727,6 +728,8 @@ __device__ __forceinline__ void cell_act(const CtxT &x, const ProgT &p, const St
     // ---- actions, 64 agents at a time
     uint32_t stay = 0, births = 0;
     uint32_t mv[6] = {0, 0, 0, 0, 0, 0};
+    // the cell's Navigate edges
+    const int32_t cne0 = kNav ? x.nav.off[c] : 0, cne1 = kNav ? x.nav.off[c + 1] : 0;
     for (int ch = 0; ch < nchunks; ch++) {
         const int32_t li = ch * kWave + lane;
         const bool valid = li < n;
@@ -772,7 +775,7 @@ __device__ __forceinline__ void cell_act(const CtxT &x, const ProgT &p, const St
                 for (int qq = 0; qq < 6; qq++) nbr[qq] = (qq < x.conn) ? x.nbr[(int64_t)c * x.conn + qq] : -1;
 #pragma unroll
                 for (int qq = 0; qq < 7; qq++) w[qq] = (qq < R) ? x.W[(int64_t)c * R + qq] : 0.0;
-                resolve_moves<kNav>(x, p, sa, a, c, cv.nn, w, nbr, dir, nave, my_moves);
+                resolve_moves<kNav>(x, p, sa, a, c, cne0, cne1, cv.nn, w, nbr, dir, nave, my_moves);
             }
         }
         const bool alive = valid && a.life != 0;
@@ -831,6 +834,7 @@ struct GroupActLds {
     int16_t srt[kGroupSlots];                       // id order: list positions
     int16_t mate[kGroupSlots];                      // partner's group position, -1
     uint8_t nn[kGroupCells];
+    int32_t noff[2 * kGroupCells];                  // Navigate: first and end edge per cell
 };
 
 struct CellActLds {
@@ -864,9 +868,14 @@ __device__ __forceinline__ void group_act(const CtxT &x, const ProgT &p, const S
     const int nk0 = lane / 6, nj0 = lane % 6, nk1 = (lane + 64) / 6, nj1 = (lane + 64) % 6;
     const int32_t cw0 = __shfl(e.c, wk0 < ng ? wk0 : 0, 64), cw1 = __shfl(e.c, wk1 < ng ? wk1 : 0, 64);
     const int32_t cn0 = __shfl(e.c, nk0 < ng ? nk0 : 0, 64), cn1 = __shfl(e.c, nk1 < ng ? nk1 : 0, 64);
+    int32_t tNo = 0, tNe = 0;
     if (lane < ng) {
         tK = x.K[e.c];
         tnn = x.nn[e.c];
+        if (kNav) {
+            tNo = x.nav.off[e.c];
+            tNe = x.nav.off[e.c + 1];
+        }
     }
     if (wk0 < ng && wj0 < R) tW0 = x.W[(int64_t)cw0 * R + wj0];
     if (wk1 < ng && wj1 < R) tW1 = x.W[(int64_t)cw1 * R + wj1];
@@ -917,6 +926,10 @@ __device__ __forceinline__ void group_act(const CtxT &x, const ProgT &p, const S
     if (lane < ng) {
         G.K[lane] = tK;
         G.nn[lane] = tnn;
+        if (kNav) {
+            G.noff[lane] = tNo;
+            G.noff[lane + kGroupCells] = tNe;
+        }
 #pragma unroll
         for (int q = 0; q < kCntN; q++) G.cnt[lane][q] = 0;
     }
@@ -1070,7 +1083,8 @@ __device__ __forceinline__ void group_act(const CtxT &x, const ProgT &p, const S
                 for (int qq = 0; qq < 6; qq++) nbr[qq] = (qq < x.conn) ? G.nbr[k][qq] : -1;
 #pragma unroll
                 for (int qq = 0; qq < 7; qq++) w[qq] = (qq < R) ? G.W[k][qq] : 0.0;
-                resolve_moves<kNav>(x, p, sa, a, cc, cv.nn, w, nbr, dir, nave, my_moves);
+                resolve_moves<kNav>(x, p, sa, a, cc, kNav ? G.noff[k] : 0, kNav ? G.noff[k + kGroupCells] : 0,
+                                    cv.nn, w, nbr, dir, nave, my_moves);
             }
         }
         const bool alive = valid && a.life != 0;
@@ -1109,8 +1123,8 @@ __device__ __forceinline__ void group_act(const CtxT &x, const ProgT &p, const S
     wave_sync();
 }
 
-template <bool kRare, bool kAssort, bool kNav = false>
-__global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs sa) {
+template <bool kRare, bool kAssort, bool kNav>
+__device__ __forceinline__ void cell_act_kernel(const Ctx &x, const Prog &p, const StepArgs &sa) {
     __shared__ ActLds lds;
     const int lane = threadIdx.x;
     uint32_t my_deaths = 0, my_moves = 0, my_fert = 0;
@@ -1178,6 +1192,19 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs
         wp[1] = (uint32_t)(dm >> 32);
         wp[2] = fe;
     }
+}
+
+template <bool kRare, bool kAssort>
+__global__ void __launch_bounds__(kCellBlock) k_cell_act(Ctx x, Prog p, StepArgs sa) {
+    cell_act_kernel<kRare, kAssort, false>(x, p, sa);
+}
+
+// with Navigate: its code would take the kernel to 193 VGPRs and 2 waves per
+// SIMD; held to 3 waves (a few spilled registers)
+template <bool kRare, bool kAssort>
+__global__ void __launch_bounds__(kCellBlock) __attribute__((amdgpu_waves_per_eu(3)))
+k_cell_act_nav(Ctx x, Prog p, StepArgs sa) {
+    cell_act_kernel<kRare, kAssort, true>(x, p, sa);
 }
 
 // ---------------------------------------------------------------------------

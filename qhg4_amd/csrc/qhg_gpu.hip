@@ -950,10 +950,11 @@ int phase_act(qhg_pop *p) {
         if (rc) return rc;
     }
     // the age-death branches, the assortative mates and Navigate compile in
-    // only where used (Navigate's instantiations carry the age deaths too)
-    auto kern = p->has_nav    ? (p->has_assort ? k_cell_act<true, true, true> : k_cell_act<true, false, true>)
-                : p->has_rare ? (p->has_assort ? k_cell_act<true, true> : k_cell_act<true, false>)
-                              : (p->has_assort ? k_cell_act<false, true> : k_cell_act<false, false>);
+    // only where used
+    auto kern = p->has_nav ? (p->has_rare ? (p->has_assort ? k_cell_act_nav<true, true> : k_cell_act_nav<true, false>)
+                                          : (p->has_assort ? k_cell_act_nav<false, true> : k_cell_act_nav<false, false>))
+              : p->has_rare ? (p->has_assort ? k_cell_act<true, true> : k_cell_act<true, false>)
+                            : (p->has_assort ? k_cell_act<false, true> : k_cell_act<false, false>);
     int rc = launch_bs(p, KB_ACT, kern, p->act_grid, kCellBlock, p->ctx(), p->prog, p->step_args());
     if (rc) return rc;
     p->acted = true;

@@ -136,3 +136,22 @@ def test_navigate_needs_its_navigation_and_stays_out_of_batches():
     gpu.run(2)
     assert gpu.stats()["device_error"] == 0
     gpu.close()
+
+
+def test_navigate_at_the_bench_size():
+    """The bench's c2nav workload (40,962 cells, 100,000 agents on a coast,
+    278 ports, 814 destinations, 25 bridges) for 30 steps: the whole state
+    equals the oracle's every 10 steps."""
+    print("case: c2nav seed 42, level 6, 100k agents, 30 steps", flush=True)
+    sc = scenarios.c2nav(seed=42, level=6, n_agents=100000, patch_cells=4000, prob0=0.15, genetics=False)
+    gpu = GpuPopulation.from_scenario(sc)
+    ora = Oracle(sc)
+    gpu.preLoop()
+    ora.pre_loop()
+    for k in range(3):
+        gpu.run(10)
+        ora.run(10)
+        same_totals(gpu, ora, "step %d" % (10 * k + 10))
+        assert_same_state(gpu.agents(), ora.agents(), "step %d" % (10 * k + 10))
+    assert ora.stats()["num_alive"] > 50000
+    gpu.close()

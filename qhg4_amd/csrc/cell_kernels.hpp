@@ -728,8 +728,10 @@ __device__ __forceinline__ void cell_act(const CtxT &x, const ProgT &p, const St
     // ---- actions, 64 agents at a time
     uint32_t stay = 0, births = 0;
     uint32_t mv[6] = {0, 0, 0, 0, 0, 0};
-    // the cell's Navigate edges
-    const int32_t cne0 = kNav ? x.nav.off[c] : 0, cne1 = kNav ? x.nav.off[c + 1] : 0;
+    // the cell's Navigate edges (a population without tables, e.g. another
+    // member of a batch, has none: its table pointers are null)
+    const bool nav_tab = kNav && x.nav.n != 0;
+    const int32_t cne0 = nav_tab ? x.nav.off[c] : 0, cne1 = nav_tab ? x.nav.off[c + 1] : 0;
     for (int ch = 0; ch < nchunks; ch++) {
         const int32_t li = ch * kWave + lane;
         const bool valid = li < n;
@@ -872,7 +874,7 @@ __device__ __forceinline__ void group_act(const CtxT &x, const ProgT &p, const S
     if (lane < ng) {
         tK = x.K[e.c];
         tnn = x.nn[e.c];
-        if (kNav) {
+        if (kNav && x.nav.n != 0) {   // no tables (null pointers) when n is 0
             tNo = x.nav.off[e.c];
             tNe = x.nav.off[e.c + 1];
         }
@@ -2582,7 +2584,7 @@ __device__ __forceinline__ void batch_groups(const BatchArgs &B, int64_t bstep, 
     }
 }
 
-template <bool kRare>
+template <bool kRare, bool kNav = false>
 __global__ void __launch_bounds__(kCellBlock) k_batch_act(BatchArgs B) {
     __shared__ ActLds lds;
     const int lane = threadIdx.x;
@@ -2616,7 +2618,7 @@ __global__ void __launch_bounds__(kCellBlock) k_batch_act(BatchArgs B) {
             const uint32_t T = readlane_u32(e.s, ng - 1) + readlane_u32(e.n, ng - 1) - S0;
             const uint32_t nmax = wave_max_u32(lane < ng ? e.n : 0u);
             if (T <= (uint32_t)kGroupSlots && nmax <= (uint32_t)kGroupCellMax) {
-                group_act<kRare>(x, pg, sa, ng, e, e.s - S0, S0, T, lds.g, deaths, moves, fert);
+                group_act<kRare, false, kNav>(x, pg, sa, ng, e, e.s - S0, S0, T, lds.g, deaths, moves, fert);
                 continue;
             }
             for (int k = 0; k < ng; k++) {
@@ -2631,7 +2633,7 @@ __global__ void __launch_bounds__(kCellBlock) k_batch_act(BatchArgs B) {
                     L = PairLists{lds.c.fu, lds.c.mu, lds.c.fid, lds.c.mid, lds.c.fs, lds.c.ms, lds.c.mate,
                                   lds.c.flag};
                 }
-                cell_act<kRare>(x, pg, sa, c, s, n, L, deaths, moves, fert);
+                cell_act<kRare, false, kNav>(x, pg, sa, c, s, n, L, deaths, moves, fert);
             }
         }
         const uint64_t dm = wave_sum_u64(((uint64_t)moves << 32) | deaths);

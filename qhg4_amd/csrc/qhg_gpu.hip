@@ -1865,7 +1865,7 @@ struct qhg_batch {
     uint64_t *d_tiles = nullptr, *d_tiles2 = nullptr;
     size_t tiles_cap = 0;
     int32_t total_tiles = 0;
-    bool any_genetics = false, any_death = false;
+    bool any_genetics = false, any_death = false, any_nav = false;
     hipEvent_t ev[kRing] = {};
     int64_t ev_step[kRing];
     int64_t done_bstep = 0;        // every batch step below this one is complete
@@ -1938,8 +1938,9 @@ int batch_act_grid(const qhg_batch *b) { return b->num_cu * b->blocks_per_cu; }
 int batch_launch_step(qhg_batch *b) {
     const BatchArgs A = batch_args(b);
     const int ag = batch_act_grid(b);
-    int rc = b->any_death ? blaunch(b, BK_ACT, k_batch_act<true>, ag, kCellBlock, A)
-                          : blaunch(b, BK_ACT, k_batch_act<false>, ag, kCellBlock, A);
+    int rc = b->any_nav     ? blaunch(b, BK_ACT, k_batch_act<true, true>, ag, kCellBlock, A)
+             : b->any_death ? blaunch(b, BK_ACT, k_batch_act<true>, ag, kCellBlock, A)
+                            : blaunch(b, BK_ACT, k_batch_act<false>, ag, kCellBlock, A);
     if (rc) return rc;
     const int sg = std::max(1, std::min(b->scan_grid, b->total_tiles));
     rc = blaunch(b, BK_SCAN_SUMS, k_batch_scan_sums, sg, kScanBlock, A);
@@ -1966,7 +1967,7 @@ void batch_drop_graph(qhg_batch *b) {
 int batch_step(qhg_batch *b) {
     if (!b->use_graph || b->timing) return batch_launch_step(b);
     const int64_t key[4] = {b->table_version, (int64_t)(b->group_agents * 1024.0), b->blocks_per_cu,
-                            (b->any_genetics ? 1 : 0) | (b->any_death ? 2 : 0)};
+                            (b->any_genetics ? 1 : 0) | (b->any_death ? 2 : 0) | (b->any_nav ? 4 : 0)};
     if (!b->graph_exec || !std::equal(key, key + 4, b->graph_key)) {
         batch_drop_graph(b);
         HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -2030,6 +2031,7 @@ int batch_upload_table(qhg_batch *b) {
     std::vector<PopDesc> desc(std::max<size_t>(1, n));
     b->any_genetics = false;
     b->any_death = false;
+    b->any_nav = false;
     for (size_t i = 0; i < n; i++) {
         qhg_pop *p = b->pops[i];
         for (int par = 0; par < 2; par++) {
@@ -2044,6 +2046,7 @@ int batch_upload_table(qhg_batch *b) {
         b->gens[i] = p->gen;
         if (p->gwords) b->any_genetics = true;
         if (p->has_rare) b->any_death = true;
+        if (p->has_nav) b->any_nav = true;
     }
     if (n) HIPCHK(hipMemcpy(b->d_desc, desc.data(), n * sizeof(PopDesc), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b->d_tile_base, tile_base.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -2115,7 +2118,6 @@ int qhg_batch_add(qhg_batch *b, qhg_pop *p) {
     if (p->acted || p->initialized) return fail(QHG_ERR_STATE, "population is inside a step");
     if (p->has_assort)
         return fail(QHG_ERR_ARG, "AssortativePairHamming and AssortativePair populations do not run in batches");
-    if (p->has_nav) return fail(QHG_ERR_ARG, "Navigate populations do not run in batches");
     // hand over: everything the population enqueued so far has finished
     HIPCHK(hipStreamSynchronize(p->stream));
     poll_ring(p);

@@ -118,7 +118,7 @@ def test_navigate_grows_capacity_and_resumes_from_a_dump(tmp_path):
     b.close()
 
 
-def test_navigate_needs_its_navigation_and_stays_out_of_batches():
+def test_navigate_needs_its_navigation():
     sc = scenarios.c2nav(seed=65, level=4, n_agents=2000, patch_cells=200)
     nav = sc.pop("nav")
     gpu = GpuPopulation.from_scenario(sc)
@@ -130,12 +130,50 @@ def test_navigate_needs_its_navigation_and_stays_out_of_batches():
         gpu.set_navigation(bad)
     gpu.set_navigation(nav)
     gpu.preLoop()
-    batch = GpuBatch()
-    with pytest.raises(QhgError, match="do not run in batches"):
-        batch.add(gpu)
     gpu.run(2)
     assert gpu.stats()["device_error"] == 0
     gpu.close()
+
+
+def test_navigate_members_of_a_batch_match_alone():
+    """Two Navigate populations (Navigate after and before WeightedMove) and
+    a plain C2 one stepped as one batch: each ends as it does stepped alone,
+    a navigation event on one member midway included."""
+    print("case: batch of 2 c2nav members and 1 c2 member, 30 steps, nav event at 12", flush=True)
+    scs = [scenarios.c2nav(seed=66, level=4, n_agents=5000, patch_cells=250, prob0=0.15),
+           scenarios.c2nav(seed=67, level=5, n_agents=9000, patch_cells=400, prob0=0.15, nav_first=True,
+                           genetics=False),
+           scenarios.c2(seed=68, level=4, n_agents=4000, patch_cells=200)]
+    nav2 = second_navigation(scs[0], 3)
+
+    def alone(sc, k):
+        pop = GpuPopulation.from_scenario(sc)
+        pop.preLoop()
+        pop.run(12)
+        if k == 0:
+            pop.set_navigation(nav2)
+        pop.run(18)
+        out = pop.agents(), pop.stats()
+        pop.close()
+        return out
+
+    ref = [alone(sc, k) for k, sc in enumerate(scs)]
+    pops = [GpuPopulation.from_scenario(sc) for sc in scs]
+    batch = GpuBatch()
+    for p in pops:
+        p.preLoop()
+        batch.add(p)
+    batch.run(12)
+    pops[0].set_navigation(nav2)
+    batch.run(18)
+    for k, p in enumerate(pops):
+        assert_same_state(p.agents(), ref[k][0], "member %d" % k)
+        s = p.stats()
+        for f in ("num_alive", "max_id", "moves_total", "births_total", "deaths_total"):
+            assert s[f] == ref[k][1][f], (k, f)
+    for p in pops:
+        batch.remove(p)
+        p.close()
 
 
 def test_navigate_at_the_bench_size():

@@ -57,6 +57,7 @@ constexpr int kScanPer = 1;           // cells per k_cell_scan thread
 constexpr int kCellTile = kScanBlock * kScanPer;   // cells per look-back tile
 constexpr int kMaxCross = 32;         // device limit on Genetics_num_crossover
 constexpr int kStatusRing = 8;        // host-mapped status snapshots
+constexpr int kStatusSealEvery = 4;   // steps whose snapshot k_cell_scatter seals for the host
 
 constexpr int32_t kErrCapacity = 1;   // the next population does not fit
 constexpr int32_t kErrLookback = 4;   // a look-back wait exceeded its spin limit
@@ -2256,8 +2257,18 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_scatter(Ctx x, StepArgs sa)
             constexpr int kStep = (int)(offsetof(DevStatus, step) / 8);
             const uint64_t step = ((uint64_t)readlane_u32((uint32_t)(w >> 32), kStep) << 32) |
                                   readlane_u32((uint32_t)w, kStep);
-            uint64_t *dst = reinterpret_cast<uint64_t *>(x.hring + (int64_t)(step % (uint64_t)kStatusRing));
+            const int64_t slot = (int64_t)(step % (uint64_t)kStatusRing);
+            uint64_t *dst = reinterpret_cast<uint64_t *>(x.hring + slot);
             if (lane < kWords) dst[lane] = w;
+            if (step % (uint64_t)kStatusSealEvery == 0) {
+                // the seal words follow the kStatusRing snapshots: the step
+                // goes there once the snapshot is visible to the host (a
+                // system-scope release), and the host polls it instead of
+                // waiting on an event between two steps' kernels
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+                int64_t *seal = reinterpret_cast<int64_t *>(x.hring + kStatusRing) + slot;
+                if (lane == 0) __hip_atomic_store(seal, (int64_t)step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
         }
         return;
     }

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -157,9 +158,10 @@ struct qhg_pop {
     double t_next = 0.0;      // time of the next step
 
     // asynchronous status readback ring
-    DevStatus *h_ring = nullptr;     // pinned, kRing entries, written by k_cell_scan
+    DevStatus *h_ring = nullptr;     // pinned, kRing entries, written by k_cell_scatter
     DevStatus *d_ring = nullptr;     // device view of h_ring
-    hipEvent_t ring_ev[kRing] = {};
+    int64_t *h_seal = nullptr;       // kRing seal words after the snapshots: a sealed step
+    hipEvent_t ring_ev[kRing] = {};  // a batch member's: its batch's events
     int64_t ring_step[kRing];
     int64_t steps_enqueued = 0;      // index of the next step to enqueue
     DevStatus known{};               // newest status known on the host
@@ -176,11 +178,10 @@ struct qhg_pop {
 
     int cur() const { return (int)(steps_enqueued & 1); }
 
-    // batched stepping: the batch this population belongs to, and what it
-    // replaced while a member (its stream and status events)
+    // batched stepping: the batch this population belongs to, and the
+    // stream it replaced while a member
     qhg_batch *batch = nullptr;
     hipStream_t own_stream = nullptr;
-    hipEvent_t own_ev[kRing] = {};
     int64_t ring_bstep[kRing];  // batch step each ring entry's event closes
     int64_t *bdone = nullptr;   // newest batch step known complete
     int64_t gen = 0;          // bumped whenever a device buffer moves
@@ -804,13 +805,20 @@ int grow_agents(qhg_pop *p, int64_t new_cap, int64_t n) {
     return alloc_slot_scratch(p, new_cap);
 }
 
+int64_t seal_of(const qhg_pop *p, int k) { return __atomic_load_n(p->h_seal + k, __ATOMIC_ACQUIRE); }
+
+void clear_seals(qhg_pop *p) {
+    for (int k = 0; k < kRing; k++) __atomic_store_n(p->h_seal + k, (int64_t)-1, __ATOMIC_RELAXED);
+}
+
 void poll_ring(qhg_pop *p) {
     // newest completed snapshot wins; its own step field says which step it
-    // precedes (a slot may already hold a later step's snapshot)
-    // (a batch member asks its batch, which polls its events once per step)
+    // precedes.  A population stepped alone reads a slot once k_cell_scatter
+    // has sealed it with its step; a batch member asks its batch, which polls
+    // its events once per step.
     for (int k = 0; k < kRing; k++) {
         if (p->ring_step[k] < 0) continue;
-        const bool done = p->bdone ? p->ring_bstep[k] <= *p->bdone : hipEventQuery(p->ring_ev[k]) == hipSuccess;
+        const bool done = p->bdone ? p->ring_bstep[k] <= *p->bdone : seal_of(p, k) == p->ring_step[k];
         if (!done) continue;
         DevStatus snap;
         std::memcpy(&snap, (const void *)&p->h_ring[k], sizeof(snap));
@@ -819,6 +827,33 @@ void poll_ring(qhg_pop *p) {
             p->known_step = snap.step;
         }
         p->ring_step[k] = -1;
+    }
+    // older snapshots not read yet are superseded; their slots are free
+    // before the step that rewrites them is enqueued
+    for (int k = 0; k < kRing; k++)
+        if (p->ring_step[k] >= 0 && p->ring_step[k] <= p->known_step) p->ring_step[k] = -1;
+}
+
+// Wait until slot k is sealed.  A step that finds the device error flag set
+// writes no snapshot: once the stream has drained, the status itself is
+// read.  The stream is asked only every 2 ms of waiting: each query puts a
+// marker into the stream, which holds the GPU idle like an event does.
+int wait_seal(qhg_pop *p, int k) {
+    using clk = std::chrono::steady_clock;
+    auto next_query = clk::now() + std::chrono::milliseconds(2);
+    for (uint32_t spin = 1;; spin++) {
+        if (seal_of(p, k) == p->ring_step[k]) return 0;
+        if (spin % 1024 == 0 && clk::now() >= next_query) {
+            const hipError_t e = hipStreamQuery(p->stream);
+            if (e == hipSuccess) {
+                if (seal_of(p, k) == p->ring_step[k]) return 0;
+                for (int j = 0; j < kRing; j++) p->ring_step[j] = -1;
+                return refresh_known(p);
+            }
+            if (e != hipErrorNotReady) return fail(QHG_ERR_HIP, std::string("status wait: ") + hipGetErrorString(e));
+            next_query = clk::now() + std::chrono::milliseconds(2);
+        }
+        __builtin_ia32_pause();
     }
 }
 
@@ -869,9 +904,13 @@ int ensure_capacity(qhg_pop *p) {
         if (k < 0) {
             int rc = refresh_known(p);
             if (rc) return rc;
-        } else {
+        } else if (p->bdone) {
             HIPCHK(hipEventSynchronize(p->ring_ev[k]));
-            if (p->bdone) *p->bdone = std::max(*p->bdone, p->ring_bstep[k]);
+            *p->bdone = std::max(*p->bdone, p->ring_bstep[k]);
+            poll_ring(p);
+        } else {
+            int rc = wait_seal(p, k);
+            if (rc) return rc;
             poll_ring(p);
         }
     }
@@ -906,23 +945,23 @@ int ensure_capacity(qhg_pop *p) {
     return 0;
 }
 
-// Every step's scatter writes its status snapshot into the ring; an event
-// marks it only every kRingEvery steps.  Each event is a packet of its own
-// between two steps' kernels and holds the GPU idle for a few microseconds
-// (the kernel trace shows ~5.6 us before every step's first kernel when
-// each step records one); a snapshot a few steps old bounds the capacity
-// just as well.
-constexpr int kRingEvery = 4;
-static_assert(kRing % kRingEvery == 0 && kRing / kRingEvery >= 2, "ring slots for the marked steps");
+// Every step's scatter writes its status snapshot into the ring and seals
+// it every kStatusSealEvery steps; the host reads the sealed ones.  No event
+// marks them: an event is a packet of its own between two steps' kernels and
+// holds the GPU idle for a few microseconds (the kernel trace shows ~5.6 us
+// before a step's first kernel after each one), and a snapshot a few steps
+// old bounds the capacity just as well.
+constexpr int kRingEvery = kStatusSealEvery;
+static_assert(kRing % kRingEvery == 0 && kRing / kRingEvery >= 2, "ring slots for the sealed steps");
 
 int enqueue_readback(qhg_pop *p) {
     if (p->steps_enqueued % kRingEvery != 0) return 0;
     const int slot = (int)(p->steps_enqueued % kRing);
     if (p->ring_step[slot] >= 0) {
-        HIPCHK(hipEventSynchronize(p->ring_ev[slot]));
+        int rc = wait_seal(p, slot);
+        if (rc) return rc;
         poll_ring(p);
     }
-    HIPCHK(hipEventRecord(p->ring_ev[slot], p->stream));
     p->ring_step[slot] = p->steps_enqueued;
     return 0;
 }
@@ -1125,16 +1164,15 @@ int qhg_gpu_create(const qhg_grid_desc *grid, const qhg_env_desc *env, const qhg
         CK(upload_array(&p->d_ice, env->ice, nc, p->stream));
         CK(upload_array(&p->d_coastal, env->coastal, nc, p->stream));
     }
-    if (hipHostMalloc((void **)&p->h_ring, sizeof(DevStatus) * kRing, hipHostMallocMapped) != hipSuccess)
+    if (hipHostMalloc((void **)&p->h_ring, sizeof(DevStatus) * kRing + sizeof(int64_t) * kRing,
+                      hipHostMallocMapped) != hipSuccess)
         return cleanup_fail(fail(QHG_ERR_NOMEM, "pinned status ring"));
     std::memset(p->h_ring, 0, sizeof(DevStatus) * kRing);
+    p->h_seal = reinterpret_cast<int64_t *>(p->h_ring + kRing);
+    clear_seals(p);
     if (hipHostGetDevicePointer((void **)&p->d_ring, p->h_ring, 0) != hipSuccess)
         return cleanup_fail(fail(QHG_ERR_HIP, "status ring device pointer"));
-    for (int k = 0; k < kRing; k++) {
-        if (hipEventCreateWithFlags(&p->ring_ev[k], hipEventDisableTiming) != hipSuccess)
-            return cleanup_fail(fail(QHG_ERR_HIP, "event"));
-        p->ring_step[k] = -1;
-    }
+    for (int k = 0; k < kRing; k++) p->ring_step[k] = -1;
     if (hipStreamSynchronize(p->stream) != hipSuccess) return cleanup_fail(fail(QHG_ERR_HIP, "create sync"));
 #undef CK
     p->known = DevStatus{};
@@ -1160,8 +1198,6 @@ int qhg_gpu_destroy(qhg_pop *p) {
     for (int k = 0; k < 7; k++)
         if (p->d_env[k]) (void)hipFree(p->d_env[k]);
     for (auto &e : p->ev_pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
-    for (int k = 0; k < kRing; k++)
-        if (p->ring_ev[k]) (void)hipEventDestroy(p->ring_ev[k]);
     if (p->h_ring) (void)hipHostFree(p->h_ring);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
@@ -1302,6 +1338,8 @@ int qhg_gpu_set_clock(qhg_pop *p, int64_t step, double t, int64_t max_id) {
     HIPCHK(hipStreamSynchronize(p->stream));
     poll_ring(p);
     for (int k = 0; k < kRing; k++) p->ring_step[k] = -1;
+    // the steps may run again from an earlier one: no old seal may match
+    clear_seals(p);
     DevStatus st;
     int rc = read_status_sync(p, &st);
     if (rc) return rc;
@@ -2131,11 +2169,7 @@ int qhg_batch_add(qhg_batch *b, qhg_pop *p) {
     }
     p->own_stream = p->stream;
     p->stream = b->stream;
-    for (int k = 0; k < kRing; k++) {
-        p->own_ev[k] = p->ring_ev[k];
-        p->ring_ev[k] = nullptr;
-        p->ring_step[k] = -1;
-    }
+    for (int k = 0; k < kRing; k++) p->ring_step[k] = -1;
     p->batch = b;
     p->bdone = &b->done_bstep;
     b->pops.push_back(p);
@@ -2156,7 +2190,7 @@ int qhg_batch_remove(qhg_batch *b, qhg_pop *p) {
     poll_ring(p);
     p->stream = p->own_stream;
     for (int k = 0; k < kRing; k++) {
-        p->ring_ev[k] = p->own_ev[k];
+        p->ring_ev[k] = nullptr;
         p->ring_step[k] = -1;
     }
     p->batch = nullptr;

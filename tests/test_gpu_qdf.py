@@ -96,3 +96,29 @@ def test_extinction_then_dump_and_resume(tmp_path):
     assert b.stats()["num_alive"] == 0 and b.stats()["step"] == gone + 15
     gpu.close()
     b.close()
+
+
+def test_resume_into_the_population_that_wrote_the_dump(tmp_path):
+    """A dump restored into the population that wrote it, after that one ran
+    on: its clock goes back and the steps repeat exactly.  The status
+    snapshots of the first pass are still in the host ring's slots and must
+    not stand for the repeated steps (capacity starts small, so growth
+    depends on them)."""
+    print("case: c2 level 4, dump at 10, run to 23, restore, run to 23 again", flush=True)
+    sc = scenarios.c2(seed=54, level=4, n_agents=6000, patch_cells=300)
+    pop = GpuPopulation.from_scenario(sc, capacity=6144)
+    pop.preLoop()
+    pop.run(10)
+    f = str(tmp_path / "again.qdf")
+    qdf.write_population(pop, f, pop_class="OoAPop")
+    pop.run(13)
+    ref_a, ref_s = pop.agents(), pop.stats()
+    assert ref_s["capacity"] > 6144
+    qdf.restore_population(pop, f)
+    assert pop.stats()["step"] == 10
+    pop.run(13)
+    assert_same_state(pop.agents(), ref_a, "repeated steps")
+    s = pop.stats()
+    for k in ("num_alive", "max_id", "step", "device_error"):
+        assert s[k] == ref_s[k], k
+    pop.close()

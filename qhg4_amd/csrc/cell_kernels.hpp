@@ -572,14 +572,17 @@ __device__ __forceinline__ void resolve_moves(const CtxT &x, const ProgT &p, con
             if (d >= 0) { dir = d; moves++; }
         }
     } else {
-        // one copy of Navigate's code, not one per place it can take
+        // the neighbour moves' directions first (each depends on its own
+        // draw only), so the weight and neighbour rows are no longer live
+        // across Navigate's draws; then the registrations in program order,
+        // with one copy of Navigate's code, not one per place it can take
+        const int d0 = (a.npend > 0) ? move_dir(x, a.pk0, a.pu0, nn, w, nbr) : -1;
+        const int d1 = (a.npend > 1) ? move_dir(x, a.pk1, a.pu1, nn, w, nbr) : -1;
 #pragma unroll 1
         for (int q = 0; q <= a.npend; q++) {
             if (a.navq == q + 1 && dir < 0) nav_execute(x, p, sa, a.id, c, ne0, ne1, dir, nave, moves);
-            if (q < a.npend) {
-                const int d = move_dir(x, q == 0 ? a.pk0 : a.pk1, q == 0 ? a.pu0 : a.pu1, nn, w, nbr);
-                if (d >= 0) { dir = d; moves++; }
-            }
+            const int d = (q == 0) ? d0 : d1;
+            if (q < a.npend && d >= 0) { dir = d; moves++; }
         }
     }
 }

@@ -2769,6 +2769,18 @@ __global__ void __launch_bounds__(kCellBlock) k_batch_scatter(BatchArgs B) {
                 for (int w = 0; w < (int)(sizeof(DevStatus) / 8); w++) dstw[w] = srcw[w];
             }
         }
+        // every member's snapshot of a step divisible by kStatusSealEvery is
+        // sealed with its step, as k_cell_scatter does, once all of them are
+        // visible to the host (one system-scope release for the wave)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        for (int r = lane; r < B.npops; r += kWave) {
+            const int64_t step = pop_desc(B, r).ctx[0].st->step;
+            DevStatus *hr = pop_desc(B, r).ctx[0].hring;
+            if (hr && step % kStatusSealEvery == 0) {
+                int64_t *seal = reinterpret_cast<int64_t *>(hr + kStatusRing) + step % kStatusRing;
+                __hip_atomic_store(seal, step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+        }
         return;
     }
     // the same group numbering as k_batch_act

@@ -161,8 +161,7 @@ struct qhg_pop {
     DevStatus *h_ring = nullptr;     // pinned, kRing entries, written by k_cell_scatter
     DevStatus *d_ring = nullptr;     // device view of h_ring
     int64_t *h_seal = nullptr;       // kRing seal words after the snapshots: a sealed step
-    hipEvent_t ring_ev[kRing] = {};  // a batch member's: its batch's events
-    int64_t ring_step[kRing];
+    int64_t ring_step[kRing];        // the step of each slot's snapshot not read yet (or -1)
     int64_t steps_enqueued = 0;      // index of the next step to enqueue
     DevStatus known{};               // newest status known on the host
     int64_t known_step = 0;          // its step field: the step it precedes
@@ -182,8 +181,6 @@ struct qhg_pop {
     // stream it replaced while a member
     qhg_batch *batch = nullptr;
     hipStream_t own_stream = nullptr;
-    int64_t ring_bstep[kRing];  // batch step each ring entry's event closes
-    int64_t *bdone = nullptr;   // newest batch step known complete
     int64_t gen = 0;          // bumped whenever a device buffer moves
     int32_t host_error = 0;   // code with which a batch step of this population failed (it then left its batch)
 
@@ -813,13 +810,11 @@ void clear_seals(qhg_pop *p) {
 
 void poll_ring(qhg_pop *p) {
     // newest completed snapshot wins; its own step field says which step it
-    // precedes.  A population stepped alone reads a slot once k_cell_scatter
-    // has sealed it with its step; a batch member asks its batch, which polls
-    // its events once per step.
+    // precedes.  A slot is read once the step's scatter (k_cell_scatter, or
+    // k_batch_scatter for a batch member) has sealed it with its step.
     for (int k = 0; k < kRing; k++) {
         if (p->ring_step[k] < 0) continue;
-        const bool done = p->bdone ? p->ring_bstep[k] <= *p->bdone : seal_of(p, k) == p->ring_step[k];
-        if (!done) continue;
+        if (seal_of(p, k) != p->ring_step[k]) continue;
         DevStatus snap;
         std::memcpy(&snap, (const void *)&p->h_ring[k], sizeof(snap));
         if (snap.step > p->known_step) {
@@ -835,8 +830,8 @@ void poll_ring(qhg_pop *p) {
 }
 
 // Wait until slot k is sealed.  A step that finds the device error flag set
-// writes no snapshot: once the stream has drained, the status itself is
-// read.  The stream is asked only every 2 ms of waiting: each query puts a
+// writes no snapshot (a batch member's step no longer advances): once the
+// stream has drained, the status itself is read.  The stream is asked only every 2 ms of waiting: each query puts a
 // marker into the stream, which holds the GPU idle like an event does.
 int wait_seal(qhg_pop *p, int k) {
     using clk = std::chrono::steady_clock;
@@ -904,10 +899,6 @@ int ensure_capacity(qhg_pop *p) {
         if (k < 0) {
             int rc = refresh_known(p);
             if (rc) return rc;
-        } else if (p->bdone) {
-            HIPCHK(hipEventSynchronize(p->ring_ev[k]));
-            *p->bdone = std::max(*p->bdone, p->ring_bstep[k]);
-            poll_ring(p);
         } else {
             int rc = wait_seal(p, k);
             if (rc) return rc;
@@ -1904,9 +1895,6 @@ struct qhg_batch {
     size_t tiles_cap = 0;
     int32_t total_tiles = 0;
     bool any_genetics = false, any_death = false, any_nav = false;
-    hipEvent_t ev[kRing] = {};
-    int64_t ev_step[kRing];
-    int64_t done_bstep = 0;        // every batch step below this one is complete
     int64_t *d_bctr = nullptr;     // the kernels' copies of the batch step index
     // launch shapes (results do not depend on them)
     double group_agents = QHG_GROUP_AGENTS;
@@ -1945,15 +1933,6 @@ int blaunch(qhg_batch *b, int kid, void (*kern)(KArgs...), int grid, int block, 
     }
     HIPCHK(hipGetLastError());
     return 0;
-}
-
-// newest completed batch step from the batch's own events (in stream order,
-// so the newest completed event covers every older one)
-void batch_poll(qhg_batch *b) {
-    int64_t best = b->done_bstep;
-    for (int k = 0; k < kRing; k++)
-        if (b->ev_step[k] > best && hipEventQuery(b->ev[k]) == hipSuccess) best = b->ev_step[k];
-    b->done_bstep = best;
 }
 
 BatchArgs batch_args(const qhg_batch *b) {
@@ -2111,13 +2090,6 @@ int qhg_batch_create(qhg_batch **out) {
         delete b;
         return fail(QHG_ERR_HIP, "stream");
     }
-    for (int k = 0; k < kRing; k++) {
-        if (hipEventCreateWithFlags(&b->ev[k], hipEventDisableTiming) != hipSuccess) {
-            qhg_batch_destroy(b);
-            return fail(QHG_ERR_HIP, "event");
-        }
-        b->ev_step[k] = -1;
-    }
     if (hipMalloc(&b->d_bctr, BC_COUNT * sizeof(int64_t)) != hipSuccess ||
         hipMemset(b->d_bctr, 0, BC_COUNT * sizeof(int64_t)) != hipSuccess) {
         qhg_batch_destroy(b);
@@ -2140,8 +2112,6 @@ int qhg_batch_destroy(qhg_batch *b) {
     void *ptrs[] = {b->d_desc, b->d_tile_base, b->d_tiles, b->d_tiles2, b->d_bctr};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
-    for (int k = 0; k < kRing; k++)
-        if (b->ev[k]) (void)hipEventDestroy(b->ev[k]);
     for (auto &e : b->ev_pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -2171,7 +2141,6 @@ int qhg_batch_add(qhg_batch *b, qhg_pop *p) {
     p->stream = b->stream;
     for (int k = 0; k < kRing; k++) p->ring_step[k] = -1;
     p->batch = b;
-    p->bdone = &b->done_bstep;
     b->pops.push_back(p);
     b->gens.push_back(p->gen);
     b->step0.push_back(p->steps_enqueued);
@@ -2189,12 +2158,8 @@ int qhg_batch_remove(qhg_batch *b, qhg_pop *p) {
     HIPCHK(hipStreamSynchronize(b->stream));
     poll_ring(p);
     p->stream = p->own_stream;
-    for (int k = 0; k < kRing; k++) {
-        p->ring_ev[k] = nullptr;
-        p->ring_step[k] = -1;
-    }
+    for (int k = 0; k < kRing; k++) p->ring_step[k] = -1;
     p->batch = nullptr;
-    p->bdone = nullptr;
     int rc = refresh_known(p);
     b->pops.erase(b->pops.begin() + i);
     b->gens.erase(b->gens.begin() + i);
@@ -2217,7 +2182,6 @@ int qhg_batch_run(qhg_batch *b, int32_t nsteps) {
     std::string first_msg;
     for (int32_t s = 0; s < nsteps; s++) {
         if (b->pops.empty()) break;
-        batch_poll(b);
         // a member whose step cannot be prepared (its device reported an
         // error, its buffers could not grow) leaves the batch with its error
         // recorded; the other members go on
@@ -2244,20 +2208,11 @@ int qhg_batch_run(qhg_batch *b, int32_t nsteps) {
             p->steps_enqueued++;
             p->t_next += 1.0;
         }
-        // one event per batch step stands for every member's status snapshot
-        const int slot = (int)(b->bstep % kRing);
-        if (b->ev_step[slot] >= 0) {
-            HIPCHK(hipEventSynchronize(b->ev[slot]));
-            b->done_bstep = std::max(b->done_bstep, b->ev_step[slot]);
-            for (qhg_pop *p : b->pops) poll_ring(p);
-        }
-        HIPCHK(hipEventRecord(b->ev[slot], b->stream));
-        b->ev_step[slot] = b->bstep;
+        // each member's snapshots are sealed by k_batch_scatter; no event
+        // goes into the stream
         for (qhg_pop *p : b->pops) {
-            const int k = (int)(p->steps_enqueued % kRing);
-            p->ring_ev[k] = b->ev[slot];
-            p->ring_step[k] = p->steps_enqueued;
-            p->ring_bstep[k] = b->bstep;
+            rc = enqueue_readback(p);
+            if (rc) return rc;
         }
         if (b->timing && b->ev_used.size() > 4096) {
             rc = batch_collect_timing(b);
@@ -2273,7 +2228,6 @@ int qhg_batch_run(qhg_batch *b, int32_t nsteps) {
 int qhg_batch_sync(qhg_batch *b) {
     if (!b) return fail(QHG_ERR_ARG, "null handle");
     HIPCHK(hipStreamSynchronize(b->stream));
-    b->done_bstep = b->bstep;
     for (qhg_pop *p : b->pops) poll_ring(p);
     return 0;
 }

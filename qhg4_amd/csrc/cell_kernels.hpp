@@ -599,23 +599,32 @@ __device__ __forceinline__ void nav_movers(const CtxT &x, bool mine, int32_t nav
     const int lane = threadIdx.x & 63;
     const uint64_t lt = lanemask_lt();
     uint64_t left = __ballot(mine);
+    if (!left) return;
+    // the movers sharing an edge, each edge led by its first lane: the
+    // leaders then take their edges' counts in one atomic instruction (one
+    // round trip for the chunk, not one per edge)
+    int leader = lane;
+    uint32_t rank = 0, cnt = 0;
     while (left) {
         const int l0 = __ffsll((unsigned long long)left) - 1;
         const int32_t e = __shfl(nave, l0, 64);
         const uint64_t same = __ballot(mine && nave == e);
-        uint32_t base = 0;
-        if (lane == l0) {
-            const uint32_t k = (uint32_t)__popcll(same);
-            base = atomicAdd(&x.nav.cnt[e], k);
-            const int32_t d = x.nav.dst[e];
-            atomicAdd(&x.carr[d], k);
-            x.tact[d / kCellTile] = 1u;
+        if ((same >> lane) & 1ull) {
+            leader = l0;
+            rank = (uint32_t)__popcll(same & lt);
         }
-        base = (uint32_t)__shfl((int)base, l0, 64);
-        if ((same >> lane) & 1ull)
-            x.nav.sel[slot] = (uint64_t)(uint32_t)e | ((uint64_t)(base + (uint32_t)__popcll(same & lt)) << 32);
+        if (lane == l0) cnt = (uint32_t)__popcll(same);
         left &= ~same;
     }
+    uint32_t base = 0;
+    if (mine && leader == lane) {
+        base = atomicAdd(&x.nav.cnt[nave], cnt);
+        const int32_t d = x.nav.dst[nave];
+        atomicAdd(&x.carr[d], cnt);
+        x.tact[d / kCellTile] = 1u;
+    }
+    base = (uint32_t)__shfl((int)base, leader, 64);
+    if (mine) x.nav.sel[slot] = (uint64_t)(uint32_t)nave | ((uint64_t)(base + rank) << 32);
 }
 
 // ---------------------------------------------------------------------------
